@@ -1,0 +1,153 @@
+/* C ABI of the MI355X restricted-power-diagram (RPD) engine.
+ *
+ * Drop-in boundary for LibMAT's GPU RPD:
+ *   compute_clipped_voro_diagram_GPU        src/rpd3d/voronoi.h:52-61,
+ *                                            src/rpd3d/voronoi.cu:455-795
+ * whose callers are RPD3D_GPU::calculate / calculate_partial
+ * (src/rpd3d_api/rpd_api.cxx:117,275).  The C++ signature of that function
+ * and the RPD3D_GPU class are re-exported by include/libmat_rpd.h on top of
+ * this ABI; this header is the plain-pointer layer a cgo/JNI/ctypes binding
+ * would use.
+ *
+ * Model: a context owns one device, one HIP stream and the device buffers of
+ * one resident instance (tet mesh + spheres + neighbour lists).  rpd_upload
+ * copies host arrays in (H2D); rpd_run executes the device pipeline
+ * (candidate gathering, convex-cell clipping, exact / capacity re-runs,
+ * geometry, compaction) on the resident instance; rpd_download_* copy
+ * results out (D2H).  Errors are negative return codes with a message in
+ * rpd_last_error(); nothing calls exit().  A context is not thread-safe.
+ */
+#ifndef LIBMAT_AMD_RPD_CAPI_H
+#define LIBMAT_AMD_RPD_CAPI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Cell status codes: the reference's Status enum
+ * (src/rpd3d_base/voronoi_common.h:10-25). */
+enum rpd_status {
+  RPD_TRIANGLE_OVERFLOW = 0,
+  RPD_VERTEX_OVERFLOW = 1,
+  RPD_INCONSISTENT_BOUNDARY = 2,
+  RPD_SECURITY_RADIUS_NOT_REACHED = 3,
+  RPD_SUCCESS = 4,
+  RPD_NEEDS_EXACT_PREDICATES = 5,
+  RPD_NO_INTERSECTION = 6,
+  RPD_EDGE_OVERFLOW = 7,
+  RPD_NEEDS_PERTURB = 8
+};
+
+/* Capacities of one cell, equal to the reference's PRESET 3
+ * (voronoi_common.h:56-63): planes, vertices (dual triangles), edges. */
+#define RPD_MAX_P 64
+#define RPD_MAX_T 96
+#define RPD_MAX_E 152
+
+/* Host arrays of one RPD instance (all little-endian, row-major).
+ * Tet face i is opposite local vertex i; local edges are ordered
+ * (0,1),(0,2),(0,3),(1,2),(1,3),(2,3).  Tets must be positively oriented
+ * (det(v1-v0, v2-v0, v3-v0) > 0).  Neighbour lists are the power-diagram
+ * 1-rings in ascending site id (the reference's site_knn columns,
+ * triangulation.cxx:324-352, as CSR). */
+typedef struct {
+  int32_t n_vert, n_tet, n_site;
+  const double* verts;      /* 3*n_vert: x, y, z                        */
+  const int32_t* tets;      /* 4*n_tet                                  */
+  const int32_t* v_adj;     /* n_vert: #tets at each vertex             */
+  const int32_t* e_adj;     /* 6*n_tet: #tets at each tet edge          */
+  const int32_t* f_adj;     /* 4*n_tet: 1 (boundary) or 2               */
+  const int32_t* f_id;      /* 4*n_tet: unique face ids                 */
+  const double* sites;      /* 4*n_site: x, y, z, w = r^2               */
+  const uint32_t* flags;    /* n_site: 1 = compute cells of this site   */
+  const int64_t* knn_off;   /* n_site+1                                 */
+  const int32_t* knn_ids;   /* knn_off[n_site]                          */
+} rpd_instance_desc;
+
+typedef struct {
+  int32_t cells_per_wave;   /* cells clipped concurrently per wavefront:
+                               1, 2, 4 or 8 (0 = default)              */
+  int32_t use_active_mask;  /* 1: skip neighbour planes proved not to cut
+                               the cell's tet (results are identical)  */
+  int32_t compute_geometry; /* 1: volume, centroid, facet areas (FP64) */
+  int32_t force_exact;      /* 1: decide every conflict sign with exact
+                               arithmetic (test hook; identical output) */
+  int32_t small_tier;       /* 1: small-capacity first pass with re-run
+                               of overflowing cells at full capacity   */
+  int32_t reserved[3];
+} rpd_run_cfg;
+
+/* One emitted (non-empty) cell.  Its payload words start at word_off:
+ * nb_v vertex words (p0 | p1<<8 | p2<<16 | adj<<24: the three plane
+ * indices of the dual triangle and #adjacent cells), then nb_p plane-source
+ * words (int32: >= 0 neighbour site id of a bisector, -1-k for tet face k),
+ * then nb_e edge words (p0 | p1<<8 | adj<<16).  Facet areas of its nb_p
+ * planes start at area_off in the area stream (0 for planes carrying no
+ * face). */
+typedef struct {
+  int32_t tet_id;
+  int32_t seed;
+  uint8_t nb_v, nb_p, nb_e, status;
+  uint32_t reserved;
+  uint64_t word_off;
+  uint64_t area_off;
+} rpd_cell;
+
+typedef struct {
+  int64_t n_cells;            /* emitted cells                           */
+  int64_t n_candidates;       /* (tet, site) pairs passing the relation  */
+  int64_t n_words;            /* payload words                           */
+  int64_t n_areas;            /* facet-area entries                      */
+  int64_t n_dropped[9];       /* candidates dropped, per rpd_status      */
+  int64_t n_rerun_large;      /* cells re-run at full capacity           */
+  int64_t n_rerun_exact;      /* cells re-run with exact arithmetic      */
+  int64_t n_active_planes;    /* neighbour planes not proved inactive    */
+  int64_t n_all_planes;       /* all neighbour planes of all candidates  */
+  int64_t n_bbox_fallback;    /* spheres whose candidate box fell back to
+                                 the whole domain                        */
+  int32_t tet_k;              /* max candidates of one tet               */
+  int32_t site_k;             /* max neighbours of one site              */
+  double ms_total;            /* device time of the last rpd_run         */
+  double ms_phase[8];         /* candidates, clip, rerun, compaction ... */
+} rpd_stats;
+
+typedef struct rpd_ctx rpd_ctx;
+
+int rpd_ctx_create(int32_t device, rpd_ctx** out);
+void rpd_ctx_destroy(rpd_ctx* ctx);
+const char* rpd_last_error(const rpd_ctx* ctx);
+/* hipStream_t of the context (as void*), for event timing by callers. */
+void* rpd_ctx_stream(rpd_ctx* ctx);
+
+/* Copy an instance to the device (replaces the resident one). */
+int rpd_upload(rpd_ctx* ctx, const rpd_instance_desc* desc);
+/* Replace only the spheres / flags / neighbour lists (same n_site or not),
+ * keeping the resident tet mesh: the per-iteration update of sphere
+ * optimisation. */
+int rpd_update_sites(rpd_ctx* ctx, int32_t n_site, const double* sites,
+                     const uint32_t* flags, const int64_t* knn_off,
+                     const int32_t* knn_ids);
+
+/* Run the device pipeline.  rpd_run blocks until done; rpd_run_async only
+ * enqueues work on the context stream whose sizes are known on the device
+ * (results become readable after rpd_sync). */
+int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg);
+int rpd_sync(rpd_ctx* ctx);
+
+int rpd_get_stats(const rpd_ctx* ctx, rpd_stats* out);
+
+/* D2H copies of the last run's results; NULL pointers are skipped.
+ * cells[n_cells], words[n_words], geom[4*n_cells] (volume, centroid x,y,z),
+ * areas[n_areas]. */
+int rpd_download_cells(rpd_ctx* ctx, rpd_cell* cells, uint32_t* words,
+                       double* geom, double* areas);
+/* Candidate relation of the last run: per-tet CSR of related site ids in
+ * ascending order (the reference's tet_knn, voronoi.cu:284-317).
+ * offsets[n_tet+1], ids[n_candidates]. */
+int rpd_download_relation(rpd_ctx* ctx, int64_t* offsets, int32_t* ids);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1124 @@
+// HIP kernels of the restricted power diagram (RPD) for CDNA4 (gfx950).
+//
+// Pipeline of one rpd_run (see rpd_engine.hip for the host side):
+//   k_site_bbox    one wavefront per flagged sphere: clips the domain box by
+//                  the neighbour bisectors pushed outwards by the largest
+//                  tet radius; the bbox of that polytope contains the
+//                  centroid of every tet the sphere can relate to
+//   k_bin_*        spheres registered into a uniform grid of tet centroids
+//   k_tet_count /  one thread per tet: the reference's tet-sphere relation
+//   k_tet_fill     (voronoi.cu:154-193) over the spheres of its bin, emitted
+//                  in ascending site id, plus the mask of neighbour planes
+//                  that may cut the tet
+//   k_clip         G cells per wavefront, 64/G lanes per cell, cell state in
+//                  LDS: the reference clip loop (convex_cell.cu:680-774)
+//                  with lane-parallel conflict tests (ballot), serial cavity
+//                  boundary walk, lane-parallel edge/vertex creation and
+//                  FP64 geometry; tiers: small capacity, full capacity,
+//                  full capacity with exact arithmetic
+//   k_compact_*    deterministic compaction into the output stream
+#include <hip/hip_runtime.h>
+
+#include "rpd_device.h"
+#include "rpd_capi.h"
+
+namespace rpd {
+
+constexpr uint32_t kEnd = 255;
+
+__device__ __forceinline__ void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) {
+  return __ballot(p);
+}
+
+__device__ __forceinline__ rpd_d4 ld4(const double4* p) {
+  double4 v = *p;
+  return rpd_d4{v.x, v.y, v.z, v.w};
+}
+
+// --------------------------------------------------------- predicates
+// det4x4(p1.x, p2.x, p3.x, e.x, p1.y, ..., p1.w, p2.w, p3.w, e.w) in the
+// reference's evaluation order (convex_cell.cu:458-459,
+// common_cuda.h:195-213), with a semi-static filter: every monomial passes
+// through at most 9 roundings, |err| <= gamma_9 * 24 * prod_r max_c |a_rc|,
+// 24 * gamma_9 = 2.398e-14; 2.5e-14 covers the rounding of the bound.
+// Returns +1 conflict, -1 no conflict, 0 undecided.
+__device__ __forceinline__ int conflict_filtered(const rpd_d4& p1,
+                                                 const rpd_d4& p2,
+                                                 const rpd_d4& p3,
+                                                 const rpd_d4& e) {
+  const double a11 = p1.x, a12 = p2.x, a13 = p3.x, a14 = e.x;
+  const double a21 = p1.y, a22 = p2.y, a23 = p3.y, a24 = e.y;
+  const double a31 = p1.z, a32 = p2.z, a33 = p3.z, a34 = e.z;
+  const double a41 = p1.w, a42 = p2.w, a43 = p3.w, a44 = e.w;
+  double m12 = a21 * a12 - a11 * a22;
+  double m13 = a31 * a12 - a11 * a32;
+  double m14 = a41 * a12 - a11 * a42;
+  double m23 = a31 * a22 - a21 * a32;
+  double m24 = a41 * a22 - a21 * a42;
+  double m34 = a41 * a32 - a31 * a42;
+  double m123 = m23 * a13 - m13 * a23 + m12 * a33;
+  double m124 = m24 * a13 - m14 * a23 + m12 * a43;
+  double m134 = m34 * a13 - m14 * a33 + m13 * a43;
+  double m234 = m34 * a23 - m24 * a33 + m23 * a43;
+  double det = m234 * a14 - m134 * a24 + m124 * a34 - m123 * a44;
+  double mx = fmax(fmax(fabs(a11), fabs(a12)), fmax(fabs(a13), fabs(a14)));
+  double my = fmax(fmax(fabs(a21), fabs(a22)), fmax(fabs(a23), fabs(a24)));
+  double mz = fmax(fmax(fabs(a31), fabs(a32)), fmax(fabs(a33), fabs(a34)));
+  double mw = fmax(fmax(fabs(a41), fabs(a42)), fmax(fabs(a43), fabs(a44)));
+  double b = mx * my * mz * mw;
+  if (!(b < 1e280)) return 0;
+  double eps = 2.5e-14 * b + 1e-290;
+  if (det > eps) return 1;
+  if (det < -eps) return -1;
+  return 0;
+}
+
+__device__ __noinline__ int conflict_exact(rpd_d4 p1, rpd_d4 p2, rpd_d4 p3,
+                                           rpd_d4 e) {
+  return rpd_exact_det4_sign(p1, p2, p3, e) > 0 ? 1 : -1;
+}
+
+// sign of det3(normals) == 0 ? (is_vertex_perturb, convex_cell.cu:274-316)
+// returns 1 degenerate, 0 fine, -1 undecided
+__device__ __forceinline__ int perturb_filtered(const rpd_d4& p1,
+                                                const rpd_d4& p2,
+                                                const rpd_d4& p3) {
+  double d, q;
+  rpd_det3p(p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z, d, q);
+  if (!(q < 1e280)) return -1;
+  double eps = 6.0e-16 * q + 1e-290;
+  if (d > eps || d < -eps) return 0;
+  if (q == 0.0) return 1;  // all products zero: exactly degenerate
+  return -1;
+}
+
+__device__ __noinline__ int perturb_exact(rpd_d4 p1, rpd_d4 p2, rpd_d4 p3) {
+  return rpd_exact_w_sign(p1, p2, p3) == 0 ? 1 : 0;
+}
+
+// --------------------------------------------------------- tet planes
+// face k opposite vertex k with vertices tet_faces_lvid[k]
+// (convex_cell.h:24-25): {2,1,3}, {0,2,3}, {1,0,3}, {0,1,2}
+__device__ __forceinline__ rpd_d4 tri(const double4& a, const double4& b,
+                                      const double4& c) {
+  return rpd_tri_plane(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z);
+}
+__device__ __forceinline__ void tet_planes(const double4 V[4], rpd_d4 pl[4]) {
+  pl[0] = tri(V[2], V[1], V[3]);
+  pl[1] = tri(V[0], V[2], V[3]);
+  pl[2] = tri(V[1], V[0], V[3]);
+  pl[3] = tri(V[0], V[1], V[2]);
+}
+// initial dual triangles (convex_cell.cu:183-186)
+__constant__ uint8_t c_init_ver[4][3] = {{1, 3, 2}, {0, 2, 3}, {0, 3, 1},
+                                         {0, 1, 2}};
+// initial edges for local vertex pairs (0,1),(0,2),(0,3),(1,2),(1,3),(2,3)
+// (convex_cell.cu:202-207)
+__constant__ uint8_t c_init_edge[6][2] = {{2, 3}, {1, 3}, {1, 2},
+                                          {0, 3}, {0, 2}, {0, 1}};
+
+__device__ __forceinline__ uint32_t meta_byte(const TetMeta& m, int k) {
+  // k in [0,16): x:0-3, y:4-7, z:8-11, w:12-15
+  uint32_t w = k < 4 ? m.x : (k < 8 ? m.y : (k < 12 ? m.z : m.w));
+  return (w >> (8 * (k & 3))) & 255u;
+}
+
+// ================================================= candidate gathering
+struct BinGrid {
+  double org[3];
+  double inv_bs;
+  int32_t dim[3];
+  int32_t pad;
+};
+
+__device__ __forceinline__ int bin_coord(double x, double org, double inv,
+                                         int dim) {
+  double f = floor((x - org) * inv);
+  int c = f < 0 ? 0 : (f >= dim ? dim - 1 : (int)f);
+  return c;
+}
+
+// One wavefront per sphere.  Polytope = domain box clipped by the
+// neighbour bisectors, each pushed out by rho (max tet circum-radius about
+// the centroid plus a margin).  Output bbox (lo, hi); empty if lo > hi.
+constexpr int kBoxV = 256;
+constexpr int kBoxP = 262;
+
+__global__ __launch_bounds__(64) void k_site_bbox(DevSites S,
+                                                  const double* dom,  // 6
+                                                  double rho,
+                                                  double* bbox,  // 6/site
+                                                  int32_t* n_fallback) {
+  __shared__ double4 s_pl[kBoxP];
+  __shared__ double4 s_X[2][kBoxV];
+  __shared__ uint32_t s_t[2][kBoxV];
+  __shared__ uint8_t s_out[kBoxV];
+  __shared__ int32_t s_new;
+  const int i = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (i >= S.n_site) return;
+  double* out = bbox + 6 * (int64_t)i;
+  if (S.flags[i] != 1u) {
+    if (lane == 0) {
+      out[0] = out[1] = out[2] = 1.0;
+      out[3] = out[4] = out[5] = -1.0;
+    }
+    return;
+  }
+  const int64_t k0 = S.knn_off[i];
+  const int deg = (int)(S.knn_off[i + 1] - k0);
+  bool fail = deg > kBoxP - 6;
+  const rpd_d4 si = ld4(S.site + i);
+  // box planes: x >= x0, x <= x1, y >= y0, y <= y1, z >= z0, z <= z1
+  if (lane < 6) {
+    int a = lane >> 1;
+    double4 p = make_double4(0, 0, 0, 0);
+    double* pp = &p.x;
+    if ((lane & 1) == 0) {
+      pp[a] = 1.0;
+      p.w = -dom[a];
+    } else {
+      pp[a] = -1.0;
+      p.w = dom[3 + a];
+    }
+    s_pl[lane] = p;
+  }
+  wsync();
+  int nv = 8, cur = 0;
+  if (lane < 8) {
+    int a = lane & 1, b = 2 + ((lane >> 1) & 1), c = 4 + ((lane >> 2) & 1);
+    rpd_d4 X, Q;
+    rpd_vertex_homog(ld4(&s_pl[a]), ld4(&s_pl[b]), ld4(&s_pl[c]), X, Q);
+    s_X[0][lane] = make_double4(X.x, X.y, X.z, X.w);
+    s_t[0][lane] = a | (b << 10) | (c << 20);
+  }
+  wsync();
+  bool empty = false;
+  for (int k = 0; k < deg && !fail && !empty; ++k) {
+    const int m = S.knn_ids[k0 + k];
+    rpd_d4 e = rpd_bisector(si, ld4(S.site + m));
+    double nrm = sqrt(e.x * e.x + e.y * e.y + e.z * e.z);
+    e.w += nrm * rho;
+    const int p = 6 + k;
+    if (lane == 0) s_pl[p] = make_double4(e.x, e.y, e.z, e.w);
+    int n_out = 0;
+    for (int v = lane; v < nv; v += 64) {
+      double4 X = s_X[cur][v];
+      double s = e.x * X.x + e.y * X.y + e.z * X.z + e.w * X.w;
+      bool o = (s < 0 && X.w > 0) || (s > 0 && X.w < 0);
+      s_out[v] = o;
+    }
+    wsync();
+    for (int v0 = 0; v0 < nv; v0 += 64) {
+      int v = v0 + lane;
+      n_out += __popcll(ballot(v < nv && s_out[v]));
+    }
+    if (n_out == 0) continue;
+    if (n_out == nv) {
+      empty = true;
+      break;
+    }
+    if (lane == 0) s_new = 0;
+    wsync();
+    const int nxt = cur ^ 1;
+    const int n_keep = nv - n_out;
+    // compact kept vertices in order
+    int base = 0;
+    for (int v0 = 0; v0 < nv; v0 += 64) {
+      int v = v0 + lane;
+      bool keep = v < nv && !s_out[v];
+      uint64_t bm = ballot(keep);
+      if (keep) {
+        int pos = base + __popcll(bm & ((1ull << lane) - 1));
+        s_X[nxt][pos] = s_X[cur][v];
+        s_t[nxt][pos] = s_t[cur][v];
+      }
+      base += __popcll(bm);
+    }
+    // new vertices on edges between removed and kept vertices
+    bool lfail = false;
+    for (int v = lane; v < nv; v += 64) {
+      if (!s_out[v]) continue;
+      uint32_t tv = s_t[cur][v];
+      int pv[3] = {(int)(tv & 1023), (int)((tv >> 10) & 1023),
+                   (int)((tv >> 20) & 1023)};
+      for (int q = 0; q < 3; ++q) {
+        int a = pv[q], b = pv[(q + 1) % 3];
+        int other = -1;
+        for (int u = 0; u < nv; ++u) {
+          if (u == v) continue;
+          uint32_t tu = s_t[cur][u];
+          int u0 = tu & 1023, u1 = (tu >> 10) & 1023, u2 = (tu >> 20) & 1023;
+          bool ha = u0 == a || u1 == a || u2 == a;
+          bool hb = u0 == b || u1 == b || u2 == b;
+          if (ha && hb) {
+            other = u;
+            break;
+          }
+        }
+        if (other < 0) {
+          lfail = true;
+          continue;
+        }
+        if (s_out[other]) continue;
+        int slot = atomicAdd(&s_new, 1);
+        if (n_keep + slot >= kBoxV) {
+          lfail = true;
+          continue;
+        }
+        rpd_d4 X, Q;
+        rpd_vertex_homog(ld4(&s_pl[a]), ld4(&s_pl[b]), ld4(&s_pl[p]), X, Q);
+        s_X[nxt][n_keep + slot] = make_double4(X.x, X.y, X.z, X.w);
+        s_t[nxt][n_keep + slot] = a | (b << 10) | (p << 20);
+      }
+    }
+    fail = fail || (ballot(lfail) != 0);
+    wsync();
+    nv = n_keep + min(s_new, kBoxV - n_keep);
+    cur = nxt;
+    wsync();
+  }
+  if (fail) {
+    if (lane == 0) {
+      for (int a = 0; a < 6; ++a) out[a] = dom[a];
+      atomicAdd(n_fallback, 1);
+    }
+    return;
+  }
+  if (empty) {
+    if (lane == 0) {
+      out[0] = out[1] = out[2] = 1.0;
+      out[3] = out[4] = out[5] = -1.0;
+    }
+    return;
+  }
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int v = lane; v < nv; v += 64) {
+    double4 X = s_X[cur][v];
+    double x = X.x / X.w, y = X.y / X.w, z = X.z / X.w;
+    lo[0] = fmin(lo[0], x);
+    lo[1] = fmin(lo[1], y);
+    lo[2] = fmin(lo[2], z);
+    hi[0] = fmax(hi[0], x);
+    hi[1] = fmax(hi[1], y);
+    hi[2] = fmax(hi[2], z);
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fmin(lo[a], __shfl_xor(lo[a], off));
+      hi[a] = fmax(hi[a], __shfl_xor(hi[a], off));
+    }
+  if (lane == 0) {
+    double ext = fmax(fmax(dom[3] - dom[0], dom[4] - dom[1]), dom[5] - dom[2]);
+    double eps = 1e-9 * ext;
+    for (int a = 0; a < 3; ++a) {
+      out[a] = lo[a] - eps;
+      out[3 + a] = hi[a] + eps;
+    }
+  }
+}
+
+template <bool kFill>
+__global__ __launch_bounds__(256) void k_bin_register(
+    int32_t n_site, const double* bbox, BinGrid g, int32_t* bin_count,
+    int32_t* bin_ids) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_site) return;
+  const double* b = bbox + 6 * (int64_t)i;
+  if (b[0] > b[3] || b[1] > b[4] || b[2] > b[5]) return;
+  int x0 = bin_coord(b[0], g.org[0], g.inv_bs, g.dim[0]);
+  int y0 = bin_coord(b[1], g.org[1], g.inv_bs, g.dim[1]);
+  int z0 = bin_coord(b[2], g.org[2], g.inv_bs, g.dim[2]);
+  int x1 = bin_coord(b[3], g.org[0], g.inv_bs, g.dim[0]);
+  int y1 = bin_coord(b[4], g.org[1], g.inv_bs, g.dim[1]);
+  int z1 = bin_coord(b[5], g.org[2], g.inv_bs, g.dim[2]);
+  for (int z = z0; z <= z1; ++z)
+    for (int y = y0; y <= y1; ++y)
+      for (int x = x0; x <= x1; ++x) {
+        int64_t bin = x + (int64_t)g.dim[0] * (y + (int64_t)g.dim[1] * z);
+        int slot = atomicAdd(&bin_count[bin], 1);
+        if (kFill) bin_ids[slot] = i;
+      }
+}
+
+// bin_cursor holds the start offsets on entry (filled slots are taken by
+// atomics from it); sort each bin's list ascending afterwards
+__global__ __launch_bounds__(256) void k_bin_sort(int64_t n_bins,
+                                                  const int32_t* start,
+                                                  int32_t* ids) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_bins) return;
+  const int s = start[b], e = start[b + 1];
+  for (int q = s + 1; q < e; ++q) {
+    int v = ids[q];
+    int r = q - 1;
+    while (r >= s && ids[r] > v) {
+      ids[r + 1] = ids[r];
+      --r;
+    }
+    ids[r + 1] = v;
+  }
+}
+
+struct TetLoad {
+  double4 V[4];
+  int4 tv;
+};
+
+__device__ __forceinline__ void load_tet(const DevMesh& M, int t,
+                                         TetLoad& L) {
+  L.tv = M.tet_v[t];
+  L.V[0] = M.vert[L.tv.x];
+  L.V[1] = M.vert[L.tv.y];
+  L.V[2] = M.vert[L.tv.z];
+  L.V[3] = M.vert[L.tv.w];
+}
+
+// reference relation test (voronoi.cu:154-193) in binary64
+__device__ __forceinline__ bool relation(const DevSites& S, const TetLoad& L,
+                                         int i) {
+  if (S.flags[i] != 1u) return false;
+  const rpd_d4 si = ld4(S.site + i);
+  double pdi[4];
+  for (int k = 0; k < 4; ++k)
+    pdi[k] = rpd_power_dist(L.V[k].x, L.V[k].y, L.V[k].z, si);
+  const int64_t k0 = S.knn_off[i], k1 = S.knn_off[i + 1];
+  for (int64_t q = k0; q < k1; ++q) {
+    const int m = S.knn_ids[q];
+    if (m < 0) continue;
+    const rpd_d4 sm = ld4(S.site + m);
+    bool ok = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      ok = ok || rpd_power_dist(L.V[k].x, L.V[k].y, L.V[k].z, sm) > pdi[k];
+    if (!ok) return false;
+  }
+  return true;
+}
+
+__device__ __forceinline__ int64_t tet_bin(const TetLoad& L,
+                                           const BinGrid& g) {
+  double cx = (((L.V[0].x + L.V[1].x) + L.V[2].x) + L.V[3].x) * 0.25;
+  double cy = (((L.V[0].y + L.V[1].y) + L.V[2].y) + L.V[3].y) * 0.25;
+  double cz = (((L.V[0].z + L.V[1].z) + L.V[2].z) + L.V[3].z) * 0.25;
+  int x = bin_coord(cx, g.org[0], g.inv_bs, g.dim[0]);
+  int y = bin_coord(cy, g.org[1], g.inv_bs, g.dim[1]);
+  int z = bin_coord(cz, g.org[2], g.inv_bs, g.dim[2]);
+  return x + (int64_t)g.dim[0] * (y + (int64_t)g.dim[1] * z);
+}
+
+__global__ __launch_bounds__(256) void k_tet_count(DevMesh M, DevSites S,
+                                                   BinGrid g,
+                                                   const int32_t* bin_start,
+                                                   const int32_t* bin_ids,
+                                                   int64_t* tet_count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M.n_tet) return;
+  TetLoad L;
+  load_tet(M, t, L);
+  const int64_t b = tet_bin(L, g);
+  int64_t n = 0;
+  for (int q = bin_start[b]; q < bin_start[b + 1]; ++q)
+    n += relation(S, L, bin_ids[q]);
+  tet_count[t] = n;
+}
+
+__global__ __launch_bounds__(256) void k_tet_fill(
+    DevMesh M, DevSites S, BinGrid g, const int32_t* bin_start,
+    const int32_t* bin_ids, const int64_t* tet_off, Cand* cand,
+    int32_t* rel_ids) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M.n_tet) return;
+  TetLoad L;
+  load_tet(M, t, L);
+  const int64_t b = tet_bin(L, g);
+  int64_t o = tet_off[t];
+  // corners of the tet cell: homogeneous points of the initial dual
+  // triangles, with filter permanents
+  rpd_d4 pl[4], X[4], Q[4];
+  tet_planes(L.V, pl);
+  for (int k = 0; k < 4; ++k)
+    rpd_vertex_homog(pl[c_init_ver[k][0]], pl[c_init_ver[k][1]],
+                     pl[c_init_ver[k][2]], X[k], Q[k]);
+  for (int q = bin_start[b]; q < bin_start[b + 1]; ++q) {
+    const int i = bin_ids[q];
+    if (!relation(S, L, i)) continue;
+    const rpd_d4 si = ld4(S.site + i);
+    const int64_t k0 = S.knn_off[i];
+    const int deg = (int)(S.knn_off[i + 1] - k0);
+    uint64_t m0 = 0, m1 = 0;
+    const int lim = deg < 128 ? deg : 128;
+    for (int k = 0; k < lim; ++k) {
+      const rpd_d4 e = rpd_bisector(si, ld4(S.site + S.knn_ids[k0 + k]));
+      // inactive iff all four corners are certainly not in conflict
+      bool inactive = true;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        inactive = inactive && rpd_filtered_sign4(e, X[c], Q[c]) < 0;
+      if (!inactive) {
+        if (k < 64)
+          m0 |= 1ull << k;
+        else
+          m1 |= 1ull << (k - 64);
+      }
+    }
+    Cand c;
+    c.tet = t;
+    c.seed = i;
+    c.m0 = m0;
+    c.m1 = m1;
+    cand[o] = c;
+    rel_ids[o] = i;
+    ++o;
+  }
+}
+
+// ============================================================ clipping
+struct ClipArgs {
+  DevMesh M;
+  DevSites S;
+  const Cand* cand;
+  const int32_t* items;   // cell indices to process (nullptr: identity)
+  int64_t n_items;
+  SlotView out;           // indexed by item
+  int32_t use_mask;
+  int32_t force_exact;
+  int32_t geometry;
+  int32_t* rerun_large;   // list + counter (small tier)
+  int32_t* n_rerun_large;
+  int32_t* rerun_exact;
+  int32_t* n_rerun_exact;
+};
+
+template <int L>
+__device__ __forceinline__ uint64_t group_bits(uint64_t b, int g) {
+  if (L == 64) return b;
+  return (b >> (g * L)) & ((1ull << L) - 1);
+}
+
+// geometry of one vertex-plane corner: see compute() in k_clip
+__device__ __forceinline__ void foot_plane(const rpd_d4& p, const double C[3],
+                                           double F[3]) {
+  double nn = p.x * p.x + p.y * p.y + p.z * p.z;
+  double s = (p.x * C[0] + p.y * C[1] + p.z * C[2] + p.w) / nn;
+  F[0] = C[0] - s * p.x;
+  F[1] = C[1] - s * p.y;
+  F[2] = C[2] - s * p.z;
+}
+__device__ __forceinline__ void foot_line(const rpd_d4& p, const rpd_d4& q,
+                                          const double A[3], const double C[3],
+                                          double E[3]) {
+  double d0 = p.y * q.z - p.z * q.y;
+  double d1 = p.z * q.x - p.x * q.z;
+  double d2 = p.x * q.y - p.y * q.x;
+  double dd = d0 * d0 + d1 * d1 + d2 * d2;
+  double s = ((C[0] - A[0]) * d0 + (C[1] - A[1]) * d1 + (C[2] - A[2]) * d2) /
+             dd;
+  E[0] = A[0] + s * d0;
+  E[1] = A[1] + s * d1;
+  E[2] = A[2] + s * d2;
+}
+// signed volume of tet (P, Q, C, A) as the reference's
+// get_tet_volume(P-A, Q-A, C-A) = -det3(P-A, Q-A, C-A)/6
+__device__ __forceinline__ double tet_vol(const double P[3], const double Q[3],
+                                          const double C[3],
+                                          const double A[3]) {
+  double a0 = P[0] - A[0], a1 = P[1] - A[1], a2 = P[2] - A[2];
+  double b0 = Q[0] - A[0], b1 = Q[1] - A[1], b2 = Q[2] - A[2];
+  double c0 = C[0] - A[0], c1 = C[1] - A[1], c2 = C[2] - A[2];
+  double d = a0 * (b1 * c2 - b2 * c1) - b0 * (a1 * c2 - a2 * c1) +
+             c0 * (a1 * b2 - a2 * b1);
+  return -d / 6.0;
+}
+// signed area of triangle (P, Q, A) along unit normal n
+__device__ __forceinline__ double tri_area(const double P[3], const double Q[3],
+                                           const double A[3],
+                                           const double n[3]) {
+  double u0 = Q[0] - P[0], u1 = Q[1] - P[1], u2 = Q[2] - P[2];
+  double v0 = A[0] - P[0], v1 = A[1] - P[1], v2 = A[2] - P[2];
+  double c0 = u1 * v2 - u2 * v1, c1 = u2 * v0 - u0 * v2,
+         c2 = u0 * v1 - u1 * v0;
+  return 0.5 * (c0 * n[0] + c1 * n[1] + c2 * n[2]);
+}
+
+template <int G, class C, bool kExact>
+__global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
+  constexpr int L = 64 / G;
+  constexpr int V = C::V, P = C::P, E = C::E;
+  constexpr int VS = (V + L - 1) / L;
+  constexpr bool kFull = (P == RPD_MAX_P);
+  __shared__ double4 s_pl[G][P];
+  __shared__ int32_t s_src[G][P];
+  __shared__ uint8_t s_h[G][P];
+  __shared__ uint8_t s_bn[G][P];
+  __shared__ uint8_t s_cyc[G][P + 1];
+  __shared__ uint32_t s_ver[G][V];
+  __shared__ uint32_t s_edge[G][E];
+
+  const int lane = threadIdx.x;
+  const int g = lane / L, j = lane % L;
+  // XCD-aware order: consecutive items stay on one XCD (8 XCDs,
+  // round-robin dispatch of workgroups)
+  const int64_t nb = gridDim.x;
+  int64_t blk = blockIdx.x;
+  if (nb >= 8) {
+    const int64_t per = nb / 8;
+    const int64_t xcd = blk % 8, pos = blk / 8;
+    if (pos < per) blk = xcd * per + pos;
+  }
+  const int64_t item = blk * G + g;
+  if (item >= a.n_items) return;
+  const int64_t cidx = a.items ? a.items[item] : item;
+  const Cand cd = a.cand[cidx];
+  const int t = cd.tet, s = cd.seed;
+
+  // ---- init the cell as its tet (convex_cell.cu:116-214)
+  const int4 tv = a.M.tet_v[t];
+  const int4 tf = a.M.tet_fid[t];
+  const TetMeta tm = a.M.tet_meta[t];
+  double4 TV[4] = {a.M.vert[tv.x], a.M.vert[tv.y], a.M.vert[tv.z],
+                   a.M.vert[tv.w]};
+  {
+    rpd_d4 pl[4];
+    tet_planes(TV, pl);
+    if (j < 4) {
+      const rpd_d4 p = pl[j];
+      s_pl[g][j] = make_double4(p.x, p.y, p.z, p.w);
+      s_src[g][j] = -1 - j;
+      s_h[g][j] = (uint8_t)meta_byte(tm, j);
+      s_ver[g][j] = c_init_ver[j][0] | (c_init_ver[j][1] << 8) |
+                    (c_init_ver[j][2] << 16) | (meta_byte(tm, 4 + j) << 24);
+    }
+    if (j < 6)
+      s_edge[g][j] = c_init_edge[j][0] | (c_init_edge[j][1] << 8) |
+                     (meta_byte(tm, 8 + j) << 16);
+  }
+  (void)tf;
+  wsync();
+  int nb_v = 4, nb_p = 4, nb_e = 6;
+  uint32_t status = RPD_SUCCESS;
+  const rpd_d4 S0 = ld4(a.S.site + s);
+  const int64_t k0 = a.S.knn_off[s];
+  const int deg = (int)(a.S.knn_off[s + 1] - k0);
+  uint64_t m0 = a.use_mask ? cd.m0 : ~0ull;
+  uint64_t m1 = a.use_mask ? cd.m1 : ~0ull;
+  int k = -1;
+  for (;;) {
+    // next neighbour position that may cut the cell
+    {
+      int nk = k + 1;
+      if (nk < 64) {
+        uint64_t r = m0 >> nk;
+        if (r)
+          nk += __ffsll((long long)r) - 1;
+        else
+          nk = 64;
+      }
+      if (nk >= 64 && nk < 128) {
+        uint64_t r = m1 >> (nk - 64);
+        if (r)
+          nk += __ffsll((long long)r) - 1;
+        else
+          nk = 128;
+      }
+      k = nk;
+    }
+    if (k >= deg) break;
+    if (nb_p >= P) {
+      status = kFull ? RPD_VERTEX_OVERFLOW : kStatusRerunLarge;
+      break;
+    }
+    const int m = a.S.knn_ids[k0 + k];
+    if (m == -1) break;
+    const rpd_d4 e = rpd_bisector(S0, ld4(a.S.site + m));
+    const int cur = nb_p;
+
+    // ---- conflict tests, lane-parallel (convex_cell.cu:706-721)
+    uint64_t conf_lo = 0, conf_hi = 0;
+    bool undecided = false;
+#pragma unroll
+    for (int q = 0; q < VS; ++q) {
+      const int v = q * L + j;
+      int c = -1;
+      if (v < nb_v) {
+        const uint32_t w = s_ver[g][v];
+        const rpd_d4 p1 = ld4(&s_pl[g][w & 255]);
+        const rpd_d4 p2 = ld4(&s_pl[g][(w >> 8) & 255]);
+        const rpd_d4 p3 = ld4(&s_pl[g][(w >> 16) & 255]);
+        c = a.force_exact ? 0 : conflict_filtered(p1, p2, p3, e);
+        if (kExact && c == 0) c = conflict_exact(p1, p2, p3, e);
+      }
+      const uint64_t bc = group_bits<L>(ballot(c > 0), g);
+      const uint64_t bu = group_bits<L>(ballot(c == 0), g);
+      if (q * L < 64)
+        conf_lo |= bc << (q * L);
+      else
+        conf_hi |= bc << (q * L - 64);
+      undecided = undecided || bu != 0;
+    }
+    if (undecided) {
+      status = kStatusRerunExact;
+      break;
+    }
+    const int nr = __popcll(conf_lo) + __popcll(conf_hi);
+    if (nr == 0) continue;  // the plane does not cut: popped (nb_p--)
+    if (nr == nb_v) {
+      status = RPD_NO_INTERSECTION;
+      break;
+    }
+    if (j == 0) {
+      s_pl[g][cur] = make_double4(e.x, e.y, e.z, e.w);
+      s_src[g][cur] = m;
+      s_h[g][cur] = 1;  // F_CELL_ADJ_DEFAULT
+    }
+    nb_p = cur + 1;
+
+    // ---- move conflicting vertices to the tail exactly as the serial
+    // swap loop does: i scans from the front, end from the back
+    {
+      int src[VS];
+#pragma unroll
+      for (int q = 0; q < VS; ++q) src[q] = -1;
+      int i = 0, end = nb_v, occ = 0;
+      while (i < end) {
+        const bool cf = occ < 64 ? ((conf_lo >> occ) & 1)
+                                 : ((conf_hi >> (occ - 64)) & 1);
+        int pos, nxt;
+        if (!cf) {
+          pos = i;
+          ++i;
+          nxt = i;
+        } else {
+          --end;
+          pos = end;
+          nxt = end;
+        }
+        if (pos % L == j) {
+          const int qq = pos / L;
+#pragma unroll
+          for (int q = 0; q < VS; ++q)
+            if (q == qq) src[q] = occ;
+        }
+        occ = nxt;
+      }
+      uint32_t val[VS];
+#pragma unroll
+      for (int q = 0; q < VS; ++q)
+        val[q] = (src[q] >= 0) ? s_ver[g][src[q]] : 0u;
+      wsync();
+#pragma unroll
+      for (int q = 0; q < VS; ++q)
+        if (src[q] >= 0) s_ver[g][q * L + j] = val[q];
+      wsync();
+    }
+    const int nv_old = nb_v;
+    nb_v -= nr;
+    (void)nv_old;
+
+    // ---- cavity boundary (convex_cell.cu:618-678), serial per cell
+    for (int p = j; p < P; p += L) s_bn[g][p] = kEnd;
+    wsync();
+    uint32_t first = kEnd;
+    {
+      int iter = 0, t = nb_v, r = nr;
+      while (r > 0) {
+        if (iter++ > 65535) {
+          status = RPD_INCONSISTENT_BOUNDARY;
+          break;
+        }
+        const uint32_t tw = s_ver[g][t];
+        const uint32_t pa = tw & 255, pb = (tw >> 8) & 255,
+                       pc = (tw >> 16) & 255;
+        const uint32_t ba = s_bn[g][pa], bb = s_bn[g][pb], bc = s_bn[g][pc];
+        const bool in0 = ba != kEnd, in1 = bb != kEnd, in2 = bc != kEnd;
+        const bool n0 = bb == pa, n1 = bc == pb, n2 = ba == pc;
+        bool simple = true;
+        if (!n0 && !n1 && in1) simple = false;
+        if (!n1 && !n2 && in2) simple = false;
+        if (!n2 && !n0 && in0) simple = false;
+        if (!n0 && !n1 && !n2) {
+          if (first == kEnd) {
+            wsync();
+            s_bn[g][pa] = pb;
+            s_bn[g][pb] = pc;
+            s_bn[g][pc] = pa;
+            first = pa;
+            wsync();
+          } else {
+            simple = false;
+          }
+        }
+        if (!simple) {
+          ++t;
+          if (t == nb_v + r) t = nb_v;
+          continue;
+        }
+        wsync();
+        if (!n0) s_bn[g][pa] = pb;
+        if (!n1) s_bn[g][pb] = pc;
+        if (!n2) s_bn[g][pc] = pa;
+        wsync();
+        if (n0 && n1) {
+          if (first == pb) first = s_bn[g][pb];
+          wsync();
+          s_bn[g][pb] = kEnd;
+          wsync();
+        }
+        if (n1 && n2) {
+          if (first == pc) first = s_bn[g][pc];
+          wsync();
+          s_bn[g][pc] = kEnd;
+          wsync();
+        }
+        if (n2 && n0) {
+          if (first == pa) first = s_bn[g][pa];
+          wsync();
+          s_bn[g][pa] = kEnd;
+          wsync();
+        }
+        const int last = nb_v + r - 1;
+        const uint32_t lw = s_ver[g][last];
+        wsync();
+        s_ver[g][t] = lw;
+        s_ver[g][last] = tw;
+        wsync();
+        t = nb_v;
+        --r;
+      }
+    }
+    if (status != RPD_SUCCESS) break;
+    if (first == kEnd) {
+      if (kFull && nb_p >= RPD_MAX_P && k + 1 < deg) {
+        status = RPD_VERTEX_OVERFLOW;
+        break;
+      }
+      continue;
+    }
+
+    // ---- boundary cycle
+    int nbd = 0;
+    {
+      uint32_t cir = first;
+      do {
+        if (nbd >= P || cir >= (uint32_t)P) break;
+        if (j == 0) s_cyc[g][nbd] = (uint8_t)cir;
+        ++nbd;
+        cir = s_bn[g][cir];
+      } while (cir != first);
+      if (cir != first) {
+        status = RPD_INCONSISTENT_BOUNDARY;
+        break;
+      }
+    }
+    wsync();
+    // ---- new edges (cur, cir) (convex_cell.cu:604-616, 752-758)
+    if (nb_e + nbd > E) {
+      status = kFull ? RPD_EDGE_OVERFLOW : kStatusRerunLarge;
+      break;
+    }
+    const uint32_t hcur = s_h[g][cur];
+    for (int q = j; q < nbd; q += L) {
+      const uint32_t c2 = s_cyc[g][q];
+      const uint32_t hz = max(hcur, (uint32_t)s_h[g][c2]);
+      const uint32_t lo = min((uint32_t)cur, c2), hi = max((uint32_t)cur, c2);
+      s_edge[g][nb_e + q] = lo | (hi << 8) | (hz << 16);
+    }
+    wsync();
+    // ---- new vertices (cur, cir, next(cir)) (convex_cell.cu:760-772)
+    int first_bad = 1 << 30;   // first index failing overflow or perturb
+    int bad_kind = 0;          // 1 overflow, 2 perturb, 3 edge lookup, 4 undecided
+    for (int q0 = 0; q0 < nbd; q0 += L) {
+      const int q = q0 + j;
+      int kind = 0;
+      if (q < nbd) {
+        const uint32_t c1 = s_cyc[g][q];
+        const uint32_t c2 = s_cyc[g][(q + 1) % nbd];
+        if (nb_v + q + 1 >= V) {
+          kind = 1;
+        } else {
+          // adjacency = max over the three edges (first match in list)
+          const uint32_t z1 = (s_edge[g][nb_e + q] >> 16) & 255;
+          const uint32_t z2 = (s_edge[g][nb_e + (q + 1) % nbd] >> 16) & 255;
+          const uint32_t lo = min(c1, c2), hi = max(c1, c2);
+          uint32_t z3 = 0;
+          bool found = false;
+          for (int ee = 0; ee < nb_e; ++ee) {
+            const uint32_t w = s_edge[g][ee];
+            if ((w & 255) == lo && ((w >> 8) & 255) == hi) {
+              z3 = (w >> 16) & 255;
+              found = true;
+              break;
+            }
+          }
+          if (!found) {
+            kind = 3;
+          } else {
+            const rpd_d4 pc = ld4(&s_pl[g][cur]);
+            const rpd_d4 p1 = ld4(&s_pl[g][c1]);
+            const rpd_d4 p2 = ld4(&s_pl[g][c2]);
+            int pf = a.force_exact ? -1 : perturb_filtered(pc, p1, p2);
+            if (kExact && pf < 0) pf = perturb_exact(pc, p1, p2);
+            if (pf == 1)
+              kind = 2;
+            else if (pf < 0)
+              kind = 4;
+            const uint32_t z = max(max(z1, z2), z3);
+            s_ver[g][nb_v + q] = cur | (c1 << 8) | (c2 << 16) | (z << 24);
+          }
+        }
+      }
+      // first failing index in this pass (in order), kinds by priority of
+      // position; overflow at q stops before the perturb test of q
+      const uint64_t bad = group_bits<L>(ballot(kind != 0), g);
+      if (bad && first_bad == (1 << 30)) {
+        const int qb = q0 + __ffsll((long long)bad) - 1;
+        first_bad = qb;
+        // broadcast the kind of lane (qb - q0) of this group
+        bad_kind = __shfl(kind, g * L + (qb - q0));
+      }
+    }
+    wsync();
+    if (first_bad != (1 << 30)) {
+      if (bad_kind == 1)
+        status = kFull ? RPD_TRIANGLE_OVERFLOW : kStatusRerunLarge;
+      else if (bad_kind == 2)
+        status = RPD_NEEDS_PERTURB;
+      else if (bad_kind == 3)
+        status = RPD_INCONSISTENT_BOUNDARY;
+      else
+        status = kStatusRerunExact;
+      break;
+    }
+    nb_e += nbd;
+    nb_v += nbd;
+    if (kFull && nb_p >= RPD_MAX_P && k + 1 < deg) {
+      status = RPD_VERTEX_OVERFLOW;
+      break;
+    }
+  }
+
+  // ---- write the slot
+  const SlotView& o = a.out;
+  if (status == RPD_SUCCESS && a.geometry) {
+    // FP64 geometry by a vertex-local decomposition (6 tets per vertex
+    // between the reference point C, the vertex A, the feet of C on its 3
+    // planes and on its 3 edge lines); C = the cell's vertex 0, so rounding
+    // errors scale with the cell, not with the tet
+    double Cc[3];
+    {
+      const uint32_t w0 = s_ver[g][0];
+      rpd_d4 X0, Q0;
+      rpd_vertex_homog(ld4(&s_pl[g][w0 & 255]), ld4(&s_pl[g][(w0 >> 8) & 255]),
+                       ld4(&s_pl[g][(w0 >> 16) & 255]), X0, Q0);
+      Cc[0] = X0.x / X0.w;
+      Cc[1] = X0.y / X0.w;
+      Cc[2] = X0.z / X0.w;
+    }
+    double vol = 0, mx = 0, my = 0, mz = 0;
+    for (int v = j; v < nb_v; v += L) {
+      const uint32_t w = s_ver[g][v];
+      const rpd_d4 pa = ld4(&s_pl[g][w & 255]);
+      const rpd_d4 pb = ld4(&s_pl[g][(w >> 8) & 255]);
+      const rpd_d4 pc = ld4(&s_pl[g][(w >> 16) & 255]);
+      rpd_d4 X, Q;
+      rpd_vertex_homog(pa, pb, pc, X, Q);
+      const double A[3] = {X.x / X.w, X.y / X.w, X.z / X.w};
+      double Pp[6][3];
+      foot_plane(pa, Cc, Pp[0]);
+      foot_line(pa, pb, A, Cc, Pp[1]);
+      foot_plane(pb, Cc, Pp[2]);
+      foot_line(pb, pc, A, Cc, Pp[3]);
+      foot_plane(pc, Cc, Pp[4]);
+      foot_line(pc, pa, A, Cc, Pp[5]);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const double* P0 = Pp[q];
+        const double* P1 = Pp[(q + 1) % 6];
+        const double tvl = tet_vol(P0, P1, Cc, A);
+        vol += tvl;
+        mx += tvl * (((P0[0] + P1[0]) + Cc[0]) + A[0]);
+        my += tvl * (((P0[1] + P1[1]) + Cc[1]) + A[1]);
+        mz += tvl * (((P0[2] + P1[2]) + Cc[2]) + A[2]);
+      }
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) {
+      vol += __shfl_xor(vol, off);
+      mx += __shfl_xor(mx, off);
+      my += __shfl_xor(my, off);
+      mz += __shfl_xor(mz, off);
+    }
+    if (j == 0) {
+      double* gm = o.geom + 4 * item;
+      const double sg = vol < 0 ? -1.0 : 1.0;
+      gm[0] = sg * vol;
+      gm[1] = vol != 0 ? mx / (4.0 * vol) : 0.0;
+      gm[2] = vol != 0 ? my / (4.0 * vol) : 0.0;
+      gm[3] = vol != 0 ? mz / (4.0 * vol) : 0.0;
+    }
+    // facet areas: lane per plane, deterministic serial sum over vertices
+    for (int p = j; p < nb_p; p += L) {
+      const rpd_d4 pp = ld4(&s_pl[g][p]);
+      const double nn = sqrt(pp.x * pp.x + pp.y * pp.y + pp.z * pp.z);
+      const double n[3] = {pp.x / nn, pp.y / nn, pp.z / nn};
+      double area = 0;
+      for (int v = 0; v < nb_v; ++v) {
+        const uint32_t w = s_ver[g][v];
+        const uint32_t t3[3] = {w & 255, (w >> 8) & 255, (w >> 16) & 255};
+        int sl = -1;
+        if (t3[0] == (uint32_t)p) sl = 0;
+        else if (t3[1] == (uint32_t)p) sl = 1;
+        else if (t3[2] == (uint32_t)p) sl = 2;
+        if (sl < 0) continue;
+        const rpd_d4 pn = ld4(&s_pl[g][t3[(sl + 1) % 3]]);
+        const rpd_d4 pq = ld4(&s_pl[g][t3[(sl + 2) % 3]]);
+        rpd_d4 X, Q;
+        rpd_vertex_homog(ld4(&s_pl[g][t3[0]]), ld4(&s_pl[g][t3[1]]),
+                         ld4(&s_pl[g][t3[2]]), X, Q);
+        const double A[3] = {X.x / X.w, X.y / X.w, X.z / X.w};
+        double F[3], En[3], Eq[3];
+        foot_plane(pp, Cc, F);
+        foot_line(pp, pn, A, Cc, En);
+        foot_line(pq, pp, A, Cc, Eq);
+        area += tri_area(F, En, A, n) + tri_area(Eq, F, A, n);
+      }
+      o.areas[(int64_t)item * o.stride_areas + p] = fabs(area);
+    }
+  }
+  if (status == RPD_SUCCESS) {
+    uint32_t* wd = o.words + (int64_t)item * o.stride_words;
+    for (int v = j; v < nb_v; v += L) wd[v] = s_ver[g][v];
+    for (int p = j; p < nb_p; p += L) wd[nb_v + p] = (uint32_t)s_src[g][p];
+    for (int q = j; q < nb_e; q += L) wd[nb_v + nb_p + q] = s_edge[g][q];
+  }
+  if (j == 0) {
+    o.hdr[item] = (uint32_t)nb_v | ((uint32_t)nb_p << 8) |
+                  ((uint32_t)nb_e << 16) | (status << 24);
+    if (status == kStatusRerunLarge) {
+      int r = atomicAdd(a.n_rerun_large, 1);
+      a.rerun_large[r] = (int32_t)cidx;
+    } else if (status == kStatusRerunExact) {
+      int r = atomicAdd(a.n_rerun_exact, 1);
+      a.rerun_exact[r] = (int32_t)cidx;
+    }
+  }
+}
+
+// ========================================================== compaction
+// per cell: location of its final slot; tier 0 small (index = cell),
+// tier 1 large / 2 exact (index = re-run position)
+__global__ __launch_bounds__(256) void k_mark_rerun(const int32_t* list,
+                                                    int32_t n, int32_t tier,
+                                                    int32_t* loc) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  loc[list[r]] = (tier << 28) | r;
+}
+
+struct Sizes3 {
+  int64_t cells, words, areas;
+};
+struct Sizes3Sum {
+  __host__ __device__ Sizes3 operator()(const Sizes3& a,
+                                        const Sizes3& b) const {
+    return Sizes3{a.cells + b.cells, a.words + b.words, a.areas + b.areas};
+  }
+};
+
+struct TierViews {
+  SlotView t[3];
+};
+
+__global__ __launch_bounds__(256) void k_sizes(int64_t n_cells,
+                                               const int32_t* loc,
+                                               TierViews tv, Sizes3* sz,
+                                               int64_t* drops /*9*/) {
+  __shared__ int32_t h[9];
+  if (threadIdx.x < 9) h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n_cells) {
+    const int32_t l = loc[c];
+    const int tier = l >> 28;
+    const int64_t idx = tier ? (l & ((1 << 28) - 1)) : c;
+    const uint32_t hd = tv.t[tier].hdr[idx];
+    const uint32_t st = hd >> 24;
+    Sizes3 z{0, 0, 0};
+    if (st == RPD_SUCCESS) {
+      const int64_t nv = hd & 255, np = (hd >> 8) & 255, ne = (hd >> 16) & 255;
+      z = Sizes3{1, nv + np + ne, np};
+    } else if (st < 9) {
+      atomicAdd(&h[st], 1);
+    }
+    sz[c] = z;
+  }
+  __syncthreads();
+  if (threadIdx.x < 9 && h[threadIdx.x])
+    atomicAdd((unsigned long long*)&drops[threadIdx.x],
+              (unsigned long long)h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_compact(
+    int64_t n_cells, const Cand* cand, const int32_t* loc, TierViews tv,
+    const Sizes3* off, rpd_cell* out_cells, uint32_t* out_words,
+    double* out_geom, double* out_areas, int32_t geometry) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  const int32_t l = loc[c];
+  const int tier = l >> 28;
+  const int64_t idx = tier ? (l & ((1 << 28) - 1)) : c;
+  const SlotView& sv = tv.t[tier];
+  const uint32_t hd = sv.hdr[idx];
+  if ((hd >> 24) != RPD_SUCCESS) return;
+  const Sizes3 o = off[c];
+  const int nv = hd & 255, np = (hd >> 8) & 255, ne = (hd >> 16) & 255;
+  rpd_cell rc;
+  rc.tet_id = cand[c].tet;
+  rc.seed = cand[c].seed;
+  rc.nb_v = (uint8_t)nv;
+  rc.nb_p = (uint8_t)np;
+  rc.nb_e = (uint8_t)ne;
+  rc.status = RPD_SUCCESS;
+  rc.reserved = 0;
+  rc.word_off = (uint64_t)o.words;
+  rc.area_off = (uint64_t)o.areas;
+  out_cells[o.cells] = rc;
+  const uint32_t* src = sv.words + idx * sv.stride_words;
+  for (int q = 0; q < nv + np + ne; ++q) out_words[o.words + q] = src[q];
+  if (geometry) {
+    const double* gs = sv.geom + 4 * idx;
+    for (int q = 0; q < 4; ++q) out_geom[4 * o.cells + q] = gs[q];
+    const double* as = sv.areas + idx * sv.stride_areas;
+    for (int q = 0; q < np; ++q) out_areas[o.areas + q] = as[q];
+  }
+}
+
+// active-plane statistics of the candidate masks
+__global__ __launch_bounds__(256) void k_mask_stats(int64_t n, const Cand* cand,
+                                                    DevSites S,
+                                                    unsigned long long* acc) {
+  __shared__ unsigned long long s_a, s_b;
+  if (threadIdx.x == 0) s_a = s_b = 0;
+  __syncthreads();
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) {
+    const Cand cd = cand[c];
+    const int64_t deg = S.knn_off[cd.seed + 1] - S.knn_off[cd.seed];
+    int64_t act = __popcll(cd.m0) + __popcll(cd.m1);
+    if (deg > 128) act += deg - 128;
+    atomicAdd(&s_a, (unsigned long long)act);
+    atomicAdd(&s_b, (unsigned long long)deg);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&acc[0], s_a);
+    atomicAdd(&acc[1], s_b);
+  }
+}
+
+}  // namespace rpd

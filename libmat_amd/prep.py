@@ -1,0 +1,155 @@
+"""Host-side preparation of RPD inputs (thin ctypes layer over rpd_prep.h).
+
+An :class:`Instance` holds exactly the arrays the reference's driver hands to
+``compute_clipped_voro_diagram_GPU`` (src/rpd3d/voronoi.h:52-61), in this
+package's layout: binary64 vertices and spheres, int32 tets, per-tet
+adjacency counts and face ids (io.cxx:238-335 semantics) and CSR neighbour
+lists (the RT 1-ring of triangulation.cxx:324-549, ascending site ids).
+"""
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._native import lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with code {rc}")
+
+
+def grid_mesh(n, jitter_rel=0.0, seed=0):
+    """Unit cube, n^3 cubes x 6 Kuhn tets; returns (verts[nv,3], tets[nt,4])."""
+    nv = (n + 1) ** 3
+    nt = 6 * n ** 3
+    verts = np.empty((nv, 3), np.float64)
+    tets = np.empty((nt, 4), np.int32)
+    _check(lib().rpd_synth_grid_mesh(n, float(jitter_rel), seed, _p(verts),
+                                     _p(tets)), "rpd_synth_grid_mesh")
+    return verts, tets
+
+
+def spheres(n, rule, seed, rmin=0.02, rmax=0.08):
+    """(x, y, z, r) per sphere; rule 0 uniform radii, 1 boundary rule, 2 GMM."""
+    out = np.empty((n, 4), np.float64)
+    _check(lib().rpd_synth_spheres(n, rule, rmin, rmax, seed, _p(out)),
+           "rpd_synth_spheres")
+    return out
+
+
+def tet_adjacency(n_vert, tets):
+    tets = np.ascontiguousarray(tets, np.int32)
+    nt = tets.shape[0]
+    v_adj = np.empty(n_vert, np.int32)
+    e_adj = np.empty((nt, 6), np.int32)
+    f_adj = np.empty((nt, 4), np.int32)
+    f_id = np.empty((nt, 4), np.int32)
+    n_sf = ctypes.c_int32(0)
+    _check(lib().rpd_tet_adjacency(n_vert, nt, _p(tets), _p(v_adj), _p(e_adj),
+                                   _p(f_adj), _p(f_id),
+                                   ctypes.cast(ctypes.byref(n_sf),
+                                               ctypes.c_void_p)),
+           "rpd_tet_adjacency")
+    return v_adj, e_adj, f_adj, f_id, n_sf.value
+
+
+def power_neighbors(xyzw, box, n_threads=0):
+    """Returns (valid[n], offsets[n+1], ids) of the power-diagram 1-rings."""
+    xyzw = np.ascontiguousarray(xyzw, np.float64)
+    box = np.ascontiguousarray(box, np.float64)
+    n = xyzw.shape[0]
+    h = ctypes.c_void_p()
+    _check(lib().rpd_power_neighbors(n, _p(xyzw), _p(box), n_threads,
+                                     ctypes.byref(h)), "rpd_power_neighbors")
+    try:
+        total = lib().rpd_nbr_total(h)
+        valid = np.empty(n, np.int32)
+        off = np.empty(n + 1, np.int64)
+        ids = np.empty(max(total, 1), np.int32)
+        _check(lib().rpd_nbr_get(h, _p(valid), _p(off), _p(ids)),
+               "rpd_nbr_get")
+    finally:
+        lib().rpd_nbr_free(h)
+    return valid, off, ids[:total]
+
+
+@dataclass
+class Instance:
+    verts: np.ndarray          # (n_vert, 3) f64
+    tets: np.ndarray           # (n_tet, 4) i32
+    v_adj: np.ndarray          # (n_vert,) i32
+    e_adj: np.ndarray          # (n_tet, 6) i32
+    f_adj: np.ndarray          # (n_tet, 4) i32
+    f_id: np.ndarray           # (n_tet, 4) i32
+    n_sf: int
+    sites: np.ndarray          # (n_site, 4) f64: x, y, z, w = r^2
+    flags: np.ndarray          # (n_site,) u32
+    knn_off: np.ndarray        # (n_site + 1,) i64
+    knn_ids: np.ndarray        # i32, ascending per site
+    site2sphere: np.ndarray = field(default=None)  # (n_site,) i32
+
+    @property
+    def n_vert(self):
+        return self.verts.shape[0]
+
+    @property
+    def n_tet(self):
+        return self.tets.shape[0]
+
+    @property
+    def n_site(self):
+        return self.sites.shape[0]
+
+    @property
+    def site_k(self):
+        d = np.diff(self.knn_off)
+        return int(d.max()) if d.size else 0
+
+
+UNIT_BOX = np.array([-1e-6, -1e-6, -1e-6, 1 + 1e-6, 1 + 1e-6, 1 + 1e-6])
+
+
+def build_instance(verts, tets, xyzr, box=UNIT_BOX, n_threads=0):
+    """Adjacency, power neighbours and sites of the non-hidden spheres.
+
+    Sites are the spheres whose power cell meets ``box``, in ascending sphere
+    order; every site is flagged (a full RPD, as RPD3D_GPU::calculate)."""
+    verts = np.ascontiguousarray(verts, np.float64)
+    tets = np.ascontiguousarray(tets, np.int32)
+    v_adj, e_adj, f_adj, f_id, n_sf = tet_adjacency(verts.shape[0], tets)
+    xyzw = np.ascontiguousarray(xyzr, np.float64).copy()
+    xyzw[:, 3] = xyzw[:, 3] * xyzw[:, 3]
+    valid, off, ids = power_neighbors(xyzw, box, n_threads)
+    keep = np.nonzero(valid)[0].astype(np.int32)
+    remap = -np.ones(xyzw.shape[0], np.int64)
+    remap[keep] = np.arange(keep.size)
+    deg = np.diff(off)[keep]
+    knn_off = np.zeros(keep.size + 1, np.int64)
+    np.cumsum(deg, out=knn_off[1:])
+    sel = np.concatenate([np.arange(off[i], off[i + 1]) for i in keep]) \
+        if keep.size else np.zeros(0, np.int64)
+    knn_ids = remap[ids[sel]].astype(np.int32) if sel.size else \
+        np.zeros(0, np.int32)
+    assert (knn_ids >= 0).all()
+    return Instance(verts, tets, v_adj, e_adj, f_adj, f_id, n_sf,
+                    np.ascontiguousarray(xyzw[keep]),
+                    np.ones(keep.size, np.uint32), knn_off, knn_ids, keep)
+
+
+def config1(seed=0, n_threads=0):
+    """10^3 Kuhn grid (6,000 tets), 500 spheres r ~ U[0.02, 0.08]."""
+    verts, tets = grid_mesh(10, 0.0, seed)
+    return build_instance(verts, tets, spheres(500, 0, seed, 0.02, 0.08),
+                          n_threads=n_threads)
+
+
+def config2(seed=0, n_threads=0, n=64, n_spheres=50000):
+    """64^3 jittered grid (1,572,864 tets), 50,000 spheres, boundary rule."""
+    verts, tets = grid_mesh(n, 0.1, seed)
+    return build_instance(verts, tets, spheres(n_spheres, 1, seed),
+                          n_threads=n_threads)

@@ -1,0 +1,222 @@
+"""ctypes layer over the C-ABI of include/rpd_capi.h.
+
+:class:`Context` mirrors the reference's one-shot entry point
+``compute_clipped_voro_diagram_GPU`` (src/rpd3d/voronoi.h:52-61) as an
+upload / run / download sequence on a resident instance.  There is no CPU
+fallback: constructing a Context without the native library or without a
+GPU raises.
+"""
+import ctypes
+
+import numpy as np
+
+from ._native import lib
+
+CELL_DTYPE = np.dtype([
+    ("tet", np.int32), ("seed", np.int32), ("nb_v", np.uint8),
+    ("nb_p", np.uint8), ("nb_e", np.uint8), ("status", np.uint8),
+    ("reserved", np.uint32), ("word_off", np.uint64), ("area_off", np.uint64),
+])
+assert CELL_DTYPE.itemsize == 32
+
+SUCCESS = 4
+STATUS_NAMES = ["triangle_overflow", "vertex_overflow",
+                "inconsistent_boundary", "security_radius_not_reached",
+                "success", "needs_exact_predicates", "no_intersection",
+                "edge_overflow", "needs_perturb"]
+
+
+class InstanceDesc(ctypes.Structure):
+    _fields_ = [("n_vert", ctypes.c_int32), ("n_tet", ctypes.c_int32),
+                ("n_site", ctypes.c_int32)] + [
+        (n, ctypes.c_void_p) for n in
+        ("verts", "tets", "v_adj", "e_adj", "f_adj", "f_id", "sites",
+         "flags", "knn_off", "knn_ids")]
+
+
+class RunCfg(ctypes.Structure):
+    _fields_ = [("cells_per_wave", ctypes.c_int32),
+                ("use_active_mask", ctypes.c_int32),
+                ("compute_geometry", ctypes.c_int32),
+                ("force_exact", ctypes.c_int32),
+                ("small_tier", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n_cells", ctypes.c_int64), ("n_candidates", ctypes.c_int64),
+                ("n_words", ctypes.c_int64), ("n_areas", ctypes.c_int64),
+                ("n_dropped", ctypes.c_int64 * 9),
+                ("n_rerun_large", ctypes.c_int64),
+                ("n_rerun_exact", ctypes.c_int64),
+                ("n_active_planes", ctypes.c_int64),
+                ("n_all_planes", ctypes.c_int64),
+                ("n_bbox_fallback", ctypes.c_int64),
+                ("tet_k", ctypes.c_int32), ("site_k", ctypes.c_int32),
+                ("ms_total", ctypes.c_double),
+                ("ms_phase", ctypes.c_double * 8)]
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+
+def _declare_capi(L):
+    vp, i32 = ctypes.c_void_p, ctypes.c_int32
+    L.rpd_ctx_create.argtypes = [i32, ctypes.POINTER(vp)]
+    L.rpd_ctx_create.restype = ctypes.c_int
+    L.rpd_ctx_destroy.argtypes = [vp]
+    L.rpd_ctx_destroy.restype = None
+    L.rpd_last_error.argtypes = [vp]
+    L.rpd_last_error.restype = ctypes.c_char_p
+    L.rpd_ctx_stream.argtypes = [vp]
+    L.rpd_ctx_stream.restype = vp
+    L.rpd_upload.argtypes = [vp, ctypes.POINTER(InstanceDesc)]
+    L.rpd_upload.restype = ctypes.c_int
+    L.rpd_update_sites.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.rpd_update_sites.restype = ctypes.c_int
+    L.rpd_run.argtypes = [vp, ctypes.POINTER(RunCfg)]
+    L.rpd_run.restype = ctypes.c_int
+    L.rpd_sync.argtypes = [vp]
+    L.rpd_sync.restype = ctypes.c_int
+    L.rpd_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.rpd_get_stats.restype = ctypes.c_int
+    L.rpd_download_cells.argtypes = [vp, vp, vp, vp, vp]
+    L.rpd_download_cells.restype = ctypes.c_int
+    L.rpd_download_relation.argtypes = [vp, vp, vp]
+    L.rpd_download_relation.restype = ctypes.c_int
+
+
+CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
+                "rpd_ctx_stream", "rpd_upload", "rpd_update_sites", "rpd_run",
+                "rpd_sync", "rpd_get_stats", "rpd_download_cells",
+                "rpd_download_relation")
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class RPDResult:
+    """Emitted cells of one run, in (tet, ascending site) order."""
+
+    def __init__(self, cells, words, geom, areas):
+        self.cells = cells
+        self.words = words
+        self.geom = geom.reshape(-1, 4) if geom is not None else None
+        self.areas = areas
+
+    def __len__(self):
+        return self.cells.size
+
+    def cell(self, q):
+        c = self.cells[q]
+        o = int(c["word_off"])
+        nv, np_, ne = int(c["nb_v"]), int(c["nb_p"]), int(c["nb_e"])
+        ver = self.words[o:o + nv]
+        src = self.words[o + nv:o + nv + np_].view(np.int32)
+        edge = self.words[o + nv + np_:o + nv + np_ + ne]
+        a = int(c["area_off"])
+        area = self.areas[a:a + np_] if self.areas is not None else None
+        return ver, src, edge, area
+
+
+class Context:
+    def __init__(self, device=0):
+        L = lib()
+        self._L = L
+        h = ctypes.c_void_p()
+        rc = L.rpd_ctx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"rpd_ctx_create failed ({rc}): no usable GPU")
+        self._h = h
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.rpd_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._L.rpd_last_error(self._h).decode()
+            raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+    @property
+    def stream(self):
+        return self._L.rpd_ctx_stream(self._h)
+
+    def upload(self, inst):
+        arrs = dict(
+            verts=np.ascontiguousarray(inst.verts, np.float64),
+            tets=np.ascontiguousarray(inst.tets, np.int32),
+            v_adj=np.ascontiguousarray(inst.v_adj, np.int32),
+            e_adj=np.ascontiguousarray(inst.e_adj, np.int32),
+            f_adj=np.ascontiguousarray(inst.f_adj, np.int32),
+            f_id=np.ascontiguousarray(inst.f_id, np.int32),
+            sites=np.ascontiguousarray(inst.sites, np.float64),
+            flags=np.ascontiguousarray(inst.flags, np.uint32),
+            knn_off=np.ascontiguousarray(inst.knn_off, np.int64),
+            knn_ids=np.ascontiguousarray(inst.knn_ids, np.int32))
+        d = InstanceDesc(inst.n_vert, inst.n_tet, inst.n_site,
+                         *[_p(arrs[n]) for n in
+                           ("verts", "tets", "v_adj", "e_adj", "f_adj",
+                            "f_id", "sites", "flags", "knn_off", "knn_ids")])
+        self._check(self._L.rpd_upload(self._h, ctypes.byref(d)),
+                    "rpd_upload")
+        self.n_tet = inst.n_tet
+
+    def update_sites(self, sites, flags, knn_off, knn_ids):
+        sites = np.ascontiguousarray(sites, np.float64)
+        flags = np.ascontiguousarray(flags, np.uint32)
+        knn_off = np.ascontiguousarray(knn_off, np.int64)
+        knn_ids = np.ascontiguousarray(knn_ids, np.int32)
+        self._check(self._L.rpd_update_sites(
+            self._h, sites.shape[0], _p(sites), _p(flags), _p(knn_off),
+            _p(knn_ids)), "rpd_update_sites")
+
+    def run(self, cells_per_wave=8, use_active_mask=True, geometry=True,
+            force_exact=False, small_tier=True):
+        cfg = RunCfg(cells_per_wave, int(use_active_mask), int(geometry),
+                     int(force_exact), int(small_tier))
+        self._check(self._L.rpd_run(self._h, ctypes.byref(cfg)), "rpd_run")
+        return self.stats()
+
+    def stats(self):
+        st = Stats()
+        self._check(self._L.rpd_get_stats(self._h, ctypes.byref(st)),
+                    "rpd_get_stats")
+        return st.as_dict()
+
+    def download(self, geometry=True):
+        st = self.stats()
+        cells = np.empty(st["n_cells"], CELL_DTYPE)
+        words = np.empty(st["n_words"], np.uint32)
+        geom = np.empty(4 * st["n_cells"], np.float64) if geometry else None
+        areas = np.empty(st["n_areas"], np.float64) if geometry else None
+        nul = ctypes.c_void_p()
+        self._check(self._L.rpd_download_cells(
+            self._h, _p(cells), _p(words),
+            _p(geom) if geometry else nul, _p(areas) if geometry else nul),
+            "rpd_download_cells")
+        return RPDResult(cells, words, geom, areas)
+
+    def relation(self):
+        st = self.stats()
+        off = np.empty(self.n_tet + 1, np.int64)
+        ids = np.empty(max(st["n_candidates"], 1), np.int32)
+        self._check(self._L.rpd_download_relation(self._h, _p(off), _p(ids)),
+                    "rpd_download_relation")
+        return off, ids[:st["n_candidates"]]
