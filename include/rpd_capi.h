@@ -106,10 +106,14 @@ typedef struct {
   int64_t n_all_planes;       /* all neighbour planes of all candidates  */
   int64_t n_bbox_fallback;    /* spheres whose candidate box fell back to
                                  the whole domain                        */
+  int64_t n_bin_registrations; /* (sphere, bin) pairs of the candidate grid */
   int32_t tet_k;              /* max candidates of one tet               */
   int32_t site_k;             /* max neighbours of one site              */
   double ms_total;            /* device time of the last rpd_run         */
-  double ms_phase[8];         /* candidates, clip, rerun, compaction ... */
+  double ms_phase[8];         /* candidates, clip, re-runs, compaction   */
+  double ms_kernel[8];        /* site boxes, binning, relation count,
+                                 relation fill, plane masks, clip (small
+                                 tier), compaction                       */
 } rpd_stats;
 
 typedef struct rpd_ctx rpd_ctx;
@@ -136,6 +140,11 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg);
 int rpd_sync(rpd_ctx* ctx);
 
 int rpd_get_stats(const rpd_ctx* ctx, rpd_stats* out);
+
+/* Measurement aid: one plain device copy of `bytes` (multiple of 16) with a
+ * streaming kernel, whose HBM traffic (bytes read, bytes written) is known;
+ * used to calibrate hardware traffic counters. */
+int rpd_calib_copy(rpd_ctx* ctx, int64_t bytes);
 
 /* D2H copies of the last run's results; NULL pointers are skipped.
  * cells[n_cells], words[n_words], geom[4*n_cells] (volume, centroid x,y,z),

@@ -81,14 +81,15 @@ struct rpd_ctx {
   Buf vert, tet_v, tet_fid, tet_meta, dom_dev;
   Buf site, flags, knn_off, knn_ids;
   // pipeline
-  Buf bbox, bin_count, bin_start, bin_ids, tet_count, tet_off, cand, rel_ids;
+  Buf bbox, reg_n, reg_off, keys_a, keys_b, bin_start, bin_ids;
+  Buf tet_count, tet_off, cand, rel_ids;
   Buf counters;  // int32[8]: fallback, n_rerun_large, n_rerun_exact
   Buf rerun_large, rerun_exact, loc, sizes, offs, drops, stats64;
   Buf scan_tmp;
   TierBufs small, large, exact;
   Buf out_cells, out_words, out_geom, out_areas;
   rpd_stats st;
-  hipEvent_t ev[10];
+  hipEvent_t ev[16];
   bool events = false;
 };
 
@@ -138,7 +139,7 @@ int rpd_ctx_create(int32_t device, rpd_ctx** out) {
     delete ctx;
     return -3;
   }
-  for (int i = 0; i < 10; ++i)
+  for (int i = 0; i < 16; ++i)
     if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
       delete ctx;
       return -3;
@@ -154,7 +155,7 @@ void rpd_ctx_destroy(rpd_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->events)
-    for (int i = 0; i < 10; ++i) (void)hipEventDestroy(ctx->ev[i]);
+    for (int i = 0; i < 16; ++i) (void)hipEventDestroy(ctx->ev[i]);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -373,32 +374,53 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
                          ctx->dom_dev.as<double>(), ctx->rho,
                          ctx->bbox.as<double>(), cnt + 0);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(ctx->bin_count.need((n_bins + 1) * 4));
-    HIP_TRY(ctx->bin_start.need((n_bins + 1) * 4));
-    HIP_TRY(hipMemsetAsync(ctx->bin_count.p, 0, (n_bins + 1) * 4, s));
-    if (ns > 0)
-      hipLaunchKernelGGL((k_bin_register<false>), dim3(ceil_div(ns, 256)),
-                         dim3(256), 0, s, ns, ctx->bbox.as<double>(), bg,
-                         ctx->bin_count.as<int32_t>(), nullptr);
+    HIP_TRY(hipEventRecord(ctx->ev[10], s));
+    // sphere -> bin registrations: count, expand, radix sort by
+    // (bin, sphere), bin starts by binary search
+    HIP_TRY(ctx->reg_n.need(((size_t)ns + 1) * 8));
+    HIP_TRY(ctx->reg_off.need(((size_t)ns + 1) * 8));
+    hipLaunchKernelGGL(k_bin_nreg, dim3(ceil_div(ns + 1, 256)), dim3(256), 0,
+                       s, ns, ctx->bbox.as<double>(), bg,
+                       ctx->reg_n.as<int64_t>());
     HIP_TRY(hipGetLastError());
-    HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->bin_count.as<int32_t>(),
-                                   ctx->bin_start.as<int32_t>(), n_bins + 1));
-    int32_t n_reg = 0;
-    HIP_TRY(hipMemcpyAsync(&n_reg, ctx->bin_start.as<int32_t>() + n_bins, 4,
+    HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->reg_n.as<int64_t>(),
+                                   ctx->reg_off.as<int64_t>(), ns + 1));
+    int64_t n_reg = 0;
+    HIP_TRY(hipMemcpyAsync(&n_reg, ctx->reg_off.as<int64_t>() + ns, 8,
                            hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    HIP_TRY(ctx->bin_ids.need((size_t)std::max(n_reg, 1) * 4));
-    HIP_TRY(hipMemcpyAsync(ctx->bin_count.p, ctx->bin_start.p,
-                           (n_bins + 1) * 4, hipMemcpyDeviceToDevice, s));
-    if (ns > 0)
-      hipLaunchKernelGGL((k_bin_register<true>), dim3(ceil_div(ns, 256)),
-                         dim3(256), 0, s, ns, ctx->bbox.as<double>(), bg,
-                         ctx->bin_count.as<int32_t>(),
-                         ctx->bin_ids.as<int32_t>());
-    hipLaunchKernelGGL(k_bin_sort, dim3(ceil_div(n_bins, 256)), dim3(256), 0,
-                       s, n_bins, ctx->bin_start.as<int32_t>(),
+    if (n_reg >= (1LL << 31))
+      return fail(ctx, "rpd_run: too many sphere-bin registrations");
+    ctx->st.n_bin_registrations = n_reg;
+    HIP_TRY(ctx->keys_a.need((size_t)std::max<int64_t>(n_reg, 1) * 8));
+    HIP_TRY(ctx->keys_b.need((size_t)std::max<int64_t>(n_reg, 1) * 8));
+    HIP_TRY(ctx->bin_start.need((n_bins + 1) * 4));
+    HIP_TRY(ctx->bin_ids.need((size_t)std::max<int64_t>(n_reg, 1) * 4));
+    if (n_reg > 0) {
+      hipLaunchKernelGGL(k_bin_expand, dim3(ceil_div(n_reg, 256)), dim3(256),
+                         0, s, ns, n_reg, ctx->bbox.as<double>(), bg,
+                         ctx->reg_off.as<int64_t>(),
+                         ctx->keys_a.as<uint64_t>());
+      HIP_TRY(hipGetLastError());
+      int end_bit = 32;
+      while ((1LL << (end_bit - 32)) < n_bins + 1) ++end_bit;
+      size_t tmp = 0;
+      HIP_TRY(hipcub::DeviceRadixSort::SortKeys(
+          nullptr, tmp, ctx->keys_a.as<uint64_t>(), ctx->keys_b.as<uint64_t>(),
+          (int)n_reg, 0, end_bit, s));
+      HIP_TRY(ctx->scan_tmp.need(tmp));
+      HIP_TRY(hipcub::DeviceRadixSort::SortKeys(
+          ctx->scan_tmp.p, tmp, ctx->keys_a.as<uint64_t>(),
+          ctx->keys_b.as<uint64_t>(), (int)n_reg, 0, end_bit, s));
+    }
+    hipLaunchKernelGGL(k_bin_starts,
+                       dim3(ceil_div(std::max<int64_t>(n_bins + 1, n_reg), 256)),
+                       dim3(256), 0, s, n_bins, n_reg,
+                       ctx->keys_b.as<uint64_t>(),
+                       ctx->bin_start.as<int32_t>(),
                        ctx->bin_ids.as<int32_t>());
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[11], s));
     HIP_TRY(ctx->tet_count.need(((size_t)nt + 1) * 8));
     HIP_TRY(ctx->tet_off.need(((size_t)nt + 1) * 8));
     HIP_TRY(hipMemsetAsync(ctx->tet_count.as<int64_t>() + nt, 0, 8, s));
@@ -407,6 +429,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
                        ctx->bin_ids.as<int32_t>(),
                        ctx->tet_count.as<int64_t>());
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[12], s));
     HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
                                    ctx->tet_off.as<int64_t>(), nt + 1));
   }
@@ -420,6 +443,11 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
                      S, bg, ctx->bin_start.as<int32_t>(),
                      ctx->bin_ids.as<int32_t>(), ctx->tet_off.as<int64_t>(),
                      ctx->cand.as<Cand>(), ctx->rel_ids.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[13], s));
+  if (cfg.use_active_mask && n_cand > 0)
+    hipLaunchKernelGGL(k_cand_mask, dim3(ceil_div(n_cand, 256)), dim3(256), 0,
+                       s, M, S, n_cand, ctx->cand.as<Cand>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->ev[1], s));
 
@@ -525,7 +553,9 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   HIP_TRY(ctx->out_geom.need((size_t)std::max<int64_t>(tot.cells, 1) * 32));
   HIP_TRY(ctx->out_areas.need((size_t)std::max<int64_t>(tot.areas, 1) * 8));
   if (n_cand > 0)
-    hipLaunchKernelGGL(k_compact, dim3(ceil_div(n_cand, 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_compact,
+                       dim3(ceil_div(ceil_div(n_cand, kCompactRun), 4)),
+                       dim3(256), 0, s,
                        n_cand, ctx->cand.as<Cand>(), ctx->loc.as<int32_t>(), tv,
                        ctx->offs.as<Sizes3>(), ctx->out_cells.as<rpd_cell>(),
                        ctx->out_words.as<uint32_t>(),
@@ -579,8 +609,51 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   st.ms_phase[2] = ms;
   HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]));
   st.ms_phase[3] = ms;
+  // kernel-level: site boxes, binning, relation count, relation fill, masks
+  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[10]));
+  st.ms_kernel[0] = ms;
+  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[10], ctx->ev[11]));
+  st.ms_kernel[1] = ms;
+  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[11], ctx->ev[12]));
+  st.ms_kernel[2] = ms;
+  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[12], ctx->ev[13]));
+  st.ms_kernel[3] = ms;
+  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[13], ctx->ev[1]));
+  st.ms_kernel[4] = ms;
+  st.ms_kernel[5] = st.ms_phase[1];
+  st.ms_kernel[6] = st.ms_phase[3];
   st.n_bbox_fallback = host_cnt[0];
   st.tet_k = (int32_t)st64[2];
+  return 0;
+}
+
+// Plain streaming copy of `bytes` (16-byte vector loads and stores), the
+// reference kernel of known traffic for calibrating HBM counters.
+__global__ __launch_bounds__(256) void k_calib_copy(const uint4* __restrict__ a,
+                                                    uint4* __restrict__ b,
+                                                    int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+int rpd_calib_copy(rpd_ctx* ctx, int64_t bytes) {
+  if (!ctx || bytes <= 0 || (bytes & 15)) return -1;
+  HIP_TRY(hipSetDevice(ctx->device));
+  void *a = nullptr, *b = nullptr;
+  HIP_TRY(hipMalloc(&a, bytes));
+  hipError_t e = hipMalloc(&b, bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    return fail(ctx, "rpd_calib_copy: allocation failed", -2);
+  }
+  (void)hipMemsetAsync(a, 1, bytes, ctx->stream);
+  hipLaunchKernelGGL(k_calib_copy, dim3(4096), dim3(256), 0, ctx->stream,
+                     (const uint4*)a, (uint4*)b, bytes / 16);
+  e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(ctx, "rpd_calib_copy failed", -2);
   return 0;
 }
 

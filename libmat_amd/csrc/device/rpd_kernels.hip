@@ -323,46 +323,81 @@ __global__ __launch_bounds__(64) void k_site_bbox(DevSites S,
   }
 }
 
-template <bool kFill>
-__global__ __launch_bounds__(256) void k_bin_register(
-    int32_t n_site, const double* bbox, BinGrid g, int32_t* bin_count,
-    int32_t* bin_ids) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_site) return;
-  const double* b = bbox + 6 * (int64_t)i;
-  if (b[0] > b[3] || b[1] > b[4] || b[2] > b[5]) return;
-  int x0 = bin_coord(b[0], g.org[0], g.inv_bs, g.dim[0]);
-  int y0 = bin_coord(b[1], g.org[1], g.inv_bs, g.dim[1]);
-  int z0 = bin_coord(b[2], g.org[2], g.inv_bs, g.dim[2]);
-  int x1 = bin_coord(b[3], g.org[0], g.inv_bs, g.dim[0]);
-  int y1 = bin_coord(b[4], g.org[1], g.inv_bs, g.dim[1]);
-  int z1 = bin_coord(b[5], g.org[2], g.inv_bs, g.dim[2]);
-  for (int z = z0; z <= z1; ++z)
-    for (int y = y0; y <= y1; ++y)
-      for (int x = x0; x <= x1; ++x) {
-        int64_t bin = x + (int64_t)g.dim[0] * (y + (int64_t)g.dim[1] * z);
-        int slot = atomicAdd(&bin_count[bin], 1);
-        if (kFill) bin_ids[slot] = i;
-      }
+// bin range of a sphere's candidate box; false if empty
+__device__ __forceinline__ bool bin_range(const double* b, const BinGrid& g,
+                                          int r[6]) {
+  if (b[0] > b[3] || b[1] > b[4] || b[2] > b[5]) return false;
+  for (int a = 0; a < 3; ++a) {
+    r[a] = bin_coord(b[a], g.org[a], g.inv_bs, g.dim[a]);
+    r[3 + a] = bin_coord(b[3 + a], g.org[a], g.inv_bs, g.dim[a]);
+  }
+  return true;
 }
 
-// bin_cursor holds the start offsets on entry (filled slots are taken by
-// atomics from it); sort each bin's list ascending afterwards
-__global__ __launch_bounds__(256) void k_bin_sort(int64_t n_bins,
-                                                  const int32_t* start,
-                                                  int32_t* ids) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n_bins) return;
-  const int s = start[b], e = start[b + 1];
-  for (int q = s + 1; q < e; ++q) {
-    int v = ids[q];
-    int r = q - 1;
-    while (r >= s && ids[r] > v) {
-      ids[r + 1] = ids[r];
-      --r;
-    }
-    ids[r + 1] = v;
+// number of bins each sphere registers into
+__global__ __launch_bounds__(256) void k_bin_nreg(int32_t n_site,
+                                                  const double* bbox,
+                                                  BinGrid g, int64_t* nreg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n_site) return;
+  if (i == n_site) {
+    nreg[i] = 0;
+    return;
   }
+  int r[6];
+  nreg[i] = bin_range(bbox + 6 * (int64_t)i, g, r)
+                ? (int64_t)(r[3] - r[0] + 1) * (r[4] - r[1] + 1) *
+                      (r[5] - r[2] + 1)
+                : 0;
+}
+
+// one thread per registration: key = bin << 32 | sphere
+__global__ __launch_bounds__(256) void k_bin_expand(int32_t n_site,
+                                                    int64_t n_reg,
+                                                    const double* bbox,
+                                                    BinGrid g,
+                                                    const int64_t* reg_off,
+                                                    uint64_t* keys) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_reg) return;
+  // sphere i with reg_off[i] <= q < reg_off[i+1]
+  int lo = 0, hi = n_site;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (reg_off[mid] <= q)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int i = lo;
+  int r[6];
+  bin_range(bbox + 6 * (int64_t)i, g, r);
+  const int64_t k = q - reg_off[i];
+  const int64_t nx = r[3] - r[0] + 1, ny = r[4] - r[1] + 1;
+  const int64_t x = r[0] + k % nx, y = r[1] + (k / nx) % ny,
+                z = r[2] + k / (nx * ny);
+  const uint64_t bin = x + (uint64_t)g.dim[0] * (y + (uint64_t)g.dim[1] * z);
+  keys[q] = (bin << 32) | (uint32_t)i;
+}
+
+// bin_start[b] = first registration of bin >= b in the sorted keys
+__global__ __launch_bounds__(256) void k_bin_starts(int64_t n_bins,
+                                                    int64_t n_reg,
+                                                    const uint64_t* keys,
+                                                    int32_t* bin_start,
+                                                    int32_t* bin_ids) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < n_reg) bin_ids[b] = (int32_t)(keys[b] & 0xffffffffu);
+  if (b > n_bins) return;
+  int64_t lo = 0, hi = n_reg;  // first index with key >> 32 >= b
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)(keys[mid] >> 32) < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  bin_start[b] = (int32_t)lo;
 }
 
 struct TetLoad {
@@ -438,44 +473,60 @@ __global__ __launch_bounds__(256) void k_tet_fill(
   load_tet(M, t, L);
   const int64_t b = tet_bin(L, g);
   int64_t o = tet_off[t];
-  // corners of the tet cell: homogeneous points of the initial dual
-  // triangles, with filter permanents
-  rpd_d4 pl[4], X[4], Q[4];
-  tet_planes(L.V, pl);
-  for (int k = 0; k < 4; ++k)
-    rpd_vertex_homog(pl[c_init_ver[k][0]], pl[c_init_ver[k][1]],
-                     pl[c_init_ver[k][2]], X[k], Q[k]);
   for (int q = bin_start[b]; q < bin_start[b + 1]; ++q) {
     const int i = bin_ids[q];
     if (!relation(S, L, i)) continue;
-    const rpd_d4 si = ld4(S.site + i);
-    const int64_t k0 = S.knn_off[i];
-    const int deg = (int)(S.knn_off[i + 1] - k0);
-    uint64_t m0 = 0, m1 = 0;
-    const int lim = deg < 128 ? deg : 128;
-    for (int k = 0; k < lim; ++k) {
-      const rpd_d4 e = rpd_bisector(si, ld4(S.site + S.knn_ids[k0 + k]));
-      // inactive iff all four corners are certainly not in conflict
-      bool inactive = true;
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        inactive = inactive && rpd_filtered_sign4(e, X[c], Q[c]) < 0;
-      if (!inactive) {
-        if (k < 64)
-          m0 |= 1ull << k;
-        else
-          m1 |= 1ull << (k - 64);
-      }
-    }
     Cand c;
     c.tet = t;
     c.seed = i;
-    c.m0 = m0;
-    c.m1 = m1;
+    c.m0 = 0;
+    c.m1 = 0;
     cand[o] = c;
     rel_ids[o] = i;
     ++o;
   }
+}
+
+// One thread per candidate cell: bit k of the mask is set unless the k-th
+// neighbour's bisector is certainly positive at all four corners of the
+// tet cell (then it cannot cut any state of the cell: every cell vertex
+// lies in the convex hull of the corners).
+__global__ __launch_bounds__(256) void k_cand_mask(DevMesh M, DevSites S,
+                                                   int64_t n_cand,
+                                                   Cand* cand) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cand) return;
+  const int t = cand[c].tet, i = cand[c].seed;
+  TetLoad L;
+  load_tet(M, t, L);
+  rpd_d4 pl[4];
+  tet_planes(L.V, pl);
+  // corners = initial dual triangles (1,3,2), (0,2,3), (0,3,1), (0,1,2)
+  rpd_d4 X0, Q0, X1, Q1, X2, Q2, X3, Q3;
+  rpd_vertex_homog(pl[1], pl[3], pl[2], X0, Q0);
+  rpd_vertex_homog(pl[0], pl[2], pl[3], X1, Q1);
+  rpd_vertex_homog(pl[0], pl[3], pl[1], X2, Q2);
+  rpd_vertex_homog(pl[0], pl[1], pl[2], X3, Q3);
+  const rpd_d4 si = ld4(S.site + i);
+  const int64_t k0 = S.knn_off[i];
+  const int deg = (int)(S.knn_off[i + 1] - k0);
+  uint64_t m0 = 0, m1 = 0;
+  const int lim = deg < 128 ? deg : 128;
+  for (int k = 0; k < lim; ++k) {
+    const rpd_d4 e = rpd_bisector(si, ld4(S.site + S.knn_ids[k0 + k]));
+    const bool inactive = rpd_filtered_sign4(e, X0, Q0) < 0 &&
+                          rpd_filtered_sign4(e, X1, Q1) < 0 &&
+                          rpd_filtered_sign4(e, X2, Q2) < 0 &&
+                          rpd_filtered_sign4(e, X3, Q3) < 0;
+    if (!inactive) {
+      if (k < 64)
+        m0 |= 1ull << k;
+      else
+        m1 |= 1ull << (k - 64);
+    }
+  }
+  cand[c].m0 = m0;
+  cand[c].m1 = m1;
 }
 
 // ============================================================ clipping
@@ -558,6 +609,7 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
   __shared__ uint8_t s_bn[G][P];
   __shared__ uint8_t s_cyc[G][P + 1];
   __shared__ uint32_t s_ver[G][V];
+  __shared__ uint8_t s_perm[G][V];
   __shared__ uint32_t s_edge[G][E];
 
   const int lane = threadIdx.x;
@@ -579,15 +631,14 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
 
   // ---- init the cell as its tet (convex_cell.cu:116-214)
   const int4 tv = a.M.tet_v[t];
-  const int4 tf = a.M.tet_fid[t];
   const TetMeta tm = a.M.tet_meta[t];
-  double4 TV[4] = {a.M.vert[tv.x], a.M.vert[tv.y], a.M.vert[tv.z],
-                   a.M.vert[tv.w]};
   {
-    rpd_d4 pl[4];
-    tet_planes(TV, pl);
     if (j < 4) {
-      const rpd_d4 p = pl[j];
+      // face j = tet_faces_lvid[j]: {2,1,3}, {0,2,3}, {1,0,3}, {0,1,2}
+      const int ia = j == 0 ? tv.z : (j == 2 ? tv.y : tv.x);
+      const int ib = j == 0 ? tv.y : (j == 1 ? tv.z : (j == 2 ? tv.x : tv.y));
+      const int ic = j == 3 ? tv.z : tv.w;
+      const rpd_d4 p = tri(a.M.vert[ia], a.M.vert[ib], a.M.vert[ic]);
       s_pl[g][j] = make_double4(p.x, p.y, p.z, p.w);
       s_src[g][j] = -1 - j;
       s_h[g][j] = (uint8_t)meta_byte(tm, j);
@@ -598,7 +649,6 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
       s_edge[g][j] = c_init_edge[j][0] | (c_init_edge[j][1] << 8) |
                      (meta_byte(tm, 8 + j) << 16);
   }
-  (void)tf;
   wsync();
   int nb_v = 4, nb_p = 4, nb_e = 6;
   uint32_t status = RPD_SUCCESS;
@@ -681,9 +731,7 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
     // ---- move conflicting vertices to the tail exactly as the serial
     // swap loop does: i scans from the front, end from the back
     {
-      int src[VS];
-#pragma unroll
-      for (int q = 0; q < VS; ++q) src[q] = -1;
+      // s_perm[pos] = old index of the vertex that ends at pos
       int i = 0, end = nb_v, occ = 0;
       while (i < end) {
         const bool cf = occ < 64 ? ((conf_lo >> occ) & 1)
@@ -698,22 +746,22 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
           pos = end;
           nxt = end;
         }
-        if (pos % L == j) {
-          const int qq = pos / L;
-#pragma unroll
-          for (int q = 0; q < VS; ++q)
-            if (q == qq) src[q] = occ;
-        }
+        if (j == 0) s_perm[g][pos] = (uint8_t)occ;
         occ = nxt;
       }
+      wsync();
       uint32_t val[VS];
 #pragma unroll
-      for (int q = 0; q < VS; ++q)
-        val[q] = (src[q] >= 0) ? s_ver[g][src[q]] : 0u;
+      for (int q = 0; q < VS; ++q) {
+        const int pos = q * L + j;
+        val[q] = pos < nb_v ? s_ver[g][s_perm[g][pos]] : 0u;
+      }
       wsync();
 #pragma unroll
-      for (int q = 0; q < VS; ++q)
-        if (src[q] >= 0) s_ver[g][q * L + j] = val[q];
+      for (int q = 0; q < VS; ++q) {
+        const int pos = q * L + j;
+        if (pos < nb_v) s_ver[g][pos] = val[q];
+      }
       wsync();
     }
     const int nv_old = nb_v;
@@ -928,23 +976,29 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
       rpd_d4 X, Q;
       rpd_vertex_homog(pa, pb, pc, X, Q);
       const double A[3] = {X.x / X.w, X.y / X.w, X.z / X.w};
-      double Pp[6][3];
-      foot_plane(pa, Cc, Pp[0]);
-      foot_line(pa, pb, A, Cc, Pp[1]);
-      foot_plane(pb, Cc, Pp[2]);
-      foot_line(pb, pc, A, Cc, Pp[3]);
-      foot_plane(pc, Cc, Pp[4]);
-      foot_line(pc, pa, A, Cc, Pp[5]);
-#pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        const double* P0 = Pp[q];
-        const double* P1 = Pp[(q + 1) % 6];
-        const double tvl = tet_vol(P0, P1, Cc, A);
-        vol += tvl;
-        mx += tvl * (((P0[0] + P1[0]) + Cc[0]) + A[0]);
-        my += tvl * (((P0[1] + P1[1]) + Cc[1]) + A[1]);
-        mz += tvl * (((P0[2] + P1[2]) + Cc[2]) + A[2]);
-      }
+      // ring of feet around A: F_a, E_ab, F_b, E_bc, F_c, E_ca
+      double F0[3], E0[3], F1[3], E1[3], F2[3], E2[3];
+      foot_plane(pa, Cc, F0);
+      foot_line(pa, pb, A, Cc, E0);
+      foot_plane(pb, Cc, F1);
+      foot_line(pb, pc, A, Cc, E1);
+      foot_plane(pc, Cc, F2);
+      foot_line(pc, pa, A, Cc, E2);
+#define RPD_ACC(P0, P1)                                          \
+  {                                                              \
+    const double tvl = tet_vol(P0, P1, Cc, A);                   \
+    vol += tvl;                                                  \
+    mx += tvl * (((P0[0] + P1[0]) + Cc[0]) + A[0]);              \
+    my += tvl * (((P0[1] + P1[1]) + Cc[1]) + A[1]);              \
+    mz += tvl * (((P0[2] + P1[2]) + Cc[2]) + A[2]);              \
+  }
+      RPD_ACC(F0, E0)
+      RPD_ACC(E0, F1)
+      RPD_ACC(F1, E1)
+      RPD_ACC(E1, F2)
+      RPD_ACC(F2, E2)
+      RPD_ACC(E2, F0)
+#undef RPD_ACC
     }
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) {
@@ -969,17 +1023,17 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
       double area = 0;
       for (int v = 0; v < nb_v; ++v) {
         const uint32_t w = s_ver[g][v];
-        const uint32_t t3[3] = {w & 255, (w >> 8) & 255, (w >> 16) & 255};
-        int sl = -1;
-        if (t3[0] == (uint32_t)p) sl = 0;
-        else if (t3[1] == (uint32_t)p) sl = 1;
-        else if (t3[2] == (uint32_t)p) sl = 2;
-        if (sl < 0) continue;
-        const rpd_d4 pn = ld4(&s_pl[g][t3[(sl + 1) % 3]]);
-        const rpd_d4 pq = ld4(&s_pl[g][t3[(sl + 2) % 3]]);
+        const uint32_t t0 = w & 255, t1 = (w >> 8) & 255, t2 = (w >> 16) & 255;
+        const uint32_t up = (uint32_t)p;
+        if (t0 != up && t1 != up && t2 != up) continue;
+        // next / previous plane of p in the dual triangle (t0, t1, t2)
+        const uint32_t nx = t0 == up ? t1 : (t1 == up ? t2 : t0);
+        const uint32_t pv = t0 == up ? t2 : (t1 == up ? t0 : t1);
+        const rpd_d4 pn = ld4(&s_pl[g][nx]);
+        const rpd_d4 pq = ld4(&s_pl[g][pv]);
         rpd_d4 X, Q;
-        rpd_vertex_homog(ld4(&s_pl[g][t3[0]]), ld4(&s_pl[g][t3[1]]),
-                         ld4(&s_pl[g][t3[2]]), X, Q);
+        rpd_vertex_homog(ld4(&s_pl[g][t0]), ld4(&s_pl[g][t1]),
+                         ld4(&s_pl[g][t2]), X, Q);
         const double A[3] = {X.x / X.w, X.y / X.w, X.z / X.w};
         double F[3], En[3], Eq[3];
         foot_plane(pp, Cc, F);
@@ -1063,38 +1117,47 @@ __global__ __launch_bounds__(256) void k_sizes(int64_t n_cells,
               (unsigned long long)h[threadIdx.x]);
 }
 
+// One wavefront per run of kCompactRun consecutive cells; the lanes copy
+// each cell's payload words and areas cooperatively (coalesced reads of the
+// slot, coalesced writes of the contiguous output).
+constexpr int kCompactRun = 16;
 __global__ __launch_bounds__(256) void k_compact(
     int64_t n_cells, const Cand* cand, const int32_t* loc, TierViews tv,
     const Sizes3* off, rpd_cell* out_cells, uint32_t* out_words,
     double* out_geom, double* out_areas, int32_t geometry) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n_cells) return;
-  const int32_t l = loc[c];
-  const int tier = l >> 28;
-  const int64_t idx = tier ? (l & ((1 << 28) - 1)) : c;
-  const SlotView& sv = tv.t[tier];
-  const uint32_t hd = sv.hdr[idx];
-  if ((hd >> 24) != RPD_SUCCESS) return;
-  const Sizes3 o = off[c];
-  const int nv = hd & 255, np = (hd >> 8) & 255, ne = (hd >> 16) & 255;
-  rpd_cell rc;
-  rc.tet_id = cand[c].tet;
-  rc.seed = cand[c].seed;
-  rc.nb_v = (uint8_t)nv;
-  rc.nb_p = (uint8_t)np;
-  rc.nb_e = (uint8_t)ne;
-  rc.status = RPD_SUCCESS;
-  rc.reserved = 0;
-  rc.word_off = (uint64_t)o.words;
-  rc.area_off = (uint64_t)o.areas;
-  out_cells[o.cells] = rc;
-  const uint32_t* src = sv.words + idx * sv.stride_words;
-  for (int q = 0; q < nv + np + ne; ++q) out_words[o.words + q] = src[q];
-  if (geometry) {
-    const double* gs = sv.geom + 4 * idx;
-    for (int q = 0; q < 4; ++q) out_geom[4 * o.cells + q] = gs[q];
-    const double* as = sv.areas + idx * sv.stride_areas;
-    for (int q = 0; q < np; ++q) out_areas[o.areas + q] = as[q];
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t c0 = wave * kCompactRun;
+  for (int64_t c = c0; c < c0 + kCompactRun && c < n_cells; ++c) {
+    const int32_t l = loc[c];
+    const int tier = l >> 28;
+    const int64_t idx = tier ? (l & ((1 << 28) - 1)) : c;
+    const SlotView& sv = tv.t[tier];
+    const uint32_t hd = sv.hdr[idx];
+    if ((hd >> 24) != RPD_SUCCESS) continue;
+    const Sizes3 o = off[c];
+    const int nv = hd & 255, np = (hd >> 8) & 255, ne = (hd >> 16) & 255;
+    if (lane == 0) {
+      rpd_cell rc;
+      rc.tet_id = cand[c].tet;
+      rc.seed = cand[c].seed;
+      rc.nb_v = (uint8_t)nv;
+      rc.nb_p = (uint8_t)np;
+      rc.nb_e = (uint8_t)ne;
+      rc.status = RPD_SUCCESS;
+      rc.reserved = 0;
+      rc.word_off = (uint64_t)o.words;
+      rc.area_off = (uint64_t)o.areas;
+      out_cells[o.cells] = rc;
+    }
+    const uint32_t* src = sv.words + idx * sv.stride_words;
+    for (int q = lane; q < nv + np + ne; q += 64)
+      out_words[o.words + q] = src[q];
+    if (geometry) {
+      if (lane < 4) out_geom[4 * o.cells + lane] = sv.geom[4 * idx + lane];
+      const double* as = sv.areas + idx * sv.stride_areas;
+      for (int q = lane; q < np; q += 64) out_areas[o.areas + q] = as[q];
+    }
   }
 }
 

@@ -52,9 +52,11 @@ class Stats(ctypes.Structure):
                 ("n_active_planes", ctypes.c_int64),
                 ("n_all_planes", ctypes.c_int64),
                 ("n_bbox_fallback", ctypes.c_int64),
+                ("n_bin_registrations", ctypes.c_int64),
                 ("tet_k", ctypes.c_int32), ("site_k", ctypes.c_int32),
                 ("ms_total", ctypes.c_double),
-                ("ms_phase", ctypes.c_double * 8)]
+                ("ms_phase", ctypes.c_double * 8),
+                ("ms_kernel", ctypes.c_double * 8)]
 
     def as_dict(self):
         d = {}
@@ -88,12 +90,14 @@ def _declare_capi(L):
     L.rpd_download_cells.restype = ctypes.c_int
     L.rpd_download_relation.argtypes = [vp, vp, vp]
     L.rpd_download_relation.restype = ctypes.c_int
+    L.rpd_calib_copy.argtypes = [vp, ctypes.c_int64]
+    L.rpd_calib_copy.restype = ctypes.c_int
 
 
 CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
                 "rpd_ctx_stream", "rpd_upload", "rpd_update_sites", "rpd_run",
                 "rpd_sync", "rpd_get_stats", "rpd_download_cells",
-                "rpd_download_relation")
+                "rpd_download_relation", "rpd_calib_copy")
 
 
 def _p(a):
@@ -213,10 +217,14 @@ class Context:
             "rpd_download_cells")
         return RPDResult(cells, words, geom, areas)
 
+    def calib_copy(self, nbytes):
+        self._check(self._L.rpd_calib_copy(self._h, int(nbytes)),
+                    "rpd_calib_copy")
+
     def relation(self):
         st = self.stats()
         off = np.empty(self.n_tet + 1, np.int64)
         ids = np.empty(max(st["n_candidates"], 1), np.int32)
         self._check(self._L.rpd_download_relation(self._h, _p(off), _p(ids)),
-                    "rpd_download_relation")
+                    "rpd_download_relation", "rpd_calib_copy")
         return off, ids[:st["n_candidates"]]

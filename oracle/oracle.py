@@ -49,6 +49,9 @@ def lib():
         L.orc_run.argtypes = [ctypes.POINTER(_Input), vp, ctypes.c_int32,
                               ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.orc_run.restype = vp
+        L.orc_clip.argtypes = [ctypes.POINTER(_Input), vp, ctypes.c_int32,
+                               vp, vp, ctypes.c_int32, ctypes.c_int32]
+        L.orc_clip.restype = vp
         L.orc_free.argtypes = [vp]
         L.orc_n_cells.argtypes = [vp]
         L.orc_n_cells.restype = ctypes.c_int64
@@ -81,30 +84,21 @@ class OracleResult:
         self.tets = tets
 
 
-def run(inst, tets=None, mode=0, brute_force=False, n_threads=8):
-    """Relation + clip for `tets` (default: all) of a prep.Instance."""
-    L = lib()
-    keep = []
-
+def _input(inst, keep):
     def arr(a, dt):
         a = np.ascontiguousarray(a, dt)
         keep.append(a)
         return _p(a)
 
-    I = _Input(inst.n_vert, inst.n_tet, inst.n_site,
-               arr(inst.verts, np.float64), arr(inst.tets, np.int32),
-               arr(inst.v_adj, np.int32), arr(inst.e_adj, np.int32),
-               arr(inst.f_adj, np.int32), arr(inst.f_id, np.int32),
-               arr(inst.sites, np.float64), arr(inst.flags, np.uint32),
-               arr(inst.knn_off, np.int64), arr(inst.knn_ids, np.int32))
-    if tets is None:
-        tl, n_list = None, 0
-        tet_arr = np.arange(inst.n_tet, dtype=np.int32)
-    else:
-        tet_arr = np.ascontiguousarray(tets, np.int32)
-        tl, n_list = _p(tet_arr), tet_arr.size
-    h = L.orc_run(ctypes.byref(I), tl, n_list, mode, int(brute_force),
-                  n_threads)
+    return _Input(inst.n_vert, inst.n_tet, inst.n_site,
+                  arr(inst.verts, np.float64), arr(inst.tets, np.int32),
+                  arr(inst.v_adj, np.int32), arr(inst.e_adj, np.int32),
+                  arr(inst.f_adj, np.int32), arr(inst.f_id, np.int32),
+                  arr(inst.sites, np.float64), arr(inst.flags, np.uint32),
+                  arr(inst.knn_off, np.int64), arr(inst.knn_ids, np.int32))
+
+
+def _collect(L, h, tet_arr):
     try:
         nc, nr = L.orc_n_cells(h), L.orc_n_rel(h)
         off = np.empty(tet_arr.size + 1, np.int64)
@@ -120,6 +114,35 @@ def run(inst, tets=None, mode=0, brute_force=False, n_threads=8):
     finally:
         L.orc_free(h)
     return OracleResult(off, ids[:nr], cells, drops, cnt, tet_arr)
+
+
+def clip(inst, tets, cand_off, cand_ids, mode=0, n_threads=8):
+    """Clip only, for the given per-tet candidate lists (CSR over tets)."""
+    L = lib()
+    keep = []
+    I = _input(inst, keep)
+    tet_arr = np.ascontiguousarray(tets, np.int32)
+    off = np.ascontiguousarray(cand_off, np.int64)
+    ids = np.ascontiguousarray(cand_ids, np.int32)
+    h = L.orc_clip(ctypes.byref(I), _p(tet_arr), tet_arr.size, _p(off),
+                   _p(ids), mode, n_threads)
+    return _collect(L, h, tet_arr)
+
+
+def run(inst, tets=None, mode=0, brute_force=False, n_threads=8):
+    """Relation + clip for `tets` (default: all) of a prep.Instance."""
+    L = lib()
+    keep = []
+    I = _input(inst, keep)
+    if tets is None:
+        tl, n_list = None, 0
+        tet_arr = np.arange(inst.n_tet, dtype=np.int32)
+    else:
+        tet_arr = np.ascontiguousarray(tets, np.int32)
+        tl, n_list = _p(tet_arr), tet_arr.size
+    h = L.orc_run(ctypes.byref(I), tl, n_list, mode, int(brute_force),
+                  n_threads)
+    return _collect(L, h, tet_arr)
 
 
 def exact_det4_sign(m):

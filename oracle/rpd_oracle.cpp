@@ -717,7 +717,9 @@ bool related(const Input& in, int t, int i) {
 
 template <int kMode>
 void run_cells(const Input& in, const std::vector<int32_t>& tet_list,
-               bool brute_force, int n_threads, Result& res) {
+               bool brute_force, int n_threads, Result& res,
+               const int64_t* given_off = nullptr,
+               const int32_t* given_ids = nullptr) {
   const int64_t nt = (int64_t)tet_list.size();
   std::vector<std::vector<int32_t>> rel(nt);
   std::vector<std::vector<CellOut>> out(nt);
@@ -736,9 +738,16 @@ void run_cells(const Input& in, const std::vector<int32_t>& tet_list,
       int64_t q = next.fetch_add(1);
       if (q >= nt) break;
       int t = tet_list[q];
-      for (int i = 0; i < in.n_site; ++i) {
-        bool r = brute_force ? in.flags[i] == 1u : related(in, t, i);
-        if (!r) continue;
+      std::vector<int32_t> cands;
+      if (given_off) {
+        cands.assign(given_ids + given_off[q], given_ids + given_off[q + 1]);
+      } else {
+        for (int i = 0; i < in.n_site; ++i) {
+          bool r = brute_force ? in.flags[i] == 1u : related(in, t, i);
+          if (r) cands.push_back(i);
+        }
+      }
+      for (int i : cands) {
         rel[q].push_back(i);
         c->init(&in, t, i, &cnts[tid]);
         if (brute_force) {
@@ -854,6 +863,25 @@ orc_result* orc_run(const orc_input* in, const int32_t* tet_list,
     run_cells<1>(I, tl, brute_force != 0, n_threads, res->r);
   else
     run_cells<0>(I, tl, brute_force != 0, n_threads, res->r);
+  return res;
+}
+
+// Clip only: candidate lists per requested tet are given (CSR over
+// tet_list), the relation is not evaluated.
+orc_result* orc_clip(const orc_input* in, const int32_t* tet_list,
+                     int32_t n_list, const int64_t* cand_off,
+                     const int32_t* cand_ids, int32_t mode,
+                     int32_t n_threads) {
+  Input I{in->n_vert, in->n_tet,  in->n_site, in->verts,
+          in->tets,   in->v_adj,  in->e_adj,  in->f_adj,
+          in->f_id,   in->sites,  in->flags,  in->knn_off,
+          in->knn_ids};
+  std::vector<int32_t> tl(tet_list, tet_list + n_list);
+  auto* res = new orc_result();
+  if (mode == 1)
+    run_cells<1>(I, tl, false, n_threads, res->r, cand_off, cand_ids);
+  else
+    run_cells<0>(I, tl, false, n_threads, res->r, cand_off, cand_ids);
   return res;
 }
 
