@@ -553,8 +553,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   HIP_TRY(ctx->out_geom.need((size_t)std::max<int64_t>(tot.cells, 1) * 32));
   HIP_TRY(ctx->out_areas.need((size_t)std::max<int64_t>(tot.areas, 1) * 8));
   if (n_cand > 0)
-    hipLaunchKernelGGL(k_compact,
-                       dim3(ceil_div(ceil_div(n_cand, kCompactRun), 4)),
+    hipLaunchKernelGGL(k_compact, dim3(ceil_div(n_cand * kCompactLanes, 256)),
                        dim3(256), 0, s,
                        n_cand, ctx->cand.as<Cand>(), ctx->loc.as<int32_t>(), tv,
                        ctx->offs.as<Sizes3>(), ctx->out_cells.as<rpd_cell>(),

@@ -552,49 +552,60 @@ __device__ __forceinline__ uint64_t group_bits(uint64_t b, int g) {
   return (b >> (g * L)) & ((1ull << L) - 1);
 }
 
-// geometry of one vertex-plane corner: see compute() in k_clip
-__device__ __forceinline__ void foot_plane(const rpd_d4& p, const double C[3],
-                                           double F[3]) {
-  double nn = p.x * p.x + p.y * p.y + p.z * p.z;
-  double s = (p.x * C[0] + p.y * C[1] + p.z * C[2] + p.w) / nn;
+// ---- FP64 geometry helpers (tolerance-checked, not bit-compared)
+// foot of C on plane p, given 1/|n_p|^2
+__device__ __forceinline__ void foot_plane(const rpd_d4& p, double inv_nn,
+                                           const double C[3], double F[3]) {
+  const double s = (p.x * C[0] + p.y * C[1] + p.z * C[2] + p.w) * inv_nn;
   F[0] = C[0] - s * p.x;
   F[1] = C[1] - s * p.y;
   F[2] = C[2] - s * p.z;
 }
+// foot of C on the line through A with direction n_p x n_q
 __device__ __forceinline__ void foot_line(const rpd_d4& p, const rpd_d4& q,
                                           const double A[3], const double C[3],
                                           double E[3]) {
-  double d0 = p.y * q.z - p.z * q.y;
-  double d1 = p.z * q.x - p.x * q.z;
-  double d2 = p.x * q.y - p.y * q.x;
-  double dd = d0 * d0 + d1 * d1 + d2 * d2;
-  double s = ((C[0] - A[0]) * d0 + (C[1] - A[1]) * d1 + (C[2] - A[2]) * d2) /
-             dd;
+  const double d0 = p.y * q.z - p.z * q.y;
+  const double d1 = p.z * q.x - p.x * q.z;
+  const double d2 = p.x * q.y - p.y * q.x;
+  const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+  const double s =
+      ((C[0] - A[0]) * d0 + (C[1] - A[1]) * d1 + (C[2] - A[2]) * d2) / dd;
   E[0] = A[0] + s * d0;
   E[1] = A[1] + s * d1;
   E[2] = A[2] + s * d2;
 }
-// signed volume of tet (P, Q, C, A) as the reference's
-// get_tet_volume(P-A, Q-A, C-A) = -det3(P-A, Q-A, C-A)/6
-__device__ __forceinline__ double tet_vol(const double P[3], const double Q[3],
-                                          const double C[3],
-                                          const double A[3]) {
-  double a0 = P[0] - A[0], a1 = P[1] - A[1], a2 = P[2] - A[2];
-  double b0 = Q[0] - A[0], b1 = Q[1] - A[1], b2 = Q[2] - A[2];
-  double c0 = C[0] - A[0], c1 = C[1] - A[1], c2 = C[2] - A[2];
-  double d = a0 * (b1 * c2 - b2 * c1) - b0 * (a1 * c2 - a2 * c1) +
-             c0 * (a1 * b2 - a2 * b1);
-  return -d / 6.0;
+// six times the signed volume of tet (P, Q, C, A), in the sign convention
+// of the reference's get_tet_volume(P-A, Q-A, C-A) = -det3(...)/6
+__device__ __forceinline__ double tet_vol6(const double P[3],
+                                           const double Q[3],
+                                           const double C[3],
+                                           const double A[3]) {
+  const double a0 = P[0] - A[0], a1 = P[1] - A[1], a2 = P[2] - A[2];
+  const double b0 = Q[0] - A[0], b1 = Q[1] - A[1], b2 = Q[2] - A[2];
+  const double c0 = C[0] - A[0], c1 = C[1] - A[1], c2 = C[2] - A[2];
+  return -(a0 * (b1 * c2 - b2 * c1) - b0 * (a1 * c2 - a2 * c1) +
+           c0 * (a1 * b2 - a2 * b1));
 }
-// signed area of triangle (P, Q, A) along unit normal n
-__device__ __forceinline__ double tri_area(const double P[3], const double Q[3],
-                                           const double A[3],
-                                           const double n[3]) {
-  double u0 = Q[0] - P[0], u1 = Q[1] - P[1], u2 = Q[2] - P[2];
-  double v0 = A[0] - P[0], v1 = A[1] - P[1], v2 = A[2] - P[2];
-  double c0 = u1 * v2 - u2 * v1, c1 = u2 * v0 - u0 * v2,
-         c2 = u0 * v1 - u1 * v0;
-  return 0.5 * (c0 * n[0] + c1 * n[1] + c2 * n[2]);
+// twice the signed area of triangle (P, Q, A) along the (unnormalised)
+// normal n
+__device__ __forceinline__ double tri_area2(const double P[3],
+                                            const double Q[3],
+                                            const double A[3],
+                                            const rpd_d4& n) {
+  const double u0 = Q[0] - P[0], u1 = Q[1] - P[1], u2 = Q[2] - P[2];
+  const double v0 = A[0] - P[0], v1 = A[1] - P[1], v2 = A[2] - P[2];
+  return (u1 * v2 - u2 * v1) * n.x + (u2 * v0 - u0 * v2) * n.y +
+         (u0 * v1 - u1 * v0) * n.z;
+}
+__device__ __forceinline__ void vertex_pos(const rpd_d4& a, const rpd_d4& b,
+                                           const rpd_d4& c, double A[3]) {
+  rpd_d4 X, Q;
+  rpd_vertex_homog(a, b, c, X, Q);
+  const double r = 1.0 / X.w;
+  A[0] = X.x * r;
+  A[1] = X.y * r;
+  A[2] = X.z * r;
 }
 
 template <int G, class C, bool kExact>
@@ -953,41 +964,36 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
   // ---- write the slot
   const SlotView& o = a.out;
   if (status == RPD_SUCCESS && a.geometry) {
-    // FP64 geometry by a vertex-local decomposition (6 tets per vertex
-    // between the reference point C, the vertex A, the feet of C on its 3
-    // planes and on its 3 edge lines); C = the cell's vertex 0, so rounding
-    // errors scale with the cell, not with the tet
+    // FP64 geometry by a vertex-local decomposition: 6 tets per vertex A
+    // between the reference point C, A, the feet of C on A's 3 planes and
+    // on its 3 edge lines (the decomposition of the reference's
+    // get_tet_decomposition_of_vertex, convex_cell.cu:986-1006); C is the
+    // cell's vertex 0 so rounding errors scale with the cell
     double Cc[3];
     {
       const uint32_t w0 = s_ver[g][0];
-      rpd_d4 X0, Q0;
-      rpd_vertex_homog(ld4(&s_pl[g][w0 & 255]), ld4(&s_pl[g][(w0 >> 8) & 255]),
-                       ld4(&s_pl[g][(w0 >> 16) & 255]), X0, Q0);
-      Cc[0] = X0.x / X0.w;
-      Cc[1] = X0.y / X0.w;
-      Cc[2] = X0.z / X0.w;
+      vertex_pos(ld4(&s_pl[g][w0 & 255]), ld4(&s_pl[g][(w0 >> 8) & 255]),
+                 ld4(&s_pl[g][(w0 >> 16) & 255]), Cc);
     }
-    double vol = 0, mx = 0, my = 0, mz = 0;
+    double vol6 = 0, mx = 0, my = 0, mz = 0;
     for (int v = j; v < nb_v; v += L) {
       const uint32_t w = s_ver[g][v];
       const rpd_d4 pa = ld4(&s_pl[g][w & 255]);
       const rpd_d4 pb = ld4(&s_pl[g][(w >> 8) & 255]);
       const rpd_d4 pc = ld4(&s_pl[g][(w >> 16) & 255]);
-      rpd_d4 X, Q;
-      rpd_vertex_homog(pa, pb, pc, X, Q);
-      const double A[3] = {X.x / X.w, X.y / X.w, X.z / X.w};
-      // ring of feet around A: F_a, E_ab, F_b, E_bc, F_c, E_ca
+      double A[3];
+      vertex_pos(pa, pb, pc, A);
       double F0[3], E0[3], F1[3], E1[3], F2[3], E2[3];
-      foot_plane(pa, Cc, F0);
+      foot_plane(pa, 1.0 / (pa.x * pa.x + pa.y * pa.y + pa.z * pa.z), Cc, F0);
       foot_line(pa, pb, A, Cc, E0);
-      foot_plane(pb, Cc, F1);
+      foot_plane(pb, 1.0 / (pb.x * pb.x + pb.y * pb.y + pb.z * pb.z), Cc, F1);
       foot_line(pb, pc, A, Cc, E1);
-      foot_plane(pc, Cc, F2);
+      foot_plane(pc, 1.0 / (pc.x * pc.x + pc.y * pc.y + pc.z * pc.z), Cc, F2);
       foot_line(pc, pa, A, Cc, E2);
 #define RPD_ACC(P0, P1)                                          \
   {                                                              \
-    const double tvl = tet_vol(P0, P1, Cc, A);                   \
-    vol += tvl;                                                  \
+    const double tvl = tet_vol6(P0, P1, Cc, A);                  \
+    vol6 += tvl;                                                 \
     mx += tvl * (((P0[0] + P1[0]) + Cc[0]) + A[0]);              \
     my += tvl * (((P0[1] + P1[1]) + Cc[1]) + A[1]);              \
     mz += tvl * (((P0[2] + P1[2]) + Cc[2]) + A[2]);              \
@@ -1002,25 +1008,27 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
     }
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) {
-      vol += __shfl_xor(vol, off);
+      vol6 += __shfl_xor(vol6, off);
       mx += __shfl_xor(mx, off);
       my += __shfl_xor(my, off);
       mz += __shfl_xor(mz, off);
     }
     if (j == 0) {
       double* gm = o.geom + 4 * item;
-      const double sg = vol < 0 ? -1.0 : 1.0;
-      gm[0] = sg * vol;
-      gm[1] = vol != 0 ? mx / (4.0 * vol) : 0.0;
-      gm[2] = vol != 0 ? my / (4.0 * vol) : 0.0;
-      gm[3] = vol != 0 ? mz / (4.0 * vol) : 0.0;
+      const double vol = vol6 / 6.0;
+      const double inv = vol6 != 0 ? 1.0 / (4.0 * vol6) : 0.0;
+      gm[0] = fabs(vol);
+      gm[1] = mx * inv;
+      gm[2] = my * inv;
+      gm[3] = mz * inv;
     }
     // facet areas: lane per plane, deterministic serial sum over vertices
     for (int p = j; p < nb_p; p += L) {
       const rpd_d4 pp = ld4(&s_pl[g][p]);
-      const double nn = sqrt(pp.x * pp.x + pp.y * pp.y + pp.z * pp.z);
-      const double n[3] = {pp.x / nn, pp.y / nn, pp.z / nn};
-      double area = 0;
+      const double nn = pp.x * pp.x + pp.y * pp.y + pp.z * pp.z;
+      double F[3];
+      foot_plane(pp, 1.0 / nn, Cc, F);
+      double area2 = 0;
       for (int v = 0; v < nb_v; ++v) {
         const uint32_t w = s_ver[g][v];
         const uint32_t t0 = w & 255, t1 = (w >> 8) & 255, t2 = (w >> 16) & 255;
@@ -1031,17 +1039,14 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
         const uint32_t pv = t0 == up ? t2 : (t1 == up ? t0 : t1);
         const rpd_d4 pn = ld4(&s_pl[g][nx]);
         const rpd_d4 pq = ld4(&s_pl[g][pv]);
-        rpd_d4 X, Q;
-        rpd_vertex_homog(ld4(&s_pl[g][t0]), ld4(&s_pl[g][t1]),
-                         ld4(&s_pl[g][t2]), X, Q);
-        const double A[3] = {X.x / X.w, X.y / X.w, X.z / X.w};
-        double F[3], En[3], Eq[3];
-        foot_plane(pp, Cc, F);
+        double A[3], En[3], Eq[3];
+        vertex_pos(ld4(&s_pl[g][t0]), ld4(&s_pl[g][t1]), ld4(&s_pl[g][t2]), A);
         foot_line(pp, pn, A, Cc, En);
         foot_line(pq, pp, A, Cc, Eq);
-        area += tri_area(F, En, A, n) + tri_area(Eq, F, A, n);
+        area2 += tri_area2(F, En, A, pp) + tri_area2(Eq, F, A, pp);
       }
-      o.areas[(int64_t)item * o.stride_areas + p] = fabs(area);
+      o.areas[(int64_t)item * o.stride_areas + p] =
+          fabs(area2) * 0.5 / sqrt(nn);
     }
   }
   if (status == RPD_SUCCESS) {
@@ -1117,47 +1122,45 @@ __global__ __launch_bounds__(256) void k_sizes(int64_t n_cells,
               (unsigned long long)h[threadIdx.x]);
 }
 
-// One wavefront per run of kCompactRun consecutive cells; the lanes copy
-// each cell's payload words and areas cooperatively (coalesced reads of the
-// slot, coalesced writes of the contiguous output).
-constexpr int kCompactRun = 16;
+// Eight lanes per cell, eight cells per wavefront: each group copies its
+// cell's payload words and areas from the slot into the contiguous output.
+constexpr int kCompactLanes = 8;
 __global__ __launch_bounds__(256) void k_compact(
     int64_t n_cells, const Cand* cand, const int32_t* loc, TierViews tv,
     const Sizes3* off, rpd_cell* out_cells, uint32_t* out_words,
     double* out_geom, double* out_areas, int32_t geometry) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t c0 = wave * kCompactRun;
-  for (int64_t c = c0; c < c0 + kCompactRun && c < n_cells; ++c) {
-    const int32_t l = loc[c];
-    const int tier = l >> 28;
-    const int64_t idx = tier ? (l & ((1 << 28) - 1)) : c;
-    const SlotView& sv = tv.t[tier];
-    const uint32_t hd = sv.hdr[idx];
-    if ((hd >> 24) != RPD_SUCCESS) continue;
-    const Sizes3 o = off[c];
-    const int nv = hd & 255, np = (hd >> 8) & 255, ne = (hd >> 16) & 255;
-    if (lane == 0) {
-      rpd_cell rc;
-      rc.tet_id = cand[c].tet;
-      rc.seed = cand[c].seed;
-      rc.nb_v = (uint8_t)nv;
-      rc.nb_p = (uint8_t)np;
-      rc.nb_e = (uint8_t)ne;
-      rc.status = RPD_SUCCESS;
-      rc.reserved = 0;
-      rc.word_off = (uint64_t)o.words;
-      rc.area_off = (uint64_t)o.areas;
-      out_cells[o.cells] = rc;
-    }
-    const uint32_t* src = sv.words + idx * sv.stride_words;
-    for (int q = lane; q < nv + np + ne; q += 64)
-      out_words[o.words + q] = src[q];
-    if (geometry) {
-      if (lane < 4) out_geom[4 * o.cells + lane] = sv.geom[4 * idx + lane];
-      const double* as = sv.areas + idx * sv.stride_areas;
-      for (int q = lane; q < np; q += 64) out_areas[o.areas + q] = as[q];
-    }
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = tid / kCompactLanes;
+  const int j = (int)(tid % kCompactLanes);
+  if (c >= n_cells) return;
+  const int32_t l = loc[c];
+  const int tier = l >> 28;
+  const int64_t idx = tier ? (l & ((1 << 28) - 1)) : c;
+  const SlotView& sv = tv.t[tier];
+  const uint32_t hd = sv.hdr[idx];
+  if ((hd >> 24) != RPD_SUCCESS) return;
+  const Sizes3 o = off[c];
+  const int nv = hd & 255, np = (hd >> 8) & 255, ne = (hd >> 16) & 255;
+  if (j == 0) {
+    rpd_cell rc;
+    rc.tet_id = cand[c].tet;
+    rc.seed = cand[c].seed;
+    rc.nb_v = (uint8_t)nv;
+    rc.nb_p = (uint8_t)np;
+    rc.nb_e = (uint8_t)ne;
+    rc.status = RPD_SUCCESS;
+    rc.reserved = 0;
+    rc.word_off = (uint64_t)o.words;
+    rc.area_off = (uint64_t)o.areas;
+    out_cells[o.cells] = rc;
+  }
+  const uint32_t* src = sv.words + idx * sv.stride_words;
+  for (int q = j; q < nv + np + ne; q += kCompactLanes)
+    out_words[o.words + q] = src[q];
+  if (geometry) {
+    if (j < 4) out_geom[4 * o.cells + j] = sv.geom[4 * idx + j];
+    const double* as = sv.areas + idx * sv.stride_areas;
+    for (int q = j; q < np; q += kCompactLanes) out_areas[o.areas + q] = as[q];
   }
 }
 

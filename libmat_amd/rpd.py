@@ -226,5 +226,5 @@ class Context:
         off = np.empty(self.n_tet + 1, np.int64)
         ids = np.empty(max(st["n_candidates"], 1), np.int32)
         self._check(self._L.rpd_download_relation(self._h, _p(off), _p(ids)),
-                    "rpd_download_relation", "rpd_calib_copy")
+                    "rpd_download_relation")
         return off, ids[:st["n_candidates"]]
