@@ -1,0 +1,296 @@
+// C++ drop-in surface of LibMAT's restricted power diagram, on top of the
+// C ABI of rpd_capi.h.
+//
+// Re-exported reference interfaces:
+//   ConvexCellHost                 src/rpd3d_base/voronoi_defs.h:19-100
+//   is_convex_cell_valid           src/rpd3d_base/voronoi_defs.cxx:293-298
+//   compute_clipped_voro_diagram_GPU  src/rpd3d/voronoi.h:52-61
+//   RPD3D_GPU                      src/rpd3d_api/rpd_api.h:8-69
+//   update_power_cells (adjacency) src/rpd3d_base/rpd_update.cxx:568-640
+// The cell record keeps the reference's field names, types, order,
+// capacities and alignment (sizeof 3776, align 32 with GCC on x86-64), so
+// consumers of std::vector<ConvexCellHost> (topology checks, feature
+// fixes, exporters) compile and read the same bytes.
+#pragma once
+#include <stdint.h>
+
+#include <array>
+#include <map>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifndef LIBMAT_RPD_NO_REFERENCE_TYPES
+typedef unsigned char uchar;
+typedef unsigned int uint;
+typedef double Scalar;
+typedef std::array<int, 2> aint2;
+typedef std::array<int, 3> aint3;
+#endif
+
+#define _MAX_P_ 64
+#define _MAX_T_ 96
+#define _MAX_E_ 152
+
+constexpr uchar END_OF_LIST = 255;
+constexpr int UNK_INT = -1;
+constexpr float UNK_FLOAT = -1.f;
+constexpr float F_TET_ADJ_DEFAULT = 2.f;
+constexpr float F_CELL_ADJ_DEFAULT = 1.f;
+
+// voronoi_common.h:8-25
+enum SiteFlag { no_flag = 0, is_selected = 1 };
+enum Status {
+  early_return = -1,
+  triangle_overflow = 0,
+  vertex_overflow = 1,
+  inconsistent_boundary = 2,
+  security_radius_not_reached = 3,
+  success = 4,
+  needs_exact_predicates = 5,
+  no_intersection = 6,
+  edge_overflow = 7,
+  needs_perturb = 8
+};
+
+// common_cxx.h:22-43 (host POD vector types and their alignments)
+struct __attribute__((aligned(8))) cint2 {
+  int x, y;
+};
+struct __attribute__((aligned(2))) cuchar2 {
+  unsigned char x, y;
+};
+struct __attribute__((aligned(4))) cuchar3 {
+  unsigned char x, y, z;
+};
+struct __attribute__((aligned(4))) cuchar4 {
+  unsigned char x, y, z, w;
+};
+struct __attribute__((aligned(16))) cfloat3 {
+  float x, y, z;
+};
+struct __attribute__((aligned(16))) cfloat4 {
+  float x, y, z, w;
+};
+struct __attribute__((aligned(32))) cfloat5 {
+  float x, y, z, w, h;
+};
+
+struct ConvexCellHost {
+  ConvexCellHost() {}
+  // float vertex of the dual triangle t, as the reference computes it
+  // (voronoi_defs.cxx:33-74)
+  cfloat4 compute_vertex_coordinates(cuchar3 t, bool persp_divide = true) const;
+  void print_info() const;
+
+  inline cuchar4& ver_trans(int t) { return ver_data_trans[t]; }
+  inline cfloat5& clip_trans(int p) { return clip_data_trans[p]; }
+  inline cuchar3& edge(int e) { return edge_data[e]; }
+  inline uchar& ith_plane(uchar t, int i) {
+    return reinterpret_cast<uchar*>(&(ver_trans(t)))[i];
+  }
+  inline cuchar4 ver_trans_const(int t) const { return ver_data_trans[t]; }
+  inline cfloat4 clip_trans_const(int f) const {
+    const cfloat5& c = clip_data_trans[f];
+    return cfloat4{c.x, c.y, c.z, c.w};
+  }
+  inline cint2 clip_id2_const(int p) const { return clip_id2_data_trans[p]; }
+  inline cuchar2 edge_id2_const(int e) const {
+    return cuchar2{edge_data[e].x, edge_data[e].y};
+  }
+  inline uchar ith_plane_const(uchar t, int i) const {
+    cuchar4 v = ver_trans_const(t);
+    return reinterpret_cast<uchar*>(&(v))[i];
+  }
+
+  // topology (voronoi_defs.cxx:76-267)
+  bool is_active_updated = false;
+  std::vector<int> active_clipping_planes;
+  std::vector<int> active_edges;
+  Scalar cal_cell_euler();
+  Scalar cal_halfplane_facet_euler(int neigh_id);
+  void reload_active();
+
+  bool is_pc_explicit = false;
+  std::vector<cfloat3> pc_points;
+  std::vector<std::vector<int>> pc_local_active_faces;
+  std::vector<int> pc_lf2active_map;
+  void reload_pc_explicit();
+  bool is_vertex_null = false;
+
+  Status status;
+  int thread_id;
+  int voro_id;
+  int tet_id;
+  float weight = -1;
+  bool is_active = false;
+  uchar nb_v, nb_p, nb_e;
+  cuchar4 ver_data_trans[_MAX_T_];
+  cfloat5 clip_data_trans[_MAX_P_];
+  cint2 clip_id2_data_trans[_MAX_P_];
+  cuchar3 edge_data[_MAX_E_];
+  float euler = -1;
+  float cell_vol = -1;
+  int id = UNK_INT;
+};
+
+bool is_convex_cell_valid(const ConvexCellHost& cell);
+
+// The reference's GPU entry point, same signature and semantics
+// (voronoi.cu:455-795): returns the non-empty cells of every flagged site in
+// every tet, ordered by (tet, ascending site id), ids = index.
+//   vertices: AoS xyz of the tet mesh, indices: 4 per tet (positively
+//   oriented), v_adjs / e_adjs (triangular n_vert^2 layout of
+//   get_edge_idx, convex_cell.h:46-60) / f_adjs / f_ids as
+//   load_tet_adj_info (io.cxx:238-335); site: SoA [x.., y.., z..] (copied
+//   back unchanged), site_weights = r^2, site_flags, site_knn
+//   (site_k+1) x n_site column-major with -1 padding.
+// Unused as in the reference: num_itr_global, v2tets, site_is_transposed
+// (always SoA), nb_Lloyd_iter, preferred_tet_k.  site_cell_vol receives the
+// per-site sum of cell volumes (the reference leaves it zero).
+// Errors throw std::runtime_error (the reference exits the process).
+std::vector<ConvexCellHost> compute_clipped_voro_diagram_GPU(
+    const int num_itr_global, const std::vector<float>& vertices,
+    const std::vector<int>& indices, const std::map<int, std::set<int>>& v2tets,
+    const std::vector<int>& v_adjs, const std::vector<int>& e_adjs,
+    const std::vector<int>& f_adjs, const std::vector<int>& f_ids,
+    std::vector<float>& site, const int n_site,
+    const std::vector<float>& site_weights, const std::vector<uint>& site_flags,
+    const std::vector<int>& site_knn, const int site_k,
+    std::vector<float>& site_cell_vol, const bool site_is_transposed,
+    int nb_Lloyd_iter = 1, int preferred_tet_k = 0);
+
+// Same, with per-tet edge counts (6 per tet, local edge order
+// (0,1),(0,2),(0,3),(1,2),(1,3),(2,3)) instead of the n_vert^2 matrix, for
+// meshes whose triangular matrix does not fit in memory.
+std::vector<ConvexCellHost> compute_clipped_voro_diagram_GPU_tet_eadj(
+    const std::vector<float>& vertices, const std::vector<int>& indices,
+    const std::vector<int>& v_adjs, const std::vector<int>& tet_e_adjs,
+    const std::vector<int>& f_adjs, const std::vector<int>& f_ids,
+    std::vector<float>& site, const int n_site,
+    const std::vector<float>& site_weights, const std::vector<uint>& site_flags,
+    const std::vector<int>& site_knn, const int site_k,
+    std::vector<float>& site_cell_vol);
+
+// ------------------------------------------------------------ adjacency
+// The adjacency part of PowerCell (src/matbase/medial_sphere.h:145-260)
+// that update_power_cells / get_all_voro_info derive from the cells of one
+// sphere (rpd_update.cxx:80-340): cell ids and tets, cell <-> cell through
+// shared tet faces, cell -> tet face ids, halfplane facets and edges of the
+// power cell, and its vertices keyed by [cell id, local vertex].
+struct PowerCellAdjacency {
+  int voro_id = -1;
+  std::set<int> cell_ids;
+  std::set<int> tet_ids;
+  std::map<int, std::set<int>> cell_neighbors;
+  std::map<int, std::set<int>> cell_to_tfids;
+  std::map<int, std::set<int>> facet_neigh_to_cells;
+  std::map<aint2, std::set<int>> e_to_cells;
+  std::map<aint2, aint3> vertex_2id;
+};
+
+// Builds the adjacency of every sphere id that owns cells (ConvexCellHost::
+// voro_id), keyed by that id; reloads the cells' active elements.
+// max_surf_fid: largest surface facet id (tet faces with f_id <= it lie on
+// the surface).  Multithreaded over spheres.
+std::map<int, PowerCellAdjacency> update_power_cells_adjacency(
+    std::vector<ConvexCellHost>& cells, int max_surf_fid, int n_threads = 0);
+
+// ------------------------------------------------------------ RPD3D_GPU
+// Minimal stand-ins for the fields of TetMesh (input_types.h:122-147) and
+// MedialSphere (medial_sphere.h:295-...) that the RPD driver reads.  The
+// driver below is a template so the reference's own types can be used.
+struct RpdTetMesh {
+  std::vector<float> tet_vertices;  // AoS xyz
+  std::vector<int> tet_indices;     // 4 per tet
+  std::map<int, std::set<int>> v2tets;
+  std::vector<int> v_adjs, e_adjs, f_adjs, f_ids;
+};
+struct RpdSphere {
+  int id = -1;
+  int site_id = -1;
+  bool is_deleted = false;
+  bool is_rt_valid = false;
+  bool is_rt_prev_valid = false;
+  int rt_change_status = 0;
+  std::set<int> rt_neigh_ids, rt_neigh_ids_prev;
+  std::array<double, 3> center{{0, 0, 0}};
+  double radius = 0;
+  struct {
+    std::set<int> cell_ids, tet_ids;
+  } pcell;
+};
+
+namespace libmat_rpd {
+// power-diagram 1-rings of the live spheres (is_deleted == false) inside
+// the box lo/hi; valid[i] = power cell meets the box (the RT validity of
+// generate_RT_CGAL_and_mark_valid_spheres, triangulation.cxx:62-144)
+void power_neighbors(const std::vector<std::array<double, 4>>& xyzr,
+                     const std::vector<char>& live, const double box[6],
+                     std::vector<char>& valid,
+                     std::vector<std::set<int>>& neighbors);
+}  // namespace libmat_rpd
+
+template <class TetMeshT = RpdTetMesh, class SurfaceMeshT = void,
+          class ParameterT = void, class SphereT = RpdSphere>
+class RPD3D_GPU_T {
+ public:
+  RPD3D_GPU_T() {}
+  // The reference deletes the borrowed pointers in its destructor
+  // (rpd_api.cxx:8-13); here they stay owned by the caller.
+  ~RPD3D_GPU_T() {}
+  void init(const TetMeshT* _tet_mesh, const SurfaceMeshT* _sf_mesh,
+            const ParameterT* _params) {
+    tet_mesh = _tet_mesh;
+    sf_mesh = _sf_mesh;
+    params = _params;
+    num_itr_rpd = 0;
+    n_site = -1;
+    site_k = -1;
+  }
+  void set_spheres(std::vector<SphereT>* _all_medial_spheres) {
+    all_medial_spheres = _all_medial_spheres;
+  }
+
+  // Full RPD of all valid spheres (rpd_api.cxx:32-145).
+  void calculate();
+
+  // site arrays of (sphere + 1ring + 2ring), flags of (sphere + 1ring)
+  // (rpd_api.cxx:343-379)
+  void load_partial_spheres_to_sites(const std::vector<int>& map_site2msphere,
+                                     const std::set<int>& spheres_and_1rings);
+  // site ids -> sphere ids, partial tet ids -> original (rpd_api.cxx:386-430)
+  void update_convex_cells_voro_and_tet_ids(
+      const std::vector<int>& map_site2msphere,
+      const std::vector<int>& map_tet_new2orig,
+      std::vector<ConvexCellHost>& cells_partials, bool is_debug);
+  // new cells of (sphere + 1ring) + previous cells of the others that are
+  // still valid, ids = position (rpd_api.cxx:432-479)
+  std::vector<ConvexCellHost> merge_convex_cells(
+      const std::set<int>& valid_sphere_ids,
+      const std::set<int>& spheres_and_1rings,
+      const std::vector<ConvexCellHost>& convex_cells_prev,
+      const std::vector<ConvexCellHost>& convex_cells_new, bool is_debug);
+
+ public:
+  std::vector<ConvexCellHost> powercells;
+  bool is_debug = false;
+  const TetMeshT* tet_mesh = nullptr;
+  const SurfaceMeshT* sf_mesh = nullptr;
+  const ParameterT* params = nullptr;
+  std::vector<SphereT>* all_medial_spheres = nullptr;
+  bool site_is_transposed = true;
+  std::vector<float> site;
+  std::vector<float> site_weights;
+  std::vector<uint> site_flags;
+  std::vector<int> site_knn;
+  std::vector<float> site_cell_vol;
+  int num_itr_rpd = 0;
+  int n_site = -1;
+  int site_k = -1;
+};
+
+using RPD3D_GPU = RPD3D_GPU_T<>;
+
+#include "libmat_rpd_impl.h"

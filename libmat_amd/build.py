@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "_lib", "librpd_amd.so")
 INC = os.path.join(ROOT, "include")
 ARCH = os.environ.get("RPD_OFFLOAD_ARCH", "gfx950")
 
-HOST_SRCS = ["csrc/host/rpd_prep.cpp"]
+HOST_SRCS = ["csrc/host/rpd_prep.cpp", "csrc/host/rpd_compat.cpp"]
 HIP_SRCS = ["csrc/device/rpd_engine.hip"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
           "-fno-fast-math", f"-I{INC}"]

@@ -1,0 +1,502 @@
+// C++ drop-in surface of the RPD (include/libmat_rpd.h) over the C ABI.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <thread>
+
+#include "libmat_rpd.h"
+#include "rpd_capi.h"
+#include "rpd_prep.h"
+
+static_assert(sizeof(ConvexCellHost) == 3776, "ConvexCellHost layout");
+static_assert(alignof(ConvexCellHost) == 32, "ConvexCellHost alignment");
+
+namespace {
+
+inline float det2f(float a11, float a12, float a21, float a22) {
+  return a11 * a22 - a12 * a21;
+}
+inline float det3f(float a11, float a12, float a13, float a21, float a22,
+                   float a23, float a31, float a32, float a33) {
+  return a11 * det2f(a22, a23, a32, a33) - a21 * det2f(a12, a13, a32, a33) +
+         a31 * det2f(a12, a13, a22, a23);
+}
+
+template <class T>
+void set_intersection2(const std::set<T>& a, const std::set<T>& b,
+                       std::set<T>& out) {
+  out.clear();
+  std::set_intersection(a.begin(), a.end(), b.begin(), b.end(),
+                        std::inserter(out, out.begin()));
+}
+
+std::mutex g_mu;
+rpd_ctx* g_ctx = nullptr;
+
+rpd_ctx* shared_ctx() {
+  if (!g_ctx) {
+    int rc = rpd_ctx_create(0, &g_ctx);
+    if (rc != 0) {
+      g_ctx = nullptr;
+      throw std::runtime_error("RPD: no usable GPU (rpd_ctx_create " +
+                               std::to_string(rc) + ")");
+    }
+  }
+  return g_ctx;
+}
+
+void check(rpd_ctx* ctx, int rc, const char* what) {
+  if (rc != 0)
+    throw std::runtime_error(std::string(what) + ": " + rpd_last_error(ctx));
+}
+
+// triangular-matrix edge index of get_edge_idx (convex_cell.h:46-60):
+// sum_{j<=vmin} (n - j) - (n - vmax)
+inline int64_t tri_edge_idx(int64_t v1, int64_t v2, int64_t n) {
+  int64_t lo = std::min(v1, v2), hi = std::max(v1, v2);
+  return (lo + 1) * n - lo * (lo + 1) / 2 - (n - hi);
+}
+
+std::vector<ConvexCellHost> run_rpd(
+    const std::vector<float>& vertices, const std::vector<int>& indices,
+    const std::vector<int>& v_adjs, const std::vector<int>& tet_e_adjs,
+    const std::vector<int>& f_adjs, const std::vector<int>& f_ids,
+    const std::vector<float>& site, int n_site,
+    const std::vector<float>& site_weights, const std::vector<uint>& site_flags,
+    const std::vector<int>& site_knn, int site_k,
+    std::vector<float>& site_cell_vol) {
+  const int32_t n_vert = (int32_t)(vertices.size() / 3);
+  const int32_t n_tet = (int32_t)(indices.size() / 4);
+  if (n_tet <= 0 || n_vert <= 0)
+    throw std::runtime_error("RPD: empty tet mesh");
+  if ((int)site.size() < 3 * n_site || (int)site_weights.size() < n_site ||
+      (int)site_flags.size() < n_site ||
+      (int64_t)site_knn.size() < (int64_t)site_k * n_site)
+    throw std::runtime_error("RPD: site arrays too short");
+  if ((int)v_adjs.size() < n_vert || (int64_t)f_adjs.size() < 4LL * n_tet ||
+      (int64_t)f_ids.size() < 4LL * n_tet ||
+      (int64_t)tet_e_adjs.size() < 6LL * n_tet)
+    throw std::runtime_error("RPD: adjacency arrays too short");
+  std::vector<double> verts(vertices.begin(), vertices.begin() + 3LL * n_vert);
+  std::vector<double> sites(4LL * n_site);
+  for (int i = 0; i < n_site; ++i) {
+    sites[4 * i + 0] = site[i];
+    sites[4 * i + 1] = site[i + n_site];
+    sites[4 * i + 2] = site[i + 2 * n_site];
+    sites[4 * i + 3] = site_weights[i];
+  }
+  std::vector<uint32_t> flags(site_flags.begin(), site_flags.begin() + n_site);
+  std::vector<int64_t> knn_off(n_site + 1, 0);
+  std::vector<int32_t> knn_ids;
+  for (int i = 0; i < n_site; ++i) {
+    for (int r = 0; r < site_k; ++r) {
+      int m = site_knn[(int64_t)r * n_site + i];
+      if (m == -1) break;
+      knn_ids.push_back(m);
+    }
+    knn_off[i + 1] = (int64_t)knn_ids.size();
+  }
+  rpd_instance_desc d;
+  d.n_vert = n_vert;
+  d.n_tet = n_tet;
+  d.n_site = n_site;
+  d.verts = verts.data();
+  d.tets = indices.data();
+  d.v_adj = v_adjs.data();
+  d.e_adj = tet_e_adjs.data();
+  d.f_adj = f_adjs.data();
+  d.f_id = f_ids.data();
+  d.sites = sites.data();
+  d.flags = flags.data();
+  d.knn_off = knn_off.data();
+  d.knn_ids = knn_ids.data();
+
+  std::lock_guard<std::mutex> lock(g_mu);
+  rpd_ctx* ctx = shared_ctx();
+  check(ctx, rpd_upload(ctx, &d), "rpd_upload");
+  rpd_run_cfg cfg{};
+  cfg.cells_per_wave = 8;
+  cfg.use_active_mask = 1;
+  cfg.compute_geometry = 1;
+  cfg.small_tier = 1;
+  check(ctx, rpd_run(ctx, &cfg), "rpd_run");
+  rpd_stats st;
+  rpd_get_stats(ctx, &st);
+  std::vector<rpd_cell> cells(st.n_cells);
+  std::vector<uint32_t> words(st.n_words);
+  std::vector<double> geom(4 * st.n_cells);
+  std::vector<double> areas(st.n_areas);
+  check(ctx,
+        rpd_download_cells(ctx, cells.data(), words.data(), geom.data(),
+                           areas.data()),
+        "rpd_download_cells");
+  std::vector<int64_t> rel_off(n_tet + 1);
+  std::vector<int32_t> rel_ids(std::max<int64_t>(st.n_candidates, 1));
+  check(ctx, rpd_download_relation(ctx, rel_off.data(), rel_ids.data()),
+        "rpd_download_relation");
+  const int tet_k = st.tet_k;
+
+  // float planes with the reference's arithmetic (tri2plane,
+  // common_cuda.h:248-254; new_plane, convex_cell.cu:561-592)
+  static const int fl[4][3] = {{2, 1, 3}, {0, 2, 3}, {1, 0, 3}, {0, 1, 2}};
+  auto tet_plane = [&](int t, int k) {
+    const float* a = &vertices[3LL * indices[4LL * t + fl[k][0]]];
+    const float* b = &vertices[3LL * indices[4LL * t + fl[k][1]]];
+    const float* c = &vertices[3LL * indices[4LL * t + fl[k][2]]];
+    float ux = b[0] - a[0], uy = b[1] - a[1], uz = b[2] - a[2];
+    float vx = c[0] - a[0], vy = c[1] - a[1], vz = c[2] - a[2];
+    float nx = uy * vz - uz * vy, ny = uz * vx - ux * vz,
+          nz = ux * vy - uy * vx;
+    return cfloat4{nx, ny, nz, -(a[0] * nx + a[1] * ny + a[2] * nz)};
+  };
+  auto bisector = [&](int s, int m) {
+    float sx = site[s], sy = site[s + n_site], sz = site[s + 2 * n_site],
+          sw = site_weights[s];
+    float bx = site[m], by = site[m + n_site], bz = site[m + 2 * n_site],
+          bw = site_weights[m];
+    float dx = sx - bx, dy = sy - by, dz = sz - bz;
+    float ax = sx + bx, ay = sy + by, az = sz + bz;
+    float dot = ax * dx + ay * dy + az * dz + (bw - sw);
+    return cfloat4{dx, dy, dz, -dot / 2.f};
+  };
+
+  site_cell_vol.assign(n_site, 0.f);
+  std::vector<double> vol_acc(n_site, 0.0);
+  std::vector<ConvexCellHost> out(st.n_cells);
+  for (int64_t q = 0; q < st.n_cells; ++q) {
+    const rpd_cell& c = cells[q];
+    ConvexCellHost& h = out[q];
+    const int t = c.tet_id, s = c.seed;
+    // the reference's thread index: tet * tet_k + slot of s in tet_knn
+    int slot = 0;
+    for (int64_t r = rel_off[t]; r < rel_off[t + 1]; ++r, ++slot)
+      if (rel_ids[r] == s) break;
+    h.status = success;
+    h.thread_id = t * tet_k + slot;
+    h.voro_id = s;
+    h.tet_id = t;
+    h.weight = site_weights[s];
+    h.is_active = true;
+    h.nb_v = c.nb_v;
+    h.nb_p = c.nb_p;
+    h.nb_e = c.nb_e;
+    const uint32_t* w = words.data() + c.word_off;
+    for (int v = 0; v < c.nb_v; ++v)
+      h.ver_data_trans[v] = cuchar4{(uchar)(w[v] & 255),
+                                    (uchar)((w[v] >> 8) & 255),
+                                    (uchar)((w[v] >> 16) & 255),
+                                    (uchar)(w[v] >> 24)};
+    const int32_t* src = reinterpret_cast<const int32_t*>(w + c.nb_v);
+    for (int p = 0; p < c.nb_p; ++p) {
+      cfloat4 pl;
+      float hh;
+      if (src[p] < 0) {
+        const int k = -1 - src[p];
+        pl = tet_plane(t, k);
+        hh = (float)f_adjs[4LL * t + k];
+        h.clip_id2_data_trans[p] = cint2{f_ids[4LL * t + k], -1};
+      } else {
+        pl = bisector(s, src[p]);
+        hh = F_CELL_ADJ_DEFAULT;
+        h.clip_id2_data_trans[p] =
+            cint2{std::min(s, src[p]), std::max(s, src[p])};
+      }
+      h.clip_data_trans[p] = cfloat5{pl.x, pl.y, pl.z, pl.w, hh};
+    }
+    const uint32_t* e = w + c.nb_v + c.nb_p;
+    for (int k = 0; k < c.nb_e; ++k)
+      h.edge_data[k] = cuchar3{(uchar)(e[k] & 255), (uchar)((e[k] >> 8) & 255),
+                               (uchar)((e[k] >> 16) & 255)};
+    h.euler = UNK_FLOAT;
+    h.cell_vol = (float)geom[4 * q];
+    vol_acc[s] += geom[4 * q];
+    h.id = (int)q;
+  }
+  for (int i = 0; i < n_site; ++i) site_cell_vol[i] = (float)vol_acc[i];
+  return out;
+}
+
+}  // namespace
+
+// ------------------------------------------------------- ConvexCellHost
+cfloat4 ConvexCellHost::compute_vertex_coordinates(cuchar3 t,
+                                                   bool persp_divide) const {
+  cfloat4 p1 = clip_trans_const(t.x), p2 = clip_trans_const(t.y),
+          p3 = clip_trans_const(t.z);
+  cfloat4 r;
+  r.x = -det3f(p1.w, p1.y, p1.z, p2.w, p2.y, p2.z, p3.w, p3.y, p3.z);
+  r.y = -det3f(p1.x, p1.w, p1.z, p2.x, p2.w, p2.z, p3.x, p3.w, p3.z);
+  r.z = -det3f(p1.x, p1.y, p1.w, p2.x, p2.y, p2.w, p3.x, p3.y, p3.w);
+  r.w = det3f(p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z);
+  if (persp_divide) return cfloat4{r.x / r.w, r.y / r.w, r.z / r.w, 1};
+  return r;
+}
+
+void ConvexCellHost::print_info() const {
+  printf(
+      "Cell of id: %d, voro_id: %d, sq_radius: %f, tet_id: %d, nb_v: %d, "
+      "nb_p: %d, nb_e: %d, status: %d, thread_id: %d\n",
+      id, voro_id, weight, tet_id, nb_v, nb_p, nb_e, (int)status, thread_id);
+}
+
+void ConvexCellHost::reload_active() {
+  active_clipping_planes.assign(nb_p + 1, 0);
+  std::map<int, std::set<int>> plane2v;
+  for (int t = 0; t < nb_v; ++t)
+    for (int k = 0; k < 3; ++k) {
+      int p = ith_plane(t, k);
+      active_clipping_planes[p]++;
+      plane2v[p].insert(t);
+    }
+  active_edges.assign(nb_e + 1, 0);
+  std::set<int> common;
+  for (int e = 0; e < nb_e; ++e) {
+    cuchar3 ed = edge_data[e];
+    if (active_clipping_planes[ed.x] <= 0 || active_clipping_planes[ed.y] <= 0)
+      continue;
+    set_intersection2(plane2v.at(ed.x), plane2v.at(ed.y), common);
+    if (common.size() < 2) continue;
+    active_edges[e]++;
+  }
+  is_active_updated = true;
+}
+
+void ConvexCellHost::reload_pc_explicit() {
+  pc_points.clear();
+  for (int i = 0; i < nb_v; ++i) {
+    cuchar4 v = ver_trans(i);
+    cfloat4 p = compute_vertex_coordinates(cuchar3{v.x, v.y, v.z});
+    if (std::isnan(p.x) || std::isnan(p.y) || std::isnan(p.z)) {
+      is_vertex_null = true;
+      return;
+    }
+    pc_points.push_back(cfloat3{p.x, p.y, p.z});
+  }
+  if (!is_active_updated) reload_active();
+  pc_local_active_faces.clear();
+  pc_lf2active_map.assign(nb_p, -1);
+  int lf = 0;
+  for (int plane = 0; plane < nb_p; ++plane) {
+    if (active_clipping_planes[plane] <= 0) continue;
+    std::vector<int> tab_v, tab_lp;
+    for (int t = 0; t < nb_v; ++t)
+      for (int k = 0; k < 3; ++k)
+        if (ith_plane(t, k) == plane) {
+          tab_v.push_back(t);
+          tab_lp.push_back(k);
+          break;
+        }
+    // walk the face: next of (t, slot) is the vertex whose plane at
+    // (slot' + 2) % 3 equals t's plane at (slot + 1) % 3
+    std::vector<int> face;
+    int i = 0;
+    std::vector<char> used(tab_v.size(), 0);
+    while (face.size() < tab_v.size()) {
+      face.push_back(tab_v[i]);
+      used[i] = 1;
+      int want = ith_plane(tab_v[i], (tab_lp[i] + 1) % 3);
+      int nxt = -1;
+      for (size_t j = 0; j < tab_v.size(); ++j)
+        if (!used[j] && ith_plane(tab_v[j], (tab_lp[j] + 2) % 3) == want) {
+          nxt = (int)j;
+          break;
+        }
+      if (nxt < 0) break;
+      i = nxt;
+    }
+    pc_local_active_faces.push_back(face);
+    pc_lf2active_map[plane] = lf++;
+  }
+  is_pc_explicit = true;
+}
+
+Scalar ConvexCellHost::cal_cell_euler() {
+  Scalar sv = 0, sf = 0, se = 0;
+  if (!is_active_updated) reload_active();
+  for (int v = 0; v < nb_v; ++v) sv += 1.0 / (int)ver_trans(v).w;
+  for (int f = 0; f < nb_p; ++f)
+    if (active_clipping_planes[f] > 0) sf += 1.0 / clip_trans(f).h;
+  for (int e = 0; e < nb_e; ++e)
+    if (active_edges[e] > 0) se += 1.0 / edge(e).z;
+  euler = (float)(sv - se + sf);
+  return sv - se + sf;
+}
+
+Scalar ConvexCellHost::cal_halfplane_facet_euler(int neigh_id) {
+  if (!is_active_updated) reload_active();
+  int clip_id = -1;
+  for (int f = 0; f < nb_p; ++f) {
+    if (active_clipping_planes[f] <= 0) continue;
+    cint2 id2 = clip_id2_const(f);
+    if (id2.x == neigh_id || id2.y == neigh_id) {
+      clip_id = f;
+      break;
+    }
+  }
+  if (clip_id < 0) throw std::runtime_error("halfplane facet not found");
+  Scalar sv = 0, se = 0, sf = 1;
+  for (int v = 0; v < nb_v; ++v) {
+    cuchar4 t = ver_trans(v);
+    if (t.x != clip_id && t.y != clip_id && t.z != clip_id) continue;
+    sv += 1.0 / (int)t.w;
+  }
+  for (int e = 0; e < nb_e; ++e) {
+    cuchar3 ed = edge(e);
+    if (ed.x != clip_id && ed.y != clip_id) continue;
+    if (active_edges[e] <= 0) continue;
+    se += 1.0 / ed.z;
+  }
+  return (float)(sv - se + sf);
+}
+
+bool is_convex_cell_valid(const ConvexCellHost& cell) {
+  return cell.status == success || cell.status == security_radius_not_reached;
+}
+
+// -------------------------------------------------------- entry points
+std::vector<ConvexCellHost> compute_clipped_voro_diagram_GPU(
+    const int, const std::vector<float>& vertices,
+    const std::vector<int>& indices, const std::map<int, std::set<int>>&,
+    const std::vector<int>& v_adjs, const std::vector<int>& e_adjs,
+    const std::vector<int>& f_adjs, const std::vector<int>& f_ids,
+    std::vector<float>& site, const int n_site,
+    const std::vector<float>& site_weights, const std::vector<uint>& site_flags,
+    const std::vector<int>& site_knn, const int site_k,
+    std::vector<float>& site_cell_vol, const bool, int, int) {
+  const int64_t n_vert = (int64_t)vertices.size() / 3;
+  const int64_t n_tet = (int64_t)indices.size() / 4;
+  std::vector<int> tet_e(6 * n_tet);
+  static const int el[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+  for (int64_t t = 0; t < n_tet; ++t)
+    for (int k = 0; k < 6; ++k) {
+      int64_t idx = tri_edge_idx(indices[4 * t + el[k][0]],
+                                 indices[4 * t + el[k][1]], n_vert);
+      if (idx < 0 || idx >= (int64_t)e_adjs.size())
+        throw std::runtime_error("RPD: e_adjs index out of range");
+      tet_e[6 * t + k] = e_adjs[idx];
+    }
+  return run_rpd(vertices, indices, v_adjs, tet_e, f_adjs, f_ids, site,
+                 n_site, site_weights, site_flags, site_knn, site_k,
+                 site_cell_vol);
+}
+
+std::vector<ConvexCellHost> compute_clipped_voro_diagram_GPU_tet_eadj(
+    const std::vector<float>& vertices, const std::vector<int>& indices,
+    const std::vector<int>& v_adjs, const std::vector<int>& tet_e_adjs,
+    const std::vector<int>& f_adjs, const std::vector<int>& f_ids,
+    std::vector<float>& site, const int n_site,
+    const std::vector<float>& site_weights, const std::vector<uint>& site_flags,
+    const std::vector<int>& site_knn, const int site_k,
+    std::vector<float>& site_cell_vol) {
+  return run_rpd(vertices, indices, v_adjs, tet_e_adjs, f_adjs, f_ids, site,
+                 n_site, site_weights, site_flags, site_knn, site_k,
+                 site_cell_vol);
+}
+
+void libmat_rpd::power_neighbors(
+    const std::vector<std::array<double, 4>>& xyzr,
+    const std::vector<char>& live, const double box[6],
+    std::vector<char>& valid, std::vector<std::set<int>>& neighbors) {
+  const int n = (int)xyzr.size();
+  std::vector<int> idx;
+  std::vector<double> xyzw;
+  for (int i = 0; i < n; ++i) {
+    if (!live[i]) continue;
+    idx.push_back(i);
+    xyzw.insert(xyzw.end(), {xyzr[i][0], xyzr[i][1], xyzr[i][2],
+                             xyzr[i][3] * xyzr[i][3]});
+  }
+  rpd_nbr* h = nullptr;
+  int rc = rpd_power_neighbors((int32_t)idx.size(), xyzw.data(), box, 0, &h);
+  if (rc != 0) throw std::runtime_error("rpd_power_neighbors failed");
+  const int m = (int)idx.size();
+  std::vector<int32_t> v(m);
+  std::vector<int64_t> off(m + 1);
+  std::vector<int32_t> ids(std::max<int64_t>(rpd_nbr_total(h), 1));
+  rpd_nbr_get(h, v.data(), off.data(), ids.data());
+  rpd_nbr_free(h);
+  valid.assign(n, 0);
+  neighbors.assign(n, {});
+  for (int a = 0; a < m; ++a) {
+    valid[idx[a]] = v[a] != 0;
+    for (int64_t q = off[a]; q < off[a + 1]; ++q)
+      neighbors[idx[a]].insert(idx[ids[q]]);
+  }
+}
+
+// ------------------------------------------------------------ adjacency
+std::map<int, PowerCellAdjacency> update_power_cells_adjacency(
+    std::vector<ConvexCellHost>& cells, int max_surf_fid, int n_threads) {
+  std::map<int, PowerCellAdjacency> pcs;
+  for (auto& c : cells) {
+    auto& pc = pcs[c.voro_id];
+    pc.voro_id = c.voro_id;
+    pc.cell_ids.insert(c.id);
+    pc.tet_ids.insert(c.tet_id);
+    if (!c.is_active_updated) c.reload_active();
+  }
+  std::vector<int> keys;
+  for (auto& kv : pcs) keys.push_back(kv.first);
+  std::map<int, int> id2pos;
+  for (size_t q = 0; q < cells.size(); ++q) id2pos[cells[q].id] = (int)q;
+  auto work = [&](int sphere) {
+    PowerCellAdjacency& pc = pcs.at(sphere);
+    std::map<int, std::set<int>> tfid_to_cells;
+    for (int cid : pc.cell_ids) {
+      const ConvexCellHost& c = cells[id2pos.at(cid)];
+      for (int p = 0; p < c.nb_p; ++p) {
+        if (c.active_clipping_planes[p] <= 0) continue;
+        cint2 hp = c.clip_id2_const(p);
+        if (hp.y != -1) {
+          int nb = hp.x == c.voro_id ? hp.y : hp.x;
+          pc.facet_neigh_to_cells[nb].insert(cid);
+        } else {
+          tfid_to_cells[hp.x].insert(cid);
+          pc.cell_to_tfids[cid].insert(hp.x);
+        }
+      }
+      for (int v = 0; v < c.nb_v; ++v) {
+        std::set<int> seeds;
+        for (int k = 0; k < 3; ++k) {
+          cint2 hp = c.clip_id2_const(c.ith_plane_const(v, k));
+          if (hp.y != -1) seeds.insert(hp.x == c.voro_id ? hp.y : hp.x);
+        }
+        if (seeds.size() < 2) continue;
+        if (seeds.size() == 2) seeds.insert(-1);
+        aint3 key3;
+        int q = 0;
+        for (int sd : seeds) key3[q++] = sd;
+        std::sort(key3.begin(), key3.end());
+        pc.vertex_2id[aint2{{cid, v}}] = key3;
+      }
+      for (int e = 0; e < c.nb_e; ++e) {
+        if (c.active_edges[e] <= 0) continue;
+        cuchar2 ef = c.edge_id2_const(e);
+        cint2 h1 = c.clip_id2_const(ef.x), h2 = c.clip_id2_const(ef.y);
+        if (h1.y == -1 || h2.y == -1) continue;
+        aint2 eu{{h1.x == c.voro_id ? h1.y : h1.x,
+                  h2.x == c.voro_id ? h2.y : h2.x}};
+        std::sort(eu.begin(), eu.end());
+        pc.e_to_cells[eu].insert(cid);
+      }
+    }
+    (void)max_surf_fid;
+    for (auto& kv : tfid_to_cells) {
+      if (kv.second.size() != 2) continue;
+      int a = *kv.second.begin(), b = *kv.second.rbegin();
+      pc.cell_neighbors[a].insert(b);
+      pc.cell_neighbors[b].insert(a);
+    }
+  };
+  int nt = n_threads > 0 ? n_threads : (int)std::thread::hardware_concurrency();
+  nt = std::max(1, std::min<int>(nt, (int)keys.size()));
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&, t]() {
+      for (size_t q = t; q < keys.size(); q += nt) work(keys[q]);
+    });
+  for (auto& th : pool) th.join();
+  return pcs;
+}

@@ -1,0 +1,115 @@
+// Drop-in C++ surface test: ConvexCellHost layout (always) and, with
+// --gpu, compute_clipped_voro_diagram_GPU / RPD3D_GPU / adjacency on the
+// 10^3-grid, 500-sphere instance built through the C ABI of rpd_prep.h.
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "libmat_rpd.h"
+#include "rpd_prep.h"
+
+#define CHECK(c)                                               \
+  do {                                                         \
+    if (!(c)) {                                                \
+      std::printf("CHECK failed line %d: %s\n", __LINE__, #c); \
+      return 1;                                                \
+    }                                                          \
+  } while (0)
+
+int main(int argc, char** argv) {
+  // byte layout of the reference record (GCC 11, x86-64)
+  CHECK(sizeof(ConvexCellHost) == 3776);
+  CHECK(alignof(ConvexCellHost) == 32);
+  CHECK(offsetof(ConvexCellHost, ver_data_trans) == 164);
+  CHECK(offsetof(ConvexCellHost, clip_data_trans) == 576);
+  CHECK(offsetof(ConvexCellHost, clip_id2_data_trans) == 2624);
+  CHECK(offsetof(ConvexCellHost, edge_data) == 3136);
+  CHECK(offsetof(ConvexCellHost, cell_vol) == 3748);
+  std::printf("layout ok\n");
+  if (argc < 2 || std::strcmp(argv[1], "--gpu") != 0) return 0;
+
+  const int n = 10, nv = 11 * 11 * 11, nt = 6 * n * n * n;
+  std::vector<double> vd(3 * nv);
+  std::vector<int> tets(4 * nt);
+  CHECK(rpd_synth_grid_mesh(n, 0.0, 0, vd.data(), tets.data()) == 0);
+  std::vector<int> v_adj(nv), e_adj(6 * nt), f_adj(4 * nt), f_id(4 * nt);
+  int n_sf = 0;
+  CHECK(rpd_tet_adjacency(nv, nt, tets.data(), v_adj.data(), e_adj.data(),
+                          f_adj.data(), f_id.data(), &n_sf) == 0);
+  std::vector<double> sph(4 * 500);
+  CHECK(rpd_synth_spheres(500, 0, 0.02, 0.08, 0, sph.data()) == 0);
+
+  // reference-format mesh: float AoS vertices, triangular e_adjs
+  RpdTetMesh mesh;
+  mesh.tet_vertices.assign(vd.begin(), vd.end());
+  mesh.tet_indices = tets;
+  mesh.v_adjs = v_adj;
+  mesh.f_adjs = f_adj;
+  mesh.f_ids = f_id;
+  mesh.e_adjs.assign((size_t)nv * (1 + nv) / 2 + 1, -1);
+  static const int el[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+  for (int t = 0; t < nt; ++t)
+    for (int k = 0; k < 6; ++k) {
+      long a = tets[4 * t + el[k][0]], b = tets[4 * t + el[k][1]];
+      long lo = std::min(a, b), hi = std::max(a, b);
+      long idx = (lo + 1) * nv - lo * (lo + 1) / 2 - (nv - hi);
+      mesh.e_adjs[idx] = e_adj[6 * t + k];
+    }
+  std::vector<RpdSphere> spheres(500);
+  for (int i = 0; i < 500; ++i) {
+    spheres[i].id = i;
+    spheres[i].center = {sph[4 * i], sph[4 * i + 1], sph[4 * i + 2]};
+    spheres[i].radius = sph[4 * i + 3];
+  }
+  RPD3D_GPU rpd;
+  rpd.init(&mesh, nullptr, nullptr);
+  rpd.set_spheres(&spheres);
+  rpd.calculate();
+  const auto& cells = rpd.powercells;
+  std::printf("powercells %zu n_site %d site_k %d\n", cells.size(), rpd.n_site,
+              rpd.site_k);
+  CHECK(!cells.empty());
+  double vol = 0;
+  for (size_t q = 0; q < cells.size(); ++q) {
+    CHECK(cells[q].id == (int)q);
+    CHECK(is_convex_cell_valid(cells[q]));
+    CHECK(cells[q].voro_id >= 0 && cells[q].voro_id < 500);
+    vol += cells[q].cell_vol;
+  }
+  std::printf("total volume %.9f\n", vol);
+  CHECK(std::fabs(vol - 1.0) < 1e-5);
+  // Euler characteristic of every (convex) cell is 1 with the reference's
+  // adjacency weights summed over all cells of a tet ... per cell: V-E+F
+  // with unit weights is 2 for a convex polytope
+  std::vector<ConvexCellHost> cc = cells;
+  int bad_euler = 0;
+  for (auto& c : cc) {
+    c.reload_active();
+    int V = c.nb_v, E = 0, F = 0;
+    for (int e = 0; e < c.nb_e; ++e) E += c.active_edges[e] > 0;
+    for (int p = 0; p < c.nb_p; ++p) F += c.active_clipping_planes[p] > 0;
+    bad_euler += (V - E + F) != 2;
+  }
+  std::printf("cells with V-E+F != 2: %d\n", bad_euler);
+  CHECK(bad_euler == 0);
+  auto adj = update_power_cells_adjacency(cc, n_sf - 1);
+  // facet symmetry: sphere a has a halfplane facet with b in tet t iff b
+  // has one with a in t
+  std::set<std::array<int, 3>> fac;
+  for (auto& c : cc)
+    for (int p = 0; p < c.nb_p; ++p) {
+      if (c.active_clipping_planes[p] <= 0) continue;
+      cint2 h = c.clip_id2_const(p);
+      if (h.y == -1) continue;
+      int nb = h.x == c.voro_id ? h.y : h.x;
+      fac.insert({c.tet_id, c.voro_id, nb});
+    }
+  int asym = 0;
+  for (auto& f : fac) asym += !fac.count({f[0], f[2], f[1]});
+  std::printf("adjacency spheres %zu, facets %zu, asymmetric %d\n",
+              adj.size(), fac.size(), asym);
+  CHECK(asym == 0);
+  std::printf("gpu ok\n");
+  return 0;
+}

@@ -63,3 +63,40 @@ def compare_with_oracle(res, orc, geom_rtol=1e-6, check_geometry=True):
 def compare_relation(gpu_off, gpu_ids, orc):
     assert np.array_equal(gpu_off, orc.rel_off), "relation offsets differ"
     assert np.array_equal(gpu_ids, orc.rel_ids), "relation ids differ"
+
+
+def digest_oracle(cells):
+    """SHA-256 over the combinatorics of oracle cells, in order."""
+    import hashlib
+    h = hashlib.sha256()
+    for c in cells:
+        nv, np_, ne = int(c["nb_v"]), int(c["nb_p"]), int(c["nb_e"])
+        h.update(np.array([c["tet"], c["seed"], nv, np_, ne],
+                          np.int64).tobytes())
+        h.update(c["ver"][:nv].astype(np.uint32).tobytes())
+        h.update(c["src"][:np_].astype(np.int32).tobytes())
+        h.update(c["edge"][:ne].astype(np.uint32).tobytes())
+    return h.hexdigest()
+
+
+def digest_gpu(res):
+    """Same digest over the compact GPU stream."""
+    import hashlib
+    h = hashlib.sha256()
+    for q in range(len(res)):
+        c = res.cells[q]
+        nv, np_, ne = int(c["nb_v"]), int(c["nb_p"]), int(c["nb_e"])
+        h.update(np.array([c["tet"], c["seed"], nv, np_, ne],
+                          np.int64).tobytes())
+        v, s, e, _ = res.cell(q)
+        h.update(v.astype(np.uint32).tobytes())
+        h.update(s.astype(np.int32).tobytes())
+        h.update(e.astype(np.uint32).tobytes())
+    return h.hexdigest()
+
+
+def tet_volumes(inst):
+    V = inst.verts[inst.tets]
+    return np.abs(np.einsum("ij,ij->i", V[:, 1] - V[:, 0],
+                            np.cross(V[:, 2] - V[:, 0],
+                                     V[:, 3] - V[:, 0]))) / 6.0

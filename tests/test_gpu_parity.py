@@ -66,3 +66,82 @@ def test_determinism(ctx):
     b = ctx.download()
     for f in ("cells", "words", "geom", "areas"):
         assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_config1_matches_golden_fixture(ctx, seed):
+    import os
+    from rpd_compare import digest_gpu
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden",
+                             f"config1_seed{seed}.npz"))
+    inst = prep.config1(seed)
+    ctx.upload(inst)
+    ctx.run()
+    off, ids = ctx.relation()
+    assert np.array_equal(off, g["rel_off"]) and np.array_equal(ids,
+                                                                g["rel_ids"])
+    res = ctx.download()
+    assert len(res) == int(g["n_cells"])
+    assert digest_gpu(res) == str(g["cells_digest"])
+
+
+def test_octant_spheres_exact_path_on_gpu(ctx):
+    # bisectors on tet faces: vertex-on-plane determinants are exactly 0 and
+    # the filtered predicate must hand them to the exact re-run
+    verts, tets = prep.grid_mesh(4, 0.0, 0)
+    sph = [[x, y, z, 0.1] for x in (0.25, 0.75) for y in (0.25, 0.75)
+           for z in (0.25, 0.75)]
+    inst = prep.build_instance(verts, tets, np.array(sph))
+    orc = oracle.run(inst)
+    ctx.upload(inst)
+    st = ctx.run()
+    print(f"[exact] reruns large/exact = {st['n_rerun_large']}/"
+          f"{st['n_rerun_exact']}", flush=True)
+    assert st["n_rerun_exact"] > 0
+    compare_with_oracle(ctx.download(), orc)
+
+
+def test_config2_sample_and_partition(ctx):
+    from libmat_amd.rpd import RPDResult
+    from rpd_compare import tet_volumes
+    inst = prep.config2(0, n_threads=16)
+    print(f"[config2] n_tet={inst.n_tet} n_site={inst.n_site}", flush=True)
+    ctx.upload(inst)
+    st = ctx.run()
+    res = ctx.download()
+    off, ids = ctx.relation()
+    vol = np.zeros(inst.n_tet)
+    np.add.at(vol, res.cells["tet"], res.geom[:, 0])
+    tv = tet_volumes(inst)
+    err = np.max(np.abs(vol - tv) / tv)
+    print(f"[config2] cells={st['n_cells']} partition max rel err {err:.2e}",
+          flush=True)
+    assert err < 1e-10
+    sample = np.sort(np.random.default_rng(7).choice(
+        inst.n_tet, 400, replace=False)).astype(np.int32)
+    orc = oracle.run(inst, tets=sample, n_threads=16)
+    for q, t in enumerate(sample):
+        assert np.array_equal(ids[off[t]:off[t + 1]],
+                              orc.rel_ids[orc.rel_off[q]:orc.rel_off[q + 1]])
+    idx = np.nonzero(np.isin(res.cells["tet"], sample))[0]
+    sub = RPDResult(res.cells[idx], res.words, res.geom[idx].reshape(-1),
+                    res.areas)
+    m = compare_with_oracle(sub, orc)
+    print(f"[config2] sample {m}", flush=True)
+
+
+def test_cpp_dropin_on_gpu(tmp_path):
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib = os.path.join(root, "libmat_amd", "_lib")
+    exe = tmp_path / "compat_test"
+    subprocess.run(["g++", "-O2", "-std=c++17",
+                    "-I" + os.path.join(root, "include"),
+                    os.path.join(root, "tests/cpp/compat_test.cpp"), "-o",
+                    str(exe), "-L" + lib, "-lrpd_amd", "-Wl,-rpath," + lib],
+                   check=True)
+    out = subprocess.run([str(exe), "--gpu"], capture_output=True, text=True,
+                         timeout=120)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0 and "gpu ok" in out.stdout
