@@ -1,15 +1,18 @@
 """Benchmark of the RPD hot path on one MI355X.
 
-One step = one rpd_run over the resident config-2 instance (64^3 jittered
-Kuhn grid, 1,572,864 tets; 50,000 spheres with the boundary radius rule,
-18,114 of them non-hidden; binary64): candidate gathering, convex-cell
-clipping with capacity / exact re-runs, FP64 geometry and compaction, with
-every input already in HBM.  Prints one JSON line.
+One step = one rpd_run of the plain path over a resident instance: the
+unit cube as a 64^3 jittered Kuhn grid (1,572,864 tets) with 50,000 spheres
+under the boundary radius rule (18,114 of them non-hidden), binary64:
+candidate gathering, convex-cell clipping with capacity / exact re-runs,
+FP64 geometry and compaction, with every input already in HBM.  Prints one
+JSON line.
 
-roofline: the dominant kernel is the small-tier clip kernel; its
+roofline: the dominant kernel is the small-tier clip kernel.  Its
 algorithmic bytes per launch are the compulsory reads and writes of the
-cells it clips (formula in clip_bytes below), divided by its average launch
-duration measured with HIP events on the context stream.
+cells it clips (clip_bytes), its algorithmic flops the FP64 work it executes
+(clip_flops, counted by the kernel in an extra untimed run); both are divided
+by its average launch duration measured with HIP events on the context
+stream.
 """
 import argparse
 import json
@@ -21,6 +24,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+METRIC = ("RPD cells/sec (tets clipped) learned+batched vs plain on 1 "
+          "MI355X; fallback %")
 
 
 def parse():
@@ -29,13 +35,16 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells-per-wave", type=int, default=8)
-    ap.add_argument("--config", default="config2",
-                    choices=["config1", "config2"])
-    ap.add_argument("--cpu-sample-tets", type=int, default=100000)
+    ap.add_argument("--workload", default="cube64",
+                    choices=["cube10", "cube64"])
+    ap.add_argument("--cpu-sample-tets", type=int, default=300000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-work-count", action="store_true",
+                    help="skip the extra untimed run that counts FP64 work "
+                         "(keeps profiler averages to the timed launches)")
     ap.add_argument("--traffic-json", default=os.path.join(
-        ROOT, "profiles", "clip_traffic_config2.json"),
+        ROOT, "profiles", "clip_traffic_cube64.json"),
         help="per-launch HBM traffic of the clip kernel measured by "
              "rocprofv3 --pmc (FETCH_SIZE / WRITE_SIZE, copy-calibrated)")
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
@@ -44,51 +53,60 @@ def parse():
     return ap.parse_args()
 
 
-def clip_bytes(st):
-    """Compulsory bytes of one small-tier clip launch.
+def clip_bytes(st, inst):
+    """Compulsory HBM bytes of one small-tier clip launch, each counted once:
+    reads of the candidate records (24 B each), the tet vertex ids and
+    packed adjacency counts (16 + 16 B per tet), the vertices (24 B), the
+    sites (32 B), the neighbour-list offsets (8 B) and ids (4 B); writes of
+    one 4 B header per candidate and of the emitted cells' payload words
+    (4 B), volume + centroid (32 B) and facet areas (8 B)."""
+    reads = (st["n_candidates"] * 24 + inst.n_tet * 32
+             + inst.verts.shape[0] * 24 + inst.n_site * (32 + 8)
+             + inst.knn_ids.size * 4)
+    writes = (st["n_candidates"] * 4 + st["n_words"] * 4
+              + st["n_cells"] * 32 + st["n_areas"] * 8)
+    return reads + writes
 
-    Per candidate cell: candidate record 24, tet vertex ids 16, packed
-    adjacency counts 16, four vertices 4 x 24, seed sphere 32, seed
-    neighbour-list offsets 16, slot header 4.  Per neighbour plane that may
-    cut: neighbour id 4 + neighbour sphere 32.  Per emitted cell: payload
-    words 4 each, volume + centroid 32, facet areas 8 per plane."""
-    n_cand = st["n_candidates"]
-    return (n_cand * (24 + 16 + 16 + 96 + 32 + 16 + 4)
-            + st["n_active_planes"] * 36
-            + st["n_words"] * 4 + st["n_cells"] * 32 + st["n_areas"] * 8)
+
+def clip_flops(w):
+    """Algorithmic FP64 flops of one clip launch from its executed-work
+    counts: 45 per 4x4 conflict determinant, 14 per bisector plane, 59 per
+    emitted vertex position (four 3x3 determinants and three divisions)."""
+    return 45 * w[0] + 14 * w[1] + 59 * w[2]
 
 
 def cpu_baseline(inst, sample_tets, threads):
-    """Reference-arithmetic oracle (float planes, FP64 determinant, clip by
-    the full neighbour list: the reference algorithm) clipping the candidate
-    cells of a random tet sample on the host cores.  Candidate lists come
-    from the oracle's own relation test (not timed)."""
+    """The reference algorithm on the host cores: for a random tet sample,
+    the dense tet-sphere relation over every site (voronoi.cu:154-193) and
+    the clip of each related cell by its full neighbour list in reference
+    arithmetic (float planes, FP64 determinant; convex_cell.cu:116-774),
+    multithreaded over tets.  value = cells / (relation + clip time)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     rng = np.random.default_rng(12345)
     tets = np.sort(rng.choice(inst.n_tet, min(sample_tets, inst.n_tet),
                               replace=False)).astype(np.int32)
+    oracle.run(inst, tets=tets[:500], mode=1, n_threads=threads)  # warm-up
     t0 = time.perf_counter()
-    rel = oracle.run(inst, tets=tets[:2000], mode=1, n_threads=threads)
-    # relation for the whole sample (dense over all sites, untimed)
     rel = oracle.run(inst, tets=tets, mode=1, n_threads=threads)
     t_rel = time.perf_counter() - t0
-    best = None
+    t_clip = None
     for _ in range(3):
         t0 = time.perf_counter()
         r = oracle.clip(inst, tets, rel.rel_off, rel.rel_ids, mode=1,
                         n_threads=threads)
         dt = time.perf_counter() - t0
-        best = dt if best is None else min(best, dt)
-    return {"value": r.cells.size / best, "unit": "cells/s",
+        t_clip = dt if t_clip is None else min(t_clip, dt)
+    n = r.cells.size
+    return {"value": n / (t_rel + t_clip), "unit": "cells/s",
             "cores": threads, "kind": "port",
-            "sample": f"{tets.size} random tets of the same instance, "
-                      f"{r.cells.size} cells: clip of the given candidate "
-                      f"cells by full neighbour lists in reference float "
-                      f"arithmetic, best of 3 ({best:.2f} s); the dense "
-                      f"relation for the sample took {t_rel:.1f} s more and "
-                      f"is not in the value"}
+            "clip_only_value": n / t_clip,
+            "sample": f"{tets.size} random tets of the same instance "
+                      f"({n} cells): dense relation over all "
+                      f"{inst.n_site} sites {t_rel:.2f} s + clip by full "
+                      f"neighbour lists in reference float arithmetic "
+                      f"{t_clip:.2f} s (best of 3), {threads} threads"}
 
 
 def main():
@@ -102,7 +120,8 @@ def main():
     from libmat_amd import prep
     from libmat_amd.rpd import Context
     t0 = time.perf_counter()
-    inst = prep.config2(0) if a.config == "config2" else prep.config1(0)
+    inst = (prep.cube64_instance(0) if a.workload == "cube64"
+            else prep.cube10_instance(0))
     t_prep = time.perf_counter() - t0
     ctx = Context(0)
     t0 = time.perf_counter()
@@ -119,42 +138,65 @@ def main():
     dt = time.perf_counter() - t0
     st = stats[-1]
     ms_step = 1e3 * dt / a.steps
+    # one more, untimed run that counts the executed FP64 work
+    work = None
+    if not a.no_work_count:
+        work = ctx.run(cells_per_wave=a.cells_per_wave,
+                       count_work=True)["work"]
     n_cells = st["n_cells"]
     value = n_cells * a.steps / dt
     ph = np.mean(np.array([s["ms_phase"] for s in stats]), axis=0)
     kn = np.mean(np.array([s["ms_kernel"] for s in stats]), axis=0)
     clip_ms = float(kn[5])
-    alg = clip_bytes(st)
+    alg = clip_bytes(st, inst)
     achieved = alg / (clip_ms * 1e-3) / 1e9
+    fp64 = None
+    if work is not None:
+        flops = clip_flops(work[0])
+        achieved_fp64 = flops / (clip_ms * 1e-3) / 1e12
+        fp64 = {"achieved": achieved_fp64, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved_fp64 / PEAK_FP64_TFLOPS,
+                "flops": flops, "n_det": work[0][0],
+                "n_bisector": work[0][1], "n_vout": work[0][2],
+                "n_exact_det_all_tiers": sum(w[3] for w in work)}
     traffic = None
     if os.path.exists(a.traffic_json):
         with open(a.traffic_json) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    # host-buffer boundary: upload + one run + download of the cell stream
+    t0 = time.perf_counter()
+    ctx.upload(inst)
+    ctx.run(cells_per_wave=a.cells_per_wave)
+    res = ctx.download()
+    t_pcie = time.perf_counter() - t0
+    del res
     if a.calib_copy_gib > 0:
         ctx.calib_copy(int(a.calib_copy_gib * (1 << 30)))
     out = {
-        "metric": "RPD cells/sec (tets clipped), plain GPU RPD on 1 MI355X",
+        "metric": METRIC,
         "value": value, "unit": "cells/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": (
-            "config2: unit cube 64^3 jittered Kuhn grid, "
+            "unit cube 64^3 jittered Kuhn grid, "
             f"{inst.n_tet} tets; 50,000 spheres, r = 0.9*dist*U[0.5,1], "
-            f"{inst.n_site} non-hidden sites" if a.config == "config2"
-            else "config1: 10^3 Kuhn grid, 500 spheres"),
+            f"{inst.n_site} non-hidden sites" if a.workload == "cube64"
+            else "unit cube 10^3 Kuhn grid, 500 spheres r ~ U[0.02,0.08]"),
             "n_tet": inst.n_tet, "n_site": inst.n_site,
-            "cells_per_wave": a.cells_per_wave},
+            "cells_per_wave": a.cells_per_wave, "path": "plain"},
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": "k_clip<8, Caps<40,24,72>, false>",
                      "kernel_ms": clip_ms, "alg_bytes": alg},
+        "roofline_fp64_valu": fp64,
         "tets_per_s": inst.n_tet * a.steps / dt,
         "fallback_pct": 0.0,
         "exact_rerun_cells": st["n_rerun_exact"],
         "capacity_rerun_cells": st["n_rerun_large"],
         "n_cells": n_cells, "n_candidates": st["n_candidates"],
+        "n_words": st["n_words"], "n_areas": st["n_areas"],
         "active_plane_frac": st["n_active_planes"] / max(1, st["n_all_planes"]),
         "phase_ms": {"candidates": ph[0], "clip": ph[1], "rerun": ph[2],
                      "compaction": ph[3]},
@@ -163,6 +205,10 @@ def main():
                       "plane_masks": kn[4], "clip": kn[5],
                       "compaction": kn[6]},
         "host_prep_s": t_prep, "upload_s": t_upload,
+        "pcie_inclusive": {"value": n_cells / t_pcie, "unit": "cells/s",
+                           "s": t_pcie, "note": "upload of host buffers + "
+                           "one run + download of cells, words, geometry "
+                           "and areas; not the metric"},
     }
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)

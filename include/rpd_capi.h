@@ -75,7 +75,10 @@ typedef struct {
                                arithmetic (test hook; identical output) */
   int32_t small_tier;       /* 1: small-capacity first pass with re-run
                                of overflowing cells at full capacity   */
-  int32_t reserved[3];
+  int32_t count_work;       /* 1: count the arithmetic the clip kernels
+                               execute into rpd_stats.work (adds atomics:
+                               for roofline figures, not for timing)    */
+  int32_t reserved[2];
 } rpd_run_cfg;
 
 /* One emitted (non-empty) cell.  Its payload words start at word_off:
@@ -114,6 +117,12 @@ typedef struct {
   double ms_kernel[8];        /* site boxes, binning, relation count,
                                  relation fill, plane masks, clip (small
                                  tier), compaction                       */
+  int64_t work[3][4];         /* with count_work, per clip tier (small,
+                                 full capacity, exact): FP64 conflict
+                                 determinants evaluated, bisector planes
+                                 built, vertices of the cells the tier
+                                 finished with success, exact-arithmetic
+                                 determinants; otherwise 0               */
 } rpd_stats;
 
 typedef struct rpd_ctx rpd_ctx;
@@ -133,9 +142,9 @@ int rpd_update_sites(rpd_ctx* ctx, int32_t n_site, const double* sites,
                      const uint32_t* flags, const int64_t* knn_off,
                      const int32_t* knn_ids);
 
-/* Run the device pipeline.  rpd_run blocks until done; rpd_run_async only
- * enqueues work on the context stream whose sizes are known on the device
- * (results become readable after rpd_sync). */
+/* Run the device pipeline on the context stream.  rpd_run returns when the
+ * results are complete (it reads sizes back between phases); rpd_sync waits
+ * for everything enqueued on the context stream. */
 int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg);
 int rpd_sync(rpd_ctx* ctx);
 

@@ -84,7 +84,7 @@ struct rpd_ctx {
   Buf bbox, reg_n, reg_off, keys_a, keys_b, bin_start, bin_ids;
   Buf tet_count, tet_off, cand, rel_ids;
   Buf counters;  // int32[8]: fallback, n_rerun_large, n_rerun_exact
-  Buf rerun_large, rerun_exact, loc, sizes, offs, drops, stats64;
+  Buf rerun_large, rerun_exact, loc, sizes, offs, drops, stats64, work;
   Buf scan_tmp;
   TierBufs small, large, exact;
   Buf out_cells, out_words, out_geom, out_areas;
@@ -465,6 +465,12 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   a.n_rerun_large = cnt + 1;
   a.rerun_exact = ctx->rerun_exact.as<int32_t>();
   a.n_rerun_exact = cnt + 2;
+  a.work = nullptr;
+  if (cfg.count_work) {
+    HIP_TRY(ctx->work.need(3 * 64 * 4 * 8));
+    HIP_TRY(hipMemsetAsync(ctx->work.p, 0, 3 * 64 * 4 * 8, s));
+    a.work = ctx->work.as<unsigned long long>();
+  }
   const int64_t n_first = n_cand;
   if (cfg.small_tier) {
     HIP_TRY(ctx->small.need(n_first, SmallCaps::V, SmallCaps::P, SmallCaps::E));
@@ -494,6 +500,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     a.items = ctx->rerun_large.as<int32_t>();
     a.n_items = n_large;
     a.out = ctx->large.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
+    if (a.work) a.work = ctx->work.as<unsigned long long>() + 256;
     if (int rc = launch_clip(ctx, 1, std::min(G, 4), a)) return rc;
     HIP_TRY(hipMemcpyAsync(host_cnt, cnt, 32, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -504,6 +511,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     a.items = ctx->rerun_exact.as<int32_t>();
     a.n_items = n_exact;
     a.out = ctx->exact.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
+    if (a.work) a.work = ctx->work.as<unsigned long long>() + 512;
     if (int rc = launch_clip(ctx, 2, 1, a)) return rc;
   }
   HIP_TRY(hipEventRecord(ctx->ev[3], s));
@@ -597,6 +605,13 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   st.n_rerun_exact = n_exact;
   st.n_active_planes = st64[0];
   st.n_all_planes = st64[1];
+  if (cfg.count_work) {
+    std::vector<uint64_t> w(3 * 64 * 4);
+    HIP_TRY(hipMemcpy(w.data(), ctx->work.p, w.size() * 8,
+                      hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < w.size(); ++q)
+      st.work[q / 256][q % 4] += (int64_t)w[q];
+  }
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[4]));
   st.ms_total = ms;

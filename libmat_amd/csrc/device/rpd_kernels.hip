@@ -544,6 +544,7 @@ struct ClipArgs {
   int32_t* n_rerun_large;
   int32_t* rerun_exact;
   int32_t* n_rerun_exact;
+  unsigned long long* work;  // 64 x 4 spread counters, or nullptr
 };
 
 template <int L>
@@ -668,6 +669,7 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
   const int deg = (int)(a.S.knn_off[s + 1] - k0);
   uint64_t m0 = a.use_mask ? cd.m0 : ~0ull;
   uint64_t m1 = a.use_mask ? cd.m1 : ~0ull;
+  uint32_t n_det = 0, n_bis = 0, n_exd = 0;  // executed work (count_work)
   int k = -1;
   for (;;) {
     // next neighbour position that may cut the cell
@@ -698,6 +700,8 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
     if (m == -1) break;
     const rpd_d4 e = rpd_bisector(S0, ld4(a.S.site + m));
     const int cur = nb_p;
+    n_bis += 1;
+    n_det += nb_v;
 
     // ---- conflict tests, lane-parallel (convex_cell.cu:706-721)
     uint64_t conf_lo = 0, conf_hi = 0;
@@ -712,7 +716,10 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
         const rpd_d4 p2 = ld4(&s_pl[g][(w >> 8) & 255]);
         const rpd_d4 p3 = ld4(&s_pl[g][(w >> 16) & 255]);
         c = a.force_exact ? 0 : conflict_filtered(p1, p2, p3, e);
-        if (kExact && c == 0) c = conflict_exact(p1, p2, p3, e);
+        if (kExact && c == 0) {
+          c = conflict_exact(p1, p2, p3, e);
+          ++n_exd;
+        }
       }
       const uint64_t bc = group_bits<L>(ballot(c > 0), g);
       const uint64_t bu = group_bits<L>(ballot(c == 0), g);
@@ -1054,6 +1061,19 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
     for (int v = j; v < nb_v; v += L) wd[v] = s_ver[g][v];
     for (int p = j; p < nb_p; p += L) wd[nb_v + p] = (uint32_t)s_src[g][p];
     for (int q = j; q < nb_e; q += L) wd[nb_v + nb_p + q] = s_edge[g][q];
+  }
+  if (a.work) {
+    if (kExact) {  // exact determinants are per lane: sum over the group
+#pragma unroll
+      for (int d = 1; d < L; d <<= 1) n_exd += __shfl_xor(n_exd, d, 64);
+    }
+    if (j == 0) {
+      unsigned long long* w = a.work + 4 * (blockIdx.x & 63);
+      atomicAdd(w + 0, (unsigned long long)n_det);
+      atomicAdd(w + 1, (unsigned long long)n_bis);
+      if (status == RPD_SUCCESS) atomicAdd(w + 2, (unsigned long long)nb_v);
+      if (n_exd) atomicAdd(w + 3, (unsigned long long)n_exd);
+    }
   }
   if (j == 0) {
     o.hdr[item] = (uint32_t)nb_v | ((uint32_t)nb_p << 8) |

@@ -141,14 +141,14 @@ def build_instance(verts, tets, xyzr, box=UNIT_BOX, n_threads=0):
                     np.ones(keep.size, np.uint32), knn_off, knn_ids, keep)
 
 
-def config1(seed=0, n_threads=0):
+def cube10_instance(seed=0, n_threads=0):
     """10^3 Kuhn grid (6,000 tets), 500 spheres r ~ U[0.02, 0.08]."""
     verts, tets = grid_mesh(10, 0.0, seed)
     return build_instance(verts, tets, spheres(500, 0, seed, 0.02, 0.08),
                           n_threads=n_threads)
 
 
-def config2(seed=0, n_threads=0, n=64, n_spheres=50000):
+def cube64_instance(seed=0, n_threads=0, n=64, n_spheres=50000):
     """64^3 jittered grid (1,572,864 tets), 50,000 spheres, boundary rule."""
     verts, tets = grid_mesh(n, 0.1, seed)
     return build_instance(verts, tets, spheres(n_spheres, 1, seed),

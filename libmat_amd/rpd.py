@@ -40,7 +40,8 @@ class RunCfg(ctypes.Structure):
                 ("compute_geometry", ctypes.c_int32),
                 ("force_exact", ctypes.c_int32),
                 ("small_tier", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 3)]
+                ("count_work", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
 
 
 class Stats(ctypes.Structure):
@@ -56,13 +57,16 @@ class Stats(ctypes.Structure):
                 ("tet_k", ctypes.c_int32), ("site_k", ctypes.c_int32),
                 ("ms_total", ctypes.c_double),
                 ("ms_phase", ctypes.c_double * 8),
-                ("ms_kernel", ctypes.c_double * 8)]
+                ("ms_kernel", ctypes.c_double * 8),
+                ("work", (ctypes.c_int64 * 4) * 3)]
 
     def as_dict(self):
         d = {}
         for name, _ in self._fields_:
             v = getattr(self, name)
-            d[name] = list(v) if hasattr(v, "__len__") else v
+            if hasattr(v, "__len__"):
+                v = [list(x) if hasattr(x, "__len__") else x for x in v]
+            d[name] = v
         return d
 
 
@@ -192,9 +196,9 @@ class Context:
             _p(knn_ids)), "rpd_update_sites")
 
     def run(self, cells_per_wave=8, use_active_mask=True, geometry=True,
-            force_exact=False, small_tier=True):
+            force_exact=False, small_tier=True, count_work=False):
         cfg = RunCfg(cells_per_wave, int(use_active_mask), int(geometry),
-                     int(force_exact), int(small_tier))
+                     int(force_exact), int(small_tier), int(count_work))
         self._check(self._L.rpd_run(self._h, ctypes.byref(cfg)), "rpd_run")
         return self.stats()
 

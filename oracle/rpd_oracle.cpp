@@ -35,6 +35,22 @@
 // its vertex cycle (voronoi_defs.cxx:108-188 logic), facet areas by Newell's
 // formula, volume and centroid by a fan of tetrahedra from the first
 // vertex.
+//
+// What pins each part (tests/test_oracle_pins.py):
+//  * exact_det4_sign: rational arithmetic (fractions.Fraction) on random,
+//    rank-deficient and near-degenerate matrices; the reference's own exact
+//    expansion determinant (src/CPU/Delaunay_psm.cpp:9539) built into
+//    oracle/_ref from the mounted sources;
+//  * planes, bisectors, related(): closed forms (one sphere owns every tet;
+//    two spheres split at their power plane with analytic volumes; eight
+//    octant spheres whose bisectors lie on grid planes) and brute force
+//    (clipping by every sphere equals clipping by the neighbour lists);
+//  * the clip and cell_geometry: the partition of every tet (volumes sum to
+//    the tet volume), and the reference's own FP64 VBW::ConvexCell
+//    (src/CPU/convex_cell.cpp) clipping the same planes;
+//  * mode 1 (reference float arithmetic): parity unpinned -- the binary32
+//    bits of the CUDA reference cannot be produced here; it only serves as
+//    the CPU baseline and to count disagreements with mode 0.
 #include <stdint.h>
 
 #include <algorithm>

@@ -1,4 +1,4 @@
-"""Writes the config-1 golden fixtures (seeds 0-3) from the oracle.
+"""Writes the 10^3-grid golden fixtures (seeds 0-3) from the oracle.
 
 The reference ships no tests, data or golden vectors, so these fixtures are
 produced by this repository's oracle (itself pinned by the closed-form,
@@ -33,7 +33,7 @@ def main():
     from libmat_amd import prep
     from rpd_compare import digest_oracle
     for seed in range(4):
-        inst = prep.config1(seed)
+        inst = prep.cube10_instance(seed)
         orc = oracle.run(inst, mode=0)
         c = orc.cells
         sub = c[c["tet"] < 300]
@@ -41,7 +41,7 @@ def main():
         src = np.concatenate([x["src"][:x["nb_p"]] for x in sub])
         edge = np.concatenate([x["edge"][:x["nb_e"]] for x in sub])
         np.savez_compressed(
-            os.path.join(HERE, f"config1_seed{seed}.npz"),
+            os.path.join(HERE, f"cube10_seed{seed}.npz"),
             input_digest=np.array(input_digest(inst)),
             cells_digest=np.array(digest_oracle(c)),
             n_cells=np.array(c.size),

@@ -18,15 +18,15 @@ _cache = {}
 
 def c1(seed):
     if seed not in _cache:
-        inst = prep.config1(seed)
+        inst = prep.cube10_instance(seed)
         _cache[seed] = (inst, oracle.run(inst, mode=0))
     return _cache[seed]
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2, 3])
-def test_config1_parity(ctx, seed):
+def test_cube10_parity(ctx, seed):
     inst, orc = c1(seed)
-    print(f"[parity] config1 seed={seed} n_tet={inst.n_tet} "
+    print(f"[parity] cube10 seed={seed} n_tet={inst.n_tet} "
           f"n_site={inst.n_site}", flush=True)
     ctx.upload(inst)
     st = ctx.run()
@@ -46,7 +46,7 @@ def test_config1_parity(ctx, seed):
     dict(cells_per_wave=1), dict(cells_per_wave=2), dict(cells_per_wave=4),
     dict(use_active_mask=False), dict(small_tier=False),
     dict(force_exact=True)])
-def test_config1_variants_identical(ctx, cfg):
+def test_cube10_variants_identical(ctx, cfg):
     inst, orc = c1(0)
     print(f"[variant] {cfg}", flush=True)
     ctx.upload(inst)
@@ -69,12 +69,12 @@ def test_determinism(ctx):
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2, 3])
-def test_config1_matches_golden_fixture(ctx, seed):
+def test_cube10_matches_golden_fixture(ctx, seed):
     import os
     from rpd_compare import digest_gpu
     g = np.load(os.path.join(os.path.dirname(__file__), "golden",
-                             f"config1_seed{seed}.npz"))
-    inst = prep.config1(seed)
+                             f"cube10_seed{seed}.npz"))
+    inst = prep.cube10_instance(seed)
     ctx.upload(inst)
     ctx.run()
     off, ids = ctx.relation()
@@ -101,11 +101,11 @@ def test_octant_spheres_exact_path_on_gpu(ctx):
     compare_with_oracle(ctx.download(), orc)
 
 
-def test_config2_sample_and_partition(ctx):
+def test_cube64_sample_and_partition(ctx):
     from libmat_amd.rpd import RPDResult
     from rpd_compare import tet_volumes
-    inst = prep.config2(0, n_threads=16)
-    print(f"[config2] n_tet={inst.n_tet} n_site={inst.n_site}", flush=True)
+    inst = prep.cube64_instance(0, n_threads=16)
+    print(f"[cube64] n_tet={inst.n_tet} n_site={inst.n_site}", flush=True)
     ctx.upload(inst)
     st = ctx.run()
     res = ctx.download()
@@ -114,7 +114,7 @@ def test_config2_sample_and_partition(ctx):
     np.add.at(vol, res.cells["tet"], res.geom[:, 0])
     tv = tet_volumes(inst)
     err = np.max(np.abs(vol - tv) / tv)
-    print(f"[config2] cells={st['n_cells']} partition max rel err {err:.2e}",
+    print(f"[cube64] cells={st['n_cells']} partition max rel err {err:.2e}",
           flush=True)
     assert err < 1e-10
     sample = np.sort(np.random.default_rng(7).choice(
@@ -127,7 +127,7 @@ def test_config2_sample_and_partition(ctx):
     sub = RPDResult(res.cells[idx], res.words, res.geom[idx].reshape(-1),
                     res.areas)
     m = compare_with_oracle(sub, orc)
-    print(f"[config2] sample {m}", flush=True)
+    print(f"[cube64] sample {m}", flush=True)
 
 
 def test_cpp_dropin_on_gpu(tmp_path):

@@ -170,7 +170,7 @@ def test_knn_clipping_equals_brute_force_clipping():
 
 @pytest.mark.parametrize("seed", [0, 1])
 def test_partition_of_every_tet(seed):
-    inst = prep.config1(seed)
+    inst = prep.cube10_instance(seed)
     orc = oracle.run(inst)
     vol = np.zeros(inst.n_tet)
     np.add.at(vol, orc.cells["tet"], orc.cells["geo"][:, 0])
@@ -187,8 +187,8 @@ def test_partition_of_every_tet(seed):
 def test_golden_fixtures_seed0_and_1():
     for seed in (0, 1):
         g = np.load(os.path.join(ROOT, "tests/golden",
-                                 f"config1_seed{seed}.npz"))
-        inst = prep.config1(seed)
+                                 f"cube10_seed{seed}.npz"))
+        inst = prep.cube10_instance(seed)
         orc = oracle.run(inst)
         assert np.array_equal(orc.rel_off, g["rel_off"])
         assert np.array_equal(orc.rel_ids, g["rel_ids"])
@@ -199,7 +199,7 @@ def test_golden_fixtures_seed0_and_1():
 
 
 def test_reference_float_arithmetic_agrees_with_exact():
-    inst = prep.config1(0)
+    inst = prep.cube10_instance(0)
     x = oracle.run(inst, mode=0)
     r = oracle.run(inst, mode=1)
     kx = set(zip(x.cells["tet"], x.cells["seed"]))
