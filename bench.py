@@ -140,9 +140,11 @@ def main():
     ms_step = 1e3 * dt / a.steps
     # one more, untimed run that counts the executed FP64 work
     work = None
+    active_frac = None
     if not a.no_work_count:
-        work = ctx.run(cells_per_wave=a.cells_per_wave,
-                       count_work=True)["work"]
+        stw = ctx.run(cells_per_wave=a.cells_per_wave, count_work=True)
+        work = stw["work"]
+        active_frac = stw["n_active_planes"] / max(1, stw["n_all_planes"])
     n_cells = st["n_cells"]
     value = n_cells * a.steps / dt
     ph = np.mean(np.array([s["ms_phase"] for s in stats]), axis=0)
@@ -197,7 +199,7 @@ def main():
         "capacity_rerun_cells": st["n_rerun_large"],
         "n_cells": n_cells, "n_candidates": st["n_candidates"],
         "n_words": st["n_words"], "n_areas": st["n_areas"],
-        "active_plane_frac": st["n_active_planes"] / max(1, st["n_all_planes"]),
+        "active_plane_frac": active_frac,
         "phase_ms": {"candidates": ph[0], "clip": ph[1], "rerun": ph[2],
                      "compaction": ph[3]},
         "kernel_ms": {"site_boxes": kn[0], "binning": kn[1],

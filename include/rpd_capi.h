@@ -105,8 +105,10 @@ typedef struct {
   int64_t n_dropped[9];       /* candidates dropped, per rpd_status      */
   int64_t n_rerun_large;      /* cells re-run at full capacity           */
   int64_t n_rerun_exact;      /* cells re-run with exact arithmetic      */
-  int64_t n_active_planes;    /* neighbour planes not proved inactive    */
-  int64_t n_all_planes;       /* all neighbour planes of all candidates  */
+  int64_t n_active_planes;    /* neighbour planes not proved inactive
+                                 (with count_work, else 0)               */
+  int64_t n_all_planes;       /* all neighbour planes of all candidates
+                                 (with count_work, else 0)               */
   int64_t n_bbox_fallback;    /* spheres whose candidate box fell back to
                                  the whole domain                        */
   int64_t n_bin_registrations; /* (sphere, bin) pairs of the candidate grid */

@@ -82,7 +82,7 @@ struct rpd_ctx {
   Buf site, flags, knn_off, knn_ids;
   // pipeline
   Buf bbox, reg_n, reg_off, keys_a, keys_b, bin_start, bin_ids;
-  Buf tet_count, tet_off, cand, rel_ids;
+  Buf tet_count, tet_off, tet_pass, cand, rel_ids;
   Buf counters;  // int32[8]: fallback, n_rerun_large, n_rerun_exact
   Buf rerun_large, rerun_exact, loc, sizes, offs, drops, stats64, work;
   Buf scan_tmp;
@@ -423,11 +423,13 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     HIP_TRY(hipEventRecord(ctx->ev[11], s));
     HIP_TRY(ctx->tet_count.need(((size_t)nt + 1) * 8));
     HIP_TRY(ctx->tet_off.need(((size_t)nt + 1) * 8));
+    HIP_TRY(ctx->tet_pass.need(((size_t)nt + 1) * 8));
     HIP_TRY(hipMemsetAsync(ctx->tet_count.as<int64_t>() + nt, 0, 8, s));
     hipLaunchKernelGGL(k_tet_count, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
                        M, S, bg, ctx->bin_start.as<int32_t>(),
                        ctx->bin_ids.as<int32_t>(),
-                       ctx->tet_count.as<int64_t>());
+                       ctx->tet_count.as<int64_t>(),
+                       ctx->tet_pass.as<uint64_t>());
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[12], s));
     HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
@@ -442,7 +444,8 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   hipLaunchKernelGGL(k_tet_fill, dim3(ceil_div(nt, 256)), dim3(256), 0, s, M,
                      S, bg, ctx->bin_start.as<int32_t>(),
                      ctx->bin_ids.as<int32_t>(), ctx->tet_off.as<int64_t>(),
-                     ctx->cand.as<Cand>(), ctx->rel_ids.as<int32_t>());
+                     ctx->tet_pass.as<uint64_t>(), ctx->cand.as<Cand>(),
+                     ctx->rel_ids.as<int32_t>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->ev[13], s));
   if (cfg.use_active_mask && n_cand > 0)
@@ -574,7 +577,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   // ---- statistics (outside the timed phases)
   HIP_TRY(ctx->stats64.need(4 * 8));
   HIP_TRY(hipMemsetAsync(ctx->stats64.p, 0, 4 * 8, s));
-  if (n_cand > 0)
+  if (n_cand > 0 && cfg.count_work)
     hipLaunchKernelGGL(k_mask_stats, dim3(ceil_div(n_cand, 256)), dim3(256), 0,
                        s, n_cand, ctx->cand.as<Cand>(), S,
                        ctx->stats64.as<unsigned long long>());

@@ -451,31 +451,42 @@ __global__ __launch_bounds__(256) void k_tet_count(DevMesh M, DevSites S,
                                                    BinGrid g,
                                                    const int32_t* bin_start,
                                                    const int32_t* bin_ids,
-                                                   int64_t* tet_count) {
+                                                   int64_t* tet_count,
+                                                   uint64_t* tet_pass) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= M.n_tet) return;
   TetLoad L;
   load_tet(M, t, L);
   const int64_t b = tet_bin(L, g);
   int64_t n = 0;
-  for (int q = bin_start[b]; q < bin_start[b + 1]; ++q)
-    n += relation(S, L, bin_ids[q]);
+  uint64_t pass = 0;  // bit r: the r-th site of the bin list relates
+  const int q0 = bin_start[b];
+  for (int q = q0; q < bin_start[b + 1]; ++q) {
+    const bool r = relation(S, L, bin_ids[q]);
+    n += r;
+    if (r && q - q0 < 64) pass |= 1ull << (q - q0);
+  }
   tet_count[t] = n;
+  tet_pass[t] = pass;
 }
 
 __global__ __launch_bounds__(256) void k_tet_fill(
     DevMesh M, DevSites S, BinGrid g, const int32_t* bin_start,
-    const int32_t* bin_ids, const int64_t* tet_off, Cand* cand,
-    int32_t* rel_ids) {
+    const int32_t* bin_ids, const int64_t* tet_off,
+    const uint64_t* tet_pass, Cand* cand, int32_t* rel_ids) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= M.n_tet) return;
   TetLoad L;
   load_tet(M, t, L);
   const int64_t b = tet_bin(L, g);
   int64_t o = tet_off[t];
-  for (int q = bin_start[b]; q < bin_start[b + 1]; ++q) {
+  const uint64_t pass = tet_pass[t];
+  const int q0 = bin_start[b];
+  for (int q = q0; q < bin_start[b + 1]; ++q) {
     const int i = bin_ids[q];
-    if (!relation(S, L, i)) continue;
+    // the first 64 outcomes come from the counting pass
+    if (q - q0 < 64 ? !((pass >> (q - q0)) & 1) : !relation(S, L, i))
+      continue;
     Cand c;
     c.tet = t;
     c.seed = i;
@@ -554,6 +565,15 @@ __device__ __forceinline__ uint64_t group_bits(uint64_t b, int g) {
 }
 
 // ---- FP64 geometry helpers (tolerance-checked, not bit-compared)
+// 1/x: hardware reciprocal estimate refined by two Newton steps (relative
+// error ~1 ulp for the normal, finite x used here)
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
 // foot of C on plane p, given 1/|n_p|^2
 __device__ __forceinline__ void foot_plane(const rpd_d4& p, double inv_nn,
                                            const double C[3], double F[3]) {
@@ -571,7 +591,8 @@ __device__ __forceinline__ void foot_line(const rpd_d4& p, const rpd_d4& q,
   const double d2 = p.x * q.y - p.y * q.x;
   const double dd = d0 * d0 + d1 * d1 + d2 * d2;
   const double s =
-      ((C[0] - A[0]) * d0 + (C[1] - A[1]) * d1 + (C[2] - A[2]) * d2) / dd;
+      ((C[0] - A[0]) * d0 + (C[1] - A[1]) * d1 + (C[2] - A[2]) * d2) *
+      rcp64(dd);
   E[0] = A[0] + s * d0;
   E[1] = A[1] + s * d1;
   E[2] = A[2] + s * d2;
@@ -603,7 +624,7 @@ __device__ __forceinline__ void vertex_pos(const rpd_d4& a, const rpd_d4& b,
                                            const rpd_d4& c, double A[3]) {
   rpd_d4 X, Q;
   rpd_vertex_homog(a, b, c, X, Q);
-  const double r = 1.0 / X.w;
+  const double r = rcp64(X.w);
   A[0] = X.x * r;
   A[1] = X.y * r;
   A[2] = X.z * r;
@@ -971,47 +992,82 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
   // ---- write the slot
   const SlotView& o = a.out;
   if (status == RPD_SUCCESS && a.geometry) {
-    // FP64 geometry by a vertex-local decomposition: 6 tets per vertex A
-    // between the reference point C, A, the feet of C on A's 3 planes and
-    // on its 3 edge lines (the decomposition of the reference's
-    // get_tet_decomposition_of_vertex, convex_cell.cu:986-1006); C is the
-    // cell's vertex 0 so rounding errors scale with the cell
+    // FP64 geometry in one pass over the vertices.  Around each vertex A
+    // lie 6 tets (C, X, Y, A) whose bases (X, Y, A) tile A's corner of its
+    // three faces: X, Y run over the feet of C on A's 3 planes and on its 3
+    // edge lines (the decomposition of the reference's
+    // get_tet_decomposition_of_vertex, convex_cell.cu:986-1006).  Their
+    // volumes give the cell volume and centroid, their bases the facet
+    // areas (signed along each plane normal, so feet outside a face
+    // cancel).  C is the cell's vertex 0, so rounding scales with the cell.
     double Cc[3];
     {
       const uint32_t w0 = s_ver[g][0];
       vertex_pos(ld4(&s_pl[g][w0 & 255]), ld4(&s_pl[g][(w0 >> 8) & 255]),
                  ld4(&s_pl[g][(w0 >> 16) & 255]), Cc);
     }
+    constexpr int PA = (P + L - 1) / L;  // planes per lane
+    double acc[PA];
+#pragma unroll
+    for (int q = 0; q < PA; ++q) acc[q] = 0.0;
     double vol6 = 0, mx = 0, my = 0, mz = 0;
-    for (int v = j; v < nb_v; v += L) {
-      const uint32_t w = s_ver[g][v];
-      const rpd_d4 pa = ld4(&s_pl[g][w & 255]);
-      const rpd_d4 pb = ld4(&s_pl[g][(w >> 8) & 255]);
-      const rpd_d4 pc = ld4(&s_pl[g][(w >> 16) & 255]);
-      double A[3];
-      vertex_pos(pa, pb, pc, A);
-      double F0[3], E0[3], F1[3], E1[3], F2[3], E2[3];
-      foot_plane(pa, 1.0 / (pa.x * pa.x + pa.y * pa.y + pa.z * pa.z), Cc, F0);
-      foot_line(pa, pb, A, Cc, E0);
-      foot_plane(pb, 1.0 / (pb.x * pb.x + pb.y * pb.y + pb.z * pb.z), Cc, F1);
-      foot_line(pb, pc, A, Cc, E1);
-      foot_plane(pc, 1.0 / (pc.x * pc.x + pc.y * pc.y + pc.z * pc.z), Cc, F2);
-      foot_line(pc, pa, A, Cc, E2);
-#define RPD_ACC(P0, P1)                                          \
+    for (int q0 = 0; q0 < nb_v; q0 += L) {
+      const int v = q0 + j;
+      uint32_t ia = kEnd, ib = kEnd, ic = kEnd;
+      double ca = 0, cb = 0, cc = 0;
+      if (v < nb_v) {
+        const uint32_t w = s_ver[g][v];
+        ia = w & 255;
+        ib = (w >> 8) & 255;
+        ic = (w >> 16) & 255;
+        const rpd_d4 pa = ld4(&s_pl[g][ia]);
+        const rpd_d4 pb = ld4(&s_pl[g][ib]);
+        const rpd_d4 pc = ld4(&s_pl[g][ic]);
+        double A[3];
+        vertex_pos(pa, pb, pc, A);
+        // walk the six (foot, foot) pairs around A so that each foot dies
+        // after its two tets and base triangles
+        double Fa[3], Xp[3], Xn[3];
+        foot_plane(pa, rcp64(pa.x * pa.x + pa.y * pa.y + pa.z * pa.z), Cc, Fa);
+#define RPD_ACC(P0, P1, PL, CONTRIB)                             \
   {                                                              \
     const double tvl = tet_vol6(P0, P1, Cc, A);                  \
     vol6 += tvl;                                                 \
     mx += tvl * (((P0[0] + P1[0]) + Cc[0]) + A[0]);              \
     my += tvl * (((P0[1] + P1[1]) + Cc[1]) + A[1]);              \
     mz += tvl * (((P0[2] + P1[2]) + Cc[2]) + A[2]);              \
+    CONTRIB += tri_area2(P0, P1, A, PL);                         \
   }
-      RPD_ACC(F0, E0)
-      RPD_ACC(E0, F1)
-      RPD_ACC(F1, E1)
-      RPD_ACC(E1, F2)
-      RPD_ACC(F2, E2)
-      RPD_ACC(E2, F0)
+        foot_line(pa, pb, A, Cc, Xp);                               // Eab
+        RPD_ACC(Fa, Xp, pa, ca)
+        foot_plane(pb, rcp64(pb.x * pb.x + pb.y * pb.y + pb.z * pb.z), Cc, Xn);
+        RPD_ACC(Xp, Xn, pb, cb)                                     // Eab, Fb
+        foot_line(pb, pc, A, Cc, Xp);                               // Ebc
+        RPD_ACC(Xn, Xp, pb, cb)                                     // Fb, Ebc
+        foot_plane(pc, rcp64(pc.x * pc.x + pc.y * pc.y + pc.z * pc.z), Cc, Xn);
+        RPD_ACC(Xp, Xn, pc, cc)                                     // Ebc, Fc
+        foot_line(pc, pa, A, Cc, Xp);                               // Eca
+        RPD_ACC(Xn, Xp, pc, cc)                                     // Fc, Eca
+        RPD_ACC(Xp, Fa, pa, ca)                                     // Eca, Fa
 #undef RPD_ACC
+      }
+      // per-plane sums over the group's lanes (fixed butterfly order);
+      // lane p % L keeps plane p
+#pragma unroll
+      for (int pi = 0; pi < PA; ++pi) {
+#pragma unroll
+        for (int pj = 0; pj < L; ++pj) {
+          const uint32_t p = (uint32_t)(pi * L + pj);
+          if (p < (uint32_t)P && p < (uint32_t)nb_p) {
+            double sp = (ia == p ? ca : 0.0) + (ib == p ? cb : 0.0) +
+                        (ic == p ? cc : 0.0);
+#pragma unroll
+            for (int off = L / 2; off > 0; off >>= 1)
+              sp += __shfl_xor(sp, off);
+            if (j == pj) acc[pi] += sp;
+          }
+        }
+      }
     }
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) {
@@ -1029,31 +1085,15 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
       gm[2] = my * inv;
       gm[3] = mz * inv;
     }
-    // facet areas: lane per plane, deterministic serial sum over vertices
-    for (int p = j; p < nb_p; p += L) {
-      const rpd_d4 pp = ld4(&s_pl[g][p]);
-      const double nn = pp.x * pp.x + pp.y * pp.y + pp.z * pp.z;
-      double F[3];
-      foot_plane(pp, 1.0 / nn, Cc, F);
-      double area2 = 0;
-      for (int v = 0; v < nb_v; ++v) {
-        const uint32_t w = s_ver[g][v];
-        const uint32_t t0 = w & 255, t1 = (w >> 8) & 255, t2 = (w >> 16) & 255;
-        const uint32_t up = (uint32_t)p;
-        if (t0 != up && t1 != up && t2 != up) continue;
-        // next / previous plane of p in the dual triangle (t0, t1, t2)
-        const uint32_t nx = t0 == up ? t1 : (t1 == up ? t2 : t0);
-        const uint32_t pv = t0 == up ? t2 : (t1 == up ? t0 : t1);
-        const rpd_d4 pn = ld4(&s_pl[g][nx]);
-        const rpd_d4 pq = ld4(&s_pl[g][pv]);
-        double A[3], En[3], Eq[3];
-        vertex_pos(ld4(&s_pl[g][t0]), ld4(&s_pl[g][t1]), ld4(&s_pl[g][t2]), A);
-        foot_line(pp, pn, A, Cc, En);
-        foot_line(pq, pp, A, Cc, Eq);
-        area2 += tri_area2(F, En, A, pp) + tri_area2(Eq, F, A, pp);
+#pragma unroll
+    for (int pi = 0; pi < PA; ++pi) {
+      const int p = pi * L + j;
+      if (p < nb_p) {
+        const rpd_d4 pp = ld4(&s_pl[g][p]);
+        const double nn = pp.x * pp.x + pp.y * pp.y + pp.z * pp.z;
+        o.areas[(int64_t)item * o.stride_areas + p] =
+            fabs(acc[pi]) * 0.5 / sqrt(nn);
       }
-      o.areas[(int64_t)item * o.stride_areas + p] =
-          fabs(area2) * 0.5 / sqrt(nn);
     }
   }
   if (status == RPD_SUCCESS) {

@@ -145,3 +145,25 @@ def test_cpp_dropin_on_gpu(tmp_path):
                          timeout=120)
     print(out.stdout, out.stderr, flush=True)
     assert out.returncode == 0 and "gpu ok" in out.stdout
+
+
+def test_work_counters_match_the_oracle(ctx):
+    # without plane masks and with every cell in the full-capacity tier the
+    # kernel evaluates exactly the oracle's conflict determinants, bisectors
+    # and output vertices; masks only remove determinants
+    inst = prep.cube10_instance(0)
+    orc = oracle.run(inst)
+    n_det, n_bis, _, n_vout = (int(x) for x in orc.counters)
+    ctx.upload(inst)
+    st = ctx.run(use_active_mask=False, small_tier=False, count_work=True)
+    w = st["work"]
+    print(f"[work] gpu {w} oracle det={n_det} bis={n_bis} vout={n_vout}",
+          flush=True)
+    assert w[0] == [0, 0, 0, 0] and st["n_rerun_exact"] == 0
+    assert w[1][0] == n_det and w[1][1] == n_bis and w[1][2] == n_vout
+    st2 = ctx.run(count_work=True)
+    w2 = st2["work"]
+    assert sum(x[2] for x in w2) == n_vout
+    assert w2[0][0] + w2[1][0] < n_det
+    assert 0 < st2["n_active_planes"] < st2["n_all_planes"]
+    compare_with_oracle(ctx.download(), orc)
