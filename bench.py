@@ -190,7 +190,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": "k_clip<8, Caps<40,24,72>, false>",
+                     "kernel": f"k_clip<{a.cells_per_wave}, Caps<32,16,48>, false>",
                      "kernel_ms": clip_ms, "alg_bytes": alg},
         "roofline_fp64_valu": fp64,
         "tets_per_s": inst.n_tet * a.steps / dt,

@@ -67,7 +67,7 @@ typedef struct {
 
 typedef struct {
   int32_t cells_per_wave;   /* cells clipped concurrently per wavefront:
-                               1, 2, 4 or 8 (0 = default)              */
+                               1, 2, 4, 8 or 16 (0 = default)          */
   int32_t use_active_mask;  /* 1: skip neighbour planes proved not to cut
                                the cell's tet (results are identical)  */
   int32_t compute_geometry; /* 1: volume, centroid, facet areas (FP64) */
@@ -125,6 +125,9 @@ typedef struct {
                                  built, vertices of the cells the tier
                                  finished with success, exact-arithmetic
                                  determinants; otherwise 0               */
+  int64_t relation_work[4];   /* with count_work: (tet, site) pairs of the
+                                 candidate bin lists tested, neighbour
+                                 tests of the relation, 0, related pairs */
 } rpd_stats;
 
 typedef struct rpd_ctx rpd_ctx;

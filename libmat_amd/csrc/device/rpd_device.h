@@ -38,7 +38,9 @@ struct DevSites {
   const double4* site;   // x, y, z, w = r^2
   const uint32_t* flags;
   const int64_t* knn_off;
-  const int32_t* knn_ids;
+  const int32_t* knn_ids;   // neighbours in ascending site id (clip order)
+  const int32_t* knn_near;  // the same lists, nearest bisector first
+  const double4* knn_site_near;  // the spheres of knn_near, inline
 };
 
 // capacities of a clip tier
@@ -46,7 +48,7 @@ template <int V_, int P_, int E_>
 struct Caps {
   static constexpr int V = V_, P = P_, E = E_;
 };
-using SmallCaps = Caps<40, 24, 72>;
+using SmallCaps = Caps<32, 16, 48>;
 using LargeCaps = Caps<96, 64, 152>;
 
 // Output slot of one cell (indexed by cell or by re-run index).

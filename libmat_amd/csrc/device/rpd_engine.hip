@@ -79,12 +79,13 @@ struct rpd_ctx {
   double rho = 0;
   bool has_mesh = false, has_sites = false;
   Buf vert, tet_v, tet_fid, tet_meta, dom_dev;
-  Buf site, flags, knn_off, knn_ids;
+  Buf site, flags, knn_off, knn_ids, knn_near, knn_site_near;
   // pipeline
   Buf bbox, reg_n, reg_off, keys_a, keys_b, bin_start, bin_ids;
   Buf tet_count, tet_off, tet_pass, cand, rel_ids;
   Buf counters;  // int32[8]: fallback, n_rerun_large, n_rerun_exact
-  Buf rerun_large, rerun_exact, loc, sizes, offs, drops, stats64, work;
+  Buf rerun_large, rerun_exact, loc, sizes, offs, drops, stats64, work,
+      rel_work;
   Buf scan_tmp;
   TierBufs small, large, exact;
   Buf out_cells, out_words, out_geom, out_areas;
@@ -187,6 +188,8 @@ int rpd_update_sites(rpd_ctx* ctx, int32_t n_site, const double* sites,
   HIP_TRY(ctx->flags.need(std::max<int64_t>(n_site, 1) * 4));
   HIP_TRY(ctx->knn_off.need((n_site + 1) * 8));
   HIP_TRY(ctx->knn_ids.need(std::max<int64_t>(n_knn, 1) * 4));
+  HIP_TRY(ctx->knn_near.need(std::max<int64_t>(n_knn, 1) * 4));
+  HIP_TRY(ctx->knn_site_near.need(std::max<int64_t>(n_knn, 1) * 32));
   if (n_site) {
     HIP_TRY(hipMemcpyAsync(ctx->site.p, sites, (size_t)n_site * 32,
                            hipMemcpyHostToDevice, ctx->stream));
@@ -300,7 +303,8 @@ static int launch_clip(rpd_ctx* ctx, int tier, int G, ClipArgs& a) {
       case 1: RPD_LAUNCH(1, SmallCaps, false); break;
       case 2: RPD_LAUNCH(2, SmallCaps, false); break;
       case 4: RPD_LAUNCH(4, SmallCaps, false); break;
-      default: RPD_LAUNCH(8, SmallCaps, false); break;
+      case 8: RPD_LAUNCH(8, SmallCaps, false); break;
+      default: RPD_LAUNCH(16, SmallCaps, false); break;
     }
   } else if (tier == 1) {
     switch (G) {
@@ -332,8 +336,8 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     if (cfg.cells_per_wave <= 0) cfg.cells_per_wave = 8;
   }
   const int G = cfg.cells_per_wave;
-  if (G != 1 && G != 2 && G != 4 && G != 8)
-    return fail(ctx, "rpd_run: cells_per_wave must be 1, 2, 4 or 8");
+  if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16)
+    return fail(ctx, "rpd_run: cells_per_wave must be 1, 2, 4, 8 or 16");
   hipStream_t s = ctx->stream;
   const int32_t nt = ctx->n_tet, ns = ctx->n_site;
   const int32_t site_k = ctx->st.site_k;
@@ -343,12 +347,18 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   DevMesh M{ctx->n_vert, nt, ctx->vert.as<double4>(), ctx->tet_v.as<int4>(),
             ctx->tet_fid.as<int4>(), ctx->tet_meta.as<TetMeta>()};
   DevSites S{ns, ctx->site.as<double4>(), ctx->flags.as<uint32_t>(),
-             ctx->knn_off.as<int64_t>(), ctx->knn_ids.as<int32_t>()};
+             ctx->knn_off.as<int64_t>(), ctx->knn_ids.as<int32_t>(),
+             ctx->knn_near.as<int32_t>(), ctx->knn_site_near.as<double4>()};
 
   HIP_TRY(hipEventRecord(ctx->ev[0], s));
   HIP_TRY(ctx->counters.need(8 * 4));
   HIP_TRY(hipMemsetAsync(ctx->counters.p, 0, 8 * 4, s));
   int32_t* cnt = ctx->counters.as<int32_t>();
+
+  if (cfg.count_work) {
+    HIP_TRY(ctx->rel_work.need(64 * 4 * 8));
+    HIP_TRY(hipMemsetAsync(ctx->rel_work.p, 0, 64 * 4 * 8, s));
+  }
 
   // ---- 1. candidate gathering
   BinGrid bg;
@@ -372,7 +382,8 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     if (ns > 0)
       hipLaunchKernelGGL(k_site_bbox, dim3(ns), dim3(64), 0, s, S,
                          ctx->dom_dev.as<double>(), ctx->rho,
-                         ctx->bbox.as<double>(), cnt + 0);
+                         ctx->bbox.as<double>(), ctx->knn_near.as<int32_t>(),
+                         ctx->knn_site_near.as<double4>(), cnt + 0);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[10], s));
     // sphere -> bin registrations: count, expand, radix sort by
@@ -429,7 +440,9 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
                        M, S, bg, ctx->bin_start.as<int32_t>(),
                        ctx->bin_ids.as<int32_t>(),
                        ctx->tet_count.as<int64_t>(),
-                       ctx->tet_pass.as<uint64_t>());
+                       ctx->tet_pass.as<uint64_t>(),
+                       cfg.count_work ? ctx->rel_work.as<unsigned long long>()
+                                      : nullptr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[12], s));
     HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
@@ -614,6 +627,12 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
                       hipMemcpyDeviceToHost));
     for (size_t q = 0; q < w.size(); ++q)
       st.work[q / 256][q % 4] += (int64_t)w[q];
+    std::vector<uint64_t> rw(64 * 4);
+    HIP_TRY(hipMemcpy(rw.data(), ctx->rel_work.p, rw.size() * 8,
+                      hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < rw.size(); ++q)
+      st.relation_work[q % 4] += (int64_t)rw[q];
+    st.relation_work[3] = n_cand;
   }
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[4]));

@@ -146,18 +146,28 @@ __device__ __forceinline__ int bin_coord(double x, double org, double inv,
 // One wavefront per sphere.  Polytope = domain box clipped by the
 // neighbour bisectors, each pushed out by rho (max tet circum-radius about
 // the centroid plus a margin).  Output bbox (lo, hi); empty if lo > hi.
-constexpr int kBoxV = 256;
-constexpr int kBoxP = 262;
+// The bisectors are applied nearest first (ascending signed distance of
+// the sphere centre to the bisector), so the polytope shrinks early and
+// later planes rarely cut; that order is also written to knn_near, where
+// the relation test uses it to meet a rejecting neighbour early (the test
+// itself does not depend on the order).
+constexpr int kBoxV = 128;      // vertices (overflow: whole-domain box)
+constexpr int kBoxP = 128;      // 6 box planes + 122 neighbours
+constexpr int kNearMax = 128;   // longest list ranked (else kept as is)
 
 __global__ __launch_bounds__(64) void k_site_bbox(DevSites S,
                                                   const double* dom,  // 6
                                                   double rho,
                                                   double* bbox,  // 6/site
+                                                  int32_t* knn_near,
+                                                  double4* knn_site_near,
                                                   int32_t* n_fallback) {
   __shared__ double4 s_pl[kBoxP];
   __shared__ double4 s_X[2][kBoxV];
   __shared__ uint32_t s_t[2][kBoxV];
   __shared__ uint8_t s_out[kBoxV];
+  __shared__ double s_key[kNearMax];
+  __shared__ int32_t s_near[kNearMax];
   __shared__ int32_t s_new;
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
@@ -174,6 +184,39 @@ __global__ __launch_bounds__(64) void k_site_bbox(DevSites S,
   const int deg = (int)(S.knn_off[i + 1] - k0);
   bool fail = deg > kBoxP - 6;
   const rpd_d4 si = ld4(S.site + i);
+  const bool ranked = deg <= kNearMax;
+  if (ranked) {
+    for (int k = lane; k < deg; k += 64) {
+      const rpd_d4 e = rpd_bisector(si, ld4(S.site + S.knn_ids[k0 + k]));
+      const double nn = e.x * e.x + e.y * e.y + e.z * e.z;
+      double key = (e.x * si.x + e.y * si.y + e.z * si.z + e.w) / sqrt(nn);
+      if (!(key == key)) key = -INFINITY;  // keep a strict total order
+      s_key[k] = key;
+    }
+    wsync();
+    // rank = position in ascending (key, index) order: a permutation
+    for (int k = lane; k < deg; k += 64) {
+      const double kk = s_key[k];
+      int r = 0;
+      for (int u = 0; u < deg; ++u) {
+        const double ku = s_key[u];
+        r += (ku < kk) || (ku == kk && u < k);
+      }
+      s_near[r] = S.knn_ids[k0 + k];
+    }
+    wsync();
+    for (int k = lane; k < deg; k += 64) {
+      const int m = s_near[k];
+      knn_near[k0 + k] = m;
+      knn_site_near[k0 + k] = S.site[m];
+    }
+  } else {
+    for (int k = lane; k < deg; k += 64) {
+      const int m = S.knn_ids[k0 + k];
+      knn_near[k0 + k] = m;
+      knn_site_near[k0 + k] = S.site[m];
+    }
+  }
   // box planes: x >= x0, x <= x1, y >= y0, y <= y1, z >= z0, z <= z1
   if (lane < 6) {
     int a = lane >> 1;
@@ -200,7 +243,7 @@ __global__ __launch_bounds__(64) void k_site_bbox(DevSites S,
   wsync();
   bool empty = false;
   for (int k = 0; k < deg && !fail && !empty; ++k) {
-    const int m = S.knn_ids[k0 + k];
+    const int m = ranked ? s_near[k] : S.knn_ids[k0 + k];
     rpd_d4 e = rpd_bisector(si, ld4(S.site + m));
     double nrm = sqrt(e.x * e.x + e.y * e.y + e.z * e.z);
     e.w += nrm * rho;
@@ -315,7 +358,7 @@ __global__ __launch_bounds__(64) void k_site_bbox(DevSites S,
     }
   if (lane == 0) {
     double ext = fmax(fmax(dom[3] - dom[0], dom[4] - dom[1]), dom[5] - dom[2]);
-    double eps = 1e-9 * ext;
+    double eps = 1e-7 * ext;  // covers the rounding of the vertices
     for (int a = 0; a < 3; ++a) {
       out[a] = lo[a] - eps;
       out[3 + a] = hi[a] + eps;
@@ -414,19 +457,25 @@ __device__ __forceinline__ void load_tet(const DevMesh& M, int t,
   L.V[3] = M.vert[L.tv.w];
 }
 
-// reference relation test (voronoi.cu:154-193) in binary64
+// reference relation test (voronoi.cu:154-193) in binary64: sphere i
+// relates to the tet iff for every neighbour m some corner v has
+// pd(v, m) > pd(v, i).  The neighbours are visited nearest bisector first
+// (the outcome does not depend on the order), their spheres read inline,
+// one ahead.  *iters counts the neighbour tests made.
 __device__ __forceinline__ bool relation(const DevSites& S, const TetLoad& L,
-                                         int i) {
+                                         int i, uint32_t* iters) {
   if (S.flags[i] != 1u) return false;
   const rpd_d4 si = ld4(S.site + i);
   double pdi[4];
   for (int k = 0; k < 4; ++k)
     pdi[k] = rpd_power_dist(L.V[k].x, L.V[k].y, L.V[k].z, si);
   const int64_t k0 = S.knn_off[i], k1 = S.knn_off[i + 1];
+  if (k0 == k1) return true;
+  double4 nxt = S.knn_site_near[k0];
   for (int64_t q = k0; q < k1; ++q) {
-    const int m = S.knn_ids[q];
-    if (m < 0) continue;
-    const rpd_d4 sm = ld4(S.site + m);
+    const rpd_d4 sm{nxt.x, nxt.y, nxt.z, nxt.w};
+    if (q + 1 < k1) nxt = S.knn_site_near[q + 1];
+    ++*iters;
     bool ok = false;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -436,14 +485,17 @@ __device__ __forceinline__ bool relation(const DevSites& S, const TetLoad& L,
   return true;
 }
 
-__device__ __forceinline__ int64_t tet_bin(const TetLoad& L,
+__device__ __forceinline__ void tet_centroid(const TetLoad& L, double c[3]) {
+  c[0] = (((L.V[0].x + L.V[1].x) + L.V[2].x) + L.V[3].x) * 0.25;
+  c[1] = (((L.V[0].y + L.V[1].y) + L.V[2].y) + L.V[3].y) * 0.25;
+  c[2] = (((L.V[0].z + L.V[1].z) + L.V[2].z) + L.V[3].z) * 0.25;
+}
+
+__device__ __forceinline__ int64_t tet_bin(const double c[3],
                                            const BinGrid& g) {
-  double cx = (((L.V[0].x + L.V[1].x) + L.V[2].x) + L.V[3].x) * 0.25;
-  double cy = (((L.V[0].y + L.V[1].y) + L.V[2].y) + L.V[3].y) * 0.25;
-  double cz = (((L.V[0].z + L.V[1].z) + L.V[2].z) + L.V[3].z) * 0.25;
-  int x = bin_coord(cx, g.org[0], g.inv_bs, g.dim[0]);
-  int y = bin_coord(cy, g.org[1], g.inv_bs, g.dim[1]);
-  int z = bin_coord(cz, g.org[2], g.inv_bs, g.dim[2]);
+  int x = bin_coord(c[0], g.org[0], g.inv_bs, g.dim[0]);
+  int y = bin_coord(c[1], g.org[1], g.inv_bs, g.dim[1]);
+  int z = bin_coord(c[2], g.org[2], g.inv_bs, g.dim[2]);
   return x + (int64_t)g.dim[0] * (y + (int64_t)g.dim[1] * z);
 }
 
@@ -452,22 +504,35 @@ __global__ __launch_bounds__(256) void k_tet_count(DevMesh M, DevSites S,
                                                    const int32_t* bin_start,
                                                    const int32_t* bin_ids,
                                                    int64_t* tet_count,
-                                                   uint64_t* tet_pass) {
+                                                   uint64_t* tet_pass,
+                                                   unsigned long long* work) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= M.n_tet) return;
   TetLoad L;
   load_tet(M, t, L);
-  const int64_t b = tet_bin(L, g);
+  double c[3];
+  tet_centroid(L, c);
+  const int64_t b = tet_bin(c, g);
   int64_t n = 0;
   uint64_t pass = 0;  // bit r: the r-th site of the bin list relates
-  const int q0 = bin_start[b];
-  for (int q = q0; q < bin_start[b + 1]; ++q) {
-    const bool r = relation(S, L, bin_ids[q]);
+  uint32_t iters = 0;
+  const int q0 = bin_start[b], q1 = bin_start[b + 1];
+  for (int q = q0; q < q1; ++q) {
+    const int i = bin_ids[q];
+    // (the candidate-box test of the centroid is left to the binning: it
+    // passes 3 of 4 entries and costs more than it saves)
+    const bool r = relation(S, L, i, &iters);
     n += r;
     if (r && q - q0 < 64) pass |= 1ull << (q - q0);
   }
   tet_count[t] = n;
   tet_pass[t] = pass;
+  if (work) {  // entries, neighbour tests (count_work)
+    unsigned long long* w = work + 4 * (blockIdx.x & 63);
+    atomicAdd(w + 0, (unsigned long long)(q1 - q0));
+
+    atomicAdd(w + 1, (unsigned long long)iters);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_tet_fill(
@@ -478,14 +543,17 @@ __global__ __launch_bounds__(256) void k_tet_fill(
   if (t >= M.n_tet) return;
   TetLoad L;
   load_tet(M, t, L);
-  const int64_t b = tet_bin(L, g);
+  double c[3];
+  tet_centroid(L, c);
+  const int64_t b = tet_bin(c, g);
   int64_t o = tet_off[t];
   const uint64_t pass = tet_pass[t];
   const int q0 = bin_start[b];
   for (int q = q0; q < bin_start[b + 1]; ++q) {
     const int i = bin_ids[q];
     // the first 64 outcomes come from the counting pass
-    if (q - q0 < 64 ? !((pass >> (q - q0)) & 1) : !relation(S, L, i))
+    uint32_t iters = 0;
+    if (q - q0 < 64 ? !((pass >> (q - q0)) & 1) : !relation(S, L, i, &iters))
       continue;
     Cand c;
     c.tet = t;
@@ -633,6 +701,7 @@ __device__ __forceinline__ void vertex_pos(const rpd_d4& a, const rpd_d4& b,
 template <int G, class C, bool kExact>
 __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
   constexpr int L = 64 / G;
+  static_assert(L >= 4, "the tet's four planes are set up one per lane");
   constexpr int V = C::V, P = C::P, E = C::E;
   constexpr int VS = (V + L - 1) / L;
   constexpr bool kFull = (P == RPD_MAX_P);
@@ -678,9 +747,9 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
       s_ver[g][j] = c_init_ver[j][0] | (c_init_ver[j][1] << 8) |
                     (c_init_ver[j][2] << 16) | (meta_byte(tm, 4 + j) << 24);
     }
-    if (j < 6)
-      s_edge[g][j] = c_init_edge[j][0] | (c_init_edge[j][1] << 8) |
-                     (meta_byte(tm, 8 + j) << 16);
+    for (int q = j; q < 6; q += L)
+      s_edge[g][q] = c_init_edge[q][0] | (c_init_edge[q][1] << 8) |
+                     (meta_byte(tm, 8 + q) << 16);
   }
   wsync();
   int nb_v = 4, nb_p = 4, nb_e = 6;

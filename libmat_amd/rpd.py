@@ -58,7 +58,8 @@ class Stats(ctypes.Structure):
                 ("ms_total", ctypes.c_double),
                 ("ms_phase", ctypes.c_double * 8),
                 ("ms_kernel", ctypes.c_double * 8),
-                ("work", (ctypes.c_int64 * 4) * 3)]
+                ("work", (ctypes.c_int64 * 4) * 3),
+                ("relation_work", ctypes.c_int64 * 4)]
 
     def as_dict(self):
         d = {}

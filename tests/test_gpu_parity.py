@@ -44,6 +44,7 @@ def test_cube10_parity(ctx, seed):
 
 @pytest.mark.parametrize("cfg", [
     dict(cells_per_wave=1), dict(cells_per_wave=2), dict(cells_per_wave=4),
+    dict(cells_per_wave=8), dict(cells_per_wave=16),
     dict(use_active_mask=False), dict(small_tier=False),
     dict(force_exact=True)])
 def test_cube10_variants_identical(ctx, cfg):
