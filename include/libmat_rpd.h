@@ -207,9 +207,19 @@ struct RpdTetMesh {
   std::map<int, std::set<int>> v2tets;
   std::vector<int> v_adjs, e_adjs, f_adjs, f_ids;
 };
+// the PowerCell fields (medial_sphere.h:145-260) the driver fills
+struct RpdPowerCell {
+  int voro_id = -1;
+  std::set<int> cell_ids, tet_ids;
+  std::map<int, std::set<int>> cell_neighbors, cell_to_tfids;
+  std::map<int, std::set<int>> facet_neigh_to_cells;
+  std::map<aint2, std::set<int>> e_to_cells;
+  std::map<aint2, aint3> vertex_2id;
+};
 struct RpdSphere {
   int id = -1;
   int site_id = -1;
+  int itr_cnt = -1;  // iteration that created or last moved the sphere
   bool is_deleted = false;
   bool is_rt_valid = false;
   bool is_rt_prev_valid = false;
@@ -217,9 +227,7 @@ struct RpdSphere {
   std::set<int> rt_neigh_ids, rt_neigh_ids_prev;
   std::array<double, 3> center{{0, 0, 0}};
   double radius = 0;
-  struct {
-    std::set<int> cell_ids, tet_ids;
-  } pcell;
+  RpdPowerCell pcell;
 };
 
 namespace libmat_rpd {
@@ -255,6 +263,32 @@ class RPD3D_GPU_T {
 
   // Full RPD of all valid spheres (rpd_api.cxx:32-145).
   void calculate();
+  // Incremental RPD of the sphere-optimisation loop (rpd_api.cxx:147-313):
+  // re-computes the cells of the spheres changed in iteration
+  // num_itr_global (created/moved there, or whose validity changed) and
+  // of their 1-rings, over the tets of those spheres' previous cells (all
+  // tets if is_given_all_tets), merges them with the other spheres'
+  // previous cells and refreshes every sphere's pcell.  Increments
+  // num_itr_global and resets num_sphere_added.  No-op when
+  // num_sphere_added == 0.
+  void calculate_partial(int& num_itr_global, int& num_sphere_added,
+                         const bool is_given_all_tets = false);
+  // pcell of every sphere from powercells (update_power_cells,
+  // rpd_update.cxx:568-640, adjacency part)
+  void update_spheres_power_cells();
+
+  // steps shared by calculate and calculate_partial
+  void mark_valid_spheres(std::set<int>& valid_sphere_ids);
+  int partial_sites_and_neighbors(const std::set<int>& given,
+                                  std::vector<int>& map_site2msphere,
+                                  std::set<int>& spheres_and_1rings);
+  void load_changed_spheres(int num_itr_global, std::set<int>& changed,
+                            std::set<int>& invalid);
+  void load_partial_tet_given_spheres(const std::set<int>& given_spheres,
+                                      std::vector<int>& partial_tet_indices,
+                                      std::vector<int>& map_tet_new2orig,
+                                      std::vector<int>& partial_tet_fids,
+                                      std::vector<int>& partial_tet_f_adjs);
 
   // site arrays of (sphere + 1ring + 2ring), flags of (sphere + 1ring)
   // (rpd_api.cxx:343-379)
@@ -289,6 +323,7 @@ class RPD3D_GPU_T {
   int num_itr_rpd = 0;
   int n_site = -1;
   int site_k = -1;
+  int max_surf_fid = -1;  // largest surface facet id (-1: from f_adjs)
 };
 
 using RPD3D_GPU = RPD3D_GPU_T<>;

@@ -4,8 +4,9 @@
 #include <cmath>
 
 template <class TM, class SM, class PM, class SP>
-void RPD3D_GPU_T<TM, SM, PM, SP>::calculate() {
-  // ---- validity and 1-rings of the live spheres (the regular-triangulation
+void RPD3D_GPU_T<TM, SM, PM, SP>::mark_valid_spheres(
+    std::set<int>& valid_sphere_ids) {
+  // validity and 1-rings of the live spheres (the regular-triangulation
   // step of generate_RT_CGAL_and_mark_valid_spheres,
   // triangulation.cxx:62-144, without the 8 bounding-box points)
   std::vector<SP>& S = *all_medial_spheres;
@@ -32,7 +33,7 @@ void RPD3D_GPU_T<TM, SM, PM, SP>::calculate() {
   }
   std::vector<std::set<int>> nbrs;
   libmat_rpd::power_neighbors(xyzr, live, box, valid, nbrs);
-  std::set<int> valid_sphere_ids;
+  valid_sphere_ids.clear();
   for (int i = 0; i < n; ++i) {
     SP& m = S[i];
     m.site_id = -1;
@@ -46,15 +47,23 @@ void RPD3D_GPU_T<TM, SM, PM, SP>::calculate() {
     m.rt_neigh_ids.clear();
     for (int j : nbrs[i]) m.rt_neigh_ids.insert(S[j].id);
   }
+}
 
-  // ---- sites of (given spheres + 1-ring + 2-ring) in first-encounter
-  // order over the map keys, and their neighbour matrix
+template <class TM, class SM, class PM, class SP>
+int RPD3D_GPU_T<TM, SM, PM, SP>::partial_sites_and_neighbors(
+    const std::set<int>& given, std::vector<int>& map_site2msphere,
+    std::set<int>& spheres_and_1rings) {
+  // sites of (given spheres + 1-ring + 2-ring) in first-encounter order
+  // over the ascending map keys, and their neighbour matrix
   // (get_RT_partial_spheres_and_neighbors, triangulation.cxx:442-549)
+  std::vector<SP>& S = *all_medial_spheres;
+  map_site2msphere.clear();
+  spheres_and_1rings.clear();
+  if (given.empty()) return -1;
   std::map<int, std::set<int>> rings;
-  std::set<int> spheres_and_1rings;
-  auto add_rings = [&](const std::set<int>& given, bool collect) {
-    for (int sid : given) {
-      if (!S[sid].is_rt_valid) continue;
+  auto add_rings = [&](const std::set<int>& from, bool collect) {
+    for (int sid : from) {
+      if (!S.at(sid).is_rt_valid) continue;  // not a vertex of the RT
       if (collect) spheres_and_1rings.insert(sid);
       for (int nb : S[sid].rt_neigh_ids) {
         rings[sid].insert(nb);
@@ -63,13 +72,12 @@ void RPD3D_GPU_T<TM, SM, PM, SP>::calculate() {
       }
     }
   };
-  add_rings(valid_sphere_ids, true);
+  add_rings(given, true);
   const std::set<int> ring1 = spheres_and_1rings;
   add_rings(ring1, false);
   // (the reference assigns MedialSphere::site_id on a by-value copy of the
   // spheres, so the caller's spheres keep site_id == -1; same here)
-  std::vector<int> map_site2msphere;
-  std::vector<int> site_id_of(n, -1);
+  std::vector<int> site_id_of(S.size(), -1);
   auto site_of = [&](int sid) {
     if (site_id_of[sid] != -1) return site_id_of[sid];
     site_id_of[sid] = (int)map_site2msphere.size();
@@ -87,28 +95,175 @@ void RPD3D_GPU_T<TM, SM, PM, SP>::calculate() {
       site_neighbors[s1].insert(s0);
     }
   }
-  if (map_site2msphere.empty()) return;
-  this->site_k = num_neigh_max;
   const int ns = (int)map_site2msphere.size();
   site_knn.assign((size_t)(num_neigh_max + 1) * ns, -1);
   for (int s = 0; s < ns; ++s) {
     int r = 0;
     for (int nb : site_neighbors[s]) site_knn[(size_t)r++ * ns + s] = nb;
   }
+  return num_neigh_max;
+}
+
+template <class TM, class SM, class PM, class SP>
+void RPD3D_GPU_T<TM, SM, PM, SP>::calculate() {
+  std::set<int> valid_sphere_ids;
+  mark_valid_spheres(valid_sphere_ids);
+  std::vector<int> map_site2msphere;
+  std::set<int> spheres_and_1rings;
+  const int k = partial_sites_and_neighbors(valid_sphere_ids, map_site2msphere,
+                                            spheres_and_1rings);
+  if (map_site2msphere.empty() || k == -1) return;
+  this->site_k = k;
   load_partial_spheres_to_sites(map_site2msphere, spheres_and_1rings);
 
-  // ---- device RPD over all tets, ids back to spheres, merge
+  // device RPD over all tets, ids back to spheres, merge
   std::vector<ConvexCellHost> cells = compute_clipped_voro_diagram_GPU(
       num_itr_rpd, tet_mesh->tet_vertices, tet_mesh->tet_indices,
       tet_mesh->v2tets, tet_mesh->v_adjs, tet_mesh->e_adjs, tet_mesh->f_adjs,
       tet_mesh->f_ids, site, n_site, site_weights, site_flags, site_knn,
-      site_k, site_cell_vol, site_is_transposed, 1, (int)S.size());
+      site_k, site_cell_vol, site_is_transposed, 1,
+      (int)all_medial_spheres->size());
   const std::vector<int> map_tet_new2orig;
   update_convex_cells_voro_and_tet_ids(map_site2msphere, map_tet_new2orig,
                                        cells, is_debug);
   powercells = merge_convex_cells(valid_sphere_ids, spheres_and_1rings,
                                   powercells, cells, is_debug);
   num_itr_rpd++;
+}
+
+template <class TM, class SM, class PM, class SP>
+void RPD3D_GPU_T<TM, SM, PM, SP>::load_changed_spheres(
+    int num_itr_global, std::set<int>& changed, std::set<int>& invalid) {
+  // medial_sphere.cxx:1223-1282: spheres created in this iteration (and
+  // their previous neighbours), spheres that became valid, and the valid
+  // facet neighbours of spheres that became invalid
+  changed.clear();
+  invalid.clear();
+  const std::vector<SP>& S = *all_medial_spheres;
+  for (const SP& m : S) {
+    if (m.is_rt_valid && m.itr_cnt == num_itr_global) {
+      changed.insert(m.id);
+      for (int nb : m.rt_neigh_ids_prev) changed.insert(nb);
+    } else if (m.rt_change_status == 1) {
+      changed.insert(m.id);
+    } else if (m.rt_change_status == -1) {
+      invalid.insert(m.id);
+    }
+  }
+  for (int iid : invalid)
+    for (const auto& kv : S.at(iid).pcell.facet_neigh_to_cells)
+      if (S.at(kv.first).is_rt_valid) changed.insert(kv.first);
+}
+
+template <class TM, class SM, class PM, class SP>
+void RPD3D_GPU_T<TM, SM, PM, SP>::load_partial_tet_given_spheres(
+    const std::set<int>& given_spheres, std::vector<int>& partial_tet_indices,
+    std::vector<int>& map_tet_new2orig, std::vector<int>& partial_tet_fids,
+    std::vector<int>& partial_tet_f_adjs) {
+  // rpd_api.cxx:482-535: the tets of the given spheres' previous cells, in
+  // first-encounter order
+  partial_tet_indices.clear();
+  map_tet_new2orig.clear();
+  partial_tet_fids.clear();
+  partial_tet_f_adjs.clear();
+  std::set<int> visited;
+  for (int sid : given_spheres) {
+    for (int t : all_medial_spheres->at(sid).pcell.tet_ids) {
+      if (!visited.insert(t).second) continue;
+      map_tet_new2orig.push_back(t);
+      for (int j = 0; j < 4; ++j) {
+        partial_tet_indices.push_back(tet_mesh->tet_indices.at(4 * t + j));
+        partial_tet_fids.push_back(tet_mesh->f_ids.at(4 * t + j));
+        partial_tet_f_adjs.push_back(tet_mesh->f_adjs.at(4 * t + j));
+      }
+    }
+  }
+}
+
+template <class TM, class SM, class PM, class SP>
+void RPD3D_GPU_T<TM, SM, PM, SP>::calculate_partial(int& num_itr_global,
+                                                    int& num_sphere_added,
+                                                    const bool all_tets) {
+  // rpd_api.cxx:147-313
+  if (num_sphere_added == 0) return;
+  std::set<int> valid_sphere_ids;
+  mark_valid_spheres(valid_sphere_ids);
+  std::set<int> spheres_changed, spheres_invalid;
+  load_changed_spheres(num_itr_global, spheres_changed, spheres_invalid);
+  if (spheres_changed.empty()) return;
+  std::vector<int> map_site2msphere;
+  std::set<int> spheres_and_1rings;
+  const int k = partial_sites_and_neighbors(spheres_changed, map_site2msphere,
+                                            spheres_and_1rings);
+  if (map_site2msphere.empty() || k == -1) return;
+  this->site_k = k;
+  load_partial_spheres_to_sites(map_site2msphere, spheres_and_1rings);
+
+  std::vector<int> tet_indices, tet_fids, tet_f_adjs, map_tet_new2orig;
+  if (all_tets) {
+    tet_indices = tet_mesh->tet_indices;
+    tet_fids = tet_mesh->f_ids;
+    tet_f_adjs = tet_mesh->f_adjs;
+  } else {
+    // the previous cells of removed spheres must be re-covered too
+    std::set<int> target = spheres_and_1rings;
+    target.insert(spheres_invalid.begin(), spheres_invalid.end());
+    load_partial_tet_given_spheres(target, tet_indices, map_tet_new2orig,
+                                   tet_fids, tet_f_adjs);
+  }
+  std::vector<ConvexCellHost> cells;
+  if (!tet_indices.empty())
+    cells = compute_clipped_voro_diagram_GPU(
+        num_itr_global, tet_mesh->tet_vertices, tet_indices, tet_mesh->v2tets,
+        tet_mesh->v_adjs, tet_mesh->e_adjs, tet_f_adjs, tet_fids, site, n_site,
+        site_weights, site_flags, site_knn, site_k, site_cell_vol,
+        site_is_transposed, 1, (int)all_medial_spheres->size());
+  update_convex_cells_voro_and_tet_ids(map_site2msphere, map_tet_new2orig,
+                                       cells, is_debug);
+  powercells = merge_convex_cells(valid_sphere_ids, spheres_and_1rings,
+                                  powercells, cells, is_debug);
+  update_spheres_power_cells();
+  num_itr_global++;
+  num_sphere_added = 0;
+}
+
+template <class TM, class SM, class PM, class SP>
+void RPD3D_GPU_T<TM, SM, PM, SP>::update_spheres_power_cells() {
+  // the adjacency part of update_power_cells (rpd_update.cxx:568-640):
+  // every sphere's cells, tets, cell/facet/edge/vertex adjacency; a live
+  // sphere without cells is marked deleted, as there
+  int max_fid = max_surf_fid;
+  if (max_fid < 0) {  // largest face id on the mesh boundary
+    const std::vector<int>& fa = tet_mesh->f_adjs;
+    for (size_t q = 0; q < fa.size(); ++q)
+      if (fa[q] == 1) max_fid = std::max(max_fid, tet_mesh->f_ids[q]);
+  }
+  std::map<int, PowerCellAdjacency> adj =
+      update_power_cells_adjacency(powercells, max_fid);
+  for (SP& m : *all_medial_spheres) {
+    auto it = adj.find(m.id);
+    auto& pc = m.pcell;
+    pc.voro_id = m.id;
+    if (it == adj.end()) {
+      pc.cell_ids.clear();
+      pc.tet_ids.clear();
+      pc.cell_neighbors.clear();
+      pc.cell_to_tfids.clear();
+      pc.facet_neigh_to_cells.clear();
+      pc.e_to_cells.clear();
+      pc.vertex_2id.clear();
+      if (!m.is_deleted) m.is_deleted = true;
+      continue;
+    }
+    PowerCellAdjacency& a = it->second;
+    pc.cell_ids = std::move(a.cell_ids);
+    pc.tet_ids = std::move(a.tet_ids);
+    pc.cell_neighbors = std::move(a.cell_neighbors);
+    pc.cell_to_tfids = std::move(a.cell_to_tfids);
+    pc.facet_neigh_to_cells = std::move(a.facet_neigh_to_cells);
+    pc.e_to_cells = std::move(a.e_to_cells);
+    pc.vertex_2id = std::move(a.vertex_2id);
+  }
 }
 
 template <class TM, class SM, class PM, class SP>

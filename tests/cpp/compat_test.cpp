@@ -4,6 +4,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <numeric>
 
 #include "libmat_rpd.h"
@@ -110,6 +111,86 @@ int main(int argc, char** argv) {
   std::printf("adjacency spheres %zu, facets %zu, asymmetric %d\n",
               adj.size(), fac.size(), asym);
   CHECK(asym == 0);
+
+  // ---- calculate_partial: move 20 spheres, add one, delete one; the
+  // merged cells must equal a full recompute, cell by cell (arrays are in
+  // clip order, which follows site ids, so compare active planes + volume)
+  rpd.update_spheres_power_cells();
+  int itr = 1, added = 0;
+  for (int i = 0; i < 500; i += 25) {
+    if (spheres[i].is_deleted) continue;
+    spheres[i].center[0] = std::min(0.95, spheres[i].center[0] + 0.01);
+    spheres[i].center[1] = std::max(0.05, spheres[i].center[1] - 0.007);
+    spheres[i].itr_cnt = itr;
+    ++added;
+  }
+  RpdSphere extra;
+  extra.id = 500;
+  extra.center = {0.51, 0.49, 0.5};
+  extra.radius = 0.06;
+  extra.itr_cnt = itr;
+  spheres.push_back(extra);
+  ++added;
+  int removed = -1;
+  for (int i = 3; i < 500; ++i)
+    if (!spheres[i].is_deleted && spheres[i].is_rt_valid) {
+      spheres[i].is_deleted = true;
+      removed = i;
+      break;
+    }
+  CHECK(removed >= 0);
+  rpd.calculate_partial(itr, added);
+  CHECK(itr == 2 && added == 0);
+  std::vector<RpdSphere> spheres2 = spheres;
+  RPD3D_GPU full;
+  full.init(&mesh, nullptr, nullptr);
+  full.set_spheres(&spheres2);
+  full.calculate();
+  auto canon = [](std::vector<ConvexCellHost> v) {
+    std::map<std::array<int, 2>, std::pair<std::set<std::array<int, 2>>, float>>
+        m;
+    for (auto& c : v) {
+      c.reload_active();
+      auto& e = m[{c.voro_id, c.tet_id}];
+      for (int p = 0; p < c.nb_p; ++p)
+        if (c.active_clipping_planes[p] > 0) {
+          // (min, max) of the site ids, remapped to sphere ids: the pair's
+          // orientation follows the site numbering of the run
+          cint2 h = c.clip_id2_const(p);
+          if (h.y != -1)
+            e.first.insert({std::min(h.x, h.y), std::max(h.x, h.y)});
+          else
+            e.first.insert({h.x, h.y});
+        }
+      e.second = c.cell_vol;
+    }
+    return m;
+  };
+  auto mp = canon(rpd.powercells), mf = canon(full.powercells);
+  int diff = 0;
+  for (auto& kv : mf) {
+    auto it = mp.find(kv.first);
+    if (it == mp.end() || it->second.first != kv.second.first ||
+        std::fabs(it->second.second - kv.second.second) >
+            1e-5f * std::max(1e-6f, kv.second.second)) {
+      if (++diff <= 4 && it != mp.end()) {
+        std::printf("  cell (sphere %d, tet %d): vol %.9g vs %.9g planes",
+                    kv.first[0], kv.first[1], it->second.second,
+                    kv.second.second);
+        for (auto& h : it->second.first) std::printf(" %d/%d", h[0], h[1]);
+        std::printf(" | full");
+        for (auto& h : kv.second.first) std::printf(" %d/%d", h[0], h[1]);
+        std::printf("\n");
+      }
+    }
+  }
+  std::printf("partial: %zu cells, full: %zu cells, differing %d, "
+              "removed sphere %d has cells %d\n",
+              mp.size(), mf.size(), diff, removed,
+              (int)spheres[removed].pcell.cell_ids.size());
+  CHECK(mp.size() == mf.size() && diff == 0);
+  CHECK(spheres[removed].pcell.cell_ids.empty());
+  CHECK(!spheres[500].pcell.cell_ids.empty());
   std::printf("gpu ok\n");
   return 0;
 }
