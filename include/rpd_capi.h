@@ -138,11 +138,36 @@ const char* rpd_last_error(const rpd_ctx* ctx);
 /* hipStream_t of the context (as void*), for event timing by callers. */
 void* rpd_ctx_stream(rpd_ctx* ctx);
 
-/* Copy an instance to the device (replaces the resident one). */
+/* Copy an instance to the device (replaces the resident one); the host
+ * staging of voronoi.cu:455-521 and copy_tet_data (voronoi.cu:324-362). */
 int rpd_upload(rpd_ctx* ctx, const rpd_instance_desc* desc);
+
+/* Spheres, flags and neighbour lists of one instance. */
+typedef struct {
+  int32_t n_site;
+  const double* sites;      /* 4*n_site: x, y, z, w = r^2               */
+  const uint32_t* flags;    /* n_site                                   */
+  const int64_t* knn_off;   /* n_site+1                                 */
+  const int32_t* knn_ids;   /* knn_off[n_site], instance-local ids      */
+} rpd_sites_desc;
+
+/* A batch of independent instances made resident together (replaces the
+ * resident batch); one rpd_run then computes all of them at once.  No
+ * reference counterpart: the reference computes one instance per call.
+ * Ids in the results (rpd_cell.tet_id / seed, the relation CSR) are
+ * batch-wide: instance b owns tets [tet_off[b], tet_off[b+1]) and sites
+ * [site_off[b], site_off[b+1]) (rpd_batch_offsets). */
+int rpd_upload_batch(rpd_ctx* ctx, int32_t n_inst,
+                     const rpd_instance_desc* descs);
+int rpd_update_sites_batch(rpd_ctx* ctx, int32_t n_inst,
+                           const rpd_sites_desc* sites);
+int rpd_batch_offsets(const rpd_ctx* ctx, int64_t* tet_off /* n_inst+1 */,
+                      int64_t* site_off /* n_inst+1 */);
+int32_t rpd_batch_size(const rpd_ctx* ctx);
 /* Replace only the spheres / flags / neighbour lists (same n_site or not),
  * keeping the resident tet mesh: the per-iteration update of sphere
- * optimisation. */
+ * optimisation (load_partial_spheres_to_sites, rpd_api.cxx:343-379, and the
+ * site H2D of voronoi.cu:469-521).  Single-instance batches only. */
 int rpd_update_sites(rpd_ctx* ctx, int32_t n_site, const double* sites,
                      const uint32_t* flags, const int64_t* knn_off,
                      const int32_t* knn_ids);

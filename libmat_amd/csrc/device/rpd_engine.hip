@@ -73,13 +73,19 @@ struct rpd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  // instance
+  // resident batch of instances (one instance = a batch of one), stored
+  // back to back: vertex, tet and site ids are batch-wide
+  struct Inst {
+    int32_t n_vert = 0, n_tet = 0, n_site = 0;
+    int64_t vert_off = 0, tet_off = 0, site_off = 0;
+    double dom[6] = {0, 0, 0, 1, 1, 1};
+    double rho = 0;
+  };
+  std::vector<Inst> inst;
   int32_t n_vert = 0, n_tet = 0, n_site = 0;
-  double dom[6] = {0, 0, 0, 1, 1, 1};
-  double rho = 0;
   bool has_mesh = false, has_sites = false;
-  Buf vert, tet_v, tet_fid, tet_meta, dom_dev;
-  Buf site, flags, knn_off, knn_ids, knn_near, knn_site_near;
+  Buf vert, tet_v, tet_fid, tet_meta, tet_inst, inst_dev;
+  Buf site, flags, knn_off, knn_ids, knn_near, knn_site_near, site_inst;
   // pipeline
   Buf bbox, reg_n, reg_off, keys_a, keys_b, bin_start, bin_ids;
   Buf tet_count, tet_off, tet_pass, cand, rel_ids;
@@ -167,131 +173,243 @@ const char* rpd_last_error(const rpd_ctx* ctx) {
 
 void* rpd_ctx_stream(rpd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
-int rpd_update_sites(rpd_ctx* ctx, int32_t n_site, const double* sites,
-                     const uint32_t* flags, const int64_t* knn_off,
-                     const int32_t* knn_ids) {
+int rpd_update_sites_batch(rpd_ctx* ctx, int32_t n_inst,
+                           const rpd_sites_desc* sd) {
   if (!ctx) return -1;
-  if (n_site < 0 || (n_site > 0 && (!sites || !flags || !knn_off)))
-    return fail(ctx, "rpd_update_sites: bad arguments");
+  if (!ctx->has_mesh) return fail(ctx, "rpd_update_sites: no resident mesh");
+  if (n_inst != (int32_t)ctx->inst.size() || (n_inst > 0 && !sd))
+    return fail(ctx, "rpd_update_sites: one site set per resident instance");
   HIP_TRY(hipSetDevice(ctx->device));
-  const int64_t n_knn = n_site ? knn_off[n_site] : 0;
-  if (n_site && knn_off[0] != 0)
-    return fail(ctx, "rpd_update_sites: knn_off[0] must be 0");
-  for (int32_t i = 0; i < n_site; ++i)
-    if (knn_off[i + 1] < knn_off[i])
-      return fail(ctx, "rpd_update_sites: knn_off not monotone");
-  for (int64_t q = 0; q < n_knn; ++q)
-    if (knn_ids[q] < 0 || knn_ids[q] >= n_site)
-      return fail(ctx, "rpd_update_sites: neighbour id out of range");
-  ctx->n_site = n_site;
-  HIP_TRY(ctx->site.need(std::max<int64_t>(n_site, 1) * 32));
-  HIP_TRY(ctx->flags.need(std::max<int64_t>(n_site, 1) * 4));
-  HIP_TRY(ctx->knn_off.need((n_site + 1) * 8));
-  HIP_TRY(ctx->knn_ids.need(std::max<int64_t>(n_knn, 1) * 4));
-  HIP_TRY(ctx->knn_near.need(std::max<int64_t>(n_knn, 1) * 4));
-  HIP_TRY(ctx->knn_site_near.need(std::max<int64_t>(n_knn, 1) * 32));
-  if (n_site) {
-    HIP_TRY(hipMemcpyAsync(ctx->site.p, sites, (size_t)n_site * 32,
+  int64_t ns_tot = 0, nk_tot = 0;
+  for (int32_t b = 0; b < n_inst; ++b) {
+    const rpd_sites_desc& d = sd[b];
+    if (d.n_site < 0 ||
+        (d.n_site > 0 && (!d.sites || !d.flags || !d.knn_off)))
+      return fail(ctx, "rpd_update_sites: bad arguments");
+    const int64_t n_knn = d.n_site ? d.knn_off[d.n_site] : 0;
+    if (d.n_site && d.knn_off[0] != 0)
+      return fail(ctx, "rpd_update_sites: knn_off[0] must be 0");
+    for (int32_t i = 0; i < d.n_site; ++i)
+      if (d.knn_off[i + 1] < d.knn_off[i])
+        return fail(ctx, "rpd_update_sites: knn_off not monotone");
+    if (n_knn && !d.knn_ids)
+      return fail(ctx, "rpd_update_sites: missing knn_ids");
+    for (int64_t q = 0; q < n_knn; ++q)
+      if (d.knn_ids[q] < 0 || d.knn_ids[q] >= d.n_site)
+        return fail(ctx, "rpd_update_sites: neighbour id out of range");
+    ns_tot += d.n_site;
+    nk_tot += n_knn;
+  }
+  if (ns_tot >= (1LL << 31) || nk_tot >= (1LL << 31))
+    return fail(ctx, "rpd_update_sites: batch too large");
+  std::vector<double> site(4 * std::max<int64_t>(ns_tot, 1));
+  std::vector<uint32_t> flags(std::max<int64_t>(ns_tot, 1));
+  std::vector<int64_t> koff(ns_tot + 1);
+  std::vector<int32_t> kids(std::max<int64_t>(nk_tot, 1));
+  std::vector<int32_t> sinst(std::max<int64_t>(ns_tot, 1));
+  int64_t so = 0, ko = 0;
+  int32_t max_deg = 0;
+  for (int32_t b = 0; b < n_inst; ++b) {
+    const rpd_sites_desc& d = sd[b];
+    ctx->inst[b].n_site = d.n_site;
+    ctx->inst[b].site_off = so;
+    if (d.n_site) std::memcpy(&site[4 * so], d.sites, (size_t)d.n_site * 32);
+    for (int32_t i = 0; i < d.n_site; ++i) {
+      flags[so + i] = d.flags[i];
+      sinst[so + i] = b;
+      koff[so + i] = ko + d.knn_off[i];
+      max_deg = std::max<int32_t>(max_deg,
+                                  (int32_t)(d.knn_off[i + 1] - d.knn_off[i]));
+    }
+    const int64_t n_knn = d.n_site ? d.knn_off[d.n_site] : 0;
+    for (int64_t q = 0; q < n_knn; ++q)
+      kids[ko + q] = (int32_t)(d.knn_ids[q] + so);
+    so += d.n_site;
+    ko += n_knn;
+  }
+  koff[ns_tot] = ko;
+  ctx->n_site = (int32_t)ns_tot;
+  HIP_TRY(ctx->site.need(std::max<int64_t>(ns_tot, 1) * 32));
+  HIP_TRY(ctx->flags.need(std::max<int64_t>(ns_tot, 1) * 4));
+  HIP_TRY(ctx->site_inst.need(std::max<int64_t>(ns_tot, 1) * 4));
+  HIP_TRY(ctx->knn_off.need((ns_tot + 1) * 8));
+  HIP_TRY(ctx->knn_ids.need(std::max<int64_t>(nk_tot, 1) * 4));
+  HIP_TRY(ctx->knn_near.need(std::max<int64_t>(nk_tot, 1) * 4));
+  HIP_TRY(ctx->knn_site_near.need(std::max<int64_t>(nk_tot, 1) * 32));
+  if (ns_tot) {
+    HIP_TRY(hipMemcpyAsync(ctx->site.p, site.data(), (size_t)ns_tot * 32,
                            hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->flags.p, flags, (size_t)n_site * 4,
+    HIP_TRY(hipMemcpyAsync(ctx->flags.p, flags.data(), (size_t)ns_tot * 4,
                            hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->knn_off.p, knn_off, (size_t)(n_site + 1) * 8,
-                           hipMemcpyHostToDevice, ctx->stream));
-    if (n_knn)
-      HIP_TRY(hipMemcpyAsync(ctx->knn_ids.p, knn_ids, (size_t)n_knn * 4,
+    HIP_TRY(hipMemcpyAsync(ctx->site_inst.p, sinst.data(),
+                           (size_t)ns_tot * 4, hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->knn_off.p, koff.data(),
+                           (size_t)(ns_tot + 1) * 8, hipMemcpyHostToDevice,
+                           ctx->stream));
+    if (nk_tot)
+      HIP_TRY(hipMemcpyAsync(ctx->knn_ids.p, kids.data(), (size_t)nk_tot * 4,
                              hipMemcpyHostToDevice, ctx->stream));
   }
-  int32_t max_deg = 0;
-  for (int32_t i = 0; i < n_site; ++i)
-    max_deg = std::max<int32_t>(max_deg, (int32_t)(knn_off[i + 1] - knn_off[i]));
   ctx->st.site_k = max_deg;
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   ctx->has_sites = true;
   return 0;
 }
 
-int rpd_upload(rpd_ctx* ctx, const rpd_instance_desc* d) {
-  if (!ctx || !d) return -1;
-  if (d->n_vert <= 0 || d->n_tet <= 0 || !d->verts || !d->tets ||
-      !d->v_adj || !d->e_adj || !d->f_adj || !d->f_id)
-    return fail(ctx, "rpd_upload: missing mesh arrays");
+int rpd_update_sites(rpd_ctx* ctx, int32_t n_site, const double* sites,
+                     const uint32_t* flags, const int64_t* knn_off,
+                     const int32_t* knn_ids) {
+  if (!ctx) return -1;
+  if (ctx->inst.size() != 1)
+    return fail(ctx, "rpd_update_sites: the resident batch has more than "
+                     "one instance; use rpd_update_sites_batch");
+  const rpd_sites_desc d{n_site, sites, flags, knn_off, knn_ids};
+  return rpd_update_sites_batch(ctx, 1, &d);
+}
+
+int rpd_upload_batch(rpd_ctx* ctx, int32_t n_inst,
+                     const rpd_instance_desc* descs) {
+  if (!ctx || n_inst <= 0 || !descs) return -1;
   HIP_TRY(hipSetDevice(ctx->device));
-  const int32_t nv = d->n_vert, nt = d->n_tet;
-  for (int64_t q = 0; q < 4LL * nt; ++q)
-    if (d->tets[q] < 0 || d->tets[q] >= nv)
-      return fail(ctx, "rpd_upload: tet vertex id out of range");
-  std::vector<double> v4(4LL * nv);
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (int64_t i = 0; i < nv; ++i) {
-    for (int a = 0; a < 3; ++a) {
-      double x = d->verts[3 * i + a];
-      if (!std::isfinite(x)) return fail(ctx, "rpd_upload: non-finite vertex");
-      v4[4 * i + a] = x;
-      lo[a] = std::min(lo[a], x);
-      hi[a] = std::max(hi[a], x);
-    }
-    v4[4 * i + 3] = 0.0;
+  int64_t nv_tot = 0, nt_tot = 0;
+  for (int32_t b = 0; b < n_inst; ++b) {
+    const rpd_instance_desc* d = descs + b;
+    if (d->n_vert <= 0 || d->n_tet <= 0 || !d->verts || !d->tets ||
+        !d->v_adj || !d->e_adj || !d->f_adj || !d->f_id)
+      return fail(ctx, "rpd_upload: missing mesh arrays");
+    for (int64_t q = 0; q < 4LL * d->n_tet; ++q)
+      if (d->tets[q] < 0 || d->tets[q] >= d->n_vert)
+        return fail(ctx, "rpd_upload: tet vertex id out of range");
+    nv_tot += d->n_vert;
+    nt_tot += d->n_tet;
   }
-  std::vector<TetMeta> meta(nt);
-  double rho = 0;
-  for (int64_t t = 0; t < nt; ++t) {
-    const int32_t* tv = d->tets + 4 * t;
-    uint32_t f = 0, va = 0, e0 = 0, e1 = 0;
-    for (int k = 0; k < 4; ++k) {
-      f |= ((uint32_t)d->f_adj[4 * t + k] & 255u) << (8 * k);
-      va |= ((uint32_t)d->v_adj[tv[k]] & 255u) << (8 * k);
-    }
-    for (int k = 0; k < 4; ++k)
-      e0 |= ((uint32_t)d->e_adj[6 * t + k] & 255u) << (8 * k);
-    for (int k = 0; k < 2; ++k)
-      e1 |= ((uint32_t)d->e_adj[6 * t + 4 + k] & 255u) << (8 * k);
-    meta[t] = TetMeta{f, va, e0, e1};
-    double c[3];
-    for (int a = 0; a < 3; ++a)
-      c[a] = (((d->verts[3 * tv[0] + a] + d->verts[3 * tv[1] + a]) +
-               d->verts[3 * tv[2] + a]) +
-              d->verts[3 * tv[3] + a]) *
-             0.25;
-    for (int k = 0; k < 4; ++k) {
-      double s = 0;
+  if (nv_tot >= (1LL << 31) || nt_tot >= (1LL << 31))
+    return fail(ctx, "rpd_upload: batch too large");
+  std::vector<double> v4(4 * nv_tot);
+  std::vector<int32_t> tv_all(4 * nt_tot), fid_all(4 * nt_tot),
+      tinst(nt_tot);
+  std::vector<TetMeta> meta(nt_tot);
+  std::vector<rpd_ctx::Inst> inst(n_inst);
+  int64_t vo = 0, to = 0;
+  for (int32_t b = 0; b < n_inst; ++b) {
+    const rpd_instance_desc* d = descs + b;
+    const int32_t nv = d->n_vert, nt = d->n_tet;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t i = 0; i < nv; ++i) {
       for (int a = 0; a < 3; ++a) {
-        double x = d->verts[3 * tv[k] + a] - c[a];
-        s += x * x;
+        double x = d->verts[3 * i + a];
+        if (!std::isfinite(x))
+          return fail(ctx, "rpd_upload: non-finite vertex");
+        v4[4 * (vo + i) + a] = x;
+        lo[a] = std::min(lo[a], x);
+        hi[a] = std::max(hi[a], x);
       }
-      rho = std::max(rho, std::sqrt(s));
+      v4[4 * (vo + i) + 3] = 0.0;
     }
+    double rho = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+      const int32_t* tv = d->tets + 4 * t;
+      uint32_t f = 0, va = 0, e0 = 0, e1 = 0;
+      for (int k = 0; k < 4; ++k) {
+        f |= ((uint32_t)d->f_adj[4 * t + k] & 255u) << (8 * k);
+        va |= ((uint32_t)d->v_adj[tv[k]] & 255u) << (8 * k);
+        tv_all[4 * (to + t) + k] = (int32_t)(tv[k] + vo);
+        fid_all[4 * (to + t) + k] = d->f_id[4 * t + k];
+      }
+      for (int k = 0; k < 4; ++k)
+        e0 |= ((uint32_t)d->e_adj[6 * t + k] & 255u) << (8 * k);
+      for (int k = 0; k < 2; ++k)
+        e1 |= ((uint32_t)d->e_adj[6 * t + 4 + k] & 255u) << (8 * k);
+      meta[to + t] = TetMeta{f, va, e0, e1};
+      tinst[to + t] = b;
+      double c[3];
+      for (int a = 0; a < 3; ++a)
+        c[a] = (((d->verts[3 * tv[0] + a] + d->verts[3 * tv[1] + a]) +
+                 d->verts[3 * tv[2] + a]) +
+                d->verts[3 * tv[3] + a]) *
+               0.25;
+      for (int k = 0; k < 4; ++k) {
+        double s2 = 0;
+        for (int a = 0; a < 3; ++a) {
+          double x = d->verts[3 * tv[k] + a] - c[a];
+          s2 += x * x;
+        }
+        rho = std::max(rho, std::sqrt(s2));
+      }
+    }
+    double ext =
+        std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+    rpd_ctx::Inst& I = inst[b];
+    I.n_vert = nv;
+    I.n_tet = nt;
+    I.vert_off = vo;
+    I.tet_off = to;
+    I.rho = rho * (1 + 1e-9) + 1e-9 * ext;
+    for (int a = 0; a < 3; ++a) {
+      I.dom[a] = lo[a];
+      I.dom[3 + a] = hi[a];
+    }
+    vo += nv;
+    to += nt;
   }
-  double ext = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
-  ctx->rho = rho * (1 + 1e-9) + 1e-9 * ext;
-  for (int a = 0; a < 3; ++a) {
-    ctx->dom[a] = lo[a];
-    ctx->dom[3 + a] = hi[a];
-  }
-  ctx->n_vert = nv;
-  ctx->n_tet = nt;
-  HIP_TRY(ctx->vert.need((size_t)nv * 32));
-  HIP_TRY(ctx->tet_v.need((size_t)nt * 16));
-  HIP_TRY(ctx->tet_fid.need((size_t)nt * 16));
-  HIP_TRY(ctx->tet_meta.need((size_t)nt * 16));
-  HIP_TRY(ctx->dom_dev.need(6 * 8));
-  HIP_TRY(hipMemcpyAsync(ctx->vert.p, v4.data(), (size_t)nv * 32,
+  ctx->inst = inst;
+  ctx->n_vert = (int32_t)nv_tot;
+  ctx->n_tet = (int32_t)nt_tot;
+  ctx->has_sites = false;
+  HIP_TRY(ctx->vert.need((size_t)nv_tot * 32));
+  HIP_TRY(ctx->tet_v.need((size_t)nt_tot * 16));
+  HIP_TRY(ctx->tet_fid.need((size_t)nt_tot * 16));
+  HIP_TRY(ctx->tet_meta.need((size_t)nt_tot * 16));
+  HIP_TRY(ctx->tet_inst.need((size_t)nt_tot * 4));
+  HIP_TRY(hipMemcpyAsync(ctx->vert.p, v4.data(), (size_t)nv_tot * 32,
                          hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(ctx->tet_v.p, d->tets, (size_t)nt * 16,
+  HIP_TRY(hipMemcpyAsync(ctx->tet_v.p, tv_all.data(), (size_t)nt_tot * 16,
                          hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(ctx->tet_fid.p, d->f_id, (size_t)nt * 16,
+  HIP_TRY(hipMemcpyAsync(ctx->tet_fid.p, fid_all.data(), (size_t)nt_tot * 16,
                          hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(ctx->tet_meta.p, meta.data(), (size_t)nt * 16,
+  HIP_TRY(hipMemcpyAsync(ctx->tet_meta.p, meta.data(), (size_t)nt_tot * 16,
                          hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(ctx->dom_dev.p, ctx->dom, 6 * 8,
+  HIP_TRY(hipMemcpyAsync(ctx->tet_inst.p, tinst.data(), (size_t)nt_tot * 4,
                          hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   ctx->has_mesh = true;
-  if (d->sites || d->n_site == 0)
-    return rpd_update_sites(ctx, d->n_site, d->sites, d->flags, d->knn_off,
-                            d->knn_ids);
+  bool all_sites = true;
+  for (int32_t b = 0; b < n_inst; ++b)
+    all_sites = all_sites && (descs[b].sites || descs[b].n_site == 0);
+  if (!all_sites) return 0;
+  std::vector<rpd_sites_desc> sd(n_inst);
+  for (int32_t b = 0; b < n_inst; ++b)
+    sd[b] = rpd_sites_desc{descs[b].n_site, descs[b].sites, descs[b].flags,
+                           descs[b].knn_off, descs[b].knn_ids};
+  return rpd_update_sites_batch(ctx, n_inst, sd.data());
+}
+
+int rpd_upload(rpd_ctx* ctx, const rpd_instance_desc* d) {
+  if (!ctx || !d) return -1;
+  return rpd_upload_batch(ctx, 1, d);
+}
+
+int rpd_batch_offsets(const rpd_ctx* ctx, int64_t* tet_off,
+                      int64_t* site_off) {
+  if (!ctx) return -1;
+  const size_t n = ctx->inst.size();
+  for (size_t b = 0; b <= n; ++b) {
+    if (tet_off)
+      tet_off[b] = b < n ? ctx->inst[b].tet_off : (int64_t)ctx->n_tet;
+    if (site_off)
+      site_off[b] = b < n ? ctx->inst[b].site_off : (int64_t)ctx->n_site;
+  }
   return 0;
 }
 
+int32_t rpd_batch_size(const rpd_ctx* ctx) {
+  return ctx ? (int32_t)ctx->inst.size() : 0;
+}
+
+}  // extern "C"
+
+namespace {
 static int launch_clip(rpd_ctx* ctx, int tier, int G, ClipArgs& a) {
   if (a.n_items <= 0) return 0;
   const int blocks = ceil_div(a.n_items, G);
@@ -319,6 +437,10 @@ static int launch_clip(rpd_ctx* ctx, int tier, int G, ClipArgs& a) {
   HIP_TRY(hipGetLastError());
   return 0;
 }
+
+}  // namespace
+
+extern "C" {
 
 int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   if (!ctx) return -1;
@@ -361,27 +483,51 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   }
 
   // ---- 1. candidate gathering
-  BinGrid bg;
+  // per instance: a uniform grid of about n_tet/6 bins over its domain,
+  // numbered from bin_base in the batch-wide bin space
+  int64_t n_bins = 0;
   {
-    double ext[3], vol = 1;
-    for (int a = 0; a < 3; ++a) {
-      ext[a] = std::max(ctx->dom[3 + a] - ctx->dom[a], 1e-300);
-      vol *= ext[a];
+    std::vector<InstInfo> ii(ctx->inst.size());
+    for (size_t b = 0; b < ctx->inst.size(); ++b) {
+      const rpd_ctx::Inst& I = ctx->inst[b];
+      InstInfo& o = ii[b];
+      double ext[3], vol = 1;
+      for (int a = 0; a < 3; ++a) {
+        ext[a] = std::max(I.dom[3 + a] - I.dom[a], 1e-300);
+        vol *= ext[a];
+      }
+      const double bs = std::cbrt(vol / std::max(1.0, I.n_tet / 6.0));
+      int64_t nb = 1;
+      for (int a = 0; a < 3; ++a) {
+        o.g.org[a] = I.dom[a];
+        o.g.dim[a] = (int32_t)std::min<double>(
+            512, std::max(1.0, std::ceil(ext[a] / bs)));
+        nb *= o.g.dim[a];
+      }
+      o.g.inv_bs = 1.0 / bs;
+      o.g.pad = 0;
+      o.bin_base = n_bins;
+      for (int a = 0; a < 6; ++a) o.dom[a] = I.dom[a];
+      o.rho = I.rho;
+      n_bins += nb;
     }
-    double bs = std::cbrt(vol / std::max(1.0, nt / 6.0));
-    int64_t nb = 1;
-    for (int a = 0; a < 3; ++a) {
-      bg.org[a] = ctx->dom[a];
-      bg.dim[a] = (int32_t)std::min<double>(512, std::max(1.0, std::ceil(ext[a] / bs)));
-      nb *= bg.dim[a];
-    }
-    bg.inv_bs = 1.0 / bs;
-    bg.pad = 0;
-    const int64_t n_bins = nb;
+    if (n_bins >= (1LL << 31))
+      return fail(ctx, "rpd_run: too many candidate bins");
+    HIP_TRY(ctx->inst_dev.need(ii.size() * sizeof(InstInfo)));
+    HIP_TRY(hipMemcpyAsync(ctx->inst_dev.p, ii.data(),
+                           ii.size() * sizeof(InstInfo),
+                           hipMemcpyHostToDevice, s));
+    // (the host vector must outlive the copy)
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  const InstInfo* inst_d = ctx->inst_dev.as<InstInfo>();
+  const int32_t* site_inst_d = ctx->site_inst.as<int32_t>();
+  const int32_t* tet_inst_d = ctx->tet_inst.as<int32_t>();
+  {
     HIP_TRY(ctx->bbox.need((size_t)std::max(ns, 1) * 48));
     if (ns > 0)
-      hipLaunchKernelGGL(k_site_bbox, dim3(ns), dim3(64), 0, s, S,
-                         ctx->dom_dev.as<double>(), ctx->rho,
+      hipLaunchKernelGGL(k_site_bbox, dim3(ns), dim3(64), 0, s, S, inst_d,
+                         site_inst_d,
                          ctx->bbox.as<double>(), ctx->knn_near.as<int32_t>(),
                          ctx->knn_site_near.as<double4>(), cnt + 0);
     HIP_TRY(hipGetLastError());
@@ -391,7 +537,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     HIP_TRY(ctx->reg_n.need(((size_t)ns + 1) * 8));
     HIP_TRY(ctx->reg_off.need(((size_t)ns + 1) * 8));
     hipLaunchKernelGGL(k_bin_nreg, dim3(ceil_div(ns + 1, 256)), dim3(256), 0,
-                       s, ns, ctx->bbox.as<double>(), bg,
+                       s, ns, ctx->bbox.as<double>(), inst_d, site_inst_d,
                        ctx->reg_n.as<int64_t>());
     HIP_TRY(hipGetLastError());
     HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->reg_n.as<int64_t>(),
@@ -409,8 +555,8 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     HIP_TRY(ctx->bin_ids.need((size_t)std::max<int64_t>(n_reg, 1) * 4));
     if (n_reg > 0) {
       hipLaunchKernelGGL(k_bin_expand, dim3(ceil_div(n_reg, 256)), dim3(256),
-                         0, s, ns, n_reg, ctx->bbox.as<double>(), bg,
-                         ctx->reg_off.as<int64_t>(),
+                         0, s, ns, n_reg, ctx->bbox.as<double>(), inst_d,
+                         site_inst_d, ctx->reg_off.as<int64_t>(),
                          ctx->keys_a.as<uint64_t>());
       HIP_TRY(hipGetLastError());
       int end_bit = 32;
@@ -437,7 +583,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     HIP_TRY(ctx->tet_pass.need(((size_t)nt + 1) * 8));
     HIP_TRY(hipMemsetAsync(ctx->tet_count.as<int64_t>() + nt, 0, 8, s));
     hipLaunchKernelGGL(k_tet_count, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
-                       M, S, bg, ctx->bin_start.as<int32_t>(),
+                       M, S, inst_d, tet_inst_d, ctx->bin_start.as<int32_t>(),
                        ctx->bin_ids.as<int32_t>(),
                        ctx->tet_count.as<int64_t>(),
                        ctx->tet_pass.as<uint64_t>(),
@@ -455,7 +601,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   HIP_TRY(ctx->cand.need((size_t)std::max<int64_t>(n_cand, 1) * sizeof(Cand)));
   HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
   hipLaunchKernelGGL(k_tet_fill, dim3(ceil_div(nt, 256)), dim3(256), 0, s, M,
-                     S, bg, ctx->bin_start.as<int32_t>(),
+                     S, inst_d, tet_inst_d, ctx->bin_start.as<int32_t>(),
                      ctx->bin_ids.as<int32_t>(), ctx->tet_off.as<int64_t>(),
                      ctx->tet_pass.as<uint64_t>(), ctx->cand.as<Cand>(),
                      ctx->rel_ids.as<int32_t>());

@@ -136,6 +136,16 @@ struct BinGrid {
   int32_t pad;
 };
 
+// One instance of a batch: its domain box, the largest distance of a tet
+// corner from its tet's centroid (rho, with margin), and its candidate bin
+// grid, whose bins are numbered from bin_base in the batch-wide bin space.
+struct InstInfo {
+  BinGrid g;
+  int64_t bin_base;
+  double dom[6];
+  double rho;
+};
+
 __device__ __forceinline__ int bin_coord(double x, double org, double inv,
                                          int dim) {
   double f = floor((x - org) * inv);
@@ -156,8 +166,8 @@ constexpr int kBoxP = 128;      // 6 box planes + 122 neighbours
 constexpr int kNearMax = 128;   // longest list ranked (else kept as is)
 
 __global__ __launch_bounds__(64) void k_site_bbox(DevSites S,
-                                                  const double* dom,  // 6
-                                                  double rho,
+                                                  const InstInfo* inst,
+                                                  const int32_t* site_inst,
                                                   double* bbox,  // 6/site
                                                   int32_t* knn_near,
                                                   double4* knn_site_near,
@@ -172,6 +182,9 @@ __global__ __launch_bounds__(64) void k_site_bbox(DevSites S,
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
   if (i >= S.n_site) return;
+  const InstInfo& I = inst[site_inst[i]];
+  const double* dom = I.dom;
+  const double rho = I.rho;
   double* out = bbox + 6 * (int64_t)i;
   if (S.flags[i] != 1u) {
     if (lane == 0) {
@@ -380,7 +393,9 @@ __device__ __forceinline__ bool bin_range(const double* b, const BinGrid& g,
 // number of bins each sphere registers into
 __global__ __launch_bounds__(256) void k_bin_nreg(int32_t n_site,
                                                   const double* bbox,
-                                                  BinGrid g, int64_t* nreg) {
+                                                  const InstInfo* inst,
+                                                  const int32_t* site_inst,
+                                                  int64_t* nreg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n_site) return;
   if (i == n_site) {
@@ -388,7 +403,7 @@ __global__ __launch_bounds__(256) void k_bin_nreg(int32_t n_site,
     return;
   }
   int r[6];
-  nreg[i] = bin_range(bbox + 6 * (int64_t)i, g, r)
+  nreg[i] = bin_range(bbox + 6 * (int64_t)i, inst[site_inst[i]].g, r)
                 ? (int64_t)(r[3] - r[0] + 1) * (r[4] - r[1] + 1) *
                       (r[5] - r[2] + 1)
                 : 0;
@@ -398,7 +413,8 @@ __global__ __launch_bounds__(256) void k_bin_nreg(int32_t n_site,
 __global__ __launch_bounds__(256) void k_bin_expand(int32_t n_site,
                                                     int64_t n_reg,
                                                     const double* bbox,
-                                                    BinGrid g,
+                                                    const InstInfo* inst,
+                                                    const int32_t* site_inst,
                                                     const int64_t* reg_off,
                                                     uint64_t* keys) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -413,13 +429,16 @@ __global__ __launch_bounds__(256) void k_bin_expand(int32_t n_site,
       hi = mid;
   }
   const int i = lo;
+  const InstInfo& I = inst[site_inst[i]];
+  const BinGrid& g = I.g;
   int r[6];
   bin_range(bbox + 6 * (int64_t)i, g, r);
   const int64_t k = q - reg_off[i];
   const int64_t nx = r[3] - r[0] + 1, ny = r[4] - r[1] + 1;
   const int64_t x = r[0] + k % nx, y = r[1] + (k / nx) % ny,
                 z = r[2] + k / (nx * ny);
-  const uint64_t bin = x + (uint64_t)g.dim[0] * (y + (uint64_t)g.dim[1] * z);
+  const uint64_t bin = (uint64_t)I.bin_base + x +
+                       (uint64_t)g.dim[0] * (y + (uint64_t)g.dim[1] * z);
   keys[q] = (bin << 32) | (uint32_t)i;
 }
 
@@ -500,7 +519,8 @@ __device__ __forceinline__ int64_t tet_bin(const double c[3],
 }
 
 __global__ __launch_bounds__(256) void k_tet_count(DevMesh M, DevSites S,
-                                                   BinGrid g,
+                                                   const InstInfo* inst,
+                                                   const int32_t* tet_inst,
                                                    const int32_t* bin_start,
                                                    const int32_t* bin_ids,
                                                    int64_t* tet_count,
@@ -512,7 +532,8 @@ __global__ __launch_bounds__(256) void k_tet_count(DevMesh M, DevSites S,
   load_tet(M, t, L);
   double c[3];
   tet_centroid(L, c);
-  const int64_t b = tet_bin(c, g);
+  const InstInfo& I = inst[tet_inst[t]];
+  const int64_t b = I.bin_base + tet_bin(c, I.g);
   int64_t n = 0;
   uint64_t pass = 0;  // bit r: the r-th site of the bin list relates
   uint32_t iters = 0;
@@ -536,8 +557,8 @@ __global__ __launch_bounds__(256) void k_tet_count(DevMesh M, DevSites S,
 }
 
 __global__ __launch_bounds__(256) void k_tet_fill(
-    DevMesh M, DevSites S, BinGrid g, const int32_t* bin_start,
-    const int32_t* bin_ids, const int64_t* tet_off,
+    DevMesh M, DevSites S, const InstInfo* inst, const int32_t* tet_inst,
+    const int32_t* bin_start, const int32_t* bin_ids, const int64_t* tet_off,
     const uint64_t* tet_pass, Cand* cand, int32_t* rel_ids) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= M.n_tet) return;
@@ -545,7 +566,8 @@ __global__ __launch_bounds__(256) void k_tet_fill(
   load_tet(M, t, L);
   double c[3];
   tet_centroid(L, c);
-  const int64_t b = tet_bin(c, g);
+  const InstInfo& I = inst[tet_inst[t]];
+  const int64_t b = I.bin_base + tet_bin(c, I.g);
   int64_t o = tet_off[t];
   const uint64_t pass = tet_pass[t];
   const int q0 = bin_start[b];

@@ -141,6 +141,69 @@ def build_instance(verts, tets, xyzr, box=UNIT_BOX, n_threads=0):
                     np.ones(keep.size, np.uint32), knn_off, knn_ids, keep)
 
 
+def sites_of(xyzr, box=UNIT_BOX, n_threads=0):
+    """Sites (non-hidden spheres in ascending order, w = r^2), flags and
+    CSR neighbour lists of a sphere set: the per-iteration host step of
+    sphere optimisation.  Returns (sites, flags, knn_off, knn_ids, keep)."""
+    xyzw = np.ascontiguousarray(xyzr, np.float64).copy()
+    xyzw[:, 3] = xyzw[:, 3] * xyzw[:, 3]
+    valid, off, ids = power_neighbors(xyzw, box, n_threads)
+    keep = np.nonzero(valid)[0].astype(np.int32)
+    remap = -np.ones(xyzw.shape[0], np.int64)
+    remap[keep] = np.arange(keep.size)
+    deg = np.diff(off)[keep]
+    knn_off = np.zeros(keep.size + 1, np.int64)
+    np.cumsum(deg, out=knn_off[1:])
+    sel = np.concatenate([np.arange(off[i], off[i + 1]) for i in keep]) \
+        if keep.size else np.zeros(0, np.int64)
+    knn_ids = remap[ids[sel]].astype(np.int32) if sel.size else \
+        np.zeros(0, np.int32)
+    assert (knn_ids >= 0).all()
+    return (np.ascontiguousarray(xyzw[keep]), np.ones(keep.size, np.uint32),
+            knn_off, knn_ids, keep)
+
+
+@dataclass
+class BatchWorkload:
+    """Independent RPD instances of uneven size and their sphere sets."""
+    instances: list
+    spheres: list     # (n_sphere, 4) x, y, z, r per instance
+    grid_n: list      # realised grid resolution per instance
+
+
+def batch_workload(n_inst=256, seed=0, tets=(2000, 200000),
+                   n_spheres=(100, 20000), n_threads=0):
+    """Tets log-uniform in ``tets`` (a jittered n^3 Kuhn grid with
+    n = round((T/6)^(1/3)), so 6 n^3 tets), spheres log-uniform in
+    ``n_spheres`` with centres U(0,1)^3 and radii U[0.02, 0.08]; sizes drawn
+    by numpy's PCG64 from ``seed``, geometry by the project generator with
+    seed ``seed * 100003 + instance``."""
+    rng = np.random.default_rng(seed)
+    insts, sph, ns = [], [], []
+    for b in range(n_inst):
+        t = np.exp(rng.uniform(np.log(tets[0]), np.log(tets[1])))
+        m = int(round(np.exp(rng.uniform(np.log(n_spheres[0]),
+                                         np.log(n_spheres[1])))))
+        n = max(1, int(round((t / 6.0) ** (1.0 / 3.0))))
+        s = seed * 100003 + b
+        verts, tt = grid_mesh(n, 0.1, s)
+        xyzr = spheres(m, 0, s, 0.02, 0.08)
+        insts.append(build_instance(verts, tt, xyzr, n_threads=n_threads))
+        sph.append(xyzr)
+        ns.append(n)
+    return BatchWorkload(insts, sph, ns)
+
+
+def jitter_spheres(xyzr, sigma, seed):
+    """Sphere centres moved by N(0, sigma^2) per coordinate (numpy PCG64
+    from ``seed``), kept inside the open unit cube; radii unchanged."""
+    rng = np.random.default_rng(seed)
+    out = np.array(xyzr, np.float64, copy=True)
+    out[:, :3] = np.clip(out[:, :3] + rng.normal(0.0, sigma, (out.shape[0], 3)),
+                         1e-6, 1 - 1e-6)
+    return out
+
+
 def cube10_instance(seed=0, n_threads=0):
     """10^3 Kuhn grid (6,000 tets), 500 spheres r ~ U[0.02, 0.08]."""
     verts, tets = grid_mesh(10, 0.0, seed)

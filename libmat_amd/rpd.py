@@ -34,6 +34,12 @@ class InstanceDesc(ctypes.Structure):
          "flags", "knn_off", "knn_ids")]
 
 
+class SitesDesc(ctypes.Structure):
+    _fields_ = [("n_site", ctypes.c_int32), ("sites", ctypes.c_void_p),
+                ("flags", ctypes.c_void_p), ("knn_off", ctypes.c_void_p),
+                ("knn_ids", ctypes.c_void_p)]
+
+
 class RunCfg(ctypes.Structure):
     _fields_ = [("cells_per_wave", ctypes.c_int32),
                 ("use_active_mask", ctypes.c_int32),
@@ -85,6 +91,14 @@ def _declare_capi(L):
     L.rpd_upload.restype = ctypes.c_int
     L.rpd_update_sites.argtypes = [vp, i32, vp, vp, vp, vp]
     L.rpd_update_sites.restype = ctypes.c_int
+    L.rpd_upload_batch.argtypes = [vp, i32, ctypes.POINTER(InstanceDesc)]
+    L.rpd_upload_batch.restype = ctypes.c_int
+    L.rpd_update_sites_batch.argtypes = [vp, i32, ctypes.POINTER(SitesDesc)]
+    L.rpd_update_sites_batch.restype = ctypes.c_int
+    L.rpd_batch_offsets.argtypes = [vp, vp, vp]
+    L.rpd_batch_offsets.restype = ctypes.c_int
+    L.rpd_batch_size.argtypes = [vp]
+    L.rpd_batch_size.restype = i32
     L.rpd_run.argtypes = [vp, ctypes.POINTER(RunCfg)]
     L.rpd_run.restype = ctypes.c_int
     L.rpd_sync.argtypes = [vp]
@@ -100,7 +114,9 @@ def _declare_capi(L):
 
 
 CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
-                "rpd_ctx_stream", "rpd_upload", "rpd_update_sites", "rpd_run",
+                "rpd_ctx_stream", "rpd_upload", "rpd_update_sites",
+                "rpd_upload_batch", "rpd_update_sites_batch",
+                "rpd_batch_offsets", "rpd_batch_size", "rpd_run",
                 "rpd_sync", "rpd_get_stats", "rpd_download_cells",
                 "rpd_download_relation", "rpd_calib_copy")
 
@@ -167,8 +183,9 @@ class Context:
     def stream(self):
         return self._L.rpd_ctx_stream(self._h)
 
-    def upload(self, inst):
-        arrs = dict(
+    @staticmethod
+    def _arrays(inst):
+        return dict(
             verts=np.ascontiguousarray(inst.verts, np.float64),
             tets=np.ascontiguousarray(inst.tets, np.int32),
             v_adj=np.ascontiguousarray(inst.v_adj, np.int32),
@@ -179,13 +196,40 @@ class Context:
             flags=np.ascontiguousarray(inst.flags, np.uint32),
             knn_off=np.ascontiguousarray(inst.knn_off, np.int64),
             knn_ids=np.ascontiguousarray(inst.knn_ids, np.int32))
-        d = InstanceDesc(inst.n_vert, inst.n_tet, inst.n_site,
-                         *[_p(arrs[n]) for n in
-                           ("verts", "tets", "v_adj", "e_adj", "f_adj",
-                            "f_id", "sites", "flags", "knn_off", "knn_ids")])
+
+    @staticmethod
+    def _desc(inst, a):
+        return InstanceDesc(inst.n_vert, inst.n_tet, inst.n_site,
+                            *[_p(a[n]) for n in
+                              ("verts", "tets", "v_adj", "e_adj", "f_adj",
+                               "f_id", "sites", "flags", "knn_off",
+                               "knn_ids")])
+
+    def upload(self, inst):
+        a = self._arrays(inst)
+        d = self._desc(inst, a)
         self._check(self._L.rpd_upload(self._h, ctypes.byref(d)),
                     "rpd_upload")
         self.n_tet = inst.n_tet
+
+    def upload_batch(self, insts):
+        """Make a list of instances resident together; returns the
+        batch-wide tet and site offsets (n_inst + 1 each)."""
+        arrs = [self._arrays(i) for i in insts]
+        ds = (InstanceDesc * len(insts))(
+            *[self._desc(i, a) for i, a in zip(insts, arrs)])
+        self._check(self._L.rpd_upload_batch(self._h, len(insts), ds),
+                    "rpd_upload_batch")
+        self.n_tet = sum(i.n_tet for i in insts)
+        return self.batch_offsets()
+
+    def batch_offsets(self):
+        n = self._L.rpd_batch_size(self._h)
+        to = np.empty(n + 1, np.int64)
+        so = np.empty(n + 1, np.int64)
+        self._check(self._L.rpd_batch_offsets(self._h, _p(to), _p(so)),
+                    "rpd_batch_offsets")
+        return to, so
 
     def update_sites(self, sites, flags, knn_off, knn_ids):
         sites = np.ascontiguousarray(sites, np.float64)
@@ -195,6 +239,17 @@ class Context:
         self._check(self._L.rpd_update_sites(
             self._h, sites.shape[0], _p(sites), _p(flags), _p(knn_off),
             _p(knn_ids)), "rpd_update_sites")
+
+    def update_sites_batch(self, site_sets):
+        """site_sets: one (sites, flags, knn_off, knn_ids) per instance."""
+        keep = [tuple(np.ascontiguousarray(x, t) for x, t in
+                      zip(ss, (np.float64, np.uint32, np.int64, np.int32)))
+                for ss in site_sets]
+        ds = (SitesDesc * len(keep))(
+            *[SitesDesc(k[0].shape[0], _p(k[0]), _p(k[1]), _p(k[2]),
+                        _p(k[3])) for k in keep])
+        self._check(self._L.rpd_update_sites_batch(self._h, len(keep), ds),
+                    "rpd_update_sites_batch")
 
     def run(self, cells_per_wave=8, use_active_mask=True, geometry=True,
             force_exact=False, small_tier=True, count_work=False):

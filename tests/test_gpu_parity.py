@@ -168,3 +168,87 @@ def test_work_counters_match_the_oracle(ctx):
     assert w2[0][0] + w2[1][0] < n_det
     assert 0 < st2["n_active_planes"] < st2["n_all_planes"]
     compare_with_oracle(ctx.download(), orc)
+
+
+def _instance_slice(res, off, ids, to, so, b):
+    """Cells and relation of instance b of a batched run, in local ids."""
+    t0, t1 = int(to[b]), int(to[b + 1])
+    roff = off[t0:t1 + 1] - off[t0]
+    rids = ids[off[t0]:off[t1]] - so[b]
+    sel = np.nonzero((res.cells["tet"] >= t0) & (res.cells["tet"] < t1))[0]
+    cells = res.cells[sel]
+    words, geom, areas = [], res.geom[sel], []
+    for q in sel:
+        c = res.cells[q]
+        o, nv, np_, ne = (int(c["word_off"]), int(c["nb_v"]), int(c["nb_p"]),
+                          int(c["nb_e"]))
+        w = res.words[o:o + nv + np_ + ne].copy()
+        src = w[nv:nv + np_].view(np.int32)
+        src[src >= 0] -= so[b]
+        words.append(w)
+        a = int(c["area_off"])
+        areas.append(res.areas[a:a + np_])
+    return (roff, rids, cells["tet"] - t0, cells["seed"] - so[b],
+            _counts(cells), words, geom, areas)
+
+
+def _counts(cells):
+    return np.stack([cells[f].astype(np.int64)
+                     for f in ("nb_v", "nb_p", "nb_e", "status")])
+
+
+def _single_arrays(res, off, ids):
+    words, areas = [], []
+    for q in range(len(res)):
+        c = res.cells[q]
+        o, n = int(c["word_off"]), int(c["nb_v"]) + int(c["nb_p"]) + int(c["nb_e"])
+        words.append(res.words[o:o + n])
+        a = int(c["area_off"])
+        areas.append(res.areas[a:a + int(c["nb_p"])])
+    return (off, ids, res.cells["tet"], res.cells["seed"],
+            _counts(res.cells), words, res.geom, areas)
+
+
+def _same(a, b):
+    for x, y in zip(a, b):
+        if isinstance(x, list):
+            assert len(x) == len(y)
+            for u, v in zip(x, y):
+                assert np.array_equal(u, v)
+        else:
+            assert np.array_equal(x, y)
+
+
+def test_batch_matches_single_instance_runs(ctx):
+    from dataclasses import replace
+    from libmat_amd.rpd import Context
+    w = prep.batch_workload(10, 3, (500, 6000), (50, 800))
+    to, so = ctx.upload_batch(w.instances)
+    print(f"[batch] tets {np.diff(to).tolist()} sites {np.diff(so).tolist()}",
+          flush=True)
+    single = Context(0)
+    insts = list(w.instances)
+    for it in range(2):
+        st = ctx.run()
+        res = ctx.download()
+        off, ids = ctx.relation()
+        assert st["n_cells"] == len(res)
+        for b, inst in enumerate(insts):
+            single.upload(inst)
+            single.run()
+            ra = _single_arrays(single.download(), *single.relation())
+            _same(_instance_slice(res, off, ids, to, so, b), ra)
+        print(f"[batch] iteration {it}: {len(res)} cells identical to the "
+              f"single-instance runs", flush=True)
+        # next sphere-update iteration: centres jittered, host neighbours
+        sets, new = [], []
+        for b, inst in enumerate(insts):
+            xyzr = prep.jitter_spheres(w.spheres[b], 0.002, 1000 + b)
+            sites, flags, koff, kids, keep = prep.sites_of(xyzr)
+            sets.append((sites, flags, koff, kids))
+            new.append(replace(inst, sites=sites, flags=flags, knn_off=koff,
+                               knn_ids=kids, site2sphere=keep))
+        ctx.update_sites_batch(sets)
+        to, so = ctx.batch_offsets()
+        insts = new
+    single.close()
