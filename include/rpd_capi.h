@@ -67,7 +67,8 @@ typedef struct {
 
 typedef struct {
   int32_t cells_per_wave;   /* cells clipped concurrently per wavefront:
-                               1, 2, 4, 8 or 16 (0 = default)          */
+                               1, 2, 4, 8 or 16; 0 = the rpd_tune choice
+                               (with its small_caps) or 8              */
   int32_t use_active_mask;  /* 1: skip neighbour planes proved not to cut
                                the cell's tet (results are identical)  */
   int32_t compute_geometry; /* 1: volume, centroid, facet areas (FP64) */
@@ -78,7 +79,9 @@ typedef struct {
   int32_t count_work;       /* 1: count the arithmetic the clip kernels
                                execute into rpd_stats.work (adds atomics:
                                for roofline figures, not for timing)    */
-  int32_t reserved[2];
+  int32_t small_caps;       /* capacities (vertices/planes/edges) of the
+                               small tier: 0 = 32/16/48, 1 = 48/24/72  */
+  int32_t reserved;
 } rpd_run_cfg;
 
 /* One emitted (non-empty) cell.  Its payload words start at word_off:
@@ -176,6 +179,15 @@ int rpd_update_sites(rpd_ctx* ctx, int32_t n_site, const double* sites,
  * results are complete (it reads sizes back between phases); rpd_sync waits
  * for everything enqueued on the context stream. */
 int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg);
+
+/* Autotuner (no reference counterpart): runs the pipeline on the resident
+ * batch for every clip shape -- 4, 8 or 16 cells per wavefront and small-
+ * tier capacities 32/16/48 or 48/24/72 -- times the clip phases (clip plus
+ * re-runs; best of two after a warm-up) and keeps the fastest for runs
+ * with cells_per_wave = 0.  The other fields of base (or the defaults) are
+ * used as given.  ms_out[3 * caps + shape] receives the times. */
+int rpd_tune(rpd_ctx* ctx, const rpd_run_cfg* base, int32_t* best_G,
+             int32_t* best_caps, double* ms_out);
 int rpd_sync(rpd_ctx* ctx);
 
 int rpd_get_stats(const rpd_ctx* ctx, rpd_stats* out);

@@ -49,6 +49,7 @@ struct Caps {
   static constexpr int V = V_, P = P_, E = E_;
 };
 using SmallCaps = Caps<32, 16, 48>;
+using WideCaps = Caps<48, 24, 72>;   // small tier for larger cells
 using LargeCaps = Caps<96, 64, 152>;
 
 // Output slot of one cell (indexed by cell or by re-run index).

@@ -96,6 +96,7 @@ struct rpd_ctx {
   TierBufs small, large, exact;
   Buf out_cells, out_words, out_geom, out_areas;
   rpd_stats st;
+  int tuned_G = 0, tuned_caps = 0;  // rpd_tune's choice (0: not tuned)
   hipEvent_t ev[16];
   bool events = false;
 };
@@ -410,19 +411,39 @@ int32_t rpd_batch_size(const rpd_ctx* ctx) {
 }  // extern "C"
 
 namespace {
-static int launch_clip(rpd_ctx* ctx, int tier, int G, ClipArgs& a) {
+// capacities of the small tier: 0 = SmallCaps, 1 = WideCaps
+struct CapsRT {
+  int V, P, E;
+};
+CapsRT small_caps(int id) {
+  return id == 1 ? CapsRT{WideCaps::V, WideCaps::P, WideCaps::E}
+                 : CapsRT{SmallCaps::V, SmallCaps::P, SmallCaps::E};
+}
+
+static int launch_clip(rpd_ctx* ctx, int tier, int G, ClipArgs& a,
+                       int caps_id = 0) {
   if (a.n_items <= 0) return 0;
   const int blocks = ceil_div(a.n_items, G);
 #define RPD_LAUNCH(GG, CAPS, EX)                                       \
   hipLaunchKernelGGL((k_clip<GG, CAPS, EX>), dim3(blocks), dim3(64), 0, \
                      ctx->stream, a)
   if (tier == 0) {
-    switch (G) {
-      case 1: RPD_LAUNCH(1, SmallCaps, false); break;
-      case 2: RPD_LAUNCH(2, SmallCaps, false); break;
-      case 4: RPD_LAUNCH(4, SmallCaps, false); break;
-      case 8: RPD_LAUNCH(8, SmallCaps, false); break;
-      default: RPD_LAUNCH(16, SmallCaps, false); break;
+    if (caps_id == 1) {
+      switch (G) {
+        case 1: RPD_LAUNCH(1, WideCaps, false); break;
+        case 2: RPD_LAUNCH(2, WideCaps, false); break;
+        case 4: RPD_LAUNCH(4, WideCaps, false); break;
+        case 8: RPD_LAUNCH(8, WideCaps, false); break;
+        default: RPD_LAUNCH(16, WideCaps, false); break;
+      }
+    } else {
+      switch (G) {
+        case 1: RPD_LAUNCH(1, SmallCaps, false); break;
+        case 2: RPD_LAUNCH(2, SmallCaps, false); break;
+        case 4: RPD_LAUNCH(4, SmallCaps, false); break;
+        case 8: RPD_LAUNCH(8, SmallCaps, false); break;
+        default: RPD_LAUNCH(16, SmallCaps, false); break;
+      }
     }
   } else if (tier == 1) {
     switch (G) {
@@ -448,15 +469,19 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     return fail(ctx, "rpd_run: no resident instance");
   HIP_TRY(hipSetDevice(ctx->device));
   rpd_run_cfg cfg{};
-  cfg.cells_per_wave = 8;
+  cfg.cells_per_wave = 0;
   cfg.use_active_mask = 1;
   cfg.compute_geometry = 1;
   cfg.force_exact = 0;
   cfg.small_tier = 1;
-  if (cfg_in) {
-    cfg = *cfg_in;
-    if (cfg.cells_per_wave <= 0) cfg.cells_per_wave = 8;
+  if (cfg_in) cfg = *cfg_in;
+  if (cfg.cells_per_wave <= 0) {  // tuned shape, else the default
+    cfg.cells_per_wave = ctx->tuned_G > 0 ? ctx->tuned_G : 8;
+    if (ctx->tuned_G > 0) cfg.small_caps = ctx->tuned_caps;
   }
+  if (cfg.small_caps != 0 && cfg.small_caps != 1)
+    return fail(ctx, "rpd_run: small_caps must be 0 or 1");
+  const CapsRT sc = small_caps(cfg.small_caps);
   const int G = cfg.cells_per_wave;
   if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16)
     return fail(ctx, "rpd_run: cells_per_wave must be 1, 2, 4, 8 or 16");
@@ -635,11 +660,11 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   }
   const int64_t n_first = n_cand;
   if (cfg.small_tier) {
-    HIP_TRY(ctx->small.need(n_first, SmallCaps::V, SmallCaps::P, SmallCaps::E));
+    HIP_TRY(ctx->small.need(n_first, sc.V, sc.P, sc.E));
     a.items = nullptr;
     a.n_items = n_first;
-    a.out = ctx->small.view(SmallCaps::V, SmallCaps::P, SmallCaps::E);
-    if (int rc = launch_clip(ctx, 0, G, a)) return rc;
+    a.out = ctx->small.view(sc.V, sc.P, sc.E);
+    if (int rc = launch_clip(ctx, 0, G, a, cfg.small_caps)) return rc;
   } else {
     // every cell goes straight to the full-capacity tier through the
     // re-run list (identity)
@@ -650,7 +675,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
                              hipMemcpyHostToDevice, s));
     int32_t nf = (int32_t)n_first;
     HIP_TRY(hipMemcpyAsync(cnt + 1, &nf, 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(ctx->small.need(1, SmallCaps::V, SmallCaps::P, SmallCaps::E));
+    HIP_TRY(ctx->small.need(1, sc.V, sc.P, sc.E));
   }
   HIP_TRY(hipEventRecord(ctx->ev[2], s));
   int32_t host_cnt[8];
@@ -690,7 +715,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
                        0, s, ctx->rerun_exact.as<int32_t>(), n_exact, 2,
                        ctx->loc.as<int32_t>());
   TierViews tv;
-  tv.t[0] = ctx->small.view(SmallCaps::V, SmallCaps::P, SmallCaps::E);
+  tv.t[0] = ctx->small.view(sc.V, sc.P, sc.E);
   tv.t[1] = ctx->large.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
   tv.t[2] = ctx->exact.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
   HIP_TRY(ctx->sizes.need(((size_t)n_cand + 1) * sizeof(Sizes3)));
@@ -836,6 +861,40 @@ int rpd_calib_copy(rpd_ctx* ctx, int64_t bytes) {
   (void)hipFree(a);
   (void)hipFree(b);
   if (e != hipSuccess) return fail(ctx, "rpd_calib_copy failed", -2);
+  return 0;
+}
+
+int rpd_tune(rpd_ctx* ctx, const rpd_run_cfg* base, int32_t* best_G,
+             int32_t* best_caps, double* ms_out /* 6, or NULL */) {
+  if (!ctx) return -1;
+  rpd_run_cfg cfg{};
+  cfg.use_active_mask = 1;
+  cfg.compute_geometry = 1;
+  cfg.small_tier = 1;
+  if (base) cfg = *base;
+  static const int kG[3] = {4, 8, 16};
+  double best = 1e300;
+  int bg = 8, bc = 0;
+  for (int c = 0; c < 2; ++c)
+    for (int q = 0; q < 3; ++q) {
+      cfg.cells_per_wave = kG[q];
+      cfg.small_caps = c;
+      double t = 1e300;
+      for (int rep = 0; rep < 3; ++rep) {  // first run warms the buffers
+        if (int rc = rpd_run(ctx, &cfg)) return rc;
+        if (rep) t = std::min(t, ctx->st.ms_phase[1] + ctx->st.ms_phase[2]);
+      }
+      if (ms_out) ms_out[3 * c + q] = t;
+      if (t < best) {
+        best = t;
+        bg = kG[q];
+        bc = c;
+      }
+    }
+  ctx->tuned_G = bg;
+  ctx->tuned_caps = bc;
+  if (best_G) *best_G = bg;
+  if (best_caps) *best_caps = bc;
   return 0;
 }
 

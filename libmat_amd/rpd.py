@@ -47,7 +47,8 @@ class RunCfg(ctypes.Structure):
                 ("force_exact", ctypes.c_int32),
                 ("small_tier", ctypes.c_int32),
                 ("count_work", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("small_caps", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -99,6 +100,8 @@ def _declare_capi(L):
     L.rpd_batch_offsets.restype = ctypes.c_int
     L.rpd_batch_size.argtypes = [vp]
     L.rpd_batch_size.restype = i32
+    L.rpd_tune.argtypes = [vp, vp, vp, vp, vp]
+    L.rpd_tune.restype = ctypes.c_int
     L.rpd_run.argtypes = [vp, ctypes.POINTER(RunCfg)]
     L.rpd_run.restype = ctypes.c_int
     L.rpd_sync.argtypes = [vp]
@@ -116,7 +119,7 @@ def _declare_capi(L):
 CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
                 "rpd_ctx_stream", "rpd_upload", "rpd_update_sites",
                 "rpd_upload_batch", "rpd_update_sites_batch",
-                "rpd_batch_offsets", "rpd_batch_size", "rpd_run",
+                "rpd_batch_offsets", "rpd_batch_size", "rpd_tune", "rpd_run",
                 "rpd_sync", "rpd_get_stats", "rpd_download_cells",
                 "rpd_download_relation", "rpd_calib_copy")
 
@@ -251,10 +254,22 @@ class Context:
         self._check(self._L.rpd_update_sites_batch(self._h, len(keep), ds),
                     "rpd_update_sites_batch")
 
+    def tune(self):
+        """Pick the fastest clip shape for the resident batch (used by
+        runs with cells_per_wave=0); returns (G, caps, {(caps, G): ms})."""
+        g, c = ctypes.c_int32(), ctypes.c_int32()
+        ms = np.zeros(6)
+        self._check(self._L.rpd_tune(self._h, None, ctypes.byref(g),
+                                     ctypes.byref(c), _p(ms)), "rpd_tune")
+        return g.value, c.value, {(cc, G): ms[3 * cc + q] for cc in (0, 1)
+                                  for q, G in enumerate((4, 8, 16))}
+
     def run(self, cells_per_wave=8, use_active_mask=True, geometry=True,
-            force_exact=False, small_tier=True, count_work=False):
+            force_exact=False, small_tier=True, count_work=False,
+            small_caps=0):
         cfg = RunCfg(cells_per_wave, int(use_active_mask), int(geometry),
-                     int(force_exact), int(small_tier), int(count_work))
+                     int(force_exact), int(small_tier), int(count_work),
+                     int(small_caps))
         self._check(self._L.rpd_run(self._h, ctypes.byref(cfg)), "rpd_run")
         return self.stats()
 

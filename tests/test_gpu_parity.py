@@ -252,3 +252,16 @@ def test_batch_matches_single_instance_runs(ctx):
         to, so = ctx.batch_offsets()
         insts = new
     single.close()
+
+
+def test_tune_picks_a_shape_with_identical_results(ctx):
+    inst, orc = c1(2)
+    ctx.upload(inst)
+    G, caps, ms = ctx.tune()
+    print(f"[tune] G={G} caps={caps} ms={ms}", flush=True)
+    assert G in (4, 8, 16) and caps in (0, 1)
+    assert all(v > 0 for v in ms.values())
+    ctx.run(cells_per_wave=0)
+    compare_with_oracle(ctx.download(), orc)
+    ctx.run(cells_per_wave=16, small_caps=1)
+    compare_with_oracle(ctx.download(), orc)
