@@ -81,7 +81,16 @@ typedef struct {
                                for roofline figures, not for timing)    */
   int32_t small_caps;       /* capacities (vertices/planes/edges) of the
                                small tier: 0 = 32/16/48, 1 = 48/24/72  */
-  int32_t reserved;
+  int32_t learned;          /* 1: per-tet sphere sets proposed by the
+                               context's model (rpd_mlp_*), certified,
+                               failing tets re-run on the full relation;
+                               2: the same pipeline with the first
+                               proposal_k proposals by power distance
+                               instead of the model; the cells are those
+                               of the plain path either way            */
+  float logit_threshold;    /* learned = 1: a proposal is clipped if its
+                               logit exceeds this (the owner always)    */
+  int32_t proposal_k;       /* learned = 2: proposals taken per tet     */
 } rpd_run_cfg;
 
 /* One emitted (non-empty) cell.  Its payload words start at word_off:
@@ -131,6 +140,9 @@ typedef struct {
   int64_t relation_work[4];   /* with count_work: (tet, site) pairs of the
                                  candidate bin lists tested, neighbour
                                  tests of the relation, 0, related pairs */
+  int64_t learned[4];         /* learned runs: proposals clipped, tets the
+                                 certificate rejected (re-run on the full
+                                 relation), their candidates, 0          */
 } rpd_stats;
 
 typedef struct rpd_ctx rpd_ctx;
@@ -196,6 +208,48 @@ int rpd_get_stats(const rpd_ctx* ctx, rpd_stats* out);
  * streaming kernel, whose HBM traffic (bytes read, bytes written) is known;
  * used to calibrate hardware traffic counters. */
 int rpd_calib_copy(rpd_ctx* ctx, int64_t bytes);
+
+/* ---- learned proposals (no reference counterpart; the reference gathers
+ * every related sphere).  Model: 48 -> 128 -> 128 -> 32, ReLU, BF16 MFMA
+ * with FP32 accumulation; RPD_MLP_NPARAM fp32 parameters in the order
+ * W1[48][128], b1[128], W2[128][128], b2[128], W3[128][32], b3[32]
+ * (row-major [in][out]). */
+#define RPD_MLP_IN 48
+#define RPD_MLP_HID 128
+#define RPD_MLP_OUT 32
+#define RPD_MLP_NPARAM 26912
+
+typedef struct {
+  int32_t epochs;
+  int32_t batch;      /* rows per step, multiple of 16 */
+  float lr;           /* Adam step size (betas 0.9, 0.999, eps 1e-8) */
+  float pos_weight;   /* weight of positive labels in the loss */
+  uint64_t seed;      /* sample order */
+} rpd_train_cfg;
+
+/* Xavier-uniform initial weights (project RNG, seed), zero biases. */
+int rpd_mlp_init(rpd_ctx* ctx, uint64_t seed);
+int rpd_mlp_set(rpd_ctx* ctx, const float* params);
+int rpd_mlp_get(rpd_ctx* ctx, float* params);
+/* Proposals and features of every resident tet (device-resident for
+ * training and scoring); optional copies: knn[n_tet*32] (site ids, -1 =
+ * none; slot 0 the owner of the centroid), X[n_tet*48] (bf16 bits). */
+int rpd_features(rpd_ctx* ctx, int32_t* knn, uint16_t* X);
+/* Labels of the proposals from the last plain rpd_run of the resident
+ * batch: Y = 1 if the sphere has a cell in the tet, M = 1 for valid
+ * slots; optional copies Y, M [n_tet*32]. */
+int rpd_labels(rpd_ctx* ctx, float* Y, float* M);
+/* Trains the context's model on its resident features and labels;
+ * loss[epochs] receives the mean loss per epoch. */
+int rpd_mlp_train(rpd_ctx* ctx, const rpd_train_cfg* cfg, float* loss);
+/* Test hook: loads params as the model, then the gradient of the loss
+ * over n given rows (X bf16 bits [n*48], Y, M [n*32]) into grad, and the
+ * loss. */
+int rpd_mlp_grad(rpd_ctx* ctx, const float* params, int32_t n,
+                 const uint16_t* X, const float* Y, const float* M,
+                 float pos_weight, float* grad, float* loss);
+/* Logits of the model for the resident features: logits[n_tet*32]. */
+int rpd_mlp_scores(rpd_ctx* ctx, float* logits);
 
 /* D2H copies of the last run's results; NULL pointers are skipped.
  * cells[n_cells], words[n_words], geom[4*n_cells] (volume, centroid x,y,z),

@@ -17,7 +17,9 @@
 
 #include "rpd_capi.h"
 #include "rpd_device.h"
+#include "rpd_prep.h"
 #include "rpd_kernels.hip"
+#include "rpd_learn.hip"
 
 namespace {
 
@@ -38,6 +40,28 @@ struct Buf {
     hipError_t e = hipMalloc(&p, b);
     if (e == hipSuccess) cap = b;
     return e;
+  }
+  // grow to `bytes` keeping the first `keep` bytes (stream-ordered copy;
+  // the old block is freed after the stream has drained)
+  hipError_t need_keep(size_t bytes, size_t keep, hipStream_t s) {
+    if (bytes <= cap && p) return hipSuccess;
+    size_t b = std::max<size_t>(bytes + bytes / 4, 256);
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, b);
+    if (e != hipSuccess) return e;
+    if (p && keep) {
+      e = hipMemcpyAsync(q, p, std::min(keep, cap), hipMemcpyDeviceToDevice,
+                         s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) {
+        (void)hipFree(q);
+        return e;
+      }
+    }
+    if (p) (void)hipFree(p);
+    p = q;
+    cap = b;
+    return hipSuccess;
   }
   template <class T>
   T* as() const {
@@ -64,6 +88,17 @@ struct TierBufs {
     if ((e = words.need(n * (V + P + E) * 4)) != hipSuccess) return e;
     if ((e = geom.need(n * 4 * 8)) != hipSuccess) return e;
     return areas.need(n * P * 8);
+  }
+  hipError_t need_keep(int64_t n, int64_t keep, int V, int P, int E,
+                       hipStream_t s) {
+    hipError_t e;
+    n = std::max<int64_t>(n, 1);
+    if ((e = hdr.need_keep(n * 4, keep * 4, s)) != hipSuccess) return e;
+    if ((e = words.need_keep(n * (V + P + E) * 4, keep * (V + P + E) * 4,
+                             s)) != hipSuccess)
+      return e;
+    if ((e = geom.need_keep(n * 32, keep * 32, s)) != hipSuccess) return e;
+    return areas.need_keep(n * P * 8, keep * P * 8, s);
   }
 };
 
@@ -95,6 +130,21 @@ struct rpd_ctx {
   Buf scan_tmp;
   TierBufs small, large, exact;
   Buf out_cells, out_words, out_geom, out_areas;
+  int64_t n_knn = 0;
+  // learned proposals: features, model (fp32 master, Adam state, bf16
+  // copies), certificate and fallback
+  Buf knn, feat, sel, sg_keys_a, sg_keys_b, sg_start, sg_ids;
+  int32_t feat_n = 0;
+  Buf mlp_master, mlp_m1, mlp_m2, mlp_w1t, mlp_w2t, mlp_w3t, mlp_w2, mlp_w3,
+      mlp_bias;
+  bool mlp_ready = false;
+  int64_t adam_step = 0;
+  Buf lab_y, lab_m;
+  int32_t lab_n = 0;
+  Buf tr_idx, tr_xb, tr_h1, tr_h2, tr_dz1, tr_dz2, tr_dz3, tr_loss, tr_p1,
+      tr_p2, tr_p3, tr_grad, tr_x, tr_y, tr_mk;
+  Buf flag, has_cell, excl, flag32, flag_pos, flag_list, brute_n, brute_off,
+      vals_a, perm;
   rpd_stats st;
   int tuned_G = 0, tuned_caps = 0;  // rpd_tune's choice (0: not tuned)
   hipEvent_t ev[16];
@@ -230,6 +280,7 @@ int rpd_update_sites_batch(rpd_ctx* ctx, int32_t n_inst,
   }
   koff[ns_tot] = ko;
   ctx->n_site = (int32_t)ns_tot;
+  ctx->n_knn = nk_tot;
   HIP_TRY(ctx->site.need(std::max<int64_t>(ns_tot, 1) * 32));
   HIP_TRY(ctx->flags.need(std::max<int64_t>(ns_tot, 1) * 4));
   HIP_TRY(ctx->site_inst.need(std::max<int64_t>(ns_tot, 1) * 4));
@@ -459,6 +510,565 @@ static int launch_clip(rpd_ctx* ctx, int tier, int G, ClipArgs& a,
   return 0;
 }
 
+// ----------------------------------------------------------- run stages
+struct RunState {
+  rpd_run_cfg cfg;
+  CapsRT sc;
+  int G = 8;
+  hipStream_t s = nullptr;
+  int32_t nt = 0, ns = 0;
+  DevMesh M;
+  DevSites S;
+  int32_t* cnt = nullptr;  // int32[8]: bbox fallbacks, re-run counters
+  const InstInfo* inst_d = nullptr;
+  const int32_t* site_inst_d = nullptr;
+  const int32_t* tet_inst_d = nullptr;
+  int64_t n_bins = 0, n_sg = 0;
+  int32_t r_large = 0, r_exact = 0;  // re-run entries clipped so far
+};
+
+MlpDev mlp_dev(rpd_ctx* ctx) {
+  const float* b = ctx->mlp_bias.as<float>();
+  return MlpDev{ctx->mlp_w1t.as<uint16_t>(), ctx->mlp_w2t.as<uint16_t>(),
+                ctx->mlp_w3t.as<uint16_t>(), ctx->mlp_w2.as<uint16_t>(),
+                ctx->mlp_w3.as<uint16_t>(), b, b + kHid, b + 2 * kHid};
+}
+
+int upload_inst_info(rpd_ctx* ctx, RunState& R);
+
+// stream, sizes, device views and per-instance grids of the resident batch
+int init_state(rpd_ctx* ctx, RunState& R) {
+  R.s = ctx->stream;
+  R.nt = ctx->n_tet;
+  R.ns = ctx->n_site;
+  R.M = DevMesh{ctx->n_vert, R.nt, ctx->vert.as<double4>(),
+                ctx->tet_v.as<int4>(), ctx->tet_fid.as<int4>(),
+                ctx->tet_meta.as<TetMeta>()};
+  R.S = DevSites{R.ns, ctx->site.as<double4>(), ctx->flags.as<uint32_t>(),
+                 ctx->knn_off.as<int64_t>(), ctx->knn_ids.as<int32_t>(),
+                 ctx->knn_near.as<int32_t>(),
+                 ctx->knn_site_near.as<double4>()};
+  HIP_TRY(ctx->counters.need(8 * 4));
+  R.cnt = ctx->counters.as<int32_t>();
+  return upload_inst_info(ctx, R);
+}
+
+// per instance: candidate bin grid (about n_tet/6 bins) and sphere-centre
+// grid (about 2 spheres per cell), numbered batch-wide
+int upload_inst_info(rpd_ctx* ctx, RunState& R) {
+  std::vector<InstInfo> ii(ctx->inst.size());
+  int64_t n_bins = 0, n_sg = 0;
+  for (size_t b = 0; b < ctx->inst.size(); ++b) {
+    const rpd_ctx::Inst& I = ctx->inst[b];
+    InstInfo& o = ii[b];
+    double ext[3], vol = 1;
+    for (int a = 0; a < 3; ++a) {
+      ext[a] = std::max(I.dom[3 + a] - I.dom[a], 1e-300);
+      vol *= ext[a];
+    }
+    auto grid = [&](double per_cell_units, BinGrid& g) {
+      const double bs = std::cbrt(vol / std::max(1.0, per_cell_units));
+      int64_t nb = 1;
+      for (int a = 0; a < 3; ++a) {
+        g.org[a] = I.dom[a];
+        g.dim[a] = (int32_t)std::min<double>(
+            512, std::max(1.0, std::ceil(ext[a] / bs)));
+        nb *= g.dim[a];
+      }
+      g.inv_bs = 1.0 / bs;
+      g.pad = 0;
+      return nb;
+    };
+    const int64_t nb = grid(I.n_tet / 6.0, o.g);
+    o.bin_base = n_bins;
+    n_bins += nb;
+    const int64_t ng = grid(I.n_site / 2.0, o.sg);
+    o.sg_base = n_sg;
+    n_sg += ng;
+    for (int a = 0; a < 6; ++a) o.dom[a] = I.dom[a];
+    o.rho = I.rho;
+    o.site_off = (int32_t)I.site_off;
+    o.n_site = I.n_site;
+  }
+  if (n_bins >= (1LL << 31) || n_sg >= (1LL << 31))
+    return fail(ctx, "rpd_run: too many grid cells");
+  HIP_TRY(ctx->inst_dev.need(ii.size() * sizeof(InstInfo)));
+  HIP_TRY(hipMemcpyAsync(ctx->inst_dev.p, ii.data(),
+                         ii.size() * sizeof(InstInfo), hipMemcpyHostToDevice,
+                         R.s));
+  HIP_TRY(hipStreamSynchronize(R.s));  // the host vector dies here
+  R.n_bins = n_bins;
+  R.n_sg = n_sg;
+  R.inst_d = ctx->inst_dev.as<InstInfo>();
+  R.site_inst_d = ctx->site_inst.as<int32_t>();
+  R.tet_inst_d = ctx->tet_inst.as<int32_t>();
+  return 0;
+}
+
+int sort_keys(rpd_ctx* ctx, hipStream_t s, Buf& a, Buf& b, int64_t n,
+              int end_bit) {
+  if (n <= 0) return 0;
+  size_t tmp = 0;
+  HIP_TRY(hipcub::DeviceRadixSort::SortKeys(
+      nullptr, tmp, a.as<uint64_t>(), b.as<uint64_t>(), (int)n, 0, end_bit,
+      s));
+  HIP_TRY(ctx->scan_tmp.need(tmp));
+  HIP_TRY(hipcub::DeviceRadixSort::SortKeys(ctx->scan_tmp.p, tmp,
+                                            a.as<uint64_t>(), b.as<uint64_t>(),
+                                            (int)n, 0, end_bit, s));
+  return 0;
+}
+
+// plain path: site boxes, binning, reference relation over the bins
+int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
+  hipStream_t s = R.s;
+  const int32_t nt = R.nt, ns = R.ns;
+  const int64_t n_bins = R.n_bins;
+  HIP_TRY(ctx->bbox.need((size_t)std::max(ns, 1) * 48));
+  if (ns > 0)
+    hipLaunchKernelGGL(k_site_bbox, dim3(ns), dim3(64), 0, s, R.S, R.inst_d,
+                       R.site_inst_d, ctx->bbox.as<double>(),
+                       ctx->knn_near.as<int32_t>(),
+                       ctx->knn_site_near.as<double4>(), R.cnt + 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[10], s));
+  // sphere -> bin registrations: count, expand, radix sort by
+  // (bin, sphere), bin starts by binary search
+  HIP_TRY(ctx->reg_n.need(((size_t)ns + 1) * 8));
+  HIP_TRY(ctx->reg_off.need(((size_t)ns + 1) * 8));
+  hipLaunchKernelGGL(k_bin_nreg, dim3(ceil_div(ns + 1, 256)), dim3(256), 0, s,
+                     ns, ctx->bbox.as<double>(), R.inst_d, R.site_inst_d,
+                     ctx->reg_n.as<int64_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->reg_n.as<int64_t>(),
+                                 ctx->reg_off.as<int64_t>(), ns + 1));
+  int64_t n_reg = 0;
+  HIP_TRY(hipMemcpyAsync(&n_reg, ctx->reg_off.as<int64_t>() + ns, 8,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (n_reg >= (1LL << 31))
+    return fail(ctx, "rpd_run: too many sphere-bin registrations");
+  ctx->st.n_bin_registrations = n_reg;
+  HIP_TRY(ctx->keys_a.need((size_t)std::max<int64_t>(n_reg, 1) * 8));
+  HIP_TRY(ctx->keys_b.need((size_t)std::max<int64_t>(n_reg, 1) * 8));
+  HIP_TRY(ctx->bin_start.need((n_bins + 1) * 4));
+  HIP_TRY(ctx->bin_ids.need((size_t)std::max<int64_t>(n_reg, 1) * 4));
+  if (n_reg > 0) {
+    hipLaunchKernelGGL(k_bin_expand, dim3(ceil_div(n_reg, 256)), dim3(256), 0,
+                       s, ns, n_reg, ctx->bbox.as<double>(), R.inst_d,
+                       R.site_inst_d, ctx->reg_off.as<int64_t>(),
+                       ctx->keys_a.as<uint64_t>());
+    HIP_TRY(hipGetLastError());
+    int end_bit = 32;
+    while ((1LL << (end_bit - 32)) < n_bins + 1) ++end_bit;
+    if (int rc = sort_keys(ctx, s, ctx->keys_a, ctx->keys_b, n_reg, end_bit))
+      return rc;
+  }
+  hipLaunchKernelGGL(k_bin_starts,
+                     dim3(ceil_div(std::max<int64_t>(n_bins + 1, n_reg), 256)),
+                     dim3(256), 0, s, n_bins, n_reg, ctx->keys_b.as<uint64_t>(),
+                     ctx->bin_start.as<int32_t>(), ctx->bin_ids.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[11], s));
+  HIP_TRY(ctx->tet_count.need(((size_t)nt + 1) * 8));
+  HIP_TRY(ctx->tet_off.need(((size_t)nt + 1) * 8));
+  HIP_TRY(ctx->tet_pass.need(((size_t)nt + 1) * 8));
+  HIP_TRY(hipMemsetAsync(ctx->tet_count.as<int64_t>() + nt, 0, 8, s));
+  hipLaunchKernelGGL(k_tet_count, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
+                     R.M, R.S, R.inst_d, R.tet_inst_d,
+                     ctx->bin_start.as<int32_t>(), ctx->bin_ids.as<int32_t>(),
+                     ctx->tet_count.as<int64_t>(), ctx->tet_pass.as<uint64_t>(),
+                     R.cfg.count_work
+                         ? ctx->rel_work.as<unsigned long long>()
+                         : nullptr);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[12], s));
+  HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
+                                 ctx->tet_off.as<int64_t>(), nt + 1));
+  int64_t n_cand = 0;
+  HIP_TRY(hipMemcpyAsync(&n_cand, ctx->tet_off.as<int64_t>() + nt, 8,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(ctx->cand.need((size_t)std::max<int64_t>(n_cand, 1) * sizeof(Cand)));
+  HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
+  hipLaunchKernelGGL(k_tet_fill, dim3(ceil_div(nt, 256)), dim3(256), 0, s, R.M,
+                     R.S, R.inst_d, R.tet_inst_d, ctx->bin_start.as<int32_t>(),
+                     ctx->bin_ids.as<int32_t>(), ctx->tet_off.as<int64_t>(),
+                     ctx->tet_pass.as<uint64_t>(), ctx->cand.as<Cand>(),
+                     ctx->rel_ids.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[13], s));
+  *n_cand_out = n_cand;
+  return 0;
+}
+
+// neighbour spheres inline in list order (descent and relation read them)
+int site_inline(rpd_ctx* ctx, RunState& R) {
+  if (ctx->n_knn > 0)
+    hipLaunchKernelGGL(k_site_inline, dim3(ceil_div(ctx->n_knn, 256)),
+                       dim3(256), 0, R.s, R.S, ctx->n_knn,
+                       ctx->knn_near.as<int32_t>(),
+                       ctx->knn_site_near.as<double4>());
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// owner descent, 32 proposals and features of every tet
+int learned_features(rpd_ctx* ctx, RunState& R) {
+  hipStream_t s = R.s;
+  const int32_t nt = R.nt, ns = R.ns;
+  if (int rc = site_inline(ctx, R)) return rc;
+  HIP_TRY(ctx->sg_keys_a.need((size_t)std::max(ns, 1) * 8));
+  HIP_TRY(ctx->sg_keys_b.need((size_t)std::max(ns, 1) * 8));
+  HIP_TRY(ctx->sg_start.need((size_t)(R.n_sg + 1) * 4));
+  HIP_TRY(ctx->sg_ids.need((size_t)std::max(ns, 1) * 4));
+  if (ns > 0) {
+    hipLaunchKernelGGL(k_sg_keys, dim3(ceil_div(ns, 256)), dim3(256), 0, s,
+                       R.S, R.inst_d, R.site_inst_d,
+                       ctx->sg_keys_a.as<uint64_t>());
+    HIP_TRY(hipGetLastError());
+    int end_bit = 32;
+    while ((1LL << (end_bit - 32)) < R.n_sg + 1) ++end_bit;
+    if (int rc = sort_keys(ctx, s, ctx->sg_keys_a, ctx->sg_keys_b, ns,
+                           end_bit))
+      return rc;
+  }
+  hipLaunchKernelGGL(k_bin_starts,
+                     dim3(ceil_div(std::max<int64_t>(R.n_sg + 1, ns), 256)),
+                     dim3(256), 0, s, R.n_sg, (int64_t)ns,
+                     ctx->sg_keys_b.as<uint64_t>(), ctx->sg_start.as<int32_t>(),
+                     ctx->sg_ids.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(ctx->knn.need((size_t)std::max(nt, 1) * kOut * 4));
+  HIP_TRY(ctx->feat.need((size_t)std::max(nt, 1) * kIn * 2));
+  hipLaunchKernelGGL(k_knn_features, dim3(ceil_div(nt, 64)), dim3(64), 0, s,
+                     R.M, R.S, R.inst_d, R.tet_inst_d,
+                     ctx->sg_start.as<int32_t>(), ctx->sg_ids.as<int32_t>(),
+                     ctx->knn.as<int32_t>(), ctx->feat.as<uint16_t>());
+  HIP_TRY(hipGetLastError());
+  ctx->feat_n = nt;
+  return 0;
+}
+
+int learned_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
+  hipStream_t s = R.s;
+  const int32_t nt = R.nt;
+  if (R.cfg.learned == 1 && !ctx->mlp_ready)
+    return fail(ctx, "rpd_run: learned path without a model (rpd_mlp_set "
+                     "or rpd_mlp_train first)");
+  if (R.cfg.learned != 1 && R.cfg.learned != 2)
+    return fail(ctx, "rpd_run: learned must be 0, 1 or 2");
+  if (int rc = learned_features(ctx, R)) return rc;
+  HIP_TRY(hipEventRecord(ctx->ev[10], s));
+  HIP_TRY(ctx->sel.need((size_t)std::max(nt, 1) * 4));
+  if (R.cfg.learned == 1) {
+    const int64_t tiles = (nt + 15) / 16;
+    hipLaunchKernelGGL(
+        k_mlp_infer,
+        dim3((int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4,
+                                                         8192))),
+        dim3(256), 0, s, mlp_dev(ctx), ctx->feat.as<uint16_t>(), (int64_t)nt,
+        R.cfg.logit_threshold, ctx->sel.as<uint32_t>(), (float*)nullptr);
+  } else {
+    hipLaunchKernelGGL(k_sel_topk, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
+                       nt, R.cfg.proposal_k, ctx->sel.as<uint32_t>());
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[11], s));
+  HIP_TRY(ctx->tet_count.need(((size_t)nt + 1) * 8));
+  HIP_TRY(ctx->tet_off.need(((size_t)nt + 1) * 8));
+  HIP_TRY(hipMemsetAsync(ctx->tet_count.as<int64_t>() + nt, 0, 8, s));
+  hipLaunchKernelGGL(k_learn_count, dim3(ceil_div(nt, 64)), dim3(64), 0, s, nt,
+                     ctx->knn.as<int32_t>(), ctx->sel.as<uint32_t>(),
+                     ctx->tet_count.as<int64_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[12], s));
+  HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
+                                 ctx->tet_off.as<int64_t>(), nt + 1));
+  int64_t n_cand = 0;
+  HIP_TRY(hipMemcpyAsync(&n_cand, ctx->tet_off.as<int64_t>() + nt, 8,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(ctx->cand.need((size_t)std::max<int64_t>(n_cand, 1) * sizeof(Cand)));
+  HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
+  hipLaunchKernelGGL(k_learn_fill, dim3(ceil_div(nt, 64)), dim3(64), 0, s, nt,
+                     ctx->knn.as<int32_t>(), ctx->sel.as<uint32_t>(),
+                     ctx->tet_off.as<int64_t>(), ctx->cand.as<Cand>(),
+                     ctx->rel_ids.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[13], s));
+  *n_cand_out = n_cand;
+  return 0;
+}
+
+ClipArgs clip_args(rpd_ctx* ctx, RunState& R) {
+  ClipArgs a;
+  a.M = R.M;
+  a.S = R.S;
+  a.cand = ctx->cand.as<Cand>();
+  a.items = nullptr;
+  a.item_base = 0;
+  a.n_items = 0;
+  a.slot_base = 0;
+  a.use_mask = R.cfg.use_active_mask;
+  a.force_exact = R.cfg.force_exact;
+  a.geometry = R.cfg.compute_geometry;
+  a.rerun_large = ctx->rerun_large.as<int32_t>();
+  a.n_rerun_large = R.cnt + 1;
+  a.rerun_exact = ctx->rerun_exact.as<int32_t>();
+  a.n_rerun_exact = R.cnt + 2;
+  a.work = R.cfg.count_work ? ctx->work.as<unsigned long long>() : nullptr;
+  return a;
+}
+
+// masks, small tier, full-capacity and exact re-runs of candidates
+// [c0, c1); buffers grow keeping earlier rounds; loc of the range is set
+int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
+               bool events) {
+  hipStream_t s = R.s;
+  const int64_t n = c1 - c0;
+  const CapsRT sc = R.sc;
+  if (R.cfg.use_active_mask && n > 0)
+    hipLaunchKernelGGL(k_cand_mask, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                       R.M, R.S, n, ctx->cand.as<Cand>() + c0);
+  HIP_TRY(hipGetLastError());
+  if (events) HIP_TRY(hipEventRecord(ctx->ev[1], s));
+  HIP_TRY(ctx->rerun_large.need_keep((size_t)std::max<int64_t>(c1, 1) * 4,
+                                     (size_t)R.r_large * 4, s));
+  HIP_TRY(ctx->rerun_exact.need_keep((size_t)std::max<int64_t>(c1, 1) * 4,
+                                     (size_t)R.r_exact * 4, s));
+  HIP_TRY(ctx->loc.need_keep((size_t)std::max<int64_t>(c1, 1) * 4,
+                             (size_t)c0 * 4, s));
+  if (n > 0)
+    HIP_TRY(hipMemsetAsync(ctx->loc.as<int32_t>() + c0, 0, (size_t)n * 4, s));
+  if (R.cfg.small_tier) {
+    HIP_TRY(ctx->small.need_keep(c1, c0, sc.V, sc.P, sc.E, s));
+  } else {
+    HIP_TRY(ctx->small.need_keep(1, 0, sc.V, sc.P, sc.E, s));
+  }
+  ClipArgs a = clip_args(ctx, R);
+  if (R.cfg.small_tier) {
+    a.item_base = c0;
+    a.slot_base = c0;
+    a.n_items = n;
+    a.out = ctx->small.view(sc.V, sc.P, sc.E);
+    if (int rc = launch_clip(ctx, 0, R.G, a, R.cfg.small_caps)) return rc;
+  } else if (n > 0) {
+    // every cell goes straight to the full-capacity tier
+    std::vector<int32_t> idx(n);
+    for (int64_t q = 0; q < n; ++q) idx[q] = (int32_t)(c0 + q);
+    HIP_TRY(hipMemcpyAsync(ctx->rerun_large.as<int32_t>() + R.r_large,
+                           idx.data(), n * 4, hipMemcpyHostToDevice, s));
+    int32_t nf = (int32_t)(R.r_large + n);
+    HIP_TRY(hipMemcpyAsync(R.cnt + 1, &nf, 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  if (events) HIP_TRY(hipEventRecord(ctx->ev[2], s));
+  int32_t host_cnt[8];
+  HIP_TRY(hipMemcpyAsync(host_cnt, R.cnt, 32, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const int32_t n_large = host_cnt[1];
+  HIP_TRY(ctx->large.need_keep(n_large, R.r_large, LargeCaps::V, LargeCaps::P,
+                               LargeCaps::E, s));
+  if (n_large > R.r_large) {
+    a.items = ctx->rerun_large.as<int32_t>() + R.r_large;
+    a.n_items = n_large - R.r_large;
+    a.slot_base = R.r_large;
+    a.out = ctx->large.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
+    if (a.work) a.work = ctx->work.as<unsigned long long>() + 256;
+    if (int rc = launch_clip(ctx, 1, std::min(R.G, 4), a)) return rc;
+    HIP_TRY(hipMemcpyAsync(host_cnt, R.cnt, 32, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  const int32_t n_exact = host_cnt[2];
+  HIP_TRY(ctx->exact.need_keep(n_exact, R.r_exact, LargeCaps::V, LargeCaps::P,
+                               LargeCaps::E, s));
+  if (n_exact > R.r_exact) {
+    a.items = ctx->rerun_exact.as<int32_t>() + R.r_exact;
+    a.n_items = n_exact - R.r_exact;
+    a.slot_base = R.r_exact;
+    a.out = ctx->exact.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
+    if (a.work) a.work = ctx->work.as<unsigned long long>() + 512;
+    if (int rc = launch_clip(ctx, 2, 1, a)) return rc;
+  }
+  if (n_large > R.r_large)
+    hipLaunchKernelGGL(k_mark_rerun, dim3(ceil_div(n_large - R.r_large, 256)),
+                       dim3(256), 0, s, ctx->rerun_large.as<int32_t>(),
+                       R.r_large, n_large, 1, ctx->loc.as<int32_t>());
+  if (n_exact > R.r_exact)
+    hipLaunchKernelGGL(k_mark_rerun, dim3(ceil_div(n_exact - R.r_exact, 256)),
+                       dim3(256), 0, s, ctx->rerun_exact.as<int32_t>(),
+                       R.r_exact, n_exact, 2, ctx->loc.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  R.r_large = n_large;
+  R.r_exact = n_exact;
+  if (events) HIP_TRY(hipEventRecord(ctx->ev[3], s));
+  return 0;
+}
+
+TierViews tier_views(rpd_ctx* ctx, const RunState& R) {
+  TierViews tv;
+  tv.t[0] = ctx->small.view(R.sc.V, R.sc.P, R.sc.E);
+  tv.t[1] = ctx->large.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
+  tv.t[2] = ctx->exact.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
+  return tv;
+}
+
+// learned path: certificate of the proposals [0, n1), full-relation
+// re-run of the tets it rejects, final (tet, site) order.  Returns the
+// number of candidates in the output order (the first n_incl of perm).
+int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
+                         int64_t* n_incl_out) {
+  hipStream_t s = R.s;
+  const int32_t nt = R.nt;
+  HIP_TRY(ctx->flag.need((size_t)nt + 1));
+  HIP_TRY(ctx->has_cell.need((size_t)nt + 1));
+  HIP_TRY(ctx->excl.need((size_t)std::max<int64_t>(n1, 1)));
+  HIP_TRY(ctx->flag32.need(((size_t)nt + 1) * 4));
+  HIP_TRY(ctx->flag_pos.need(((size_t)nt + 1) * 4));
+  HIP_TRY(hipMemsetAsync(ctx->flag.p, 0, (size_t)nt + 1, s));
+  HIP_TRY(hipMemsetAsync(ctx->has_cell.p, 0, (size_t)nt + 1, s));
+  if (n1 > 0)
+    hipLaunchKernelGGL(k_certify, dim3(ceil_div(n1, 256)), dim3(256), 0, s, n1,
+                       R.M, R.S, ctx->cand.as<Cand>(), ctx->loc.as<int32_t>(),
+                       tier_views(ctx, R), ctx->tet_off.as<int64_t>(),
+                       ctx->rel_ids.as<int32_t>(), R.cfg.compute_geometry,
+                       ctx->flag.as<uint8_t>(), ctx->has_cell.as<uint8_t>(),
+                       ctx->excl.as<uint8_t>());
+  hipLaunchKernelGGL(k_flag_final, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
+                     nt, ctx->flag.as<uint8_t>(), ctx->has_cell.as<uint8_t>(),
+                     ctx->flag32.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemsetAsync(ctx->flag32.as<int32_t>() + nt, 0, 4, s));
+  HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->flag32.as<int32_t>(),
+                                 ctx->flag_pos.as<int32_t>(), nt + 1));
+  int32_t nf = 0;
+  HIP_TRY(hipMemcpyAsync(&nf, ctx->flag_pos.as<int32_t>() + nt, 4,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  int64_t n_fb = 0;
+  if (nf > 0) {
+    HIP_TRY(ctx->flag_list.need((size_t)nf * 4));
+    hipLaunchKernelGGL(k_flag_list, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
+                       nt, ctx->flag32.as<int32_t>(),
+                       ctx->flag_pos.as<int32_t>(),
+                       ctx->flag_list.as<int32_t>());
+    HIP_TRY(ctx->brute_n.need(((size_t)nf + 1) * 8));
+    HIP_TRY(ctx->brute_off.need(((size_t)nf + 1) * 8));
+    HIP_TRY(hipMemsetAsync(ctx->brute_n.as<int64_t>() + nf, 0, 8, s));
+    hipLaunchKernelGGL(k_brute, dim3(nf), dim3(64), 0, s, nf,
+                       ctx->flag_list.as<int32_t>(), R.M, R.S, R.inst_d,
+                       R.tet_inst_d, (const int64_t*)nullptr,
+                       ctx->brute_n.as<int64_t>(), (Cand*)nullptr, (int64_t)0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->brute_n.as<int64_t>(),
+                                   ctx->brute_off.as<int64_t>(), nf + 1));
+    HIP_TRY(hipMemcpyAsync(&n_fb, ctx->brute_off.as<int64_t>() + nf, 8,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(ctx->cand.need_keep((size_t)(n1 + n_fb) * sizeof(Cand),
+                                (size_t)n1 * sizeof(Cand), s));
+    hipLaunchKernelGGL(k_brute, dim3(nf), dim3(64), 0, s, nf,
+                       ctx->flag_list.as<int32_t>(), R.M, R.S, R.inst_d,
+                       R.tet_inst_d, ctx->brute_off.as<int64_t>(),
+                       (int64_t*)nullptr, ctx->cand.as<Cand>(), n1);
+    HIP_TRY(hipGetLastError());
+    if (int rc = clip_range(ctx, R, n1, n1 + n_fb, false)) return rc;
+  }
+  const int64_t n_all = n1 + n_fb;
+  ctx->st.learned[0] = n1;
+  ctx->st.learned[1] = nf;
+  ctx->st.learned[2] = n_fb;
+  // output order: included candidates by (tet, site)
+  HIP_TRY(ctx->keys_a.need((size_t)std::max<int64_t>(n_all, 1) * 8));
+  HIP_TRY(ctx->keys_b.need((size_t)std::max<int64_t>(n_all, 1) * 8));
+  HIP_TRY(ctx->vals_a.need((size_t)std::max<int64_t>(n_all, 1) * 4));
+  HIP_TRY(ctx->perm.need((size_t)std::max<int64_t>(n_all, 1) * 4));
+  HIP_TRY(hipMemsetAsync(R.cnt + 3, 0, 4, s));
+  if (n_all > 0)
+    hipLaunchKernelGGL(k_final_keys, dim3(ceil_div(n_all, 256)), dim3(256), 0,
+                       s, n_all, n1, ctx->cand.as<Cand>(),
+                       ctx->flag.as<uint8_t>(), ctx->excl.as<uint8_t>(),
+                       ctx->keys_a.as<uint64_t>(), ctx->vals_a.as<int32_t>(),
+                       R.cnt + 3);
+  HIP_TRY(hipGetLastError());
+  if (n_all > 0) {
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+        nullptr, tmp, ctx->keys_a.as<uint64_t>(), ctx->keys_b.as<uint64_t>(),
+        ctx->vals_a.as<int32_t>(), ctx->perm.as<int32_t>(), (int)n_all, 0, 64,
+        s));
+    HIP_TRY(ctx->scan_tmp.need(tmp));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+        ctx->scan_tmp.p, tmp, ctx->keys_a.as<uint64_t>(),
+        ctx->keys_b.as<uint64_t>(), ctx->vals_a.as<int32_t>(),
+        ctx->perm.as<int32_t>(), (int)n_all, 0, 64, s));
+  }
+  int32_t n_incl = 0;
+  HIP_TRY(hipMemcpyAsync(&n_incl, R.cnt + 3, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  // relation CSR of the final candidate sets
+  HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_incl, 1) * 4));
+  HIP_TRY(hipMemsetAsync(ctx->tet_count.p, 0, ((size_t)nt + 1) * 8, s));
+  if (n_incl > 0)
+    hipLaunchKernelGGL(k_final_rel, dim3(ceil_div(n_incl, 256)), dim3(256), 0,
+                       s, (int64_t)n_incl, ctx->cand.as<Cand>(),
+                       ctx->perm.as<int32_t>(), ctx->tet_count.as<int64_t>(),
+                       ctx->rel_ids.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
+                                 ctx->tet_off.as<int64_t>(), nt + 1));
+  *n_incl_out = n_incl;
+  return 0;
+}
+
+// sizes, scan and copy of the emitted cells of candidates perm[0..n)
+// (identity if perm is null) into the output stream
+int compact(rpd_ctx* ctx, RunState& R, int64_t n, const int32_t* perm,
+            Sizes3* tot) {
+  hipStream_t s = R.s;
+  TierViews tv = tier_views(ctx, R);
+  HIP_TRY(ctx->sizes.need(((size_t)n + 1) * sizeof(Sizes3)));
+  HIP_TRY(ctx->offs.need(((size_t)n + 1) * sizeof(Sizes3)));
+  HIP_TRY(ctx->drops.need(9 * 8));
+  HIP_TRY(hipMemsetAsync(ctx->drops.p, 0, 9 * 8, s));
+  HIP_TRY(hipMemsetAsync(ctx->sizes.as<Sizes3>() + n, 0, sizeof(Sizes3), s));
+  if (n > 0)
+    hipLaunchKernelGGL(k_sizes, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
+                       perm, ctx->loc.as<int32_t>(), tv,
+                       ctx->sizes.as<Sizes3>(), ctx->drops.as<int64_t>());
+  HIP_TRY(hipGetLastError());
+  {
+    size_t tmp = 0;
+    Sizes3 zero{0, 0, 0};
+    HIP_TRY(hipcub::DeviceScan::ExclusiveScan(
+        nullptr, tmp, ctx->sizes.as<Sizes3>(), ctx->offs.as<Sizes3>(),
+        Sizes3Sum(), zero, (int)(n + 1), s));
+    HIP_TRY(ctx->scan_tmp.need(tmp));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveScan(
+        ctx->scan_tmp.p, tmp, ctx->sizes.as<Sizes3>(), ctx->offs.as<Sizes3>(),
+        Sizes3Sum(), zero, (int)(n + 1), s));
+  }
+  HIP_TRY(hipMemcpyAsync(tot, ctx->offs.as<Sizes3>() + n, sizeof(Sizes3),
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(ctx->out_cells.need((size_t)std::max<int64_t>(tot->cells, 1) *
+                              sizeof(rpd_cell)));
+  HIP_TRY(ctx->out_words.need((size_t)std::max<int64_t>(tot->words, 1) * 4));
+  HIP_TRY(ctx->out_geom.need((size_t)std::max<int64_t>(tot->cells, 1) * 32));
+  HIP_TRY(ctx->out_areas.need((size_t)std::max<int64_t>(tot->areas, 1) * 8));
+  if (n > 0)
+    hipLaunchKernelGGL(k_compact, dim3(ceil_div(n * kCompactLanes, 256)),
+                       dim3(256), 0, s, n, perm, ctx->cand.as<Cand>(),
+                       ctx->loc.as<int32_t>(), tv, ctx->offs.as<Sizes3>(),
+                       ctx->out_cells.as<rpd_cell>(),
+                       ctx->out_words.as<uint32_t>(),
+                       ctx->out_geom.as<double>(), ctx->out_areas.as<double>(),
+                       R.cfg.compute_geometry);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -468,11 +1078,12 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   if (!ctx->has_mesh || !ctx->has_sites)
     return fail(ctx, "rpd_run: no resident instance");
   HIP_TRY(hipSetDevice(ctx->device));
-  rpd_run_cfg cfg{};
+  RunState R;
+  rpd_run_cfg& cfg = R.cfg;
+  cfg = rpd_run_cfg{};
   cfg.cells_per_wave = 0;
   cfg.use_active_mask = 1;
   cfg.compute_geometry = 1;
-  cfg.force_exact = 0;
   cfg.small_tier = 1;
   if (cfg_in) cfg = *cfg_in;
   if (cfg.cells_per_wave <= 0) {  // tuned shape, else the default
@@ -481,281 +1092,46 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   }
   if (cfg.small_caps != 0 && cfg.small_caps != 1)
     return fail(ctx, "rpd_run: small_caps must be 0 or 1");
-  const CapsRT sc = small_caps(cfg.small_caps);
-  const int G = cfg.cells_per_wave;
-  if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16)
+  R.sc = small_caps(cfg.small_caps);
+  R.G = cfg.cells_per_wave;
+  if (R.G != 1 && R.G != 2 && R.G != 4 && R.G != 8 && R.G != 16)
     return fail(ctx, "rpd_run: cells_per_wave must be 1, 2, 4, 8 or 16");
-  hipStream_t s = ctx->stream;
-  const int32_t nt = ctx->n_tet, ns = ctx->n_site;
   const int32_t site_k = ctx->st.site_k;
   std::memset(&ctx->st, 0, sizeof(ctx->st));
   ctx->st.site_k = site_k;
-
-  DevMesh M{ctx->n_vert, nt, ctx->vert.as<double4>(), ctx->tet_v.as<int4>(),
-            ctx->tet_fid.as<int4>(), ctx->tet_meta.as<TetMeta>()};
-  DevSites S{ns, ctx->site.as<double4>(), ctx->flags.as<uint32_t>(),
-             ctx->knn_off.as<int64_t>(), ctx->knn_ids.as<int32_t>(),
-             ctx->knn_near.as<int32_t>(), ctx->knn_site_near.as<double4>()};
-
-  HIP_TRY(hipEventRecord(ctx->ev[0], s));
   HIP_TRY(ctx->counters.need(8 * 4));
+  R.s = ctx->stream;
+  hipStream_t s = R.s;
+  HIP_TRY(hipEventRecord(ctx->ev[0], s));
   HIP_TRY(hipMemsetAsync(ctx->counters.p, 0, 8 * 4, s));
-  int32_t* cnt = ctx->counters.as<int32_t>();
-
+  if (int rc = init_state(ctx, R)) return rc;
+  const int32_t nt = R.nt;
   if (cfg.count_work) {
     HIP_TRY(ctx->rel_work.need(64 * 4 * 8));
     HIP_TRY(hipMemsetAsync(ctx->rel_work.p, 0, 64 * 4 * 8, s));
-  }
-
-  // ---- 1. candidate gathering
-  // per instance: a uniform grid of about n_tet/6 bins over its domain,
-  // numbered from bin_base in the batch-wide bin space
-  int64_t n_bins = 0;
-  {
-    std::vector<InstInfo> ii(ctx->inst.size());
-    for (size_t b = 0; b < ctx->inst.size(); ++b) {
-      const rpd_ctx::Inst& I = ctx->inst[b];
-      InstInfo& o = ii[b];
-      double ext[3], vol = 1;
-      for (int a = 0; a < 3; ++a) {
-        ext[a] = std::max(I.dom[3 + a] - I.dom[a], 1e-300);
-        vol *= ext[a];
-      }
-      const double bs = std::cbrt(vol / std::max(1.0, I.n_tet / 6.0));
-      int64_t nb = 1;
-      for (int a = 0; a < 3; ++a) {
-        o.g.org[a] = I.dom[a];
-        o.g.dim[a] = (int32_t)std::min<double>(
-            512, std::max(1.0, std::ceil(ext[a] / bs)));
-        nb *= o.g.dim[a];
-      }
-      o.g.inv_bs = 1.0 / bs;
-      o.g.pad = 0;
-      o.bin_base = n_bins;
-      for (int a = 0; a < 6; ++a) o.dom[a] = I.dom[a];
-      o.rho = I.rho;
-      n_bins += nb;
-    }
-    if (n_bins >= (1LL << 31))
-      return fail(ctx, "rpd_run: too many candidate bins");
-    HIP_TRY(ctx->inst_dev.need(ii.size() * sizeof(InstInfo)));
-    HIP_TRY(hipMemcpyAsync(ctx->inst_dev.p, ii.data(),
-                           ii.size() * sizeof(InstInfo),
-                           hipMemcpyHostToDevice, s));
-    // (the host vector must outlive the copy)
-    HIP_TRY(hipStreamSynchronize(s));
-  }
-  const InstInfo* inst_d = ctx->inst_dev.as<InstInfo>();
-  const int32_t* site_inst_d = ctx->site_inst.as<int32_t>();
-  const int32_t* tet_inst_d = ctx->tet_inst.as<int32_t>();
-  {
-    HIP_TRY(ctx->bbox.need((size_t)std::max(ns, 1) * 48));
-    if (ns > 0)
-      hipLaunchKernelGGL(k_site_bbox, dim3(ns), dim3(64), 0, s, S, inst_d,
-                         site_inst_d,
-                         ctx->bbox.as<double>(), ctx->knn_near.as<int32_t>(),
-                         ctx->knn_site_near.as<double4>(), cnt + 0);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[10], s));
-    // sphere -> bin registrations: count, expand, radix sort by
-    // (bin, sphere), bin starts by binary search
-    HIP_TRY(ctx->reg_n.need(((size_t)ns + 1) * 8));
-    HIP_TRY(ctx->reg_off.need(((size_t)ns + 1) * 8));
-    hipLaunchKernelGGL(k_bin_nreg, dim3(ceil_div(ns + 1, 256)), dim3(256), 0,
-                       s, ns, ctx->bbox.as<double>(), inst_d, site_inst_d,
-                       ctx->reg_n.as<int64_t>());
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->reg_n.as<int64_t>(),
-                                   ctx->reg_off.as<int64_t>(), ns + 1));
-    int64_t n_reg = 0;
-    HIP_TRY(hipMemcpyAsync(&n_reg, ctx->reg_off.as<int64_t>() + ns, 8,
-                           hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if (n_reg >= (1LL << 31))
-      return fail(ctx, "rpd_run: too many sphere-bin registrations");
-    ctx->st.n_bin_registrations = n_reg;
-    HIP_TRY(ctx->keys_a.need((size_t)std::max<int64_t>(n_reg, 1) * 8));
-    HIP_TRY(ctx->keys_b.need((size_t)std::max<int64_t>(n_reg, 1) * 8));
-    HIP_TRY(ctx->bin_start.need((n_bins + 1) * 4));
-    HIP_TRY(ctx->bin_ids.need((size_t)std::max<int64_t>(n_reg, 1) * 4));
-    if (n_reg > 0) {
-      hipLaunchKernelGGL(k_bin_expand, dim3(ceil_div(n_reg, 256)), dim3(256),
-                         0, s, ns, n_reg, ctx->bbox.as<double>(), inst_d,
-                         site_inst_d, ctx->reg_off.as<int64_t>(),
-                         ctx->keys_a.as<uint64_t>());
-      HIP_TRY(hipGetLastError());
-      int end_bit = 32;
-      while ((1LL << (end_bit - 32)) < n_bins + 1) ++end_bit;
-      size_t tmp = 0;
-      HIP_TRY(hipcub::DeviceRadixSort::SortKeys(
-          nullptr, tmp, ctx->keys_a.as<uint64_t>(), ctx->keys_b.as<uint64_t>(),
-          (int)n_reg, 0, end_bit, s));
-      HIP_TRY(ctx->scan_tmp.need(tmp));
-      HIP_TRY(hipcub::DeviceRadixSort::SortKeys(
-          ctx->scan_tmp.p, tmp, ctx->keys_a.as<uint64_t>(),
-          ctx->keys_b.as<uint64_t>(), (int)n_reg, 0, end_bit, s));
-    }
-    hipLaunchKernelGGL(k_bin_starts,
-                       dim3(ceil_div(std::max<int64_t>(n_bins + 1, n_reg), 256)),
-                       dim3(256), 0, s, n_bins, n_reg,
-                       ctx->keys_b.as<uint64_t>(),
-                       ctx->bin_start.as<int32_t>(),
-                       ctx->bin_ids.as<int32_t>());
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[11], s));
-    HIP_TRY(ctx->tet_count.need(((size_t)nt + 1) * 8));
-    HIP_TRY(ctx->tet_off.need(((size_t)nt + 1) * 8));
-    HIP_TRY(ctx->tet_pass.need(((size_t)nt + 1) * 8));
-    HIP_TRY(hipMemsetAsync(ctx->tet_count.as<int64_t>() + nt, 0, 8, s));
-    hipLaunchKernelGGL(k_tet_count, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
-                       M, S, inst_d, tet_inst_d, ctx->bin_start.as<int32_t>(),
-                       ctx->bin_ids.as<int32_t>(),
-                       ctx->tet_count.as<int64_t>(),
-                       ctx->tet_pass.as<uint64_t>(),
-                       cfg.count_work ? ctx->rel_work.as<unsigned long long>()
-                                      : nullptr);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ctx->ev[12], s));
-    HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
-                                   ctx->tet_off.as<int64_t>(), nt + 1));
-  }
-  int64_t n_cand = 0;
-  HIP_TRY(hipMemcpyAsync(&n_cand, ctx->tet_off.as<int64_t>() + nt, 8,
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  HIP_TRY(ctx->cand.need((size_t)std::max<int64_t>(n_cand, 1) * sizeof(Cand)));
-  HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
-  hipLaunchKernelGGL(k_tet_fill, dim3(ceil_div(nt, 256)), dim3(256), 0, s, M,
-                     S, inst_d, tet_inst_d, ctx->bin_start.as<int32_t>(),
-                     ctx->bin_ids.as<int32_t>(), ctx->tet_off.as<int64_t>(),
-                     ctx->tet_pass.as<uint64_t>(), ctx->cand.as<Cand>(),
-                     ctx->rel_ids.as<int32_t>());
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[13], s));
-  if (cfg.use_active_mask && n_cand > 0)
-    hipLaunchKernelGGL(k_cand_mask, dim3(ceil_div(n_cand, 256)), dim3(256), 0,
-                       s, M, S, n_cand, ctx->cand.as<Cand>());
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[1], s));
-
-  // ---- 2. clipping: small tier, then full capacity, then exact
-  HIP_TRY(ctx->rerun_large.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
-  HIP_TRY(ctx->rerun_exact.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
-  ClipArgs a;
-  a.M = M;
-  a.S = S;
-  a.cand = ctx->cand.as<Cand>();
-  a.use_mask = cfg.use_active_mask;
-  a.force_exact = cfg.force_exact;
-  a.geometry = cfg.compute_geometry;
-  a.rerun_large = ctx->rerun_large.as<int32_t>();
-  a.n_rerun_large = cnt + 1;
-  a.rerun_exact = ctx->rerun_exact.as<int32_t>();
-  a.n_rerun_exact = cnt + 2;
-  a.work = nullptr;
-  if (cfg.count_work) {
     HIP_TRY(ctx->work.need(3 * 64 * 4 * 8));
     HIP_TRY(hipMemsetAsync(ctx->work.p, 0, 3 * 64 * 4 * 8, s));
-    a.work = ctx->work.as<unsigned long long>();
   }
-  const int64_t n_first = n_cand;
-  if (cfg.small_tier) {
-    HIP_TRY(ctx->small.need(n_first, sc.V, sc.P, sc.E));
-    a.items = nullptr;
-    a.n_items = n_first;
-    a.out = ctx->small.view(sc.V, sc.P, sc.E);
-    if (int rc = launch_clip(ctx, 0, G, a, cfg.small_caps)) return rc;
-  } else {
-    // every cell goes straight to the full-capacity tier through the
-    // re-run list (identity)
-    std::vector<int32_t> idx(n_first);
-    for (int64_t q = 0; q < n_first; ++q) idx[q] = (int32_t)q;
-    if (n_first)
-      HIP_TRY(hipMemcpyAsync(ctx->rerun_large.p, idx.data(), n_first * 4,
-                             hipMemcpyHostToDevice, s));
-    int32_t nf = (int32_t)n_first;
-    HIP_TRY(hipMemcpyAsync(cnt + 1, &nf, 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(ctx->small.need(1, sc.V, sc.P, sc.E));
-  }
-  HIP_TRY(hipEventRecord(ctx->ev[2], s));
-  int32_t host_cnt[8];
-  HIP_TRY(hipMemcpyAsync(host_cnt, cnt, 32, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  const int32_t n_large = host_cnt[1];
-  HIP_TRY(ctx->large.need(n_large, LargeCaps::V, LargeCaps::P, LargeCaps::E));
-  if (n_large > 0) {
-    a.items = ctx->rerun_large.as<int32_t>();
-    a.n_items = n_large;
-    a.out = ctx->large.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
-    if (a.work) a.work = ctx->work.as<unsigned long long>() + 256;
-    if (int rc = launch_clip(ctx, 1, std::min(G, 4), a)) return rc;
-    HIP_TRY(hipMemcpyAsync(host_cnt, cnt, 32, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-  }
-  const int32_t n_exact = host_cnt[2];
-  HIP_TRY(ctx->exact.need(n_exact, LargeCaps::V, LargeCaps::P, LargeCaps::E));
-  if (n_exact > 0) {
-    a.items = ctx->rerun_exact.as<int32_t>();
-    a.n_items = n_exact;
-    a.out = ctx->exact.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
-    if (a.work) a.work = ctx->work.as<unsigned long long>() + 512;
-    if (int rc = launch_clip(ctx, 2, 1, a)) return rc;
-  }
-  HIP_TRY(hipEventRecord(ctx->ev[3], s));
 
-  // ---- 3. compaction
-  HIP_TRY(ctx->loc.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
-  HIP_TRY(hipMemsetAsync(ctx->loc.p, 0, (size_t)std::max<int64_t>(n_cand, 1) * 4, s));
-  if (n_large > 0)
-    hipLaunchKernelGGL(k_mark_rerun, dim3(ceil_div(n_large, 256)), dim3(256),
-                       0, s, ctx->rerun_large.as<int32_t>(), n_large, 1,
-                       ctx->loc.as<int32_t>());
-  if (n_exact > 0)
-    hipLaunchKernelGGL(k_mark_rerun, dim3(ceil_div(n_exact, 256)), dim3(256),
-                       0, s, ctx->rerun_exact.as<int32_t>(), n_exact, 2,
-                       ctx->loc.as<int32_t>());
-  TierViews tv;
-  tv.t[0] = ctx->small.view(sc.V, sc.P, sc.E);
-  tv.t[1] = ctx->large.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
-  tv.t[2] = ctx->exact.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
-  HIP_TRY(ctx->sizes.need(((size_t)n_cand + 1) * sizeof(Sizes3)));
-  HIP_TRY(ctx->offs.need(((size_t)n_cand + 1) * sizeof(Sizes3)));
-  HIP_TRY(ctx->drops.need(9 * 8));
-  HIP_TRY(hipMemsetAsync(ctx->drops.p, 0, 9 * 8, s));
-  HIP_TRY(hipMemsetAsync(ctx->sizes.as<Sizes3>() + n_cand, 0, sizeof(Sizes3), s));
-  if (n_cand > 0)
-    hipLaunchKernelGGL(k_sizes, dim3(ceil_div(n_cand, 256)), dim3(256), 0, s,
-                       n_cand, ctx->loc.as<int32_t>(), tv,
-                       ctx->sizes.as<Sizes3>(), ctx->drops.as<int64_t>());
-  HIP_TRY(hipGetLastError());
-  {
-    size_t tmp = 0;
-    Sizes3 zero{0, 0, 0};
-    HIP_TRY(hipcub::DeviceScan::ExclusiveScan(
-        nullptr, tmp, ctx->sizes.as<Sizes3>(), ctx->offs.as<Sizes3>(),
-        Sizes3Sum(), zero, (int)(n_cand + 1), s));
-    HIP_TRY(ctx->scan_tmp.need(tmp));
-    HIP_TRY(hipcub::DeviceScan::ExclusiveScan(
-        ctx->scan_tmp.p, tmp, ctx->sizes.as<Sizes3>(), ctx->offs.as<Sizes3>(),
-        Sizes3Sum(), zero, (int)(n_cand + 1), s));
+  // ---- 1. candidates
+  int64_t n_cand = 0;
+  if (cfg.learned) {
+    if (int rc = learned_candidates(ctx, R, &n_cand)) return rc;
+  } else {
+    if (int rc = plain_candidates(ctx, R, &n_cand)) return rc;
   }
+  // ---- 2. clipping: small tier, then full capacity, then exact
+  if (int rc = clip_range(ctx, R, 0, n_cand, true)) return rc;
+  // ---- 3. (learned) certificate and fallback; 4. compaction
+  int64_t n_out = n_cand;
+  const int32_t* perm = nullptr;
+  if (cfg.learned) {
+    if (int rc = certify_and_fallback(ctx, R, n_cand, &n_out)) return rc;
+    perm = ctx->perm.as<int32_t>();
+  }
+  HIP_TRY(hipEventRecord(ctx->ev[5], s));
   Sizes3 tot;
-  HIP_TRY(hipMemcpyAsync(&tot, ctx->offs.as<Sizes3>() + n_cand, sizeof(Sizes3),
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  HIP_TRY(ctx->out_cells.need((size_t)std::max<int64_t>(tot.cells, 1) * sizeof(rpd_cell)));
-  HIP_TRY(ctx->out_words.need((size_t)std::max<int64_t>(tot.words, 1) * 4));
-  HIP_TRY(ctx->out_geom.need((size_t)std::max<int64_t>(tot.cells, 1) * 32));
-  HIP_TRY(ctx->out_areas.need((size_t)std::max<int64_t>(tot.areas, 1) * 8));
-  if (n_cand > 0)
-    hipLaunchKernelGGL(k_compact, dim3(ceil_div(n_cand * kCompactLanes, 256)),
-                       dim3(256), 0, s,
-                       n_cand, ctx->cand.as<Cand>(), ctx->loc.as<int32_t>(), tv,
-                       ctx->offs.as<Sizes3>(), ctx->out_cells.as<rpd_cell>(),
-                       ctx->out_words.as<uint32_t>(),
-                       ctx->out_geom.as<double>(), ctx->out_areas.as<double>(),
-                       cfg.compute_geometry);
-  HIP_TRY(hipGetLastError());
+  if (int rc = compact(ctx, R, n_out, perm, &tot)) return rc;
   HIP_TRY(hipEventRecord(ctx->ev[4], s));
 
   // ---- statistics (outside the timed phases)
@@ -763,12 +1139,13 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   HIP_TRY(hipMemsetAsync(ctx->stats64.p, 0, 4 * 8, s));
   if (n_cand > 0 && cfg.count_work)
     hipLaunchKernelGGL(k_mask_stats, dim3(ceil_div(n_cand, 256)), dim3(256), 0,
-                       s, n_cand, ctx->cand.as<Cand>(), S,
+                       s, n_cand, ctx->cand.as<Cand>(), R.S,
                        ctx->stats64.as<unsigned long long>());
   int64_t st64[4], drops[9];
+  int32_t host_cnt[8];
   HIP_TRY(hipMemcpyAsync(st64, ctx->stats64.p, 32, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(drops, ctx->drops.p, 72, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(host_cnt, cnt, 32, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(host_cnt, R.cnt, 32, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   // tet_k: max candidates of one tet
   {
@@ -784,12 +1161,12 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   }
   rpd_stats& st = ctx->st;
   st.n_cells = tot.cells;
-  st.n_candidates = n_cand;
+  st.n_candidates = n_out;
   st.n_words = tot.words;
   st.n_areas = tot.areas;
   for (int q = 0; q < 9; ++q) st.n_dropped[q] = drops[q];
-  st.n_rerun_large = n_large;
-  st.n_rerun_exact = n_exact;
+  st.n_rerun_large = R.r_large;
+  st.n_rerun_exact = R.r_exact;
   st.n_active_planes = st64[0];
   st.n_all_planes = st64[1];
   if (cfg.count_work) {
@@ -806,29 +1183,29 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     st.relation_work[3] = n_cand;
   }
   float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[4]));
-  st.ms_total = ms;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-  st.ms_phase[0] = ms;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
-  st.ms_phase[1] = ms;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
-  st.ms_phase[2] = ms;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]));
-  st.ms_phase[3] = ms;
-  // kernel-level: site boxes, binning, relation count, relation fill, masks
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[10]));
-  st.ms_kernel[0] = ms;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[10], ctx->ev[11]));
-  st.ms_kernel[1] = ms;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[11], ctx->ev[12]));
-  st.ms_kernel[2] = ms;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[12], ctx->ev[13]));
-  st.ms_kernel[3] = ms;
-  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[13], ctx->ev[1]));
-  st.ms_kernel[4] = ms;
+  auto el = [&](int a, int b) -> double {
+    float m = 0;
+    return hipEventElapsedTime(&m, ctx->ev[a], ctx->ev[b]) == hipSuccess ? m
+                                                                         : -1;
+  };
+  (void)ms;
+  st.ms_total = el(0, 4);
+  st.ms_phase[0] = el(0, 1);
+  st.ms_phase[1] = el(1, 2);
+  st.ms_phase[2] = el(2, 3);
+  st.ms_phase[3] = el(5, 4);
+  st.ms_phase[4] = el(3, 5);
+  // kernel-level: plain: site boxes, binning, relation count, relation
+  // fill, masks; learned: features, MLP, proposal count, proposal fill,
+  // masks
+  st.ms_kernel[0] = el(0, 10);
+  st.ms_kernel[1] = el(10, 11);
+  st.ms_kernel[2] = el(11, 12);
+  st.ms_kernel[3] = el(12, 13);
+  st.ms_kernel[4] = el(13, 1);
   st.ms_kernel[5] = st.ms_phase[1];
   st.ms_kernel[6] = st.ms_phase[3];
+  st.ms_kernel[7] = st.ms_phase[4];
   st.n_bbox_fallback = host_cnt[0];
   st.tet_k = (int32_t)st64[2];
   return 0;
@@ -947,3 +1324,290 @@ int rpd_download_relation(rpd_ctx* ctx, int64_t* offsets, int32_t* ids) {
 }
 
 }  // extern "C"
+
+namespace {
+
+int mlp_alloc(rpd_ctx* ctx) {
+  HIP_TRY(ctx->mlp_master.need(kNParam * 4));
+  HIP_TRY(ctx->mlp_m1.need(kNParam * 4));
+  HIP_TRY(ctx->mlp_m2.need(kNParam * 4));
+  HIP_TRY(ctx->mlp_w1t.need(kIn * kHid * 2));
+  HIP_TRY(ctx->mlp_w2t.need(kHid * kHid * 2));
+  HIP_TRY(ctx->mlp_w3t.need(kHid * kOut * 2));
+  HIP_TRY(ctx->mlp_w2.need(kHid * kHid * 2));
+  HIP_TRY(ctx->mlp_w3.need(kHid * kOut * 2));
+  HIP_TRY(ctx->mlp_bias.need((2 * kHid + kOut) * 4));
+  return 0;
+}
+
+int mlp_pack(rpd_ctx* ctx) {
+  hipLaunchKernelGGL(k_mlp_pack, dim3(ceil_div(kHid * kHid, 256)), dim3(256),
+                     0, ctx->stream, ctx->mlp_master.as<float>(),
+                     ctx->mlp_w1t.as<uint16_t>(), ctx->mlp_w2t.as<uint16_t>(),
+                     ctx->mlp_w3t.as<uint16_t>(), ctx->mlp_w2.as<uint16_t>(),
+                     ctx->mlp_w3.as<uint16_t>(), ctx->mlp_bias.as<float>());
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// one training step (or a gradient) over rows idx[0..n_rows) of X/Y/M;
+// n_rows is a multiple of 16; loss of the step returned
+int mlp_step(rpd_ctx* ctx, const uint16_t* X, const float* Y, const float* M,
+             const int32_t* idx, int32_t n_rows, int32_t n_valid,
+             float pos_w, const AdamArgs& adam, float* grad_out,
+             float* loss) {
+  hipStream_t s = ctx->stream;
+  const int tiles = n_rows / 16;
+  const int tpc = 8;
+  const int chunks = (tiles + tpc - 1) / tpc;
+  HIP_TRY(ctx->tr_xb.need((size_t)n_rows * kIn * 2));
+  HIP_TRY(ctx->tr_h1.need((size_t)n_rows * kHid * 2));
+  HIP_TRY(ctx->tr_h2.need((size_t)n_rows * kHid * 2));
+  HIP_TRY(ctx->tr_dz1.need((size_t)n_rows * kHid * 2));
+  HIP_TRY(ctx->tr_dz2.need((size_t)n_rows * kHid * 2));
+  HIP_TRY(ctx->tr_dz3.need((size_t)n_rows * kOut * 2));
+  HIP_TRY(ctx->tr_loss.need((size_t)tiles * 4));
+  HIP_TRY(ctx->tr_p1.need((size_t)chunks * kIn * kHid * 4));
+  HIP_TRY(ctx->tr_p2.need((size_t)chunks * kHid * kHid * 4));
+  HIP_TRY(ctx->tr_p3.need((size_t)chunks * kHid * kOut * 4));
+  const float inv_count = 1.f / (float)std::max(1, n_valid * kOut);
+  hipLaunchKernelGGL(k_mlp_fb, dim3(ceil_div(tiles, 4)), dim3(256), 0, s,
+                     mlp_dev(ctx), X, Y, M, idx, n_rows, pos_w, inv_count,
+                     ctx->tr_xb.as<uint16_t>(), ctx->tr_h1.as<uint16_t>(),
+                     ctx->tr_h2.as<uint16_t>(), ctx->tr_dz1.as<uint16_t>(),
+                     ctx->tr_dz2.as<uint16_t>(), ctx->tr_dz3.as<uint16_t>(),
+                     ctx->tr_loss.as<float>());
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3((kIn / 16) * (kHid / 16), chunks),
+                     dim3(64), 0, s, ctx->tr_xb.as<uint16_t>(),
+                     ctx->tr_dz1.as<uint16_t>(), kIn, kHid, tpc, tiles,
+                     ctx->tr_p1.as<float>());
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3((kHid / 16) * (kHid / 16), chunks),
+                     dim3(64), 0, s, ctx->tr_h1.as<uint16_t>(),
+                     ctx->tr_dz2.as<uint16_t>(), kHid, kHid, tpc, tiles,
+                     ctx->tr_p2.as<float>());
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3((kHid / 16) * (kOut / 16), chunks),
+                     dim3(64), 0, s, ctx->tr_h2.as<uint16_t>(),
+                     ctx->tr_dz3.as<uint16_t>(), kHid, kOut, tpc, tiles,
+                     ctx->tr_p3.as<float>());
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_mlp_update, dim3(ceil_div(kNParam, 256)), dim3(256), 0,
+                     s, ctx->mlp_master.as<float>(), ctx->mlp_m1.as<float>(),
+                     ctx->mlp_m2.as<float>(), ctx->tr_p1.as<float>(),
+                     ctx->tr_p2.as<float>(), ctx->tr_p3.as<float>(), chunks,
+                     ctx->tr_dz1.as<uint16_t>(), ctx->tr_dz2.as<uint16_t>(),
+                     ctx->tr_dz3.as<uint16_t>(), n_rows, adam, grad_out);
+  HIP_TRY(hipGetLastError());
+  if (adam.apply)
+    if (int rc = mlp_pack(ctx)) return rc;
+  std::vector<float> lt(tiles);
+  HIP_TRY(hipMemcpyAsync(lt.data(), ctx->tr_loss.p, (size_t)tiles * 4,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  double sum = 0;
+  for (float v : lt) sum += v;
+  *loss = (float)sum;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rpd_mlp_set(rpd_ctx* ctx, const float* params) {
+  if (!ctx || !params) return -1;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (int rc = mlp_alloc(ctx)) return rc;
+  for (int p = 0; p < kNParam; ++p)
+    if (!std::isfinite(params[p]))
+      return fail(ctx, "rpd_mlp_set: non-finite parameter");
+  HIP_TRY(hipMemcpyAsync(ctx->mlp_master.p, params, kNParam * 4,
+                         hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipMemsetAsync(ctx->mlp_m1.p, 0, kNParam * 4, ctx->stream));
+  HIP_TRY(hipMemsetAsync(ctx->mlp_m2.p, 0, kNParam * 4, ctx->stream));
+  if (int rc = mlp_pack(ctx)) return rc;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->adam_step = 0;
+  ctx->mlp_ready = true;
+  return 0;
+}
+
+int rpd_mlp_get(rpd_ctx* ctx, float* params) {
+  if (!ctx || !params) return -1;
+  if (!ctx->mlp_ready) return fail(ctx, "rpd_mlp_get: no model");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipMemcpyAsync(params, ctx->mlp_master.p, kNParam * 4,
+                         hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int rpd_mlp_init(rpd_ctx* ctx, uint64_t seed) {
+  if (!ctx) return -1;
+  std::vector<float> p(kNParam, 0.f);
+  auto fill = [&](int off, int in, int out, uint64_t stream) {
+    const double lim = std::sqrt(6.0 / (in + out));
+    for (int q = 0; q < in * out; ++q)
+      p[off + q] =
+          (float)((2.0 * rpd_rng_uniform(seed, stream, (uint64_t)q) - 1.0) *
+                  lim);
+  };
+  fill(kOffW1, kIn, kHid, 11);
+  fill(kOffW2, kHid, kHid, 12);
+  fill(kOffW3, kHid, kOut, 13);
+  return rpd_mlp_set(ctx, p.data());
+}
+
+int rpd_features(rpd_ctx* ctx, int32_t* knn, uint16_t* X) {
+  if (!ctx) return -1;
+  if (!ctx->has_mesh || !ctx->has_sites)
+    return fail(ctx, "rpd_features: no resident instance");
+  HIP_TRY(hipSetDevice(ctx->device));
+  RunState R;
+  if (int rc = init_state(ctx, R)) return rc;
+  if (int rc = learned_features(ctx, R)) return rc;
+  const size_t nt = (size_t)ctx->n_tet;
+  if (knn)
+    HIP_TRY(hipMemcpyAsync(knn, ctx->knn.p, nt * kOut * 4,
+                           hipMemcpyDeviceToHost, ctx->stream));
+  if (X)
+    HIP_TRY(hipMemcpyAsync(X, ctx->feat.p, nt * kIn * 2, hipMemcpyDeviceToHost,
+                           ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int rpd_labels(rpd_ctx* ctx, float* Y, float* M) {
+  if (!ctx) return -1;
+  if (ctx->feat_n != ctx->n_tet || ctx->n_tet <= 0)
+    return fail(ctx, "rpd_labels: compute rpd_features first");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int32_t nt = ctx->n_tet;
+  HIP_TRY(ctx->lab_y.need((size_t)nt * kOut * 4));
+  HIP_TRY(ctx->lab_m.need((size_t)nt * kOut * 4));
+  hipLaunchKernelGGL(k_labels, dim3(ceil_div(nt, 256)), dim3(256), 0,
+                     ctx->stream, nt, ctx->out_cells.as<rpd_cell>(),
+                     ctx->st.n_cells, ctx->knn.as<int32_t>(),
+                     ctx->lab_y.as<float>(), ctx->lab_m.as<float>());
+  HIP_TRY(hipGetLastError());
+  if (Y)
+    HIP_TRY(hipMemcpyAsync(Y, ctx->lab_y.p, (size_t)nt * kOut * 4,
+                           hipMemcpyDeviceToHost, ctx->stream));
+  if (M)
+    HIP_TRY(hipMemcpyAsync(M, ctx->lab_m.p, (size_t)nt * kOut * 4,
+                           hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->lab_n = nt;
+  return 0;
+}
+
+int rpd_mlp_train(rpd_ctx* ctx, const rpd_train_cfg* tc, float* loss) {
+  if (!ctx || !tc) return -1;
+  if (!ctx->mlp_ready) return fail(ctx, "rpd_mlp_train: no model");
+  if (ctx->lab_n != ctx->n_tet || ctx->feat_n != ctx->n_tet ||
+      ctx->n_tet <= 0)
+    return fail(ctx, "rpd_mlp_train: features and labels first");
+  if (tc->batch <= 0 || tc->batch % 16 || tc->epochs <= 0)
+    return fail(ctx, "rpd_mlp_train: batch must be a positive multiple of "
+                     "16, epochs positive");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int64_t n = ctx->n_tet, B = tc->batch;
+  const int64_t n_pad = (n + B - 1) / B * B;
+  std::vector<int32_t> order(n_pad, -1);
+  HIP_TRY(ctx->tr_idx.need((size_t)n_pad * 4));
+  for (int e = 0; e < tc->epochs; ++e) {
+    for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+    for (int64_t i = n - 1; i > 0; --i) {  // Fisher-Yates
+      const int64_t j = (int64_t)(rpd_rng_uniform(tc->seed, 1000 + e,
+                                                  (uint64_t)i) *
+                                  (double)(i + 1));
+      std::swap(order[i], order[std::min(j, i)]);
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->tr_idx.p, order.data(), (size_t)n_pad * 4,
+                           hipMemcpyHostToDevice, ctx->stream));
+    double el = 0;
+    int nb = 0;
+    for (int64_t b0 = 0; b0 < n_pad; b0 += B) {
+      ++ctx->adam_step;
+      AdamArgs a;
+      a.lr = tc->lr;
+      a.beta1 = 0.9f;
+      a.beta2 = 0.999f;
+      a.eps = 1e-8f;
+      a.bc1 = 1.f - std::pow(0.9f, (float)ctx->adam_step);
+      a.bc2 = 1.f - std::pow(0.999f, (float)ctx->adam_step);
+      a.apply = 1;
+      float l = 0;
+      const int32_t valid = (int32_t)std::min<int64_t>(B, n - b0);
+      if (int rc = mlp_step(ctx, ctx->feat.as<uint16_t>(),
+                            ctx->lab_y.as<float>(), ctx->lab_m.as<float>(),
+                            ctx->tr_idx.as<int32_t>() + b0, (int32_t)B, valid,
+                            tc->pos_weight, a, nullptr, &l))
+        return rc;
+      el += l;
+      ++nb;
+    }
+    if (loss) loss[e] = (float)(el / std::max(1, nb));
+  }
+  return 0;
+}
+
+int rpd_mlp_grad(rpd_ctx* ctx, const float* params, int32_t n,
+                 const uint16_t* X, const float* Y, const float* M,
+                 float pos_weight, float* grad, float* loss) {
+  if (!ctx || !params || n <= 0 || !X || !Y || !M || !grad || !loss)
+    return -1;
+  if (int rc = rpd_mlp_set(ctx, params)) return rc;
+  const int32_t n_pad = (n + 15) / 16 * 16;
+  std::vector<int32_t> idx(n_pad, -1);
+  for (int32_t i = 0; i < n; ++i) idx[i] = i;
+  HIP_TRY(ctx->tr_x.need((size_t)n * kIn * 2));
+  HIP_TRY(ctx->tr_y.need((size_t)n * kOut * 4));
+  HIP_TRY(ctx->tr_mk.need((size_t)n * kOut * 4));
+  HIP_TRY(ctx->tr_idx.need((size_t)n_pad * 4));
+  HIP_TRY(ctx->tr_grad.need(kNParam * 4));
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(ctx->tr_x.p, X, (size_t)n * kIn * 2,
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(ctx->tr_y.p, Y, (size_t)n * kOut * 4,
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(ctx->tr_mk.p, M, (size_t)n * kOut * 4,
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(ctx->tr_idx.p, idx.data(), (size_t)n_pad * 4,
+                         hipMemcpyHostToDevice, s));
+  AdamArgs a{};
+  a.apply = 0;
+  if (int rc = mlp_step(ctx, ctx->tr_x.as<uint16_t>(), ctx->tr_y.as<float>(),
+                        ctx->tr_mk.as<float>(), ctx->tr_idx.as<int32_t>(),
+                        n_pad, n, pos_weight, a, ctx->tr_grad.as<float>(),
+                        loss))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(grad, ctx->tr_grad.p, kNParam * 4,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int rpd_mlp_scores(rpd_ctx* ctx, float* logits) {
+  if (!ctx || !logits) return -1;
+  if (!ctx->mlp_ready || ctx->feat_n != ctx->n_tet || ctx->n_tet <= 0)
+    return fail(ctx, "rpd_mlp_scores: model and features first");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int32_t nt = ctx->n_tet;
+  HIP_TRY(ctx->sel.need((size_t)nt * 4));
+  HIP_TRY(ctx->tr_grad.need((size_t)nt * kOut * 4));
+  const int64_t tiles = (nt + 15) / 16;
+  hipLaunchKernelGGL(k_mlp_infer,
+                     dim3((int)std::min<int64_t>((tiles + 3) / 4, 8192)),
+                     dim3(256), 0, ctx->stream, mlp_dev(ctx),
+                     ctx->feat.as<uint16_t>(), (int64_t)nt, 0.f,
+                     ctx->sel.as<uint32_t>(), ctx->tr_grad.as<float>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(logits, ctx->tr_grad.p, (size_t)nt * kOut * 4,
+                         hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+}  // extern "C"
+

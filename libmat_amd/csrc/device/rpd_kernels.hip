@@ -144,6 +144,10 @@ struct InstInfo {
   int64_t bin_base;
   double dom[6];
   double rho;
+  BinGrid sg;        // grid of sphere centres (learned proposals)
+  int64_t sg_base;
+  int32_t site_off;  // first site of the instance, number of sites
+  int32_t n_site;
 };
 
 __device__ __forceinline__ int bin_coord(double x, double org, double inv,
@@ -635,9 +639,12 @@ struct ClipArgs {
   DevMesh M;
   DevSites S;
   const Cand* cand;
-  const int32_t* items;   // cell indices to process (nullptr: identity)
+  const int32_t* items;   // cell indices to process (nullptr: item_base +
+                          // item)
+  int64_t item_base;
   int64_t n_items;
-  SlotView out;           // indexed by item
+  SlotView out;           // indexed by slot_base + item
+  int64_t slot_base;
   int32_t use_mask;
   int32_t force_exact;
   int32_t geometry;
@@ -749,7 +756,8 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
   }
   const int64_t item = blk * G + g;
   if (item >= a.n_items) return;
-  const int64_t cidx = a.items ? a.items[item] : item;
+  const int64_t cidx = a.items ? a.items[item] : a.item_base + item;
+  const int64_t slot = a.slot_base + item;
   const Cand cd = a.cand[cidx];
   const int t = cd.tet, s = cd.seed;
 
@@ -1168,7 +1176,7 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
       mz += __shfl_xor(mz, off);
     }
     if (j == 0) {
-      double* gm = o.geom + 4 * item;
+      double* gm = o.geom + 4 * slot;
       const double vol = vol6 / 6.0;
       const double inv = vol6 != 0 ? 1.0 / (4.0 * vol6) : 0.0;
       gm[0] = fabs(vol);
@@ -1182,13 +1190,13 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
       if (p < nb_p) {
         const rpd_d4 pp = ld4(&s_pl[g][p]);
         const double nn = pp.x * pp.x + pp.y * pp.y + pp.z * pp.z;
-        o.areas[(int64_t)item * o.stride_areas + p] =
+        o.areas[slot * o.stride_areas + p] =
             fabs(acc[pi]) * 0.5 / sqrt(nn);
       }
     }
   }
   if (status == RPD_SUCCESS) {
-    uint32_t* wd = o.words + (int64_t)item * o.stride_words;
+    uint32_t* wd = o.words + slot * o.stride_words;
     for (int v = j; v < nb_v; v += L) wd[v] = s_ver[g][v];
     for (int p = j; p < nb_p; p += L) wd[nb_v + p] = (uint32_t)s_src[g][p];
     for (int q = j; q < nb_e; q += L) wd[nb_v + nb_p + q] = s_edge[g][q];
@@ -1207,7 +1215,7 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
     }
   }
   if (j == 0) {
-    o.hdr[item] = (uint32_t)nb_v | ((uint32_t)nb_p << 8) |
+    o.hdr[slot] = (uint32_t)nb_v | ((uint32_t)nb_p << 8) |
                   ((uint32_t)nb_e << 16) | (status << 24);
     if (status == kStatusRerunLarge) {
       int r = atomicAdd(a.n_rerun_large, 1);
@@ -1223,10 +1231,11 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
 // per cell: location of its final slot; tier 0 small (index = cell),
 // tier 1 large / 2 exact (index = re-run position)
 __global__ __launch_bounds__(256) void k_mark_rerun(const int32_t* list,
-                                                    int32_t n, int32_t tier,
+                                                    int32_t r0, int32_t r1,
+                                                    int32_t tier,
                                                     int32_t* loc) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
+  const int r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= r1) return;
   loc[list[r]] = (tier << 28) | r;
 }
 
@@ -1244,15 +1253,19 @@ struct TierViews {
   SlotView t[3];
 };
 
+// r-th cell of the output order: candidate perm[r] (perm == nullptr:
+// candidate r)
 __global__ __launch_bounds__(256) void k_sizes(int64_t n_cells,
+                                               const int32_t* perm,
                                                const int32_t* loc,
                                                TierViews tv, Sizes3* sz,
                                                int64_t* drops /*9*/) {
   __shared__ int32_t h[9];
   if (threadIdx.x < 9) h[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < n_cells) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n_cells) {
+    const int64_t c = perm ? perm[r] : r;
     const int32_t l = loc[c];
     const int tier = l >> 28;
     const int64_t idx = tier ? (l & ((1 << 28) - 1)) : c;
@@ -1265,7 +1278,7 @@ __global__ __launch_bounds__(256) void k_sizes(int64_t n_cells,
     } else if (st < 9) {
       atomicAdd(&h[st], 1);
     }
-    sz[c] = z;
+    sz[r] = z;
   }
   __syncthreads();
   if (threadIdx.x < 9 && h[threadIdx.x])
@@ -1277,20 +1290,22 @@ __global__ __launch_bounds__(256) void k_sizes(int64_t n_cells,
 // cell's payload words and areas from the slot into the contiguous output.
 constexpr int kCompactLanes = 8;
 __global__ __launch_bounds__(256) void k_compact(
-    int64_t n_cells, const Cand* cand, const int32_t* loc, TierViews tv,
-    const Sizes3* off, rpd_cell* out_cells, uint32_t* out_words,
-    double* out_geom, double* out_areas, int32_t geometry) {
+    int64_t n_cells, const int32_t* perm, const Cand* cand,
+    const int32_t* loc, TierViews tv, const Sizes3* off, rpd_cell* out_cells,
+    uint32_t* out_words, double* out_geom, double* out_areas,
+    int32_t geometry) {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t c = tid / kCompactLanes;
+  const int64_t r = tid / kCompactLanes;
   const int j = (int)(tid % kCompactLanes);
-  if (c >= n_cells) return;
+  if (r >= n_cells) return;
+  const int64_t c = perm ? perm[r] : r;
   const int32_t l = loc[c];
   const int tier = l >> 28;
   const int64_t idx = tier ? (l & ((1 << 28) - 1)) : c;
   const SlotView& sv = tv.t[tier];
   const uint32_t hd = sv.hdr[idx];
   if ((hd >> 24) != RPD_SUCCESS) return;
-  const Sizes3 o = off[c];
+  const Sizes3 o = off[r];
   const int nv = hd & 255, np = (hd >> 8) & 255, ne = (hd >> 16) & 255;
   if (j == 0) {
     rpd_cell rc;

@@ -40,6 +40,15 @@ class SitesDesc(ctypes.Structure):
                 ("knn_ids", ctypes.c_void_p)]
 
 
+class TrainCfg(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_int32), ("batch", ctypes.c_int32),
+                ("lr", ctypes.c_float), ("pos_weight", ctypes.c_float),
+                ("seed", ctypes.c_uint64)]
+
+
+MLP_NPARAM = 26912
+
+
 class RunCfg(ctypes.Structure):
     _fields_ = [("cells_per_wave", ctypes.c_int32),
                 ("use_active_mask", ctypes.c_int32),
@@ -48,7 +57,9 @@ class RunCfg(ctypes.Structure):
                 ("small_tier", ctypes.c_int32),
                 ("count_work", ctypes.c_int32),
                 ("small_caps", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("learned", ctypes.c_int32),
+                ("logit_threshold", ctypes.c_float),
+                ("proposal_k", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -66,7 +77,8 @@ class Stats(ctypes.Structure):
                 ("ms_phase", ctypes.c_double * 8),
                 ("ms_kernel", ctypes.c_double * 8),
                 ("work", (ctypes.c_int64 * 4) * 3),
-                ("relation_work", ctypes.c_int64 * 4)]
+                ("relation_work", ctypes.c_int64 * 4),
+                ("learned", ctypes.c_int64 * 4)]
 
     def as_dict(self):
         d = {}
@@ -102,6 +114,16 @@ def _declare_capi(L):
     L.rpd_batch_size.restype = i32
     L.rpd_tune.argtypes = [vp, vp, vp, vp, vp]
     L.rpd_tune.restype = ctypes.c_int
+    for n, a in (("rpd_mlp_init", [vp, ctypes.c_uint64]),
+                 ("rpd_mlp_set", [vp, vp]), ("rpd_mlp_get", [vp, vp]),
+                 ("rpd_features", [vp, vp, vp]),
+                 ("rpd_labels", [vp, vp, vp]),
+                 ("rpd_mlp_train", [vp, ctypes.POINTER(TrainCfg), vp]),
+                 ("rpd_mlp_grad", [vp, vp, i32, vp, vp, vp, ctypes.c_float,
+                                   vp, vp]),
+                 ("rpd_mlp_scores", [vp, vp])):
+        getattr(L, n).argtypes = a
+        getattr(L, n).restype = ctypes.c_int
     L.rpd_run.argtypes = [vp, ctypes.POINTER(RunCfg)]
     L.rpd_run.restype = ctypes.c_int
     L.rpd_sync.argtypes = [vp]
@@ -120,6 +142,9 @@ CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
                 "rpd_ctx_stream", "rpd_upload", "rpd_update_sites",
                 "rpd_upload_batch", "rpd_update_sites_batch",
                 "rpd_batch_offsets", "rpd_batch_size", "rpd_tune", "rpd_run",
+                "rpd_mlp_init", "rpd_mlp_set", "rpd_mlp_get", "rpd_features",
+                "rpd_labels", "rpd_mlp_train", "rpd_mlp_grad",
+                "rpd_mlp_scores",
                 "rpd_sync", "rpd_get_stats", "rpd_download_cells",
                 "rpd_download_relation", "rpd_calib_copy")
 
@@ -264,12 +289,72 @@ class Context:
         return g.value, c.value, {(cc, G): ms[3 * cc + q] for cc in (0, 1)
                                   for q, G in enumerate((4, 8, 16))}
 
+    # ---- learned proposals
+    def mlp_init(self, seed=0):
+        self._check(self._L.rpd_mlp_init(self._h, seed), "rpd_mlp_init")
+
+    def mlp_set(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == MLP_NPARAM
+        self._check(self._L.rpd_mlp_set(self._h, _p(p)), "rpd_mlp_set")
+
+    def mlp_get(self):
+        p = np.empty(MLP_NPARAM, np.float32)
+        self._check(self._L.rpd_mlp_get(self._h, _p(p)), "rpd_mlp_get")
+        return p
+
+    def features(self, download=True):
+        """Proposals (n_tet, 32) and bf16 feature bits (n_tet, 48)."""
+        if not download:
+            self._check(self._L.rpd_features(self._h, None, None),
+                        "rpd_features")
+            return None
+        knn = np.empty((self.n_tet, 32), np.int32)
+        X = np.empty((self.n_tet, 48), np.uint16)
+        self._check(self._L.rpd_features(self._h, _p(knn), _p(X)),
+                    "rpd_features")
+        return knn, X
+
+    def labels(self, download=True):
+        if not download:
+            self._check(self._L.rpd_labels(self._h, None, None), "rpd_labels")
+            return None
+        Y = np.empty((self.n_tet, 32), np.float32)
+        M = np.empty((self.n_tet, 32), np.float32)
+        self._check(self._L.rpd_labels(self._h, _p(Y), _p(M)), "rpd_labels")
+        return Y, M
+
+    def train(self, epochs=4, batch=1024, lr=2e-3, pos_weight=4.0, seed=0):
+        loss = np.empty(epochs, np.float32)
+        cfg = TrainCfg(epochs, batch, lr, pos_weight, seed)
+        self._check(self._L.rpd_mlp_train(self._h, ctypes.byref(cfg),
+                                          _p(loss)), "rpd_mlp_train")
+        return loss
+
+    def mlp_grad(self, params, X, Y, M, pos_weight):
+        params = np.ascontiguousarray(params, np.float32)
+        X = np.ascontiguousarray(X, np.uint16)
+        Y = np.ascontiguousarray(Y, np.float32)
+        M = np.ascontiguousarray(M, np.float32)
+        g = np.empty(MLP_NPARAM, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self._L.rpd_mlp_grad(
+            self._h, _p(params), X.shape[0], _p(X), _p(Y), _p(M),
+            float(pos_weight), _p(g), _p(loss)), "rpd_mlp_grad")
+        return g, float(loss[0])
+
+    def scores(self):
+        z = np.empty((self.n_tet, 32), np.float32)
+        self._check(self._L.rpd_mlp_scores(self._h, _p(z)), "rpd_mlp_scores")
+        return z
+
     def run(self, cells_per_wave=8, use_active_mask=True, geometry=True,
             force_exact=False, small_tier=True, count_work=False,
-            small_caps=0):
+            small_caps=0, learned=0, logit_threshold=0.0, proposal_k=8):
         cfg = RunCfg(cells_per_wave, int(use_active_mask), int(geometry),
                      int(force_exact), int(small_tier), int(count_work),
-                     int(small_caps))
+                     int(small_caps), int(learned), float(logit_threshold),
+                     int(proposal_k))
         self._check(self._L.rpd_run(self._h, ctypes.byref(cfg)), "rpd_run")
         return self.stats()
 

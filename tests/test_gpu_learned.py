@@ -1,0 +1,114 @@
+"""Learned proposals on the GPU: model gradients against the CPU oracle,
+proposals, bit-exact learned runs, training gain."""
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+import oracle
+from libmat_amd import learn, prep
+from rpd_compare import compare_with_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _same_result(a, b):
+    for f in ("cells", "words", "geom", "areas"):
+        x, y = getattr(a, f), getattr(b, f)
+        if f == "cells":
+            for k in ("tet", "seed", "nb_v", "nb_p", "nb_e", "status",
+                      "word_off", "area_off"):
+                assert np.array_equal(x[k], y[k]), (f, k)
+        else:
+            assert np.array_equal(x, y), f
+
+
+def test_mlp_gradient_matches_oracle(ctx):
+    rng = np.random.default_rng(5)
+    n = 203
+    params = mo.xavier(7).astype(np.float32)
+    Xb = mo.bf16_bits(rng.normal(size=(n, mo.IN)).astype(np.float32))
+    Y = (rng.random((n, mo.OUT)) < 0.25).astype(np.float32)
+    M = (rng.random((n, mo.OUT)) < 0.9).astype(np.float32)
+    g_gpu, l_gpu = ctx.mlp_grad(params, Xb, Y, M, 4.0)
+    l_orc, g_orc = mo.loss_and_grad(params.astype(np.float64),
+                                    mo.from_bits(Xb), Y, M, 4.0, True)
+    print(f"[mlp] loss gpu {l_gpu:.6f} oracle {l_orc:.6f}", flush=True)
+    assert abs(l_gpu - l_orc) <= 1e-4 * abs(l_orc)
+    o = 0
+    for name, shape in mo.SHAPES:
+        k = int(np.prod(shape))
+        a, b = g_gpu[o:o + k], g_orc[o:o + k]
+        err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-12)
+        cos = a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
+        print(f"[mlp] {name}: max rel err {err:.2e} cosine {cos:.6f}",
+              flush=True)
+        assert err < 2e-2 and cos > 0.9995, name
+        o += k
+
+
+def test_proposals_start_at_the_owner(ctx):
+    inst = prep.cube10_instance(0)
+    ctx.upload(inst)
+    knn, X = ctx.features()
+    V = inst.verts[inst.tets]
+    c = V.mean(axis=1)
+    pd = ((c[:, None, :] - inst.sites[None, :, :3]) ** 2).sum(-1) - \
+        inst.sites[None, :, 3]
+    owner = np.argmin(pd, axis=1)
+    assert np.array_equal(knn[:, 0], owner)
+    for t in range(0, inst.n_tet, 97):
+        ids = knn[t][knn[t] >= 0]
+        assert len(set(ids)) == ids.size
+        assert np.all(np.diff(pd[t, ids]) >= -1e-12)
+    assert X.shape == (inst.n_tet, 48) and (X[:, 43:] == 0).all()
+
+
+@pytest.mark.parametrize("mode", ["untrained", "topk"])
+def test_learned_runs_match_plain(ctx, mode):
+    inst = prep.cube10_instance(1)
+    orc = oracle.run(inst)
+    ctx.upload(inst)
+    ctx.run()
+    plain = ctx.download()
+    if mode == "untrained":
+        ctx.mlp_init(3)
+        st = ctx.run(learned=1, logit_threshold=0.0)
+    else:
+        st = ctx.run(learned=2, proposal_k=4)
+    res = ctx.download()
+    print(f"[learned] {mode}: proposals {st['learned'][0]} fallback tets "
+          f"{st['learned'][1]} of {inst.n_tet} ({st['learned'][2]} "
+          f"candidates)", flush=True)
+    _same_result(res, plain)
+    compare_with_oracle(res, orc)
+
+
+def test_training_gains_over_untrained_and_topk(ctx):
+    train = [prep.cube10_instance(s) for s in range(16)]
+    params, loss, thr = learn.train_proposals(ctx, train, epochs=6)
+    print(f"[train] loss per epoch {np.round(loss, 4).tolist()} "
+          f"threshold {thr:.3f}", flush=True)
+    assert loss[-1] < 0.8 * loss[0]
+    test = [prep.cube10_instance(s) for s in (1000, 1001)]
+    ctx.upload_batch(test)
+    ctx.run()
+    plain = ctx.download()
+    ctx.features(download=False)
+    Y, M = ctx.labels()
+    z = ctx.scores()
+    prec, rec = learn.proposal_quality(z, Y, M, thr)
+    st = ctx.run(learned=1, logit_threshold=thr)
+    _same_result(ctx.download(), plain)
+    fb_learned = st["learned"][1]
+    ctx.mlp_init(3)
+    st0 = ctx.run(learned=1, logit_threshold=0.0)
+    fb_untrained = st0["learned"][1]
+    st1 = ctx.run(learned=2, proposal_k=4)
+    fb_topk = st1["learned"][1]
+    n = sum(i.n_tet for i in test)
+    print(f"[train] held-out precision {prec:.3f} recall {rec:.4f}; fallback "
+          f"tets learned {fb_learned}, untrained {fb_untrained}, top-4 "
+          f"{fb_topk} of {n}; proposals learned {st['learned'][0]} top-4 "
+          f"{st1['learned'][0]}", flush=True)
+    assert rec > 0.98
+    assert fb_learned < fb_untrained
