@@ -119,7 +119,7 @@ struct rpd_ctx {
   std::vector<Inst> inst;
   int32_t n_vert = 0, n_tet = 0, n_site = 0;
   bool has_mesh = false, has_sites = false;
-  Buf vert, tet_v, tet_fid, tet_meta, tet_inst, inst_dev;
+  Buf vert, tet_v, tet_fid, tet_meta, tet_inst, vert_inst, inst_dev;
   Buf site, flags, knn_off, knn_ids, knn_near, knn_site_near, site_inst;
   // pipeline
   Buf bbox, reg_n, reg_off, keys_a, keys_b, bin_start, bin_ids;
@@ -133,7 +133,7 @@ struct rpd_ctx {
   int64_t n_knn = 0;
   // learned proposals: features, model (fp32 master, Adam state, bf16
   // copies), certificate and fallback
-  Buf knn, feat, sel, sg_keys_a, sg_keys_b, sg_start, sg_ids;
+  Buf knn, feat, sel, omask, vowner, sg_keys_a, sg_keys_b, sg_start, sg_ids;
   int32_t feat_n = 0;
   Buf mlp_master, mlp_m1, mlp_m2, mlp_w1t, mlp_w2t, mlp_w3t, mlp_w2, mlp_w3,
       mlp_bias;
@@ -340,7 +340,7 @@ int rpd_upload_batch(rpd_ctx* ctx, int32_t n_inst,
     return fail(ctx, "rpd_upload: batch too large");
   std::vector<double> v4(4 * nv_tot);
   std::vector<int32_t> tv_all(4 * nt_tot), fid_all(4 * nt_tot),
-      tinst(nt_tot);
+      tinst(nt_tot), vinst(nv_tot);
   std::vector<TetMeta> meta(nt_tot);
   std::vector<rpd_ctx::Inst> inst(n_inst);
   int64_t vo = 0, to = 0;
@@ -358,6 +358,7 @@ int rpd_upload_batch(rpd_ctx* ctx, int32_t n_inst,
         hi[a] = std::max(hi[a], x);
       }
       v4[4 * (vo + i) + 3] = 0.0;
+      vinst[vo + i] = b;
     }
     double rho = 0;
     for (int64_t t = 0; t < nt; ++t) {
@@ -414,6 +415,7 @@ int rpd_upload_batch(rpd_ctx* ctx, int32_t n_inst,
   HIP_TRY(ctx->tet_fid.need((size_t)nt_tot * 16));
   HIP_TRY(ctx->tet_meta.need((size_t)nt_tot * 16));
   HIP_TRY(ctx->tet_inst.need((size_t)nt_tot * 4));
+  HIP_TRY(ctx->vert_inst.need((size_t)nv_tot * 4));
   HIP_TRY(hipMemcpyAsync(ctx->vert.p, v4.data(), (size_t)nv_tot * 32,
                          hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(hipMemcpyAsync(ctx->tet_v.p, tv_all.data(), (size_t)nt_tot * 16,
@@ -423,6 +425,8 @@ int rpd_upload_batch(rpd_ctx* ctx, int32_t n_inst,
   HIP_TRY(hipMemcpyAsync(ctx->tet_meta.p, meta.data(), (size_t)nt_tot * 16,
                          hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(hipMemcpyAsync(ctx->tet_inst.p, tinst.data(), (size_t)nt_tot * 4,
+                         hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(ctx->vert_inst.p, vinst.data(), (size_t)nv_tot * 4,
                          hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   ctx->has_mesh = true;
@@ -741,10 +745,17 @@ int learned_features(rpd_ctx* ctx, RunState& R) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(ctx->knn.need((size_t)std::max(nt, 1) * kOut * 4));
   HIP_TRY(ctx->feat.need((size_t)std::max(nt, 1) * kIn * 2));
+  HIP_TRY(ctx->omask.need((size_t)std::max(nt, 1) * 4));
+  HIP_TRY(ctx->vowner.need((size_t)std::max(ctx->n_vert, 1) * 4));
+  hipLaunchKernelGGL(k_vertex_owner, dim3(ceil_div(ctx->n_vert, 256)),
+                     dim3(256), 0, s, R.M, R.S, R.inst_d,
+                     ctx->vert_inst.as<int32_t>(), ctx->sg_start.as<int32_t>(),
+                     ctx->sg_ids.as<int32_t>(), ctx->vowner.as<int32_t>());
+  HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_knn_features, dim3(ceil_div(nt, 64)), dim3(64), 0, s,
                      R.M, R.S, R.inst_d, R.tet_inst_d,
-                     ctx->sg_start.as<int32_t>(), ctx->sg_ids.as<int32_t>(),
-                     ctx->knn.as<int32_t>(), ctx->feat.as<uint16_t>());
+                     ctx->vowner.as<int32_t>(), ctx->knn.as<int32_t>(),
+                     ctx->feat.as<uint16_t>(), ctx->omask.as<uint32_t>());
   HIP_TRY(hipGetLastError());
   ctx->feat_n = nt;
   return 0;
@@ -780,7 +791,7 @@ int learned_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
   HIP_TRY(hipMemsetAsync(ctx->tet_count.as<int64_t>() + nt, 0, 8, s));
   hipLaunchKernelGGL(k_learn_count, dim3(ceil_div(nt, 64)), dim3(64), 0, s, nt,
                      ctx->knn.as<int32_t>(), ctx->sel.as<uint32_t>(),
-                     ctx->tet_count.as<int64_t>());
+                     ctx->omask.as<uint32_t>(), ctx->tet_count.as<int64_t>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->ev[12], s));
   HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
@@ -793,7 +804,8 @@ int learned_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
   HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
   hipLaunchKernelGGL(k_learn_fill, dim3(ceil_div(nt, 64)), dim3(64), 0, s, nt,
                      ctx->knn.as<int32_t>(), ctx->sel.as<uint32_t>(),
-                     ctx->tet_off.as<int64_t>(), ctx->cand.as<Cand>(),
+                     ctx->omask.as<uint32_t>(), ctx->tet_off.as<int64_t>(),
+                     ctx->cand.as<Cand>(),
                      ctx->rel_ids.as<int32_t>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->ev[13], s));

@@ -1,15 +1,17 @@
 // Learned candidate proposal for the RPD (device side).
 //
-// Per tet, the owner of its centroid (the sphere of least power distance) is
-// found by greedy descent on the neighbour graph -- in a power diagram a
-// sphere that does not own a point always has a neighbour of smaller power
-// distance to it -- and the owner with its 31 power-nearest neighbours
-// (flagged spheres only) are the tet's 32 proposals.  A small MLP
-// (48 -> 128 -> 128 -> 32, ReLU, BF16 MFMA with FP32 accumulation) scores
-// them from 48 features: the centroid in domain coordinates, and for the 8
-// power-nearest proposals their centre offset and radius over the tet's
-// radius and their power distance over its square.  Proposals scoring
-// above a threshold (the owner always) are clipped; a completeness
+// Per tet, the owners (spheres of least power distance) of its centroid and
+// of its 4 corners are found by greedy descent on the neighbour graph -- in
+// a power diagram a sphere that does not own a point always has a
+// neighbour of smaller power distance to it; each owner certainly has a
+// cell in the tet.  The tet's 32 proposals are the owners and their
+// neighbours (flagged spheres only) of least key = min over the 5 points of
+// the power distance minus the point owner's.  A small MLP (48 -> 128 ->
+// 128 -> 32, ReLU, BF16 MFMA with FP32 accumulation) scores them from 48
+// features: the centroid in domain coordinates, and for the first 8
+// proposals their centre offset and radius over the tet's radius and their
+// key over its square.  Proposals scoring above a threshold (owners
+// always) are clipped; a completeness
 // certificate then checks every tet: each active bisector facet of each of
 // its cells must lead to a sphere of its proposal set, no cell may have
 // been dropped, and some cell must exist (the cells of a set closed under
@@ -108,104 +110,180 @@ __global__ __launch_bounds__(256) void k_sg_keys(DevSites S,
   keys[i] = (b << 32) | (uint32_t)i;
 }
 
-// One thread per tet: owner by descent, 32 proposals sorted by (power
-// distance, id), features.  LDS holds each thread's sorted list.
+// grid start for a descent at point c: least power distance among the
+// spheres of c's cell of the sphere grid (rings up to 2), else the first
+// sphere of the instance
+__device__ __forceinline__ int grid_start(const DevSites& S,
+                                         const InstInfo& I,
+                                         const int32_t* sg_start,
+                                         const int32_t* sg_ids,
+                                         const double c[3], double* best) {
+  int cur = -1;
+  *best = INFINITY;
+  const int cx = bin_coord(c[0], I.sg.org[0], I.sg.inv_bs, I.sg.dim[0]);
+  const int cy = bin_coord(c[1], I.sg.org[1], I.sg.inv_bs, I.sg.dim[1]);
+  const int cz = bin_coord(c[2], I.sg.org[2], I.sg.inv_bs, I.sg.dim[2]);
+  for (int ring = 0; ring <= 2 && cur < 0; ++ring)
+    for (int dz = -ring; dz <= ring; ++dz)
+      for (int dy = -ring; dy <= ring; ++dy)
+        for (int dx = -ring; dx <= ring; ++dx) {
+          if (max(abs(dx), max(abs(dy), abs(dz))) != ring) continue;
+          const int xx = cx + dx, yy = cy + dy, zz = cz + dz;
+          if (xx < 0 || yy < 0 || zz < 0 || xx >= I.sg.dim[0] ||
+              yy >= I.sg.dim[1] || zz >= I.sg.dim[2])
+            continue;
+          const int64_t b = I.sg_base + xx +
+                            (int64_t)I.sg.dim[0] *
+                                (yy + (int64_t)I.sg.dim[1] * zz);
+          for (int q = sg_start[b]; q < sg_start[b + 1]; ++q) {
+            const int s = sg_ids[q];
+            const double pd = pdist(c, S.site[s]);
+            if (pd < *best || (pd == *best && s < cur)) {
+              *best = pd;
+              cur = s;
+            }
+          }
+        }
+  if (cur < 0 && I.n_site > 0) {
+    cur = I.site_off;
+    *best = pdist(c, S.site[cur]);
+  }
+  return cur;
+}
+
+// greedy descent to the owner of c: (pd, id) decreases strictly; in a
+// power diagram a sphere that does not own c has a neighbour closer to it
+__device__ __forceinline__ int descend(const DevSites& S, const double c[3],
+                                       int cur, double* best) {
+  for (int step = 0; step < (1 << 16); ++step) {
+    int nxt = cur;
+    double nb = *best;
+    for (int64_t q = S.knn_off[cur]; q < S.knn_off[cur + 1]; ++q) {
+      const int id = S.knn_near[q];
+      const double pd = pdist(c, S.knn_site_near[q]);
+      if (pd < nb || (pd == nb && id < nxt)) {
+        nb = pd;
+        nxt = id;
+      }
+    }
+    if (nxt == cur) break;
+    cur = nxt;
+    *best = nb;
+  }
+  return cur;
+}
+
+// owner of every mesh vertex (-1 if its instance has no sphere)
+__global__ __launch_bounds__(256) void k_vertex_owner(
+    DevMesh M, DevSites S, const InstInfo* inst, const int32_t* vert_inst,
+    const int32_t* sg_start, const int32_t* sg_ids, int32_t* owner) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= M.n_vert) return;
+  const double4 p = M.vert[v];
+  const double c[3] = {p.x, p.y, p.z};
+  double best;
+  const int st = grid_start(S, inst[vert_inst[v]], sg_start, sg_ids, c, &best);
+  owner[v] = st < 0 ? -1 : descend(S, c, st, &best);
+}
+
+// One thread per tet.  The owners of its centroid and of its 4 corners
+// belong to its sphere set; the proposals are the union of their 1-rings
+// (with them), ranked by key(s) = min over the 5 points x of
+// pd(x, s) - pd(x, owner of x) -- 0 exactly for the owners -- then id;
+// 32 kept, flagged spheres only.  omask marks the owner slots.
 __global__ __launch_bounds__(64) void k_knn_features(
     DevMesh M, DevSites S, const InstInfo* inst, const int32_t* tet_inst,
-    const int32_t* sg_start, const int32_t* sg_ids, int32_t* knn,
-    uint16_t* X) {
-  __shared__ float s_pd[kOut][64];
+    const int32_t* vowner, int32_t* knn, uint16_t* X, uint32_t* omask) {
+  __shared__ float s_key[kOut][64];
   __shared__ int32_t s_id[kOut][64];
   const int lane = threadIdx.x;
   const int t = blockIdx.x * 64 + lane;
   if (t >= M.n_tet) return;
   TetLoad L;
   load_tet(M, t, L);
-  double c[3];
-  tet_centroid(L, c);
+  double P[5][3];
+  tet_centroid(L, P[0]);
+  for (int k = 0; k < 4; ++k) {
+    P[1 + k][0] = L.V[k].x;
+    P[1 + k][1] = L.V[k].y;
+    P[1 + k][2] = L.V[k].z;
+  }
   double rho2 = 0;
   for (int k = 0; k < 4; ++k) {
-    const double dx = L.V[k].x - c[0], dy = L.V[k].y - c[1],
-                 dz = L.V[k].z - c[2];
+    const double dx = P[1 + k][0] - P[0][0], dy = P[1 + k][1] - P[0][1],
+                 dz = P[1 + k][2] - P[0][2];
     rho2 = fmax(rho2, (dx * dx + dy * dy) + dz * dz);
   }
   const InstInfo& I = inst[tet_inst[t]];
   uint16_t* x = X + (int64_t)t * kIn;
   int32_t* kt = knn + (int64_t)t * kOut;
-  // start: least power distance in the centroid's grid cell (rings <= 2)
-  int cur = -1;
-  double best = INFINITY;
-  {
-    const int cx = bin_coord(c[0], I.sg.org[0], I.sg.inv_bs, I.sg.dim[0]);
-    const int cy = bin_coord(c[1], I.sg.org[1], I.sg.inv_bs, I.sg.dim[1]);
-    const int cz = bin_coord(c[2], I.sg.org[2], I.sg.inv_bs, I.sg.dim[2]);
-    for (int ring = 0; ring <= 2 && cur < 0; ++ring)
-      for (int dz = -ring; dz <= ring; ++dz)
-        for (int dy = -ring; dy <= ring; ++dy)
-          for (int dx = -ring; dx <= ring; ++dx) {
-            if (max(abs(dx), max(abs(dy), abs(dz))) != ring) continue;
-            const int xx = cx + dx, yy = cy + dy, zz = cz + dz;
-            if (xx < 0 || yy < 0 || zz < 0 || xx >= I.sg.dim[0] ||
-                yy >= I.sg.dim[1] || zz >= I.sg.dim[2])
-              continue;
-            const int64_t b =
-                I.sg_base + xx +
-                (int64_t)I.sg.dim[0] * (yy + (int64_t)I.sg.dim[1] * zz);
-            for (int q = sg_start[b]; q < sg_start[b + 1]; ++q) {
-              const int s = sg_ids[q];
-              const double pd = pdist(c, S.site[s]);
-              if (pd < best || (pd == best && s < cur)) {
-                best = pd;
-                cur = s;
-              }
-            }
-          }
-    if (cur < 0 && I.n_site > 0) {
-      cur = I.site_off;
-      best = pdist(c, S.site[cur]);
-    }
-  }
+  int own[5];
+  double pmin[5];
+  own[1] = vowner[L.tv.x];
+  own[2] = vowner[L.tv.y];
+  own[3] = vowner[L.tv.z];
+  own[4] = vowner[L.tv.w];
   int cnt = 0;
-  if (cur >= 0) {
-    // greedy descent to the owner: (pd, id) decreases strictly
-    for (int step = 0; step < (1 << 16); ++step) {
-      int nxt = cur;
-      double nb = best;
-      for (int64_t q = S.knn_off[cur]; q < S.knn_off[cur + 1]; ++q) {
-        const int id = S.knn_near[q];
-        const double pd = pdist(c, S.knn_site_near[q]);
-        if (pd < nb || (pd == nb && id < nxt)) {
-          nb = pd;
-          nxt = id;
-        }
+  uint32_t om = 0;
+  if (own[1] >= 0) {
+    for (int k = 1; k < 5; ++k) pmin[k] = pdist(P[k], S.site[own[k]]);
+    // centroid owner: descend from the corner owner closest to it
+    int st = own[1];
+    double best = pdist(P[0], S.site[st]);
+    for (int k = 2; k < 5; ++k) {
+      const double pd = pdist(P[0], S.site[own[k]]);
+      if (pd < best || (pd == best && own[k] < st)) {
+        best = pd;
+        st = own[k];
       }
-      if (nxt == cur) break;
-      cur = nxt;
-      best = nb;
     }
-    auto ins = [&](float pd, int id) {
+    own[0] = descend(S, P[0], st, &best);
+    pmin[0] = best;
+    auto key_of = [&](const double4& s) {
+      double k = INFINITY;
+#pragma unroll
+      for (int j = 0; j < 5; ++j) k = fmin(k, pdist(P[j], s) - pmin[j]);
+      return k;
+    };
+    auto ins = [&](float key, int id) {
+      for (int q = 0; q < cnt; ++q)
+        if (s_id[q][lane] == id) return;
       if (cnt == kOut &&
-          !(pd < s_pd[kOut - 1][lane] ||
-            (pd == s_pd[kOut - 1][lane] && id < s_id[kOut - 1][lane])))
+          !(key < s_key[kOut - 1][lane] ||
+            (key == s_key[kOut - 1][lane] && id < s_id[kOut - 1][lane])))
         return;
       int j = cnt < kOut ? cnt : kOut - 1;
-      while (j > 0 && (s_pd[j - 1][lane] > pd ||
-                       (s_pd[j - 1][lane] == pd && s_id[j - 1][lane] > id))) {
-        s_pd[j][lane] = s_pd[j - 1][lane];
+      while (j > 0 && (s_key[j - 1][lane] > key ||
+                       (s_key[j - 1][lane] == key && s_id[j - 1][lane] > id))) {
+        s_key[j][lane] = s_key[j - 1][lane];
         s_id[j][lane] = s_id[j - 1][lane];
         --j;
       }
-      s_pd[j][lane] = pd;
+      s_key[j][lane] = key;
       s_id[j][lane] = id;
       if (cnt < kOut) ++cnt;
     };
-    if (S.flags[cur] == 1u) ins((float)best, cur);
-    for (int64_t q = S.knn_off[cur]; q < S.knn_off[cur + 1]; ++q) {
-      const int id = S.knn_near[q];
-      if (S.flags[id] != 1u) continue;
-      ins((float)pdist(c, S.knn_site_near[q]), id);
+    for (int j = 0; j < 5; ++j) {
+      const int o = own[j];
+      bool seen = false;
+      for (int q = 0; q < j; ++q) seen = seen || own[q] == o;
+      if (seen) continue;
+      if (S.flags[o] == 1u) ins(0.f, o);
+      for (int64_t q = S.knn_off[o]; q < S.knn_off[o + 1]; ++q) {
+        const int id = S.knn_near[q];
+        if (S.flags[id] != 1u) continue;
+        ins((float)key_of(S.knn_site_near[q]), id);
+      }
     }
+    for (int k = 0; k < cnt; ++k)
+      if (s_key[k][lane] == 0.f) {
+        const int id = s_id[k][lane];
+        bool is_owner = false;
+        for (int j = 0; j < 5; ++j) is_owner = is_owner || own[j] == id;
+        if (is_owner) om |= 1u << k;
+      }
   }
+  omask[t] = om;
   for (int k = 0; k < kOut; ++k) kt[k] = k < cnt ? s_id[k][lane] : -1;
   const double ext =
       fmax(fmax(I.dom[3] - I.dom[0], I.dom[4] - I.dom[1]), I.dom[5] - I.dom[2]);
@@ -214,18 +292,18 @@ __global__ __launch_bounds__(64) void k_knn_features(
   float f[kIn];
 #pragma unroll
   for (int q = 0; q < kIn; ++q) f[q] = 0.f;
-  f[0] = (float)((c[0] - I.dom[0]) * inv_ext);
-  f[1] = (float)((c[1] - I.dom[1]) * inv_ext);
-  f[2] = (float)((c[2] - I.dom[2]) * inv_ext);
+  f[0] = (float)((P[0][0] - I.dom[0]) * inv_ext);
+  f[1] = (float)((P[0][1] - I.dom[1]) * inv_ext);
+  f[2] = (float)((P[0][2] - I.dom[2]) * inv_ext);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     if (k < cnt) {
-      const double4 s = S.site[s_id[k][lane]];
-      f[3 + 5 * k + 0] = (float)((s.x - c[0]) * inv_r);
-      f[3 + 5 * k + 1] = (float)((s.y - c[1]) * inv_r);
-      f[3 + 5 * k + 2] = (float)((s.z - c[2]) * inv_r);
-      f[3 + 5 * k + 3] = (float)(sqrt(s.w) * inv_r);
-      f[3 + 5 * k + 4] = (float)(s_pd[k][lane] * (inv_r * inv_r));
+      const double4 sp = S.site[s_id[k][lane]];
+      f[3 + 5 * k + 0] = (float)((sp.x - P[0][0]) * inv_r);
+      f[3 + 5 * k + 1] = (float)((sp.y - P[0][1]) * inv_r);
+      f[3 + 5 * k + 2] = (float)((sp.z - P[0][2]) * inv_r);
+      f[3 + 5 * k + 3] = (float)(sqrt(sp.w) * inv_r);
+      f[3 + 5 * k + 4] = (float)(s_key[k][lane] * (inv_r * inv_r));
     }
   }
 #pragma unroll
@@ -611,14 +689,14 @@ __global__ __launch_bounds__(256) void k_sel_topk(int32_t n_tet, int32_t k,
   sel[t] = k >= 32 ? ~0u : (k <= 0 ? 0u : ((1u << k) - 1u));
 }
 
-// proposals of tet t: the owner (slot 0) always, others if selected
+// proposals of tet t: the owner slots always, others if selected
 __device__ __forceinline__ int proposals(const int32_t* knn, uint32_t sel,
                                          int t, int32_t out[kOut]) {
   int n = 0;
   for (int k = 0; k < kOut; ++k) {
     const int id = knn[(int64_t)t * kOut + k];
     if (id < 0) break;
-    if (k == 0 || ((sel >> k) & 1u)) out[n++] = id;
+    if ((sel >> k) & 1u) out[n++] = id;
   }
   // ascending site id (the clip order of the candidate list)
   for (int a = 1; a < n; ++a) {
@@ -636,14 +714,15 @@ __device__ __forceinline__ int proposals(const int32_t* knn, uint32_t sel,
 __global__ __launch_bounds__(64) void k_learn_count(int32_t n_tet,
                                                     const int32_t* knn,
                                                     const uint32_t* sel,
+                                                    const uint32_t* omask,
                                                     int64_t* count) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_tet) return;
   int n = 0;
-  const uint32_t s = sel[t];
+  const uint32_t s = sel[t] | omask[t];
   for (int k = 0; k < kOut; ++k) {
     if (knn[(int64_t)t * kOut + k] < 0) break;
-    n += (k == 0 || ((s >> k) & 1u)) ? 1 : 0;
+    n += ((s >> k) & 1u) ? 1 : 0;
   }
   count[t] = n;
 }
@@ -651,13 +730,14 @@ __global__ __launch_bounds__(64) void k_learn_count(int32_t n_tet,
 __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
                                                    const int32_t* knn,
                                                    const uint32_t* sel,
+                                                   const uint32_t* omask,
                                                    const int64_t* off,
                                                    Cand* cand,
                                                    int32_t* rel_ids) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_tet) return;
   int32_t ids[kOut];
-  const int n = proposals(knn, sel[t], t, ids);
+  const int n = proposals(knn, sel[t] | omask[t], t, ids);
   const int64_t o = off[t];
   for (int k = 0; k < n; ++k) {
     Cand c;

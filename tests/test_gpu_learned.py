@@ -46,20 +46,22 @@ def test_mlp_gradient_matches_oracle(ctx):
         o += k
 
 
-def test_proposals_start_at_the_owner(ctx):
+def test_proposals_start_with_the_owners(ctx):
+    # the owners (least power distance) of the centroid and of the 4
+    # corners, by brute force, lead every proposal list (key 0, by id)
     inst = prep.cube10_instance(0)
     ctx.upload(inst)
     knn, X = ctx.features()
     V = inst.verts[inst.tets]
-    c = V.mean(axis=1)
-    pd = ((c[:, None, :] - inst.sites[None, :, :3]) ** 2).sum(-1) - \
-        inst.sites[None, :, 3]
-    owner = np.argmin(pd, axis=1)
-    assert np.array_equal(knn[:, 0], owner)
-    for t in range(0, inst.n_tet, 97):
+    pts = np.concatenate([V.mean(axis=1)[:, None, :], V], axis=1)  # (nt,5,3)
+    pd = ((pts[:, :, None, :] - inst.sites[None, None, :, :3]) ** 2).sum(-1) \
+        - inst.sites[None, None, :, 3]
+    owners = np.argmin(pd, axis=2)
+    for t in range(inst.n_tet):
+        U = sorted(set(owners[t].tolist()))
+        assert knn[t][:len(U)].tolist() == U, t
         ids = knn[t][knn[t] >= 0]
-        assert len(set(ids)) == ids.size
-        assert np.all(np.diff(pd[t, ids]) >= -1e-12)
+        assert len(set(ids.tolist())) == ids.size
     assert X.shape == (inst.n_tet, 48) and (X[:, 43:] == 0).all()
 
 
