@@ -140,9 +140,11 @@ typedef struct {
   int64_t relation_work[4];   /* with count_work: (tet, site) pairs of the
                                  candidate bin lists tested, neighbour
                                  tests of the relation, 0, related pairs */
-  int64_t learned[4];         /* learned runs: proposals clipped, tets the
-                                 certificate rejected (re-run on the full
-                                 relation), their candidates, 0          */
+  int64_t learned[4];         /* learned runs: first-round candidates,
+                                 tets re-run (the fallback: closure over
+                                 missing facet neighbours or the full
+                                 relation), candidates those re-runs
+                                 added, tets re-run on the full relation */
 } rpd_stats;
 
 typedef struct rpd_ctx rpd_ctx;

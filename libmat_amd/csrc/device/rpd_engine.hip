@@ -28,6 +28,13 @@ using namespace rpd;
 struct Buf {
   void* p = nullptr;
   size_t cap = 0;
+  Buf() = default;
+  Buf(const Buf&) = delete;  // owns its block
+  Buf& operator=(const Buf&) = delete;
+  void swap(Buf& o) {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+  }
   ~Buf() {
     if (p) (void)hipFree(p);
   }
@@ -143,8 +150,9 @@ struct rpd_ctx {
   int32_t lab_n = 0;
   Buf tr_idx, tr_xb, tr_h1, tr_h2, tr_dz1, tr_dz2, tr_dz3, tr_loss, tr_p1,
       tr_p2, tr_p3, tr_grad, tr_x, tr_y, tr_mk;
-  Buf flag, has_cell, excl, flag32, flag_pos, flag_list, brute_n, brute_off,
-      vals_a, perm;
+  Buf flag, has_cell, flag32, flag_pos, flag_list, brute_n, brute_off,
+      vals_a, vals_b, perm, miss_a, miss_b, umark, upos, ukeys, upass, ppos,
+      mem_a, mem_b, reopened, out_off, out_cnt, out_ids;
   rpd_stats st;
   int tuned_G = 0, tuned_caps = 0;  // rpd_tune's choice (0: not tuned)
   hipEvent_t ev[16];
@@ -776,10 +784,12 @@ int learned_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
     const int64_t tiles = (nt + 15) / 16;
     hipLaunchKernelGGL(
         k_mlp_infer,
-        dim3((int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4,
-                                                         8192))),
-        dim3(256), 0, s, mlp_dev(ctx), ctx->feat.as<uint16_t>(), (int64_t)nt,
-        R.cfg.logit_threshold, ctx->sel.as<uint32_t>(), (float*)nullptr);
+        dim3((int)std::max<int64_t>(
+            1, std::min<int64_t>((tiles + kInferWaves - 1) / kInferWaves,
+                                 1024))),
+        dim3(64 * kInferWaves), 0, s, mlp_dev(ctx), ctx->feat.as<uint16_t>(),
+        (int64_t)nt, R.cfg.logit_threshold, ctx->sel.as<uint32_t>(),
+        (float*)nullptr);
   } else {
     hipLaunchKernelGGL(k_sel_topk, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
                        nt, R.cfg.proposal_k, ctx->sel.as<uint32_t>());
@@ -790,7 +800,7 @@ int learned_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
   HIP_TRY(ctx->tet_off.need(((size_t)nt + 1) * 8));
   HIP_TRY(hipMemsetAsync(ctx->tet_count.as<int64_t>() + nt, 0, 8, s));
   hipLaunchKernelGGL(k_learn_count, dim3(ceil_div(nt, 64)), dim3(64), 0, s, nt,
-                     ctx->knn.as<int32_t>(), ctx->sel.as<uint32_t>(),
+                     R.M, R.S, ctx->knn.as<int32_t>(), ctx->sel.as<uint32_t>(),
                      ctx->omask.as<uint32_t>(), ctx->tet_count.as<int64_t>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->ev[12], s));
@@ -804,8 +814,7 @@ int learned_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
   HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
   hipLaunchKernelGGL(k_learn_fill, dim3(ceil_div(nt, 64)), dim3(64), 0, s, nt,
                      ctx->knn.as<int32_t>(), ctx->sel.as<uint32_t>(),
-                     ctx->omask.as<uint32_t>(), ctx->tet_off.as<int64_t>(),
-                     ctx->cand.as<Cand>(),
+                     ctx->tet_off.as<int64_t>(), ctx->cand.as<Cand>(),
                      ctx->rel_ids.as<int32_t>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->ev[13], s));
@@ -926,27 +935,128 @@ TierViews tier_views(rpd_ctx* ctx, const RunState& R) {
   return tv;
 }
 
-// learned path: certificate of the proposals [0, n1), full-relation
-// re-run of the tets it rejects, final (tet, site) order.  Returns the
-// number of candidates in the output order (the first n_incl of perm).
+// learned path: certificate of the first-round candidates [0, n1),
+// closure rounds (missing related facet neighbours added and clipped, new
+// cells certified), full-relation re-run of the tets left open, final
+// (tet, site) order.  Returns the number of candidates in the output order
+// (the first n_incl of perm).
+constexpr int kClosureRounds = 6;
+
+int sort_u64(rpd_ctx* ctx, hipStream_t s, const uint64_t* in, uint64_t* out,
+             int64_t n) {
+  if (n <= 0) return 0;
+  size_t tmp = 0;
+  HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, in, out, (int)n, 0,
+                                            64, s));
+  HIP_TRY(ctx->scan_tmp.need(tmp));
+  HIP_TRY(hipcub::DeviceRadixSort::SortKeys(ctx->scan_tmp.p, tmp, in, out,
+                                            (int)n, 0, 64, s));
+  return 0;
+}
+
 int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
                          int64_t* n_incl_out) {
   hipStream_t s = R.s;
   const int32_t nt = R.nt;
   HIP_TRY(ctx->flag.need((size_t)nt + 1));
   HIP_TRY(ctx->has_cell.need((size_t)nt + 1));
-  HIP_TRY(ctx->excl.need((size_t)std::max<int64_t>(n1, 1)));
+  HIP_TRY(ctx->reopened.need((size_t)nt + 1));
   HIP_TRY(ctx->flag32.need(((size_t)nt + 1) * 4));
   HIP_TRY(ctx->flag_pos.need(((size_t)nt + 1) * 4));
   HIP_TRY(hipMemsetAsync(ctx->flag.p, 0, (size_t)nt + 1, s));
   HIP_TRY(hipMemsetAsync(ctx->has_cell.p, 0, (size_t)nt + 1, s));
-  if (n1 > 0)
-    hipLaunchKernelGGL(k_certify, dim3(ceil_div(n1, 256)), dim3(256), 0, s, n1,
-                       R.M, R.S, ctx->cand.as<Cand>(), ctx->loc.as<int32_t>(),
-                       tier_views(ctx, R), ctx->tet_off.as<int64_t>(),
-                       ctx->rel_ids.as<int32_t>(), R.cfg.compute_geometry,
-                       ctx->flag.as<uint8_t>(), ctx->has_cell.as<uint8_t>(),
-                       ctx->excl.as<uint8_t>());
+  HIP_TRY(hipMemsetAsync(ctx->reopened.p, 0, (size_t)nt + 1, s));
+  // members: the first-round sets (CSR tet_off / rel_ids of the learned
+  // fill) and ext, the sorted keys of every pair added since
+  int64_t n_ext = 0;
+  int64_t c_lo = 0, c_hi = n1, added = 0;
+  for (int round = 0;; ++round) {
+    const int64_t cap = std::max<int64_t>(4096, 2 * (c_hi - c_lo));
+    HIP_TRY(ctx->miss_a.need((size_t)cap * 8));
+    HIP_TRY(ctx->miss_b.need((size_t)cap * 8));
+    HIP_TRY(hipMemsetAsync(R.cnt + 4, 0, 4, s));
+    if (c_hi > c_lo)
+      hipLaunchKernelGGL(k_certify, dim3(ceil_div(c_hi - c_lo, 256)),
+                         dim3(256), 0, s, c_lo, c_hi, ctx->cand.as<Cand>(),
+                         ctx->loc.as<int32_t>(), tier_views(ctx, R),
+                         ctx->tet_off.as<int64_t>(),
+                         ctx->rel_ids.as<int32_t>(),
+                         ctx->mem_a.as<uint64_t>(), n_ext,
+                         ctx->miss_a.as<uint64_t>(), (int32_t)cap, R.cnt + 4,
+                         ctx->flag.as<uint8_t>(), ctx->has_cell.as<uint8_t>());
+    HIP_TRY(hipGetLastError());
+    int32_t nm = 0;
+    HIP_TRY(hipMemcpyAsync(&nm, R.cnt + 4, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const int64_t n_miss = std::min<int64_t>(nm, cap);
+    if (n_miss == 0) break;
+    if (round == kClosureRounds) {  // still open: full relation
+      hipLaunchKernelGGL(k_flag_miss, dim3(ceil_div(n_miss, 256)), dim3(256),
+                         0, s, n_miss, ctx->miss_a.as<uint64_t>(),
+                         ctx->flag.as<uint8_t>());
+      HIP_TRY(hipGetLastError());
+      break;
+    }
+    // unique missing pairs; related ones become candidates, all members
+    if (int rc = sort_u64(ctx, s, ctx->miss_a.as<uint64_t>(),
+                          ctx->miss_b.as<uint64_t>(), n_miss))
+      return rc;
+    HIP_TRY(ctx->umark.need(((size_t)n_miss + 1) * 4));
+    HIP_TRY(ctx->upos.need(((size_t)n_miss + 1) * 4));
+    hipLaunchKernelGGL(k_unique_mark, dim3(ceil_div(n_miss + 1, 256)),
+                       dim3(256), 0, s, n_miss, ctx->miss_b.as<uint64_t>(),
+                       ctx->umark.as<int32_t>());
+    HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->umark.as<int32_t>(),
+                                   ctx->upos.as<int32_t>(), n_miss + 1));
+    int32_t n_u = 0;
+    HIP_TRY(hipMemcpyAsync(&n_u, ctx->upos.as<int32_t>() + n_miss, 4,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(ctx->ukeys.need((size_t)std::max(n_u, 1) * 8));
+    HIP_TRY(ctx->upass.need((size_t)n_u + 1));
+    HIP_TRY(ctx->ppos.need(((size_t)n_u + 1) * 4));
+    hipLaunchKernelGGL(k_closure_split, dim3(ceil_div(n_miss, 256)), dim3(256),
+                       0, s, n_miss, ctx->miss_b.as<uint64_t>(),
+                       ctx->umark.as<int32_t>(), ctx->upos.as<int32_t>(), R.M,
+                       R.S, ctx->ukeys.as<uint64_t>(),
+                       ctx->upass.as<uint8_t>());
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_pass32, dim3(ceil_div((int64_t)n_u + 1, 256)),
+                       dim3(256), 0, s, (int64_t)n_u, ctx->upass.as<uint8_t>(),
+                       ctx->umark.as<int32_t>());
+    HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->umark.as<int32_t>(),
+                                   ctx->ppos.as<int32_t>(), n_u + 1));
+    int32_t n_new = 0;
+    HIP_TRY(hipMemcpyAsync(&n_new, ctx->ppos.as<int32_t>() + n_u, 4,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(ctx->cand.need_keep((size_t)(c_hi + n_new) * sizeof(Cand),
+                                (size_t)c_hi * sizeof(Cand), s));
+    if (n_u > 0)
+      hipLaunchKernelGGL(k_closure_fill, dim3(ceil_div(n_u, 256)), dim3(256),
+                         0, s, (int64_t)n_u, ctx->ukeys.as<uint64_t>(),
+                         ctx->upass.as<uint8_t>(), ctx->ppos.as<int32_t>(),
+                         ctx->cand.as<Cand>(), c_hi,
+                         ctx->reopened.as<uint8_t>());
+    HIP_TRY(hipGetLastError());
+    // ext += the unique pairs (disjoint from it), sorted again
+    HIP_TRY(ctx->mem_a.need_keep((size_t)(n_ext + n_u) * 8, (size_t)n_ext * 8,
+                                 s));
+    HIP_TRY(hipMemcpyAsync(ctx->mem_a.as<uint64_t>() + n_ext, ctx->ukeys.p,
+                           (size_t)n_u * 8, hipMemcpyDeviceToDevice, s));
+    n_ext += n_u;
+    HIP_TRY(ctx->mem_b.need((size_t)n_ext * 8));
+    if (int rc = sort_u64(ctx, s, ctx->mem_a.as<uint64_t>(),
+                          ctx->mem_b.as<uint64_t>(), n_ext))
+      return rc;
+    ctx->mem_a.swap(ctx->mem_b);
+    if (n_new > 0)
+      if (int rc = clip_range(ctx, R, c_hi, c_hi + n_new, false)) return rc;
+    added += n_new;
+    c_lo = c_hi;
+    c_hi += n_new;
+  }
+  // tets without a cell, with a dropped cell or left open: full relation
   hipLaunchKernelGGL(k_flag_final, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
                      nt, ctx->flag.as<uint8_t>(), ctx->has_cell.as<uint8_t>(),
                      ctx->flag32.as<int32_t>());
@@ -954,11 +1064,18 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
   HIP_TRY(hipMemsetAsync(ctx->flag32.as<int32_t>() + nt, 0, 4, s));
   HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->flag32.as<int32_t>(),
                                  ctx->flag_pos.as<int32_t>(), nt + 1));
-  int32_t nf = 0;
+  HIP_TRY(hipMemsetAsync(R.cnt + 5, 0, 4, s));
+  hipLaunchKernelGGL(k_count_reopened, dim3(ceil_div(nt, 256)), dim3(256), 0,
+                     s, nt, ctx->reopened.as<uint8_t>(),
+                     ctx->flag32.as<int32_t>(), R.cnt + 5);
+  HIP_TRY(hipGetLastError());
+  int32_t nf = 0, n_reopened = 0;
   HIP_TRY(hipMemcpyAsync(&nf, ctx->flag_pos.as<int32_t>() + nt, 4,
                          hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&n_reopened, R.cnt + 5, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   int64_t n_fb = 0;
+  const int64_t n_brute = c_hi;
   if (nf > 0) {
     HIP_TRY(ctx->flag_list.need((size_t)nf * 4));
     hipLaunchKernelGGL(k_flag_list, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
@@ -978,58 +1095,86 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
     HIP_TRY(hipMemcpyAsync(&n_fb, ctx->brute_off.as<int64_t>() + nf, 8,
                            hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    HIP_TRY(ctx->cand.need_keep((size_t)(n1 + n_fb) * sizeof(Cand),
-                                (size_t)n1 * sizeof(Cand), s));
+    HIP_TRY(ctx->cand.need_keep((size_t)(c_hi + n_fb) * sizeof(Cand),
+                                (size_t)c_hi * sizeof(Cand), s));
     hipLaunchKernelGGL(k_brute, dim3(nf), dim3(64), 0, s, nf,
                        ctx->flag_list.as<int32_t>(), R.M, R.S, R.inst_d,
                        R.tet_inst_d, ctx->brute_off.as<int64_t>(),
-                       (int64_t*)nullptr, ctx->cand.as<Cand>(), n1);
+                       (int64_t*)nullptr, ctx->cand.as<Cand>(), c_hi);
     HIP_TRY(hipGetLastError());
-    if (int rc = clip_range(ctx, R, n1, n1 + n_fb, false)) return rc;
+    if (int rc = clip_range(ctx, R, c_hi, c_hi + n_fb, false)) return rc;
   }
-  const int64_t n_all = n1 + n_fb;
+  const int64_t n_all = c_hi + n_fb;
   ctx->st.learned[0] = n1;
-  ctx->st.learned[1] = nf;
-  ctx->st.learned[2] = n_fb;
-  // output order: included candidates by (tet, site)
-  HIP_TRY(ctx->keys_a.need((size_t)std::max<int64_t>(n_all, 1) * 8));
-  HIP_TRY(ctx->keys_b.need((size_t)std::max<int64_t>(n_all, 1) * 8));
-  HIP_TRY(ctx->vals_a.need((size_t)std::max<int64_t>(n_all, 1) * 4));
-  HIP_TRY(ctx->perm.need((size_t)std::max<int64_t>(n_all, 1) * 4));
+  ctx->st.learned[1] = n_reopened;
+  ctx->st.learned[2] = added + n_fb;
+  ctx->st.learned[3] = nf;
+  // output order: A (first round, kept order) merged per tet with the
+  // sorted added candidates B
+  const int64_t nB = n_all - n1;
+  HIP_TRY(ctx->keys_a.need((size_t)std::max<int64_t>(nB, 1) * 8));
+  HIP_TRY(ctx->keys_b.need((size_t)std::max<int64_t>(nB, 1) * 8));
+  HIP_TRY(ctx->vals_a.need((size_t)std::max<int64_t>(nB, 1) * 4));
+  HIP_TRY(ctx->vals_b.need((size_t)std::max<int64_t>(nB, 1) * 4));
   HIP_TRY(hipMemsetAsync(R.cnt + 3, 0, 4, s));
-  if (n_all > 0)
-    hipLaunchKernelGGL(k_final_keys, dim3(ceil_div(n_all, 256)), dim3(256), 0,
-                       s, n_all, n1, ctx->cand.as<Cand>(),
-                       ctx->flag.as<uint8_t>(), ctx->excl.as<uint8_t>(),
-                       ctx->keys_a.as<uint64_t>(), ctx->vals_a.as<int32_t>(),
-                       R.cnt + 3);
-  HIP_TRY(hipGetLastError());
-  if (n_all > 0) {
+  if (nB > 0) {
+    hipLaunchKernelGGL(k_b_keys, dim3(ceil_div(nB, 256)), dim3(256), 0, s, n1,
+                       n_all, n_brute, ctx->cand.as<Cand>(),
+                       ctx->flag.as<uint8_t>(), ctx->keys_a.as<uint64_t>(),
+                       ctx->vals_a.as<int32_t>(), R.cnt + 3);
+    HIP_TRY(hipGetLastError());
     size_t tmp = 0;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
         nullptr, tmp, ctx->keys_a.as<uint64_t>(), ctx->keys_b.as<uint64_t>(),
-        ctx->vals_a.as<int32_t>(), ctx->perm.as<int32_t>(), (int)n_all, 0, 64,
+        ctx->vals_a.as<int32_t>(), ctx->vals_b.as<int32_t>(), (int)nB, 0, 64,
         s));
     HIP_TRY(ctx->scan_tmp.need(tmp));
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
         ctx->scan_tmp.p, tmp, ctx->keys_a.as<uint64_t>(),
         ctx->keys_b.as<uint64_t>(), ctx->vals_a.as<int32_t>(),
-        ctx->perm.as<int32_t>(), (int)n_all, 0, 64, s));
+        ctx->vals_b.as<int32_t>(), (int)nB, 0, 64, s));
   }
-  int32_t n_incl = 0;
-  HIP_TRY(hipMemcpyAsync(&n_incl, R.cnt + 3, 4, hipMemcpyDeviceToHost, s));
+  int32_t n_b = 0;
+  HIP_TRY(hipMemcpyAsync(&n_b, R.cnt + 3, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  // relation CSR of the final candidate sets
-  HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_incl, 1) * 4));
-  HIP_TRY(hipMemsetAsync(ctx->tet_count.p, 0, ((size_t)nt + 1) * 8, s));
-  if (n_incl > 0)
-    hipLaunchKernelGGL(k_final_rel, dim3(ceil_div(n_incl, 256)), dim3(256), 0,
-                       s, (int64_t)n_incl, ctx->cand.as<Cand>(),
-                       ctx->perm.as<int32_t>(), ctx->tet_count.as<int64_t>(),
-                       ctx->rel_ids.as<int32_t>());
+  HIP_TRY(ctx->out_off.need(((size_t)nt + 1) * 8));
+  HIP_TRY(ctx->out_cnt.need(((size_t)nt + 1) * 8));
+  hipLaunchKernelGGL(k_out_counts, dim3(ceil_div(nt + 1, 256)), dim3(256), 0,
+                     s, nt, ctx->flag.as<uint8_t>(), ctx->reopened.as<uint8_t>(),
+                     ctx->tet_off.as<int64_t>(), ctx->keys_b.as<uint64_t>(),
+                     (int64_t)n_b, ctx->out_cnt.as<int64_t>());
   HIP_TRY(hipGetLastError());
-  HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
-                                 ctx->tet_off.as<int64_t>(), nt + 1));
+  HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->out_cnt.as<int64_t>(),
+                                 ctx->out_off.as<int64_t>(), nt + 1));
+  int64_t n_incl = 0;
+  HIP_TRY(hipMemcpyAsync(&n_incl, ctx->out_off.as<int64_t>() + nt, 8,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(ctx->perm.need((size_t)std::max<int64_t>(n_incl, 1) * 4));
+  if (n1 > 0)
+    hipLaunchKernelGGL(k_place_a, dim3(ceil_div(n1, 256)), dim3(256), 0, s, n1,
+                       ctx->cand.as<Cand>(), ctx->flag.as<uint8_t>(),
+                       ctx->reopened.as<uint8_t>(), ctx->tet_off.as<int64_t>(),
+                       ctx->keys_b.as<uint64_t>(), (int64_t)n_b,
+                       ctx->out_off.as<int64_t>(), ctx->perm.as<int32_t>());
+  if (n_b > 0)
+    hipLaunchKernelGGL(k_place_b, dim3(ceil_div(n_b, 256)), dim3(256), 0, s,
+                       (int64_t)n_b, ctx->keys_b.as<uint64_t>(),
+                       ctx->vals_b.as<int32_t>(), ctx->flag.as<uint8_t>(),
+                       ctx->tet_off.as<int64_t>(), ctx->rel_ids.as<int32_t>(),
+                       ctx->out_off.as<int64_t>(), ctx->perm.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  // relation CSR of the output sets (tet_off, rel_ids of rpd_download_*)
+  HIP_TRY(ctx->out_ids.need((size_t)std::max<int64_t>(n_incl, 1) * 4));
+  if (n_incl > 0)
+    hipLaunchKernelGGL(k_out_ids, dim3(ceil_div(n_incl, 256)), dim3(256), 0, s,
+                       n_incl, ctx->cand.as<Cand>(), ctx->perm.as<int32_t>(),
+                       ctx->out_ids.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  ctx->tet_off.swap(ctx->out_off);
+  ctx->rel_ids.swap(ctx->out_ids);
+  ctx->tet_count.swap(ctx->out_cnt);
   *n_incl_out = n_incl;
   return 0;
 }
@@ -1610,8 +1755,11 @@ int rpd_mlp_scores(rpd_ctx* ctx, float* logits) {
   HIP_TRY(ctx->tr_grad.need((size_t)nt * kOut * 4));
   const int64_t tiles = (nt + 15) / 16;
   hipLaunchKernelGGL(k_mlp_infer,
-                     dim3((int)std::min<int64_t>((tiles + 3) / 4, 8192)),
-                     dim3(256), 0, ctx->stream, mlp_dev(ctx),
+                     dim3((int)std::max<int64_t>(
+                         1, std::min<int64_t>(
+                                (tiles + kInferWaves - 1) / kInferWaves,
+                                1024))),
+                     dim3(64 * kInferWaves), 0, ctx->stream, mlp_dev(ctx),
                      ctx->feat.as<uint16_t>(), (int64_t)nt, 0.f,
                      ctx->sel.as<uint32_t>(), ctx->tr_grad.as<float>());
   HIP_TRY(hipGetLastError());

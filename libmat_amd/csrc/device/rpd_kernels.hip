@@ -657,8 +657,11 @@ struct ClipArgs {
 
 template <int L>
 __device__ __forceinline__ uint64_t group_bits(uint64_t b, int g) {
-  if (L == 64) return b;
-  return (b >> (g * L)) & ((1ull << L) - 1);
+  if constexpr (L == 64) {
+    return b;
+  } else {
+    return (b >> (g * L)) & ((1ull << L) - 1);
+  }
 }
 
 // ---- FP64 geometry helpers (tolerance-checked, not bit-compared)

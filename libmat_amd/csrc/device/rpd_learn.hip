@@ -15,10 +15,13 @@
 // certificate then checks every tet: each active bisector facet of each of
 // its cells must lead to a sphere of its proposal set, no cell may have
 // been dropped, and some cell must exist (the cells of a set closed under
-// facet neighbours partition the tet).  Tets that fail are re-run with the
-// reference relation (voronoi.cu:154-193) over every sphere of their
-// instance.  Cells are clipped exactly as on the plain path, so the output
-// is the plain path's.
+// facet neighbours partition the tet).  Only proposals passing the
+// reference relation (voronoi.cu:154-193) are clipped.  A missing facet
+// neighbour is added (if related) and clipped, and the new cells are
+// certified in turn, until every tet's set is closed; tets with a dropped
+// cell, without any cell, or not closed after a few rounds are re-run with
+// the relation over every sphere of their instance.  Cells are clipped as
+// on the plain path, so the output is the plain path's.
 //
 // Weights are row-major [in][out] (fp32 master copy); the MFMA kernels read
 // bf16 copies: transposed [out][in] for the forward pass (B operand of
@@ -194,10 +197,7 @@ __global__ __launch_bounds__(256) void k_vertex_owner(
 __global__ __launch_bounds__(64) void k_knn_features(
     DevMesh M, DevSites S, const InstInfo* inst, const int32_t* tet_inst,
     const int32_t* vowner, int32_t* knn, uint16_t* X, uint32_t* omask) {
-  __shared__ float s_key[kOut][64];
-  __shared__ int32_t s_id[kOut][64];
-  const int lane = threadIdx.x;
-  const int t = blockIdx.x * 64 + lane;
+  const int t = blockIdx.x * 64 + threadIdx.x;
   if (t >= M.n_tet) return;
   TetLoad L;
   load_tet(M, t, L);
@@ -223,7 +223,31 @@ __global__ __launch_bounds__(64) void k_knn_features(
   own[2] = vowner[L.tv.y];
   own[3] = vowner[L.tv.z];
   own[4] = vowner[L.tv.w];
-  int cnt = 0;
+  // the 32 least (key, id), sorted, in registers: a new entry bubbles
+  // through the list (fully unrolled compare/select, no divergence); an
+  // entry equal to one listed is dropped
+  float kk[kOut];
+  int32_t ii[kOut];
+#pragma unroll
+  for (int q = 0; q < kOut; ++q) {
+    kk[q] = INFINITY;
+    ii[q] = 0x7fffffff;
+  }
+  auto ins = [&](float key, int32_t id) {
+    if (!(key < kk[kOut - 1] || (key == kk[kOut - 1] && id < ii[kOut - 1])))
+      return;
+#pragma unroll
+    for (int q = 0; q < kOut; ++q) {
+      const bool eq = key == kk[q] && id == ii[q];
+      const bool lt = key < kk[q] || (key == kk[q] && id < ii[q]);
+      const float ck = lt ? kk[q] : key;
+      const int32_t ci = lt ? ii[q] : id;
+      kk[q] = lt ? key : kk[q];
+      ii[q] = lt ? id : ii[q];
+      key = eq ? INFINITY : ck;
+      id = eq ? 0x7fffffff : ci;
+    }
+  };
   uint32_t om = 0;
   if (own[1] >= 0) {
     for (int k = 1; k < 5; ++k) pmin[k] = pdist(P[k], S.site[own[k]]);
@@ -239,52 +263,42 @@ __global__ __launch_bounds__(64) void k_knn_features(
     }
     own[0] = descend(S, P[0], st, &best);
     pmin[0] = best;
-    auto key_of = [&](const double4& s) {
-      double k = INFINITY;
-#pragma unroll
-      for (int j = 0; j < 5; ++j) k = fmin(k, pdist(P[j], s) - pmin[j]);
-      return k;
-    };
-    auto ins = [&](float key, int id) {
-      for (int q = 0; q < cnt; ++q)
-        if (s_id[q][lane] == id) return;
-      if (cnt == kOut &&
-          !(key < s_key[kOut - 1][lane] ||
-            (key == s_key[kOut - 1][lane] && id < s_id[kOut - 1][lane])))
-        return;
-      int j = cnt < kOut ? cnt : kOut - 1;
-      while (j > 0 && (s_key[j - 1][lane] > key ||
-                       (s_key[j - 1][lane] == key && s_id[j - 1][lane] > id))) {
-        s_key[j][lane] = s_key[j - 1][lane];
-        s_id[j][lane] = s_id[j - 1][lane];
-        --j;
-      }
-      s_key[j][lane] = key;
-      s_id[j][lane] = id;
-      if (cnt < kOut) ++cnt;
-    };
+    // rings: the centroid owner's, and a corner owner's only if that
+    // owner is not already in it (the rings overlap otherwise)
+    const int64_t r0 = S.knn_off[own[0]], r1 = S.knn_off[own[0] + 1];
     for (int j = 0; j < 5; ++j) {
       const int o = own[j];
       bool seen = false;
       for (int q = 0; q < j; ++q) seen = seen || own[q] == o;
       if (seen) continue;
       if (S.flags[o] == 1u) ins(0.f, o);
+      if (j > 0) {
+        bool in_ring = false;
+        for (int64_t q = r0; q < r1 && !in_ring; ++q)
+          in_ring = S.knn_near[q] == o;
+        if (in_ring) continue;
+      }
       for (int64_t q = S.knn_off[o]; q < S.knn_off[o + 1]; ++q) {
         const int id = S.knn_near[q];
         if (S.flags[id] != 1u) continue;
-        ins((float)key_of(S.knn_site_near[q]), id);
+        const double4 sp = S.knn_site_near[q];
+        double key = INFINITY;
+#pragma unroll
+        for (int a = 0; a < 5; ++a) key = fmin(key, pdist(P[a], sp) - pmin[a]);
+        ins((float)key, id);
       }
     }
-    for (int k = 0; k < cnt; ++k)
-      if (s_key[k][lane] == 0.f) {
-        const int id = s_id[k][lane];
+#pragma unroll
+    for (int k = 0; k < kOut; ++k)
+      if (kk[k] == 0.f) {
         bool is_owner = false;
-        for (int j = 0; j < 5; ++j) is_owner = is_owner || own[j] == id;
+        for (int a = 0; a < 5; ++a) is_owner = is_owner || own[a] == ii[k];
         if (is_owner) om |= 1u << k;
       }
   }
   omask[t] = om;
-  for (int k = 0; k < kOut; ++k) kt[k] = k < cnt ? s_id[k][lane] : -1;
+#pragma unroll
+  for (int k = 0; k < kOut; ++k) kt[k] = kk[k] < INFINITY ? ii[k] : -1;
   const double ext =
       fmax(fmax(I.dom[3] - I.dom[0], I.dom[4] - I.dom[1]), I.dom[5] - I.dom[2]);
   const double inv_ext = ext > 0 ? 1.0 / ext : 0.0;
@@ -297,13 +311,13 @@ __global__ __launch_bounds__(64) void k_knn_features(
   f[2] = (float)((P[0][2] - I.dom[2]) * inv_ext);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    if (k < cnt) {
-      const double4 sp = S.site[s_id[k][lane]];
+    if (kk[k] < INFINITY) {
+      const double4 sp = S.site[ii[k]];
       f[3 + 5 * k + 0] = (float)((sp.x - P[0][0]) * inv_r);
       f[3 + 5 * k + 1] = (float)((sp.y - P[0][1]) * inv_r);
       f[3 + 5 * k + 2] = (float)((sp.z - P[0][2]) * inv_r);
       f[3 + 5 * k + 3] = (float)(sqrt(sp.w) * inv_r);
-      f[3 + 5 * k + 4] = (float)(s_key[k][lane] * (inv_r * inv_r));
+      f[3 + 5 * k + 4] = (float)(kk[k] * (inv_r * inv_r));
     }
   }
 #pragma unroll
@@ -423,24 +437,83 @@ __device__ __forceinline__ void mlp_forward_tile(const MlpDev& P,
   }
 }
 
-// Inference: selection mask per row (bit k: logit_k > thr); optional logits
-__global__ __launch_bounds__(256) void k_mlp_infer(MlpDev P,
-                                                   const uint16_t* X,
-                                                   int64_t n, float thr,
-                                                   uint32_t* sel,
-                                                   float* logits) {
-  __shared__ WaveLds s_w[4];
+// Inference: selection mask per row (bit k: logit_k > thr); optional
+// logits.  Eight waves share the weights, staged once per block in LDS
+// (rows padded against bank conflicts); each wave runs 16-row tiles with
+// one activation tile in LDS.
+constexpr int kS1 = kIn + 8, kS2 = kHid + 8;
+constexpr int kInferWaves = 8;
+
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(
+    MlpDev P, const uint16_t* X, int64_t n, float thr, uint32_t* sel,
+    float* logits) {
+  __shared__ uint16_t s_w1[kHid * kS1];
+  __shared__ uint16_t s_w2[kHid * kS2];
+  __shared__ uint16_t s_w3[kOut * kS2];
+  __shared__ uint16_t s_act[kInferWaves][16][kActStride];
+  for (int q = threadIdx.x; q < kHid * kIn; q += 64 * kInferWaves)
+    s_w1[(q / kIn) * kS1 + q % kIn] = P.w1t[q];
+  for (int q = threadIdx.x; q < kHid * kHid; q += 64 * kInferWaves)
+    s_w2[(q / kHid) * kS2 + q % kHid] = P.w2t[q];
+  for (int q = threadIdx.x; q < kOut * kHid; q += 64 * kInferWaves)
+    s_w3[(q / kHid) * kS2 + q % kHid] = P.w3t[q];
+  __syncthreads();
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = l % 16, rq = 4 * (l / 16);
+  uint16_t(*act)[kActStride] = s_act[wv];
   const int64_t n_tiles = (n + 15) / 16;
-  for (int64_t tile = (int64_t)blockIdx.x * 4 + wv; tile < n_tiles;
-       tile += (int64_t)gridDim.x * 4) {
+  for (int64_t tile = (int64_t)blockIdx.x * kInferWaves + wv; tile < n_tiles;
+       tile += (int64_t)gridDim.x * kInferWaves) {
     const int64_t row0 = tile * 16;
-    const int64_t my_row = row0 + l % 16;
+    const int64_t my_row = row0 + col;
     bf16x4 a1[3];
     load_x(X, my_row < n ? my_row : -1, l, a1);
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 3; ++kb)
+        acc = mfma(a1[kb], ld_bf4(&s_w1[(16 * nb + col) * kS1 + 16 * kb + rq]),
+                   acc);
+      const float b = P.b1[16 * nb + col];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        act[rq + i][16 * nb + col] = f2bf(fmaxf(acc[i] + b, 0.f));
+    }
+    wsync_l();
+    bf16x4 a2[8];
+#pragma unroll
+    for (int kb = 0; kb < 8; ++kb) a2[kb] = ld_bf4(&act[col][16 * kb + rq]);
+    wsync_l();
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 8; ++kb)
+        acc = mfma(a2[kb],
+                   ld_bf4(&s_w2[(16 * nb + col) * kS2 + 16 * kb + rq]), acc);
+      const float b = P.b2[16 * nb + col];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        act[rq + i][16 * nb + col] = f2bf(fmaxf(acc[i] + b, 0.f));
+    }
+    wsync_l();
+    bf16x4 a3[8];
+#pragma unroll
+    for (int kb = 0; kb < 8; ++kb) a3[kb] = ld_bf4(&act[col][16 * kb + rq]);
     f32x4 z[2];
-    mlp_forward_tile(P, a1, l, s_w[wv], z);
-    const int rq = 4 * (l / 16), col = l % 16;
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 8; ++kb)
+        acc = mfma(a3[kb],
+                   ld_bf4(&s_w3[(16 * nb + col) * kS2 + 16 * kb + rq]), acc);
+      const float b = P.b3[16 * nb + col];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] += b;
+      z[nb] = acc;
+    }
     uint64_t bm[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -449,8 +522,10 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev P,
     if (l < 16) {
       // row l = 4 g + i: lanes 16 g .. 16 g + 15 of ballot (i, nb)
       const int g = l / 4, i = l % 4;
-      uint64_t b0 = i == 0 ? bm[0][0] : (i == 1 ? bm[1][0] : (i == 2 ? bm[2][0] : bm[3][0]));
-      uint64_t b1 = i == 0 ? bm[0][1] : (i == 1 ? bm[1][1] : (i == 2 ? bm[2][1] : bm[3][1]));
+      const uint64_t b0 =
+          i == 0 ? bm[0][0] : (i == 1 ? bm[1][0] : (i == 2 ? bm[2][0] : bm[3][0]));
+      const uint64_t b1 =
+          i == 0 ? bm[0][1] : (i == 1 ? bm[1][1] : (i == 2 ? bm[2][1] : bm[3][1]));
       const uint32_t m = (uint32_t)((b0 >> (16 * g)) & 0xffffu) |
                          ((uint32_t)((b1 >> (16 * g)) & 0xffffu) << 16);
       if (row0 + l < n) sel[row0 + l] = m;
@@ -689,110 +764,119 @@ __global__ __launch_bounds__(256) void k_sel_topk(int32_t n_tet, int32_t k,
   sel[t] = k >= 32 ? ~0u : (k <= 0 ? 0u : ((1u << k) - 1u));
 }
 
-// proposals of tet t: the owner slots always, others if selected
-__device__ __forceinline__ int proposals(const int32_t* knn, uint32_t sel,
-                                         int t, int32_t out[kOut]) {
-  int n = 0;
-  for (int k = 0; k < kOut; ++k) {
-    const int id = knn[(int64_t)t * kOut + k];
-    if (id < 0) break;
-    if ((sel >> k) & 1u) out[n++] = id;
-  }
-  // ascending site id (the clip order of the candidate list)
-  for (int a = 1; a < n; ++a) {
-    const int v = out[a];
-    int b = a;
-    while (b > 0 && out[b - 1] > v) {
-      out[b] = out[b - 1];
-      --b;
-    }
-    out[b] = v;
-  }
-  return n;
-}
-
-__global__ __launch_bounds__(64) void k_learn_count(int32_t n_tet,
+// selected proposals (owners always) that pass the reference relation
+// (voronoi.cu:154-193): their slot mask replaces sel, their count is
+// written; only such spheres are clipped, as on the plain path
+__global__ __launch_bounds__(64) void k_learn_count(int32_t n_tet, DevMesh M,
+                                                    DevSites S,
                                                     const int32_t* knn,
-                                                    const uint32_t* sel,
+                                                    uint32_t* sel,
                                                     const uint32_t* omask,
                                                     int64_t* count) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_tet) return;
-  int n = 0;
+  TetLoad L;
+  load_tet(M, t, L);
   const uint32_t s = sel[t] | omask[t];
+  uint32_t pass = 0;
+  int n = 0;
   for (int k = 0; k < kOut; ++k) {
-    if (knn[(int64_t)t * kOut + k] < 0) break;
-    n += ((s >> k) & 1u) ? 1 : 0;
+    const int id = knn[(int64_t)t * kOut + k];
+    if (id < 0) break;
+    if (!((s >> k) & 1u)) continue;
+    uint32_t it = 0;
+    if (relation(S, L, id, &it)) {
+      pass |= 1u << k;
+      ++n;
+    }
   }
+  sel[t] = pass;
   count[t] = n;
 }
 
 __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
                                                    const int32_t* knn,
                                                    const uint32_t* sel,
-                                                   const uint32_t* omask,
                                                    const int64_t* off,
                                                    Cand* cand,
                                                    int32_t* rel_ids) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_tet) return;
-  int32_t ids[kOut];
-  const int n = proposals(knn, sel[t] | omask[t], t, ids);
+  // selected ids ascending: bitonic sorting network on 32 registers
+  const uint32_t m = sel[t];
+  int32_t v[kOut];
+#pragma unroll
+  for (int k = 0; k < kOut; ++k) {
+    const int32_t id = knn[(int64_t)t * kOut + k];
+    v[k] = ((m >> k) & 1u) && id >= 0 ? id : 0x7fffffff;
+  }
+#pragma unroll
+  for (int size = 2; size <= kOut; size <<= 1)
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+      for (int i = 0; i < kOut; ++i) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const int32_t a = v[i], b = v[j];
+          const bool sw = up ? a > b : a < b;
+          v[i] = sw ? b : a;
+          v[j] = sw ? a : b;
+        }
+      }
   const int64_t o = off[t];
-  for (int k = 0; k < n; ++k) {
-    Cand c;
-    c.tet = t;
-    c.seed = ids[k];
-    c.m0 = 0;
-    c.m1 = 0;
-    cand[o + k] = c;
-    rel_ids[o + k] = ids[k];
+  const int n = __popc(m);
+#pragma unroll
+  for (int k = 0; k < kOut; ++k) {
+    if (k < n) {
+      Cand c;
+      c.tet = t;
+      c.seed = v[k];
+      c.m0 = 0;
+      c.m1 = 0;
+      cand[o + k] = c;
+      rel_ids[o + k] = v[k];
+    }
   }
 }
 
-// Certificate over candidates [0, n): flags tets whose proposal set is not
-// shown complete; marks for exclusion the near-zero cells of spheres the
-// reference relation rejects (only such slivers can differ between the
-// relation's rounding and the exact clip).
+// Certificate of the cells of candidates [c0, c1): every active bisector
+// facet (i, j) of a cell of tet t needs j in t's member set: the first-round
+// set (sorted CSR tet_off / mem_ids) or the sorted keys ext of pairs added
+// since.  Missing pairs (t << 32 | j) are appended to miss (capacity cap;
+// beyond it, and for dropped cells, the tet is flagged for the
+// full-relation re-run).
+__device__ __forceinline__ bool has_key(const uint64_t* keys, int64_t n,
+                                        uint64_t k) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < n && keys[lo] == k;
+}
+
 __global__ __launch_bounds__(256) void k_certify(
-    int64_t n, DevMesh M, DevSites S, const Cand* cand, const int32_t* loc,
-    TierViews tv, const int64_t* tet_off, const int32_t* rel_ids,
-    int32_t geometry, uint8_t* flag, uint8_t* has_cell, uint8_t* excl) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const int t = cand[c].tet, s = cand[c].seed;
+    int64_t c0, int64_t c1, const Cand* cand, const int32_t* loc,
+    TierViews tv, const int64_t* tet_off, const int32_t* mem_ids,
+    const uint64_t* ext, int64_t n_ext, uint64_t* miss, int32_t cap,
+    int32_t* n_miss, uint8_t* flag, uint8_t* has_cell) {
+  const int64_t c = c0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= c1) return;
+  const int t = cand[c].tet;
   const int32_t l = loc[c];
   const int tier = l >> 28;
   const int64_t idx = tier ? (l & ((1 << 28) - 1)) : c;
   const SlotView& sv = tv.t[tier];
   const uint32_t hd = sv.hdr[idx];
   const uint32_t st = hd >> 24;
-  excl[c] = 0;
   if (st == RPD_NO_INTERSECTION) return;
   if (st != RPD_SUCCESS) {
     flag[t] = 1;
-    return;
-  }
-  TetLoad L;
-  load_tet(M, t, L);
-  bool keep = true;
-  if (geometry) {
-    const double4 a = L.V[0], b = L.V[1], cc = L.V[2], d = L.V[3];
-    const double ux = b.x - a.x, uy = b.y - a.y, uz = b.z - a.z;
-    const double vx = cc.x - a.x, vy = cc.y - a.y, vz = cc.z - a.z;
-    const double wx = d.x - a.x, wy = d.y - a.y, wz = d.z - a.z;
-    const double vol6 = fabs(ux * (vy * wz - vz * wy) - uy * (vx * wz - vz * wx) +
-                             uz * (vx * wy - vy * wx));
-    if (sv.geom[4 * idx] < 1e-9 * vol6) {
-      uint32_t it = 0;
-      keep = relation(S, L, s, &it);
-    }
-  } else {
-    uint32_t it = 0;
-    keep = relation(S, L, s, &it);
-  }
-  if (!keep) {
-    excl[c] = 1;
     return;
   }
   has_cell[t] = 1;
@@ -817,16 +901,77 @@ __global__ __launch_bounds__(256) void k_certify(
     int64_t lo = o0, hi = o1;
     while (lo < hi) {
       const int64_t mid = (lo + hi) >> 1;
-      if (rel_ids[mid] < src)
+      if (mem_ids[mid] < src)
         lo = mid + 1;
       else
         hi = mid;
     }
-    if (lo == o1 || rel_ids[lo] != src) {
+    if (lo < o1 && mem_ids[lo] == src) continue;
+    const uint64_t key = ((uint64_t)(uint32_t)t << 32) | (uint32_t)src;
+    if (n_ext && has_key(ext, n_ext, key)) continue;
+    const int r = atomicAdd(n_miss, 1);
+    if (r < cap)
+      miss[r] = key;
+    else
       flag[t] = 1;
-      return;
-    }
   }
+}
+
+// tets of missing pairs left after the last closure round
+__global__ __launch_bounds__(256) void k_flag_miss(int64_t n,
+                                                   const uint64_t* miss,
+                                                   uint8_t* flag) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) flag[miss[q] >> 32] = 1;
+}
+
+// sorted keys -> first occurrences (1) / repeats (0)
+__global__ __launch_bounds__(256) void k_unique_mark(int64_t n,
+                                                     const uint64_t* keys,
+                                                     int32_t* mark) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > n) return;
+  mark[q] = q < n && (q == 0 || keys[q] != keys[q - 1]) ? 1 : 0;
+}
+
+// unique missing pairs: the relation decides which become candidates
+// (appended at cand_base in key order); all become members
+__global__ __launch_bounds__(256) void k_closure_split(
+    int64_t n, const uint64_t* keys, const int32_t* mark, const int32_t* pos,
+    DevMesh M, DevSites S, uint64_t* ukeys, uint8_t* pass) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n || !mark[q]) return;
+  const uint64_t k = keys[q];
+  const int64_t u = pos[q];
+  ukeys[u] = k;
+  TetLoad L;
+  load_tet(M, (int)(k >> 32), L);
+  uint32_t it = 0;
+  pass[u] = relation(S, L, (int)(k & 0xffffffffu), &it) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_closure_fill(int64_t n_u,
+                                                      const uint64_t* ukeys,
+                                                      const uint8_t* pass,
+                                                      const int32_t* ppos,
+                                                      Cand* cand,
+                                                      int64_t cand_base,
+                                                      uint8_t* reopened) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n_u || !pass[u]) return;
+  reopened[ukeys[u] >> 32] = 1;
+  Cand c;
+  c.tet = (int)(ukeys[u] >> 32);
+  c.seed = (int)(ukeys[u] & 0xffffffffu);
+  c.m0 = 0;
+  c.m1 = 0;
+  cand[cand_base + ppos[u]] = c;
+}
+
+__global__ __launch_bounds__(256) void k_pass32(int64_t n, const uint8_t* p,
+                                                int32_t* out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q <= n) out[q] = q < n ? p[q] : 0;
 }
 
 __global__ __launch_bounds__(256) void k_flag_final(int32_t n_tet,
@@ -838,6 +983,20 @@ __global__ __launch_bounds__(256) void k_flag_final(int32_t n_tet,
   const uint8_t f = flag[t] || !has_cell[t];
   flag[t] = f;
   flag32[t] = f;
+}
+
+// tets re-run in any way (closure or full relation)
+__global__ __launch_bounds__(256) void k_count_reopened(int32_t n_tet,
+                                                        const uint8_t* reopened,
+                                                        const int32_t* flag32,
+                                                        int32_t* n) {
+  __shared__ int32_t c;
+  if (threadIdx.x == 0) c = 0;
+  __syncthreads();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n_tet && (reopened[t] || flag32[t])) atomicAdd(&c, 1);
+  __syncthreads();
+  if (threadIdx.x == 0 && c) atomicAdd(n, c);
 }
 
 __global__ __launch_bounds__(256) void k_flag_list(int32_t n_tet,
@@ -885,30 +1044,108 @@ __global__ __launch_bounds__(64) void k_brute(
   if (count && lane == 0) count[f] = n;
 }
 
-// keys of the final candidate order; excluded candidates sort last
-__global__ __launch_bounds__(256) void k_final_keys(
-    int64_t n, int64_t n_first, const Cand* cand, const uint8_t* flag,
-    const uint8_t* excl, uint64_t* keys, int32_t* vals, int32_t* n_incl) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+// Output order.  A = first-round candidates [0, n1) in (tet, id) order
+// (CSR a_off / a_ids); B = the candidates added after, sorted by key.  A
+// candidate of a tet re-run on the full relation (flag) is left out, as
+// are B entries before n_brute of such tets.  Each tet's output is A and B
+// merged by id.
+__global__ __launch_bounds__(256) void k_b_keys(int64_t n1, int64_t n_all,
+                                                int64_t n_brute,
+                                                const Cand* cand,
+                                                const uint8_t* flag,
+                                                uint64_t* keys, int32_t* vals,
+                                                int32_t* n_b) {
+  const int64_t c = n1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_all) return;
   const Cand cd = cand[c];
-  const bool out = c < n_first && (flag[cd.tet] || excl[c]);
-  keys[c] = out ? ~0ull : (((uint64_t)(uint32_t)cd.tet << 32) | (uint32_t)cd.seed);
-  vals[c] = (int32_t)c;
-  if (!out) atomicAdd(n_incl, 1);
+  const bool out = c < n_brute && flag[cd.tet];
+  keys[c - n1] =
+      out ? ~0ull : (((uint64_t)(uint32_t)cd.tet << 32) | (uint32_t)cd.seed);
+  vals[c - n1] = (int32_t)c;
+  if (!out) atomicAdd(n_b, 1);
 }
 
-// per-tet CSR of the final candidate sets
-__global__ __launch_bounds__(256) void k_final_rel(int64_t n_incl,
-                                                   const Cand* cand,
-                                                   const int32_t* perm,
-                                                   int64_t* tet_count,
-                                                   int32_t* rel_ids) {
+__device__ __forceinline__ int64_t lower_u64(const uint64_t* keys, int64_t n,
+                                             uint64_t k) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_out_counts(
+    int32_t n_tet, const uint8_t* flag, const uint8_t* reopened,
+    const int64_t* a_off, const uint64_t* bkeys, int64_t n_b,
+    int64_t* count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > n_tet) return;
+  if (t == n_tet) {
+    count[t] = 0;
+    return;
+  }
+  const int64_t na = flag[t] ? 0 : a_off[t + 1] - a_off[t];
+  int64_t nb = 0;
+  if (flag[t] || reopened[t]) {  // only such tets have added candidates
+    const uint64_t k0 = (uint64_t)(uint32_t)t << 32;
+    nb = lower_u64(bkeys, n_b, k0 + (1ull << 32)) - lower_u64(bkeys, n_b, k0);
+  }
+  count[t] = na + nb;
+}
+
+__global__ __launch_bounds__(256) void k_place_a(
+    int64_t n1, const Cand* cand, const uint8_t* flag,
+    const uint8_t* reopened, const int64_t* a_off, const uint64_t* bkeys,
+    int64_t n_b, const int64_t* out_off, int32_t* perm) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n1) return;
+  const Cand cd = cand[c];
+  if (flag[cd.tet]) return;
+  int64_t before_b = 0;
+  if (reopened[cd.tet]) {  // only such tets have added candidates
+    const uint64_t k0 = (uint64_t)(uint32_t)cd.tet << 32;
+    before_b = lower_u64(bkeys, n_b, k0 | (uint32_t)cd.seed) -
+               lower_u64(bkeys, n_b, k0);
+  }
+  perm[out_off[cd.tet] + (c - a_off[cd.tet]) + before_b] = (int32_t)c;
+}
+
+__global__ __launch_bounds__(256) void k_place_b(
+    int64_t n_b, const uint64_t* bkeys, const int32_t* bvals,
+    const uint8_t* flag, const int64_t* a_off, const int32_t* a_ids,
+    const int64_t* out_off, int32_t* perm) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_incl) return;
-  const Cand cd = cand[perm[r]];
-  rel_ids[r] = cd.seed;
-  atomicAdd((unsigned long long*)&tet_count[cd.tet], 1ull);
+  if (r >= n_b) return;
+  const uint64_t k = bkeys[r];
+  const int t = (int)(k >> 32);
+  const int32_t j = (int32_t)(k & 0xffffffffu);
+  const int64_t start = lower_u64(bkeys, n_b, (uint64_t)(uint32_t)t << 32);
+  int64_t before_a = 0;
+  if (!flag[t]) {
+    int64_t lo = a_off[t], hi = a_off[t + 1];
+    const int64_t base = lo;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (a_ids[mid] < j)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    before_a = lo - base;
+  }
+  perm[out_off[t] + (r - start) + before_a] = bvals[r];
+}
+
+// relation CSR ids of the output order
+__global__ __launch_bounds__(256) void k_out_ids(int64_t n, const Cand* cand,
+                                                 const int32_t* perm,
+                                                 int32_t* ids) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) ids[r] = cand[perm[r]].seed;
 }
 
 }  // namespace rpd

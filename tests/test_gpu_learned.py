@@ -78,9 +78,10 @@ def test_learned_runs_match_plain(ctx, mode):
     else:
         st = ctx.run(learned=2, proposal_k=4)
     res = ctx.download()
-    print(f"[learned] {mode}: proposals {st['learned'][0]} fallback tets "
-          f"{st['learned'][1]} of {inst.n_tet} ({st['learned'][2]} "
-          f"candidates)", flush=True)
+    print(f"[learned] {mode}: first-round candidates {st['learned'][0]}, "
+          f"re-run tets {st['learned'][1]} of {inst.n_tet} (+"
+          f"{st['learned'][2]} candidates, {st['learned'][3]} on the full "
+          f"relation)", flush=True)
     _same_result(res, plain)
     compare_with_oracle(res, orc)
 
