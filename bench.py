@@ -1,18 +1,23 @@
 """Benchmark of the RPD hot path on one MI355X.
 
-One step = one rpd_run of the plain path over a resident instance: the
-unit cube as a 64^3 jittered Kuhn grid (1,572,864 tets) with 50,000 spheres
-under the boundary radius rule (18,114 of them non-hidden), binary64:
-candidate gathering, convex-cell clipping with capacity / exact re-runs,
-FP64 geometry and compaction, with every input already in HBM.  Prints one
-JSON line.
+Default workload: the unit cube as a 64^3 jittered Kuhn grid (1,572,864
+tets) with 50,000 spheres under the boundary radius rule (18,114 of them
+non-hidden), binary64, every input already in HBM.  A step is one rpd_run
+over the resident instance.  Two paths are timed on the same instance: the
+plain path (candidates from the full tet-sphere relation) and the learned
+path (candidates proposed by the MLP, trained beforehand on the GPU on two
+other seeds of the same generator, then certified, with the missed cells
+re-clipped); ``value`` is the learned path's rate, whose output is checked
+bit-identical to the plain path's.  ``--workload batched`` times 256
+instances in one batch over sphere-update iterations (plain batched, learned
++batched and one-at-a-time plain).  Prints one JSON line.
 
-roofline: the dominant kernel is the small-tier clip kernel.  Its
-algorithmic bytes per launch are the compulsory reads and writes of the
-cells it clips (clip_bytes), its algorithmic flops the FP64 work it executes
-(clip_flops, counted by the kernel in an extra untimed run); both are divided
-by its average launch duration measured with HIP events on the context
-stream.
+roofline: the dominant kernel is the small-tier clip kernel, measured on the
+plain step's launch.  Its algorithmic bytes per launch are the compulsory
+reads and writes of the cells it clips (clip_bytes), its algorithmic flops
+the FP64 work it executes (clip_flops, counted by the kernel in an extra
+untimed run); both are divided by its average launch duration measured with
+HIP events on the context stream.
 """
 import argparse
 import json
@@ -37,6 +42,8 @@ def parse():
     ap.add_argument("--cells-per-wave", type=int, default=8)
     ap.add_argument("--workload", default="cube64",
                     choices=["cube10", "cube64", "batched"])
+    ap.add_argument("--no-learned", action="store_true",
+                    help="time only the plain path")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-iterations", type=int, default=8)
     ap.add_argument("--cpu-sample-tets", type=int, default=300000)
@@ -111,6 +118,34 @@ def cpu_baseline(inst, sample_tets, threads):
                       f"{t_clip:.2f} s (best of 3), {threads} threads"}
 
 
+def same_result(a, b):
+    """Bit-identical cell streams (records, payload words, geometry,
+    areas)."""
+    import numpy as np
+    for f in ("words", "geom", "areas"):
+        if not np.array_equal(getattr(a, f), getattr(b, f)):
+            return False
+    for k in ("tet", "seed", "nb_v", "nb_p", "nb_e", "status", "word_off",
+              "area_off"):
+        if not np.array_equal(a.cells[k], b.cells[k]):
+            return False
+    return True
+
+
+def train_model(instances, epochs, batch, recall=0.999):
+    """The proposal model trained on the GPU (own kernels) on the plain
+    path's cells of ``instances`` (other seeds of the workload's
+    generator), threshold at ``recall`` of the training proposals; returns
+    (params, logit threshold, seconds, loss per epoch)."""
+    from libmat_amd import learn
+    from libmat_amd.rpd import Context
+    t0 = time.perf_counter()
+    with Context(0) as tc:
+        params, loss, thr = learn.train_proposals(
+            tc, instances, epochs=epochs, batch=batch, recall=recall)
+    return params, thr, time.perf_counter() - t0, [float(x) for x in loss]
+
+
 def bench_batched(a):
     """256 independent instances (tets log-uniform in [2k, 200k] as jittered
     Kuhn grids, spheres log-uniform in [100, 20k], r ~ U[0.02, 0.08]) in
@@ -151,6 +186,43 @@ def bench_batched(a):
         st = ctx.run(cells_per_wave=0)
         ms.append(st["ms_total"])
         cells.append(st["n_cells"])
+    batched = {"value": sum(cells) / (sum(ms) * 1e-3), "unit": "cells/s",
+               "ms_per_iteration": sum(ms) / len(ms), "iteration_ms": ms,
+               "iteration_cells": cells}
+    learned = None
+    if not a.no_learned:
+        # learned+batched: model trained on 64 instances of the 10^3 Kuhn
+        # grid generator (seeds 0-63), same iterations of the same batch
+        train = [prep.cube10_instance(sd) for sd in range(64)]
+        params, thr, t_train, loss = train_model(train, 6, 1024)
+        ctx.mlp_set(params)
+        kw = dict(cells_per_wave=0, learned=1, logit_threshold=thr)
+        ctx.update_sites_batch(sets[0])
+        for _ in range(a.warmup):
+            ctx.run(**kw)
+        lms, lcells, lfb, lfull = [], [], 0, 0
+        for it in range(a.batch_iterations):
+            if it:
+                ctx.update_sites_batch(sets[it])
+            st = ctx.run(**kw)
+            lms.append(st["ms_total"])
+            lcells.append(st["n_cells"])
+            lfb += st["learned"][1]
+            lfull += st["learned"][3]
+        # the last iteration's cells, learned against plain, bit for bit
+        ref = ctx.download()
+        ctx.run(cells_per_wave=0)
+        identical = same_result(ctx.download(), ref) and lcells == cells
+        del ref
+        learned = {
+            "value": sum(lcells) / (sum(lms) * 1e-3), "unit": "cells/s",
+            "ms_per_iteration": sum(lms) / len(lms), "iteration_ms": lms,
+            "identical_to_plain": identical,
+            "fallback_pct": 100.0 * lfb / (n_tet * a.batch_iterations),
+            "fallback_tets": lfb, "full_relation_tets": lfull,
+            "model": "trained on the 10^3 Kuhn grid generator, seeds 0-63 "
+                     "(6 epochs)",
+            "logit_threshold": thr, "train_loss": loss, "train_s": t_train}
     ctx.close()
     # the same instances, one plain run each (first iteration's spheres)
     one = Context(0)
@@ -162,27 +234,34 @@ def bench_batched(a):
         p_ms += st["ms_total"]
         p_cells += st["n_cells"]
     one.close()
-    value = sum(cells) / (sum(ms) * 1e-3)
     plain = p_cells / (p_ms * 1e-3)
+    top = learned if learned is not None else batched
+    value = top["value"]
     out = {
         "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1,
         "steps": a.batch_iterations, "warmup": a.warmup,
-        "ms_per_step": sum(ms) / len(ms), "higher_is_better": True,
+        "ms_per_step": top["ms_per_iteration"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": (
             f"{a.batch_instances} instances, tets log-U[2k,200k] (jittered "
             f"Kuhn grids), spheres log-U[100,20k] r~U[0.02,0.08], "
             f"{a.batch_iterations} iterations of centre jitter N(0,0.002^2)"),
-            "n_tet": n_tet, "path": "batched", "tuned_cells_per_wave": G,
-            "tuned_small_caps": caps},
+            "n_tet": n_tet,
+            "path": "learned+batched" if learned is not None else "batched",
+            "tuned_cells_per_wave": G, "tuned_small_caps": caps},
+        "batched": batched,
+        "learned": learned,
         "plain_per_instance": {"value": plain, "unit": "cells/s",
                                "ms_total": p_ms, "cells": p_cells},
-        "batched_vs_plain": value / plain,
-        "iteration_ms": ms, "iteration_cells": cells,
+        "batched_vs_plain": batched["value"] / plain,
+        "learned_batched_vs_plain": (learned["value"] / plain
+                                     if learned is not None else None),
+        "learned_batched_vs_batched": (learned["value"] / batched["value"]
+                                       if learned is not None else None),
         "tune_ms": {f"caps{c}_G{g}": v for (c, g), v in tune_ms.items()},
         "host_prep_s": t_prep, "site_update_h2d_s": t_sites,
-        "fallback_pct": 0.0,
+        "fallback_pct": learned["fallback_pct"] if learned else 0.0,
     }
     print(json.dumps(out))
 
@@ -245,6 +324,59 @@ def main():
     if os.path.exists(a.traffic_json):
         with open(a.traffic_json) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    # ---- learned path on the same resident instance (model trained on
+    # other seeds of the same generator, untimed)
+    learned = None
+    if not a.no_learned:
+        t0 = time.perf_counter()
+        if a.workload == "cube64":
+            train = [prep.cube64_instance(sd) for sd in (1, 2)]
+            params, thr, t_train, loss = train_model(train, 3, 2048)
+            model = "trained on the 64^3 generator, seeds 1-2 (3 epochs)"
+        else:
+            train = [prep.cube10_instance(sd) for sd in range(1, 65)]
+            params, thr, t_train, loss = train_model(train, 6, 1024)
+            model = "trained on the 10^3 generator, seeds 1-64 (6 epochs)"
+        t_learn_prep = time.perf_counter() - t0
+        ctx.mlp_set(params)
+        kw = dict(cells_per_wave=a.cells_per_wave, learned=1,
+                  logit_threshold=thr)
+        ctx.run(cells_per_wave=a.cells_per_wave)
+        ref_res = ctx.download()
+        lst = ctx.run(**kw)
+        identical = same_result(ctx.download(), ref_res)
+        del ref_res
+        for _ in range(a.warmup):
+            ctx.run(**kw)
+        ctx._L.rpd_sync(ctx._h)
+        lstats = []
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            lstats.append(ctx.run(**kw))
+        ctx._L.rpd_sync(ctx._h)
+        ldt = time.perf_counter() - t0
+        lk = np.mean(np.array([x["ms_kernel"] for x in lstats]), axis=0)
+        lp = np.mean(np.array([x["ms_phase"] for x in lstats]), axis=0)
+        lst = lstats[-1]
+        learned = {
+            "value": lst["n_cells"] * a.steps / ldt,
+            "ms_per_step": 1e3 * ldt / a.steps,
+            "identical_to_plain": identical,
+            "fallback_pct": 100.0 * lst["learned"][1] / inst.n_tet,
+            "fallback_tets": lst["learned"][1],
+            "full_relation_tets": lst["learned"][3],
+            "first_round_candidates": lst["learned"][0],
+            "added_candidates": lst["learned"][2],
+            "model": model, "logit_threshold": thr, "train_loss": loss,
+            "train_s": t_train, "train_prep_s": t_learn_prep,
+            "phase_ms": {"candidates": lp[0], "clip": lp[1], "rerun": lp[2],
+                         "compaction": lp[3], "certificate": lp[4]},
+            "kernel_ms": {"features": lk[0], "mlp": lk[1],
+                          "proposal_count": lk[2], "proposal_fill": lk[3],
+                          "plane_masks": lk[4], "clip": lk[5],
+                          "compaction": lk[6], "certificate": lk[7]},
+            "clip_alg_gbs": clip_bytes(lst, inst) / (lk[5] * 1e-3) / 1e9,
+        }
     # host-buffer boundary: upload + one run + download of the cell stream
     t0 = time.perf_counter()
     ctx.upload(inst)
@@ -254,6 +386,9 @@ def main():
     del res
     if a.calib_copy_gib > 0:
         ctx.calib_copy(int(a.calib_copy_gib * (1 << 30)))
+    plain = {"value": value, "ms_per_step": ms_step}
+    if learned is not None:
+        value, ms_step = learned["value"], learned["ms_per_step"]
     out = {
         "metric": METRIC,
         "value": value, "unit": "cells/s", "n_gpus": 1, "steps": a.steps,
@@ -266,15 +401,21 @@ def main():
             f"{inst.n_site} non-hidden sites" if a.workload == "cube64"
             else "unit cube 10^3 Kuhn grid, 500 spheres r ~ U[0.02,0.08]"),
             "n_tet": inst.n_tet, "n_site": inst.n_site,
-            "cells_per_wave": a.cells_per_wave, "path": "plain"},
+            "cells_per_wave": a.cells_per_wave,
+            "path": "learned" if learned is not None else "plain"},
+        "plain": plain,
+        "learned": learned,
+        "learned_vs_plain": (learned["value"] / plain["value"]
+                             if learned is not None else None),
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": f"k_clip<{a.cells_per_wave}, Caps<32,16,48>, false>",
-                     "kernel_ms": clip_ms, "alg_bytes": alg},
+                     "kernel_ms": clip_ms, "alg_bytes": alg,
+                     "measured_on": "the plain step's clip launch"},
         "roofline_fp64_valu": fp64,
-        "tets_per_s": inst.n_tet * a.steps / dt,
-        "fallback_pct": 0.0,
+        "tets_per_s": inst.n_tet * 1e3 / ms_step,
+        "fallback_pct": learned["fallback_pct"] if learned else 0.0,
         "exact_rerun_cells": st["n_rerun_exact"],
         "capacity_rerun_cells": st["n_rerun_large"],
         "n_cells": n_cells, "n_candidates": st["n_candidates"],
