@@ -156,6 +156,7 @@ def time_batch(ctx, sets, warmup, **kw):
     for _ in range(warmup):
         ctx.run(**kw)
     ms, cells, lrn, t_sites = [], [], [0, 0, 0, 0], 0.0
+    kern = []
     for it, st_set in enumerate(sets):
         if it and st_set is not None:
             t1 = time.perf_counter()
@@ -164,7 +165,11 @@ def time_batch(ctx, sets, warmup, **kw):
         st = ctx.run(**kw)
         ms.append(st["ms_total"])
         cells.append(st["n_cells"])
+        kern.append(list(st["ms_kernel"]) + list(st["ms_phase"][:5]))
         lrn = [x + y for x, y in zip(lrn, st["learned"])]
+    time_batch.kernel_ms = [round(float(x), 3) for x in
+                            (sum(k[i] for k in kern) / len(kern)
+                             for i in range(len(kern[0])))]
     return ms, cells, lrn, t_sites
 
 
@@ -224,7 +229,8 @@ def bench_batched(a):
     ms, cells, _, t_sites = time_batch(ctx, sets, a.warmup, cells_per_wave=0)
     batched = {"value": sum(cells) / (sum(ms) * 1e-3), "unit": "cells/s",
                "ms_per_step": sum(ms) / len(ms), "step_ms": ms,
-               "step_cells": cells}
+               "step_cells": cells,
+               "kernel_and_phase_ms": time_batch.kernel_ms}
     print(f"[{a.workload}] plain batched timed", file=sys.stderr, flush=True)
     learned = None
     if not a.no_learned:
@@ -255,7 +261,8 @@ def bench_batched(a):
             "fallback_tets": lrn[1], "full_relation_tets": lrn[3],
             "first_round_candidates": lrn[0], "added_candidates": lrn[2],
             "model": model, "logit_threshold": thr, "train_loss": loss,
-            "train_s": t_train}
+            "train_s": t_train,
+            "kernel_and_phase_ms": time_batch.kernel_ms}
         print(f"[{a.workload}] learned+batched timed", file=sys.stderr,
               flush=True)
     ctx.close()
