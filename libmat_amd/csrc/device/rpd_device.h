@@ -39,7 +39,9 @@ struct DevSites {
   const uint32_t* flags;
   const int64_t* knn_off;
   const int32_t* knn_ids;   // neighbours in ascending site id (clip order)
-  const int32_t* knn_near;  // the same lists, nearest bisector first
+  const int32_t* knn_near;  // the same lists (nearest bisector first for
+                            // the relation, list order for the learned
+                            // path), bit 31 set for unflagged sites
   const double4* knn_site_near;  // the spheres of knn_near, inline
 };
 

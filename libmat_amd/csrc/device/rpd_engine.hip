@@ -537,6 +537,7 @@ struct RunState {
   const int32_t* tet_inst_d = nullptr;
   int64_t n_bins = 0, n_sg = 0;
   int32_t r_large = 0, r_exact = 0;  // re-run entries clipped so far
+  CertArgs cert{};  // learned path: certificate evaluated in the clip
 };
 
 MlpDev mlp_dev(rpd_ctx* ctx) {
@@ -839,6 +840,7 @@ ClipArgs clip_args(rpd_ctx* ctx, RunState& R) {
   a.rerun_exact = ctx->rerun_exact.as<int32_t>();
   a.n_rerun_exact = R.cnt + 2;
   a.work = R.cfg.count_work ? ctx->work.as<unsigned long long>() : nullptr;
+  a.cert = R.cert;
   return a;
 }
 
@@ -954,41 +956,54 @@ int sort_u64(rpd_ctx* ctx, hipStream_t s, const uint64_t* in, uint64_t* out,
   return 0;
 }
 
-int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
-                         int64_t* n_incl_out) {
+// certificate state for a round of n new candidates: a cleared miss list
+// of capacity max(4096, 2n) and the current ext keys
+int certify_round(rpd_ctx* ctx, RunState& R, int64_t n, int64_t n_ext) {
+  const int64_t cap = std::max<int64_t>(4096, 2 * n);
+  HIP_TRY(ctx->miss_a.need((size_t)cap * 8));
+  HIP_TRY(ctx->miss_b.need((size_t)cap * 8));
+  HIP_TRY(hipMemsetAsync(R.cnt + 4, 0, 4, R.s));
+  R.cert.ext = ctx->mem_a.as<uint64_t>();
+  R.cert.n_ext = n_ext;
+  R.cert.miss = ctx->miss_a.as<uint64_t>();
+  R.cert.cap = (int32_t)std::min<int64_t>(cap, INT32_MAX);
+  R.cert.n_miss = R.cnt + 4;
+  return 0;
+}
+
+// before the first round's clip: cleared tet flags, members = the
+// first-round sets (CSR tet_off / rel_ids of the learned fill)
+int certify_begin(rpd_ctx* ctx, RunState& R, int64_t n1) {
   hipStream_t s = R.s;
   const int32_t nt = R.nt;
   HIP_TRY(ctx->flag.need((size_t)nt + 1));
   HIP_TRY(ctx->has_cell.need((size_t)nt + 1));
   HIP_TRY(ctx->reopened.need((size_t)nt + 1));
-  HIP_TRY(ctx->flag32.need(((size_t)nt + 1) * 4));
-  HIP_TRY(ctx->flag_pos.need(((size_t)nt + 1) * 4));
   HIP_TRY(hipMemsetAsync(ctx->flag.p, 0, (size_t)nt + 1, s));
   HIP_TRY(hipMemsetAsync(ctx->has_cell.p, 0, (size_t)nt + 1, s));
   HIP_TRY(hipMemsetAsync(ctx->reopened.p, 0, (size_t)nt + 1, s));
-  // members: the first-round sets (CSR tet_off / rel_ids of the learned
-  // fill) and ext, the sorted keys of every pair added since
+  R.cert.tet_off = ctx->tet_off.as<int64_t>();
+  R.cert.mem_ids = ctx->rel_ids.as<int32_t>();
+  R.cert.flag = ctx->flag.as<uint8_t>();
+  R.cert.has_cell = ctx->has_cell.as<uint8_t>();
+  return certify_round(ctx, R, n1, 0);
+}
+
+int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
+                         int64_t* n_incl_out) {
+  hipStream_t s = R.s;
+  const int32_t nt = R.nt;
+  HIP_TRY(ctx->flag32.need(((size_t)nt + 1) * 4));
+  HIP_TRY(ctx->flag_pos.need(((size_t)nt + 1) * 4));
+  // the cells of [c_lo, c_hi) were certified while clipped; ext = the
+  // sorted keys of every pair added since the first round
   int64_t n_ext = 0;
   int64_t c_lo = 0, c_hi = n1, added = 0;
   for (int round = 0;; ++round) {
-    const int64_t cap = std::max<int64_t>(4096, 2 * (c_hi - c_lo));
-    HIP_TRY(ctx->miss_a.need((size_t)cap * 8));
-    HIP_TRY(ctx->miss_b.need((size_t)cap * 8));
-    HIP_TRY(hipMemsetAsync(R.cnt + 4, 0, 4, s));
-    if (c_hi > c_lo)
-      hipLaunchKernelGGL(k_certify, dim3(ceil_div(c_hi - c_lo, 256)),
-                         dim3(256), 0, s, c_lo, c_hi, ctx->cand.as<Cand>(),
-                         ctx->loc.as<int32_t>(), tier_views(ctx, R),
-                         ctx->tet_off.as<int64_t>(),
-                         ctx->rel_ids.as<int32_t>(),
-                         ctx->mem_a.as<uint64_t>(), n_ext,
-                         ctx->miss_a.as<uint64_t>(), (int32_t)cap, R.cnt + 4,
-                         ctx->flag.as<uint8_t>(), ctx->has_cell.as<uint8_t>());
-    HIP_TRY(hipGetLastError());
     int32_t nm = 0;
     HIP_TRY(hipMemcpyAsync(&nm, R.cnt + 4, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    const int64_t n_miss = std::min<int64_t>(nm, cap);
+    const int64_t n_miss = std::min<int64_t>(nm, R.cert.cap);
     if (n_miss == 0) break;
     if (round == kClosureRounds) {  // still open: full relation
       hipLaunchKernelGGL(k_flag_miss, dim3(ceil_div(n_miss, 256)), dim3(256),
@@ -1050,12 +1065,15 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
                           ctx->mem_b.as<uint64_t>(), n_ext))
       return rc;
     ctx->mem_a.swap(ctx->mem_b);
+    if (int rc = certify_round(ctx, R, n_new, n_ext)) return rc;
     if (n_new > 0)
       if (int rc = clip_range(ctx, R, c_hi, c_hi + n_new, false)) return rc;
     added += n_new;
     c_lo = c_hi;
     c_hi += n_new;
   }
+  (void)c_lo;
+  R.cert.tet_off = nullptr;  // the full-relation re-runs are final
   // tets without a cell, with a dropped cell or left open: full relation
   hipLaunchKernelGGL(k_flag_final, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
                      nt, ctx->flag.as<uint8_t>(), ctx->has_cell.as<uint8_t>(),
@@ -1277,7 +1295,10 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   } else {
     if (int rc = plain_candidates(ctx, R, &n_cand)) return rc;
   }
-  // ---- 2. clipping: small tier, then full capacity, then exact
+  // ---- 2. clipping: small tier, then full capacity, then exact (learned:
+  // each cell certified as it is clipped)
+  if (cfg.learned)
+    if (int rc = certify_begin(ctx, R, n_cand)) return rc;
   if (int rc = clip_range(ctx, R, 0, n_cand, true)) return rc;
   // ---- 3. (learned) certificate and fallback; 4. compaction
   int64_t n_out = n_cand;

@@ -169,6 +169,14 @@ constexpr int kBoxV = 128;      // vertices (overflow: whole-domain box)
 constexpr int kBoxP = 128;      // 6 box planes + 122 neighbours
 constexpr int kNearMax = 128;   // longest list ranked (else kept as is)
 
+// inline neighbour word: the site id, with bit 31 set when the site is not
+// flagged (it bounds cells but has none of its own)
+constexpr uint32_t kNearUnflagged = 0x80000000u;
+__device__ __forceinline__ int32_t near_word(const DevSites& S, int m) {
+  return (int32_t)((uint32_t)m | (S.flags[m] == 1u ? 0u : kNearUnflagged));
+}
+__device__ __forceinline__ int near_id(int32_t w) { return w & 0x7fffffff; }
+
 __global__ __launch_bounds__(64) void k_site_bbox(DevSites S,
                                                   const InstInfo* inst,
                                                   const int32_t* site_inst,
@@ -224,13 +232,13 @@ __global__ __launch_bounds__(64) void k_site_bbox(DevSites S,
     wsync();
     for (int k = lane; k < deg; k += 64) {
       const int m = s_near[k];
-      knn_near[k0 + k] = m;
+      knn_near[k0 + k] = near_word(S, m);
       knn_site_near[k0 + k] = S.site[m];
     }
   } else {
     for (int k = lane; k < deg; k += 64) {
       const int m = S.knn_ids[k0 + k];
-      knn_near[k0 + k] = m;
+      knn_near[k0 + k] = near_word(S, m);
       knn_site_near[k0 + k] = S.site[m];
     }
   }
@@ -635,6 +643,40 @@ __global__ __launch_bounds__(256) void k_cand_mask(DevMesh M, DevSites S,
 }
 
 // ============================================================ clipping
+// binary search in a sorted u64 array
+__device__ __forceinline__ bool has_key(const uint64_t* keys, int64_t n,
+                                        uint64_t k) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < n && keys[lo] == k;
+}
+
+// Completeness certificate of the learned path, evaluated on each cell
+// when its status is final (tet_off == nullptr: off).  The cells of a tet's
+// sphere set partition the tet iff the set is closed under facet
+// neighbours: every active bisector facet (seed, j) of a cell must have j
+// in the set -- the tet's CSR members, or a pair (tet, j) of the sorted
+// ext keys added since.  A missing pair goes to miss (past cap: the tet is
+// flagged); a cell that ends in a failure status flags its tet; a tet with
+// a cell is marked in has_cell.
+struct CertArgs {
+  const int64_t* tet_off;
+  const int32_t* mem_ids;
+  const uint64_t* ext;
+  int64_t n_ext;
+  uint64_t* miss;
+  int32_t cap;
+  int32_t* n_miss;
+  uint8_t* flag;
+  uint8_t* has_cell;
+};
+
 struct ClipArgs {
   DevMesh M;
   DevSites S;
@@ -653,6 +695,7 @@ struct ClipArgs {
   int32_t* rerun_exact;
   int32_t* n_rerun_exact;
   unsigned long long* work;  // 64 x 4 spread counters, or nullptr
+  CertArgs cert;
 };
 
 template <int L>
@@ -763,10 +806,15 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
   const int64_t slot = a.slot_base + item;
   const Cand cd = a.cand[cidx];
   const int t = cd.tet, s = cd.seed;
-
-  // ---- init the cell as its tet (convex_cell.cu:116-214)
   const int4 tv = a.M.tet_v[t];
   const TetMeta tm = a.M.tet_meta[t];
+  const rpd_d4 S0 = ld4(a.S.site + s);
+  const int64_t k0 = a.S.knn_off[s];
+  const int deg = (int)(a.S.knn_off[s + 1] - k0);
+  uint64_t m0 = a.use_mask ? cd.m0 : ~0ull;
+  uint64_t m1 = a.use_mask ? cd.m1 : ~0ull;
+
+  // ---- init the cell as its tet (convex_cell.cu:116-214)
   {
     if (j < 4) {
       // face j = tet_faces_lvid[j]: {2,1,3}, {0,2,3}, {1,0,3}, {0,1,2}
@@ -787,11 +835,6 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
   wsync();
   int nb_v = 4, nb_p = 4, nb_e = 6;
   uint32_t status = RPD_SUCCESS;
-  const rpd_d4 S0 = ld4(a.S.site + s);
-  const int64_t k0 = a.S.knn_off[s];
-  const int deg = (int)(a.S.knn_off[s + 1] - k0);
-  uint64_t m0 = a.use_mask ? cd.m0 : ~0ull;
-  uint64_t m1 = a.use_mask ? cd.m1 : ~0ull;
   uint32_t n_det = 0, n_bis = 0, n_exd = 0;  // executed work (count_work)
   int k = -1;
   for (;;) {
@@ -1203,6 +1246,56 @@ __global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
     for (int v = j; v < nb_v; v += L) wd[v] = s_ver[g][v];
     for (int p = j; p < nb_p; p += L) wd[nb_v + p] = (uint32_t)s_src[g][p];
     for (int q = j; q < nb_e; q += L) wd[nb_v + nb_p + q] = s_edge[g][q];
+  }
+  if (a.cert.tet_off && status != kStatusRerunLarge &&
+      status != kStatusRerunExact && status != RPD_NO_INTERSECTION) {
+    const CertArgs& c = a.cert;
+    if (status != RPD_SUCCESS) {
+      if (j == 0) c.flag[t] = 1;
+    } else {
+      if (j == 0) c.has_cell[t] = 1;
+      // planes with a vertex (the active facets), OR over the group
+      uint64_t act0 = 0, act1 = 0;
+      for (int v = j; v < nb_v; v += L) {
+        const uint32_t w = s_ver[g][v];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const uint32_t p = (w >> (8 * q)) & 255u;
+          if (p < 64)
+            act0 |= 1ull << p;
+          else
+            act1 |= 1ull << (p - 64);
+        }
+      }
+#pragma unroll
+      for (int off = L / 2; off > 0; off >>= 1) {
+        act0 |= (uint64_t)__shfl_xor((unsigned long long)act0, off);
+        act1 |= (uint64_t)__shfl_xor((unsigned long long)act1, off);
+      }
+      const int64_t o0 = c.tet_off[t], o1 = c.tet_off[t + 1];
+      for (int p = j; p < nb_p; p += L) {
+        const bool active =
+            p < 64 ? ((act0 >> p) & 1) : ((act1 >> (p - 64)) & 1);
+        const int32_t src = s_src[g][p];
+        if (!active || src < 0) continue;
+        int64_t lo = o0, hi = o1;
+        while (lo < hi) {
+          const int64_t mid = (lo + hi) >> 1;
+          if (c.mem_ids[mid] < src)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        if (lo < o1 && c.mem_ids[lo] == src) continue;
+        const uint64_t key = ((uint64_t)(uint32_t)t << 32) | (uint32_t)src;
+        if (c.n_ext && has_key(c.ext, c.n_ext, key)) continue;
+        const int r = atomicAdd(c.n_miss, 1);
+        if (r < c.cap)
+          c.miss[r] = key;
+        else
+          c.flag[t] = 1;
+      }
+    }
   }
   if (a.work) {
     if (kExact) {  // exact determinants are per lane: sum over the group

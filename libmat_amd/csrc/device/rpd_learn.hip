@@ -12,7 +12,8 @@
 // proposals their centre offset and radius over the tet's radius and their
 // key over its square.  Proposals scoring above a threshold (owners
 // always) are clipped; a completeness
-// certificate then checks every tet: each active bisector facet of each of
+// certificate, evaluated by the clip kernel on each cell whose status is
+// final (CertArgs), then checks every tet: each active bisector facet of each of
 // its cells must lead to a sphere of its proposal set, no cell may have
 // been dropped, and some cell must exist (the cells of a set closed under
 // facet neighbours partition the tet).  Only proposals passing the
@@ -80,14 +81,15 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
 }
 
 // ------------------------------------------------------------- features
-// neighbour lists in list order with their spheres inline
+// neighbour lists in list order with their spheres inline (flag in the
+// id word, near_word)
 __global__ __launch_bounds__(256) void k_site_inline(DevSites S, int64_t n_knn,
                                                      int32_t* knn_near,
                                                      double4* knn_site_near) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_knn) return;
   const int m = S.knn_ids[q];
-  knn_near[q] = m;
+  knn_near[q] = near_word(S, m);
   knn_site_near[q] = S.site[m];
 }
 
@@ -162,7 +164,7 @@ __device__ __forceinline__ int descend(const DevSites& S, const double c[3],
     int nxt = cur;
     double nb = *best;
     for (int64_t q = S.knn_off[cur]; q < S.knn_off[cur + 1]; ++q) {
-      const int id = S.knn_near[q];
+      const int id = near_id(S.knn_near[q]);
       const double pd = pdist(c, S.knn_site_near[q]);
       if (pd < nb || (pd == nb && id < nxt)) {
         nb = pd;
@@ -192,8 +194,12 @@ __global__ __launch_bounds__(256) void k_vertex_owner(
 // One thread per tet.  The owners of its centroid and of its 4 corners
 // belong to its sphere set; the proposals are the union of their 1-rings
 // (with them), ranked by key(s) = min over the 5 points x of
-// pd(x, s) - pd(x, owner of x) -- 0 exactly for the owners -- then id;
-// 32 kept, flagged spheres only.  omask marks the owner slots.
+// pd(x, s) - pd(x, owner of x) -- 0 for the owners -- then id; 32 kept,
+// flagged spheres only.  omask marks the owner slots.  Keys only rank
+// proposals (the certificate makes the output exact whatever they are), so
+// they are evaluated in binary32 around the centroid c: with u = s - c and
+// the corner offsets d_x = x - c, pd(x, s) - pd_min(x) =
+// (|u|^2 - w) - 2 d_x.u + (|d_x|^2 - pd_min(x)).
 __global__ __launch_bounds__(64) void k_knn_features(
     DevMesh M, DevSites S, const InstInfo* inst, const int32_t* tet_inst,
     const int32_t* vowner, int32_t* knn, uint16_t* X, uint32_t* omask) {
@@ -201,24 +207,21 @@ __global__ __launch_bounds__(64) void k_knn_features(
   if (t >= M.n_tet) return;
   TetLoad L;
   load_tet(M, t, L);
-  double P[5][3];
-  tet_centroid(L, P[0]);
+  double C[3];
+  tet_centroid(L, C);
+  float D[4][3];
+  float rho2f = 0.f;
   for (int k = 0; k < 4; ++k) {
-    P[1 + k][0] = L.V[k].x;
-    P[1 + k][1] = L.V[k].y;
-    P[1 + k][2] = L.V[k].z;
-  }
-  double rho2 = 0;
-  for (int k = 0; k < 4; ++k) {
-    const double dx = P[1 + k][0] - P[0][0], dy = P[1 + k][1] - P[0][1],
-                 dz = P[1 + k][2] - P[0][2];
-    rho2 = fmax(rho2, (dx * dx + dy * dy) + dz * dz);
+    D[k][0] = (float)(L.V[k].x - C[0]);
+    D[k][1] = (float)(L.V[k].y - C[1]);
+    D[k][2] = (float)(L.V[k].z - C[2]);
+    rho2f = fmaxf(rho2f, (D[k][0] * D[k][0] + D[k][1] * D[k][1]) +
+                             D[k][2] * D[k][2]);
   }
   const InstInfo& I = inst[tet_inst[t]];
   uint16_t* x = X + (int64_t)t * kIn;
   int32_t* kt = knn + (int64_t)t * kOut;
   int own[5];
-  double pmin[5];
   own[1] = vowner[L.tv.x];
   own[2] = vowner[L.tv.y];
   own[3] = vowner[L.tv.z];
@@ -250,19 +253,27 @@ __global__ __launch_bounds__(64) void k_knn_features(
   };
   uint32_t om = 0;
   if (own[1] >= 0) {
-    for (int k = 1; k < 5; ++k) pmin[k] = pdist(P[k], S.site[own[k]]);
     // centroid owner: descend from the corner owner closest to it
+    double pc[5];
+    for (int k = 1; k < 5; ++k) pc[k] = pdist(C, S.site[own[k]]);
     int st = own[1];
-    double best = pdist(P[0], S.site[st]);
-    for (int k = 2; k < 5; ++k) {
-      const double pd = pdist(P[0], S.site[own[k]]);
-      if (pd < best || (pd == best && own[k] < st)) {
-        best = pd;
+    double best = pc[1];
+    for (int k = 2; k < 5; ++k)
+      if (pc[k] < best || (pc[k] == best && own[k] < st)) {
+        best = pc[k];
         st = own[k];
       }
+    own[0] = descend(S, C, st, &best);
+    // q_x = |d_x|^2 - pd_min(x), binary32
+    float qx[5];
+    qx[0] = (float)(-best);
+    for (int k = 1; k < 5; ++k) {
+      const double4 so = S.site[own[k]];
+      const double P[3] = {L.V[k - 1].x, L.V[k - 1].y, L.V[k - 1].z};
+      const float dd = (D[k - 1][0] * D[k - 1][0] + D[k - 1][1] * D[k - 1][1]) +
+                       D[k - 1][2] * D[k - 1][2];
+      qx[k] = dd - (float)pdist(P, so);
     }
-    own[0] = descend(S, P[0], st, &best);
-    pmin[0] = best;
     // rings: the centroid owner's, and a corner owner's only if that
     // owner is not already in it (the rings overlap otherwise)
     const int64_t r0 = S.knn_off[own[0]], r1 = S.knn_off[own[0] + 1];
@@ -275,17 +286,44 @@ __global__ __launch_bounds__(64) void k_knn_features(
       if (j > 0) {
         bool in_ring = false;
         for (int64_t q = r0; q < r1 && !in_ring; ++q)
-          in_ring = S.knn_near[q] == o;
+          in_ring = near_id(S.knn_near[q]) == o;
         if (in_ring) continue;
       }
-      for (int64_t q = S.knn_off[o]; q < S.knn_off[o + 1]; ++q) {
-        const int id = S.knn_near[q];
-        if (S.flags[id] != 1u) continue;
-        const double4 sp = S.knn_site_near[q];
-        double key = INFINITY;
+      const int64_t e0 = S.knn_off[o], e1 = S.knn_off[o + 1];
+      // four ring entries per round: their loads are in flight together
+      for (int64_t q0 = e0; q0 < e1; q0 += 4) {
+        int32_t w4[4];
+        double4 s4[4];
 #pragma unroll
-        for (int a = 0; a < 5; ++a) key = fmin(key, pdist(P[a], sp) - pmin[a]);
-        ins((float)key, id);
+        for (int u = 0; u < 4; ++u) {
+          const int64_t q = q0 + u < e1 ? q0 + u : e1 - 1;
+          w4[u] = S.knn_near[q];
+          s4[u] = S.knn_site_near[q];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (q0 + u >= e1 || w4[u] < 0) continue;  // past the end, unflagged
+          const int id = w4[u];
+          bool is_owner = false;
+#pragma unroll
+          for (int a = 0; a < 5; ++a) is_owner = is_owner || own[a] == id;
+          if (is_owner) continue;  // listed with key 0
+          const float ux = (float)(s4[u].x - C[0]);
+          const float uy = (float)(s4[u].y - C[1]);
+          const float uz = (float)(s4[u].z - C[2]);
+          const float base =
+              (float)(((s4[u].x - C[0]) * (s4[u].x - C[0]) +
+                       (s4[u].y - C[1]) * (s4[u].y - C[1])) +
+                      (s4[u].z - C[2]) * (s4[u].z - C[2]) - s4[u].w);
+          float key = base + qx[0];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const float du = (D[k][0] * ux + D[k][1] * uy) + D[k][2] * uz;
+            key = fminf(key, (base + qx[1 + k]) - 2.f * du);
+          }
+          // non-owners rank after the owners (key 0) whatever the rounding
+          ins(fmaxf(key, 1.17549435e-38f), id);
+        }
       }
     }
 #pragma unroll
@@ -302,22 +340,22 @@ __global__ __launch_bounds__(64) void k_knn_features(
   const double ext =
       fmax(fmax(I.dom[3] - I.dom[0], I.dom[4] - I.dom[1]), I.dom[5] - I.dom[2]);
   const double inv_ext = ext > 0 ? 1.0 / ext : 0.0;
-  const double inv_r = rho2 > 0 ? 1.0 / sqrt(rho2) : 0.0;
+  const float inv_r = rho2f > 0.f ? 1.f / sqrtf(rho2f) : 0.f;
   float f[kIn];
 #pragma unroll
   for (int q = 0; q < kIn; ++q) f[q] = 0.f;
-  f[0] = (float)((P[0][0] - I.dom[0]) * inv_ext);
-  f[1] = (float)((P[0][1] - I.dom[1]) * inv_ext);
-  f[2] = (float)((P[0][2] - I.dom[2]) * inv_ext);
+  f[0] = (float)((C[0] - I.dom[0]) * inv_ext);
+  f[1] = (float)((C[1] - I.dom[1]) * inv_ext);
+  f[2] = (float)((C[2] - I.dom[2]) * inv_ext);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     if (kk[k] < INFINITY) {
       const double4 sp = S.site[ii[k]];
-      f[3 + 5 * k + 0] = (float)((sp.x - P[0][0]) * inv_r);
-      f[3 + 5 * k + 1] = (float)((sp.y - P[0][1]) * inv_r);
-      f[3 + 5 * k + 2] = (float)((sp.z - P[0][2]) * inv_r);
-      f[3 + 5 * k + 3] = (float)(sqrt(sp.w) * inv_r);
-      f[3 + 5 * k + 4] = (float)(kk[k] * (inv_r * inv_r));
+      f[3 + 5 * k + 0] = (float)(sp.x - C[0]) * inv_r;
+      f[3 + 5 * k + 1] = (float)(sp.y - C[1]) * inv_r;
+      f[3 + 5 * k + 2] = (float)(sp.z - C[2]) * inv_r;
+      f[3 + 5 * k + 3] = (float)sqrt(sp.w) * inv_r;
+      f[3 + 5 * k + 4] = kk[k] * (inv_r * inv_r);
     }
   }
 #pragma unroll
@@ -850,82 +888,6 @@ __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
       cand[o + k] = c;
       rel_ids[o + k] = v[k];
     }
-  }
-}
-
-// Certificate of the cells of candidates [c0, c1): every active bisector
-// facet (i, j) of a cell of tet t needs j in t's member set: the first-round
-// set (sorted CSR tet_off / mem_ids) or the sorted keys ext of pairs added
-// since.  Missing pairs (t << 32 | j) are appended to miss (capacity cap;
-// beyond it, and for dropped cells, the tet is flagged for the
-// full-relation re-run).
-__device__ __forceinline__ bool has_key(const uint64_t* keys, int64_t n,
-                                        uint64_t k) {
-  int64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (keys[mid] < k)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo < n && keys[lo] == k;
-}
-
-__global__ __launch_bounds__(256) void k_certify(
-    int64_t c0, int64_t c1, const Cand* cand, const int32_t* loc,
-    TierViews tv, const int64_t* tet_off, const int32_t* mem_ids,
-    const uint64_t* ext, int64_t n_ext, uint64_t* miss, int32_t cap,
-    int32_t* n_miss, uint8_t* flag, uint8_t* has_cell) {
-  const int64_t c = c0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= c1) return;
-  const int t = cand[c].tet;
-  const int32_t l = loc[c];
-  const int tier = l >> 28;
-  const int64_t idx = tier ? (l & ((1 << 28) - 1)) : c;
-  const SlotView& sv = tv.t[tier];
-  const uint32_t hd = sv.hdr[idx];
-  const uint32_t st = hd >> 24;
-  if (st == RPD_NO_INTERSECTION) return;
-  if (st != RPD_SUCCESS) {
-    flag[t] = 1;
-    return;
-  }
-  has_cell[t] = 1;
-  const int nv = hd & 255, np = (hd >> 8) & 255;
-  const uint32_t* w = sv.words + idx * sv.stride_words;
-  uint64_t act0 = 0, act1 = 0;
-  for (int v = 0; v < nv; ++v) {
-    const uint32_t x = w[v];
-    for (int k = 0; k < 3; ++k) {
-      const uint32_t p = (x >> (8 * k)) & 255u;
-      if (p < 64)
-        act0 |= 1ull << p;
-      else
-        act1 |= 1ull << (p - 64);
-    }
-  }
-  const int64_t o0 = tet_off[t], o1 = tet_off[t + 1];
-  for (int p = 0; p < np; ++p) {
-    const bool active = p < 64 ? ((act0 >> p) & 1) : ((act1 >> (p - 64)) & 1);
-    const int32_t src = (int32_t)w[nv + p];
-    if (!active || src < 0) continue;
-    int64_t lo = o0, hi = o1;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (mem_ids[mid] < src)
-        lo = mid + 1;
-      else
-        hi = mid;
-    }
-    if (lo < o1 && mem_ids[lo] == src) continue;
-    const uint64_t key = ((uint64_t)(uint32_t)t << 32) | (uint32_t)src;
-    if (n_ext && has_key(ext, n_ext, key)) continue;
-    const int r = atomicAdd(n_miss, 1);
-    if (r < cap)
-      miss[r] = key;
-    else
-      flag[t] = 1;
   }
 }
 
