@@ -32,6 +32,9 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 METRIC = ("RPD cells/sec (tets clipped) learned+batched vs plain on 1 "
           "MI355X; fallback %")
+# small-tier capacities (vertices, planes, edges) by rpd_run_cfg.small_caps
+CAPS_NAMES = {0: "Caps<32, 16, 48>", 1: "Caps<48, 24, 72>",
+              2: "Caps<16, 12, 32>", 3: "Caps<24, 14, 40>"}
 
 
 def parse():
@@ -39,7 +42,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cells-per-wave", type=int, default=8)
+    ap.add_argument("--cells-per-wave", type=int, default=0,
+                    help="cells per wavefront of the clip kernel; 0 = the "
+                         "autotuner's choice on the instance (untimed)")
+    ap.add_argument("--small-caps", type=int, default=0,
+                    help="small-tier capacities with --cells-per-wave > 0")
     ap.add_argument("--workload", default="cube64",
                     choices=["cube10", "cube64", "batched", "gmm",
                              "heldout10"])
@@ -384,13 +391,19 @@ def main():
     t0 = time.perf_counter()
     ctx.upload(inst)
     t_upload = time.perf_counter() - t0
+    tune_ms = None
+    if a.cells_per_wave == 0:
+        G, caps, tune_ms = ctx.tune()
+    else:
+        G, caps = a.cells_per_wave, a.small_caps
+    shape = dict(cells_per_wave=G, small_caps=caps)
     for _ in range(a.warmup):
-        ctx.run(cells_per_wave=a.cells_per_wave)
+        ctx.run(**shape)
     ctx._L.rpd_sync(ctx._h)
     stats = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        stats.append(ctx.run(cells_per_wave=a.cells_per_wave))
+        stats.append(ctx.run(**shape))
     ctx._L.rpd_sync(ctx._h)
     dt = time.perf_counter() - t0
     st = stats[-1]
@@ -399,7 +412,7 @@ def main():
     work = None
     active_frac = None
     if not a.no_work_count:
-        stw = ctx.run(cells_per_wave=a.cells_per_wave, count_work=True)
+        stw = ctx.run(count_work=True, **shape)
         work = stw["work"]
         active_frac = stw["n_active_planes"] / max(1, stw["n_all_planes"])
     n_cells = st["n_cells"]
@@ -418,10 +431,15 @@ def main():
                 "flops": flops, "n_det": work[0][0],
                 "n_bisector": work[0][1], "n_vout": work[0][2],
                 "n_exact_det_all_tiers": sum(w[3] for w in work)}
+    clip_kernel = f"k_clip<{G}, {CAPS_NAMES[caps]}, false>"
     traffic = None
     if os.path.exists(a.traffic_json):
         with open(a.traffic_json) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        # only a measurement of this very kernel counts
+        if tj.get("kernel", "").replace("rpd::", "").replace(
+                "void ", "").startswith(clip_kernel):
+            traffic = tj.get("bytes_per_launch")
     # ---- learned path on the same resident instance (model trained on
     # other seeds of the same generator, untimed)
     learned = None
@@ -437,9 +455,9 @@ def main():
             model = "trained on the 10^3 generator, seeds 1-64 (6 epochs)"
         t_learn_prep = time.perf_counter() - t0
         ctx.mlp_set(params)
-        kw = dict(cells_per_wave=a.cells_per_wave, learned=1,
+        kw = dict(shape, learned=1,
                   logit_threshold=thr)
-        ctx.run(cells_per_wave=a.cells_per_wave)
+        ctx.run(**shape)
         ref_res = ctx.download()
         lst = ctx.run(**kw)
         identical = same_result(ctx.download(), ref_res)
@@ -478,7 +496,7 @@ def main():
     # host-buffer boundary: upload + one run + download of the cell stream
     t0 = time.perf_counter()
     ctx.upload(inst)
-    ctx.run(cells_per_wave=a.cells_per_wave)
+    ctx.run(**shape)
     res = ctx.download()
     t_pcie = time.perf_counter() - t0
     del res
@@ -499,7 +517,8 @@ def main():
             f"{inst.n_site} non-hidden sites" if a.workload == "cube64"
             else "unit cube 10^3 Kuhn grid, 500 spheres r ~ U[0.02,0.08]"),
             "n_tet": inst.n_tet, "n_site": inst.n_site,
-            "cells_per_wave": a.cells_per_wave,
+            "cells_per_wave": G, "small_caps": caps,
+            "shape": "autotuned" if tune_ms else "given",
             "path": "learned" if learned is not None else "plain"},
         "plain": plain,
         "learned": learned,
@@ -508,10 +527,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": f"k_clip<{a.cells_per_wave}, Caps<32,16,48>, false>",
+                     "kernel": clip_kernel,
                      "kernel_ms": clip_ms, "alg_bytes": alg,
                      "measured_on": "the plain step's clip launch"},
         "roofline_fp64_valu": fp64,
+        "tune_ms": ({f"caps{c}_G{g}": v for (c, g), v in tune_ms.items()}
+                    if tune_ms else None),
         "tets_per_s": inst.n_tet * 1e3 / ms_step,
         "fallback_pct": learned["fallback_pct"] if learned else 0.0,
         "exact_rerun_cells": st["n_rerun_exact"],

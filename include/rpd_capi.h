@@ -80,7 +80,9 @@ typedef struct {
                                execute into rpd_stats.work (adds atomics:
                                for roofline figures, not for timing)    */
   int32_t small_caps;       /* capacities (vertices/planes/edges) of the
-                               small tier: 0 = 32/16/48, 1 = 48/24/72  */
+                               small tier: 0 = 32/16/48, 1 = 48/24/72,
+                               2 = 16/12/32, 3 = 24/14/40 (2 and 3 with
+                               cells_per_wave 4, 8 or 16)               */
   int32_t learned;          /* 1: per-tet sphere sets proposed by the
                                context's model (rpd_mlp_*), certified,
                                failing tets re-run on the full relation;
@@ -195,11 +197,12 @@ int rpd_update_sites(rpd_ctx* ctx, int32_t n_site, const double* sites,
 int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg);
 
 /* Autotuner (no reference counterpart): runs the pipeline on the resident
- * batch for every clip shape -- 4, 8 or 16 cells per wavefront and small-
- * tier capacities 32/16/48 or 48/24/72 -- times the clip phases (clip plus
- * re-runs; best of two after a warm-up) and keeps the fastest for runs
- * with cells_per_wave = 0.  The other fields of base (or the defaults) are
- * used as given.  ms_out[3 * caps + shape] receives the times. */
+ * batch for every clip shape -- 4, 8 or 16 cells per wavefront and the four
+ * small-tier capacities of rpd_run_cfg.small_caps -- times the clip phases
+ * (clip plus re-runs; best of two after a warm-up) and keeps the fastest
+ * for runs with cells_per_wave = 0.  The other fields of base (or the
+ * defaults) are used as given.  ms_out[3 * caps + shape] (12 entries)
+ * receives the times. */
 int rpd_tune(rpd_ctx* ctx, const rpd_run_cfg* base, int32_t* best_G,
              int32_t* best_caps, double* ms_out);
 int rpd_sync(rpd_ctx* ctx);

@@ -52,6 +52,8 @@ struct Caps {
 };
 using SmallCaps = Caps<32, 16, 48>;
 using WideCaps = Caps<48, 24, 72>;   // small tier for larger cells
+using TightCaps = Caps<16, 12, 32>;  // small tier for small cells
+using MidCaps = Caps<24, 14, 40>;
 using LargeCaps = Caps<96, 64, 152>;
 
 // Output slot of one cell (indexed by cell or by re-run index).

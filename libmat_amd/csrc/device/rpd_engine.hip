@@ -474,13 +474,22 @@ int32_t rpd_batch_size(const rpd_ctx* ctx) {
 }  // extern "C"
 
 namespace {
-// capacities of the small tier: 0 = SmallCaps, 1 = WideCaps
+// capacities of the small tier: 0 = SmallCaps, 1 = WideCaps, 2 = TightCaps,
+// 3 = MidCaps (2 and 3 with 4, 8 or 16 cells per wavefront)
 struct CapsRT {
   int V, P, E;
 };
+template <class C>
+CapsRT caps_rt() {
+  return CapsRT{C::V, C::P, C::E};
+}
 CapsRT small_caps(int id) {
-  return id == 1 ? CapsRT{WideCaps::V, WideCaps::P, WideCaps::E}
-                 : CapsRT{SmallCaps::V, SmallCaps::P, SmallCaps::E};
+  switch (id) {
+    case 1: return caps_rt<WideCaps>();
+    case 2: return caps_rt<TightCaps>();
+    case 3: return caps_rt<MidCaps>();
+    default: return caps_rt<SmallCaps>();
+  }
 }
 
 static int launch_clip(rpd_ctx* ctx, int tier, int G, ClipArgs& a,
@@ -491,7 +500,21 @@ static int launch_clip(rpd_ctx* ctx, int tier, int G, ClipArgs& a,
   hipLaunchKernelGGL((k_clip<GG, CAPS, EX>), dim3(blocks), dim3(64), 0, \
                      ctx->stream, a)
   if (tier == 0) {
-    if (caps_id == 1) {
+    if (caps_id == 2 || caps_id == 3) {
+      if (caps_id == 2) {
+        switch (G) {
+          case 4: RPD_LAUNCH(4, TightCaps, false); break;
+          case 8: RPD_LAUNCH(8, TightCaps, false); break;
+          default: RPD_LAUNCH(16, TightCaps, false); break;
+        }
+      } else {
+        switch (G) {
+          case 4: RPD_LAUNCH(4, MidCaps, false); break;
+          case 8: RPD_LAUNCH(8, MidCaps, false); break;
+          default: RPD_LAUNCH(16, MidCaps, false); break;
+        }
+      }
+    } else if (caps_id == 1) {
       switch (G) {
         case 1: RPD_LAUNCH(1, WideCaps, false); break;
         case 2: RPD_LAUNCH(2, WideCaps, false); break;
@@ -1265,12 +1288,14 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     cfg.cells_per_wave = ctx->tuned_G > 0 ? ctx->tuned_G : 8;
     if (ctx->tuned_G > 0) cfg.small_caps = ctx->tuned_caps;
   }
-  if (cfg.small_caps != 0 && cfg.small_caps != 1)
-    return fail(ctx, "rpd_run: small_caps must be 0 or 1");
+  if (cfg.small_caps < 0 || cfg.small_caps > 3)
+    return fail(ctx, "rpd_run: small_caps must be 0, 1, 2 or 3");
   R.sc = small_caps(cfg.small_caps);
   R.G = cfg.cells_per_wave;
   if (R.G != 1 && R.G != 2 && R.G != 4 && R.G != 8 && R.G != 16)
     return fail(ctx, "rpd_run: cells_per_wave must be 1, 2, 4, 8 or 16");
+  if (cfg.small_caps >= 2 && R.G < 4)
+    return fail(ctx, "rpd_run: small_caps 2 and 3 need cells_per_wave >= 4");
   const int32_t site_k = ctx->st.site_k;
   std::memset(&ctx->st, 0, sizeof(ctx->st));
   ctx->st.site_k = site_k;
@@ -1420,7 +1445,7 @@ int rpd_calib_copy(rpd_ctx* ctx, int64_t bytes) {
 }
 
 int rpd_tune(rpd_ctx* ctx, const rpd_run_cfg* base, int32_t* best_G,
-             int32_t* best_caps, double* ms_out /* 6, or NULL */) {
+             int32_t* best_caps, double* ms_out /* 12, or NULL */) {
   if (!ctx) return -1;
   rpd_run_cfg cfg{};
   cfg.use_active_mask = 1;
@@ -1430,7 +1455,7 @@ int rpd_tune(rpd_ctx* ctx, const rpd_run_cfg* base, int32_t* best_G,
   static const int kG[3] = {4, 8, 16};
   double best = 1e300;
   int bg = 8, bc = 0;
-  for (int c = 0; c < 2; ++c)
+  for (int c = 0; c < 4; ++c)
     for (int q = 0; q < 3; ++q) {
       cfg.cells_per_wave = kG[q];
       cfg.small_caps = c;

@@ -283,10 +283,11 @@ class Context:
         """Pick the fastest clip shape for the resident batch (used by
         runs with cells_per_wave=0); returns (G, caps, {(caps, G): ms})."""
         g, c = ctypes.c_int32(), ctypes.c_int32()
-        ms = np.zeros(6)
+        ms = np.zeros(12)
         self._check(self._L.rpd_tune(self._h, None, ctypes.byref(g),
                                      ctypes.byref(c), _p(ms)), "rpd_tune")
-        return g.value, c.value, {(cc, G): ms[3 * cc + q] for cc in (0, 1)
+        return g.value, c.value, {(cc, G): ms[3 * cc + q]
+                                  for cc in range(4)
                                   for q, G in enumerate((4, 8, 16))}
 
     # ---- learned proposals

@@ -86,6 +86,28 @@ def test_learned_runs_match_plain(ctx, mode):
     compare_with_oracle(res, orc)
 
 
+def test_learned_batch_matches_plain_batch(ctx):
+    # uneven instances in one batch, uniform and clustered spheres: the
+    # learned and top-k pipelines emit the plain batch's cells exactly
+    insts = [prep.cube10_instance(5), prep.cube10_instance(6)]
+    verts, tets = prep.grid_mesh(12, 0.1, 8)
+    insts.append(prep.build_instance(verts, tets, prep.spheres(2500, 2, 8)))
+    verts, tets = prep.grid_mesh(7, 0.1, 9)
+    insts.append(prep.build_instance(verts, tets, prep.spheres(300, 1, 9)))
+    ctx.upload_batch(insts)
+    ctx.run()
+    plain = ctx.download()
+    ctx.mlp_init(4)
+    for kw in (dict(learned=1, logit_threshold=0.0),
+               dict(learned=1, logit_threshold=2.0),
+               dict(learned=2, proposal_k=6)):
+        st = ctx.run(**kw)
+        print(f"[learned] batch {kw}: first-round {st['learned'][0]}, "
+              f"re-run tets {st['learned'][1]}, full relation "
+              f"{st['learned'][3]}", flush=True)
+        _same_result(ctx.download(), plain)
+
+
 def test_training_gains_over_untrained_and_topk(ctx):
     train = [prep.cube10_instance(s) for s in range(16)]
     params, loss, thr = learn.train_proposals(ctx, train, epochs=6)

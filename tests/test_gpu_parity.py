@@ -42,11 +42,31 @@ def test_cube10_parity(ctx, seed):
     assert st["n_dropped"][6] == orc.drops[6]
 
 
+def test_clustered_spheres_parity(ctx):
+    # spheres from the 32-component Gaussian mixture (radii lognormal,
+    # clipped to the boundary): dense clusters, many hidden spheres
+    verts, tets = prep.grid_mesh(12, 0.1, 7)
+    inst = prep.build_instance(verts, tets, prep.spheres(3000, 2, 7))
+    orc = oracle.run(inst, mode=0)
+    print(f"[parity] clustered n_tet={inst.n_tet} n_site={inst.n_site} "
+          f"oracle cells={orc.cells.size}", flush=True)
+    ctx.upload(inst)
+    st = ctx.run()
+    off, ids = ctx.relation()
+    compare_relation(off, ids, orc)
+    m = compare_with_oracle(ctx.download(), orc)
+    print(f"[parity] clustered {m} cells={st['n_cells']} "
+          f"rerun={st['n_rerun_large']}/{st['n_rerun_exact']}", flush=True)
+    assert st["n_cells"] == orc.cells.size
+
+
 @pytest.mark.parametrize("cfg", [
     dict(cells_per_wave=1), dict(cells_per_wave=2), dict(cells_per_wave=4),
     dict(cells_per_wave=8), dict(cells_per_wave=16),
     dict(use_active_mask=False), dict(small_tier=False),
-    dict(force_exact=True)])
+    dict(force_exact=True), dict(cells_per_wave=4, small_caps=2),
+    dict(cells_per_wave=8, small_caps=3), dict(cells_per_wave=16,
+                                               small_caps=2)])
 def test_cube10_variants_identical(ctx, cfg):
     inst, orc = c1(0)
     print(f"[variant] {cfg}", flush=True)
@@ -259,9 +279,12 @@ def test_tune_picks_a_shape_with_identical_results(ctx):
     ctx.upload(inst)
     G, caps, ms = ctx.tune()
     print(f"[tune] G={G} caps={caps} ms={ms}", flush=True)
-    assert G in (4, 8, 16) and caps in (0, 1)
-    assert all(v > 0 for v in ms.values())
+    assert G in (4, 8, 16) and caps in (0, 1, 2, 3)
+    assert all(v > 0 for v in ms.values()) and len(ms) == 12
     ctx.run(cells_per_wave=0)
     compare_with_oracle(ctx.download(), orc)
-    ctx.run(cells_per_wave=16, small_caps=1)
-    compare_with_oracle(ctx.download(), orc)
+    for c in range(4):
+        st = ctx.run(cells_per_wave=16, small_caps=c)
+        print(f"[tune] caps={c}: capacity re-runs {st['n_rerun_large']}",
+              flush=True)
+        compare_with_oracle(ctx.download(), orc)
