@@ -19,6 +19,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #ifndef LIBMAT_RPD_NO_REFERENCE_TYPES
@@ -27,7 +28,15 @@ typedef unsigned int uint;
 typedef double Scalar;
 typedef std::array<int, 2> aint2;
 typedef std::array<int, 3> aint3;
+typedef std::array<int, 4> aint4;
+typedef std::array<int, 5> aint5;
 #endif
+// a point with a surface facet id (LibMAT's v2int = pair<Vector3, int>)
+typedef std::array<double, 3> rpd_vec3;
+typedef std::pair<rpd_vec3, int> rpd_v2int;
+// feature edge types of TetMesh::tet_es2fe_map (input_types.h:13-17)
+constexpr int RPD_EDGE_SE = 1;  // convex sharp edge
+constexpr int RPD_EDGE_CE = 2;  // concave edge
 
 #define _MAX_P_ 64
 #define _MAX_T_ 96
@@ -174,28 +183,59 @@ std::vector<ConvexCellHost> compute_clipped_voro_diagram_GPU_tet_eadj(
     std::vector<float>& site_cell_vol);
 
 // ------------------------------------------------------------ adjacency
-// The adjacency part of PowerCell (src/matbase/medial_sphere.h:145-260)
-// that update_power_cells / get_all_voro_info derive from the cells of one
-// sphere (rpd_update.cxx:80-340): cell ids and tets, cell <-> cell through
-// shared tet faces, cell -> tet face ids, halfplane facets and edges of the
-// power cell, and its vertices keyed by [cell id, local vertex].
+// The adjacency part of PowerCell (src/matbase/medial_sphere.h:145-270)
+// that update_power_cells derives from the cells of one sphere: what
+// get_all_voro_info fills (rpd_update.cxx:80-340) -- cell ids and tets,
+// cell <-> cell through shared tet faces, cell -> tet face ids, surface face
+// centroids, halfplane facets and edges of the power cell, its vertices and
+// their positions keyed by [cell id, local vertex], the end vertices of its
+// edges, the covered sharp / concave feature edges and sharp-line end
+// points -- and the connected components of update_pc_cc_info,
+// update_pc_facet_cc_info and update_pc_edge_cc_info (rpd_update.cxx:
+// 409-521).
 struct PowerCellAdjacency {
   int voro_id = -1;
   std::set<int> cell_ids;
   std::set<int> tet_ids;
   std::map<int, std::set<int>> cell_neighbors;
   std::map<int, std::set<int>> cell_to_tfids;
+  // cell id -> [(centroid of the cell's face on the surface, surface fid)]
+  std::map<int, std::vector<rpd_v2int>> cell_to_surfv2fid;
   std::map<int, std::set<int>> facet_neigh_to_cells;
   std::map<aint2, std::set<int>> e_to_cells;
+  // [neigh min, neigh max] -> [[cell, lvid], [cell, lvid]] per convex edge
+  std::map<aint2, std::vector<std::array<aint2, 2>>> edge_2endvertices;
   std::map<aint2, aint3> vertex_2id;
+  // [cell, lvid] -> (position, surface fid or -1)
+  std::map<aint2, rpd_v2int> vertex_2pos;
+  // [cell, lvid1, lvid2, fe_line_id, fe_id] of covered feature edges
+  std::set<aint5> se_covered_lvids, ce_covered_lvids;
+  // [cell, lvid, neigh id, se_line_id] -> position
+  std::map<aint4, rpd_vec3> se_line_endpos;
+  // connected components (through cell_neighbors), each as a set of cells
+  std::vector<std::set<int>> cc_cells;
+  std::vector<std::vector<rpd_v2int>> cc_surf_v2fids;
+  std::map<int, std::vector<std::set<int>>> facet_cc_cells;
+  std::map<int, std::vector<std::vector<rpd_v2int>>> facet_cc_surf_v2fids;
+  std::map<aint2, std::vector<std::set<int>>> edge_cc_cells;
 };
 
 // Builds the adjacency of every sphere id that owns cells (ConvexCellHost::
-// voro_id), keyed by that id; reloads the cells' active elements.
-// max_surf_fid: largest surface facet id (tet faces with f_id <= it lie on
-// the surface).  Multithreaded over spheres.
+// voro_id), keyed by that id; reloads the cells' active elements and
+// explicit faces.  max_surf_fid: largest surface facet id (tet faces with
+// f_id <= it lie on the surface).  tet_es2fe_map: TetMesh::tet_es2fe_map,
+// <tet, local fid min, local fid max> -> <fe_type, fe_id, fe_line_id>
+// (nullptr: no feature edges).  Multithreaded over spheres.
 std::map<int, PowerCellAdjacency> update_power_cells_adjacency(
-    std::vector<ConvexCellHost>& cells, int max_surf_fid, int n_threads = 0);
+    std::vector<ConvexCellHost>& cells, int max_surf_fid,
+    const std::map<aint3, aint3>* tet_es2fe_map = nullptr, int n_threads = 0);
+
+// Connected components of the elements of `to_visit` under `neighbors`
+// (edges to elements outside the set are ignored), in order of their least
+// element (common_cxx.h get_CC_given_neighbors); returns their number.
+int rpd_connected_components(const std::set<int>& to_visit,
+                             const std::map<int, std::set<int>>& neighbors,
+                             std::vector<std::set<int>>& out);
 
 // ------------------------------------------------------------ RPD3D_GPU
 // Minimal stand-ins for the fields of TetMesh (input_types.h:122-147) and
@@ -206,16 +246,10 @@ struct RpdTetMesh {
   std::vector<int> tet_indices;     // 4 per tet
   std::map<int, std::set<int>> v2tets;
   std::vector<int> v_adjs, e_adjs, f_adjs, f_ids;
+  std::map<aint3, aint3> tet_es2fe_map;  // feature edges (input_types.h:147)
 };
 // the PowerCell fields (medial_sphere.h:145-260) the driver fills
-struct RpdPowerCell {
-  int voro_id = -1;
-  std::set<int> cell_ids, tet_ids;
-  std::map<int, std::set<int>> cell_neighbors, cell_to_tfids;
-  std::map<int, std::set<int>> facet_neigh_to_cells;
-  std::map<aint2, std::set<int>> e_to_cells;
-  std::map<aint2, aint3> vertex_2id;
-};
+struct RpdPowerCell : PowerCellAdjacency {};
 struct RpdSphere {
   int id = -1;
   int site_id = -1;

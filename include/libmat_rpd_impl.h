@@ -227,6 +227,83 @@ void RPD3D_GPU_T<TM, SM, PM, SP>::calculate_partial(int& num_itr_global,
   num_sphere_added = 0;
 }
 
+namespace rpd_detail {
+// points of the adjacency into the caller's vector type (LibMAT: Vector3)
+template <class V>
+struct vec3_from {
+  static V make(const rpd_vec3& p) { return V(p[0], p[1], p[2]); }
+};
+template <>
+struct vec3_from<rpd_vec3> {
+  static rpd_vec3 make(const rpd_vec3& p) { return p; }
+};
+template <class P>
+P v2int_from(const rpd_v2int& x) {
+  return P(vec3_from<typename P::first_type>::make(x.first), x.second);
+}
+template <class DV>
+void copy_v2ints(const std::vector<rpd_v2int>& src, DV& dst) {
+  dst.clear();
+  for (const auto& x : src)
+    dst.push_back(v2int_from<typename DV::value_type>(x));
+}
+template <class DVV>
+void copy_v2int_groups(const std::vector<std::vector<rpd_v2int>>& src,
+                       DVV& dst) {
+  dst.clear();
+  for (const auto& g : src) {
+    dst.emplace_back();
+    copy_v2ints(g, dst.back());
+  }
+}
+template <class PC>
+void fill_power_cell(PowerCellAdjacency& a, PC& pc) {
+  pc.cell_ids = std::move(a.cell_ids);
+  pc.tet_ids = std::move(a.tet_ids);
+  pc.cell_neighbors = std::move(a.cell_neighbors);
+  pc.cell_to_tfids = std::move(a.cell_to_tfids);
+  pc.facet_neigh_to_cells = std::move(a.facet_neigh_to_cells);
+  pc.e_to_cells = std::move(a.e_to_cells);
+  pc.vertex_2id = std::move(a.vertex_2id);
+  pc.edge_2endvertices = std::move(a.edge_2endvertices);
+  pc.se_covered_lvids = std::move(a.se_covered_lvids);
+  pc.ce_covered_lvids = std::move(a.ce_covered_lvids);
+  pc.cc_cells = std::move(a.cc_cells);
+  pc.facet_cc_cells = std::move(a.facet_cc_cells);
+  pc.edge_cc_cells = std::move(a.edge_cc_cells);
+  pc.cell_to_surfv2fid.clear();
+  for (const auto& kv : a.cell_to_surfv2fid)
+    copy_v2ints(kv.second, pc.cell_to_surfv2fid[kv.first]);
+  pc.vertex_2pos.clear();
+  for (const auto& kv : a.vertex_2pos)
+    pc.vertex_2pos[kv.first] =
+        v2int_from<typename std::decay_t<decltype(pc.vertex_2pos)>::
+                       mapped_type>(kv.second);
+  pc.se_line_endpos.clear();
+  for (const auto& kv : a.se_line_endpos)
+    pc.se_line_endpos[kv.first] = vec3_from<typename std::decay_t<
+        decltype(pc.se_line_endpos)>::mapped_type>::make(kv.second);
+  copy_v2int_groups(a.cc_surf_v2fids, pc.cc_surf_v2fids);
+  pc.facet_cc_surf_v2fids.clear();
+  for (const auto& kv : a.facet_cc_surf_v2fids)
+    copy_v2int_groups(kv.second, pc.facet_cc_surf_v2fids[kv.first]);
+}
+template <class PC>
+void clear_power_cell(PC& pc) {
+  PowerCellAdjacency empty;
+  fill_power_cell(empty, pc);
+}
+// TetMesh::tet_es2fe_map when the mesh type has one
+template <class TM>
+auto es2fe_of(const TM* m, int) -> decltype(&m->tet_es2fe_map) {
+  return &m->tet_es2fe_map;
+}
+template <class TM>
+const std::map<aint3, aint3>* es2fe_of(const TM*, long) {
+  return nullptr;
+}
+}  // namespace rpd_detail
+
 template <class TM, class SM, class PM, class SP>
 void RPD3D_GPU_T<TM, SM, PM, SP>::update_spheres_power_cells() {
   // the adjacency part of update_power_cells (rpd_update.cxx:568-640):
@@ -238,31 +315,18 @@ void RPD3D_GPU_T<TM, SM, PM, SP>::update_spheres_power_cells() {
     for (size_t q = 0; q < fa.size(); ++q)
       if (fa[q] == 1) max_fid = std::max(max_fid, tet_mesh->f_ids[q]);
   }
-  std::map<int, PowerCellAdjacency> adj =
-      update_power_cells_adjacency(powercells, max_fid);
+  std::map<int, PowerCellAdjacency> adj = update_power_cells_adjacency(
+      powercells, max_fid, rpd_detail::es2fe_of(tet_mesh, 0));
   for (SP& m : *all_medial_spheres) {
     auto it = adj.find(m.id);
     auto& pc = m.pcell;
     pc.voro_id = m.id;
     if (it == adj.end()) {
-      pc.cell_ids.clear();
-      pc.tet_ids.clear();
-      pc.cell_neighbors.clear();
-      pc.cell_to_tfids.clear();
-      pc.facet_neigh_to_cells.clear();
-      pc.e_to_cells.clear();
-      pc.vertex_2id.clear();
+      rpd_detail::clear_power_cell(pc);
       if (!m.is_deleted) m.is_deleted = true;
       continue;
     }
-    PowerCellAdjacency& a = it->second;
-    pc.cell_ids = std::move(a.cell_ids);
-    pc.tet_ids = std::move(a.tet_ids);
-    pc.cell_neighbors = std::move(a.cell_neighbors);
-    pc.cell_to_tfids = std::move(a.cell_to_tfids);
-    pc.facet_neigh_to_cells = std::move(a.facet_neigh_to_cells);
-    pc.e_to_cells = std::move(a.e_to_cells);
-    pc.vertex_2id = std::move(a.vertex_2id);
+    rpd_detail::fill_power_cell(it->second, pc);
   }
 }
 

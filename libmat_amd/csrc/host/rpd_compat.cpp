@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <iterator>
 #include <mutex>
 #include <thread>
 
@@ -427,8 +428,72 @@ void libmat_rpd::power_neighbors(
 }
 
 // ------------------------------------------------------------ adjacency
+int rpd_connected_components(const std::set<int>& to_visit,
+                             const std::map<int, std::set<int>>& neighbors,
+                             std::vector<std::set<int>>& out) {
+  out.clear();
+  std::set<int> seen;
+  for (int start : to_visit) {  // ascending: components by least element
+    if (seen.count(start)) continue;
+    std::set<int> comp;
+    std::vector<int> stack{start};
+    seen.insert(start);
+    while (!stack.empty()) {
+      const int c = stack.back();
+      stack.pop_back();
+      comp.insert(c);
+      auto it = neighbors.find(c);
+      if (it == neighbors.end()) continue;
+      for (int nb : it->second)
+        if (to_visit.count(nb) && seen.insert(nb).second) stack.push_back(nb);
+    }
+    out.push_back(std::move(comp));
+  }
+  return (int)out.size();
+}
+
+namespace {
+
+rpd_vec3 vertex_pos(const ConvexCellHost& c, int v) {
+  const cuchar4 w = c.ver_trans_const(v);
+  const cfloat4 p = c.compute_vertex_coordinates(cuchar3{w.x, w.y, w.z});
+  return rpd_vec3{{(double)p.x, (double)p.y, (double)p.z}};
+}
+
+// the two local vertices shared by two active faces of an explicit cell
+aint2 shared_vertices(const ConvexCellHost& c, int lf1, int lf2) {
+  const auto& f1 = c.pc_local_active_faces.at(c.pc_lf2active_map.at(lf1));
+  const auto& f2 = c.pc_local_active_faces.at(c.pc_lf2active_map.at(lf2));
+  std::set<int> s1(f1.begin(), f1.end()), s2(f2.begin(), f2.end());
+  std::vector<int> common;
+  std::set_intersection(s1.begin(), s1.end(), s2.begin(), s2.end(),
+                        std::back_inserter(common));
+  if (common.size() != 2)
+    throw std::runtime_error("adjacency: an active edge without two vertices");
+  return aint2{{common[0], common[1]}};
+}
+
+std::vector<std::vector<rpd_v2int>> surf_groups(
+    const std::map<int, std::vector<rpd_v2int>>& cell_to_surfv2fid,
+    const std::vector<std::set<int>>& groups) {
+  std::vector<std::vector<rpd_v2int>> out;
+  for (const auto& g : groups) {
+    std::vector<rpd_v2int> v;
+    for (int cid : g) {
+      auto it = cell_to_surfv2fid.find(cid);
+      if (it != cell_to_surfv2fid.end())
+        v.insert(v.end(), it->second.begin(), it->second.end());
+    }
+    out.push_back(std::move(v));
+  }
+  return out;
+}
+
+}  // namespace
+
 std::map<int, PowerCellAdjacency> update_power_cells_adjacency(
-    std::vector<ConvexCellHost>& cells, int max_surf_fid, int n_threads) {
+    std::vector<ConvexCellHost>& cells, int max_surf_fid,
+    const std::map<aint3, aint3>* tet_es2fe_map, int n_threads) {
   std::map<int, PowerCellAdjacency> pcs;
   for (auto& c : cells) {
     auto& pc = pcs[c.voro_id];
@@ -436,32 +501,57 @@ std::map<int, PowerCellAdjacency> update_power_cells_adjacency(
     pc.cell_ids.insert(c.id);
     pc.tet_ids.insert(c.tet_id);
     if (!c.is_active_updated) c.reload_active();
+    if (!c.is_pc_explicit) c.reload_pc_explicit();
   }
   std::vector<int> keys;
   for (auto& kv : pcs) keys.push_back(kv.first);
   std::map<int, int> id2pos;
   for (size_t q = 0; q < cells.size(); ++q) id2pos[cells[q].id] = (int)q;
+  auto neigh_of = [](const ConvexCellHost& c, cint2 hp) {
+    return hp.x == c.voro_id ? hp.y : hp.x;
+  };
   auto work = [&](int sphere) {
     PowerCellAdjacency& pc = pcs.at(sphere);
     std::map<int, std::set<int>> tfid_to_cells;
     for (int cid : pc.cell_ids) {
       const ConvexCellHost& c = cells[id2pos.at(cid)];
+      // facets: halfplanes by neighbour, tet faces (and their centroid on
+      // the surface) by face id
+      int lfa = 0;
       for (int p = 0; p < c.nb_p; ++p) {
         if (c.active_clipping_planes[p] <= 0) continue;
         cint2 hp = c.clip_id2_const(p);
         if (hp.y != -1) {
-          int nb = hp.x == c.voro_id ? hp.y : hp.x;
-          pc.facet_neigh_to_cells[nb].insert(cid);
+          pc.facet_neigh_to_cells[neigh_of(c, hp)].insert(cid);
         } else {
+          if (hp.x <= max_surf_fid) {
+            cfloat3 cen{0.f, 0.f, 0.f};
+            const auto& face = c.pc_local_active_faces.at(lfa);
+            for (int lv : face) {
+              cen.x += c.pc_points.at(lv).x;
+              cen.y += c.pc_points.at(lv).y;
+              cen.z += c.pc_points.at(lv).z;
+            }
+            const float n = (float)face.size();
+            pc.cell_to_surfv2fid[cid].push_back(rpd_v2int(
+                rpd_vec3{{cen.x / n, cen.y / n, cen.z / n}}, hp.x));
+          }
           tfid_to_cells[hp.x].insert(cid);
           pc.cell_to_tfids[cid].insert(hp.x);
         }
+        ++lfa;
       }
+      // vertices of the power cell: 3 neighbour seeds, or 2 and -1 on the
+      // surface; convex-cell-only vertices are skipped
       for (int v = 0; v < c.nb_v; ++v) {
         std::set<int> seeds;
+        int surf_fid = -1;
         for (int k = 0; k < 3; ++k) {
           cint2 hp = c.clip_id2_const(c.ith_plane_const(v, k));
-          if (hp.y != -1) seeds.insert(hp.x == c.voro_id ? hp.y : hp.x);
+          if (hp.y != -1)
+            seeds.insert(neigh_of(c, hp));
+          else if (hp.x <= max_surf_fid)
+            surf_fid = hp.x;
         }
         if (seeds.size() < 2) continue;
         if (seeds.size() == 2) seeds.insert(-1);
@@ -469,26 +559,76 @@ std::map<int, PowerCellAdjacency> update_power_cells_adjacency(
         int q = 0;
         for (int sd : seeds) key3[q++] = sd;
         std::sort(key3.begin(), key3.end());
-        pc.vertex_2id[aint2{{cid, v}}] = key3;
+        const aint2 key{{cid, v}};
+        pc.vertex_2id[key] = key3;
+        pc.vertex_2pos[key] = rpd_v2int(vertex_pos(c, v), surf_fid);
       }
+      // edges: feature edges of the tet; power-cell edges between two
+      // halfplanes with their end vertices
       for (int e = 0; e < c.nb_e; ++e) {
         if (c.active_edges[e] <= 0) continue;
         cuchar2 ef = c.edge_id2_const(e);
         cint2 h1 = c.clip_id2_const(ef.x), h2 = c.clip_id2_const(ef.y);
+        if (h1.y == -1 && h2.y == -1 && tet_es2fe_map) {
+          aint2 lfs{{ef.x, ef.y}};
+          std::sort(lfs.begin(), lfs.end());
+          auto it = tet_es2fe_map->find(aint3{{c.tet_id, lfs[0], lfs[1]}});
+          if (it != tet_es2fe_map->end()) {
+            const aint2 lv = shared_vertices(c, lfs[0], lfs[1]);
+            const aint3& fe = it->second;  // <fe_type, fe_id, fe_line_id>
+            const aint5 k5{{cid, lv[0], lv[1], fe[2], fe[1]}};
+            if (fe[0] == RPD_EDGE_SE)
+              pc.se_covered_lvids.insert(k5);
+            else if (fe[0] == RPD_EDGE_CE)
+              pc.ce_covered_lvids.insert(k5);
+            // an end of the feature line inside the power cell: the first
+            // halfplane at either end vertex names it
+            for (int lvid : lv)
+              for (int k = 0; k < 3; ++k) {
+                cint2 hp = c.clip_id2_const(c.ith_plane_const(lvid, k));
+                if (hp.y == -1) continue;
+                pc.se_line_endpos[aint4{{cid, lvid, neigh_of(c, hp), fe[2]}}] =
+                    vertex_pos(c, lvid);
+                break;
+              }
+          }
+          continue;
+        }
         if (h1.y == -1 || h2.y == -1) continue;
-        aint2 eu{{h1.x == c.voro_id ? h1.y : h1.x,
-                  h2.x == c.voro_id ? h2.y : h2.x}};
+        aint2 eu{{neigh_of(c, h1), neigh_of(c, h2)}};
         std::sort(eu.begin(), eu.end());
         pc.e_to_cells[eu].insert(cid);
+        const aint2 ends = shared_vertices(c, ef.x, ef.y);
+        const aint2 k1{{cid, ends[0]}}, k2{{cid, ends[1]}};
+        if (!pc.vertex_2id.count(k1) || !pc.vertex_2id.count(k2))
+          throw std::runtime_error(
+              "adjacency: a power-cell edge ends at a non-power-cell vertex");
+        pc.edge_2endvertices[eu].push_back({{k1, k2}});
       }
     }
-    (void)max_surf_fid;
     for (auto& kv : tfid_to_cells) {
       if (kv.second.size() != 2) continue;
       int a = *kv.second.begin(), b = *kv.second.rbegin();
       pc.cell_neighbors[a].insert(b);
       pc.cell_neighbors[b].insert(a);
     }
+    // connected components (update_pc_cc_info, update_pc_facet_cc_info,
+    // update_pc_edge_cc_info)
+    rpd_connected_components(pc.cell_ids, pc.cell_neighbors, pc.cc_cells);
+    pc.cc_surf_v2fids = surf_groups(pc.cell_to_surfv2fid, pc.cc_cells);
+    for (auto& kv : pc.facet_neigh_to_cells) {
+      auto& groups = pc.facet_cc_cells[kv.first];
+      rpd_connected_components(kv.second, pc.cell_neighbors, groups);
+      std::stable_sort(groups.begin(), groups.end(),
+                       [](const std::set<int>& x, const std::set<int>& y) {
+                         return x.size() < y.size();
+                       });
+      pc.facet_cc_surf_v2fids[kv.first] =
+          surf_groups(pc.cell_to_surfv2fid, groups);
+    }
+    for (auto& kv : pc.e_to_cells)
+      rpd_connected_components(kv.second, pc.cell_neighbors,
+                               pc.edge_cc_cells[kv.first]);
   };
   int nt = n_threads > 0 ? n_threads : (int)std::thread::hardware_concurrency();
   nt = std::max(1, std::min<int>(nt, (int)keys.size()));

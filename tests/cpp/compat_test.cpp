@@ -1,6 +1,8 @@
 // Drop-in C++ surface test: ConvexCellHost layout (always) and, with
 // --gpu, compute_clipped_voro_diagram_GPU / RPD3D_GPU / adjacency on the
 // 10^3-grid, 500-sphere instance built through the C ABI of rpd_prep.h.
+#include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
@@ -111,6 +113,105 @@ int main(int argc, char** argv) {
   std::printf("adjacency spheres %zu, facets %zu, asymmetric %d\n",
               adj.size(), fac.size(), asym);
   CHECK(asym == 0);
+
+  // ---- feature edges: the 12 cube edges are sharp; every tet edge on one
+  // maps <tet, lfid min, lfid max> (the two faces at the edge, face k
+  // opposite vertex k) -> <SE, cube edge, cube edge>
+  auto on_bnd = [](double x) { return x == 0.0 || x == 1.0; };
+  auto cube_edge = [&](const double* a, const double* b) {
+    for (int ax = 0; ax < 3; ++ax) {
+      const int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+      if (on_bnd(a[o1]) && on_bnd(a[o2]) && a[o1] == b[o1] && a[o2] == b[o2])
+        return ax * 4 + 2 * (int)a[o1] + (int)a[o2];
+    }
+    return -1;
+  };
+  std::map<aint3, aint3> es2fe;
+  for (int t = 0; t < nt; ++t)
+    for (int k = 0; k < 6; ++k) {
+      const int i = el[k][0], j = el[k][1];
+      const int ce = cube_edge(&vd[3 * tets[4 * t + i]], &vd[3 * tets[4 * t + j]]);
+      if (ce < 0) continue;
+      int f[2], q = 0;
+      for (int l = 0; l < 4; ++l)
+        if (l != i && l != j) f[q++] = l;
+      es2fe[aint3{{t, f[0], f[1]}}] = aint3{{RPD_EDGE_SE, ce, ce}};
+    }
+  auto adj2 = update_power_cells_adjacency(cc, n_sf - 1, &es2fe);
+  std::map<int, const ConvexCellHost*> byid;
+  for (auto& c : cc) byid[c.id] = &c;
+  auto pos_of = [&](int cid, int lv) {
+    const ConvexCellHost& c = *byid.at(cid);
+    cuchar4 w = c.ver_trans_const(lv);
+    cfloat4 p = c.compute_vertex_coordinates(cuchar3{w.x, w.y, w.z});
+    return std::array<double, 3>{{p.x, p.y, p.z}};
+  };
+  auto bdist = [](const std::array<double, 3>& p) {
+    double d = 1e9;
+    for (int a = 0; a < 3; ++a)
+      d = std::min(d, std::min(std::fabs(p[a]), std::fabs(1.0 - p[a])));
+    return d;
+  };
+  auto on_cube_edge = [](const std::array<double, 3>& p, int ce) {
+    const int ax = ce / 4, o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+    return std::fabs(p[o1] - (double)((ce >> 1) & 1)) < 1e-5 &&
+           std::fabs(p[o2] - (double)(ce & 1)) < 1e-5;
+  };
+  auto unite = [](const std::vector<std::set<int>>& groups, std::set<int>& u) {
+    size_t n = 0;
+    for (auto& g : groups) {
+      u.insert(g.begin(), g.end());
+      n += g.size();
+    }
+    return n == u.size();  // disjoint
+  };
+  size_t n_surf = 0, n_se = 0, n_end = 0, n_e2v = 0;
+  for (auto& kv : adj2) {
+    const PowerCellAdjacency& a = kv.second;
+    std::set<int> u;
+    CHECK(!a.cc_cells.empty() && unite(a.cc_cells, u) && u == a.cell_ids);
+    CHECK(a.cc_surf_v2fids.size() == a.cc_cells.size());
+    for (auto& f : a.facet_neigh_to_cells) {
+      std::set<int> uf;
+      CHECK(unite(a.facet_cc_cells.at(f.first), uf) && uf == f.second);
+    }
+    for (auto& e : a.e_to_cells) {
+      std::set<int> ue;
+      CHECK(unite(a.edge_cc_cells.at(e.first), ue) && ue == e.second);
+    }
+    for (auto& c2 : a.cell_to_surfv2fid)
+      for (auto& v2 : c2.second) {
+        CHECK(v2.second >= 0 && v2.second <= n_sf - 1);
+        CHECK(bdist(v2.first) < 1e-5);
+        ++n_surf;
+      }
+    for (auto& vp : a.vertex_2pos) {
+      CHECK(a.vertex_2id.count(vp.first));
+      if (vp.second.second != -1) CHECK(bdist(vp.second.first) < 1e-5);
+    }
+    for (auto& e2 : a.edge_2endvertices)
+      for (auto& ends : e2.second) {
+        for (auto& key : ends) {
+          const aint3& ids = a.vertex_2id.at(key);
+          CHECK(std::count(ids.begin(), ids.end(), e2.first[0]) == 1);
+          CHECK(std::count(ids.begin(), ids.end(), e2.first[1]) == 1);
+        }
+        ++n_e2v;
+      }
+    for (auto& se : a.se_covered_lvids) {
+      CHECK(on_cube_edge(pos_of(se[0], se[1]), se[3]));
+      CHECK(on_cube_edge(pos_of(se[0], se[2]), se[3]));
+      ++n_se;
+    }
+    CHECK(a.ce_covered_lvids.empty());
+    for (auto& ep : a.se_line_endpos) {
+      CHECK(on_cube_edge(ep.second, ep.first[3]));
+      ++n_end;
+    }
+  }
+  std::printf("surface face centroids %zu, edge end pairs %zu, sharp-edge "
+              "pieces %zu, sharp-line ends %zu\n", n_surf, n_e2v, n_se, n_end);
+  CHECK(n_surf > 0 && n_e2v > 0 && n_se > 0 && n_end > 0);
 
   // ---- calculate_partial: move 20 spheres, add one, delete one; the
   // merged cells must equal a full recompute, cell by cell (arrays are in
