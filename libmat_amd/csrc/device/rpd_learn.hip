@@ -226,31 +226,35 @@ __global__ __launch_bounds__(64) void k_knn_features(
   own[2] = vowner[L.tv.y];
   own[3] = vowner[L.tv.z];
   own[4] = vowner[L.tv.w];
-  // the 32 least (key, id), sorted, in registers: a new entry bubbles
-  // through the list (fully unrolled compare/select, no divergence); an
-  // entry equal to one listed is dropped
-  float kk[kOut];
-  int32_t ii[kOut];
+  // the 32 least (key, id), sorted, in registers, as one u64 each: the key's
+  // bits (keys are >= 0, so their bit patterns order as the keys do) above
+  // the id.  An id already listed is dropped; otherwise the entry bubbles
+  // through the list by 64-bit min / max (fully unrolled, no divergence).
+  constexpr uint64_t kEmpty = (0x7f800000ull << 32) | 0x7fffffffull;
+  uint64_t lst[kOut];
 #pragma unroll
-  for (int q = 0; q < kOut; ++q) {
-    kk[q] = INFINITY;
-    ii[q] = 0x7fffffff;
-  }
-  auto ins = [&](float key, int32_t id) {
-    if (!(key < kk[kOut - 1] || (key == kk[kOut - 1] && id < ii[kOut - 1])))
-      return;
+  for (int q = 0; q < kOut; ++q) lst[q] = kEmpty;
+  // chk: the id may be listed already (entries of the second and later
+  // rings; the owners and the first ring hold distinct ids)
+  auto ins = [&](float key, int32_t id, bool chk) {
+    uint64_t x = ((uint64_t)__float_as_uint(key) << 32) | (uint32_t)id;
+    if (!(x < lst[kOut - 1])) return;
+    if (chk) {
+      bool dup = false;
+#pragma unroll
+      for (int q = 0; q < kOut; ++q) dup |= (uint32_t)lst[q] == (uint32_t)id;
+      if (dup) return;
+    }
 #pragma unroll
     for (int q = 0; q < kOut; ++q) {
-      const bool eq = key == kk[q] && id == ii[q];
-      const bool lt = key < kk[q] || (key == kk[q] && id < ii[q]);
-      const float ck = lt ? kk[q] : key;
-      const int32_t ci = lt ? ii[q] : id;
-      kk[q] = lt ? key : kk[q];
-      ii[q] = lt ? id : ii[q];
-      key = eq ? INFINITY : ck;
-      id = eq ? 0x7fffffff : ci;
+      const uint64_t y = lst[q];
+      const bool lt = x < y;
+      lst[q] = lt ? x : y;
+      x = lt ? y : x;
     }
   };
+  auto key_of = [&](int q) { return __uint_as_float((uint32_t)(lst[q] >> 32)); };
+  auto id_of = [&](int q) { return (int32_t)(uint32_t)lst[q]; };
   uint32_t om = 0;
   if (own[1] >= 0) {
     // centroid owner: descend from the corner owner closest to it
@@ -282,7 +286,7 @@ __global__ __launch_bounds__(64) void k_knn_features(
       bool seen = false;
       for (int q = 0; q < j; ++q) seen = seen || own[q] == o;
       if (seen) continue;
-      if (S.flags[o] == 1u) ins(0.f, o);
+      if (S.flags[o] == 1u) ins(0.f, o, false);
       if (j > 0) {
         bool in_ring = false;
         for (int64_t q = r0; q < r1 && !in_ring; ++q)
@@ -290,18 +294,19 @@ __global__ __launch_bounds__(64) void k_knn_features(
         if (in_ring) continue;
       }
       const int64_t e0 = S.knn_off[o], e1 = S.knn_off[o + 1];
-      // four ring entries per round: their loads are in flight together
-      for (int64_t q0 = e0; q0 < e1; q0 += 4) {
-        int32_t w4[4];
-        double4 s4[4];
+      // kRing ring entries per round: their loads are in flight together
+      constexpr int kRing = 2;
+      for (int64_t q0 = e0; q0 < e1; q0 += kRing) {
+        int32_t w4[kRing];
+        double4 s4[kRing];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kRing; ++u) {
           const int64_t q = q0 + u < e1 ? q0 + u : e1 - 1;
           w4[u] = S.knn_near[q];
           s4[u] = S.knn_site_near[q];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kRing; ++u) {
           if (q0 + u >= e1 || w4[u] < 0) continue;  // past the end, unflagged
           const int id = w4[u];
           bool is_owner = false;
@@ -322,21 +327,21 @@ __global__ __launch_bounds__(64) void k_knn_features(
             key = fminf(key, (base + qx[1 + k]) - 2.f * du);
           }
           // non-owners rank after the owners (key 0) whatever the rounding
-          ins(fmaxf(key, 1.17549435e-38f), id);
+          ins(fmaxf(key, 1.17549435e-38f), id, j > 0);
         }
       }
     }
 #pragma unroll
     for (int k = 0; k < kOut; ++k)
-      if (kk[k] == 0.f) {
+      if (key_of(k) == 0.f) {
         bool is_owner = false;
-        for (int a = 0; a < 5; ++a) is_owner = is_owner || own[a] == ii[k];
+        for (int a = 0; a < 5; ++a) is_owner = is_owner || own[a] == id_of(k);
         if (is_owner) om |= 1u << k;
       }
   }
   omask[t] = om;
 #pragma unroll
-  for (int k = 0; k < kOut; ++k) kt[k] = kk[k] < INFINITY ? ii[k] : -1;
+  for (int k = 0; k < kOut; ++k) kt[k] = lst[k] != kEmpty ? id_of(k) : -1;
   const double ext =
       fmax(fmax(I.dom[3] - I.dom[0], I.dom[4] - I.dom[1]), I.dom[5] - I.dom[2]);
   const double inv_ext = ext > 0 ? 1.0 / ext : 0.0;
@@ -349,13 +354,13 @@ __global__ __launch_bounds__(64) void k_knn_features(
   f[2] = (float)((C[2] - I.dom[2]) * inv_ext);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    if (kk[k] < INFINITY) {
-      const double4 sp = S.site[ii[k]];
+    if (lst[k] != kEmpty) {
+      const double4 sp = S.site[id_of(k)];
       f[3 + 5 * k + 0] = (float)(sp.x - C[0]) * inv_r;
       f[3 + 5 * k + 1] = (float)(sp.y - C[1]) * inv_r;
       f[3 + 5 * k + 2] = (float)(sp.z - C[2]) * inv_r;
       f[3 + 5 * k + 3] = (float)sqrt(sp.w) * inv_r;
-      f[3 + 5 * k + 4] = kk[k] * (inv_r * inv_r);
+      f[3 + 5 * k + 4] = key_of(k) * (inv_r * inv_r);
     }
   }
 #pragma unroll
