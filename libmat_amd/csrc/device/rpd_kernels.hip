@@ -773,8 +773,18 @@ __device__ __forceinline__ void vertex_pos(const rpd_d4& a, const rpd_d4& b,
   A[2] = X.z * r;
 }
 
+// resident wavefronts per SIMD the compiler must allow: 4 for the 16-cell
+// shape with the tightest capacities (its register count otherwise lands
+// just above the 4-wave limit of 128; measured 8 % faster), no constraint
+// otherwise (the same hint slows the 8-cell shapes)
+template <int G, class C>
+constexpr int clip_min_waves() {
+  return (G == 16 && C::V <= 16) ? 4 : 1;
+}
+
 template <int G, class C, bool kExact>
-__global__ __launch_bounds__(64) void k_clip(ClipArgs a) {
+__global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
+    ClipArgs a) {
   constexpr int L = 64 / G;
   static_assert(L >= 4, "the tet's four planes are set up one per lane");
   constexpr int V = C::V, P = C::P, E = C::E;
