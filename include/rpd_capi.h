@@ -117,7 +117,10 @@ typedef struct {
   int64_t n_words;            /* payload words                           */
   int64_t n_areas;            /* facet-area entries                      */
   int64_t n_dropped[9];       /* candidates dropped, per rpd_status      */
-  int64_t n_rerun_large;      /* cells re-run at full capacity           */
+  int64_t n_rerun_large;      /* cells over the small tier's capacities,
+                                 re-run (small tiers of 16 or 24 vertices:
+                                 at 32/16/48 first, then at full capacity
+                                 if still over)                          */
   int64_t n_rerun_exact;      /* cells re-run with exact arithmetic      */
   int64_t n_active_planes;    /* neighbour planes not proved inactive
                                  (with count_work, else 0)               */
@@ -134,7 +137,7 @@ typedef struct {
                                  relation fill, plane masks, clip (small
                                  tier), compaction                       */
   int64_t work[3][4];         /* with count_work, per clip tier (small,
-                                 full capacity, exact): FP64 conflict
+                                 capacity re-runs, exact): FP64 conflict
                                  determinants evaluated, bisector planes
                                  built, vertices of the cells the tier
                                  finished with success, exact-arithmetic

@@ -131,11 +131,12 @@ struct rpd_ctx {
   // pipeline
   Buf bbox, reg_n, reg_off, keys_a, keys_b, bin_start, bin_ids;
   Buf tet_count, tet_off, tet_pass, cand, rel_ids;
-  Buf counters;  // int32[8]: fallback, n_rerun_large, n_rerun_exact
-  Buf rerun_large, rerun_exact, loc, sizes, offs, drops, stats64, work,
-      rel_work;
+  Buf counters;  // int32[8]: fallback, n_rerun_large, n_rerun_exact, ...,
+                 // [6] n_rerun_full (after the middle tier)
+  Buf rerun_large, rerun_full, rerun_exact, loc, sizes, offs, drops, stats64,
+      work, rel_work;
   Buf scan_tmp;
-  TierBufs small, large, exact;
+  TierBufs small, large, exact, mid;
   Buf out_cells, out_words, out_geom, out_areas;
   int64_t n_knn = 0;
   // learned proposals: features, model (fp32 master, Adam state, bf16
@@ -560,6 +561,8 @@ struct RunState {
   const int32_t* tet_inst_d = nullptr;
   int64_t n_bins = 0, n_sg = 0;
   int32_t r_large = 0, r_exact = 0;  // re-run entries clipped so far
+  int32_t r_full = 0;  // entries of the full-capacity list after a middle
+                       // tier (tight small tiers only)
   CertArgs cert{};  // learned path: certificate evaluated in the clip
 };
 
@@ -913,13 +916,47 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
   int32_t host_cnt[8];
   HIP_TRY(hipMemcpyAsync(host_cnt, R.cnt, 32, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  const int32_t n_large = host_cnt[1];
-  HIP_TRY(ctx->large.need_keep(n_large, R.r_large, LargeCaps::V, LargeCaps::P,
+  const int32_t n_over = host_cnt[1];
+  // cells over a tight small tier first get the middle tier (SmallCaps, 8
+  // per wavefront; tier 3, slot = position in rerun_large); the ones over
+  // that, or over any other small tier, get the full-capacity tier (tier 1,
+  // slot = position in its list)
+  const bool mid = R.cfg.small_tier && R.sc.V < SmallCaps::V;
+  const int32_t* full_list = ctx->rerun_large.as<int32_t>();
+  int32_t full_r0 = R.r_large, full_r1 = n_over;
+  if (mid) {
+    HIP_TRY(ctx->mid.need_keep(n_over, R.r_large, SmallCaps::V, SmallCaps::P,
+                               SmallCaps::E, s));
+    HIP_TRY(ctx->rerun_full.need_keep(
+        (size_t)std::max<int64_t>(R.r_full + (n_over - R.r_large), 1) * 4,
+        (size_t)R.r_full * 4, s));
+    if (n_over > R.r_large) {
+      ClipArgs m = a;
+      m.items = ctx->rerun_large.as<int32_t>() + R.r_large;
+      m.n_items = n_over - R.r_large;
+      m.slot_base = R.r_large;
+      m.out = ctx->mid.view(SmallCaps::V, SmallCaps::P, SmallCaps::E);
+      m.rerun_large = ctx->rerun_full.as<int32_t>();
+      m.n_rerun_large = R.cnt + 6;
+      if (m.work) m.work = ctx->work.as<unsigned long long>() + 256;
+      if (int rc = launch_clip(ctx, 0, 8, m, 0)) return rc;
+      hipLaunchKernelGGL(k_mark_rerun, dim3(ceil_div(n_over - R.r_large, 256)),
+                         dim3(256), 0, s, ctx->rerun_large.as<int32_t>(),
+                         R.r_large, n_over, 3, ctx->loc.as<int32_t>());
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(host_cnt, R.cnt, 32, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+    }
+    full_list = ctx->rerun_full.as<int32_t>();
+    full_r0 = R.r_full;
+    full_r1 = host_cnt[6];
+  }
+  HIP_TRY(ctx->large.need_keep(full_r1, full_r0, LargeCaps::V, LargeCaps::P,
                                LargeCaps::E, s));
-  if (n_large > R.r_large) {
-    a.items = ctx->rerun_large.as<int32_t>() + R.r_large;
-    a.n_items = n_large - R.r_large;
-    a.slot_base = R.r_large;
+  if (full_r1 > full_r0) {
+    a.items = full_list + full_r0;
+    a.n_items = full_r1 - full_r0;
+    a.slot_base = full_r0;
     a.out = ctx->large.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
     if (a.work) a.work = ctx->work.as<unsigned long long>() + 256;
     if (int rc = launch_clip(ctx, 1, std::min(R.G, 4), a)) return rc;
@@ -937,16 +974,17 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     if (a.work) a.work = ctx->work.as<unsigned long long>() + 512;
     if (int rc = launch_clip(ctx, 2, 1, a)) return rc;
   }
-  if (n_large > R.r_large)
-    hipLaunchKernelGGL(k_mark_rerun, dim3(ceil_div(n_large - R.r_large, 256)),
-                       dim3(256), 0, s, ctx->rerun_large.as<int32_t>(),
-                       R.r_large, n_large, 1, ctx->loc.as<int32_t>());
+  if (full_r1 > full_r0)
+    hipLaunchKernelGGL(k_mark_rerun, dim3(ceil_div(full_r1 - full_r0, 256)),
+                       dim3(256), 0, s, full_list, full_r0, full_r1, 1,
+                       ctx->loc.as<int32_t>());
   if (n_exact > R.r_exact)
     hipLaunchKernelGGL(k_mark_rerun, dim3(ceil_div(n_exact - R.r_exact, 256)),
                        dim3(256), 0, s, ctx->rerun_exact.as<int32_t>(),
                        R.r_exact, n_exact, 2, ctx->loc.as<int32_t>());
   HIP_TRY(hipGetLastError());
-  R.r_large = n_large;
+  R.r_large = n_over;
+  if (mid) R.r_full = full_r1;
   R.r_exact = n_exact;
   if (events) HIP_TRY(hipEventRecord(ctx->ev[3], s));
   return 0;
@@ -957,6 +995,7 @@ TierViews tier_views(rpd_ctx* ctx, const RunState& R) {
   tv.t[0] = ctx->small.view(R.sc.V, R.sc.P, R.sc.E);
   tv.t[1] = ctx->large.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
   tv.t[2] = ctx->exact.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
+  tv.t[3] = ctx->mid.view(SmallCaps::V, SmallCaps::P, SmallCaps::E);
   return tv;
 }
 

@@ -1356,7 +1356,7 @@ struct Sizes3Sum {
 };
 
 struct TierViews {
-  SlotView t[3];
+  SlotView t[4];  // small, full capacity, exact, middle
 };
 
 // r-th cell of the output order: candidate perm[r] (perm == nullptr:
