@@ -711,7 +711,7 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
   HIP_TRY(hipEventRecord(ctx->ev[11], s));
   HIP_TRY(ctx->tet_count.need(((size_t)nt + 1) * 8));
   HIP_TRY(ctx->tet_off.need(((size_t)nt + 1) * 8));
-  HIP_TRY(ctx->tet_pass.need(((size_t)nt + 1) * 8));
+  HIP_TRY(ctx->tet_pass.need(((size_t)nt + 1) * 2 * 8));  // 2 words per tet
   HIP_TRY(hipMemsetAsync(ctx->tet_count.as<int64_t>() + nt, 0, 8, s));
   hipLaunchKernelGGL(k_tet_count, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
                      R.M, R.S, R.inst_d, R.tet_inst_d,

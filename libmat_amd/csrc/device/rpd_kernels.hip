@@ -547,7 +547,8 @@ __global__ __launch_bounds__(256) void k_tet_count(DevMesh M, DevSites S,
   const InstInfo& I = inst[tet_inst[t]];
   const int64_t b = I.bin_base + tet_bin(c, I.g);
   int64_t n = 0;
-  uint64_t pass = 0;  // bit r: the r-th site of the bin list relates
+  // bit r of (pass0, pass1): the r-th site of the bin list relates
+  uint64_t pass0 = 0, pass1 = 0;
   uint32_t iters = 0;
   const int q0 = bin_start[b], q1 = bin_start[b + 1];
   for (int q = q0; q < q1; ++q) {
@@ -556,10 +557,13 @@ __global__ __launch_bounds__(256) void k_tet_count(DevMesh M, DevSites S,
     // passes 3 of 4 entries and costs more than it saves)
     const bool r = relation(S, L, i, &iters);
     n += r;
-    if (r && q - q0 < 64) pass |= 1ull << (q - q0);
+    const int d = q - q0;
+    if (r && d < 64) pass0 |= 1ull << d;
+    if (r && d >= 64 && d < 128) pass1 |= 1ull << (d - 64);
   }
   tet_count[t] = n;
-  tet_pass[t] = pass;
+  tet_pass[t] = pass0;
+  tet_pass[(int64_t)M.n_tet + t] = pass1;
   if (work) {  // entries, neighbour tests (count_work)
     unsigned long long* w = work + 4 * (blockIdx.x & 63);
     atomicAdd(w + 0, (unsigned long long)(q1 - q0));
@@ -581,14 +585,18 @@ __global__ __launch_bounds__(256) void k_tet_fill(
   const InstInfo& I = inst[tet_inst[t]];
   const int64_t b = I.bin_base + tet_bin(c, I.g);
   int64_t o = tet_off[t];
-  const uint64_t pass = tet_pass[t];
+  const uint64_t pass0 = tet_pass[t];
+  const uint64_t pass1 = tet_pass[(int64_t)M.n_tet + t];
   const int q0 = bin_start[b];
   for (int q = q0; q < bin_start[b + 1]; ++q) {
     const int i = bin_ids[q];
-    // the first 64 outcomes come from the counting pass
+    // the first 128 outcomes come from the counting pass
     uint32_t iters = 0;
-    if (q - q0 < 64 ? !((pass >> (q - q0)) & 1) : !relation(S, L, i, &iters))
-      continue;
+    const int d = q - q0;
+    const bool r = d < 64    ? ((pass0 >> d) & 1)
+                   : d < 128 ? ((pass1 >> (d - 64)) & 1)
+                             : relation(S, L, i, &iters);
+    if (!r) continue;
     Cand c;
     c.tet = t;
     c.seed = i;
