@@ -8,7 +8,10 @@ plain path (candidates from the full tet-sphere relation) and the learned
 path (candidates proposed by the MLP, trained beforehand on the GPU on two
 other seeds of the same generator, then certified, with the missed cells
 re-clipped); ``value`` is the learned path's rate, whose output is checked
-bit-identical to the plain path's.  ``--workload batched`` times 256
+bit-identical to the plain path's.  The same line carries
+``learned_batched``: 256 instances in one batch over 3 sphere-update
+iterations, learned+batched against plain batched and against the
+instances run one at a time.  ``--workload batched`` times 256
 instances in one batch over sphere-update iterations (plain batched, learned
 +batched and one-at-a-time plain).  Prints one JSON line.
 
@@ -54,6 +57,11 @@ def parse():
                     help="time only the plain path")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-iterations", type=int, default=8)
+    ap.add_argument("--no-batched", action="store_true",
+                    help="cube64: skip the embedded learned+batched run")
+    ap.add_argument("--embedded-batch-iterations", type=int, default=3,
+                    help="sphere-update iterations of the learned+batched "
+                         "run embedded in the cube64 line")
     ap.add_argument("--batch-model", default="cube10",
                     choices=["same", "cube10"],
                     help="batched: train the proposal model on other "
@@ -185,7 +193,7 @@ def time_batch(ctx, sets, warmup, **kw):
     return ms, cells, lrn, t_sites
 
 
-def bench_batched(a):
+def bench_batched(a, emit=True):
     """Many independent instances in one resident batch.
 
     ``batched``: 256 instances (tets log-uniform in [2k, 200k] as jittered
@@ -322,7 +330,9 @@ def bench_batched(a):
         "host_prep_s": t_prep, "site_update_h2d_s": t_sites,
         "fallback_pct": learned["fallback_pct"] if learned else 0.0,
     }
-    print(json.dumps(out))
+    if emit:
+        print(json.dumps(out))
+    return out
 
 
 def bench_heldout10(a):
@@ -564,10 +574,28 @@ def main():
                            "one run + download of cells, words, geometry "
                            "and areas; not the metric"},
     }
+    ctx.close()
+    if a.workload == "cube64" and not a.no_learned and not a.no_batched:
+        # the learned+batched comparison on the 256-instance batch (fewer
+        # sphere-update iterations than --workload batched)
+        b = bench_batched(argparse.Namespace(**{
+            **vars(a), "workload": "batched",
+            "batch_iterations": a.embedded_batch_iterations}), emit=False)
+        out["learned_batched"] = {
+            "workload": b["config"]["workload"],
+            "value": b["value"], "unit": "cells/s",
+            "ms_per_iteration": b["ms_per_step"],
+            "plain_batched": b["batched"]["value"],
+            "plain_one_at_a_time": b["plain_per_instance"]["value"],
+            "vs_plain_batched": b["learned_batched_vs_batched"],
+            "vs_plain_one_at_a_time": b["learned_batched_vs_plain"],
+            "fallback_pct": b["fallback_pct"],
+            "identical_to_plain": b["learned"]["identical_to_plain"],
+            "clip_shape": [b["config"]["tuned_cells_per_wave"],
+                           b["config"]["tuned_small_caps"]]}
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(inst, a.cpu_sample_tets, threads)
-    ctx.close()
     print(json.dumps(out))
 
 
