@@ -1,4 +1,5 @@
 """Host-side preparation and the C-ABI surface (CPU only)."""
+import json
 import os
 import re
 import subprocess
@@ -136,3 +137,23 @@ def test_rng_is_deterministic():
     b = prep.spheres(10, 2, 9)
     assert np.array_equal(a, b)
     assert (a[:, :3] > 0).all() and (a[:, :3] < 1).all()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"),
+                        "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
+
+
+def test_gmm_workload_sizes_and_radius_rule():
+    w = prep.gmm_workload(2, 3, tets=(6000, 20000), n_spheres=(500, 1500))
+    assert len(w.instances) == 2
+    for inst, xyzr, n in zip(w.instances, w.spheres, w.grid_n):
+        assert inst.n_tet == 6 * n ** 3 and 500 <= len(xyzr) <= 1500
+        c, r = xyzr[:, :3], xyzr[:, 3]
+        bd = np.minimum(c, 1 - c).min(axis=1)
+        assert (r > 0).all() and (r <= bd + 1e-15).all()
+        assert ((c > 0) & (c < 1)).all()
