@@ -206,8 +206,8 @@ def bench_batched(a, emit=True):
     spheres (32-component Gaussian mixture), ``--steps`` runs of the same
     batch; the model is trained on 16 instances of the same generator with
     another seed.  A step's time is the device time of rpd_run (HIP events);
-    per-iteration neighbour lists are built on the host beforehand and their
-    H2D copy is reported apart.  Three paths on the same inputs: plain
+    per-iteration neighbour lists are built beforehand (device builder,
+    rpd_power_neighbors_gpu) and their H2D copy is reported apart.  Three paths on the same inputs: plain
     batched, learned+batched (checked bit-identical to plain on the last
     step) and the instances one at a time through the plain path."""
     import numpy as np
@@ -223,11 +223,28 @@ def bench_batched(a, emit=True):
         sets = [[(i.sites, i.flags, i.knn_off, i.knn_ids)
                  for i in w.instances]]
         cur = list(w.spheres)
+        # the neighbour lists of each sphere-update iteration come from the
+        # device builder (identical to the host builder's)
+        nb_ctx = Context(0)
+        nb_host = 0
+
+        def sites(x):
+            nonlocal nb_host
+            try:
+                return prep.sites_of(x, ctx=nb_ctx)[:4]
+            except RuntimeError:  # a cell over the device builder's caps
+                nb_host += 1
+                return prep.sites_of(x)[:4]
         for it in range(1, a.batch_iterations):
             cur = [prep.jitter_spheres(x, 0.002, 7919 * it + b)
                    for b, x in enumerate(cur)]
-            sets.append([prep.sites_of(x)[:4] for x in cur])
-            print(f"[batched] host neighbour lists of iteration {it} ready",
+            sets.append([sites(x) for x in cur])
+            print(f"[batched] neighbour lists of iteration {it} ready",
+                  file=sys.stderr, flush=True)
+        nb_ctx.close()
+        if nb_host:
+            print(f"[batched] {nb_host} sphere sets over the device "
+                  f"neighbour builder's capacities: host builder used",
                   file=sys.stderr, flush=True)
         desc = (f"{a.batch_instances} instances, tets log-U[2k,200k] "
                 f"(jittered Kuhn grids), spheres log-U[100,20k] "
@@ -515,6 +532,29 @@ def main():
                           "compaction": lk[6], "certificate": lk[7]},
             "clip_alg_gbs": clip_bytes(lst, inst) / (lk[5] * 1e-3) / 1e9,
         }
+    # the regular-triangulation neighbours of the same sphere set: the host
+    # builder against the device builder (same exact predicates)
+    xyzw = (prep.spheres(50000, 1, 0) if a.workload == "cube64"
+            else prep.spheres(500, 0, 0, 0.02, 0.08))
+    xyzw[:, 3] **= 2
+    t0 = time.perf_counter()
+    v_h, o_h, i_h = prep.power_neighbors(xyzw, prep.UNIT_BOX)
+    t_host = time.perf_counter() - t0
+    neighbours = {"n_spheres": int(xyzw.shape[0]), "host_s": t_host,
+                  "host_threads": os.cpu_count()}
+    try:
+        ctx.power_neighbors(xyzw, prep.UNIT_BOX)  # warm-up
+        t0 = time.perf_counter()
+        v_d, o_d, i_d, nb_ms = ctx.power_neighbors(xyzw, prep.UNIT_BOX)
+        t_dev = time.perf_counter() - t0
+        neighbours.update({
+            "valid": int((v_d > 0).sum()), "pairs": int(i_d.size),
+            "device_clip_ms": nb_ms, "device_total_s": t_dev,
+            "identical": bool(np.array_equal(v_h > 0, v_d > 0)
+                              and np.array_equal(o_h, o_d)
+                              and np.array_equal(i_h, i_d))})
+    except RuntimeError as e:  # a cell over the builder's capacities
+        neighbours["error"] = str(e)
     # host-buffer boundary: upload + one run + download of the cell stream
     t0 = time.perf_counter()
     ctx.upload(inst)
@@ -569,6 +609,7 @@ def main():
                       "plane_masks": kn[4], "clip": kn[5],
                       "compaction": kn[6]},
         "host_prep_s": t_prep, "upload_s": t_upload,
+        "neighbour_builder": neighbours,
         "pcie_inclusive": {"value": n_cells / t_pcie, "unit": "cells/s",
                            "s": t_pcie, "note": "upload of host buffers + "
                            "one run + download of cells, words, geometry "

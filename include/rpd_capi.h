@@ -212,6 +212,22 @@ int rpd_sync(rpd_ctx* ctx);
 
 int rpd_get_stats(const rpd_ctx* ctx, rpd_stats* out);
 
+/* Regular-triangulation neighbours of n spheres (x, y, z, w = r^2) in the
+ * box [x0, y0, z0, x1, y1, z1], computed on the device: replaces the
+ * reference's CGAL regular triangulation and neighbour extraction
+ * (triangulation.cxx:62-144, 396-438) as the host builder
+ * rpd_power_neighbors (rpd_prep.h) does, with the same exact predicates and
+ * the same result (valid flags, symmetric ascending CSR lists; read with
+ * rpd_nbr_get, free with rpd_nbr_free).  ms_out (nullable) receives the
+ * device time of the cell clipping.  Fails (non-zero, rpd_last_error)
+ * when a cell needs more than 186 bisector planes at once, 256 vertices or
+ * 160 neighbours; planes that no longer carry a vertex are dropped as the
+ * cell is clipped, so only the final cell's size counts. */
+typedef struct rpd_nbr rpd_nbr;
+int rpd_power_neighbors_gpu(rpd_ctx* ctx, int32_t n, const double* xyzw,
+                            const double* box, double* ms_out,
+                            rpd_nbr** out);
+
 /* Measurement aid: one plain device copy of `bytes` (multiple of 16) with a
  * streaming kernel, whose HBM traffic (bytes read, bytes written) is known;
  * used to calibrate hardware traffic counters. */

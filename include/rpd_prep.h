@@ -71,6 +71,12 @@ int32_t rpd_nbr_max_degree(const rpd_nbr* h);
 int rpd_nbr_get(const rpd_nbr* h, int32_t* valid, int64_t* offsets,
                 int32_t* ids);
 void rpd_nbr_free(rpd_nbr* h);
+/* A neighbour handle from per-sphere lists (the device builder's output,
+ * rpd_power_neighbors_gpu in rpd_capi.h): valid[n], counts[n], lists[n *
+ * stride]; the result is the symmetric closure over valid spheres, in the
+ * layout of rpd_power_neighbors. */
+int rpd_nbr_from_lists(int32_t n, const int32_t* valid, const int32_t* counts,
+                       const int32_t* lists, int32_t stride, rpd_nbr** out);
 
 #ifdef __cplusplus
 }

@@ -18,8 +18,10 @@
 #include "rpd_capi.h"
 #include "rpd_device.h"
 #include "rpd_prep.h"
+#include "../common/rpd_classgrid.h"
 #include "rpd_kernels.hip"
 #include "rpd_learn.hip"
+#include "rpd_nbr.hip"
 
 namespace {
 
@@ -1480,6 +1482,87 @@ int rpd_calib_copy(rpd_ctx* ctx, int64_t bytes) {
   (void)hipFree(a);
   (void)hipFree(b);
   if (e != hipSuccess) return fail(ctx, "rpd_calib_copy failed", -2);
+  return 0;
+}
+
+int rpd_power_neighbors_gpu(rpd_ctx* ctx, int32_t n, const double* xyzw,
+                            const double* box, double* ms_out,
+                            rpd_nbr** out) {
+  if (!ctx || n < 0 || !out || (n > 0 && !xyzw) || !box) return -1;
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  // weight-class grids on the host (O(n)), flattened: per class the cell
+  // starts (absolute into ids) and the members
+  std::vector<RpdClassGrid> grids;
+  rpd_build_class_grids(n, xyzw, box, grids);
+  NbGrids G{};
+  std::vector<int64_t> starts;
+  std::vector<int32_t> ids;
+  for (int c = 0; c < kRpdClasses; ++c) {
+    const RpdClassGrid& g = grids[c];
+    NbGrid& d = G.g[c];
+    d.used = !g.ids.empty();
+    if (!d.used) continue;
+    for (int a = 0; a < 3; ++a) {
+      d.org[a] = g.org[a];
+      d.dim[a] = g.dim[a];
+    }
+    d.cs = g.cs;
+    d.wmax = g.wmax;
+    d.cell_base = (int64_t)starts.size();
+    const int64_t id_base = (int64_t)ids.size();
+    for (int64_t v : g.start) starts.push_back(id_base + v);
+    ids.insert(ids.end(), g.ids.begin(), g.ids.end());
+  }
+  Buf d_sph, d_box, d_start, d_ids, d_valid, d_cnt, d_lists;
+  const int64_t nn = std::max<int32_t>(n, 1);
+  HIP_TRY(d_sph.need(nn * 32));
+  HIP_TRY(d_box.need(6 * 8));
+  HIP_TRY(d_start.need(std::max<size_t>(starts.size(), 1) * 8));
+  HIP_TRY(d_ids.need(std::max<size_t>(ids.size(), 1) * 4));
+  HIP_TRY(d_valid.need(nn * 4));
+  HIP_TRY(d_cnt.need(nn * 4));
+  HIP_TRY(d_lists.need(nn * kNbOut * 4));
+  HIP_TRY(hipMemcpyAsync(d_sph.p, xyzw, (size_t)n * 32, hipMemcpyHostToDevice,
+                         s));
+  HIP_TRY(hipMemcpyAsync(d_box.p, box, 6 * 8, hipMemcpyHostToDevice, s));
+  if (!starts.empty())
+    HIP_TRY(hipMemcpyAsync(d_start.p, starts.data(), starts.size() * 8,
+                           hipMemcpyHostToDevice, s));
+  if (!ids.empty())
+    HIP_TRY(hipMemcpyAsync(d_ids.p, ids.data(), ids.size() * 4,
+                           hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(ctx->ev[14], s));
+  if (n > 0)
+    hipLaunchKernelGGL(k_power_nbrs, dim3(n), dim3(64), 0, s, n,
+                       d_sph.as<double4>(), d_box.as<double>(), G,
+                       d_start.as<int64_t>(), d_ids.as<int32_t>(),
+                       d_valid.as<int32_t>(), d_cnt.as<int32_t>(),
+                       d_lists.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[15], s));
+  std::vector<int32_t> valid(n), cnt(n), lists((size_t)n * kNbOut);
+  if (n > 0) {
+    HIP_TRY(hipMemcpyAsync(valid.data(), d_valid.p, (size_t)n * 4,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(cnt.data(), d_cnt.p, (size_t)n * 4,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(lists.data(), d_lists.p, (size_t)n * kNbOut * 4,
+                           hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  if (ms_out) {
+    float m = 0;
+    HIP_TRY(hipEventElapsedTime(&m, ctx->ev[14], ctx->ev[15]));
+    *ms_out = m;
+  }
+  for (int32_t i = 0; i < n; ++i)
+    if (valid[i] < 0)
+      return fail(ctx, "rpd_power_neighbors_gpu: sphere " + std::to_string(i) +
+                           " exceeds the cell capacities");
+  if (int rc = rpd_nbr_from_lists(n, valid.data(), cnt.data(), lists.data(),
+                                  kNbOut, out))
+    return fail(ctx, "rpd_power_neighbors_gpu: bad neighbour lists", rc);
   return 0;
 }
 

@@ -10,6 +10,7 @@
 #include <thread>
 #include <vector>
 
+#include "../common/rpd_classgrid.h"
 #include "../common/rpd_predicates.h"
 
 namespace {
@@ -398,15 +399,6 @@ struct Poly {
   }
 };
 
-struct ClassGrid {
-  double wmax = 0;
-  double org[3] = {0, 0, 0};
-  double cs = 1;
-  int32_t dim[3] = {1, 1, 1};
-  std::vector<int64_t> start;  // per grid cell
-  std::vector<int32_t> ids;
-};
-
 }  // namespace
 
 struct rpd_nbr {
@@ -416,6 +408,36 @@ struct rpd_nbr {
   int32_t max_degree = 0;
 };
 
+namespace {
+// CSR of the symmetric closure over valid spheres: j is a neighbour of i
+// iff either list names the other and both are valid; ascending ids
+void symmetrize(int32_t n, const std::vector<std::vector<int32_t>>& nb,
+                rpd_nbr* res) {
+  std::vector<std::vector<int32_t>> sym(n);
+  for (int32_t i = 0; i < n; ++i) {
+    if (!res->valid[i]) continue;
+    for (int32_t j : nb[i]) {
+      if (!res->valid[j]) continue;
+      sym[i].push_back(j);
+      sym[j].push_back(i);
+    }
+  }
+  res->offsets.assign(n + 1, 0);
+  res->max_degree = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    auto& L = sym[i];
+    std::sort(L.begin(), L.end());
+    L.erase(std::unique(L.begin(), L.end()), L.end());
+    res->offsets[i + 1] = res->offsets[i] + (int64_t)L.size();
+    res->max_degree = std::max(res->max_degree, (int32_t)L.size());
+  }
+  res->ids.resize(res->offsets[n]);
+  for (int32_t i = 0; i < n; ++i)
+    std::copy(sym[i].begin(), sym[i].end(),
+              res->ids.begin() + res->offsets[i]);
+}
+}  // namespace
+
 extern "C" int rpd_power_neighbors(int32_t n, const double* xyzw,
                                    const double* box, int32_t n_threads,
                                    rpd_nbr** out) {
@@ -424,67 +446,9 @@ extern "C" int rpd_power_neighbors(int32_t n, const double* xyzw,
     return rpd_d4{xyzw[4 * i], xyzw[4 * i + 1], xyzw[4 * i + 2],
                   xyzw[4 * i + 3]};
   };
-  // weight classes: [wmax/4^(c+1), wmax/4^c), last class catches the rest
-  double wmax = 0;
-  for (int32_t i = 0; i < n; ++i) wmax = std::max(wmax, xyzw[4 * i + 3]);
-  const int kClasses = 8;
-  std::vector<std::vector<int32_t>> members(kClasses);
-  for (int32_t i = 0; i < n; ++i) {
-    double w = xyzw[4 * i + 3];
-    int c = 0;
-    double lim = wmax * 0.25;
-    while (c < kClasses - 1 && !(w >= lim)) {
-      ++c;
-      lim *= 0.25;
-    }
-    if (wmax <= 0) c = kClasses - 1;
-    members[c].push_back(i);
-  }
-  std::vector<ClassGrid> grids(kClasses);
-  for (int c = 0; c < kClasses; ++c) {
-    auto& g = grids[c];
-    auto& m = members[c];
-    if (m.empty()) continue;
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int32_t i : m) {
-      g.wmax = std::max(g.wmax, xyzw[4 * i + 3]);
-      for (int a = 0; a < 3; ++a) {
-        lo[a] = std::min(lo[a], xyzw[4 * i + a]);
-        hi[a] = std::max(hi[a], xyzw[4 * i + a]);
-      }
-    }
-    // cell size: about 3 members per cell, at most 256 cells along the
-    // class's extent, and never below 1/1024 of the box (a class of one or
-    // two spheres has a vanishing extent)
-    const double box_ext = std::max(std::max(box[3] - box[0], box[4] - box[1]),
-                                    box[5] - box[2]);
-    double ext = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]),
-                          std::max(hi[2] - lo[2], 1e-12));
-    double vol = std::max((hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]),
-                          ext * ext * ext * 1e-6);
-    g.cs = std::max(std::max(std::cbrt(vol * 3.0 / m.size()), ext / 256.0),
-                    box_ext / 1024.0);
-    int64_t cells = 1;
-    for (int a = 0; a < 3; ++a) {
-      g.org[a] = lo[a];
-      g.dim[a] = std::max(1, (int32_t)std::ceil((hi[a] - lo[a]) / g.cs) + 1);
-      cells *= g.dim[a];
-    }
-    g.start.assign(cells + 1, 0);
-    auto cell_of = [&](int32_t i) {
-      int64_t q[3];
-      for (int a = 0; a < 3; ++a) {
-        q[a] = (int64_t)std::floor((xyzw[4 * i + a] - g.org[a]) / g.cs);
-        q[a] = std::min<int64_t>(std::max<int64_t>(q[a], 0), g.dim[a] - 1);
-      }
-      return q[0] + g.dim[0] * (q[1] + (int64_t)g.dim[1] * q[2]);
-    };
-    for (int32_t i : m) g.start[cell_of(i) + 1]++;
-    for (int64_t q = 0; q < cells; ++q) g.start[q + 1] += g.start[q];
-    g.ids.resize(m.size());
-    std::vector<int64_t> fill(g.start.begin(), g.start.end() - 1);
-    for (int32_t i : m) g.ids[fill[cell_of(i)]++] = i;
-  }
+  std::vector<RpdClassGrid> grids;
+  rpd_build_class_grids(n, xyzw, box, grids);
+  const int kClasses = kRpdClasses;
 
   auto* res = new rpd_nbr();
   res->valid.assign(n, 0);
@@ -585,28 +549,34 @@ extern "C" int rpd_power_neighbors(int32_t n, const double* xyzw,
     delete res;
     return -4;
   }
-  // symmetrise over valid spheres
-  std::vector<std::vector<int32_t>> sym(n);
+  symmetrize(n, nb, res);
+  *out = res;
+  return 0;
+}
+
+extern "C" int rpd_nbr_from_lists(int32_t n, const int32_t* valid,
+                                  const int32_t* counts, const int32_t* lists,
+                                  int32_t stride, rpd_nbr** out) {
+  if (n < 0 || !out || (n > 0 && (!valid || !counts || !lists)) ||
+      stride < 0)
+    return -1;
+  auto* res = new rpd_nbr();
+  res->valid.assign(valid, valid + n);
+  std::vector<std::vector<int32_t>> nb(n);
   for (int32_t i = 0; i < n; ++i) {
-    if (!res->valid[i]) continue;
-    for (int32_t j : nb[i]) {
-      if (!res->valid[j]) continue;
-      sym[i].push_back(j);
-      sym[j].push_back(i);
+    if (counts[i] < 0 || counts[i] > stride) {
+      delete res;
+      return -1;
     }
+    nb[i].assign(lists + (int64_t)i * stride,
+                 lists + (int64_t)i * stride + counts[i]);
+    for (int32_t j : nb[i])
+      if (j < 0 || j >= n) {
+        delete res;
+        return -1;
+      }
   }
-  res->offsets.assign(n + 1, 0);
-  for (int32_t i = 0; i < n; ++i) {
-    auto& L = sym[i];
-    std::sort(L.begin(), L.end());
-    L.erase(std::unique(L.begin(), L.end()), L.end());
-    res->offsets[i + 1] = res->offsets[i] + (int64_t)L.size();
-    res->max_degree = std::max(res->max_degree, (int32_t)L.size());
-  }
-  res->ids.resize(res->offsets[n]);
-  for (int32_t i = 0; i < n; ++i)
-    std::copy(sym[i].begin(), sym[i].end(),
-              res->ids.begin() + res->offsets[i]);
+  symmetrize(n, nb, res);
   *out = res;
   return 0;
 }

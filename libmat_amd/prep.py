@@ -141,13 +141,18 @@ def build_instance(verts, tets, xyzr, box=UNIT_BOX, n_threads=0):
                     np.ones(keep.size, np.uint32), knn_off, knn_ids, keep)
 
 
-def sites_of(xyzr, box=UNIT_BOX, n_threads=0):
+def sites_of(xyzr, box=UNIT_BOX, n_threads=0, ctx=None):
     """Sites (non-hidden spheres in ascending order, w = r^2), flags and
-    CSR neighbour lists of a sphere set: the per-iteration host step of
-    sphere optimisation.  Returns (sites, flags, knn_off, knn_ids, keep)."""
+    CSR neighbour lists of a sphere set: the per-iteration step of sphere
+    optimisation, with the neighbours from the host builder, or from the
+    device builder of ``ctx`` (an rpd.Context; same result).  Returns
+    (sites, flags, knn_off, knn_ids, keep)."""
     xyzw = np.ascontiguousarray(xyzr, np.float64).copy()
     xyzw[:, 3] = xyzw[:, 3] * xyzw[:, 3]
-    valid, off, ids = power_neighbors(xyzw, box, n_threads)
+    if ctx is not None:
+        valid, off, ids, _ = ctx.power_neighbors(xyzw, box)
+    else:
+        valid, off, ids = power_neighbors(xyzw, box, n_threads)
     keep = np.nonzero(valid)[0].astype(np.int32)
     remap = -np.ones(xyzw.shape[0], np.int64)
     remap[keep] = np.arange(keep.size)

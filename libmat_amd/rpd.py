@@ -136,6 +136,9 @@ def _declare_capi(L):
     L.rpd_download_relation.restype = ctypes.c_int
     L.rpd_calib_copy.argtypes = [vp, ctypes.c_int64]
     L.rpd_calib_copy.restype = ctypes.c_int
+    L.rpd_power_neighbors_gpu.argtypes = [vp, ctypes.c_int32, vp, vp, vp,
+                                          ctypes.POINTER(vp)]
+    L.rpd_power_neighbors_gpu.restype = ctypes.c_int
 
 
 CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
@@ -146,7 +149,8 @@ CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
                 "rpd_labels", "rpd_mlp_train", "rpd_mlp_grad",
                 "rpd_mlp_scores",
                 "rpd_sync", "rpd_get_stats", "rpd_download_cells",
-                "rpd_download_relation", "rpd_calib_copy")
+                "rpd_download_relation", "rpd_calib_copy",
+                "rpd_power_neighbors_gpu")
 
 
 def _p(a):
@@ -377,6 +381,29 @@ class Context:
             _p(geom) if geometry else nul, _p(areas) if geometry else nul),
             "rpd_download_cells")
         return RPDResult(cells, words, geom, areas)
+
+    def power_neighbors(self, xyzw, box):
+        """Regular-triangulation neighbours computed on the device:
+        (valid[n], offsets[n+1], ids, device ms), the layout of
+        prep.power_neighbors."""
+        xyzw = np.ascontiguousarray(xyzw, np.float64)
+        box = np.ascontiguousarray(box, np.float64)
+        n = xyzw.shape[0]
+        h = ctypes.c_void_p()
+        ms = ctypes.c_double(0.0)
+        self._check(self._L.rpd_power_neighbors_gpu(
+            self._h, n, _p(xyzw), _p(box), ctypes.byref(ms),
+            ctypes.byref(h)), "rpd_power_neighbors_gpu")
+        try:
+            total = self._L.rpd_nbr_total(h)
+            valid = np.empty(n, np.int32)
+            off = np.empty(n + 1, np.int64)
+            ids = np.empty(max(total, 1), np.int32)
+            self._check(self._L.rpd_nbr_get(h, _p(valid), _p(off), _p(ids)),
+                        "rpd_nbr_get")
+        finally:
+            self._L.rpd_nbr_free(h)
+        return valid, off, ids[:total], ms.value
 
     def calib_copy(self, nbytes):
         self._check(self._L.rpd_calib_copy(self._h, int(nbytes)),

@@ -288,3 +288,23 @@ def test_tune_picks_a_shape_with_identical_results(ctx):
         print(f"[tune] caps={c}: capacity re-runs {st['n_rerun_large']}",
               flush=True)
         compare_with_oracle(ctx.download(), orc)
+
+
+@pytest.mark.parametrize("case", ["uniform500", "boundary20k", "gmm5k",
+                                  "boundary50k"])
+def test_device_power_neighbours_match_host(ctx, case):
+    # the device builder (rpd_power_neighbors_gpu) against the host builder:
+    # same exact predicates, so the same validity and the same lists
+    n, rule, seed = {"uniform500": (500, 0, 0), "boundary20k": (20000, 1, 3),
+                     "gmm5k": (5000, 2, 4),
+                     "boundary50k": (50000, 1, 0)}[case]
+    xyzr = prep.spheres(n, rule, seed)
+    xyzw = xyzr.copy()
+    xyzw[:, 3] **= 2
+    v_h, off_h, ids_h = prep.power_neighbors(xyzw, prep.UNIT_BOX)
+    v_d, off_d, ids_d, ms = ctx.power_neighbors(xyzw, prep.UNIT_BOX)
+    print(f"[neighbours] {case}: valid {int(v_d.sum())} of {n}, pairs "
+          f"{ids_d.size}, device clip {ms:.2f} ms", flush=True)
+    assert np.array_equal(v_h > 0, v_d > 0)
+    assert np.array_equal(off_h, off_d)
+    assert np.array_equal(ids_h, ids_d)
