@@ -138,6 +138,7 @@ struct rpd_ctx {
   Buf rerun_large, rerun_full, rerun_exact, loc, sizes, offs, drops, stats64,
       work, rel_work;
   Buf scan_tmp;
+  Buf trip_k, trip_k2, trip_v, trip_v2;  // step-count order of a clip range
   TierBufs small, large, exact, mid;
   Buf out_cells, out_words, out_geom, out_areas;
   int64_t n_knn = 0;
@@ -860,6 +861,7 @@ ClipArgs clip_args(rpd_ctx* ctx, RunState& R) {
   a.item_base = 0;
   a.n_items = 0;
   a.slot_base = 0;
+  a.slot_by_cell = 0;
   a.use_mask = R.cfg.use_active_mask;
   a.force_exact = R.cfg.force_exact;
   a.geometry = R.cfg.compute_geometry;
@@ -874,15 +876,42 @@ ClipArgs clip_args(rpd_ctx* ctx, RunState& R) {
 
 // masks, small tier, full-capacity and exact re-runs of candidates
 // [c0, c1); buffers grow keeping earlier rounds; loc of the range is set
+constexpr int64_t kStepOrderMin = 2048;
+
 int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
                bool events) {
   hipStream_t s = R.s;
   const int64_t n = c1 - c0;
   const CapsRT sc = R.sc;
+  // the small tier takes its cells in decreasing order of clip steps (a
+  // stable sort of the step counts), so that the cells of one wavefront
+  // need about as many steps as each other
+  const bool by_steps = R.cfg.use_active_mask && R.cfg.small_tier &&
+                        n >= kStepOrderMin;
+  if (by_steps) {
+    HIP_TRY(ctx->trip_k.need((size_t)n));
+    HIP_TRY(ctx->trip_k2.need((size_t)n));
+    HIP_TRY(ctx->trip_v.need((size_t)n * 4));
+    HIP_TRY(ctx->trip_v2.need((size_t)n * 4));
+  }
   if (R.cfg.use_active_mask && n > 0)
     hipLaunchKernelGGL(k_cand_mask, dim3(ceil_div(n, 256)), dim3(256), 0, s,
-                       R.M, R.S, n, ctx->cand.as<Cand>() + c0);
+                       R.M, R.S, n, ctx->cand.as<Cand>() + c0,
+                       by_steps ? ctx->trip_k.as<uint8_t>() : nullptr,
+                       by_steps ? ctx->trip_v.as<int32_t>() : nullptr, c0);
   HIP_TRY(hipGetLastError());
+  if (by_steps) {
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(
+        nullptr, tmp, ctx->trip_k.as<uint8_t>(), ctx->trip_k2.as<uint8_t>(),
+        ctx->trip_v.as<int32_t>(), ctx->trip_v2.as<int32_t>(), (int)n, 0, 8,
+        s));
+    HIP_TRY(ctx->scan_tmp.need(tmp));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairsDescending(
+        ctx->scan_tmp.p, tmp, ctx->trip_k.as<uint8_t>(),
+        ctx->trip_k2.as<uint8_t>(), ctx->trip_v.as<int32_t>(),
+        ctx->trip_v2.as<int32_t>(), (int)n, 0, 8, s));
+  }
   if (events) HIP_TRY(hipEventRecord(ctx->ev[1], s));
   HIP_TRY(ctx->rerun_large.need_keep((size_t)std::max<int64_t>(c1, 1) * 4,
                                      (size_t)R.r_large * 4, s));
@@ -902,8 +931,14 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     a.item_base = c0;
     a.slot_base = c0;
     a.n_items = n;
+    if (by_steps) {  // slot = cell index, as with the identity order
+      a.items = ctx->trip_v2.as<int32_t>();
+      a.slot_by_cell = 1;
+    }
     a.out = ctx->small.view(sc.V, sc.P, sc.E);
     if (int rc = launch_clip(ctx, 0, R.G, a, R.cfg.small_caps)) return rc;
+    a.items = nullptr;  // the re-run tiers index their own lists
+    a.slot_by_cell = 0;
   } else if (n > 0) {
     // every cell goes straight to the full-capacity tier
     std::vector<int32_t> idx(n);

@@ -612,9 +612,15 @@ __global__ __launch_bounds__(256) void k_tet_fill(
 // neighbour's bisector is certainly positive at all four corners of the
 // tet cell (then it cannot cut any state of the cell: every cell vertex
 // lies in the convex hull of the corners).
+// With trips != nullptr it also writes each cell's number of clip steps
+// (mask bits below the degree, plus the positions past 128) and its index
+// c0 + c for the step-count ordering of the clip launch.
 __global__ __launch_bounds__(256) void k_cand_mask(DevMesh M, DevSites S,
                                                    int64_t n_cand,
-                                                   Cand* cand) {
+                                                   Cand* cand,
+                                                   uint8_t* trips,
+                                                   int32_t* trip_ids,
+                                                   int64_t c0) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_cand) return;
   const int t = cand[c].tet, i = cand[c].seed;
@@ -648,6 +654,11 @@ __global__ __launch_bounds__(256) void k_cand_mask(DevMesh M, DevSites S,
   }
   cand[c].m0 = m0;
   cand[c].m1 = m1;
+  if (trips) {
+    const int n = __popcll(m0) + __popcll(m1) + (deg > 128 ? deg - 128 : 0);
+    trips[c] = (uint8_t)(n < 255 ? n : 255);
+    trip_ids[c] = (int32_t)(c0 + c);
+  }
 }
 
 // ============================================================ clipping
@@ -693,8 +704,11 @@ struct ClipArgs {
                           // item)
   int64_t item_base;
   int64_t n_items;
-  SlotView out;           // indexed by slot_base + item
+  SlotView out;           // indexed by slot_base + item, or by the cell
+                          // index when slot_by_cell
   int64_t slot_base;
+  int32_t slot_by_cell;   // items in step-count order: slot = cell index,
+                          // and no XCD regrouping of the blocks
   int32_t use_mask;
   int32_t force_exact;
   int32_t geometry;
@@ -813,7 +827,7 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
   // round-robin dispatch of workgroups)
   const int64_t nb = gridDim.x;
   int64_t blk = blockIdx.x;
-  if (nb >= 8) {
+  if (nb >= 8 && !a.slot_by_cell) {
     const int64_t per = nb / 8;
     const int64_t xcd = blk % 8, pos = blk / 8;
     if (pos < per) blk = xcd * per + pos;
@@ -821,7 +835,7 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
   const int64_t item = blk * G + g;
   if (item >= a.n_items) return;
   const int64_t cidx = a.items ? a.items[item] : a.item_base + item;
-  const int64_t slot = a.slot_base + item;
+  const int64_t slot = a.slot_by_cell ? cidx : a.slot_base + item;
   const Cand cd = a.cand[cidx];
   const int t = cd.tet, s = cd.seed;
   const int4 tv = a.M.tet_v[t];
