@@ -1302,28 +1302,29 @@ int compact(rpd_ctx* ctx, RunState& R, int64_t n, const int32_t* perm,
             Sizes3* tot) {
   hipStream_t s = R.s;
   TierViews tv = tier_views(ctx, R);
-  HIP_TRY(ctx->sizes.need(((size_t)n + 1) * sizeof(Sizes3)));
-  HIP_TRY(ctx->offs.need(((size_t)n + 1) * sizeof(Sizes3)));
+  const int64_t nb = std::max<int64_t>(ceil_div(n, kCB), 0);
+  HIP_TRY(ctx->sizes.need(((size_t)nb + 1) * sizeof(Sizes3)));
+  HIP_TRY(ctx->offs.need(((size_t)nb + 1) * sizeof(Sizes3)));
   HIP_TRY(ctx->drops.need(9 * 8));
   HIP_TRY(hipMemsetAsync(ctx->drops.p, 0, 9 * 8, s));
-  HIP_TRY(hipMemsetAsync(ctx->sizes.as<Sizes3>() + n, 0, sizeof(Sizes3), s));
-  if (n > 0)
-    hipLaunchKernelGGL(k_sizes, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
-                       perm, ctx->loc.as<int32_t>(), tv,
-                       ctx->sizes.as<Sizes3>(), ctx->drops.as<int64_t>());
+  HIP_TRY(hipMemsetAsync(ctx->sizes.as<Sizes3>() + nb, 0, sizeof(Sizes3), s));
+  if (nb > 0)
+    hipLaunchKernelGGL(k_block_sizes, dim3(nb), dim3(kCB), 0, s, n, perm,
+                       ctx->loc.as<int32_t>(), tv, ctx->sizes.as<Sizes3>(),
+                       ctx->drops.as<int64_t>());
   HIP_TRY(hipGetLastError());
   {
     size_t tmp = 0;
     Sizes3 zero{0, 0, 0};
     HIP_TRY(hipcub::DeviceScan::ExclusiveScan(
         nullptr, tmp, ctx->sizes.as<Sizes3>(), ctx->offs.as<Sizes3>(),
-        Sizes3Sum(), zero, (int)(n + 1), s));
+        Sizes3Sum(), zero, (int)(nb + 1), s));
     HIP_TRY(ctx->scan_tmp.need(tmp));
     HIP_TRY(hipcub::DeviceScan::ExclusiveScan(
         ctx->scan_tmp.p, tmp, ctx->sizes.as<Sizes3>(), ctx->offs.as<Sizes3>(),
-        Sizes3Sum(), zero, (int)(n + 1), s));
+        Sizes3Sum(), zero, (int)(nb + 1), s));
   }
-  HIP_TRY(hipMemcpyAsync(tot, ctx->offs.as<Sizes3>() + n, sizeof(Sizes3),
+  HIP_TRY(hipMemcpyAsync(tot, ctx->offs.as<Sizes3>() + nb, sizeof(Sizes3),
                          hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   HIP_TRY(ctx->out_cells.need((size_t)std::max<int64_t>(tot->cells, 1) *
@@ -1331,11 +1332,10 @@ int compact(rpd_ctx* ctx, RunState& R, int64_t n, const int32_t* perm,
   HIP_TRY(ctx->out_words.need((size_t)std::max<int64_t>(tot->words, 1) * 4));
   HIP_TRY(ctx->out_geom.need((size_t)std::max<int64_t>(tot->cells, 1) * 32));
   HIP_TRY(ctx->out_areas.need((size_t)std::max<int64_t>(tot->areas, 1) * 8));
-  if (n > 0)
-    hipLaunchKernelGGL(k_compact, dim3(ceil_div(n * kCompactLanes, 256)),
-                       dim3(256), 0, s, n, perm, ctx->cand.as<Cand>(),
-                       ctx->loc.as<int32_t>(), tv, ctx->offs.as<Sizes3>(),
-                       ctx->out_cells.as<rpd_cell>(),
+  if (nb > 0)
+    hipLaunchKernelGGL(k_compact, dim3(nb), dim3(kCB), 0, s, n, perm,
+                       ctx->cand.as<Cand>(), ctx->loc.as<int32_t>(), tv,
+                       ctx->offs.as<Sizes3>(), ctx->out_cells.as<rpd_cell>(),
                        ctx->out_words.as<uint32_t>(),
                        ctx->out_geom.as<double>(), ctx->out_areas.as<double>(),
                        R.cfg.compute_geometry);

@@ -1381,80 +1381,156 @@ struct TierViews {
   SlotView t[4];  // small, full capacity, exact, middle
 };
 
-// r-th cell of the output order: candidate perm[r] (perm == nullptr:
-// candidate r)
-__global__ __launch_bounds__(256) void k_sizes(int64_t n_cells,
-                                               const int32_t* perm,
-                                               const int32_t* loc,
-                                               TierViews tv, Sizes3* sz,
-                                               int64_t* drops /*9*/) {
+// Compaction in blocks of kCB consecutive output positions r (cell
+// perm[r], or r when perm is null).  A cell's sizes pack into one u64
+// (cells in bits 0-15, areas 16-35, words 36-63), so one integer sum
+// scans all three: a block holds at most 256 cells, 256 x 64 areas and
+// 256 x 312 words.
+constexpr int kCB = 256;
+
+struct CellRef {
+  int tier;
+  int64_t idx;
+  uint32_t hdr;
+};
+
+__device__ __forceinline__ CellRef cell_ref(int64_t r, const int32_t* perm,
+                                            const int32_t* loc,
+                                            const TierViews& tv) {
+  const int64_t c = perm ? perm[r] : r;
+  const int32_t l = loc[c];
+  CellRef x;
+  x.tier = l >> 28;
+  x.idx = x.tier ? (l & ((1 << 28) - 1)) : c;
+  x.hdr = tv.t[x.tier].hdr[x.idx];
+  return x;
+}
+
+__device__ __forceinline__ uint64_t packed_size(uint32_t hd) {
+  if ((hd >> 24) != RPD_SUCCESS) return 0;
+  const uint64_t nv = hd & 255, np = (hd >> 8) & 255, ne = (hd >> 16) & 255;
+  return 1ull | (np << 16) | ((nv + np + ne) << 36);
+}
+
+// per block: totals of its cells (bsum[b]) and the dropped cells by status
+__global__ __launch_bounds__(kCB) void k_block_sizes(int64_t n_cells,
+                                                     const int32_t* perm,
+                                                     const int32_t* loc,
+                                                     TierViews tv,
+                                                     Sizes3* bsum,
+                                                     int64_t* drops /*9*/) {
+  typedef hipcub::BlockReduce<uint64_t, kCB> Red;
+  __shared__ typename Red::TempStorage tmp;
   __shared__ int32_t h[9];
   if (threadIdx.x < 9) h[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * kCB + threadIdx.x;
+  uint64_t z = 0;
   if (r < n_cells) {
-    const int64_t c = perm ? perm[r] : r;
-    const int32_t l = loc[c];
-    const int tier = l >> 28;
-    const int64_t idx = tier ? (l & ((1 << 28) - 1)) : c;
-    const uint32_t hd = tv.t[tier].hdr[idx];
-    const uint32_t st = hd >> 24;
-    Sizes3 z{0, 0, 0};
-    if (st == RPD_SUCCESS) {
-      const int64_t nv = hd & 255, np = (hd >> 8) & 255, ne = (hd >> 16) & 255;
-      z = Sizes3{1, nv + np + ne, np};
-    } else if (st < 9) {
-      atomicAdd(&h[st], 1);
-    }
-    sz[r] = z;
+    const CellRef x = cell_ref(r, perm, loc, tv);
+    const uint32_t st = x.hdr >> 24;
+    z = packed_size(x.hdr);
+    if (st != RPD_SUCCESS && st < 9) atomicAdd(&h[st], 1);
   }
+  const uint64_t tot = Red(tmp).Sum(z);
+  if (threadIdx.x == 0)
+    bsum[blockIdx.x] = Sizes3{(int64_t)(tot & 0xffff),
+                              (int64_t)(tot >> 36),
+                              (int64_t)((tot >> 16) & 0xfffff)};
   __syncthreads();
   if (threadIdx.x < 9 && h[threadIdx.x])
     atomicAdd((unsigned long long*)&drops[threadIdx.x],
               (unsigned long long)h[threadIdx.x]);
 }
 
-// Eight lanes per cell, eight cells per wavefront: each group copies its
-// cell's payload words and areas from the slot into the contiguous output.
-constexpr int kCompactLanes = 8;
-__global__ __launch_bounds__(256) void k_compact(
+// block b writes its cells from boff[b] on: cell records and geometry one
+// per thread, payload words and areas by the whole block over the block's
+// contiguous output range (each output element finds its cell by binary
+// search in the block's offsets), so every store is coalesced.
+__global__ __launch_bounds__(kCB) void k_compact(
     int64_t n_cells, const int32_t* perm, const Cand* cand,
-    const int32_t* loc, TierViews tv, const Sizes3* off, rpd_cell* out_cells,
+    const int32_t* loc, TierViews tv, const Sizes3* boff, rpd_cell* out_cells,
     uint32_t* out_words, double* out_geom, double* out_areas,
     int32_t geometry) {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t r = tid / kCompactLanes;
-  const int j = (int)(tid % kCompactLanes);
-  if (r >= n_cells) return;
-  const int64_t c = perm ? perm[r] : r;
-  const int32_t l = loc[c];
-  const int tier = l >> 28;
-  const int64_t idx = tier ? (l & ((1 << 28) - 1)) : c;
-  const SlotView& sv = tv.t[tier];
-  const uint32_t hd = sv.hdr[idx];
-  if ((hd >> 24) != RPD_SUCCESS) return;
-  const Sizes3 o = off[r];
-  const int nv = hd & 255, np = (hd >> 8) & 255, ne = (hd >> 16) & 255;
-  if (j == 0) {
+  typedef hipcub::BlockScan<uint64_t, kCB> Scan;
+  __shared__ typename Scan::TempStorage tmp;
+  __shared__ uint32_t s_w[kCB + 1];
+  __shared__ uint32_t s_a[kCB + 1];
+  __shared__ const uint32_t* s_wsrc[kCB];
+  __shared__ const double* s_asrc[kCB];
+  const int i = threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * kCB + i;
+  CellRef x{0, 0, 0};
+  int64_t c = 0;
+  uint64_t z = 0;
+  if (r < n_cells) {
+    c = perm ? perm[r] : r;
+    x = cell_ref(r, perm, loc, tv);
+    z = packed_size(x.hdr);
+  }
+  uint64_t off, tot;
+  Scan(tmp).ExclusiveSum(z, off, tot);
+  const Sizes3 B = boff[blockIdx.x];
+  const uint32_t w_off = (uint32_t)(off >> 36);
+  const uint32_t a_off = (uint32_t)((off >> 16) & 0xfffff);
+  const int64_t c_off = B.cells + (int64_t)(off & 0xffff);
+  s_w[i] = w_off;
+  s_a[i] = a_off;
+  if (i == 0) {
+    s_w[kCB] = (uint32_t)(tot >> 36);
+    s_a[kCB] = (uint32_t)((tot >> 16) & 0xfffff);
+  }
+  const SlotView& sv = tv.t[x.tier];
+  s_wsrc[i] = sv.words + x.idx * sv.stride_words;
+  s_asrc[i] = sv.areas + x.idx * sv.stride_areas;
+  if (z) {
     rpd_cell rc;
     rc.tet_id = cand[c].tet;
     rc.seed = cand[c].seed;
-    rc.nb_v = (uint8_t)nv;
-    rc.nb_p = (uint8_t)np;
-    rc.nb_e = (uint8_t)ne;
+    rc.nb_v = (uint8_t)(x.hdr & 255);
+    rc.nb_p = (uint8_t)((x.hdr >> 8) & 255);
+    rc.nb_e = (uint8_t)((x.hdr >> 16) & 255);
     rc.status = RPD_SUCCESS;
     rc.reserved = 0;
-    rc.word_off = (uint64_t)o.words;
-    rc.area_off = (uint64_t)o.areas;
-    out_cells[o.cells] = rc;
+    rc.word_off = (uint64_t)(B.words + w_off);
+    rc.area_off = (uint64_t)(B.areas + a_off);
+    out_cells[c_off] = rc;
+    if (geometry) {
+      const double* gs = sv.geom + 4 * x.idx;
+      double* gd = out_geom + 4 * c_off;
+      gd[0] = gs[0];
+      gd[1] = gs[1];
+      gd[2] = gs[2];
+      gd[3] = gs[3];
+    }
   }
-  const uint32_t* src = sv.words + idx * sv.stride_words;
-  for (int q = j; q < nv + np + ne; q += kCompactLanes)
-    out_words[o.words + q] = src[q];
+  __syncthreads();
+  // the cell holding element e: the last k with s[k] <= e (cells without
+  // output have s[k] == s[k + 1] and are skipped by "last")
+  auto owner = [](const uint32_t* sa, uint32_t e) {
+    int lo = 0, hi = kCB;  // sa[lo] <= e < sa[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (sa[mid] <= e)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    return lo;
+  };
+  const uint32_t W = s_w[kCB];
+  uint32_t* wd = out_words + B.words;
+  for (uint32_t e = i; e < W; e += kCB) {
+    const int k = owner(s_w, e);
+    wd[e] = s_wsrc[k][e - s_w[k]];
+  }
   if (geometry) {
-    if (j < 4) out_geom[4 * o.cells + j] = sv.geom[4 * idx + j];
-    const double* as = sv.areas + idx * sv.stride_areas;
-    for (int q = j; q < np; q += kCompactLanes) out_areas[o.areas + q] = as[q];
+    const uint32_t A = s_a[kCB];
+    double* ad = out_areas + B.areas;
+    for (uint32_t e = i; e < A; e += kCB) {
+      const int k = owner(s_a, e);
+      ad[e] = s_asrc[k][e - s_a[k]];
+    }
   }
 }
 
