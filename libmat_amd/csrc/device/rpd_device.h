@@ -43,6 +43,8 @@ struct DevSites {
                             // the relation, list order for the learned
                             // path), bit 31 set for unflagged sites
   const double4* knn_site_near;  // the spheres of knn_near, inline
+  const double4* knn_bis;  // power bisector of each knn_ids entry (zero
+                           // for a -1 entry), in knn_ids order
 };
 
 // capacities of a clip tier

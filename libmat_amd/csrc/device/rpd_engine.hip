@@ -130,6 +130,7 @@ struct rpd_ctx {
   bool has_mesh = false, has_sites = false;
   Buf vert, tet_v, tet_fid, tet_meta, tet_inst, vert_inst, inst_dev;
   Buf site, flags, knn_off, knn_ids, knn_near, knn_site_near, site_inst;
+  Buf knn_bis;
   // pipeline
   Buf bbox, reg_n, reg_off, keys_a, keys_b, bin_start, bin_ids;
   Buf tet_count, tet_off, tet_pass, cand, rel_ids;
@@ -589,9 +590,17 @@ int init_state(rpd_ctx* ctx, RunState& R) {
   R.S = DevSites{R.ns, ctx->site.as<double4>(), ctx->flags.as<uint32_t>(),
                  ctx->knn_off.as<int64_t>(), ctx->knn_ids.as<int32_t>(),
                  ctx->knn_near.as<int32_t>(),
-                 ctx->knn_site_near.as<double4>()};
+                 ctx->knn_site_near.as<double4>(), nullptr};
   HIP_TRY(ctx->counters.need(8 * 4));
   R.cnt = ctx->counters.as<int32_t>();
+  // the bisector of every neighbour-list entry, read by the plane masks
+  // and the clip instead of two dependent loads and the arithmetic
+  HIP_TRY(ctx->knn_bis.need((size_t)std::max<int64_t>(ctx->n_knn, 1) * 32));
+  R.S.knn_bis = ctx->knn_bis.as<double4>();
+  if (R.ns > 0)
+    hipLaunchKernelGGL(k_knn_bis, dim3(R.ns), dim3(64), 0, R.s, R.S,
+                       ctx->knn_bis.as<double4>());
+  HIP_TRY(hipGetLastError());
   return upload_inst_info(ctx, R);
 }
 
@@ -936,6 +945,9 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
       a.slot_by_cell = 1;
     }
     a.out = ctx->small.view(sc.V, sc.P, sc.E);
+    // the tier's shape is the same for every launch of a run: the FP64
+    // geometry sums over the lanes of a cell, so its rounding depends on
+    // the cells per wavefront
     if (int rc = launch_clip(ctx, 0, R.G, a, R.cfg.small_caps)) return rc;
     a.items = nullptr;  // the re-run tiers index their own lists
     a.slot_by_cell = 0;

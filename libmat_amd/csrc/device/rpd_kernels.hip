@@ -608,6 +608,22 @@ __global__ __launch_bounds__(256) void k_tet_fill(
   }
 }
 
+// the bisector table of the neighbour lists: one wavefront per site
+__global__ __launch_bounds__(64) void k_knn_bis(DevSites S, double4* bis) {
+  const int i = blockIdx.x;
+  const int64_t k0 = S.knn_off[i], k1 = S.knn_off[i + 1];
+  const rpd_d4 si = ld4(S.site + i);
+  for (int64_t q = k0 + threadIdx.x; q < k1; q += 64) {
+    const int m = S.knn_ids[q];
+    double4 e = make_double4(0, 0, 0, 0);
+    if (m >= 0) {
+      const rpd_d4 b = rpd_bisector(si, ld4(S.site + m));
+      e = make_double4(b.x, b.y, b.z, b.w);
+    }
+    bis[q] = e;
+  }
+}
+
 // One thread per candidate cell: bit k of the mask is set unless the k-th
 // neighbour's bisector is certainly positive at all four corners of the
 // tet cell (then it cannot cut any state of the cell: every cell vertex
@@ -634,13 +650,12 @@ __global__ __launch_bounds__(256) void k_cand_mask(DevMesh M, DevSites S,
   rpd_vertex_homog(pl[0], pl[2], pl[3], X1, Q1);
   rpd_vertex_homog(pl[0], pl[3], pl[1], X2, Q2);
   rpd_vertex_homog(pl[0], pl[1], pl[2], X3, Q3);
-  const rpd_d4 si = ld4(S.site + i);
   const int64_t k0 = S.knn_off[i];
   const int deg = (int)(S.knn_off[i + 1] - k0);
   uint64_t m0 = 0, m1 = 0;
   const int lim = deg < 128 ? deg : 128;
   for (int k = 0; k < lim; ++k) {
-    const rpd_d4 e = rpd_bisector(si, ld4(S.site + S.knn_ids[k0 + k]));
+    const rpd_d4 e = ld4(S.knn_bis + k0 + k);
     const bool inactive = rpd_filtered_sign4(e, X0, Q0) < 0 &&
                           rpd_filtered_sign4(e, X1, Q1) < 0 &&
                           rpd_filtered_sign4(e, X2, Q2) < 0 &&
@@ -840,7 +855,6 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
   const int t = cd.tet, s = cd.seed;
   const int4 tv = a.M.tet_v[t];
   const TetMeta tm = a.M.tet_meta[t];
-  const rpd_d4 S0 = ld4(a.S.site + s);
   const int64_t k0 = a.S.knn_off[s];
   const int deg = (int)(a.S.knn_off[s + 1] - k0);
   uint64_t m0 = a.use_mask ? cd.m0 : ~0ull;
@@ -895,8 +909,8 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
       break;
     }
     const int m = a.S.knn_ids[k0 + k];
+    const rpd_d4 e = ld4(a.S.knn_bis + k0 + k);
     if (m == -1) break;
-    const rpd_d4 e = rpd_bisector(S0, ld4(a.S.site + m));
     const int cur = nb_p;
     n_bis += 1;
     n_det += nb_v;
