@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -162,6 +163,10 @@ struct rpd_ctx {
   int tuned_G = 0, tuned_caps = 0;  // rpd_tune's choice (0: not tuned)
   hipEvent_t ev[16];
   bool events = false;
+  // host waits: an event polled by wait_stream and a pinned block that
+  // receives the counts read back between pipeline phases
+  hipEvent_t ev_sync = nullptr;
+  char* pin = nullptr;
 };
 
 namespace {
@@ -180,6 +185,46 @@ int fail(rpd_ctx* c, const std::string& m, int code = -1) {
   } while (0)
 
 int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// Waits until the stream has drained by polling an event.  The pipeline
+// reads counts back between its phases to size the next launches; a
+// blocking wait adds its wake-up latency to each of these round trips,
+// polling returns as soon as the event has completed.
+hipError_t wait_stream(rpd_ctx* ctx, hipStream_t s) {
+  hipError_t e = hipEventRecord(ctx->ev_sync, s);
+  if (e != hipSuccess) return e;
+  while ((e = hipEventQuery(ctx->ev_sync)) == hipErrorNotReady) {
+  }
+  return e;
+}
+
+// Small device-to-host reads (counts, totals): copied into the context's
+// pinned block, the stream waited on, then handed to the host variables.
+constexpr size_t kPinBytes = 4096;
+struct D2H {
+  void* dst;
+  const void* src;
+  size_t n;
+};
+hipError_t read_back(rpd_ctx* ctx, hipStream_t s,
+                     std::initializer_list<D2H> reads) {
+  size_t off = 0;
+  for (const D2H& r : reads) {
+    if (off + r.n > kPinBytes) return hipErrorInvalidValue;
+    hipError_t e = hipMemcpyAsync(ctx->pin + off, r.src, r.n,
+                                  hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    off += (r.n + 15) & ~(size_t)15;
+  }
+  hipError_t e = wait_stream(ctx, s);
+  if (e != hipSuccess) return e;
+  off = 0;
+  for (const D2H& r : reads) {
+    std::memcpy(r.dst, ctx->pin + off, r.n);
+    off += (r.n + 15) & ~(size_t)15;
+  }
+  return hipSuccess;
+}
 
 template <class T>
 hipError_t exclusive_sum(rpd_ctx* ctx, const T* in, T* out, int64_t n) {
@@ -216,6 +261,13 @@ int rpd_ctx_create(int32_t device, rpd_ctx** out) {
       return -3;
     }
   ctx->events = true;
+  if (hipEventCreateWithFlags(&ctx->ev_sync, hipEventDisableTiming) !=
+          hipSuccess ||
+      hipHostMalloc((void**)&ctx->pin, kPinBytes, hipHostMallocDefault) !=
+          hipSuccess) {
+    rpd_ctx_destroy(ctx);
+    return -3;
+  }
   std::memset(&ctx->st, 0, sizeof(ctx->st));
   *out = ctx;
   return 0;
@@ -227,6 +279,8 @@ void rpd_ctx_destroy(rpd_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->events)
     for (int i = 0; i < 16; ++i) (void)hipEventDestroy(ctx->ev[i]);
+  if (ctx->ev_sync) (void)hipEventDestroy(ctx->ev_sync);
+  if (ctx->pin) (void)hipHostFree(ctx->pin);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -317,7 +371,7 @@ int rpd_update_sites_batch(rpd_ctx* ctx, int32_t n_inst,
                              hipMemcpyHostToDevice, ctx->stream));
   }
   ctx->st.site_k = max_deg;
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(wait_stream(ctx, ctx->stream));
   ctx->has_sites = true;
   return 0;
 }
@@ -441,7 +495,7 @@ int rpd_upload_batch(rpd_ctx* ctx, int32_t n_inst,
                          hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(hipMemcpyAsync(ctx->vert_inst.p, vinst.data(), (size_t)nv_tot * 4,
                          hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(wait_stream(ctx, ctx->stream));
   ctx->has_mesh = true;
   bool all_sites = true;
   for (int32_t b = 0; b < n_inst; ++b)
@@ -647,7 +701,7 @@ int upload_inst_info(rpd_ctx* ctx, RunState& R) {
   HIP_TRY(hipMemcpyAsync(ctx->inst_dev.p, ii.data(),
                          ii.size() * sizeof(InstInfo), hipMemcpyHostToDevice,
                          R.s));
-  HIP_TRY(hipStreamSynchronize(R.s));  // the host vector dies here
+  HIP_TRY(wait_stream(ctx, R.s));  // the host vector dies here
   R.n_bins = n_bins;
   R.n_sg = n_sg;
   R.inst_d = ctx->inst_dev.as<InstInfo>();
@@ -694,9 +748,7 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
   HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->reg_n.as<int64_t>(),
                                  ctx->reg_off.as<int64_t>(), ns + 1));
   int64_t n_reg = 0;
-  HIP_TRY(hipMemcpyAsync(&n_reg, ctx->reg_off.as<int64_t>() + ns, 8,
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(read_back(ctx, s, {{&n_reg, ctx->reg_off.as<int64_t>() + ns, 8}}));
   if (n_reg >= (1LL << 31))
     return fail(ctx, "rpd_run: too many sphere-bin registrations");
   ctx->st.n_bin_registrations = n_reg;
@@ -737,9 +789,7 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
   HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
                                  ctx->tet_off.as<int64_t>(), nt + 1));
   int64_t n_cand = 0;
-  HIP_TRY(hipMemcpyAsync(&n_cand, ctx->tet_off.as<int64_t>() + nt, 8,
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(read_back(ctx, s, {{&n_cand, ctx->tet_off.as<int64_t>() + nt, 8}}));
   HIP_TRY(ctx->cand.need((size_t)std::max<int64_t>(n_cand, 1) * sizeof(Cand)));
   HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
   hipLaunchKernelGGL(k_tet_fill, dim3(ceil_div(nt, 256)), dim3(256), 0, s, R.M,
@@ -846,9 +896,7 @@ int learned_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
   HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
                                  ctx->tet_off.as<int64_t>(), nt + 1));
   int64_t n_cand = 0;
-  HIP_TRY(hipMemcpyAsync(&n_cand, ctx->tet_off.as<int64_t>() + nt, 8,
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(read_back(ctx, s, {{&n_cand, ctx->tet_off.as<int64_t>() + nt, 8}}));
   HIP_TRY(ctx->cand.need((size_t)std::max<int64_t>(n_cand, 1) * sizeof(Cand)));
   HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
   hipLaunchKernelGGL(k_learn_fill, dim3(ceil_div(nt, 64)), dim3(64), 0, s, nt,
@@ -959,12 +1007,11 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
                            idx.data(), n * 4, hipMemcpyHostToDevice, s));
     int32_t nf = (int32_t)(R.r_large + n);
     HIP_TRY(hipMemcpyAsync(R.cnt + 1, &nf, 4, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(wait_stream(ctx, s));
   }
   if (events) HIP_TRY(hipEventRecord(ctx->ev[2], s));
   int32_t host_cnt[8];
-  HIP_TRY(hipMemcpyAsync(host_cnt, R.cnt, 32, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(read_back(ctx, s, {{host_cnt, R.cnt, 32}}));
   const int32_t n_over = host_cnt[1];
   // cells over a tight small tier first get the middle tier (SmallCaps, 8
   // per wavefront; tier 3, slot = position in rerun_large); the ones over
@@ -993,8 +1040,7 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
                          dim3(256), 0, s, ctx->rerun_large.as<int32_t>(),
                          R.r_large, n_over, 3, ctx->loc.as<int32_t>());
       HIP_TRY(hipGetLastError());
-      HIP_TRY(hipMemcpyAsync(host_cnt, R.cnt, 32, hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipStreamSynchronize(s));
+      HIP_TRY(read_back(ctx, s, {{host_cnt, R.cnt, 32}}));
     }
     full_list = ctx->rerun_full.as<int32_t>();
     full_r0 = R.r_full;
@@ -1009,8 +1055,7 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     a.out = ctx->large.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
     if (a.work) a.work = ctx->work.as<unsigned long long>() + 256;
     if (int rc = launch_clip(ctx, 1, std::min(R.G, 4), a)) return rc;
-    HIP_TRY(hipMemcpyAsync(host_cnt, R.cnt, 32, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(read_back(ctx, s, {{host_cnt, R.cnt, 32}}));
   }
   const int32_t n_exact = host_cnt[2];
   HIP_TRY(ctx->exact.need_keep(n_exact, R.r_exact, LargeCaps::V, LargeCaps::P,
@@ -1112,8 +1157,7 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
   int64_t c_lo = 0, c_hi = n1, added = 0;
   for (int round = 0;; ++round) {
     int32_t nm = 0;
-    HIP_TRY(hipMemcpyAsync(&nm, R.cnt + 4, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(read_back(ctx, s, {{&nm, R.cnt + 4, 4}}));
     const int64_t n_miss = std::min<int64_t>(nm, R.cert.cap);
     if (n_miss == 0) break;
     if (round == kClosureRounds) {  // still open: full relation
@@ -1135,9 +1179,7 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
     HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->umark.as<int32_t>(),
                                    ctx->upos.as<int32_t>(), n_miss + 1));
     int32_t n_u = 0;
-    HIP_TRY(hipMemcpyAsync(&n_u, ctx->upos.as<int32_t>() + n_miss, 4,
-                           hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(read_back(ctx, s, {{&n_u, ctx->upos.as<int32_t>() + n_miss, 4}}));
     HIP_TRY(ctx->ukeys.need((size_t)std::max(n_u, 1) * 8));
     HIP_TRY(ctx->upass.need((size_t)n_u + 1));
     HIP_TRY(ctx->ppos.need(((size_t)n_u + 1) * 4));
@@ -1153,9 +1195,7 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
     HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->umark.as<int32_t>(),
                                    ctx->ppos.as<int32_t>(), n_u + 1));
     int32_t n_new = 0;
-    HIP_TRY(hipMemcpyAsync(&n_new, ctx->ppos.as<int32_t>() + n_u, 4,
-                           hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(read_back(ctx, s, {{&n_new, ctx->ppos.as<int32_t>() + n_u, 4}}));
     HIP_TRY(ctx->cand.need_keep((size_t)(c_hi + n_new) * sizeof(Cand),
                                 (size_t)c_hi * sizeof(Cand), s));
     if (n_u > 0)
@@ -1199,10 +1239,9 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
                      ctx->flag32.as<int32_t>(), R.cnt + 5);
   HIP_TRY(hipGetLastError());
   int32_t nf = 0, n_reopened = 0;
-  HIP_TRY(hipMemcpyAsync(&nf, ctx->flag_pos.as<int32_t>() + nt, 4,
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(&n_reopened, R.cnt + 5, 4, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(read_back(ctx, s,
+                    {{&nf, ctx->flag_pos.as<int32_t>() + nt, 4},
+                     {&n_reopened, R.cnt + 5, 4}}));
   int64_t n_fb = 0;
   const int64_t n_brute = c_hi;
   if (nf > 0) {
@@ -1221,9 +1260,7 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
     HIP_TRY(hipGetLastError());
     HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->brute_n.as<int64_t>(),
                                    ctx->brute_off.as<int64_t>(), nf + 1));
-    HIP_TRY(hipMemcpyAsync(&n_fb, ctx->brute_off.as<int64_t>() + nf, 8,
-                           hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(read_back(ctx, s, {{&n_fb, ctx->brute_off.as<int64_t>() + nf, 8}}));
     HIP_TRY(ctx->cand.need_keep((size_t)(c_hi + n_fb) * sizeof(Cand),
                                 (size_t)c_hi * sizeof(Cand), s));
     hipLaunchKernelGGL(k_brute, dim3(nf), dim3(64), 0, s, nf,
@@ -1264,8 +1301,7 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
         ctx->vals_b.as<int32_t>(), (int)nB, 0, 64, s));
   }
   int32_t n_b = 0;
-  HIP_TRY(hipMemcpyAsync(&n_b, R.cnt + 3, 4, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(read_back(ctx, s, {{&n_b, R.cnt + 3, 4}}));
   HIP_TRY(ctx->out_off.need(((size_t)nt + 1) * 8));
   HIP_TRY(ctx->out_cnt.need(((size_t)nt + 1) * 8));
   hipLaunchKernelGGL(k_out_counts, dim3(ceil_div(nt + 1, 256)), dim3(256), 0,
@@ -1276,9 +1312,7 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
   HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->out_cnt.as<int64_t>(),
                                  ctx->out_off.as<int64_t>(), nt + 1));
   int64_t n_incl = 0;
-  HIP_TRY(hipMemcpyAsync(&n_incl, ctx->out_off.as<int64_t>() + nt, 8,
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(read_back(ctx, s, {{&n_incl, ctx->out_off.as<int64_t>() + nt, 8}}));
   HIP_TRY(ctx->perm.need((size_t)std::max<int64_t>(n_incl, 1) * 4));
   if (n1 > 0)
     hipLaunchKernelGGL(k_place_a, dim3(ceil_div(n1, 256)), dim3(256), 0, s, n1,
@@ -1300,7 +1334,7 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
                        n_incl, ctx->cand.as<Cand>(), ctx->perm.as<int32_t>(),
                        ctx->out_ids.as<int32_t>());
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(wait_stream(ctx, s));
   ctx->tet_off.swap(ctx->out_off);
   ctx->rel_ids.swap(ctx->out_ids);
   ctx->tet_count.swap(ctx->out_cnt);
@@ -1336,9 +1370,8 @@ int compact(rpd_ctx* ctx, RunState& R, int64_t n, const int32_t* perm,
         ctx->scan_tmp.p, tmp, ctx->sizes.as<Sizes3>(), ctx->offs.as<Sizes3>(),
         Sizes3Sum(), zero, (int)(nb + 1), s));
   }
-  HIP_TRY(hipMemcpyAsync(tot, ctx->offs.as<Sizes3>() + nb, sizeof(Sizes3),
-                         hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(read_back(ctx, s,
+                    {{tot, ctx->offs.as<Sizes3>() + nb, sizeof(Sizes3)}}));
   HIP_TRY(ctx->out_cells.need((size_t)std::max<int64_t>(tot->cells, 1) *
                               sizeof(rpd_cell)));
   HIP_TRY(ctx->out_words.need((size_t)std::max<int64_t>(tot->words, 1) * 4));
@@ -1434,10 +1467,10 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
                        ctx->stats64.as<unsigned long long>());
   int64_t st64[4], drops[9];
   int32_t host_cnt[8];
-  HIP_TRY(hipMemcpyAsync(st64, ctx->stats64.p, 32, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(drops, ctx->drops.p, 72, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(host_cnt, R.cnt, 32, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(read_back(ctx, s,
+                    {{st64, ctx->stats64.p, 32},
+                     {drops, ctx->drops.p, 72},
+                     {host_cnt, R.cnt, 32}}));
   // tet_k: max candidates of one tet
   {
     size_t tmp = 0;
@@ -1447,8 +1480,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     HIP_TRY(hipcub::DeviceReduce::Max(ctx->scan_tmp.p, tmp,
                                       ctx->tet_count.as<int64_t>(),
                                       ctx->stats64.as<int64_t>() + 2, nt, s));
-    HIP_TRY(hipMemcpyAsync(st64, ctx->stats64.p, 32, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(read_back(ctx, s, {{st64, ctx->stats64.p, 32}}));
   }
   rpd_stats& st = ctx->st;
   st.n_cells = tot.cells;
@@ -1597,7 +1629,7 @@ int rpd_power_neighbors_gpu(rpd_ctx* ctx, int32_t n, const double* xyzw,
     HIP_TRY(hipMemcpyAsync(lists.data(), d_lists.p, (size_t)n * kNbOut * 4,
                            hipMemcpyDeviceToHost, s));
   }
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(wait_stream(ctx, s));
   if (ms_out) {
     float m = 0;
     HIP_TRY(hipEventElapsedTime(&m, ctx->ev[14], ctx->ev[15]));
@@ -1649,7 +1681,7 @@ int rpd_tune(rpd_ctx* ctx, const rpd_run_cfg* base, int32_t* best_G,
 
 int rpd_sync(rpd_ctx* ctx) {
   if (!ctx) return -1;
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(wait_stream(ctx, ctx->stream));
   return 0;
 }
 
@@ -1677,7 +1709,7 @@ int rpd_download_cells(rpd_ctx* ctx, rpd_cell* cells, uint32_t* words,
   if (areas && st.n_areas)
     HIP_TRY(hipMemcpyAsync(areas, ctx->out_areas.p, st.n_areas * 8,
                            hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(wait_stream(ctx, s));
   return 0;
 }
 
@@ -1691,7 +1723,7 @@ int rpd_download_relation(rpd_ctx* ctx, int64_t* offsets, int32_t* ids) {
   if (ids && ctx->st.n_candidates)
     HIP_TRY(hipMemcpyAsync(ids, ctx->rel_ids.p, ctx->st.n_candidates * 4,
                            hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(wait_stream(ctx, s));
   return 0;
 }
 
@@ -1781,7 +1813,7 @@ int sum_device(rpd_ctx* ctx, const float* d, int64_t n, double* out) {
   std::vector<float> h(n);
   HIP_TRY(hipMemcpyAsync(h.data(), d, (size_t)n * 4, hipMemcpyDeviceToHost,
                          ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(wait_stream(ctx, ctx->stream));
   double sum = 0;
   for (float v : h) sum += v;
   *out = sum;
@@ -1804,7 +1836,7 @@ int rpd_mlp_set(rpd_ctx* ctx, const float* params) {
   HIP_TRY(hipMemsetAsync(ctx->mlp_m1.p, 0, kNParam * 4, ctx->stream));
   HIP_TRY(hipMemsetAsync(ctx->mlp_m2.p, 0, kNParam * 4, ctx->stream));
   if (int rc = mlp_pack(ctx)) return rc;
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(wait_stream(ctx, ctx->stream));
   ctx->adam_step = 0;
   ctx->mlp_ready = true;
   return 0;
@@ -1816,7 +1848,7 @@ int rpd_mlp_get(rpd_ctx* ctx, float* params) {
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipMemcpyAsync(params, ctx->mlp_master.p, kNParam * 4,
                          hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(wait_stream(ctx, ctx->stream));
   return 0;
 }
 
@@ -1851,7 +1883,7 @@ int rpd_features(rpd_ctx* ctx, int32_t* knn, uint16_t* X) {
   if (X)
     HIP_TRY(hipMemcpyAsync(X, ctx->feat.p, nt * kIn * 2, hipMemcpyDeviceToHost,
                            ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(wait_stream(ctx, ctx->stream));
   return 0;
 }
 
@@ -1874,7 +1906,7 @@ int rpd_labels(rpd_ctx* ctx, float* Y, float* M) {
   if (M)
     HIP_TRY(hipMemcpyAsync(M, ctx->lab_m.p, (size_t)nt * kOut * 4,
                            hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(wait_stream(ctx, ctx->stream));
   ctx->lab_n = nt;
   return 0;
 }
@@ -1970,7 +2002,7 @@ int rpd_mlp_grad(rpd_ctx* ctx, const float* params, int32_t n,
   *loss = (float)lsum;
   HIP_TRY(hipMemcpyAsync(grad, ctx->tr_grad.p, kNParam * 4,
                          hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(wait_stream(ctx, s));
   return 0;
 }
 
@@ -1994,7 +2026,7 @@ int rpd_mlp_scores(rpd_ctx* ctx, float* logits) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(logits, ctx->tr_grad.p, (size_t)nt * kOut * 4,
                          hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(wait_stream(ctx, ctx->stream));
   return 0;
 }
 

@@ -754,13 +754,15 @@ __device__ __forceinline__ double rcp64(double x) {
   e = fma(-x, r, 1.0);
   return fma(r, e, r);
 }
-// foot of C on plane p, given 1/|n_p|^2
-__device__ __forceinline__ void foot_plane(const rpd_d4& p, double inv_nn,
-                                           const double C[3], double F[3]) {
+// foot F = C - s n_p of C on plane p, given 1/|n_p|^2; returns s =
+// (n_p.C + d_p) / |n_p|^2
+__device__ __forceinline__ double foot_plane(const rpd_d4& p, double inv_nn,
+                                             const double C[3], double F[3]) {
   const double s = (p.x * C[0] + p.y * C[1] + p.z * C[2] + p.w) * inv_nn;
   F[0] = C[0] - s * p.x;
   F[1] = C[1] - s * p.y;
   F[2] = C[2] - s * p.z;
+  return s;
 }
 // foot of C on the line through A with direction n_p x n_q
 __device__ __forceinline__ void foot_line(const rpd_d4& p, const rpd_d4& q,
@@ -776,18 +778,6 @@ __device__ __forceinline__ void foot_line(const rpd_d4& p, const rpd_d4& q,
   E[0] = A[0] + s * d0;
   E[1] = A[1] + s * d1;
   E[2] = A[2] + s * d2;
-}
-// six times the signed volume of tet (P, Q, C, A), in the sign convention
-// of the reference's get_tet_volume(P-A, Q-A, C-A) = -det3(...)/6
-__device__ __forceinline__ double tet_vol6(const double P[3],
-                                           const double Q[3],
-                                           const double C[3],
-                                           const double A[3]) {
-  const double a0 = P[0] - A[0], a1 = P[1] - A[1], a2 = P[2] - A[2];
-  const double b0 = Q[0] - A[0], b1 = Q[1] - A[1], b2 = Q[2] - A[2];
-  const double c0 = C[0] - A[0], c1 = C[1] - A[1], c2 = C[2] - A[2];
-  return -(a0 * (b1 * c2 - b2 * c1) - b0 * (a1 * c2 - a2 * c1) +
-           c0 * (a1 * b2 - a2 * b1));
 }
 // twice the signed area of triangle (P, Q, A) along the (unnormalised)
 // normal n
@@ -1191,6 +1181,9 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
     // volumes give the cell volume and centroid, their bases the facet
     // areas (signed along each plane normal, so feet outside a face
     // cancel).  C is the cell's vertex 0, so rounding scales with the cell.
+    // A base (X, Y, A) lies in A's plane p, so six times the signed volume
+    // of (X, Y, C, A) is -area2(X, Y, A; n_p) * s_p, with s_p the foot
+    // parameter of C on p (C - F_p = s_p n_p): one product per tet.
     double Cc[3];
     {
       const uint32_t w0 = s_ver[g][0];
@@ -1219,27 +1212,31 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
         // walk the six (foot, foot) pairs around A so that each foot dies
         // after its two tets and base triangles
         double Fa[3], Xp[3], Xn[3];
-        foot_plane(pa, rcp64(pa.x * pa.x + pa.y * pa.y + pa.z * pa.z), Cc, Fa);
-#define RPD_ACC(P0, P1, PL, CONTRIB)                             \
+        const double sa = foot_plane(
+            pa, rcp64(pa.x * pa.x + pa.y * pa.y + pa.z * pa.z), Cc, Fa);
+#define RPD_ACC(P0, P1, PL, SP, CONTRIB)                         \
   {                                                              \
-    const double tvl = tet_vol6(P0, P1, Cc, A);                  \
+    const double ar2 = tri_area2(P0, P1, A, PL);                 \
+    const double tvl = -ar2 * SP;                                \
     vol6 += tvl;                                                 \
     mx += tvl * (((P0[0] + P1[0]) + Cc[0]) + A[0]);              \
     my += tvl * (((P0[1] + P1[1]) + Cc[1]) + A[1]);              \
     mz += tvl * (((P0[2] + P1[2]) + Cc[2]) + A[2]);              \
-    CONTRIB += tri_area2(P0, P1, A, PL);                         \
+    CONTRIB += ar2;                                              \
   }
         foot_line(pa, pb, A, Cc, Xp);                               // Eab
-        RPD_ACC(Fa, Xp, pa, ca)
-        foot_plane(pb, rcp64(pb.x * pb.x + pb.y * pb.y + pb.z * pb.z), Cc, Xn);
-        RPD_ACC(Xp, Xn, pb, cb)                                     // Eab, Fb
+        RPD_ACC(Fa, Xp, pa, sa, ca)
+        const double sb = foot_plane(
+            pb, rcp64(pb.x * pb.x + pb.y * pb.y + pb.z * pb.z), Cc, Xn);
+        RPD_ACC(Xp, Xn, pb, sb, cb)                                 // Eab, Fb
         foot_line(pb, pc, A, Cc, Xp);                               // Ebc
-        RPD_ACC(Xn, Xp, pb, cb)                                     // Fb, Ebc
-        foot_plane(pc, rcp64(pc.x * pc.x + pc.y * pc.y + pc.z * pc.z), Cc, Xn);
-        RPD_ACC(Xp, Xn, pc, cc)                                     // Ebc, Fc
+        RPD_ACC(Xn, Xp, pb, sb, cb)                                 // Fb, Ebc
+        const double sc = foot_plane(
+            pc, rcp64(pc.x * pc.x + pc.y * pc.y + pc.z * pc.z), Cc, Xn);
+        RPD_ACC(Xp, Xn, pc, sc, cc)                                 // Ebc, Fc
         foot_line(pc, pa, A, Cc, Xp);                               // Eca
-        RPD_ACC(Xn, Xp, pc, cc)                                     // Fc, Eca
-        RPD_ACC(Xp, Fa, pa, ca)                                     // Eca, Fa
+        RPD_ACC(Xn, Xp, pc, sc, cc)                                 // Fc, Eca
+        RPD_ACC(Xp, Fa, pa, sa, ca)                                 // Eca, Fa
 #undef RPD_ACC
       }
       // per-plane sums over the group's lanes (fixed butterfly order);
