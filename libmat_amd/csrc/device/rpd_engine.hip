@@ -1178,24 +1178,27 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
                        ctx->umark.as<int32_t>());
     HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->umark.as<int32_t>(),
                                    ctx->upos.as<int32_t>(), n_miss + 1));
-    int32_t n_u = 0;
-    HIP_TRY(read_back(ctx, s, {{&n_u, ctx->upos.as<int32_t>() + n_miss, 4}}));
-    HIP_TRY(ctx->ukeys.need((size_t)std::max(n_u, 1) * 8));
-    HIP_TRY(ctx->upass.need((size_t)n_u + 1));
-    HIP_TRY(ctx->ppos.need(((size_t)n_u + 1) * 4));
+    // the unique count n_u = upos[n_miss] stays on the device: buffers and
+    // launches are sized by n_miss >= n_u, and the pass flags past n_u are
+    // zero, so the number of new candidates is ppos[n_miss]
+    HIP_TRY(ctx->ukeys.need((size_t)std::max<int64_t>(n_miss, 1) * 8));
+    HIP_TRY(ctx->upass.need((size_t)n_miss + 1));
+    HIP_TRY(ctx->ppos.need(((size_t)n_miss + 1) * 4));
     hipLaunchKernelGGL(k_closure_split, dim3(ceil_div(n_miss, 256)), dim3(256),
                        0, s, n_miss, ctx->miss_b.as<uint64_t>(),
                        ctx->umark.as<int32_t>(), ctx->upos.as<int32_t>(), R.M,
                        R.S, ctx->ukeys.as<uint64_t>(),
                        ctx->upass.as<uint8_t>());
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_pass32, dim3(ceil_div((int64_t)n_u + 1, 256)),
-                       dim3(256), 0, s, (int64_t)n_u, ctx->upass.as<uint8_t>(),
-                       ctx->umark.as<int32_t>());
+    hipLaunchKernelGGL(k_pass32, dim3(ceil_div(n_miss + 1, 256)), dim3(256), 0,
+                       s, n_miss, ctx->upos.as<int32_t>() + n_miss,
+                       ctx->upass.as<uint8_t>(), ctx->umark.as<int32_t>());
     HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->umark.as<int32_t>(),
-                                   ctx->ppos.as<int32_t>(), n_u + 1));
-    int32_t n_new = 0;
-    HIP_TRY(read_back(ctx, s, {{&n_new, ctx->ppos.as<int32_t>() + n_u, 4}}));
+                                   ctx->ppos.as<int32_t>(), n_miss + 1));
+    int32_t n_u = 0, n_new = 0;
+    HIP_TRY(read_back(ctx, s,
+                      {{&n_u, ctx->upos.as<int32_t>() + n_miss, 4},
+                       {&n_new, ctx->ppos.as<int32_t>() + n_miss, 4}}));
     HIP_TRY(ctx->cand.need_keep((size_t)(c_hi + n_new) * sizeof(Cand),
                                 (size_t)c_hi * sizeof(Cand), s));
     if (n_u > 0)
@@ -1300,8 +1303,9 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
         ctx->keys_b.as<uint64_t>(), ctx->vals_a.as<int32_t>(),
         ctx->vals_b.as<int32_t>(), (int)nB, 0, 64, s));
   }
-  int32_t n_b = 0;
-  HIP_TRY(read_back(ctx, s, {{&n_b, R.cnt + 3, 4}}));
+  // the excluded B keys (~0) sort after every real key, so searches over
+  // all nB keys find what searches over the n_b real ones would
+  const int64_t n_b = nB;
   HIP_TRY(ctx->out_off.need(((size_t)nt + 1) * 8));
   HIP_TRY(ctx->out_cnt.need(((size_t)nt + 1) * 8));
   hipLaunchKernelGGL(k_out_counts, dim3(ceil_div(nt + 1, 256)), dim3(256), 0,
@@ -1334,7 +1338,6 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
                        n_incl, ctx->cand.as<Cand>(), ctx->perm.as<int32_t>(),
                        ctx->out_ids.as<int32_t>());
   HIP_TRY(hipGetLastError());
-  HIP_TRY(wait_stream(ctx, s));
   ctx->tet_off.swap(ctx->out_off);
   ctx->rel_ids.swap(ctx->out_ids);
   ctx->tet_count.swap(ctx->out_cnt);
@@ -1465,13 +1468,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     hipLaunchKernelGGL(k_mask_stats, dim3(ceil_div(n_cand, 256)), dim3(256), 0,
                        s, n_cand, ctx->cand.as<Cand>(), R.S,
                        ctx->stats64.as<unsigned long long>());
-  int64_t st64[4], drops[9];
-  int32_t host_cnt[8];
-  HIP_TRY(read_back(ctx, s,
-                    {{st64, ctx->stats64.p, 32},
-                     {drops, ctx->drops.p, 72},
-                     {host_cnt, R.cnt, 32}}));
-  // tet_k: max candidates of one tet
+  // tet_k: max candidates of one tet, into stats64[2]
   {
     size_t tmp = 0;
     HIP_TRY(hipcub::DeviceReduce::Max(nullptr, tmp, ctx->tet_count.as<int64_t>(),
@@ -1480,8 +1477,13 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     HIP_TRY(hipcub::DeviceReduce::Max(ctx->scan_tmp.p, tmp,
                                       ctx->tet_count.as<int64_t>(),
                                       ctx->stats64.as<int64_t>() + 2, nt, s));
-    HIP_TRY(read_back(ctx, s, {{st64, ctx->stats64.p, 32}}));
   }
+  int64_t st64[4], drops[9];
+  int32_t host_cnt[8];
+  HIP_TRY(read_back(ctx, s,
+                    {{st64, ctx->stats64.p, 32},
+                     {drops, ctx->drops.p, 72},
+                     {host_cnt, R.cnt, 32}}));
   rpd_stats& st = ctx->st;
   st.n_cells = tot.cells;
   st.n_candidates = n_out;

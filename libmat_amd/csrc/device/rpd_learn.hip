@@ -947,10 +947,14 @@ __global__ __launch_bounds__(256) void k_closure_fill(int64_t n_u,
   cand[cand_base + ppos[u]] = c;
 }
 
-__global__ __launch_bounds__(256) void k_pass32(int64_t n, const uint8_t* p,
+// out[q] = p[q] for q < *n_dev, 0 for *n_dev <= q <= n (n >= *n_dev: the
+// host sizes the launch by an upper bound of the device count)
+__global__ __launch_bounds__(256) void k_pass32(int64_t n,
+                                                const int32_t* n_dev,
+                                                const uint8_t* p,
                                                 int32_t* out) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q <= n) out[q] = q < n ? p[q] : 0;
+  if (q <= n) out[q] = q < *n_dev ? p[q] : 0;
 }
 
 __global__ __launch_bounds__(256) void k_flag_final(int32_t n_tet,
@@ -1100,6 +1104,7 @@ __global__ __launch_bounds__(256) void k_place_b(
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_b) return;
   const uint64_t k = bkeys[r];
+  if (k == ~0ull) return;  // excluded (sorted after every real key)
   const int t = (int)(k >> 32);
   const int32_t j = (int32_t)(k & 0xffffffffu);
   const int64_t start = lower_u64(bkeys, n_b, (uint64_t)(uint32_t)t << 32);
