@@ -19,6 +19,8 @@
 //   k_compact_*    deterministic compaction into the output stream
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "rpd_device.h"
 #include "rpd_capi.h"
 
@@ -758,6 +760,7 @@ __device__ __forceinline__ double rcp64(double x) {
 // (n_p.C + d_p) / |n_p|^2
 __device__ __forceinline__ double foot_plane(const rpd_d4& p, double inv_nn,
                                              const double C[3], double F[3]) {
+#pragma clang fp contract(fast)
   const double s = (p.x * C[0] + p.y * C[1] + p.z * C[2] + p.w) * inv_nn;
   F[0] = C[0] - s * p.x;
   F[1] = C[1] - s * p.y;
@@ -768,6 +771,7 @@ __device__ __forceinline__ double foot_plane(const rpd_d4& p, double inv_nn,
 __device__ __forceinline__ void foot_line(const rpd_d4& p, const rpd_d4& q,
                                           const double A[3], const double C[3],
                                           double E[3]) {
+#pragma clang fp contract(fast)
   const double d0 = p.y * q.z - p.z * q.y;
   const double d1 = p.z * q.x - p.x * q.z;
   const double d2 = p.x * q.y - p.y * q.x;
@@ -785,6 +789,7 @@ __device__ __forceinline__ double tri_area2(const double P[3],
                                             const double Q[3],
                                             const double A[3],
                                             const rpd_d4& n) {
+#pragma clang fp contract(fast)
   const double u0 = Q[0] - P[0], u1 = Q[1] - P[1], u2 = Q[2] - P[2];
   const double v0 = A[0] - P[0], v1 = A[1] - P[1], v2 = A[2] - P[2];
   return (u1 * v2 - u2 * v1) * n.x + (u2 * v0 - u0 * v2) * n.y +
@@ -825,6 +830,11 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
   __shared__ uint32_t s_ver[G][V];
   __shared__ uint8_t s_perm[G][V];
   __shared__ uint32_t s_edge[G][E];
+  // s_adj[p]: bit q (q < p) set iff the edge list holds the pair (q, p)
+  using AdjT = typename std::conditional<
+      (P <= 16), uint16_t,
+      typename std::conditional<(P <= 32), uint32_t, uint64_t>::type>::type;
+  __shared__ AdjT s_adj[G][P];
 
   const int lane = threadIdx.x;
   const int g = lane / L, j = lane % L;
@@ -863,6 +873,7 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
       s_h[g][j] = (uint8_t)meta_byte(tm, j);
       s_ver[g][j] = c_init_ver[j][0] | (c_init_ver[j][1] << 8) |
                     (c_init_ver[j][2] << 16) | (meta_byte(tm, 4 + j) << 24);
+      s_adj[g][j] = (AdjT)((1u << j) - 1u);  // the six edges of the tet
     }
     for (int q = j; q < 6; q += L)
       s_edge[g][q] = c_init_edge[q][0] | (c_init_edge[q][1] << 8) |
@@ -945,6 +956,7 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
       s_pl[g][cur] = make_double4(e.x, e.y, e.z, e.w);
       s_src[g][cur] = m;
       s_h[g][cur] = 1;  // F_CELL_ADJ_DEFAULT
+      s_adj[g][cur] = 0;
     }
     nb_p = cur + 1;
 
@@ -1090,12 +1102,23 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
       break;
     }
     const uint32_t hcur = s_h[g][cur];
+    uint32_t adj_lo = 0, adj_hi = 0;  // the planes now joined to cur
     for (int q = j; q < nbd; q += L) {
       const uint32_t c2 = s_cyc[g][q];
       const uint32_t hz = max(hcur, (uint32_t)s_h[g][c2]);
       const uint32_t lo = min((uint32_t)cur, c2), hi = max((uint32_t)cur, c2);
       s_edge[g][nb_e + q] = lo | (hi << 8) | (hz << 16);
+      if (P <= 32 || c2 < 32)
+        adj_lo |= 1u << (c2 & 31);
+      else
+        adj_hi |= 1u << (c2 - 32);
     }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) {
+      adj_lo |= __shfl_xor(adj_lo, off);
+      if constexpr (P > 32) adj_hi |= __shfl_xor(adj_hi, off);
+    }
+    if (j == 0) s_adj[g][cur] = (AdjT)(((uint64_t)adj_hi << 32) | adj_lo);
     wsync();
     // ---- new vertices (cur, cir, next(cir)) (convex_cell.cu:760-772)
     int first_bad = 1 << 30;   // first index failing overflow or perturb
@@ -1113,15 +1136,19 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
           const uint32_t z1 = (s_edge[g][nb_e + q] >> 16) & 255;
           const uint32_t z2 = (s_edge[g][nb_e + (q + 1) % nbd] >> 16) & 255;
           const uint32_t lo = min(c1, c2), hi = max(c1, c2);
+          // the edge (lo, hi), without a scan of the list: a pair of tet
+          // faces is one of the six initial records (list positions 0-5,
+          // pairs (2,3), (1,3), (1,2), (0,3), (0,2), (0,1)); a pair with a
+          // bisector was appended once, when its later plane was clipped,
+          // with z = max of the two planes' adjacency bytes
+          const bool found = (s_adj[g][hi] >> lo) & 1;
           uint32_t z3 = 0;
-          bool found = false;
-          for (int ee = 0; ee < nb_e; ++ee) {
-            const uint32_t w = s_edge[g][ee];
-            if ((w & 255) == lo && ((w >> 8) & 255) == hi) {
-              z3 = (w >> 16) & 255;
-              found = true;
-              break;
-            }
+          if (found) {
+            if (hi < 4)
+              z3 = (s_edge[g][5 - (lo * (7 - lo) / 2 + (hi - lo - 1))] >> 16) &
+                   255;
+            else
+              z3 = max((uint32_t)s_h[g][lo], (uint32_t)s_h[g][hi]);
           }
           if (!found) {
             kind = 3;
@@ -1173,6 +1200,8 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
   // ---- write the slot
   const SlotView& o = a.out;
   if (status == RPD_SUCCESS && a.geometry) {
+    // (tolerance-checked arithmetic: products and sums may fuse)
+#pragma clang fp contract(fast)
     // FP64 geometry in one pass over the vertices.  Around each vertex A
     // lie 6 tets (C, X, Y, A) whose bases (X, Y, A) tile A's corner of its
     // three faces: X, Y run over the feet of C on A's 3 planes and on its 3
