@@ -737,6 +737,46 @@ struct ClipArgs {
   CertArgs cert;
 };
 
+// 128-bit vertex masks as (lo, hi) words
+__device__ __forceinline__ bool mbit(uint64_t lo, uint64_t hi, int v) {
+  return v < 64 ? (lo >> v) & 1 : (hi >> (v - 64)) & 1;
+}
+__device__ __forceinline__ uint64_t low_bits(int n) {  // bits [0, n), n <= 64
+  return n >= 64 ? ~0ull : (1ull << n) - 1;
+}
+// bits [a, b), 0 <= a <= b <= 128
+__device__ __forceinline__ void mrange(int a, int b, uint64_t& lo,
+                                       uint64_t& hi) {
+  lo = low_bits(min(b, 64)) & ~low_bits(min(a, 64));
+  hi = low_bits(max(b, 64) - 64) & ~low_bits(max(a, 64) - 64);
+}
+// position of the k-th lowest (k >= 1) set bit
+__device__ __forceinline__ int mkth_low(uint64_t lo, uint64_t hi, int k) {
+  const int nl = __popcll(lo);
+  uint64_t w = lo;
+  int base = 0;
+  if (k > nl) {
+    k -= nl;
+    w = hi;
+    base = 64;
+  }
+  for (int i = 1; i < k; ++i) w &= w - 1;
+  return base + __ffsll((long long)w) - 1;
+}
+// position of the k-th highest (k >= 1) set bit
+__device__ __forceinline__ int mkth_high(uint64_t lo, uint64_t hi, int k) {
+  const int nh = __popcll(hi);
+  uint64_t w = hi;
+  int base = 64;
+  if (k > nh) {
+    k -= nh;
+    w = lo;
+    base = 0;
+  }
+  for (int i = 1; i < k; ++i) w &= ~(1ull << (63 - __clzll((long long)w)));
+  return base + 63 - __clzll((long long)w);
+}
+
 template <int L>
 __device__ __forceinline__ uint64_t group_bits(uint64_t b, int g) {
   if constexpr (L == 64) {
@@ -828,7 +868,6 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
   __shared__ uint8_t s_bn[G][P];
   __shared__ uint8_t s_cyc[G][P + 1];
   __shared__ uint32_t s_ver[G][V];
-  __shared__ uint8_t s_perm[G][V];
   __shared__ uint32_t s_edge[G][E];
   // s_adj[p]: bit q (q < p) set iff the edge list holds the pair (q, p)
   using AdjT = typename std::conditional<
@@ -961,39 +1000,58 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
     nb_p = cur + 1;
 
     // ---- move conflicting vertices to the tail exactly as the serial
-    // swap loop does: i scans from the front, end from the back
+    // swap loop does (i scans from the front, end from the back, a
+    // conflicting vertex at i is swapped with the one at end - 1), in
+    // closed form: with nk = nb_v - nr kept vertices, the front pass reads
+    // F = [0, m), m = nk + (vertex nk conflicts), the back pass reads
+    // B = [m, nb_v) downwards.  A kept vertex of F stays; the j-th
+    // conflicting vertex of F goes to nb_v - 1 (j = 1) or just below the
+    // (j-1)-th kept vertex of B; the k-th kept vertex of B takes the place
+    // of the k-th conflicting vertex of F; a conflicting vertex v of B goes
+    // to v - 1.
     {
-      // s_perm[pos] = old index of the vertex that ends at pos
-      int i = 0, end = nb_v, occ = 0;
-      while (i < end) {
-        const bool cf = occ < 64 ? ((conf_lo >> occ) & 1)
-                                 : ((conf_hi >> (occ - 64)) & 1);
-        int pos, nxt;
-        if (!cf) {
-          pos = i;
-          ++i;
-          nxt = i;
-        } else {
-          --end;
-          pos = end;
-          nxt = end;
-        }
-        if (j == 0) s_perm[g][pos] = (uint8_t)occ;
-        occ = nxt;
-      }
-      wsync();
+      const int nk = nb_v - nr;
+      const int m = nk + (int)mbit(conf_lo, conf_hi, nk);
+      uint64_t f_lo, f_hi, b_lo, b_hi;  // F's conflicts, B's kept vertices
+      mrange(0, m, f_lo, f_hi);
+      mrange(m, nb_v, b_lo, b_hi);
+      f_lo &= conf_lo;
+      f_hi &= conf_hi;
+      b_lo &= ~conf_lo;
+      b_hi &= ~conf_hi;
       uint32_t val[VS];
+      int dst[VS];
 #pragma unroll
       for (int q = 0; q < VS; ++q) {
-        const int pos = q * L + j;
-        val[q] = pos < nb_v ? s_ver[g][s_perm[g][pos]] : 0u;
+        const int v = q * L + j;
+        val[q] = 0;
+        dst[q] = -1;
+        if (v < nb_v) {
+          val[q] = s_ver[g][v];
+          const bool c = mbit(conf_lo, conf_hi, v);
+          if (v < m) {
+            if (!c) {
+              dst[q] = v;
+            } else {
+              uint64_t r_lo, r_hi;
+              mrange(0, v + 1, r_lo, r_hi);
+              const int jj = __popcll(f_lo & r_lo) + __popcll(f_hi & r_hi);
+              dst[q] = jj == 1 ? nb_v - 1 : mkth_high(b_lo, b_hi, jj - 1) - 1;
+            }
+          } else if (!c) {
+            uint64_t r_lo, r_hi;
+            mrange(v, nb_v, r_lo, r_hi);
+            const int kk = __popcll(b_lo & r_lo) + __popcll(b_hi & r_hi);
+            dst[q] = mkth_low(f_lo, f_hi, kk);
+          } else {
+            dst[q] = v - 1;
+          }
+        }
       }
       wsync();
 #pragma unroll
-      for (int q = 0; q < VS; ++q) {
-        const int pos = q * L + j;
-        if (pos < nb_v) s_ver[g][pos] = val[q];
-      }
+      for (int q = 0; q < VS; ++q)
+        if (dst[q] >= 0) s_ver[g][dst[q]] = val[q];
       wsync();
     }
     const int nv_old = nb_v;
