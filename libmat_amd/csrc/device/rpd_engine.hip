@@ -200,7 +200,7 @@ hipError_t wait_stream(rpd_ctx* ctx, hipStream_t s) {
 
 // Small device-to-host reads (counts, totals): copied into the context's
 // pinned block, the stream waited on, then handed to the host variables.
-constexpr size_t kPinBytes = 4096;
+constexpr size_t kPinBytes = 8192;
 struct D2H {
   void* dst;
   const void* src;
@@ -1354,8 +1354,8 @@ int compact(rpd_ctx* ctx, RunState& R, int64_t n, const int32_t* perm,
   const int64_t nb = std::max<int64_t>(ceil_div(n, kCB), 0);
   HIP_TRY(ctx->sizes.need(((size_t)nb + 1) * sizeof(Sizes3)));
   HIP_TRY(ctx->offs.need(((size_t)nb + 1) * sizeof(Sizes3)));
-  HIP_TRY(ctx->drops.need(9 * 8));
-  HIP_TRY(hipMemsetAsync(ctx->drops.p, 0, 9 * 8, s));
+  HIP_TRY(ctx->drops.need(kDropSlots * 9 * 8));
+  HIP_TRY(hipMemsetAsync(ctx->drops.p, 0, kDropSlots * 9 * 8, s));
   HIP_TRY(hipMemsetAsync(ctx->sizes.as<Sizes3>() + nb, 0, sizeof(Sizes3), s));
   if (nb > 0)
     hipLaunchKernelGGL(k_block_sizes, dim3(nb), dim3(kCB), 0, s, n, perm,
@@ -1478,18 +1478,21 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
                                       ctx->tet_count.as<int64_t>(),
                                       ctx->stats64.as<int64_t>() + 2, nt, s));
   }
-  int64_t st64[4], drops[9];
+  int64_t st64[4], drops[kDropSlots * 9];
   int32_t host_cnt[8];
   HIP_TRY(read_back(ctx, s,
                     {{st64, ctx->stats64.p, 32},
-                     {drops, ctx->drops.p, 72},
+                     {drops, ctx->drops.p, sizeof(drops)},
                      {host_cnt, R.cnt, 32}}));
   rpd_stats& st = ctx->st;
   st.n_cells = tot.cells;
   st.n_candidates = n_out;
   st.n_words = tot.words;
   st.n_areas = tot.areas;
-  for (int q = 0; q < 9; ++q) st.n_dropped[q] = drops[q];
+  for (int q = 0; q < 9; ++q) {
+    st.n_dropped[q] = 0;
+    for (int b = 0; b < kDropSlots; ++b) st.n_dropped[q] += drops[9 * b + q];
+  }
   st.n_rerun_large = R.r_large;
   st.n_rerun_exact = R.r_exact;
   st.n_active_planes = st64[0];

@@ -1511,12 +1511,16 @@ __device__ __forceinline__ uint64_t packed_size(uint32_t hd) {
 }
 
 // per block: totals of its cells (bsum[b]) and the dropped cells by status
+// per-status counts of the cells not emitted, spread over kDropSlots copies
+// of the 9 counters (a block adds to copy blockIdx % kDropSlots) so that
+// the blocks' atomics do not all meet at one address
+constexpr int kDropSlots = 64;
 __global__ __launch_bounds__(kCB) void k_block_sizes(int64_t n_cells,
                                                      const int32_t* perm,
                                                      const int32_t* loc,
                                                      TierViews tv,
                                                      Sizes3* bsum,
-                                                     int64_t* drops /*9*/) {
+                                                     int64_t* drops) {
   typedef hipcub::BlockReduce<uint64_t, kCB> Red;
   __shared__ typename Red::TempStorage tmp;
   __shared__ int32_t h[9];
@@ -1537,7 +1541,8 @@ __global__ __launch_bounds__(kCB) void k_block_sizes(int64_t n_cells,
                               (int64_t)((tot >> 16) & 0xfffff)};
   __syncthreads();
   if (threadIdx.x < 9 && h[threadIdx.x])
-    atomicAdd((unsigned long long*)&drops[threadIdx.x],
+    atomicAdd((unsigned long long*)&drops[9 * (blockIdx.x % kDropSlots) +
+                                          threadIdx.x],
               (unsigned long long)h[threadIdx.x]);
 }
 
