@@ -48,11 +48,15 @@ __device__ __forceinline__ rpd_d4 ld4(const double4* p) {
 // common_cuda.h:195-213), with a semi-static filter: every monomial passes
 // through at most 9 roundings, |err| <= gamma_9 * 24 * prod_r max_c |a_rc|,
 // 24 * gamma_9 = 2.398e-14; 2.5e-14 covers the rounding of the bound.
+// Products may fuse with the following sum: a fused monomial passes
+// through fewer roundings, so the bound holds, and a decided sign is the
+// exact one either way (undecided signs go to the exact predicate).
 // Returns +1 conflict, -1 no conflict, 0 undecided.
 __device__ __forceinline__ int conflict_filtered(const rpd_d4& p1,
                                                  const rpd_d4& p2,
                                                  const rpd_d4& p3,
                                                  const rpd_d4& e) {
+#pragma clang fp contract(fast)
   const double a11 = p1.x, a12 = p2.x, a13 = p3.x, a14 = e.x;
   const double a21 = p1.y, a22 = p2.y, a23 = p3.y, a24 = e.y;
   const double a31 = p1.z, a32 = p2.z, a33 = p3.z, a34 = e.z;
@@ -626,6 +630,19 @@ __global__ __launch_bounds__(64) void k_knn_bis(DevSites S, double4* bis) {
   }
 }
 
+// dot(e, X) < 0 for certain: the filter of rpd_filtered_sign4 with the
+// products fused into the sums (no monomial passes through more roundings
+// than unfused, so its error constant holds)
+__device__ __forceinline__ bool certainly_neg4(const rpd_d4& e,
+                                               const rpd_d4& X,
+                                               const rpd_d4& Q) {
+#pragma clang fp contract(fast)
+  const double s = e.x * X.x + e.y * X.y + e.z * X.z + e.w * X.w;
+  const double b =
+      fabs(e.x) * Q.x + fabs(e.y) * Q.y + fabs(e.z) * Q.z + fabs(e.w) * Q.w;
+  return b < RPD_FILTER_MAX && s < -(RPD_FILTER_C * b + RPD_FILTER_ABS);
+}
+
 // One thread per candidate cell: bit k of the mask is set unless the k-th
 // neighbour's bisector is certainly positive at all four corners of the
 // tet cell (then it cannot cut any state of the cell: every cell vertex
@@ -658,10 +675,9 @@ __global__ __launch_bounds__(256) void k_cand_mask(DevMesh M, DevSites S,
   const int lim = deg < 128 ? deg : 128;
   for (int k = 0; k < lim; ++k) {
     const rpd_d4 e = ld4(S.knn_bis + k0 + k);
-    const bool inactive = rpd_filtered_sign4(e, X0, Q0) < 0 &&
-                          rpd_filtered_sign4(e, X1, Q1) < 0 &&
-                          rpd_filtered_sign4(e, X2, Q2) < 0 &&
-                          rpd_filtered_sign4(e, X3, Q3) < 0;
+    const bool inactive =
+        certainly_neg4(e, X0, Q0) && certainly_neg4(e, X1, Q1) &&
+        certainly_neg4(e, X2, Q2) && certainly_neg4(e, X3, Q3);
     if (!inactive) {
       if (k < 64)
         m0 |= 1ull << k;
