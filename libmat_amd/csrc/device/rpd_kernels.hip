@@ -802,6 +802,46 @@ __device__ __forceinline__ uint64_t group_bits(uint64_t b, int g) {
   }
 }
 
+// ---- sums and ORs over the L lanes of a cell's group, butterfly with
+// offsets L/2, ..., 1.  Groups of 4 lanes are quads: their exchanges are
+// DPP quad permutations (one VALU move per 32 bits) instead of LDS-routed
+// lane shuffles.
+template <int C>
+__device__ __forceinline__ uint32_t quad_perm(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, C, 0xF, 0xF, false);
+}
+template <int C>
+__device__ __forceinline__ double quad_perm(double v) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint64_t lo = quad_perm<C>((uint32_t)u);
+  const uint64_t hi = quad_perm<C>((uint32_t)(u >> 32));
+  return __longlong_as_double((long long)((hi << 32) | lo));
+}
+constexpr int kXor2 = 0x4E;  // quad lanes (2, 3, 0, 1)
+constexpr int kXor1 = 0xB1;  // quad lanes (1, 0, 3, 2)
+template <int L>
+__device__ __forceinline__ double group_sum(double x) {
+  if constexpr (L == 4) {
+    x += quad_perm<kXor2>(x);
+    x += quad_perm<kXor1>(x);
+  } else {
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  }
+  return x;
+}
+template <int L>
+__device__ __forceinline__ uint32_t group_or(uint32_t x) {
+  if constexpr (L == 4) {
+    x |= quad_perm<kXor2>(x);
+    x |= quad_perm<kXor1>(x);
+  } else {
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) x |= __shfl_xor(x, off);
+  }
+  return x;
+}
+
 // ---- FP64 geometry helpers (tolerance-checked, not bit-compared)
 // 1/x: hardware reciprocal estimate refined by two Newton steps (relative
 // error ~1 ulp for the normal, finite x used here)
@@ -1187,11 +1227,8 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
       else
         adj_hi |= 1u << (c2 - 32);
     }
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) {
-      adj_lo |= __shfl_xor(adj_lo, off);
-      if constexpr (P > 32) adj_hi |= __shfl_xor(adj_hi, off);
-    }
+    adj_lo = group_or<L>(adj_lo);
+    if constexpr (P > 32) adj_hi = group_or<L>(adj_hi);
     if (j == 0) s_adj[g][cur] = (AdjT)(((uint64_t)adj_hi << 32) | adj_lo);
     wsync();
     // ---- new vertices (cur, cir, next(cir)) (convex_cell.cu:760-772)
@@ -1352,21 +1389,16 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
           if (p < (uint32_t)P && p < (uint32_t)nb_p) {
             double sp = (ia == p ? ca : 0.0) + (ib == p ? cb : 0.0) +
                         (ic == p ? cc : 0.0);
-#pragma unroll
-            for (int off = L / 2; off > 0; off >>= 1)
-              sp += __shfl_xor(sp, off);
+            sp = group_sum<L>(sp);
             if (j == pj) acc[pi] += sp;
           }
         }
       }
     }
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) {
-      vol6 += __shfl_xor(vol6, off);
-      mx += __shfl_xor(mx, off);
-      my += __shfl_xor(my, off);
-      mz += __shfl_xor(mz, off);
-    }
+    vol6 = group_sum<L>(vol6);
+    mx = group_sum<L>(mx);
+    my = group_sum<L>(my);
+    mz = group_sum<L>(mz);
     if (j == 0) {
       double* gm = o.geom + 4 * slot;
       const double vol = vol6 / 6.0;
