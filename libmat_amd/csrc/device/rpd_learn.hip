@@ -279,20 +279,21 @@ __global__ __launch_bounds__(64) void k_knn_features(
       qx[k] = dd - (float)pdist(P, so);
     }
     // rings: the centroid owner's, and a corner owner's only if that
-    // owner is not already in it (the rings overlap otherwise)
-    const int64_t r0 = S.knn_off[own[0]], r1 = S.knn_off[own[0] + 1];
+    // owner is not already in it (the rings overlap otherwise); which
+    // corner owners the centroid owner's ring holds (flagged or not) is
+    // noted while that ring is scanned
+    uint32_t in_ring = 0;  // bit k: own[k] is in the centroid owner's ring
+    int n_own = 0;         // distinct flagged owners listed (key 0)
     for (int j = 0; j < 5; ++j) {
       const int o = own[j];
       bool seen = false;
       for (int q = 0; q < j; ++q) seen = seen || own[q] == o;
       if (seen) continue;
-      if (S.flags[o] == 1u) ins(0.f, o, false);
-      if (j > 0) {
-        bool in_ring = false;
-        for (int64_t q = r0; q < r1 && !in_ring; ++q)
-          in_ring = near_id(S.knn_near[q]) == o;
-        if (in_ring) continue;
+      if (S.flags[o] == 1u) {
+        ins(0.f, o, false);
+        ++n_own;
       }
+      if (j > 0 && ((in_ring >> j) & 1u)) continue;
       const int64_t e0 = S.knn_off[o], e1 = S.knn_off[o + 1];
       // kRing ring entries per round: their loads are in flight together
       constexpr int kRing = 2;
@@ -307,7 +308,14 @@ __global__ __launch_bounds__(64) void k_knn_features(
         }
 #pragma unroll
         for (int u = 0; u < kRing; ++u) {
-          if (q0 + u >= e1 || w4[u] < 0) continue;  // past the end, unflagged
+          if (q0 + u >= e1) continue;  // past the end
+          if (j == 0) {
+            const int nid = near_id(w4[u]);
+#pragma unroll
+            for (int a = 1; a < 5; ++a)
+              in_ring |= (uint32_t)(own[a] == nid) << a;
+          }
+          if (w4[u] < 0) continue;  // unflagged
           const int id = w4[u];
           bool is_owner = false;
 #pragma unroll
@@ -331,13 +339,9 @@ __global__ __launch_bounds__(64) void k_knn_features(
         }
       }
     }
-#pragma unroll
-    for (int k = 0; k < kOut; ++k)
-      if (key_of(k) == 0.f) {
-        bool is_owner = false;
-        for (int a = 0; a < 5; ++a) is_owner = is_owner || own[a] == id_of(k);
-        if (is_owner) om |= 1u << k;
-      }
+    // only owners have key 0 (the others are clamped above it) and none
+    // is ever pushed out, so they hold the first n_own slots
+    om = n_own >= 32 ? ~0u : (1u << n_own) - 1u;
   }
   omask[t] = om;
 #pragma unroll
