@@ -345,7 +345,14 @@ __global__ __launch_bounds__(64) void k_knn_features(
   }
   omask[t] = om;
 #pragma unroll
-  for (int k = 0; k < kOut; ++k) kt[k] = lst[k] != kEmpty ? id_of(k) : -1;
+  for (int k = 0; k < kOut; k += 4) {  // 16-byte stores of the row
+    int4 w;
+    w.x = lst[k] != kEmpty ? id_of(k) : -1;
+    w.y = lst[k + 1] != kEmpty ? id_of(k + 1) : -1;
+    w.z = lst[k + 2] != kEmpty ? id_of(k + 2) : -1;
+    w.w = lst[k + 3] != kEmpty ? id_of(k + 3) : -1;
+    *reinterpret_cast<int4*>(kt + k) = w;
+  }
   const double ext =
       fmax(fmax(I.dom[3] - I.dom[0], I.dom[4] - I.dom[1]), I.dom[5] - I.dom[2]);
   const double inv_ext = ext > 0 ? 1.0 / ext : 0.0;
@@ -368,11 +375,13 @@ __global__ __launch_bounds__(64) void k_knn_features(
     }
   }
 #pragma unroll
-  for (int q = 0; q < kIn; q += 4) {
-    uint2 v;
+  for (int q = 0; q < kIn; q += 8) {  // 16-byte stores of the row
+    uint4 v;
     v.x = (uint32_t)f2bf(f[q]) | ((uint32_t)f2bf(f[q + 1]) << 16);
     v.y = (uint32_t)f2bf(f[q + 2]) | ((uint32_t)f2bf(f[q + 3]) << 16);
-    *reinterpret_cast<uint2*>(x + q) = v;
+    v.z = (uint32_t)f2bf(f[q + 4]) | ((uint32_t)f2bf(f[q + 5]) << 16);
+    v.w = (uint32_t)f2bf(f[q + 6]) | ((uint32_t)f2bf(f[q + 7]) << 16);
+    *reinterpret_cast<uint4*>(x + q) = v;
   }
 }
 
@@ -839,10 +848,11 @@ __global__ __launch_bounds__(64) void k_learn_count(int32_t n_tet, DevMesh M,
   const uint32_t s = sel[t] | omask[t];
   uint32_t pass = 0;
   int n = 0;
-  for (int k = 0; k < kOut; ++k) {
+  // only the selected slots are read (slots past the last proposal hold -1)
+  for (uint32_t rest = s; rest; rest &= rest - 1) {
+    const int k = __ffs(rest) - 1;
     const int id = knn[(int64_t)t * kOut + k];
     if (id < 0) break;
-    if (!((s >> k) & 1u)) continue;
     uint32_t it = 0;
     if (relation(S, L, id, &it)) {
       pass |= 1u << k;
@@ -866,8 +876,12 @@ __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
   int32_t v[kOut];
 #pragma unroll
   for (int k = 0; k < kOut; ++k) {
-    const int32_t id = knn[(int64_t)t * kOut + k];
-    v[k] = ((m >> k) & 1u) && id >= 0 ? id : 0x7fffffff;
+    int32_t id = 0x7fffffff;
+    if ((m >> k) & 1u) {  // only the selected slots are read
+      const int32_t x = knn[(int64_t)t * kOut + k];
+      if (x >= 0) id = x;
+    }
+    v[k] = id;
   }
 #pragma unroll
   for (int size = 2; size <= kOut; size <<= 1)
