@@ -933,7 +933,9 @@ ClipArgs clip_args(rpd_ctx* ctx, RunState& R) {
 
 // masks, small tier, full-capacity and exact re-runs of candidates
 // [c0, c1); buffers grow keeping earlier rounds; loc of the range is set
-constexpr int64_t kStepOrderMin = 2048;
+// (below this many cells every wavefront of the launch is resident at once,
+// so the order cannot shorten it and the sort would only add latency)
+constexpr int64_t kStepOrderMin = 65536;
 
 int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
                bool events) {
