@@ -167,6 +167,12 @@ struct rpd_ctx {
   // receives the counts read back between pipeline phases
   hipEvent_t ev_sync = nullptr;
   char* pin = nullptr;
+  // pinned staging of the per-run instance table; ev_inst marks the end of
+  // its last copy to the device
+  void* pin_inst = nullptr;
+  size_t pin_inst_cap = 0;
+  hipEvent_t ev_inst = nullptr;
+  bool inst_copy_recorded = false;
 };
 
 namespace {
@@ -263,6 +269,8 @@ int rpd_ctx_create(int32_t device, rpd_ctx** out) {
   ctx->events = true;
   if (hipEventCreateWithFlags(&ctx->ev_sync, hipEventDisableTiming) !=
           hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_inst, hipEventDisableTiming) !=
+          hipSuccess ||
       hipHostMalloc((void**)&ctx->pin, kPinBytes, hipHostMallocDefault) !=
           hipSuccess) {
     rpd_ctx_destroy(ctx);
@@ -281,6 +289,8 @@ void rpd_ctx_destroy(rpd_ctx* ctx) {
     for (int i = 0; i < 16; ++i) (void)hipEventDestroy(ctx->ev[i]);
   if (ctx->ev_sync) (void)hipEventDestroy(ctx->ev_sync);
   if (ctx->pin) (void)hipHostFree(ctx->pin);
+  if (ctx->pin_inst) (void)hipHostFree(ctx->pin_inst);
+  if (ctx->ev_inst) (void)hipEventDestroy(ctx->ev_inst);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -697,11 +707,28 @@ int upload_inst_info(rpd_ctx* ctx, RunState& R) {
   }
   if (n_bins >= (1LL << 31) || n_sg >= (1LL << 31))
     return fail(ctx, "rpd_run: too many grid cells");
-  HIP_TRY(ctx->inst_dev.need(ii.size() * sizeof(InstInfo)));
-  HIP_TRY(hipMemcpyAsync(ctx->inst_dev.p, ii.data(),
-                         ii.size() * sizeof(InstInfo), hipMemcpyHostToDevice,
-                         R.s));
-  HIP_TRY(wait_stream(ctx, R.s));  // the host vector dies here
+  const size_t bytes = ii.size() * sizeof(InstInfo);
+  HIP_TRY(ctx->inst_dev.need(bytes));
+  // the previous copy out of the pinned block must have completed before
+  // the block is rewritten (normally long done: runs end with a wait)
+  if (ctx->inst_copy_recorded) {
+    hipError_t e;
+    while ((e = hipEventQuery(ctx->ev_inst)) == hipErrorNotReady) {
+    }
+    HIP_TRY(e);
+  }
+  if (bytes > ctx->pin_inst_cap) {
+    if (ctx->pin_inst) (void)hipHostFree(ctx->pin_inst);
+    ctx->pin_inst = nullptr;
+    ctx->pin_inst_cap = 0;
+    HIP_TRY(hipHostMalloc(&ctx->pin_inst, bytes, hipHostMallocDefault));
+    ctx->pin_inst_cap = bytes;
+  }
+  std::memcpy(ctx->pin_inst, ii.data(), bytes);
+  HIP_TRY(hipMemcpyAsync(ctx->inst_dev.p, ctx->pin_inst, bytes,
+                         hipMemcpyHostToDevice, R.s));
+  HIP_TRY(hipEventRecord(ctx->ev_inst, R.s));
+  ctx->inst_copy_recorded = true;
   R.n_bins = n_bins;
   R.n_sg = n_sg;
   R.inst_d = ctx->inst_dev.as<InstInfo>();
