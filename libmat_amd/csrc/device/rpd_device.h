@@ -60,7 +60,9 @@ using LargeCaps = Caps<96, 64, 152>;
 
 // Output slot of one cell (indexed by cell or by re-run index).
 struct SlotView {
-  int32_t stride_words;   // V + P + E
+  int32_t stride_words;   // V + P + E: vertices at 0, plane sources at
+  int32_t ofs_src;        // ofs_src = V, edges at ofs_edges = V + P
+  int32_t ofs_edges;
   int32_t stride_areas;   // P
   uint32_t* hdr;          // nb_v | nb_p<<8 | nb_e<<16 | status<<24
   uint32_t* words;

@@ -84,6 +84,8 @@ struct TierBufs {
   SlotView view(int V, int P, int E) const {
     SlotView s;
     s.stride_words = V + P + E;
+    s.ofs_src = V;
+    s.ofs_edges = V + P;
     s.stride_areas = P;
     s.hdr = hdr.as<uint32_t>();
     s.words = words.as<uint32_t>();

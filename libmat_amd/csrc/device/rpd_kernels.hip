@@ -924,7 +924,6 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
   __shared__ uint8_t s_bn[G][P];
   __shared__ uint8_t s_cyc[G][P + 1];
   __shared__ uint32_t s_ver[G][V];
-  __shared__ uint32_t s_edge[G][E];
   // s_adj[p]: bit q (q < p) set iff the edge list holds the pair (q, p)
   using AdjT = typename std::conditional<
       (P <= 16), uint16_t,
@@ -946,6 +945,10 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
   if (item >= a.n_items) return;
   const int64_t cidx = a.items ? a.items[item] : a.item_base + item;
   const int64_t slot = a.slot_by_cell ? cidx : a.slot_base + item;
+  // the slot's edge records are written as the edges are appended (the
+  // reference never removes an edge), so they need no LDS copy
+  uint32_t* const wslot = a.out.words + slot * a.out.stride_words;
+  uint32_t* const wedge = wslot + a.out.ofs_edges;
   const Cand cd = a.cand[cidx];
   const int t = cd.tet, s = cd.seed;
   const int4 tv = a.M.tet_v[t];
@@ -971,8 +974,8 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
       s_adj[g][j] = (AdjT)((1u << j) - 1u);  // the six edges of the tet
     }
     for (int q = j; q < 6; q += L)
-      s_edge[g][q] = c_init_edge[q][0] | (c_init_edge[q][1] << 8) |
-                     (meta_byte(tm, 8 + q) << 16);
+      wedge[q] = c_init_edge[q][0] | (c_init_edge[q][1] << 8) |
+                 (meta_byte(tm, 8 + q) << 16);
   }
   wsync();
   int nb_v = 4, nb_p = 4, nb_e = 6;
@@ -1221,7 +1224,7 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
       const uint32_t c2 = s_cyc[g][q];
       const uint32_t hz = max(hcur, (uint32_t)s_h[g][c2]);
       const uint32_t lo = min((uint32_t)cur, c2), hi = max((uint32_t)cur, c2);
-      s_edge[g][nb_e + q] = lo | (hi << 8) | (hz << 16);
+      wedge[nb_e + q] = lo | (hi << 8) | (hz << 16);
       if (P <= 32 || c2 < 32)
         adj_lo |= 1u << (c2 & 31);
       else
@@ -1244,8 +1247,9 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
           kind = 1;
         } else {
           // adjacency = max over the three edges (first match in list)
-          const uint32_t z1 = (s_edge[g][nb_e + q] >> 16) & 255;
-          const uint32_t z2 = (s_edge[g][nb_e + (q + 1) % nbd] >> 16) & 255;
+          // the new edges (cur, c1), (cur, c2) carry max(h_cur, h_c)
+          const uint32_t z1 = max(hcur, (uint32_t)s_h[g][c1]);
+          const uint32_t z2 = max(hcur, (uint32_t)s_h[g][c2]);
           const uint32_t lo = min(c1, c2), hi = max(c1, c2);
           // the edge (lo, hi), without a scan of the list: a pair of tet
           // faces is one of the six initial records (list positions 0-5,
@@ -1256,8 +1260,7 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
           uint32_t z3 = 0;
           if (found) {
             if (hi < 4)
-              z3 = (s_edge[g][5 - (lo * (7 - lo) / 2 + (hi - lo - 1))] >> 16) &
-                   255;
+              z3 = meta_byte(tm, 8 + 5 - (lo * (7 - lo) / 2 + (hi - lo - 1)));
             else
               z3 = max((uint32_t)s_h[g][lo], (uint32_t)s_h[g][hi]);
           }
@@ -1420,10 +1423,10 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
     }
   }
   if (status == RPD_SUCCESS) {
-    uint32_t* wd = o.words + slot * o.stride_words;
-    for (int v = j; v < nb_v; v += L) wd[v] = s_ver[g][v];
-    for (int p = j; p < nb_p; p += L) wd[nb_v + p] = (uint32_t)s_src[g][p];
-    for (int q = j; q < nb_e; q += L) wd[nb_v + nb_p + q] = s_edge[g][q];
+    // vertices at 0, plane sources at ofs_src (edges are in place)
+    for (int v = j; v < nb_v; v += L) wslot[v] = s_ver[g][v];
+    for (int p = j; p < nb_p; p += L)
+      wslot[o.ofs_src + p] = (uint32_t)s_src[g][p];
   }
   if (a.cert.tet_off && status != kStatusRerunLarge &&
       status != kStatusRerunExact && status != RPD_NO_INTERSECTION) {
@@ -1609,6 +1612,10 @@ __global__ __launch_bounds__(kCB) void k_compact(
   __shared__ uint32_t s_a[kCB + 1];
   __shared__ const uint32_t* s_wsrc[kCB];
   __shared__ const double* s_asrc[kCB];
+  // a slot holds vertices at 0, plane sources at ofs_src, edges at
+  // ofs_edges; the stream holds them back to back
+  __shared__ uint32_t s_lay[kCB];  // nb_v | nb_p << 8 | ofs_src << 16
+  __shared__ uint32_t s_oe[kCB];
   const int i = threadIdx.x;
   const int64_t r = (int64_t)blockIdx.x * kCB + i;
   CellRef x{0, 0, 0};
@@ -1634,6 +1641,8 @@ __global__ __launch_bounds__(kCB) void k_compact(
   const SlotView& sv = tv.t[x.tier];
   s_wsrc[i] = sv.words + x.idx * sv.stride_words;
   s_asrc[i] = sv.areas + x.idx * sv.stride_areas;
+  s_lay[i] = (x.hdr & 0xffffu) | ((uint32_t)sv.ofs_src << 16);
+  s_oe[i] = (uint32_t)sv.ofs_edges;
   if (z) {
     rpd_cell rc;
     rc.tet_id = cand[c].tet;
@@ -1673,7 +1682,12 @@ __global__ __launch_bounds__(kCB) void k_compact(
   uint32_t* wd = out_words + B.words;
   for (uint32_t e = i; e < W; e += kCB) {
     const int k = owner(s_w, e);
-    wd[e] = s_wsrc[k][e - s_w[k]];
+    const uint32_t w = e - s_w[k], lay = s_lay[k];
+    const uint32_t nv = lay & 255, np = (lay >> 8) & 255;
+    const uint32_t src = w < nv ? w
+                         : (w < nv + np ? (lay >> 16) + (w - nv)
+                                        : s_oe[k] + (w - nv - np));
+    wd[e] = s_wsrc[k][src];
   }
   if (geometry) {
     const uint32_t A = s_a[kCB];
