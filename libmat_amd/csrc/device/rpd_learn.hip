@@ -200,7 +200,9 @@ __global__ __launch_bounds__(256) void k_vertex_owner(
 // they are evaluated in binary32 around the centroid c: with u = s - c and
 // the corner offsets d_x = x - c, pd(x, s) - pd_min(x) =
 // (|u|^2 - w) - 2 d_x.u + (|d_x|^2 - pd_min(x)).
-__global__ __launch_bounds__(64) void k_knn_features(
+// (compiled for 4 resident wavefronts per SIMD: measured faster than the
+// 3 its register count otherwise allows)
+__global__ __launch_bounds__(64, 4) void k_knn_features(
     DevMesh M, DevSites S, const InstInfo* inst, const int32_t* tet_inst,
     const int32_t* vowner, int32_t* knn, uint16_t* X, uint32_t* omask) {
   const int t = blockIdx.x * 64 + threadIdx.x;
