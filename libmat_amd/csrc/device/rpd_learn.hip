@@ -502,13 +502,17 @@ __device__ __forceinline__ void mlp_forward_tile(const MlpDev& P,
 constexpr int kS1 = kIn + 8, kS2 = kHid + 8;
 constexpr int kInferWaves = 8;
 
+// The layers run transposed (Z^T = W^T X^T: weights as the A operand,
+// activations as the B operand), so a layer's output fragment in a lane --
+// 4 consecutive features of tet l % 16 -- is exactly the next layer's B
+// fragment in the same lane: activations stay in registers, and only the
+// weights live in LDS (58 KB, two blocks per CU).
 __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(
     MlpDev P, const uint16_t* X, int64_t n, float thr, uint32_t* sel,
     float* logits) {
   __shared__ uint16_t s_w1[kHid * kS1];
   __shared__ uint16_t s_w2[kHid * kS2];
   __shared__ uint16_t s_w3[kOut * kS2];
-  __shared__ uint16_t s_act[kInferWaves][16][kActStride];
   for (int q = threadIdx.x; q < kHid * kIn; q += 64 * kInferWaves)
     s_w1[(q / kIn) * kS1 + q % kIn] = P.w1t[q];
   for (int q = threadIdx.x; q < kHid * kHid; q += 64 * kInferWaves)
@@ -518,86 +522,70 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(
   __syncthreads();
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int col = l % 16, rq = 4 * (l / 16);
-  uint16_t(*act)[kActStride] = s_act[wv];
   const int64_t n_tiles = (n + 15) / 16;
   for (int64_t tile = (int64_t)blockIdx.x * kInferWaves + wv; tile < n_tiles;
        tile += (int64_t)gridDim.x * kInferWaves) {
     const int64_t row0 = tile * 16;
     const int64_t my_row = row0 + col;
-    bf16x4 a1[3];
-    load_x(X, my_row < n ? my_row : -1, l, a1);
+    bf16x4 x[3];
+    load_x(X, my_row < n ? my_row : -1, l, x);
+    // lane l: features 16 nb + rq + i of tet row0 + col
+    auto act = [](f32x4 c, const float* b) {
+      bf16x4 h;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h[i] = (short)f2bf(fmaxf(c[i] + b[i], 0.f));
+      return h;
+    };
+    bf16x4 h1[8];
 #pragma unroll
     for (int nb = 0; nb < 8; ++nb) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int kb = 0; kb < 3; ++kb)
-        acc = mfma(a1[kb], ld_bf4(&s_w1[(16 * nb + col) * kS1 + 16 * kb + rq]),
+        acc = mfma(ld_bf4(&s_w1[(16 * nb + col) * kS1 + 16 * kb + rq]), x[kb],
                    acc);
-      const float b = P.b1[16 * nb + col];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        act[rq + i][16 * nb + col] = f2bf(fmaxf(acc[i] + b, 0.f));
+      h1[nb] = act(acc, P.b1 + 16 * nb + rq);
     }
-    wsync_l();
-    bf16x4 a2[8];
-#pragma unroll
-    for (int kb = 0; kb < 8; ++kb) a2[kb] = ld_bf4(&act[col][16 * kb + rq]);
-    wsync_l();
+    bf16x4 h2[8];
 #pragma unroll
     for (int nb = 0; nb < 8; ++nb) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int kb = 0; kb < 8; ++kb)
-        acc = mfma(a2[kb],
-                   ld_bf4(&s_w2[(16 * nb + col) * kS2 + 16 * kb + rq]), acc);
-      const float b = P.b2[16 * nb + col];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        act[rq + i][16 * nb + col] = f2bf(fmaxf(acc[i] + b, 0.f));
+        acc = mfma(ld_bf4(&s_w2[(16 * nb + col) * kS2 + 16 * kb + rq]), h1[kb],
+                   acc);
+      h2[nb] = act(acc, P.b2 + 16 * nb + rq);
     }
-    wsync_l();
-    bf16x4 a3[8];
-#pragma unroll
-    for (int kb = 0; kb < 8; ++kb) a3[kb] = ld_bf4(&act[col][16 * kb + rq]);
     f32x4 z[2];
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int kb = 0; kb < 8; ++kb)
-        acc = mfma(a3[kb],
-                   ld_bf4(&s_w3[(16 * nb + col) * kS2 + 16 * kb + rq]), acc);
-      const float b = P.b3[16 * nb + col];
+        acc = mfma(ld_bf4(&s_w3[(16 * nb + col) * kS2 + 16 * kb + rq]), h2[kb],
+                   acc);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) acc[i] += b;
+      for (int i = 0; i < 4; ++i) acc[i] += P.b3[16 * nb + rq + i];
       z[nb] = acc;
     }
-    uint64_t bm[4][2];
+    // proposal mask of tet row0 + col: this lane's 8 bits, OR over the four
+    // lanes of the tet (l, l ^ 16, l ^ 32, l ^ 48)
+    uint32_t m = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
-      for (int nb = 0; nb < 2; ++nb) bm[i][nb] = __ballot(z[nb][i] > thr);
-    if (l < 16) {
-      // row l = 4 g + i: lanes 16 g .. 16 g + 15 of ballot (i, nb)
-      const int g = l / 4, i = l % 4;
-      const uint64_t b0 =
-          i == 0 ? bm[0][0] : (i == 1 ? bm[1][0] : (i == 2 ? bm[2][0] : bm[3][0]));
-      const uint64_t b1 =
-          i == 0 ? bm[0][1] : (i == 1 ? bm[1][1] : (i == 2 ? bm[2][1] : bm[3][1]));
-      const uint32_t m = (uint32_t)((b0 >> (16 * g)) & 0xffffu) |
-                         ((uint32_t)((b1 >> (16 * g)) & 0xffffu) << 16);
-      if (row0 + l < n) sel[row0 + l] = m;
-    }
-    if (logits) {
+      for (int i = 0; i < 4; ++i)
+        m |= (uint32_t)(z[nb][i] > thr) << (16 * nb + rq + i);
+    m |= (uint32_t)__shfl_xor((int)m, 16);
+    m |= (uint32_t)__shfl_xor((int)m, 32);
+    if (l < 16 && my_row < n) sel[my_row] = m;
+    if (logits && my_row < n) {
 #pragma unroll
       for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int64_t r = row0 + rq + i;
-          if (r < n) logits[r * kOut + 16 * nb + col] = z[nb][i];
-        }
+        for (int i = 0; i < 4; ++i)
+          logits[my_row * kOut + 16 * nb + rq + i] = z[nb][i];
     }
-    wsync_l();
   }
 }
 
