@@ -160,7 +160,7 @@ struct rpd_ctx {
       tr_p2, tr_p3, tr_pb, tr_grad, tr_x, tr_y, tr_mk;
   Buf flag, has_cell, flag32, flag_pos, flag_list, brute_n, brute_off,
       vals_a, vals_b, perm, miss_a, miss_b, umark, upos, ukeys, upass, ppos,
-      mem_a, mem_b, reopened, out_off, out_cnt, out_ids;
+      mem_a, mem_b, reopened, out_off, out_cnt, out_ids, reop64;
   rpd_stats st;
   int tuned_G = 0, tuned_caps = 0;  // rpd_tune's choice (0: not tuned)
   hipEvent_t ev[16];
@@ -1260,22 +1260,21 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
   (void)c_lo;
   R.cert.tet_off = nullptr;  // the full-relation re-runs are final
   // tets without a cell, with a dropped cell or left open: full relation
+  HIP_TRY(ctx->reop64.need(64 * 4));
+  HIP_TRY(hipMemsetAsync(ctx->reop64.p, 0, 64 * 4, s));
   hipLaunchKernelGGL(k_flag_final, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
                      nt, ctx->flag.as<uint8_t>(), ctx->has_cell.as<uint8_t>(),
-                     ctx->flag32.as<int32_t>());
+                     ctx->reopened.as<uint8_t>(), ctx->flag32.as<int32_t>(),
+                     ctx->reop64.as<int32_t>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemsetAsync(ctx->flag32.as<int32_t>() + nt, 0, 4, s));
   HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->flag32.as<int32_t>(),
                                  ctx->flag_pos.as<int32_t>(), nt + 1));
-  HIP_TRY(hipMemsetAsync(R.cnt + 5, 0, 4, s));
-  hipLaunchKernelGGL(k_count_reopened, dim3(ceil_div(nt, 256)), dim3(256), 0,
-                     s, nt, ctx->reopened.as<uint8_t>(),
-                     ctx->flag32.as<int32_t>(), R.cnt + 5);
-  HIP_TRY(hipGetLastError());
-  int32_t nf = 0, n_reopened = 0;
+  int32_t nf = 0, n_reopened = 0, reop[64];
   HIP_TRY(read_back(ctx, s,
                     {{&nf, ctx->flag_pos.as<int32_t>() + nt, 4},
-                     {&n_reopened, R.cnt + 5, 4}}));
+                     {reop, ctx->reop64.p, sizeof(reop)}}));
+  for (int q = 0; q < 64; ++q) n_reopened += reop[q];
   int64_t n_fb = 0;
   const int64_t n_brute = c_hi;
   if (nf > 0) {

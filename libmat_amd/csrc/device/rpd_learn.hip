@@ -965,31 +965,28 @@ __global__ __launch_bounds__(256) void k_pass32(int64_t n,
   if (q <= n) out[q] = q < *n_dev ? p[q] : 0;
 }
 
+// final flags (failed, dropped or cell-less tets) and the count of tets
+// re-run in any way, added per block to one of 64 counters (blocks
+// spread their atomics over them)
 __global__ __launch_bounds__(256) void k_flag_final(int32_t n_tet,
                                                     uint8_t* flag,
                                                     const uint8_t* has_cell,
-                                                    int32_t* flag32) {
+                                                    const uint8_t* reopened,
+                                                    int32_t* flag32,
+                                                    int32_t* n_reopened64) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_tet) return;
-  const uint8_t f = flag[t] || !has_cell[t];
-  flag[t] = f;
-  flag32[t] = f;
+  bool re = false;
+  if (t < n_tet) {
+    const uint8_t f = flag[t] || !has_cell[t];
+    flag[t] = f;
+    flag32[t] = f;
+    re = f || reopened[t];
+  }
+  const int c = __syncthreads_count(re);
+  if (threadIdx.x == 0 && c) atomicAdd(&n_reopened64[blockIdx.x & 63], c);
 }
 
 // tets re-run in any way (closure or full relation)
-__global__ __launch_bounds__(256) void k_count_reopened(int32_t n_tet,
-                                                        const uint8_t* reopened,
-                                                        const int32_t* flag32,
-                                                        int32_t* n) {
-  __shared__ int32_t c;
-  if (threadIdx.x == 0) c = 0;
-  __syncthreads();
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n_tet && (reopened[t] || flag32[t])) atomicAdd(&c, 1);
-  __syncthreads();
-  if (threadIdx.x == 0 && c) atomicAdd(n, c);
-}
-
 __global__ __launch_bounds__(256) void k_flag_list(int32_t n_tet,
                                                    const int32_t* flag32,
                                                    const int32_t* pos,
@@ -1053,7 +1050,6 @@ __global__ __launch_bounds__(256) void k_b_keys(int64_t n1, int64_t n_all,
   keys[c - n1] =
       out ? ~0ull : (((uint64_t)(uint32_t)cd.tet << 32) | (uint32_t)cd.seed);
   vals[c - n1] = (int32_t)c;
-  if (!out) atomicAdd(n_b, 1);
 }
 
 __device__ __forceinline__ int64_t lower_u64(const uint64_t* keys, int64_t n,
