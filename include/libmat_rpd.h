@@ -119,6 +119,9 @@ struct ConvexCellHost {
   std::vector<int> active_edges;
   Scalar cal_cell_euler();
   Scalar cal_halfplane_facet_euler(int neigh_id);
+  // declared, never defined, in the reference too (voronoi_defs.h:53)
+  Scalar cal_edge_euler(aint2 neigh_min_max);
+  void print_cell_detail_euler() const;
   void reload_active();
 
   bool is_pc_explicit = false;

@@ -1315,12 +1315,11 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
   HIP_TRY(ctx->keys_b.need((size_t)std::max<int64_t>(nB, 1) * 8));
   HIP_TRY(ctx->vals_a.need((size_t)std::max<int64_t>(nB, 1) * 4));
   HIP_TRY(ctx->vals_b.need((size_t)std::max<int64_t>(nB, 1) * 4));
-  HIP_TRY(hipMemsetAsync(R.cnt + 3, 0, 4, s));
   if (nB > 0) {
     hipLaunchKernelGGL(k_b_keys, dim3(ceil_div(nB, 256)), dim3(256), 0, s, n1,
                        n_all, n_brute, ctx->cand.as<Cand>(),
                        ctx->flag.as<uint8_t>(), ctx->keys_a.as<uint64_t>(),
-                       ctx->vals_a.as<int32_t>(), R.cnt + 3);
+                       ctx->vals_a.as<int32_t>());
     HIP_TRY(hipGetLastError());
     size_t tmp = 0;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(

@@ -84,9 +84,11 @@ __device__ __forceinline__ int conflict_filtered(const rpd_d4& p1,
   return 0;
 }
 
+// exact sign of the same determinant: +1 conflict, 0 the vertex lies on
+// the plane (no conflict), -1 no conflict
 __device__ __noinline__ int conflict_exact(rpd_d4 p1, rpd_d4 p2, rpd_d4 p3,
                                            rpd_d4 e) {
-  return rpd_exact_det4_sign(p1, p2, p3, e) > 0 ? 1 : -1;
+  return rpd_exact_det4_sign(p1, p2, p3, e);
 }
 
 // sign of det3(normals) == 0 ? (is_vertex_perturb, convex_cell.cu:274-316)
@@ -981,8 +983,16 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
   int nb_v = 4, nb_p = 4, nb_e = 6;
   uint32_t status = RPD_SUCCESS;
   uint32_t n_det = 0, n_bis = 0, n_exd = 0;  // executed work (count_work)
+  bool on_plane = false;  // exact tier: a vertex lay exactly on a plane
   int k = -1;
   for (;;) {
+    // at the reference's plane capacity every further neighbour overflows,
+    // whether or not it can cut (new_plane checks the count before the
+    // plane is tested, convex_cell.cu:561-565), so a masked one does too
+    if (kFull && nb_p >= P && k + 1 < deg) {
+      status = RPD_VERTEX_OVERFLOW;
+      break;
+    }
     // next neighbour position that may cut the cell
     {
       int nk = k + 1;
@@ -1028,7 +1038,9 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
         const rpd_d4 p3 = ld4(&s_pl[g][(w >> 16) & 255]);
         c = a.force_exact ? 0 : conflict_filtered(p1, p2, p3, e);
         if (kExact && c == 0) {
-          c = conflict_exact(p1, p2, p3, e);
+          const int sg = conflict_exact(p1, p2, p3, e);
+          on_plane = on_plane || sg == 0;
+          c = sg > 0 ? 1 : -1;
           ++n_exd;
         }
       }
@@ -1427,6 +1439,15 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
     for (int v = j; v < nb_v; v += L) wslot[v] = s_ver[g][v];
     for (int p = j; p < nb_p; p += L)
       wslot[o.ofs_src + p] = (uint32_t)s_src[g][p];
+  }
+  if constexpr (kExact) {
+    // a vertex exactly on a plane: the configuration is degenerate and a
+    // sphere whose cell meets the tet only in a face, an edge or a point
+    // (a zero-volume cell, which the plain path emits) may touch this cell
+    // without leaving an active facet here; the certificate cannot see it,
+    // so the tet goes to the full relation (the exact tier runs one cell
+    // per wavefront, so the ballot covers exactly this cell's lanes)
+    if (a.cert.tet_off && ballot(on_plane) != 0 && j == 0) a.cert.flag[t] = 1;
   }
   if (a.cert.tet_off && status != kStatusRerunLarge &&
       status != kStatusRerunExact && status != RPD_NO_INTERSECTION) {

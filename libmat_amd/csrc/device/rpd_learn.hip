@@ -1041,8 +1041,7 @@ __global__ __launch_bounds__(256) void k_b_keys(int64_t n1, int64_t n_all,
                                                 int64_t n_brute,
                                                 const Cand* cand,
                                                 const uint8_t* flag,
-                                                uint64_t* keys, int32_t* vals,
-                                                int32_t* n_b) {
+                                                uint64_t* keys, int32_t* vals) {
   const int64_t c = n1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_all) return;
   const Cand cd = cand[c];

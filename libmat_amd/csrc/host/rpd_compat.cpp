@@ -313,16 +313,20 @@ void ConvexCellHost::reload_pc_explicit() {
   is_pc_explicit = true;
 }
 
+// Euler sums with the reference's rounding (voronoi_defs.cxx:190-267):
+// vertex terms are binary32 quotients 1.f / #adjacent cells, face and edge
+// terms binary64 quotients, accumulated in binary64 (Scalar), and the
+// result passes through a float
 Scalar ConvexCellHost::cal_cell_euler() {
   Scalar sv = 0, sf = 0, se = 0;
   if (!is_active_updated) reload_active();
-  for (int v = 0; v < nb_v; ++v) sv += 1.0 / (int)ver_trans(v).w;
+  for (int v = 0; v < nb_v; ++v) sv += 1.f / (int)ver_trans(v).w;
   for (int f = 0; f < nb_p; ++f)
-    if (active_clipping_planes[f] > 0) sf += 1.0 / clip_trans(f).h;
+    if (active_clipping_planes[f] > 0) sf += 1. / clip_trans(f).h;
   for (int e = 0; e < nb_e; ++e)
-    if (active_edges[e] > 0) se += 1.0 / edge(e).z;
+    if (active_edges[e] > 0) se += 1. / edge(e).z;
   euler = (float)(sv - se + sf);
-  return sv - se + sf;
+  return euler;
 }
 
 Scalar ConvexCellHost::cal_halfplane_facet_euler(int neigh_id) {
@@ -341,15 +345,38 @@ Scalar ConvexCellHost::cal_halfplane_facet_euler(int neigh_id) {
   for (int v = 0; v < nb_v; ++v) {
     cuchar4 t = ver_trans(v);
     if (t.x != clip_id && t.y != clip_id && t.z != clip_id) continue;
-    sv += 1.0 / (int)t.w;
+    sv += 1.f / (int)t.w;
   }
   for (int e = 0; e < nb_e; ++e) {
     cuchar3 ed = edge(e);
     if (ed.x != clip_id && ed.y != clip_id) continue;
     if (active_edges[e] <= 0) continue;
-    se += 1.0 / ed.z;
+    se += 1. / ed.z;
   }
-  return (float)(sv - se + sf);
+  const float facet_euler = (float)(sv - se + sf);
+  return facet_euler;
+}
+
+void ConvexCellHost::print_cell_detail_euler() const {
+  // (voronoi_defs.cxx:269-291) needs reload_active() first
+  std::printf("--------- \n");
+  for (int v = 0; v < nb_v; ++v)
+    std::printf("[Detail] seed %d id %d has vertex %d with #adj_cells: %d \n",
+                voro_id, id, v, (int)ver_data_trans[v].w);
+  for (int e = 0; e < nb_e; ++e) {
+    if (e >= (int)active_edges.size() || active_edges[e] <= 0) continue;
+    std::printf("[Detail] seed %d id %d has edge %d: (%d, %d) with "
+                "#adj_cells: %d\n",
+                voro_id, id, e, (int)edge_data[e].x, (int)edge_data[e].y,
+                (int)edge_data[e].z);
+  }
+  for (int f = 0; f < nb_p; ++f) {
+    if (f >= (int)active_clipping_planes.size() ||
+        active_clipping_planes[f] <= 0)
+      continue;
+    std::printf("[Detail] seed %d id %d has plane %d with #adj_cells: %f \n",
+                voro_id, id, f, clip_data_trans[f].h);
+  }
 }
 
 bool is_convex_cell_valid(const ConvexCellHost& cell) {

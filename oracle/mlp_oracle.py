@@ -51,8 +51,9 @@ def pack(d):
                            for n, _ in SHAPES])
 
 
-def loss_and_grad(params, X, Y, M, pos_weight, rounding=True):
-    """X: (n, 48) float64 inputs (already bf16 values when rounding)."""
+def _layers(params, X, rounding):
+    """Weights as the device holds them and the forward pass: returns
+    (W1, W2, W3, h1, h2, z3)."""
     r = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
     P = unpack(params)
     W1, W2, W3 = r(P["W1"]), r(P["W2"]), r(P["W3"])
@@ -60,10 +61,26 @@ def loss_and_grad(params, X, Y, M, pos_weight, rounding=True):
     b2 = P["b2"].astype(np.float32).astype(np.float64) if rounding else P["b2"]
     b3 = P["b3"].astype(np.float32).astype(np.float64) if rounding else P["b3"]
     X = np.asarray(X, np.float64)
-    n = X.shape[0]
     h1 = r(np.maximum(X @ W1 + b1, 0.0))
     h2 = r(np.maximum(h1 @ W2 + b2, 0.0))
     z3 = h2 @ W3 + b3
+    return W1, W2, W3, h1, h2, z3
+
+
+def forward(params, X, rounding=True):
+    """Logits (n, 32) of the model for inputs X (n, 48); with ``rounding``
+    the weights and hidden activations pass through bfloat16 as on the
+    device (the inference kernel's rounding points are the training
+    kernel's)."""
+    return _layers(params, X, rounding)[5]
+
+
+def loss_and_grad(params, X, Y, M, pos_weight, rounding=True):
+    """X: (n, 48) float64 inputs (already bf16 values when rounding)."""
+    r = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
+    W1, W2, W3, h1, h2, z3 = _layers(params, X, rounding)
+    X = np.asarray(X, np.float64)
+    n = X.shape[0]
     inv = 1.0 / (n * OUT)
     w = np.where(Y > 0.5, pos_weight, 1.0)
     p = 1.0 / (1.0 + np.exp(-z3))

@@ -62,6 +62,9 @@ def lib():
         L.orc_get_cells.argtypes = [vp, vp]
         L.orc_get_drops.argtypes = [vp, vp]
         L.orc_get_counters.argtypes = [vp, vp]
+        L.orc_adjacency.argtypes = [ctypes.c_char_p, ctypes.c_char_p,
+                                    ctypes.c_int32, ctypes.c_char_p]
+        L.orc_adjacency.restype = ctypes.c_int
         L.orc_exact_det4_sign.argtypes = [vp]
         L.orc_exact_det4_sign.restype = ctypes.c_int
         assert L.orc_cell_bytes() == CELL_DTYPE.itemsize, \
@@ -148,3 +151,13 @@ def run(inst, tets=None, mode=0, brute_force=False, n_threads=8):
 def exact_det4_sign(m):
     m = np.ascontiguousarray(m, np.float64).reshape(16)
     return lib().orc_exact_det4_sign(_p(m))
+
+
+def adjacency(cell_path, es2fe_path, max_surf_fid, out_path):
+    """Per-sphere power-cell adjacency and Euler sums of the cells in a cell
+    file (format in adj_oracle.cpp), written as the canonical text dump."""
+    rc = lib().orc_adjacency(str(cell_path).encode(),
+                             str(es2fe_path).encode() if es2fe_path else b"",
+                             int(max_surf_fid), str(out_path).encode())
+    if rc != 0:
+        raise RuntimeError(f"orc_adjacency failed ({rc})")

@@ -20,6 +20,29 @@
     }                                                          \
   } while (0)
 
+// cell file (the format of oracle/adj_oracle.cpp) of the drop-in's cells
+static bool write_cells(const char* path,
+                        const std::vector<ConvexCellHost>& cells) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return false;
+  const int32_t hdr[2] = {1, (int32_t)cells.size()};
+  std::fwrite("RPDC", 1, 4, f);
+  std::fwrite(hdr, 4, 2, f);
+  for (const auto& c : cells) {
+    const int32_t h[8] = {c.id,        c.voro_id, c.tet_id, c.thread_id,
+                          (int)c.status, c.nb_v,  c.nb_p,   c.nb_e};
+    std::fwrite(h, 4, 8, f);
+    std::fwrite(&c.weight, 4, 1, f);
+    std::fwrite(&c.cell_vol, 4, 1, f);
+    for (int v = 0; v < c.nb_v; ++v) std::fwrite(&c.ver_data_trans[v].x, 1, 4, f);
+    for (int p = 0; p < c.nb_p; ++p) std::fwrite(&c.clip_data_trans[p].x, 4, 5, f);
+    for (int p = 0; p < c.nb_p; ++p)
+      std::fwrite(&c.clip_id2_data_trans[p].x, 4, 2, f);
+    for (int e = 0; e < c.nb_e; ++e) std::fwrite(&c.edge_data[e].x, 1, 3, f);
+  }
+  return std::fclose(f) == 0;
+}
+
 int main(int argc, char** argv) {
   // byte layout of the reference record (GCC 11, x86-64)
   CHECK(sizeof(ConvexCellHost) == 3776);
@@ -70,6 +93,9 @@ int main(int argc, char** argv) {
   rpd.set_spheres(&spheres);
   rpd.calculate();
   const auto& cells = rpd.powercells;
+  // --gpu --dump PATH: the cells of this first calculate() to PATH
+  if (argc >= 4 && std::strcmp(argv[2], "--dump") == 0)
+    CHECK(write_cells(argv[3], cells));
   std::printf("powercells %zu n_site %d site_k %d\n", cells.size(), rpd.n_site,
               rpd.site_k);
   CHECK(!cells.empty());

@@ -151,9 +151,70 @@ def test_cube64_sample_and_partition(ctx):
     print(f"[cube64] sample {m}", flush=True)
 
 
+def compare_dropin_with_oracle(cells, inst, site2sphere, orc):
+    """ConvexCellHost records of RPD3D_GPU::calculate against the oracle's
+    cells of the same instance: order, ids, thread_id = tet * tet_k + slot,
+    dual vertices, edges and plane ids bit-exact, planes by what they must
+    satisfy in binary32, cell volumes to binary32 precision."""
+    oc = orc.cells
+    assert len(cells) == oc.size
+    rel_n = np.diff(orc.rel_off)
+    tet_k = int(rel_n.max())
+    rng = np.random.default_rng(0)
+    max_vol_err = 0.0
+    for q, (c, o) in enumerate(zip(cells, oc)):
+        t, s = int(o["tet"]), int(o["seed"])
+        nv, np_, ne = int(o["nb_v"]), int(o["nb_p"]), int(o["nb_e"])
+        assert c["id"] == q and c["tet_id"] == t, q
+        assert c["voro_id"] == site2sphere[s], q
+        assert c["status"] == 4, q
+        slot = int(np.nonzero(orc.rel_ids[orc.rel_off[t]:orc.rel_off[t + 1]]
+                              == s)[0][0])
+        assert c["thread_id"] == t * tet_k + slot, q
+        assert (c["nb_v"], c["nb_p"], c["nb_e"]) == (nv, np_, ne), q
+        assert np.array_equal(c["ver"], o["ver"][:nv]), q
+        assert np.array_equal(c["edge"], o["edge"][:ne] & 0xFFFFFF), q
+        w = np.float32(inst.sites[s, 3])
+        assert np.float32(c["weight"]) == w, q
+        for p in range(np_):
+            src = int(o["src"][p])
+            a, b, cc, d, h = (float(x) for x in c["clip"][p])
+            if src < 0:
+                k = -1 - src
+                assert tuple(c["id2"][p]) == (int(inst.f_id[t, k]), -1), q
+                assert h == float(inst.f_adj[t, k])
+                face = [inst.tets[t, i] for i in ((2, 1, 3), (0, 2, 3),
+                                                  (1, 0, 3), (0, 1, 2))[k]]
+                n = np.sqrt(a * a + b * b + cc * cc)
+                for v in face:
+                    x = inst.verts[v]
+                    assert abs(a * x[0] + b * x[1] + cc * x[2] + d) <= \
+                        1e-6 * n, (q, p)
+                x = inst.verts[inst.tets[t, k]]
+                assert a * x[0] + b * x[1] + cc * x[2] + d > 0, (q, p)
+            else:
+                lo, hi = min(s, src), max(s, src)
+                assert tuple(c["id2"][p]) == (site2sphere[lo],
+                                              site2sphere[hi]), q
+                assert h == 1.0
+                S, B = inst.sites[s], inst.sites[src]
+                for x in rng.random((2, 3)):
+                    pd_s = ((x - S[:3]) ** 2).sum() - S[3]
+                    pd_b = ((x - B[:3]) ** 2).sum() - B[3]
+                    val = a * x[0] + b * x[1] + cc * x[2] + d
+                    assert abs(val - 0.5 * (pd_b - pd_s)) <= 1e-6, (q, p)
+        vo = float(o["geo"][0])
+        err = abs(c["cell_vol"] - vo) / max(abs(vo), 1e-30)
+        max_vol_err = max(max_vol_err, err)
+    assert max_vol_err < 1e-6
+    return {"cells": len(cells), "tet_k": tet_k,
+            "max_cell_vol_rel_err": max_vol_err}
+
+
 def test_cpp_dropin_on_gpu(tmp_path):
     import os
     import subprocess
+    from cellfile import dropin_instance, read_cells, write_cube_es2fe
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     lib = os.path.join(root, "libmat_amd", "_lib")
     exe = tmp_path / "compat_test"
@@ -162,10 +223,35 @@ def test_cpp_dropin_on_gpu(tmp_path):
                     os.path.join(root, "tests/cpp/compat_test.cpp"), "-o",
                     str(exe), "-L" + lib, "-lrpd_amd", "-Wl,-rpath," + lib],
                    check=True)
-    out = subprocess.run([str(exe), "--gpu"], capture_output=True, text=True,
-                         timeout=120)
+    dump = tmp_path / "dropin_cells.bin"
+    out = subprocess.run([str(exe), "--gpu", "--dump", str(dump)],
+                         capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr, flush=True)
     assert out.returncode == 0 and "gpu ok" in out.stdout
+    # the drop-in's cells against the oracle on the instance it builds
+    inst, site2sphere = dropin_instance(0)
+    orc = oracle.run(inst)
+    cells = read_cells(dump)
+    m = compare_dropin_with_oracle(cells, inst, site2sphere, orc)
+    print(f"[dropin] {m}", flush=True)
+    # and the adjacency the consumers read, on these very cells, against
+    # the oracle's restatement of get_all_voro_info
+    adj = tmp_path / "adj_dump"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off",
+                    "-I" + os.path.join(root, "include"),
+                    os.path.join(root, "tests/cpp/adj_dump.cpp"), "-o",
+                    str(adj), "-L" + lib, "-lrpd_amd", "-Wl,-rpath," + lib],
+                   check=True)
+    ef = tmp_path / "es2fe.bin"
+    write_cube_es2fe(ef, inst)
+    pd_, od = tmp_path / "product.txt", tmp_path / "oracle.txt"
+    subprocess.run([str(adj), str(dump), str(ef), str(inst.n_sf - 1),
+                    str(pd_)], check=True, timeout=120)
+    oracle.adjacency(dump, ef, inst.n_sf - 1, od)
+    a, b = open(pd_).read(), open(od).read()
+    print(f"[dropin] adjacency dump {len(a.splitlines())} lines, identical "
+          f"{a == b}", flush=True)
+    assert a == b
 
 
 def test_work_counters_match_the_oracle(ctx):
@@ -258,6 +344,15 @@ def test_batch_matches_single_instance_runs(ctx):
             single.run()
             ra = _single_arrays(single.download(), *single.relation())
             _same(_instance_slice(res, off, ids, to, so, b), ra)
+        # two instances against the oracle as well
+        from test_gpu_configs import cells_of, subset
+        for b in (0, len(insts) - 1):
+            t0, t1, s0 = int(to[b]), int(to[b + 1]), int(so[b])
+            orc = oracle.run(insts[b])
+            assert np.array_equal(off[t0:t1 + 1] - off[t0], orc.rel_off)
+            assert np.array_equal(ids[off[t0]:off[t1]] - s0, orc.rel_ids)
+            sel = cells_of(res, np.arange(t0, t1))
+            compare_with_oracle(subset(res, sel, t0, s0), orc)
         print(f"[batch] iteration {it}: {len(res)} cells identical to the "
               f"single-instance runs", flush=True)
         # next sphere-update iteration: centres jittered, host neighbours

@@ -48,3 +48,34 @@ def test_gradient_matches_central_differences():
         fd = (mo.loss_only(p1, X, Y, M, 4.0) -
               mo.loss_only(p0, X, Y, M, 4.0)) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-6 * max(1e-3, abs(g[i])) + 1e-9, i
+
+
+def test_forward_closed_form():
+    # one path through the network: x0 = 2 -> h1[0] = relu(3*2) = 6 ->
+    # h2[1] = relu(0.5*6 + 1) = 4 (W2[0][1]; W2 is [in][out]) -> z[5] =
+    # -1*4 + 0.25; a negative pre-activation is cut by the ReLU (h1[2])
+    d = {n: np.zeros(s) for n, s in mo.SHAPES}
+    d["W1"][0, 0] = 3.0
+    d["W1"][0, 2] = -1.0
+    d["W2"][0, 1] = 0.5
+    d["W2"][2, 1] = 100.0
+    d["b2"][1] = 1.0
+    d["W3"][1, 5] = -1.0
+    d["b3"][5] = 0.25
+    d["b3"][7] = -2.0
+    X = np.zeros((2, mo.IN))
+    X[0, 0] = 2.0
+    z = mo.forward(mo.pack(d), X)
+    want = np.zeros((2, mo.OUT))
+    want[:, 7] = -2.0
+    want[0, 5] = -4.0 + 0.25
+    want[1, 5] = -1.0 + 0.25  # h2[1] = relu(0 + 1) = 1 for the zero row
+    assert np.array_equal(z, want)
+    # bfloat16 rounding of the activations: x0 = 2.015625 (a bf16 value)
+    # gives 3 * x0 = 6.046875, which needs 8 fraction bits and rounds
+    # (half to even) to 6.0625; unrounded the logit differs
+    X[0, 0] = 2.015625
+    z = mo.forward(mo.pack(d), X)
+    assert z[0, 5] == -(0.5 * 6.0625 + 1.0) + 0.25
+    z = mo.forward(mo.pack(d), X, rounding=False)
+    assert z[0, 5] == -(0.5 * 6.046875 + 1.0) + 0.25
