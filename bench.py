@@ -1,26 +1,28 @@
 """Benchmark of the RPD hot path on one MI355X.
 
-Default workload: the unit cube as a 64^3 jittered Kuhn grid (1,572,864
-tets) with 50,000 spheres under the boundary radius rule (18,114 of them
-non-hidden), binary64, every input already in HBM.  A step is one rpd_run
-over the resident instance.  Two paths are timed on the same instance: the
-plain path (candidates from the full tet-sphere relation) and the learned
-path (candidates proposed by the MLP, trained beforehand on the GPU on two
-other seeds of the same generator, then certified, with the missed cells
-re-clipped); ``value`` is the learned path's rate, whose output is checked
-bit-identical to the plain path's.  The same line carries
-``learned_batched``: 256 instances in one batch over 3 sphere-update
-iterations, learned+batched against plain batched and against the
-instances run one at a time.  ``--workload batched`` times 256
-instances in one batch over sphere-update iterations (plain batched, learned
-+batched and one-at-a-time plain).  Prints one JSON line.
+Default line (``--workload headline``): learned+batched against plain on
+the clustered-sphere stress batch -- 128 independent instances of
+50k-500k tets (jittered Kuhn grids) with 5k-40k spheres from a
+32-component Gaussian mixture, 25.4 M tets in all, resident in HBM as one
+batch.  A step is one rpd_run over the whole batch.  ``value`` is the
+learned+batched rate (cells/s): candidates proposed by the MLP (trained
+beforehand on the GPU, by the library's own kernels, on 16 other instances
+of the same generator), certified, the missed cells re-clipped, the output
+checked bit-identical to the plain batched step's.  ``plain`` holds the
+plain path batched and run one instance at a time, ``fallback_pct`` the
+re-run tets.  Sub-keys: ``cube64`` (one 64^3 instance, 1.57 M tets, plain
+vs learned) and ``config3`` (256 instances of 2k-200k tets over 8
+sphere-update iterations).  ``cpu_baseline``: the reference algorithm on
+the host cores, at one thread and at all threads, on a sample of the same
+batch.  Prints one JSON line.
 
-roofline: the dominant kernel is the small-tier clip kernel, measured on the
-plain step's launch.  Its algorithmic bytes per launch are the compulsory
-reads and writes of the cells it clips (clip_bytes), its algorithmic flops
-the FP64 work it executes (clip_flops, counted by the kernel in an extra
-untimed run); both are divided by its average launch duration measured with
-HIP events on the context stream.
+roofline: the dominant kernel of the learned+batched step, its average
+launch duration measured with HIP events on the context stream over all of
+its launches in a step.  For the clip kernel (FP64 VALU work) achieved =
+the algorithmic FP64 flops it executes (45 per 4x4 conflict determinant, 14
+per bisector, 59 per output vertex, counted by the kernel in an extra
+untimed run) over that time; the HBM bytes per launch it moves come from
+rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/).
 """
 import argparse
 import json
@@ -43,31 +45,35 @@ CAPS_NAMES = {0: "Caps<32, 16, 48>", 1: "Caps<48, 24, 72>",
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells-per-wave", type=int, default=0,
                     help="cells per wavefront of the clip kernel; 0 = the "
                          "autotuner's choice on the instance (untimed)")
     ap.add_argument("--small-caps", type=int, default=0,
                     help="small-tier capacities with --cells-per-wave > 0")
-    ap.add_argument("--workload", default="cube64",
-                    choices=["cube10", "cube64", "batched", "gmm",
-                             "heldout10"])
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "cube10", "cube64", "batched",
+                             "gmm", "heldout10"])
     ap.add_argument("--no-learned", action="store_true",
                     help="time only the plain path")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-iterations", type=int, default=8)
-    ap.add_argument("--no-batched", action="store_true",
-                    help="cube64: skip the embedded learned+batched run")
-    ap.add_argument("--embedded-batch-iterations", type=int, default=3,
-                    help="sphere-update iterations of the learned+batched "
-                         "run embedded in the cube64 line")
+    ap.add_argument("--no-cube64", action="store_true",
+                    help="headline: skip the cube64 sub-key")
+    ap.add_argument("--no-config3", action="store_true",
+                    help="headline: skip the config3 sub-key")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="gmm: time only the learned+batched step (the "
+                         "command the rocprofv3 summaries are taken of)")
     ap.add_argument("--batch-model", default="cube10",
                     choices=["same", "cube10"],
                     help="batched: train the proposal model on other "
                          "instances of the batch generator, or on the 10^3 "
                          "Kuhn grid generator (seeds 0-63)")
-    ap.add_argument("--cpu-sample-tets", type=int, default=300000)
+    ap.add_argument("--cpu-sample-tets", type=int, default=200000,
+                    help="tets of the CPU baseline's sample at all threads "
+                         "(1/16 of it at one thread)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-work-count", action="store_true",
@@ -75,8 +81,11 @@ def parse():
                          "(keeps profiler averages to the timed launches)")
     ap.add_argument("--traffic-json", default=os.path.join(
         ROOT, "profiles", "clip_traffic_cube64.json"),
-        help="per-launch HBM traffic of the clip kernel measured by "
+        help="cube64: per-launch HBM traffic of the clip kernel measured by "
              "rocprofv3 --pmc (FETCH_SIZE / WRITE_SIZE, copy-calibrated)")
+    ap.add_argument("--traffic-json-gmm", default=os.path.join(
+        ROOT, "profiles", "clip_traffic_gmm128.json"),
+        help="config 5: the same for the learned step's dominant kernel")
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="also run one device copy of this many GiB (the "
                          "plain copy kernel that calibrates PMC counters)")
@@ -105,38 +114,72 @@ def clip_flops(w):
     return 45 * w[0] + 14 * w[1] + 59 * w[2]
 
 
-def cpu_baseline(inst, sample_tets, threads):
-    """The reference algorithm on the host cores: for a random tet sample,
-    the dense tet-sphere relation over every site (voronoi.cu:154-193) and
-    the clip of each related cell by its full neighbour list in reference
-    arithmetic (float planes, FP64 determinant; convex_cell.cu:116-774),
-    multithreaded over tets.  value = cells / (relation + clip time)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def _cpu_rate(insts, n_tets, threads, rng):
+    """Cells/s of the reference algorithm on n_tets random tets spread over
+    insts in proportion to their sizes: dense relation over every site, then
+    the clip of each related cell by its full neighbour list (best of 2)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    rng = np.random.default_rng(12345)
-    tets = np.sort(rng.choice(inst.n_tet, min(sample_tets, inst.n_tet),
-                              replace=False)).astype(np.int32)
-    oracle.run(inst, tets=tets[:500], mode=1, n_threads=threads)  # warm-up
-    t0 = time.perf_counter()
-    rel = oracle.run(inst, tets=tets, mode=1, n_threads=threads)
-    t_rel = time.perf_counter() - t0
-    t_clip = None
-    for _ in range(3):
+    sizes = np.array([i.n_tet for i in insts], np.float64)
+    per = np.maximum(1, np.round(n_tets * sizes / sizes.sum())).astype(int)
+    t_rel = t_clip = 0.0
+    cells = tets_done = 0
+    for inst, k in zip(insts, per):
+        tets = np.sort(rng.choice(inst.n_tet, min(int(k), inst.n_tet),
+                                  replace=False)).astype(np.int32)
+        oracle.run(inst, tets=tets[:64], mode=1, n_threads=threads)  # warm
         t0 = time.perf_counter()
-        r = oracle.clip(inst, tets, rel.rel_off, rel.rel_ids, mode=1,
-                        n_threads=threads)
-        dt = time.perf_counter() - t0
-        t_clip = dt if t_clip is None else min(t_clip, dt)
-    n = r.cells.size
-    return {"value": n / (t_rel + t_clip), "unit": "cells/s",
-            "cores": threads, "kind": "port",
-            "clip_only_value": n / t_clip,
-            "sample": f"{tets.size} random tets of the same instance "
-                      f"({n} cells): dense relation over all "
-                      f"{inst.n_site} sites {t_rel:.2f} s + clip by full "
-                      f"neighbour lists in reference float arithmetic "
-                      f"{t_clip:.2f} s (best of 3), {threads} threads"}
+        rel = oracle.run(inst, tets=tets, mode=1, n_threads=threads)
+        t_rel += time.perf_counter() - t0
+        best = None
+        for _ in range(2):
+            t0 = time.perf_counter()
+            r = oracle.clip(inst, tets, rel.rel_off, rel.rel_ids, mode=1,
+                            n_threads=threads)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        t_clip += best
+        cells += r.cells.size
+        tets_done += tets.size
+    return cells / (t_rel + t_clip), cells, tets_done, t_rel, t_clip
+
+
+def cpu_baseline(insts, sample_tets, threads, label):
+    """The reference algorithm on the host cores (the reference has no CPU
+    RPD: src/CPU/voronoi_fallback.cpp:21-22 is an empty stub, so the
+    oracle's restatement in reference arithmetic -- float planes, FP64
+    determinant, convex_cell.cu:116-774; dense relation, voronoi.cu:154-193
+    -- is the baseline), multithreaded over tets, on a random sample of
+    the benchmark's own instances; once at ``threads`` threads and once at
+    one thread on 1/16 of the sample."""
+    import numpy as np
+    rng = np.random.default_rng(12345)
+    v, n, nt, tr, tc = _cpu_rate(insts, sample_tets, threads, rng)
+    v1, n1, nt1, tr1, tc1 = _cpu_rate(insts, max(1, sample_tets // 16), 1,
+                                      rng)
+    return {"value": v, "unit": "cells/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "host_threads": os.cpu_count(),
+            "one_thread": {"value": v1, "cores": 1,
+                           "sample": f"{nt1} tets ({n1} cells): relation "
+                                     f"{tr1:.2f} s + clip {tc1:.2f} s"},
+            "sample": f"{nt} random tets of {label} ({n} cells): dense "
+                      f"relation over all sites of each instance {tr:.2f} s "
+                      f"+ clip by full neighbour lists in reference float "
+                      f"arithmetic {tc:.2f} s (best of 2), {threads} "
+                      f"threads"}
 
 
 def same_result(a, b):
@@ -170,7 +213,8 @@ def train_model(instances, epochs, batch, recall=0.999):
 def time_batch(ctx, sets, warmup, **kw):
     """Runs of one resident batch: site set ``sets[i]`` for step i (None =
     keep the resident sites).  Returns per-step device ms, cells, learned
-    counters summed, and the H2D seconds of the site updates."""
+    counters summed, and the H2D seconds of the site updates; the averages
+    of the per-kernel times are left in time_batch.kernel."""
     if sets[0] is not None:
         ctx.update_sites_batch(sets[0])
     for _ in range(warmup):
@@ -185,34 +229,106 @@ def time_batch(ctx, sets, warmup, **kw):
         st = ctx.run(**kw)
         ms.append(st["ms_total"])
         cells.append(st["n_cells"])
-        kern.append(list(st["ms_kernel"]) + list(st["ms_phase"][:5]))
+        kern.append(st)
         lrn = [x + y for x, y in zip(lrn, st["learned"])]
-    time_batch.kernel_ms = [round(float(x), 3) for x in
-                            (sum(k[i] for k in kern) / len(kern)
-                             for i in range(len(kern[0])))]
+    n = len(kern)
+
+    def avg(f):
+        return sum(f(k) for k in kern) / n
+    if kw.get("learned"):
+        names = ["features_phase", "mlp", "proposal_count", "proposal_fill",
+                 "plane_masks"]
+    else:
+        names = ["site_boxes", "binning", "relation_count", "relation_fill",
+                 "plane_masks"]
+    km = {nm: avg(lambda k, i=i: k["ms_kernel"][i])
+          for i, nm in enumerate(names)}
+    km.update({
+        "clip_first_round": avg(lambda k: k["ms_kernel"][5]),
+        "clip_small_tier_all_rounds": avg(lambda k: k["ms_clip_small_sum"]),
+        "clip_small_tier_launches": avg(lambda k: k["n_clip_small"]),
+        "capacity_and_exact_reruns": avg(lambda k: k["ms_phase"][2]),
+        "compaction": avg(lambda k: k["ms_phase"][3])})
+    if kw.get("learned"):
+        km["features_kernel"] = avg(lambda k: k["ms_features_kernel"])
+        km["certificate_closure_fallback"] = avg(lambda k: k["ms_phase"][4])
+    time_batch.kernel = {k: round(float(v), 4) for k, v in km.items()}
     return ms, cells, lrn, t_sites
+
+
+def feature_bytes(insts, n_knn):
+    """Compulsory HBM bytes of one proposal/feature launch: per tet its
+    vertex ids (16 B) and instance id (4 B) read, its 32 proposals (128 B),
+    48 bf16 features (96 B) and owner mask (4 B) written; per vertex its
+    coordinates (24 B) and owner (4 B); per site 32 B; per neighbour-list
+    entry its id and the neighbour's sphere (36 B)."""
+    n_tet = sum(i.n_tet for i in insts)
+    n_vert = sum(i.n_vert for i in insts)
+    n_site = sum(i.n_site for i in insts)
+    return n_tet * (16 + 4 + 128 + 96 + 4) + n_vert * 28 + n_site * 32 + \
+        n_knn * 36
+
+
+def roofline_of_step(kernel_ms, work, n_launch, insts, G, caps, traffic_json):
+    """Roofline of the dominant kernel of a learned step: the small-tier
+    clip kernel (FP64 VALU work: achieved = algorithmic flops over its
+    launches' summed time) or the proposal/feature kernel (compulsory HBM
+    bytes over its time), whichever takes longer."""
+    clip_ms = kernel_ms["clip_small_tier_all_rounds"]
+    feat_ms = kernel_ms.get("features_kernel", 0.0)
+    name = f"k_clip<{G}, {CAPS_NAMES[caps]}, false, true>"
+    traffic = None
+    if traffic_json and os.path.exists(traffic_json):
+        with open(traffic_json) as f:
+            tj = json.load(f)
+        if tj.get("kernel", "").replace("rpd::", "").replace(
+                "void ", "").startswith(name if clip_ms >= feat_ms else
+                                        "k_knn_features"):
+            traffic = tj.get("bytes_per_launch")
+    if clip_ms >= feat_ms:
+        flops = clip_flops(work)
+        achieved = flops / (clip_ms * 1e-3) / 1e12
+        return {"bound": "valu", "achieved": achieved,
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                "kernel": name, "launches_per_step": n_launch,
+                "kernel_ms": clip_ms / max(1, n_launch),
+                "flops_per_step": flops, "n_det": work[0],
+                "n_bisector": work[1], "n_vout": work[2],
+                "note": "FP64 VALU-issue-bound (SQ counters in profiles/); "
+                        "achieved = executed algorithmic FP64 flops of the "
+                        "step's launches of this kernel / their summed "
+                        "device time; kernel_ms = average per launch"}
+    n_knn = sum(int(i.knn_ids.size) for i in insts)
+    b = feature_bytes(insts, n_knn)
+    achieved = b / (feat_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+            "traffic": traffic, "kernel": "k_knn_features",
+            "launches_per_step": 1, "kernel_ms": feat_ms, "alg_bytes": b}
 
 
 def bench_batched(a, emit=True):
     """Many independent instances in one resident batch.
 
-    ``batched``: 256 instances (tets log-uniform in [2k, 200k] as jittered
-    Kuhn grids, spheres log-uniform in [100, 20k], r ~ U[0.02, 0.08]) over
-    sphere-update iterations with centre jitter N(0, 0.002^2); the learned
-    model is trained on the 10^3 Kuhn grid generator, seeds 0-63 (or, with
-    --batch-model same, on 32 other instances of the batch generator: fewer
-    re-runs, 1.9 % instead of 3.7 % of tets, but more first-round
-    candidates, 1 % slower in all).  ``gmm``: 128 instances of 50k-500k tets with 5k-40k clustered
-    spheres (32-component Gaussian mixture), ``--steps`` runs of the same
-    batch; the model is trained on 16 instances of the same generator with
-    another seed.  A step's time is the device time of rpd_run (HIP events);
-    per-iteration neighbour lists are built beforehand (device builder,
-    rpd_power_neighbors_gpu) and their H2D copy is reported apart.  Three paths on the same inputs: plain
-    batched, learned+batched (checked bit-identical to plain on the last
-    step) and the instances one at a time through the plain path."""
+    ``batched`` (config 3): 256 instances (tets log-uniform in [2k, 200k]
+    as jittered Kuhn grids, spheres log-uniform in [100, 20k],
+    r ~ U[0.02, 0.08]) over sphere-update iterations with centre jitter
+    N(0, 0.002^2); the learned model is trained on the 10^3 Kuhn grid
+    generator, seeds 0-63 (or, with --batch-model same, on 32 other
+    instances of the batch generator).  ``gmm`` (config 5): 128 instances
+    of 50k-500k tets with 5k-40k clustered spheres (32-component Gaussian
+    mixture), ``--steps`` runs of the same batch; the model is trained on
+    16 instances of the same generator with another seed.  A step's time
+    is the device time of rpd_run (HIP events); per-iteration neighbour
+    lists are built beforehand (device builder, rpd_power_neighbors_gpu)
+    and their H2D copy is reported apart.  Three paths on the same inputs:
+    plain batched, learned+batched (checked bit-identical to plain on the
+    last step) and the instances one at a time through the plain path."""
     import numpy as np
     from libmat_amd import prep
     from libmat_amd.rpd import Context
+
     def progress(b):
         if b % 16 == 15:
             print(f"[{a.workload}] {b + 1} instances prepared",
@@ -265,13 +381,16 @@ def bench_batched(a, emit=True):
     ctx = Context(0)
     ctx.upload_batch(w.instances)
     G, caps, tune_ms = ctx.tune()
-    ms, cells, _, t_sites = time_batch(ctx, sets, a.warmup, cells_per_wave=0)
-    batched = {"value": sum(cells) / (sum(ms) * 1e-3), "unit": "cells/s",
-               "ms_per_step": sum(ms) / len(ms), "step_ms": ms,
-               "step_cells": cells,
-               "kernel_and_phase_ms": time_batch.kernel_ms}
-    print(f"[{a.workload}] plain batched timed", file=sys.stderr, flush=True)
-    learned = None
+    batched = None
+    if not a.learned_only:
+        ms, cells, _, t_sites = time_batch(ctx, sets, a.warmup,
+                                           cells_per_wave=0)
+        batched = {"value": sum(cells) / (sum(ms) * 1e-3), "unit": "cells/s",
+                   "ms_per_step": sum(ms) / len(ms), "step_ms": ms,
+                   "step_cells": cells, "kernel_ms": time_batch.kernel}
+        print(f"[{a.workload}] plain batched timed", file=sys.stderr,
+              flush=True)
+    learned = roofline = None
     if not a.no_learned:
         if a.workload == "batched" and a.batch_model == "cube10":
             train = [prep.cube10_instance(sd) for sd in range(64)]
@@ -288,15 +407,26 @@ def bench_batched(a, emit=True):
             params, thr, t_train, loss = train_model(train, 3, 2048)
             model = ("trained on 16 instances of the same generator, seed 1 "
                      "(3 epochs)")
+        del train
         print(f"[{a.workload}] model trained", file=sys.stderr, flush=True)
         ctx.mlp_set(params)
-        lms, lcells, lrn, _ = time_batch(ctx, sets, a.warmup, cells_per_wave=0,
-                                         learned=1, logit_threshold=thr)
-        # the last step's cells, learned against plain, bit for bit
-        ref = ctx.download()
-        ctx.run(cells_per_wave=0)
-        identical = same_result(ctx.download(), ref) and lcells == cells
-        del ref
+        lkw = dict(cells_per_wave=0, learned=1, logit_threshold=thr)
+        lms, lcells, lrn, _ = time_batch(ctx, sets, a.warmup, **lkw)
+        lkern = time_batch.kernel
+        identical = None
+        if not a.learned_only:
+            # the last step's cells, learned against plain, bit for bit
+            ref = ctx.download()
+            ctx.run(cells_per_wave=0)
+            identical = same_result(ctx.download(), ref) and lcells == cells
+            del ref
+        # executed FP64 work of the learned step's clip launches (an extra,
+        # untimed run with the kernels' work counters)
+        if not a.no_work_count:
+            stw = ctx.run(count_work=True, **lkw)
+            roofline = roofline_of_step(
+                lkern, stw["work"][0], stw["n_clip_small"], w.instances, G,
+                caps, a.traffic_json_gmm if a.workload == "gmm" else None)
         learned = {
             "value": sum(lcells) / (sum(lms) * 1e-3), "unit": "cells/s",
             "ms_per_step": sum(lms) / len(lms), "step_ms": lms,
@@ -305,22 +435,27 @@ def bench_batched(a, emit=True):
             "fallback_tets": lrn[1], "full_relation_tets": lrn[3],
             "first_round_candidates": lrn[0], "added_candidates": lrn[2],
             "model": model, "logit_threshold": thr, "train_loss": loss,
-            "train_s": t_train,
-            "kernel_and_phase_ms": time_batch.kernel_ms}
+            "train_s": t_train, "kernel_ms": lkern}
         print(f"[{a.workload}] learned+batched timed", file=sys.stderr,
               flush=True)
     ctx.close()
-    # the same instances, one plain run each (first step's spheres)
-    one = Context(0)
-    p_ms, p_cells = 0.0, 0
-    for inst in w.instances:
-        one.upload(inst)
-        one.run()
-        st = one.run()
-        p_ms += st["ms_total"]
-        p_cells += st["n_cells"]
-    one.close()
-    plain = p_cells / (p_ms * 1e-3)
+    if a.calib_copy_gib > 0:  # the copy kernel that calibrates PMC counters
+        with Context(0) as cc:
+            cc.calib_copy(int(a.calib_copy_gib * (1 << 30)))
+    one = None
+    if not a.learned_only:
+        # the same instances, one plain run each (first step's spheres)
+        c1 = Context(0)
+        p_ms, p_cells = 0.0, 0
+        for inst in w.instances:
+            c1.upload(inst)
+            c1.run()
+            st = c1.run()
+            p_ms += st["ms_total"]
+            p_cells += st["n_cells"]
+        c1.close()
+        one = {"value": p_cells / (p_ms * 1e-3), "unit": "cells/s",
+               "ms_total": p_ms, "cells": p_cells}
     top = learned if learned is not None else batched
     out = {
         "metric": METRIC, "value": top["value"], "unit": "cells/s",
@@ -332,23 +467,28 @@ def bench_batched(a, emit=True):
                    "n_tet": n_tet,
                    "n_site": sum(i.n_site for i in w.instances),
                    "path": ("learned+batched" if learned is not None
-                            else "batched"),
+                            else "plain batched"),
                    "tuned_cells_per_wave": G, "tuned_small_caps": caps},
-        "batched": batched,
+        "plain": {"batched": batched, "one_at_a_time": one},
         "learned": learned,
-        "plain_per_instance": {"value": plain, "unit": "cells/s",
-                               "ms_total": p_ms, "cells": p_cells},
-        "batched_vs_plain": batched["value"] / plain,
-        "learned_batched_vs_plain": (learned["value"] / plain
-                                     if learned is not None else None),
-        "learned_batched_vs_batched": (learned["value"] / batched["value"]
-                                       if learned is not None else None),
-        "tune_ms": {f"caps{c}_G{g}": v for (c, g), v in tune_ms.items()},
-        "host_prep_s": t_prep, "site_update_h2d_s": t_sites,
+        "learned_vs_plain_batched": (
+            learned["value"] / batched["value"]
+            if learned is not None and batched is not None else None),
+        "learned_vs_plain_one_at_a_time": (
+            learned["value"] / one["value"]
+            if learned is not None and one is not None else None),
         "fallback_pct": learned["fallback_pct"] if learned else 0.0,
+        "identical_to_plain": learned["identical_to_plain"] if learned
+        else None,
+        "roofline": roofline,
+        "tets_per_s": n_tet * 1e3 / top["ms_per_step"],
+        "tune_ms": {f"caps{c}_G{g}": v for (c, g), v in tune_ms.items()},
+        "host_prep_s": t_prep,
     }
+    out["_instances"] = w.instances
     if emit:
-        print(json.dumps(out))
+        print(json.dumps({k: v for k, v in out.items()
+                          if not k.startswith("_")}))
     return out
 
 
@@ -408,17 +548,11 @@ def bench_heldout10(a):
     print(json.dumps(out))
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only "
-                          "(multi-GPU sharding is a design note)",
-                          "n_gpus": a.gpus}))
-        sys.exit(2)
-    if a.workload in ("batched", "gmm"):
-        return bench_batched(a)
-    if a.workload == "heldout10":
-        return bench_heldout10(a)
+def bench_cube(a, emit=True):
+    """One instance: the 64^3 jittered Kuhn grid (1,572,864 tets, 50,000
+    spheres under the boundary radius rule) or the 10^3 grid; plain path
+    against the learned path (model trained on two other seeds of the same
+    generator), 20 timed steps each."""
     import numpy as np
     from libmat_amd import prep
     from libmat_amd.rpd import Context
@@ -470,7 +604,7 @@ def main():
                 "flops": flops, "n_det": work[0][0],
                 "n_bisector": work[0][1], "n_vout": work[0][2],
                 "n_exact_det_all_tiers": sum(w[3] for w in work)}
-    clip_kernel = f"k_clip<{G}, {CAPS_NAMES[caps]}, false>"
+    clip_kernel = f"k_clip<{G}, {CAPS_NAMES[caps]}, false, false>"
     traffic = None
     if os.path.exists(a.traffic_json):
         with open(a.traffic_json) as f:
@@ -586,12 +720,12 @@ def main():
         "learned": learned,
         "learned_vs_plain": (learned["value"] / plain["value"]
                              if learned is not None else None),
-        "roofline": {"bound": "hbm", "achieved": achieved,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": clip_kernel,
-                     "kernel_ms": clip_ms, "alg_bytes": alg,
-                     "measured_on": "the plain step's clip launch"},
+        "roofline_hbm": {"achieved": achieved,
+                         "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                         "kernel": clip_kernel,
+                         "kernel_ms": clip_ms, "alg_bytes": alg,
+                         "measured_on": "the plain step's clip launch"},
         "roofline_fp64_valu": fp64,
         "tune_ms": ({f"caps{c}_G{g}": v for (c, g), v in tune_ms.items()}
                     if tune_ms else None),
@@ -616,28 +750,68 @@ def main():
                            "and areas; not the metric"},
     }
     ctx.close()
-    if a.workload == "cube64" and not a.no_learned and not a.no_batched:
-        # the learned+batched comparison on the 256-instance batch (fewer
-        # sphere-update iterations than --workload batched)
+    if not a.no_cpu and emit:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline([inst], a.cpu_sample_tets,
+                                           threads, "the same instance")
+    if emit:
+        print(json.dumps(out))
+    return out
+
+
+def headline(a):
+    """Config 5 learned+batched (the metric's configuration), with the
+    cube64 single instance and config 3 (8 sphere-update iterations) as
+    sub-keys and the CPU baseline on a sample of config 5."""
+    ns = vars(a)
+    out = bench_batched(argparse.Namespace(**{**ns, "workload": "gmm"}),
+                        emit=False)
+    insts = out.pop("_instances")
+    out["config"]["workload"] = ("config 5 (learned+batched under stress): "
+                                 + out["config"]["workload"])
+    if not a.no_cube64:
+        c = bench_cube(argparse.Namespace(**{**ns, "workload": "cube64",
+                                             "steps": 20}), emit=False)
+        out["cube64"] = {k: c[k] for k in (
+            "value", "ms_per_step", "plain", "learned", "learned_vs_plain",
+            "fallback_pct", "roofline_hbm", "roofline_fp64_valu", "n_cells",
+            "n_candidates", "kernel_ms", "phase_ms", "neighbour_builder",
+            "pcie_inclusive")}
+        out["cube64"]["config"] = c["config"]
+        print("[headline] cube64 done", file=sys.stderr, flush=True)
+    if not a.no_config3:
         b = bench_batched(argparse.Namespace(**{
-            **vars(a), "workload": "batched",
-            "batch_iterations": a.embedded_batch_iterations}), emit=False)
-        out["learned_batched"] = {
-            "workload": b["config"]["workload"],
-            "value": b["value"], "unit": "cells/s",
-            "ms_per_iteration": b["ms_per_step"],
-            "plain_batched": b["batched"]["value"],
-            "plain_one_at_a_time": b["plain_per_instance"]["value"],
-            "vs_plain_batched": b["learned_batched_vs_batched"],
-            "vs_plain_one_at_a_time": b["learned_batched_vs_plain"],
-            "fallback_pct": b["fallback_pct"],
-            "identical_to_plain": b["learned"]["identical_to_plain"],
-            "clip_shape": [b["config"]["tuned_cells_per_wave"],
-                           b["config"]["tuned_small_caps"]]}
+            **ns, "workload": "batched", "batch_iterations": 8}), emit=False)
+        b.pop("_instances")
+        out["config3"] = {k: b[k] for k in (
+            "value", "ms_per_step", "config", "plain", "learned",
+            "learned_vs_plain_batched", "learned_vs_plain_one_at_a_time",
+            "fallback_pct", "identical_to_plain", "roofline")}
+        print("[headline] config 3 done", file=sys.stderr, flush=True)
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(inst, a.cpu_sample_tets, threads)
+        # every 8th instance of the batch, the sample spread over them
+        out["cpu_baseline"] = cpu_baseline(
+            insts[::8], a.cpu_sample_tets, threads,
+            f"every 8th instance of the config-5 batch "
+            f"({len(insts[::8])} instances)")
     print(json.dumps(out))
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only "
+                          "(multi-GPU sharding is a design note)",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    if a.workload == "headline":
+        return headline(a)
+    if a.workload in ("batched", "gmm"):
+        return bench_batched(a)
+    if a.workload == "heldout10":
+        return bench_heldout10(a)
+    return bench_cube(a)
 
 
 if __name__ == "__main__":

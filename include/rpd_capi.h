@@ -150,6 +150,14 @@ typedef struct {
                                  missing facet neighbours or the full
                                  relation), candidates those re-runs
                                  added, tets re-run on the full relation */
+  double ms_clip_small_sum;   /* summed device time of the run's small-tier
+                                 clip launches (one per clip round: the
+                                 plain path has one, the learned path one
+                                 per round of its certificate)            */
+  int32_t n_clip_small;       /* number of those launches                */
+  int32_t reserved0;
+  double ms_features_kernel;  /* learned runs: device time of the
+                                 proposal/feature kernel alone            */
 } rpd_stats;
 
 typedef struct rpd_ctx rpd_ctx;
@@ -284,6 +292,7 @@ int rpd_download_cells(rpd_ctx* ctx, rpd_cell* cells, uint32_t* words,
  * ascending order (the reference's tet_knn, voronoi.cu:284-317).
  * offsets[n_tet+1], ids[n_candidates]. */
 int rpd_download_relation(rpd_ctx* ctx, int64_t* offsets, int32_t* ids);
+
 
 #ifdef __cplusplus
 }

@@ -163,8 +163,11 @@ struct rpd_ctx {
       mem_a, mem_b, reopened, out_off, out_cnt, out_ids, reop64;
   rpd_stats st;
   int tuned_G = 0, tuned_caps = 0;  // rpd_tune's choice (0: not tuned)
-  hipEvent_t ev[16];
+  // ev[0..15]: pipeline phases; ev[16 + 2k], ev[17 + 2k]: around the k-th
+  // small-tier clip launch of a run (up to kSmallEv)
+  hipEvent_t ev[48];
   bool events = false;
+  int n_small_ev = 0;
   // host waits: an event polled by wait_stream and a pinned block that
   // receives the counts read back between pipeline phases
   hipEvent_t ev_sync = nullptr;
@@ -263,8 +266,10 @@ int rpd_ctx_create(int32_t device, rpd_ctx** out) {
     delete ctx;
     return -3;
   }
-  for (int i = 0; i < 16; ++i)
+  for (int i = 0; i < 48; ++i)
     if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
+      for (int k = 0; k < i; ++k) (void)hipEventDestroy(ctx->ev[k]);
+      (void)hipStreamDestroy(ctx->stream);
       delete ctx;
       return -3;
     }
@@ -288,7 +293,7 @@ void rpd_ctx_destroy(rpd_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->events)
-    for (int i = 0; i < 16; ++i) (void)hipEventDestroy(ctx->ev[i]);
+    for (int i = 0; i < 48; ++i) (void)hipEventDestroy(ctx->ev[i]);
   if (ctx->ev_sync) (void)hipEventDestroy(ctx->ev_sync);
   if (ctx->pin) (void)hipHostFree(ctx->pin);
   if (ctx->pin_inst) (void)hipHostFree(ctx->pin_inst);
@@ -567,9 +572,15 @@ static int launch_clip(rpd_ctx* ctx, int tier, int G, ClipArgs& a,
                        int caps_id = 0) {
   if (a.n_items <= 0) return 0;
   const int blocks = ceil_div(a.n_items, G);
-#define RPD_LAUNCH(GG, CAPS, EX)                                       \
-  hipLaunchKernelGGL((k_clip<GG, CAPS, EX>), dim3(blocks), dim3(64), 0, \
-                     ctx->stream, a)
+#define RPD_LAUNCH(GG, CAPS, EX)                                           \
+  do {                                                                    \
+    if (a.cert.tet_off)                                                   \
+      hipLaunchKernelGGL((k_clip<GG, CAPS, EX, true>), dim3(blocks),      \
+                         dim3(64), 0, ctx->stream, a);                    \
+    else                                                                  \
+      hipLaunchKernelGGL((k_clip<GG, CAPS, EX, false>), dim3(blocks),     \
+                         dim3(64), 0, ctx->stream, a);                    \
+  } while (0)
   if (tier == 0) {
     if (caps_id == 2 || caps_id == 3) {
       if (caps_id == 2) {
@@ -878,11 +889,13 @@ int learned_features(rpd_ctx* ctx, RunState& R) {
                      ctx->vert_inst.as<int32_t>(), ctx->sg_start.as<int32_t>(),
                      ctx->sg_ids.as<int32_t>(), ctx->vowner.as<int32_t>());
   HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[14], s));
   hipLaunchKernelGGL(k_knn_features, dim3(ceil_div(nt, 64)), dim3(64), 0, s,
                      R.M, R.S, R.inst_d, R.tet_inst_d,
                      ctx->vowner.as<int32_t>(), ctx->knn.as<int32_t>(),
                      ctx->feat.as<uint16_t>(), ctx->omask.as<uint32_t>());
   HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev[15], s));
   ctx->feat_n = nt;
   return 0;
 }
@@ -960,6 +973,9 @@ ClipArgs clip_args(rpd_ctx* ctx, RunState& R) {
   return a;
 }
 
+// event pairs around the small-tier clip launches of one run
+constexpr int kSmallEv = 16;
+
 // masks, small tier, full-capacity and exact re-runs of candidates
 // [c0, c1); buffers grow keeping earlier rounds; loc of the range is set
 // (below this many cells every wavefront of the launch is resident at once,
@@ -1027,7 +1043,14 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     // the tier's shape is the same for every launch of a run: the FP64
     // geometry sums over the lanes of a cell, so its rounding depends on
     // the cells per wavefront
+    const int k_ev = ctx->n_small_ev;
+    if (n > 0 && k_ev < kSmallEv)
+      HIP_TRY(hipEventRecord(ctx->ev[16 + 2 * k_ev], s));
     if (int rc = launch_clip(ctx, 0, R.G, a, R.cfg.small_caps)) return rc;
+    if (n > 0 && k_ev < kSmallEv) {
+      HIP_TRY(hipEventRecord(ctx->ev[17 + 2 * k_ev], s));
+      ctx->n_small_ev = k_ev + 1;
+    }
     a.items = nullptr;  // the re-run tiers index their own lists
     a.slot_by_cell = 0;
   } else if (n > 0) {
@@ -1456,6 +1479,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   R.s = ctx->stream;
   hipStream_t s = R.s;
   HIP_TRY(hipEventRecord(ctx->ev[0], s));
+  ctx->n_small_ev = 0;
   HIP_TRY(hipMemsetAsync(ctx->counters.p, 0, 8 * 4, s));
   if (int rc = init_state(ctx, R)) return rc;
   const int32_t nt = R.nt;
@@ -1565,6 +1589,11 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   st.ms_kernel[7] = st.ms_phase[4];
   st.n_bbox_fallback = host_cnt[0];
   st.tet_k = (int32_t)st64[2];
+  st.ms_features_kernel = cfg.learned ? el(14, 15) : 0.0;
+  st.n_clip_small = ctx->n_small_ev;
+  st.ms_clip_small_sum = 0;
+  for (int k = 0; k < ctx->n_small_ev; ++k)
+    st.ms_clip_small_sum += el(16 + 2 * k, 17 + 2 * k);
   return 0;
 }
 

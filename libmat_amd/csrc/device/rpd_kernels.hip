@@ -912,7 +912,10 @@ constexpr int clip_min_waves() {
   return (G == 16 && C::V <= 16) ? 4 : 1;
 }
 
-template <int G, class C, bool kExact>
+// kCert: the learned path's launches, which evaluate the completeness
+// certificate on every final cell (a.cert); the plain path's launches are
+// compiled without it
+template <int G, class C, bool kExact, bool kCert>
 __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
     ClipArgs a) {
   constexpr int L = 64 / G;
@@ -1447,10 +1450,10 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
     // without leaving an active facet here; the certificate cannot see it,
     // so the tet goes to the full relation (the exact tier runs one cell
     // per wavefront, so the ballot covers exactly this cell's lanes)
-    if (a.cert.tet_off && ballot(on_plane) != 0 && j == 0) a.cert.flag[t] = 1;
+    if (kCert && ballot(on_plane) != 0 && j == 0) a.cert.flag[t] = 1;
   }
-  if (a.cert.tet_off && status != kStatusRerunLarge &&
-      status != kStatusRerunExact && status != RPD_NO_INTERSECTION) {
+  if (kCert && status != kStatusRerunLarge && status != kStatusRerunExact &&
+      status != RPD_NO_INTERSECTION) {
     const CertArgs& c = a.cert;
     if (status != RPD_SUCCESS) {
       if (j == 0) c.flag[t] = 1;

@@ -78,7 +78,11 @@ class Stats(ctypes.Structure):
                 ("ms_kernel", ctypes.c_double * 8),
                 ("work", (ctypes.c_int64 * 4) * 3),
                 ("relation_work", ctypes.c_int64 * 4),
-                ("learned", ctypes.c_int64 * 4)]
+                ("learned", ctypes.c_int64 * 4),
+                ("ms_clip_small_sum", ctypes.c_double),
+                ("n_clip_small", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32),
+                ("ms_features_kernel", ctypes.c_double)]
 
     def as_dict(self):
         d = {}
