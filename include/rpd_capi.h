@@ -158,6 +158,8 @@ typedef struct {
   int32_t reserved0;
   double ms_features_kernel;  /* learned runs: device time of the
                                  proposal/feature kernel alone            */
+  int64_t scratch_bytes;      /* rpd_run_noalloc: scratch bytes the call
+                                 used (high-water mark)                   */
 } rpd_stats;
 
 typedef struct rpd_ctx rpd_ctx;
@@ -293,6 +295,37 @@ int rpd_download_cells(rpd_ctx* ctx, rpd_cell* cells, uint32_t* words,
  * offsets[n_tet+1], ids[n_candidates]. */
 int rpd_download_relation(rpd_ctx* ctx, int64_t* offsets, int32_t* ids);
 
+/* ---- allocation-free RPD.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  One call = rpd_upload +
+ * the plain rpd_run + the cell stream copied into the caller's device
+ * buffers, with every intermediate buffer carved from the library's static
+ * device scratch of RPD_NOALLOC_SCRATCH_MIB MiB per GPU (held from the
+ * library's load until the process ends); it calls no hipMalloc, hipFree or
+ * allocating library routine.  Inputs are host arrays as for rpd_upload;
+ * outputs are device buffers of the given capacities (entries), laid out
+ * as rpd_download_cells writes them (geom: 4 doubles per cell).  Plain path
+ * only (cfg->learned = 0, count_work = 0).  Errors: -4 when the instance
+ * needs more scratch than the maximum, -5 when an output buffer is too
+ * small (stats then hold nothing), with the sizes in rpd_last_error.  The
+ * context's resident instance and last results are left as they were
+ * (rpd_download_* still return the last rpd_run's); *st_out receives this
+ * call's statistics.  Every call on one GPU shares that GPU's scratch and
+ * leaves its contents behind: calls that overlap on one GPU (from several
+ * contexts or threads) race. */
+#define RPD_NOALLOC_SCRATCH_MIB 256
+typedef struct {
+  rpd_cell* cells;   /* device, cells_cap records                     */
+  int64_t cells_cap;
+  uint32_t* words;   /* device, words_cap payload words               */
+  int64_t words_cap;
+  double* geom;      /* device, 4 * cells_cap: volume, centroid       */
+  double* areas;     /* device, areas_cap facet areas                 */
+  int64_t areas_cap;
+} rpd_noalloc_out;
+int rpd_run_noalloc(rpd_ctx* ctx, const rpd_instance_desc* desc,
+                    const rpd_run_cfg* cfg, const rpd_noalloc_out* out,
+                    rpd_stats* st_out);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <initializer_list>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -24,9 +25,50 @@
 #include "rpd_learn.hip"
 #include "rpd_nbr.hip"
 
+// Device scratch of rpd_run_noalloc, the entry point for callers that
+// cannot allocate: every buffer of one upload + plain run + compaction is
+// carved from it.  Each device holds its own copy of the array, and host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// that device (taken once per device when a context is created, kept in
+// g_scratch_dev); the array's name is not a device address on the host.
+constexpr size_t kScratchBytes = (size_t)RPD_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 using namespace rpd;
+
+struct Buf;
+
+// Bump allocator over the scratch for the duration of one rpd_run_noalloc
+// call: Bufs that ask for memory get a block of it; their own blocks are
+// remembered and put back when the call ends.  Nothing is freed or
+// allocated while it is active.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0, high = 0;
+  std::vector<std::pair<Buf*, std::pair<void*, size_t>>> saved;
+  bool has(const Buf* b) const {
+    for (const auto& x : saved)
+      if (x.first == b) return true;
+    return false;
+  }
+  void touch(Buf* b);
+  bool exhausted = false;
+  void* take(size_t bytes) {
+    const size_t b = (std::max<size_t>(bytes, 256) + 255) & ~(size_t)255;
+    if (used + b > cap) {
+      exhausted = true;
+      return nullptr;
+    }
+    void* q = base + used;
+    used += b;
+    high = std::max(high, used);
+    return q;
+  }
+  void restore();
+};
+thread_local Arena* t_arena = nullptr;
 
 struct Buf {
   void* p = nullptr;
@@ -35,13 +77,40 @@ struct Buf {
   Buf(const Buf&) = delete;  // owns its block
   Buf& operator=(const Buf&) = delete;
   void swap(Buf& o) {
+    if (t_arena) {  // only scratch blocks change hands inside an arena
+      for (Buf* b : {this, &o})
+        if (!t_arena->has(b)) {
+          t_arena->touch(b);
+          b->p = nullptr;
+          b->cap = 0;
+        }
+    }
     std::swap(p, o.p);
     std::swap(cap, o.cap);
   }
   ~Buf() {
     if (p) (void)hipFree(p);
   }
+  // in an arena: a scratch block (keeping the first `keep` bytes of the
+  // current one, copied on stream s), never the Buf's own block
+  hipError_t arena_need(size_t bytes, size_t keep, hipStream_t s) {
+    Arena& A = *t_arena;
+    const bool mine = A.has(this);
+    if (mine && bytes <= cap && p) return hipSuccess;
+    void* q = A.take(bytes);
+    if (!q) return hipErrorOutOfMemory;
+    if (mine && p && keep) {
+      hipError_t e = hipMemcpyAsync(q, p, std::min(keep, cap),
+                                    hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) return e;
+    }
+    A.touch(this);
+    p = q;
+    cap = (std::max<size_t>(bytes, 256) + 255) & ~(size_t)255;
+    return hipSuccess;
+  }
   hipError_t need(size_t bytes) {
+    if (t_arena) return arena_need(bytes, 0, nullptr);
     if (bytes <= cap && p) return hipSuccess;
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -54,6 +123,7 @@ struct Buf {
   // grow to `bytes` keeping the first `keep` bytes (stream-ordered copy;
   // the old block is freed after the stream has drained)
   hipError_t need_keep(size_t bytes, size_t keep, hipStream_t s) {
+    if (t_arena) return arena_need(bytes, keep, s);
     if (bytes <= cap && p) return hipSuccess;
     size_t b = std::max<size_t>(bytes + bytes / 4, 256);
     void* q = nullptr;
@@ -78,6 +148,17 @@ struct Buf {
     return reinterpret_cast<T*>(p);
   }
 };
+
+void Arena::touch(Buf* b) {
+  if (!has(b)) saved.push_back({b, {b->p, b->cap}});
+}
+void Arena::restore() {
+  for (auto& x : saved) {
+    x.first->p = x.second.first;
+    x.first->cap = x.second.second;
+  }
+  saved.clear();
+}
 
 struct TierBufs {
   Buf hdr, words, geom, areas;
@@ -178,9 +259,21 @@ struct rpd_ctx {
   size_t pin_inst_cap = 0;
   hipEvent_t ev_inst = nullptr;
   bool inst_copy_recorded = false;
+  // this device's copy of the no-alloc scratch (nullptr if its address
+  // could not be taken)
+  void* scratch_dev = nullptr;
 };
 
 namespace {
+
+// the scratch's address on each device, taken once per device
+std::mutex g_scratch_mu;
+void* g_scratch_dev[64] = {};
+bool g_scratch_tried[64] = {};
+
+// per-run instance table staging allocated with the context, so that a
+// single-instance run never allocates pinned memory
+constexpr size_t kPinInstBytes = 64 << 10;
 
 int fail(rpd_ctx* c, const std::string& m, int code = -1) {
   if (c) c->err = m;
@@ -279,9 +372,22 @@ int rpd_ctx_create(int32_t device, rpd_ctx** out) {
       hipEventCreateWithFlags(&ctx->ev_inst, hipEventDisableTiming) !=
           hipSuccess ||
       hipHostMalloc((void**)&ctx->pin, kPinBytes, hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc(&ctx->pin_inst, kPinInstBytes, hipHostMallocDefault) !=
           hipSuccess) {
     rpd_ctx_destroy(ctx);
     return -3;
+  }
+  ctx->pin_inst_cap = kPinInstBytes;
+  if (device < 64) {  // the scratch's address on this device, once
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_tried[device]) {
+      g_scratch_tried[device] = true;
+      void* ptr = nullptr;
+      if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess)
+        g_scratch_dev[device] = ptr;
+    }
+    ctx->scratch_dev = g_scratch_dev[device];
   }
   std::memset(&ctx->st, 0, sizeof(ctx->st));
   *out = ctx;
@@ -730,6 +836,9 @@ int upload_inst_info(rpd_ctx* ctx, RunState& R) {
     }
     HIP_TRY(e);
   }
+  if (bytes > ctx->pin_inst_cap && t_arena)
+    return fail(ctx, "rpd_run_noalloc: instance table over the pinned "
+                     "staging block");
   if (bytes > ctx->pin_inst_cap) {
     if (ctx->pin_inst) (void)hipHostFree(ctx->pin_inst);
     ctx->pin_inst = nullptr;
@@ -1787,6 +1896,104 @@ int rpd_download_relation(rpd_ctx* ctx, int64_t* offsets, int32_t* ids) {
     HIP_TRY(hipMemcpyAsync(ids, ctx->rel_ids.p, ctx->st.n_candidates * 4,
                            hipMemcpyDeviceToHost, s));
   HIP_TRY(wait_stream(ctx, s));
+  return 0;
+}
+
+int rpd_run_noalloc(rpd_ctx* ctx, const rpd_instance_desc* desc,
+                    const rpd_run_cfg* cfg, const rpd_noalloc_out* out,
+                    rpd_stats* st_out) {
+  if (!ctx || !desc || !out) return -1;
+  if (cfg && cfg->learned != 0)
+    return fail(ctx, "rpd_run_noalloc: plain path only (learned = 0)");
+  if (cfg && cfg->count_work)
+    return fail(ctx, "rpd_run_noalloc: count_work is not available");
+  if (!out->cells || !out->words || !out->geom || !out->areas)
+    return fail(ctx, "rpd_run_noalloc: missing output buffers");
+  if (!ctx->scratch_dev)
+    return fail(ctx, "rpd_run_noalloc: the scratch address of this device "
+                     "is not available", -3);
+  if (t_arena) return fail(ctx, "rpd_run_noalloc: nested call");
+  HIP_TRY(hipSetDevice(ctx->device));
+  // the context's resident instance and last results are put back when the
+  // call returns: it runs entirely in the scratch
+  struct Saved {
+    std::vector<rpd_ctx::Inst> inst;
+    int32_t n_vert, n_tet, n_site;
+    int64_t n_knn;
+    bool has_mesh, has_sites;
+    rpd_stats st;
+  } sv{ctx->inst,     ctx->n_vert,   ctx->n_tet,     ctx->n_site,
+       ctx->n_knn,    ctx->has_mesh, ctx->has_sites, ctx->st};
+  Arena A;
+  A.base = static_cast<char*>(ctx->scratch_dev);
+  A.cap = kScratchBytes;
+  struct Guard {
+    rpd_ctx* c;
+    Arena& a;
+    Saved& s;
+    ~Guard() {
+      // the call's work must be complete before its blocks can be handed
+      // to anyone else
+      (void)hipStreamSynchronize(c->stream);
+      t_arena = nullptr;
+      a.restore();
+      c->inst = s.inst;
+      c->n_vert = s.n_vert;
+      c->n_tet = s.n_tet;
+      c->n_site = s.n_site;
+      c->n_knn = s.n_knn;
+      c->has_mesh = s.has_mesh;
+      c->has_sites = s.has_sites;
+      c->st = s.st;
+    }
+  } guard{ctx, A, sv};
+  t_arena = &A;
+  auto scratch_err = [&](int rc) {
+    if (A.exhausted)
+      return fail(ctx,
+                  "rpd_run_noalloc: the instance needs more than the " +
+                      std::to_string(kScratchBytes >> 20) +
+                      " MiB device scratch (used " +
+                      std::to_string(A.used >> 20) + " MiB before the "
+                      "request that failed)",
+                  -4);
+    return rc;
+  };
+  if (desc->n_tet <= 0 || desc->n_vert <= 0 || desc->n_site <= 0 ||
+      !desc->sites)
+    return fail(ctx, "rpd_run_noalloc: instance without tets or sites");
+  if (int rc = rpd_upload(ctx, desc)) return scratch_err(rc);
+  if (int rc = rpd_run(ctx, cfg)) return scratch_err(rc);
+  const rpd_stats& st = ctx->st;
+  if (st.n_cells > out->cells_cap || st.n_words > out->words_cap ||
+      st.n_areas > out->areas_cap)
+    return fail(ctx, "rpd_run_noalloc: output buffers too small (cells " +
+                         std::to_string(st.n_cells) + ", words " +
+                         std::to_string(st.n_words) + ", areas " +
+                         std::to_string(st.n_areas) + ")",
+                -5);
+  hipStream_t s = ctx->stream;
+  if (st.n_cells) {
+    HIP_TRY(hipMemcpyAsync(out->cells, ctx->out_cells.p,
+                           (size_t)st.n_cells * sizeof(rpd_cell),
+                           hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(out->geom, ctx->out_geom.p,
+                           (size_t)st.n_cells * 32, hipMemcpyDeviceToDevice,
+                           s));
+  }
+  if (st.n_words)
+    HIP_TRY(hipMemcpyAsync(out->words, ctx->out_words.p,
+                           (size_t)st.n_words * 4, hipMemcpyDeviceToDevice,
+                           s));
+  if (st.n_areas)
+    HIP_TRY(hipMemcpyAsync(out->areas, ctx->out_areas.p,
+                           (size_t)st.n_areas * 8, hipMemcpyDeviceToDevice,
+                           s));
+  HIP_TRY(wait_stream(ctx, s));
+  if (st_out) {
+    *st_out = st;
+    st_out->scratch_bytes = (int64_t)A.high;
+  }
   return 0;
 }
 

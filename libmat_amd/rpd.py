@@ -62,6 +62,13 @@ class RunCfg(ctypes.Structure):
                 ("proposal_k", ctypes.c_int32)]
 
 
+class NoAllocOut(ctypes.Structure):
+    _fields_ = [("cells", ctypes.c_void_p), ("cells_cap", ctypes.c_int64),
+                ("words", ctypes.c_void_p), ("words_cap", ctypes.c_int64),
+                ("geom", ctypes.c_void_p), ("areas", ctypes.c_void_p),
+                ("areas_cap", ctypes.c_int64)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("n_cells", ctypes.c_int64), ("n_candidates", ctypes.c_int64),
                 ("n_words", ctypes.c_int64), ("n_areas", ctypes.c_int64),
@@ -82,7 +89,8 @@ class Stats(ctypes.Structure):
                 ("ms_clip_small_sum", ctypes.c_double),
                 ("n_clip_small", ctypes.c_int32),
                 ("reserved0", ctypes.c_int32),
-                ("ms_features_kernel", ctypes.c_double)]
+                ("ms_features_kernel", ctypes.c_double),
+                ("scratch_bytes", ctypes.c_int64)]
 
     def as_dict(self):
         d = {}
@@ -143,6 +151,11 @@ def _declare_capi(L):
     L.rpd_power_neighbors_gpu.argtypes = [vp, ctypes.c_int32, vp, vp, vp,
                                           ctypes.POINTER(vp)]
     L.rpd_power_neighbors_gpu.restype = ctypes.c_int
+    L.rpd_run_noalloc.argtypes = [vp, ctypes.POINTER(InstanceDesc),
+                                  ctypes.POINTER(RunCfg),
+                                  ctypes.POINTER(NoAllocOut),
+                                  ctypes.POINTER(Stats)]
+    L.rpd_run_noalloc.restype = ctypes.c_int
 
 
 CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
@@ -154,7 +167,7 @@ CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
                 "rpd_mlp_scores",
                 "rpd_sync", "rpd_get_stats", "rpd_download_cells",
                 "rpd_download_relation", "rpd_calib_copy",
-                "rpd_power_neighbors_gpu")
+                "rpd_power_neighbors_gpu", "rpd_run_noalloc")
 
 
 def _p(a):
@@ -366,6 +379,23 @@ class Context:
                      int(proposal_k))
         self._check(self._L.rpd_run(self._h, ctypes.byref(cfg)), "rpd_run")
         return self.stats()
+
+    def run_noalloc(self, inst, out, cells_per_wave=8, small_caps=0):
+        """rpd_run_noalloc of ``inst`` (host arrays) into the caller's device
+        buffers ``out`` = (cells, cells_cap, words, words_cap, geom, areas,
+        areas_cap) (device addresses and capacities); returns (rc, stats
+        dict, last error)."""
+        a = self._arrays(inst)
+        d = self._desc(inst, a)
+        cfg = RunCfg(cells_per_wave, 1, 1, 0, 1, 0, int(small_caps), 0, 0.0,
+                     8)
+        o = NoAllocOut(*out)
+        st = Stats()
+        rc = self._L.rpd_run_noalloc(self._h, ctypes.byref(d),
+                                     ctypes.byref(cfg), ctypes.byref(o),
+                                     ctypes.byref(st))
+        err = self._L.rpd_last_error(self._h).decode() if rc else ""
+        return rc, st.as_dict(), err
 
     def stats(self):
         st = Stats()
