@@ -71,7 +71,7 @@ def parse():
                     help="batched: train the proposal model on other "
                          "instances of the batch generator, or on the 10^3 "
                          "Kuhn grid generator (seeds 0-63)")
-    ap.add_argument("--cpu-sample-tets", type=int, default=200000,
+    ap.add_argument("--cpu-sample-tets", type=int, default=2000000,
                     help="tets of the CPU baseline's sample at all threads "
                          "(1/16 of it at one thread)")
     ap.add_argument("--cpu-threads", type=int, default=0)
