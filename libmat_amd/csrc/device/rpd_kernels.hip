@@ -51,11 +51,17 @@ __device__ __forceinline__ rpd_d4 ld4(const double4* p) {
 // Products may fuse with the following sum: a fused monomial passes
 // through fewer roundings, so the bound holds, and a decided sign is the
 // exact one either way (undecided signs go to the exact predicate).
+// The caller passes the bound eps, computed from row maxima that dominate
+// the four columns': the clip kernel keeps, per cell, the largest |a|,
+// |b|, |c|, |d| over all planes it has added, so each row's maximum over a
+// vertex's three planes and the new plane e is at most max(cell maximum,
+// |e|) -- one bound per clip step, no smaller than the per-vertex one.
 // Returns +1 conflict, -1 no conflict, 0 undecided.
-__device__ __forceinline__ int conflict_filtered(const rpd_d4& p1,
-                                                 const rpd_d4& p2,
-                                                 const rpd_d4& p3,
-                                                 const rpd_d4& e) {
+__device__ __forceinline__ int conflict_filtered_eps(const rpd_d4& p1,
+                                                     const rpd_d4& p2,
+                                                     const rpd_d4& p3,
+                                                     const rpd_d4& e,
+                                                     double eps) {
 #pragma clang fp contract(fast)
   const double a11 = p1.x, a12 = p2.x, a13 = p3.x, a14 = e.x;
   const double a21 = p1.y, a22 = p2.y, a23 = p3.y, a24 = e.y;
@@ -72,20 +78,11 @@ __device__ __forceinline__ int conflict_filtered(const rpd_d4& p1,
   double m134 = m34 * a13 - m14 * a33 + m13 * a43;
   double m234 = m34 * a23 - m24 * a33 + m23 * a43;
   double det = m234 * a14 - m134 * a24 + m124 * a34 - m123 * a44;
-  double mx = fmax(fmax(fabs(a11), fabs(a12)), fmax(fabs(a13), fabs(a14)));
-  double my = fmax(fmax(fabs(a21), fabs(a22)), fmax(fabs(a23), fabs(a24)));
-  double mz = fmax(fmax(fabs(a31), fabs(a32)), fmax(fabs(a33), fabs(a34)));
-  double mw = fmax(fmax(fabs(a41), fabs(a42)), fmax(fabs(a43), fabs(a44)));
-  double b = mx * my * mz * mw;
-  if (!(b < 1e280)) return 0;
-  double eps = 2.5e-14 * b + 1e-290;
   if (det > eps) return 1;
   if (det < -eps) return -1;
   return 0;
 }
 
-// exact sign of the same determinant: +1 conflict, 0 the vertex lies on
-// the plane (no conflict), -1 no conflict
 __device__ __noinline__ int conflict_exact(rpd_d4 p1, rpd_d4 p2, rpd_d4 p3,
                                            rpd_d4 e) {
   return rpd_exact_det4_sign(p1, p2, p3, e);
@@ -545,9 +542,14 @@ __global__ __launch_bounds__(256) void k_tet_count(DevMesh M, DevSites S,
                                                    const int32_t* bin_ids,
                                                    int64_t* tet_count,
                                                    uint64_t* tet_pass,
-                                                   unsigned long long* work) {
+                                                   unsigned long long* work,
+                                                   const uint8_t* tet_sel) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= M.n_tet) return;
+  if (tet_sel && !tet_sel[t]) {  // outside the tet subset: no candidates
+    tet_count[t] = 0;
+    return;
+  }
   TetLoad L;
   load_tet(M, t, L);
   double c[3];
@@ -583,9 +585,11 @@ __global__ __launch_bounds__(256) void k_tet_count(DevMesh M, DevSites S,
 __global__ __launch_bounds__(256) void k_tet_fill(
     DevMesh M, DevSites S, const InstInfo* inst, const int32_t* tet_inst,
     const int32_t* bin_start, const int32_t* bin_ids, const int64_t* tet_off,
-    const uint64_t* tet_pass, Cand* cand, int32_t* rel_ids) {
+    const uint64_t* tet_pass, Cand* cand, int32_t* rel_ids,
+    const uint8_t* tet_sel) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= M.n_tet) return;
+  if (tet_sel && !tet_sel[t]) return;
   TetLoad L;
   load_tet(M, t, L);
   double c[3];
@@ -645,6 +649,36 @@ __device__ __forceinline__ bool certainly_neg4(const rpd_d4& e,
   return b < RPD_FILTER_MAX && s < -(RPD_FILTER_C * b + RPD_FILTER_ABS);
 }
 
+// The tet's four corners as the clip kernel's initial vertices: homogeneous
+// coordinates X and cofactor magnitudes Q of the dual triangles (1,3,2),
+// (0,2,3), (0,3,1), (0,1,2) of its binary64 face planes.
+struct CornerFrames {
+  rpd_d4 X[4], Q[4];
+  // the plane e is certainly positive at all four corners, so it cannot
+  // cut any state of a cell of this tet (every vertex of the cell lies in
+  // the convex hull of the corners)
+  __device__ __forceinline__ bool inactive(const rpd_d4& e) const {
+    return certainly_neg4(e, X[0], Q[0]) && certainly_neg4(e, X[1], Q[1]) &&
+           certainly_neg4(e, X[2], Q[2]) && certainly_neg4(e, X[3], Q[3]);
+  }
+};
+__device__ __forceinline__ void corner_frames(const TetLoad& L,
+                                              CornerFrames& F) {
+  rpd_d4 pl[4];
+  tet_planes(L.V, pl);
+  rpd_vertex_homog(pl[1], pl[3], pl[2], F.X[0], F.Q[0]);
+  rpd_vertex_homog(pl[0], pl[2], pl[3], F.X[1], F.Q[1]);
+  rpd_vertex_homog(pl[0], pl[3], pl[1], F.X[2], F.Q[2]);
+  rpd_vertex_homog(pl[0], pl[1], pl[2], F.X[3], F.Q[3]);
+}
+// a cell's number of clip steps: mask bits, plus the list positions past
+// 128 (always clipped), saturated to a byte
+__device__ __forceinline__ uint8_t clip_steps(uint64_t m0, uint64_t m1,
+                                              int deg) {
+  const int n = __popcll(m0) + __popcll(m1) + (deg > 128 ? deg - 128 : 0);
+  return (uint8_t)(n < 255 ? n : 255);
+}
+
 // One thread per candidate cell: bit k of the mask is set unless the k-th
 // neighbour's bisector is certainly positive at all four corners of the
 // tet cell (then it cannot cut any state of the cell: every cell vertex
@@ -663,35 +697,23 @@ __global__ __launch_bounds__(256) void k_cand_mask(DevMesh M, DevSites S,
   const int t = cand[c].tet, i = cand[c].seed;
   TetLoad L;
   load_tet(M, t, L);
-  rpd_d4 pl[4];
-  tet_planes(L.V, pl);
-  // corners = initial dual triangles (1,3,2), (0,2,3), (0,3,1), (0,1,2)
-  rpd_d4 X0, Q0, X1, Q1, X2, Q2, X3, Q3;
-  rpd_vertex_homog(pl[1], pl[3], pl[2], X0, Q0);
-  rpd_vertex_homog(pl[0], pl[2], pl[3], X1, Q1);
-  rpd_vertex_homog(pl[0], pl[3], pl[1], X2, Q2);
-  rpd_vertex_homog(pl[0], pl[1], pl[2], X3, Q3);
+  CornerFrames F;
+  corner_frames(L, F);
   const int64_t k0 = S.knn_off[i];
   const int deg = (int)(S.knn_off[i + 1] - k0);
   uint64_t m0 = 0, m1 = 0;
   const int lim = deg < 128 ? deg : 128;
-  for (int k = 0; k < lim; ++k) {
-    const rpd_d4 e = ld4(S.knn_bis + k0 + k);
-    const bool inactive =
-        certainly_neg4(e, X0, Q0) && certainly_neg4(e, X1, Q1) &&
-        certainly_neg4(e, X2, Q2) && certainly_neg4(e, X3, Q3);
-    if (!inactive) {
+  for (int k = 0; k < lim; ++k)
+    if (!F.inactive(ld4(S.knn_bis + k0 + k))) {
       if (k < 64)
         m0 |= 1ull << k;
       else
         m1 |= 1ull << (k - 64);
     }
-  }
   cand[c].m0 = m0;
   cand[c].m1 = m1;
   if (trips) {
-    const int n = __popcll(m0) + __popcll(m1) + (deg > 128 ? deg - 128 : 0);
-    trips[c] = (uint8_t)(n < 255 ? n : 255);
+    trips[c] = clip_steps(m0, m1, deg);
     trip_ids[c] = (int32_t)(c0 + c);
   }
 }
@@ -934,6 +956,8 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
       (P <= 16), uint16_t,
       typename std::conditional<(P <= 32), uint32_t, uint64_t>::type>::type;
   __shared__ AdjT s_adj[G][P];
+  // per cell: the largest |a|, |b|, |c|, |d| over its planes (filter bound)
+  __shared__ double s_mx[G][4];
 
   const int lane = threadIdx.x;
   const int g = lane / L, j = lane % L;
@@ -983,6 +1007,17 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
                  (meta_byte(tm, 8 + q) << 16);
   }
   wsync();
+  if (j < 4) {  // row maxima of the four tet planes, one row per lane
+    double m = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double4 pq = s_pl[g][q];
+      const double v = j == 0 ? pq.x : j == 1 ? pq.y : j == 2 ? pq.z : pq.w;
+      m = fmax(m, fabs(v));
+    }
+    s_mx[g][j] = m;
+  }
+  wsync();
   int nb_v = 4, nb_p = 4, nb_e = 6;
   uint32_t status = RPD_SUCCESS;
   uint32_t n_det = 0, n_bis = 0, n_exd = 0;  // executed work (count_work)
@@ -1027,11 +1062,21 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
     n_bis += 1;
     n_det += nb_v;
 
-    // ---- conflict tests, lane-parallel (convex_cell.cu:706-721)
+    // ---- conflict tests, lane-parallel (convex_cell.cu:706-721); one
+    // filter bound per step from the cell's row maxima and the new plane
+    const double mx = fmax(s_mx[g][0], fabs(e.x));
+    const double my = fmax(s_mx[g][1], fabs(e.y));
+    const double mz = fmax(s_mx[g][2], fabs(e.z));
+    const double mw = fmax(s_mx[g][3], fabs(e.w));
+    const double bnd = (mx * my) * (mz * mw);
+    // an unrepresentable bound leaves every sign undecided
+    const double eps = bnd < 1e280 ? 2.5e-14 * bnd + 1e-290 : INFINITY;
     uint64_t conf_lo = 0, conf_hi = 0;
     bool undecided = false;
 #pragma unroll
     for (int q = 0; q < VS; ++q) {
+      // slots no cell of the wavefront has reached are not issued
+      if (q > 0 && ballot(nb_v > q * L) == 0) break;
       const int v = q * L + j;
       int c = -1;
       if (v < nb_v) {
@@ -1039,7 +1084,7 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
         const rpd_d4 p1 = ld4(&s_pl[g][w & 255]);
         const rpd_d4 p2 = ld4(&s_pl[g][(w >> 8) & 255]);
         const rpd_d4 p3 = ld4(&s_pl[g][(w >> 16) & 255]);
-        c = a.force_exact ? 0 : conflict_filtered(p1, p2, p3, e);
+        c = a.force_exact ? 0 : conflict_filtered_eps(p1, p2, p3, e, eps);
         if (kExact && c == 0) {
           const int sg = conflict_exact(p1, p2, p3, e);
           on_plane = on_plane || sg == 0;
@@ -1070,6 +1115,10 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
       s_src[g][cur] = m;
       s_h[g][cur] = 1;  // F_CELL_ADJ_DEFAULT
       s_adj[g][cur] = 0;
+      s_mx[g][0] = mx;
+      s_mx[g][1] = my;
+      s_mx[g][2] = mz;
+      s_mx[g][3] = mw;
     }
     nb_p = cur + 1;
 
@@ -1097,9 +1146,13 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
       int dst[VS];
 #pragma unroll
       for (int q = 0; q < VS; ++q) {
-        const int v = q * L + j;
         val[q] = 0;
         dst[q] = -1;
+      }
+#pragma unroll
+      for (int q = 0; q < VS; ++q) {
+        if (q > 0 && ballot(nb_v > q * L) == 0) break;
+        const int v = q * L + j;
         if (v < nb_v) {
           val[q] = s_ver[g][v];
           const bool c = mbit(conf_lo, conf_hi, v);

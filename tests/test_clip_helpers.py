@@ -31,6 +31,8 @@ static inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 static inline int __clzll(long long x) {
   return __builtin_clzll((unsigned long long)x);
 }
+// one lane per cell: the wavefront vote is the lane's own predicate
+static inline unsigned long long ballot(bool p) { return p ? 1ull : 0ull; }
 """
 
 MAIN = r"""
