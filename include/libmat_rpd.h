@@ -16,6 +16,7 @@
 
 #include <array>
 #include <map>
+#include <memory>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -267,6 +268,8 @@ struct RpdSphere {
   RpdPowerCell pcell;
 };
 
+struct rpd_ctx;  // the device context of rpd_capi.h
+
 namespace libmat_rpd {
 // power-diagram 1-rings of the live spheres (is_deleted == false) inside
 // the box lo/hi; valid[i] = power cell meets the box (the RT validity of
@@ -275,6 +278,44 @@ void power_neighbors(const std::vector<std::array<double, 4>>& xyzr,
                      const std::vector<char>& live, const double box[6],
                      std::vector<char>& valid,
                      std::vector<std::set<int>>& neighbors);
+
+// A tet mesh kept on the GPU between RPD calls: uploaded once, after which
+// each run sends only the spheres and, for a partial RPD, the ids of the
+// tets to clip (calculate_partial's partial tets, rpd_api.cxx:249-262,
+// clipped in place on the full mesh instead of sent as a partial mesh).
+// Owns its device context; one object is not for concurrent use.
+class ResidentMesh {
+ public:
+  explicit ResidentMesh(int device = 0);
+  ~ResidentMesh();
+  ResidentMesh(const ResidentMesh&) = delete;
+  ResidentMesh& operator=(const ResidentMesh&) = delete;
+  // the mesh arrays of compute_clipped_voro_diagram_GPU (e_adjs in the
+  // triangular layout of get_edge_idx); replaces a resident mesh
+  void upload(const std::vector<float>& vertices,
+              const std::vector<int>& indices, const std::vector<int>& v_adjs,
+              const std::vector<int>& e_adjs, const std::vector<int>& f_adjs,
+              const std::vector<int>& f_ids);
+  bool uploaded() const { return n_tet_ > 0; }
+  // compute_clipped_voro_diagram_GPU on the resident mesh for the given
+  // spheres, over the tets in `tets` (nullptr: all).  Cells keep the
+  // original tet ids; ids = position; thread_id = (position of the tet in
+  // `tets`) * tet_k + slot, as a run over the partial mesh of those tets
+  // numbers them.
+  std::vector<ConvexCellHost> run(const std::vector<float>& site, int n_site,
+                                  const std::vector<float>& site_weights,
+                                  const std::vector<uint>& site_flags,
+                                  const std::vector<int>& site_knn, int site_k,
+                                  std::vector<float>& site_cell_vol,
+                                  const std::vector<int>* tets = nullptr);
+
+ private:
+  int device_;
+  ::rpd_ctx* ctx_ = nullptr;
+  int n_tet_ = 0;
+  std::vector<float> vertices_;
+  std::vector<int> indices_, f_adjs_, f_ids_;
+};
 }  // namespace libmat_rpd
 
 template <class TetMeshT = RpdTetMesh, class SurfaceMeshT = void,
@@ -293,6 +334,7 @@ class RPD3D_GPU_T {
     num_itr_rpd = 0;
     n_site = -1;
     site_k = -1;
+    gpu_.reset();  // the new mesh is uploaded by the next RPD
   }
   void set_spheres(std::vector<SphereT>* _all_medial_spheres) {
     all_medial_spheres = _all_medial_spheres;
@@ -360,7 +402,23 @@ class RPD3D_GPU_T {
   int num_itr_rpd = 0;
   int n_site = -1;
   int site_k = -1;
+  int device = 0;  // GPU of the resident mesh
+
   int max_surf_fid = -1;  // largest surface facet id (-1: from f_adjs)
+
+ private:
+  // the tet mesh on the GPU, uploaded at the first RPD after init()
+  std::shared_ptr<libmat_rpd::ResidentMesh> gpu_;
+  libmat_rpd::ResidentMesh& gpu() {
+    if (!gpu_) {
+      auto g = std::make_shared<libmat_rpd::ResidentMesh>(device);
+      g->upload(tet_mesh->tet_vertices, tet_mesh->tet_indices,
+                tet_mesh->v_adjs, tet_mesh->e_adjs, tet_mesh->f_adjs,
+                tet_mesh->f_ids);
+      gpu_ = g;
+    }
+    return *gpu_;
+  }
 };
 
 using RPD3D_GPU = RPD3D_GPU_T<>;

@@ -116,13 +116,11 @@ void RPD3D_GPU_T<TM, SM, PM, SP>::calculate() {
   this->site_k = k;
   load_partial_spheres_to_sites(map_site2msphere, spheres_and_1rings);
 
-  // device RPD over all tets, ids back to spheres, merge
-  std::vector<ConvexCellHost> cells = compute_clipped_voro_diagram_GPU(
-      num_itr_rpd, tet_mesh->tet_vertices, tet_mesh->tet_indices,
-      tet_mesh->v2tets, tet_mesh->v_adjs, tet_mesh->e_adjs, tet_mesh->f_adjs,
-      tet_mesh->f_ids, site, n_site, site_weights, site_flags, site_knn,
-      site_k, site_cell_vol, site_is_transposed, 1,
-      (int)all_medial_spheres->size());
+  // device RPD over all tets of the resident mesh, ids back to spheres,
+  // merge
+  std::vector<ConvexCellHost> cells =
+      gpu().run(site, n_site, site_weights, site_flags, site_knn, site_k,
+                site_cell_vol);
   const std::vector<int> map_tet_new2orig;
   update_convex_cells_voro_and_tet_ids(map_site2msphere, map_tet_new2orig,
                                        cells, is_debug);
@@ -199,25 +197,23 @@ void RPD3D_GPU_T<TM, SM, PM, SP>::calculate_partial(int& num_itr_global,
   this->site_k = k;
   load_partial_spheres_to_sites(map_site2msphere, spheres_and_1rings);
 
-  std::vector<int> tet_indices, tet_fids, tet_f_adjs, map_tet_new2orig;
+  // the partial tets are clipped in place on the resident mesh: the
+  // cells come back with their original tet ids, so no tet id remap
+  std::vector<ConvexCellHost> cells;
   if (all_tets) {
-    tet_indices = tet_mesh->tet_indices;
-    tet_fids = tet_mesh->f_ids;
-    tet_f_adjs = tet_mesh->f_adjs;
+    cells = gpu().run(site, n_site, site_weights, site_flags, site_knn,
+                      site_k, site_cell_vol);
   } else {
     // the previous cells of removed spheres must be re-covered too
     std::set<int> target = spheres_and_1rings;
     target.insert(spheres_invalid.begin(), spheres_invalid.end());
-    load_partial_tet_given_spheres(target, tet_indices, map_tet_new2orig,
+    std::vector<int> tet_indices, tet_fids, tet_f_adjs, partial_tets;
+    load_partial_tet_given_spheres(target, tet_indices, partial_tets,
                                    tet_fids, tet_f_adjs);
+    cells = gpu().run(site, n_site, site_weights, site_flags, site_knn,
+                      site_k, site_cell_vol, &partial_tets);
   }
-  std::vector<ConvexCellHost> cells;
-  if (!tet_indices.empty())
-    cells = compute_clipped_voro_diagram_GPU(
-        num_itr_global, tet_mesh->tet_vertices, tet_indices, tet_mesh->v2tets,
-        tet_mesh->v_adjs, tet_mesh->e_adjs, tet_f_adjs, tet_fids, site, n_site,
-        site_weights, site_flags, site_knn, site_k, site_cell_vol,
-        site_is_transposed, 1, (int)all_medial_spheres->size());
+  const std::vector<int> map_tet_new2orig;
   update_convex_cells_voro_and_tet_ids(map_site2msphere, map_tet_new2orig,
                                        cells, is_debug);
   powercells = merge_convex_cells(valid_sphere_ids, spheres_and_1rings,

@@ -295,6 +295,28 @@ int rpd_download_cells(rpd_ctx* ctx, rpd_cell* cells, uint32_t* words,
  * offsets[n_tet+1], ids[n_candidates]. */
 int rpd_download_relation(rpd_ctx* ctx, int64_t* offsets, int32_t* ids);
 
+/* Per-site centroids of the last run's cells: the Lloyd update of
+ * compute_new_site (voronoi.cu:40-69) over the per-seed barycentre and
+ * volume sums of the clip kernel (convex_cell.cu:1009-1054).  For each
+ * (batch-wide) site s: out[4s..4s+2] = sum(vol * centroid) / sum(vol) over
+ * its cells, out[4s+3] = sum(vol); a site without volume keeps its centre
+ * (compute_new_site skips cell_vol == 0).  FP64 in a fixed summation
+ * order, so the result does not depend on scheduling.  Needs a last run
+ * with compute_geometry on the current sites.  out: host, 4*n_site
+ * doubles (nullable); *dev_out (nullable) receives the device copy, owned
+ * by the context and valid until the next call. */
+int rpd_site_centroids(rpd_ctx* ctx, double* out, const double** dev_out);
+
+/* Restrict the following rpd_run calls to n tets of the resident mesh (ids
+ * in [0, n_tet)); other tets take no candidates.  The partial RPD of
+ * calculate_partial (rpd_api.cxx:249-262) clips only the tets of the
+ * changed spheres' previous cells; a tet's cells depend only on the tet
+ * and the spheres, so a subset run gives the cells of a full run on those
+ * tets, with their original tet ids, without sending a partial mesh.  n = 0
+ * clears the subset; rpd_upload / rpd_upload_batch clear it too.  Plain
+ * runs only (a learned run fails while a subset is set). */
+int rpd_set_tet_subset(rpd_ctx* ctx, int32_t n, const int32_t* tet_ids);
+
 /* ---- allocation-free RPD.  No counterpart in the reference: for callers
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  One call = rpd_upload +

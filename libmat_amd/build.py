@@ -18,7 +18,12 @@ OUT = os.path.join(HERE, "_lib", "librpd_amd.so")
 INC = os.path.join(ROOT, "include")
 ARCH = os.environ.get("RPD_OFFLOAD_ARCH", "gfx950")
 
-HOST_SRCS = ["csrc/host/rpd_prep.cpp", "csrc/host/rpd_compat.cpp"]
+HOST_SRCS = ["csrc/host/rpd_prep.cpp", "csrc/host/rpd_compat.cpp",
+             "csrc/host/rpd_cellhost.cpp"]
+# librpd_amd_core.so leaves out the ConvexCellHost methods and adjacency
+# (rpd_cellhost.cpp) for programs that compile LibMAT's voronoi_defs.cxx
+CORE_OUT = os.path.join(HERE, "_lib", "librpd_amd_core.so")
+CELLHOST_OBJ = "rpd_cellhost.cpp.o"
 HIP_SRCS = ["csrc/device/rpd_engine.hip"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
           "-fno-fast-math", f"-I{INC}"]
@@ -60,10 +65,13 @@ def build(jobs=4, verbose=False):
         for r in ex.map(_run, cmds):
             if verbose:
                 sys.stdout.write(r.stderr)
-    tmp = OUT + ".tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-o", tmp] + objs +
-         ["-lpthread"])
-    os.replace(tmp, OUT)
+    for out, these in ((OUT, objs),
+                       (CORE_OUT, [o for o in objs
+                                   if os.path.basename(o) != CELLHOST_OBJ])):
+        tmp = out + ".tmp"
+        _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-o", tmp] +
+             these + ["-lpthread"])
+        os.replace(tmp, out)
     return OUT
 
 

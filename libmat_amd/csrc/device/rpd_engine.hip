@@ -24,6 +24,7 @@
 #include "rpd_kernels.hip"
 #include "rpd_learn.hip"
 #include "rpd_nbr.hip"
+#include "rpd_cells.hip"
 
 // Device scratch of rpd_run_noalloc, the entry point for callers that
 // cannot allocate: every buffer of one upload + plain run + compaction is
@@ -226,7 +227,13 @@ struct rpd_ctx {
   Buf trip_k, trip_k2, trip_v, trip_v2;  // step-count order of a clip range
   TierBufs small, large, exact, mid;
   Buf out_cells, out_words, out_geom, out_areas;
+  bool last_geometry = false;  // the last run computed cell geometry
   int64_t n_knn = 0;
+  // tet subset of the partial RPD (one byte per tet, 1 = clipped)
+  Buf tet_sel;
+  bool has_subset = false;
+  // per-site centroids of the last run's cells
+  Buf cs_keys_a, cs_keys_b, cs_vals_a, cs_vals_b, cs_start, cs_out;
   // learned proposals: features, model (fp32 master, Adam state, bf16
   // copies), certificate and fallback
   Buf knn, feat, sel, omask, vowner, sg_keys_a, sg_keys_b, sg_start, sg_ids;
@@ -496,6 +503,7 @@ int rpd_update_sites_batch(rpd_ctx* ctx, int32_t n_inst,
   ctx->st.site_k = max_deg;
   HIP_TRY(wait_stream(ctx, ctx->stream));
   ctx->has_sites = true;
+  ctx->last_geometry = false;  // the last run's cells belong to other sites
   return 0;
 }
 
@@ -600,6 +608,7 @@ int rpd_upload_batch(rpd_ctx* ctx, int32_t n_inst,
   ctx->n_vert = (int32_t)nv_tot;
   ctx->n_tet = (int32_t)nt_tot;
   ctx->has_sites = false;
+  ctx->has_subset = false;
   HIP_TRY(ctx->vert.need((size_t)nv_tot * 32));
   HIP_TRY(ctx->tet_v.need((size_t)nt_tot * 16));
   HIP_TRY(ctx->tet_fid.need((size_t)nt_tot * 16));
@@ -932,7 +941,8 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
                      ctx->tet_count.as<int64_t>(), ctx->tet_pass.as<uint64_t>(),
                      R.cfg.count_work
                          ? ctx->rel_work.as<unsigned long long>()
-                         : nullptr);
+                         : nullptr,
+                     ctx->has_subset ? ctx->tet_sel.as<uint8_t>() : nullptr);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->ev[12], s));
   HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
@@ -945,7 +955,8 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
                      R.S, R.inst_d, R.tet_inst_d, ctx->bin_start.as<int32_t>(),
                      ctx->bin_ids.as<int32_t>(), ctx->tet_off.as<int64_t>(),
                      ctx->tet_pass.as<uint64_t>(), ctx->cand.as<Cand>(),
-                     ctx->rel_ids.as<int32_t>());
+                     ctx->rel_ids.as<int32_t>(),
+                     ctx->has_subset ? ctx->tet_sel.as<uint8_t>() : nullptr);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->ev[13], s));
   *n_cand_out = n_cand;
@@ -1581,6 +1592,10 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     return fail(ctx, "rpd_run: cells_per_wave must be 1, 2, 4, 8 or 16");
   if (cfg.small_caps >= 2 && R.G < 4)
     return fail(ctx, "rpd_run: small_caps 2 and 3 need cells_per_wave >= 4");
+  if (cfg.learned && ctx->has_subset)
+    return fail(ctx, "rpd_run: a tet subset is for the plain path only "
+                     "(learned = 0)");
+  ctx->last_geometry = false;
   const int32_t site_k = ctx->st.site_k;
   std::memset(&ctx->st, 0, sizeof(ctx->st));
   ctx->st.site_k = site_k;
@@ -1703,6 +1718,87 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   st.ms_clip_small_sum = 0;
   for (int k = 0; k < ctx->n_small_ev; ++k)
     st.ms_clip_small_sum += el(16 + 2 * k, 17 + 2 * k);
+  ctx->last_geometry = cfg.compute_geometry != 0;
+  return 0;
+}
+
+int rpd_set_tet_subset(rpd_ctx* ctx, int32_t n, const int32_t* tet_ids) {
+  if (!ctx) return -1;
+  if (!ctx->has_mesh) return fail(ctx, "rpd_set_tet_subset: no resident mesh");
+  if (n < 0 || (n > 0 && !tet_ids))
+    return fail(ctx, "rpd_set_tet_subset: bad arguments");
+  if (n == 0) {
+    ctx->has_subset = false;
+    return 0;
+  }
+  std::vector<uint8_t> sel(ctx->n_tet, 0);
+  for (int32_t i = 0; i < n; ++i) {
+    if (tet_ids[i] < 0 || tet_ids[i] >= ctx->n_tet)
+      return fail(ctx, "rpd_set_tet_subset: tet id " +
+                           std::to_string(tet_ids[i]) + " out of range");
+    sel[tet_ids[i]] = 1;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(ctx->tet_sel.need((size_t)ctx->n_tet));
+  HIP_TRY(hipMemcpyAsync(ctx->tet_sel.p, sel.data(), sel.size(),
+                         hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(wait_stream(ctx, ctx->stream));
+  ctx->has_subset = true;
+  return 0;
+}
+
+int rpd_site_centroids(rpd_ctx* ctx, double* out, const double** dev_out) {
+  if (!ctx) return -1;
+  if (!ctx->has_sites) return fail(ctx, "rpd_site_centroids: no resident sites");
+  if (!ctx->last_geometry)
+    return fail(ctx, "rpd_site_centroids: the last rpd_run computed no cell "
+                     "geometry (compute_geometry = 0, or no run)");
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int32_t ns = ctx->n_site;
+  const int64_t n = ctx->st.n_cells;
+  if (n >= (1LL << 31))
+    return fail(ctx, "rpd_site_centroids: too many cells");
+  HIP_TRY(ctx->cs_keys_a.need((size_t)std::max<int64_t>(n, 1) * 4));
+  HIP_TRY(ctx->cs_keys_b.need((size_t)std::max<int64_t>(n, 1) * 4));
+  HIP_TRY(ctx->cs_vals_a.need((size_t)std::max<int64_t>(n, 1) * 4));
+  HIP_TRY(ctx->cs_vals_b.need((size_t)std::max<int64_t>(n, 1) * 4));
+  HIP_TRY(ctx->cs_start.need(((size_t)ns + 1) * 8));
+  HIP_TRY(ctx->cs_out.need((size_t)std::max(ns, 1) * 32));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_cell_seed_keys, dim3(ceil_div(n, 256)), dim3(256), 0,
+                       s, n, ctx->out_cells.as<rpd_cell>(),
+                       ctx->cs_keys_a.as<uint32_t>(),
+                       ctx->cs_vals_a.as<int32_t>());
+    HIP_TRY(hipGetLastError());
+    int end_bit = 1;
+    while (end_bit < 32 && (1LL << end_bit) <= (int64_t)ns) ++end_bit;
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+        nullptr, tmp, ctx->cs_keys_a.as<uint32_t>(),
+        ctx->cs_keys_b.as<uint32_t>(), ctx->cs_vals_a.as<int32_t>(),
+        ctx->cs_vals_b.as<int32_t>(), (int)n, 0, end_bit, s));
+    HIP_TRY(ctx->scan_tmp.need(tmp));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+        ctx->scan_tmp.p, tmp, ctx->cs_keys_a.as<uint32_t>(),
+        ctx->cs_keys_b.as<uint32_t>(), ctx->cs_vals_a.as<int32_t>(),
+        ctx->cs_vals_b.as<int32_t>(), (int)n, 0, end_bit, s));
+  }
+  hipLaunchKernelGGL(k_seed_starts, dim3(ceil_div((int64_t)ns + 1, 256)),
+                     dim3(256), 0, s, ns, n, ctx->cs_keys_b.as<uint32_t>(),
+                     ctx->cs_start.as<int64_t>());
+  HIP_TRY(hipGetLastError());
+  if (ns > 0)
+    hipLaunchKernelGGL(k_site_centroid, dim3(ceil_div(ns, 4)), dim3(256), 0, s,
+                       ns, ctx->cs_start.as<int64_t>(),
+                       ctx->cs_vals_b.as<int32_t>(), ctx->out_geom.as<double>(),
+                       ctx->site.as<double4>(), ctx->cs_out.as<double>());
+  HIP_TRY(hipGetLastError());
+  if (out && ns > 0)
+    HIP_TRY(hipMemcpyAsync(out, ctx->cs_out.p, (size_t)ns * 32,
+                           hipMemcpyDeviceToHost, s));
+  HIP_TRY(wait_stream(ctx, s));
+  if (dev_out) *dev_out = ctx->cs_out.as<double>();
   return 0;
 }
 
@@ -1920,10 +2016,11 @@ int rpd_run_noalloc(rpd_ctx* ctx, const rpd_instance_desc* desc,
     std::vector<rpd_ctx::Inst> inst;
     int32_t n_vert, n_tet, n_site;
     int64_t n_knn;
-    bool has_mesh, has_sites;
+    bool has_mesh, has_sites, has_subset, last_geometry;
     rpd_stats st;
-  } sv{ctx->inst,     ctx->n_vert,   ctx->n_tet,     ctx->n_site,
-       ctx->n_knn,    ctx->has_mesh, ctx->has_sites, ctx->st};
+  } sv{ctx->inst,     ctx->n_vert,   ctx->n_tet,      ctx->n_site,
+       ctx->n_knn,    ctx->has_mesh, ctx->has_sites,  ctx->has_subset,
+       ctx->last_geometry, ctx->st};
   Arena A;
   A.base = static_cast<char*>(ctx->scratch_dev);
   A.cap = kScratchBytes;
@@ -1944,6 +2041,8 @@ int rpd_run_noalloc(rpd_ctx* ctx, const rpd_instance_desc* desc,
       c->n_knn = s.n_knn;
       c->has_mesh = s.has_mesh;
       c->has_sites = s.has_sites;
+      c->has_subset = s.has_subset;
+      c->last_geometry = s.last_geometry;
       c->st = s.st;
     }
   } guard{ctx, A, sv};

@@ -16,23 +16,6 @@ static_assert(alignof(ConvexCellHost) == 32, "ConvexCellHost alignment");
 
 namespace {
 
-inline float det2f(float a11, float a12, float a21, float a22) {
-  return a11 * a22 - a12 * a21;
-}
-inline float det3f(float a11, float a12, float a13, float a21, float a22,
-                   float a23, float a31, float a32, float a33) {
-  return a11 * det2f(a22, a23, a32, a33) - a21 * det2f(a12, a13, a32, a33) +
-         a31 * det2f(a12, a13, a22, a23);
-}
-
-template <class T>
-void set_intersection2(const std::set<T>& a, const std::set<T>& b,
-                       std::set<T>& out) {
-  out.clear();
-  std::set_intersection(a.begin(), a.end(), b.begin(), b.end(),
-                        std::inserter(out, out.begin()));
-}
-
 std::mutex g_mu;
 rpd_ctx* g_ctx = nullptr;
 
@@ -60,69 +43,116 @@ inline int64_t tri_edge_idx(int64_t v1, int64_t v2, int64_t n) {
   return (lo + 1) * n - lo * (lo + 1) / 2 - (n - hi);
 }
 
-std::vector<ConvexCellHost> run_rpd(
-    const std::vector<float>& vertices, const std::vector<int>& indices,
-    const std::vector<int>& v_adjs, const std::vector<int>& tet_e_adjs,
-    const std::vector<int>& f_adjs, const std::vector<int>& f_ids,
-    const std::vector<float>& site, int n_site,
-    const std::vector<float>& site_weights, const std::vector<uint>& site_flags,
-    const std::vector<int>& site_knn, int site_k,
-    std::vector<float>& site_cell_vol) {
-  const int32_t n_vert = (int32_t)(vertices.size() / 3);
-  const int32_t n_tet = (int32_t)(indices.size() / 4);
-  if (n_tet <= 0 || n_vert <= 0)
-    throw std::runtime_error("RPD: empty tet mesh");
-  if ((int)site.size() < 3 * n_site || (int)site_weights.size() < n_site ||
-      (int)site_flags.size() < n_site ||
-      (int64_t)site_knn.size() < (int64_t)site_k * n_site)
-    throw std::runtime_error("RPD: site arrays too short");
-  if ((int)v_adjs.size() < n_vert || (int64_t)f_adjs.size() < 4LL * n_tet ||
-      (int64_t)f_ids.size() < 4LL * n_tet ||
-      (int64_t)tet_e_adjs.size() < 6LL * n_tet)
-    throw std::runtime_error("RPD: adjacency arrays too short");
-  std::vector<double> verts(vertices.begin(), vertices.begin() + 3LL * n_vert);
-  std::vector<double> sites(4LL * n_site);
-  for (int i = 0; i < n_site; ++i) {
-    sites[4 * i + 0] = site[i];
-    sites[4 * i + 1] = site[i + n_site];
-    sites[4 * i + 2] = site[i + 2 * n_site];
-    sites[4 * i + 3] = site_weights[i];
-  }
-  std::vector<uint32_t> flags(site_flags.begin(), site_flags.begin() + n_site);
-  std::vector<int64_t> knn_off(n_site + 1, 0);
-  std::vector<int32_t> knn_ids;
-  for (int i = 0; i < n_site; ++i) {
-    for (int r = 0; r < site_k; ++r) {
-      int m = site_knn[(int64_t)r * n_site + i];
-      if (m == -1) break;
-      knn_ids.push_back(m);
-    }
-    knn_off[i + 1] = (int64_t)knn_ids.size();
-  }
-  rpd_instance_desc d;
-  d.n_vert = n_vert;
-  d.n_tet = n_tet;
-  d.n_site = n_site;
-  d.verts = verts.data();
-  d.tets = indices.data();
-  d.v_adj = v_adjs.data();
-  d.e_adj = tet_e_adjs.data();
-  d.f_adj = f_adjs.data();
-  d.f_id = f_ids.data();
-  d.sites = sites.data();
-  d.flags = flags.data();
-  d.knn_off = knn_off.data();
-  d.knn_ids = knn_ids.data();
-
-  std::lock_guard<std::mutex> lock(g_mu);
-  rpd_ctx* ctx = shared_ctx();
-  check(ctx, rpd_upload(ctx, &d), "rpd_upload");
+// the drop-in's run configuration
+rpd_run_cfg dropin_cfg() {
   rpd_run_cfg cfg{};
   cfg.cells_per_wave = 8;
   cfg.use_active_mask = 1;
   cfg.compute_geometry = 1;
   cfg.small_tier = 1;
-  check(ctx, rpd_run(ctx, &cfg), "rpd_run");
+  return cfg;
+}
+
+void check_site_arrays(const std::vector<float>& site, int n_site,
+                       const std::vector<float>& site_weights,
+                       const std::vector<uint>& site_flags,
+                       const std::vector<int>& site_knn, int site_k) {
+  if (n_site < 0 || (int64_t)site.size() < 3LL * n_site ||
+      (int64_t)site_weights.size() < n_site ||
+      (int64_t)site_flags.size() < n_site ||
+      (int64_t)site_knn.size() < (int64_t)site_k * n_site)
+    throw std::runtime_error("RPD: site arrays too short");
+}
+
+// the spheres as the C ABI takes them: x, y, z, w = r^2 in binary64,
+// neighbour lists as CSR (the column-major site_knn, -1 ends a list)
+struct SiteArrays {
+  std::vector<double> sites;
+  std::vector<uint32_t> flags;
+  std::vector<int64_t> knn_off;
+  std::vector<int32_t> knn_ids;
+};
+
+SiteArrays site_arrays(const std::vector<float>& site, int n_site,
+                       const std::vector<float>& site_weights,
+                       const std::vector<uint>& site_flags,
+                       const std::vector<int>& site_knn, int site_k) {
+  SiteArrays a;
+  a.sites.resize(4LL * n_site);
+  for (int i = 0; i < n_site; ++i) {
+    a.sites[4 * i + 0] = site[i];
+    a.sites[4 * i + 1] = site[i + n_site];
+    a.sites[4 * i + 2] = site[i + 2 * n_site];
+    a.sites[4 * i + 3] = site_weights[i];
+  }
+  a.flags.assign(site_flags.begin(), site_flags.begin() + n_site);
+  a.knn_off.assign(n_site + 1, 0);
+  for (int i = 0; i < n_site; ++i) {
+    for (int r = 0; r < site_k; ++r) {
+      int m = site_knn[(int64_t)r * n_site + i];
+      if (m == -1) break;
+      a.knn_ids.push_back(m);
+    }
+    a.knn_off[i + 1] = (int64_t)a.knn_ids.size();
+  }
+  return a;
+}
+
+// the tet mesh as the C ABI takes it (binary64 vertices, per-tet edge
+// counts), checked against the lengths the reference indexes
+struct MeshArrays {
+  int32_t n_vert = 0, n_tet = 0;
+  std::vector<double> verts;
+};
+
+MeshArrays mesh_arrays(const std::vector<float>& vertices,
+                       const std::vector<int>& indices,
+                       const std::vector<int>& v_adjs,
+                       const std::vector<int>& tet_e_adjs,
+                       const std::vector<int>& f_adjs,
+                       const std::vector<int>& f_ids) {
+  MeshArrays m;
+  m.n_vert = (int32_t)(vertices.size() / 3);
+  m.n_tet = (int32_t)(indices.size() / 4);
+  if (m.n_tet <= 0 || m.n_vert <= 0)
+    throw std::runtime_error("RPD: empty tet mesh");
+  if ((int64_t)v_adjs.size() < m.n_vert ||
+      (int64_t)f_adjs.size() < 4LL * m.n_tet ||
+      (int64_t)f_ids.size() < 4LL * m.n_tet ||
+      (int64_t)tet_e_adjs.size() < 6LL * m.n_tet)
+    throw std::runtime_error("RPD: adjacency arrays too short");
+  m.verts.assign(vertices.begin(), vertices.begin() + 3LL * m.n_vert);
+  return m;
+}
+
+// per-tet edge counts from the triangular n_vert^2 matrix of get_edge_idx
+std::vector<int> tet_edge_counts(const std::vector<int>& indices,
+                                 const std::vector<int>& e_adjs,
+                                 int64_t n_vert) {
+  const int64_t n_tet = (int64_t)indices.size() / 4;
+  std::vector<int> tet_e(6 * n_tet);
+  static const int el[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+  for (int64_t t = 0; t < n_tet; ++t)
+    for (int k = 0; k < 6; ++k) {
+      int64_t idx = tri_edge_idx(indices[4 * t + el[k][0]],
+                                 indices[4 * t + el[k][1]], n_vert);
+      if (idx < 0 || idx >= (int64_t)e_adjs.size())
+        throw std::runtime_error("RPD: e_adjs index out of range");
+      tet_e[6 * t + k] = e_adjs[idx];
+    }
+  return tet_e;
+}
+
+// The last run's cells on ctx as ConvexCellHost records, planes in
+// binary32 with the reference's arithmetic.  tet_pos (nullable): position
+// of each tet in the clipped tet list of a partial run (the reference's
+// thread index counts tets of the partial mesh); else the tet id itself.
+std::vector<ConvexCellHost> cells_of_run(
+    rpd_ctx* ctx, int32_t n_tet, const std::vector<float>& vertices,
+    const std::vector<int>& indices, const std::vector<int>& f_adjs,
+    const std::vector<int>& f_ids, const std::vector<float>& site, int n_site,
+    const std::vector<float>& site_weights, std::vector<float>& site_cell_vol,
+    const std::vector<int>* tet_pos) {
   rpd_stats st;
   rpd_get_stats(ctx, &st);
   std::vector<rpd_cell> cells(st.n_cells);
@@ -175,7 +205,7 @@ std::vector<ConvexCellHost> run_rpd(
     for (int64_t r = rel_off[t]; r < rel_off[t + 1]; ++r, ++slot)
       if (rel_ids[r] == s) break;
     h.status = success;
-    h.thread_id = t * tet_k + slot;
+    h.thread_id = (tet_pos ? (*tet_pos)[t] : t) * tet_k + slot;
     h.voro_id = s;
     h.tet_id = t;
     h.weight = site_weights[s];
@@ -219,169 +249,45 @@ std::vector<ConvexCellHost> run_rpd(
   return out;
 }
 
+
+std::vector<ConvexCellHost> run_rpd(
+    const std::vector<float>& vertices, const std::vector<int>& indices,
+    const std::vector<int>& v_adjs, const std::vector<int>& tet_e_adjs,
+    const std::vector<int>& f_adjs, const std::vector<int>& f_ids,
+    const std::vector<float>& site, int n_site,
+    const std::vector<float>& site_weights, const std::vector<uint>& site_flags,
+    const std::vector<int>& site_knn, int site_k,
+    std::vector<float>& site_cell_vol) {
+  const MeshArrays m =
+      mesh_arrays(vertices, indices, v_adjs, tet_e_adjs, f_adjs, f_ids);
+  check_site_arrays(site, n_site, site_weights, site_flags, site_knn, site_k);
+  const SiteArrays s =
+      site_arrays(site, n_site, site_weights, site_flags, site_knn, site_k);
+  rpd_instance_desc d;
+  d.n_vert = m.n_vert;
+  d.n_tet = m.n_tet;
+  d.n_site = n_site;
+  d.verts = m.verts.data();
+  d.tets = indices.data();
+  d.v_adj = v_adjs.data();
+  d.e_adj = tet_e_adjs.data();
+  d.f_adj = f_adjs.data();
+  d.f_id = f_ids.data();
+  d.sites = s.sites.data();
+  d.flags = s.flags.data();
+  d.knn_off = s.knn_off.data();
+  d.knn_ids = s.knn_ids.data();
+
+  std::lock_guard<std::mutex> lock(g_mu);
+  rpd_ctx* ctx = shared_ctx();
+  check(ctx, rpd_upload(ctx, &d), "rpd_upload");
+  const rpd_run_cfg cfg = dropin_cfg();
+  check(ctx, rpd_run(ctx, &cfg), "rpd_run");
+  return cells_of_run(ctx, m.n_tet, vertices, indices, f_adjs, f_ids, site,
+                      n_site, site_weights, site_cell_vol, nullptr);
+}
+
 }  // namespace
-
-// ------------------------------------------------------- ConvexCellHost
-cfloat4 ConvexCellHost::compute_vertex_coordinates(cuchar3 t,
-                                                   bool persp_divide) const {
-  cfloat4 p1 = clip_trans_const(t.x), p2 = clip_trans_const(t.y),
-          p3 = clip_trans_const(t.z);
-  cfloat4 r;
-  r.x = -det3f(p1.w, p1.y, p1.z, p2.w, p2.y, p2.z, p3.w, p3.y, p3.z);
-  r.y = -det3f(p1.x, p1.w, p1.z, p2.x, p2.w, p2.z, p3.x, p3.w, p3.z);
-  r.z = -det3f(p1.x, p1.y, p1.w, p2.x, p2.y, p2.w, p3.x, p3.y, p3.w);
-  r.w = det3f(p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z);
-  if (persp_divide) return cfloat4{r.x / r.w, r.y / r.w, r.z / r.w, 1};
-  return r;
-}
-
-void ConvexCellHost::print_info() const {
-  printf(
-      "Cell of id: %d, voro_id: %d, sq_radius: %f, tet_id: %d, nb_v: %d, "
-      "nb_p: %d, nb_e: %d, status: %d, thread_id: %d\n",
-      id, voro_id, weight, tet_id, nb_v, nb_p, nb_e, (int)status, thread_id);
-}
-
-void ConvexCellHost::reload_active() {
-  active_clipping_planes.assign(nb_p + 1, 0);
-  std::map<int, std::set<int>> plane2v;
-  for (int t = 0; t < nb_v; ++t)
-    for (int k = 0; k < 3; ++k) {
-      int p = ith_plane(t, k);
-      active_clipping_planes[p]++;
-      plane2v[p].insert(t);
-    }
-  active_edges.assign(nb_e + 1, 0);
-  std::set<int> common;
-  for (int e = 0; e < nb_e; ++e) {
-    cuchar3 ed = edge_data[e];
-    if (active_clipping_planes[ed.x] <= 0 || active_clipping_planes[ed.y] <= 0)
-      continue;
-    set_intersection2(plane2v.at(ed.x), plane2v.at(ed.y), common);
-    if (common.size() < 2) continue;
-    active_edges[e]++;
-  }
-  is_active_updated = true;
-}
-
-void ConvexCellHost::reload_pc_explicit() {
-  pc_points.clear();
-  for (int i = 0; i < nb_v; ++i) {
-    cuchar4 v = ver_trans(i);
-    cfloat4 p = compute_vertex_coordinates(cuchar3{v.x, v.y, v.z});
-    if (std::isnan(p.x) || std::isnan(p.y) || std::isnan(p.z)) {
-      is_vertex_null = true;
-      return;
-    }
-    pc_points.push_back(cfloat3{p.x, p.y, p.z});
-  }
-  if (!is_active_updated) reload_active();
-  pc_local_active_faces.clear();
-  pc_lf2active_map.assign(nb_p, -1);
-  int lf = 0;
-  for (int plane = 0; plane < nb_p; ++plane) {
-    if (active_clipping_planes[plane] <= 0) continue;
-    std::vector<int> tab_v, tab_lp;
-    for (int t = 0; t < nb_v; ++t)
-      for (int k = 0; k < 3; ++k)
-        if (ith_plane(t, k) == plane) {
-          tab_v.push_back(t);
-          tab_lp.push_back(k);
-          break;
-        }
-    // walk the face: next of (t, slot) is the vertex whose plane at
-    // (slot' + 2) % 3 equals t's plane at (slot + 1) % 3
-    std::vector<int> face;
-    int i = 0;
-    std::vector<char> used(tab_v.size(), 0);
-    while (face.size() < tab_v.size()) {
-      face.push_back(tab_v[i]);
-      used[i] = 1;
-      int want = ith_plane(tab_v[i], (tab_lp[i] + 1) % 3);
-      int nxt = -1;
-      for (size_t j = 0; j < tab_v.size(); ++j)
-        if (!used[j] && ith_plane(tab_v[j], (tab_lp[j] + 2) % 3) == want) {
-          nxt = (int)j;
-          break;
-        }
-      if (nxt < 0) break;
-      i = nxt;
-    }
-    pc_local_active_faces.push_back(face);
-    pc_lf2active_map[plane] = lf++;
-  }
-  is_pc_explicit = true;
-}
-
-// Euler sums with the reference's rounding (voronoi_defs.cxx:190-267):
-// vertex terms are binary32 quotients 1.f / #adjacent cells, face and edge
-// terms binary64 quotients, accumulated in binary64 (Scalar), and the
-// result passes through a float
-Scalar ConvexCellHost::cal_cell_euler() {
-  Scalar sv = 0, sf = 0, se = 0;
-  if (!is_active_updated) reload_active();
-  for (int v = 0; v < nb_v; ++v) sv += 1.f / (int)ver_trans(v).w;
-  for (int f = 0; f < nb_p; ++f)
-    if (active_clipping_planes[f] > 0) sf += 1. / clip_trans(f).h;
-  for (int e = 0; e < nb_e; ++e)
-    if (active_edges[e] > 0) se += 1. / edge(e).z;
-  euler = (float)(sv - se + sf);
-  return euler;
-}
-
-Scalar ConvexCellHost::cal_halfplane_facet_euler(int neigh_id) {
-  if (!is_active_updated) reload_active();
-  int clip_id = -1;
-  for (int f = 0; f < nb_p; ++f) {
-    if (active_clipping_planes[f] <= 0) continue;
-    cint2 id2 = clip_id2_const(f);
-    if (id2.x == neigh_id || id2.y == neigh_id) {
-      clip_id = f;
-      break;
-    }
-  }
-  if (clip_id < 0) throw std::runtime_error("halfplane facet not found");
-  Scalar sv = 0, se = 0, sf = 1;
-  for (int v = 0; v < nb_v; ++v) {
-    cuchar4 t = ver_trans(v);
-    if (t.x != clip_id && t.y != clip_id && t.z != clip_id) continue;
-    sv += 1.f / (int)t.w;
-  }
-  for (int e = 0; e < nb_e; ++e) {
-    cuchar3 ed = edge(e);
-    if (ed.x != clip_id && ed.y != clip_id) continue;
-    if (active_edges[e] <= 0) continue;
-    se += 1. / ed.z;
-  }
-  const float facet_euler = (float)(sv - se + sf);
-  return facet_euler;
-}
-
-void ConvexCellHost::print_cell_detail_euler() const {
-  // (voronoi_defs.cxx:269-291) needs reload_active() first
-  std::printf("--------- \n");
-  for (int v = 0; v < nb_v; ++v)
-    std::printf("[Detail] seed %d id %d has vertex %d with #adj_cells: %d \n",
-                voro_id, id, v, (int)ver_data_trans[v].w);
-  for (int e = 0; e < nb_e; ++e) {
-    if (e >= (int)active_edges.size() || active_edges[e] <= 0) continue;
-    std::printf("[Detail] seed %d id %d has edge %d: (%d, %d) with "
-                "#adj_cells: %d\n",
-                voro_id, id, e, (int)edge_data[e].x, (int)edge_data[e].y,
-                (int)edge_data[e].z);
-  }
-  for (int f = 0; f < nb_p; ++f) {
-    if (f >= (int)active_clipping_planes.size() ||
-        active_clipping_planes[f] <= 0)
-      continue;
-    std::printf("[Detail] seed %d id %d has plane %d with #adj_cells: %f \n",
-                voro_id, id, f, clip_data_trans[f].h);
-  }
-}
-
-bool is_convex_cell_valid(const ConvexCellHost& cell) {
-  return cell.status == success || cell.status == security_radius_not_reached;
-}
 
 // -------------------------------------------------------- entry points
 std::vector<ConvexCellHost> compute_clipped_voro_diagram_GPU(
@@ -393,18 +299,8 @@ std::vector<ConvexCellHost> compute_clipped_voro_diagram_GPU(
     const std::vector<float>& site_weights, const std::vector<uint>& site_flags,
     const std::vector<int>& site_knn, const int site_k,
     std::vector<float>& site_cell_vol, const bool, int, int) {
-  const int64_t n_vert = (int64_t)vertices.size() / 3;
-  const int64_t n_tet = (int64_t)indices.size() / 4;
-  std::vector<int> tet_e(6 * n_tet);
-  static const int el[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
-  for (int64_t t = 0; t < n_tet; ++t)
-    for (int k = 0; k < 6; ++k) {
-      int64_t idx = tri_edge_idx(indices[4 * t + el[k][0]],
-                                 indices[4 * t + el[k][1]], n_vert);
-      if (idx < 0 || idx >= (int64_t)e_adjs.size())
-        throw std::runtime_error("RPD: e_adjs index out of range");
-      tet_e[6 * t + k] = e_adjs[idx];
-    }
+  const std::vector<int> tet_e =
+      tet_edge_counts(indices, e_adjs, (int64_t)vertices.size() / 3);
   return run_rpd(vertices, indices, v_adjs, tet_e, f_adjs, f_ids, site,
                  n_site, site_weights, site_flags, site_knn, site_k,
                  site_cell_vol);
@@ -454,216 +350,89 @@ void libmat_rpd::power_neighbors(
   }
 }
 
-// ------------------------------------------------------------ adjacency
-int rpd_connected_components(const std::set<int>& to_visit,
-                             const std::map<int, std::set<int>>& neighbors,
-                             std::vector<std::set<int>>& out) {
-  out.clear();
-  std::set<int> seen;
-  for (int start : to_visit) {  // ascending: components by least element
-    if (seen.count(start)) continue;
-    std::set<int> comp;
-    std::vector<int> stack{start};
-    seen.insert(start);
-    while (!stack.empty()) {
-      const int c = stack.back();
-      stack.pop_back();
-      comp.insert(c);
-      auto it = neighbors.find(c);
-      if (it == neighbors.end()) continue;
-      for (int nb : it->second)
-        if (to_visit.count(nb) && seen.insert(nb).second) stack.push_back(nb);
+// ------------------------------------------------------- resident mesh
+namespace libmat_rpd {
+
+ResidentMesh::ResidentMesh(int device) : device_(device) {}
+
+ResidentMesh::~ResidentMesh() {
+  if (ctx_) rpd_ctx_destroy(ctx_);
+}
+
+void ResidentMesh::upload(const std::vector<float>& vertices,
+                          const std::vector<int>& indices,
+                          const std::vector<int>& v_adjs,
+                          const std::vector<int>& e_adjs,
+                          const std::vector<int>& f_adjs,
+                          const std::vector<int>& f_ids) {
+  const std::vector<int> tet_e =
+      tet_edge_counts(indices, e_adjs, (int64_t)vertices.size() / 3);
+  const MeshArrays m =
+      mesh_arrays(vertices, indices, v_adjs, tet_e, f_adjs, f_ids);
+  if (!ctx_) {
+    const int rc = rpd_ctx_create(device_, &ctx_);
+    if (rc != 0) {
+      ctx_ = nullptr;
+      throw std::runtime_error("RPD: no usable GPU (rpd_ctx_create " +
+                               std::to_string(rc) + ")");
     }
-    out.push_back(std::move(comp));
   }
-  return (int)out.size();
+  n_tet_ = 0;
+  rpd_instance_desc d{};
+  d.n_vert = m.n_vert;
+  d.n_tet = m.n_tet;
+  d.n_site = 0;  // mesh only: the spheres come with each run
+  d.verts = m.verts.data();
+  d.tets = indices.data();
+  d.v_adj = v_adjs.data();
+  d.e_adj = tet_e.data();
+  d.f_adj = f_adjs.data();
+  d.f_id = f_ids.data();
+  check(ctx_, rpd_upload(ctx_, &d), "rpd_upload");
+  vertices_.assign(vertices.begin(), vertices.begin() + 3LL * m.n_vert);
+  indices_.assign(indices.begin(), indices.begin() + 4LL * m.n_tet);
+  f_adjs_.assign(f_adjs.begin(), f_adjs.begin() + 4LL * m.n_tet);
+  f_ids_.assign(f_ids.begin(), f_ids.begin() + 4LL * m.n_tet);
+  n_tet_ = m.n_tet;
 }
 
-namespace {
-
-rpd_vec3 vertex_pos(const ConvexCellHost& c, int v) {
-  const cuchar4 w = c.ver_trans_const(v);
-  const cfloat4 p = c.compute_vertex_coordinates(cuchar3{w.x, w.y, w.z});
-  return rpd_vec3{{(double)p.x, (double)p.y, (double)p.z}};
-}
-
-// the two local vertices shared by two active faces of an explicit cell
-aint2 shared_vertices(const ConvexCellHost& c, int lf1, int lf2) {
-  const auto& f1 = c.pc_local_active_faces.at(c.pc_lf2active_map.at(lf1));
-  const auto& f2 = c.pc_local_active_faces.at(c.pc_lf2active_map.at(lf2));
-  std::set<int> s1(f1.begin(), f1.end()), s2(f2.begin(), f2.end());
-  std::vector<int> common;
-  std::set_intersection(s1.begin(), s1.end(), s2.begin(), s2.end(),
-                        std::back_inserter(common));
-  if (common.size() != 2)
-    throw std::runtime_error("adjacency: an active edge without two vertices");
-  return aint2{{common[0], common[1]}};
-}
-
-std::vector<std::vector<rpd_v2int>> surf_groups(
-    const std::map<int, std::vector<rpd_v2int>>& cell_to_surfv2fid,
-    const std::vector<std::set<int>>& groups) {
-  std::vector<std::vector<rpd_v2int>> out;
-  for (const auto& g : groups) {
-    std::vector<rpd_v2int> v;
-    for (int cid : g) {
-      auto it = cell_to_surfv2fid.find(cid);
-      if (it != cell_to_surfv2fid.end())
-        v.insert(v.end(), it->second.begin(), it->second.end());
+std::vector<ConvexCellHost> ResidentMesh::run(
+    const std::vector<float>& site, int n_site,
+    const std::vector<float>& site_weights, const std::vector<uint>& site_flags,
+    const std::vector<int>& site_knn, int site_k,
+    std::vector<float>& site_cell_vol, const std::vector<int>* tets) {
+  if (!ctx_ || n_tet_ <= 0)
+    throw std::runtime_error("RPD: ResidentMesh::run before upload");
+  check_site_arrays(site, n_site, site_weights, site_flags, site_knn, site_k);
+  const SiteArrays s =
+      site_arrays(site, n_site, site_weights, site_flags, site_knn, site_k);
+  check(ctx_,
+        rpd_update_sites(ctx_, n_site, s.sites.data(), s.flags.data(),
+                         s.knn_off.data(), s.knn_ids.data()),
+        "rpd_update_sites");
+  std::vector<int> pos;
+  if (tets) {
+    if (tets->empty()) {  // nothing to clip
+      site_cell_vol.assign(n_site, 0.f);
+      return {};
     }
-    out.push_back(std::move(v));
+    pos.assign(n_tet_, 0);
+    for (size_t i = 0; i < tets->size(); ++i) {
+      const int t = (*tets)[i];
+      if (t < 0 || t >= n_tet_)
+        throw std::runtime_error("RPD: tet id out of range");
+      pos[t] = (int)i;
+    }
   }
-  return out;
+  check(ctx_,
+        rpd_set_tet_subset(ctx_, tets ? (int32_t)tets->size() : 0,
+                           tets ? tets->data() : nullptr),
+        "rpd_set_tet_subset");
+  const rpd_run_cfg cfg = dropin_cfg();
+  check(ctx_, rpd_run(ctx_, &cfg), "rpd_run");
+  return cells_of_run(ctx_, n_tet_, vertices_, indices_, f_adjs_, f_ids_, site,
+                      n_site, site_weights, site_cell_vol,
+                      tets ? &pos : nullptr);
 }
 
-}  // namespace
-
-std::map<int, PowerCellAdjacency> update_power_cells_adjacency(
-    std::vector<ConvexCellHost>& cells, int max_surf_fid,
-    const std::map<aint3, aint3>* tet_es2fe_map, int n_threads) {
-  std::map<int, PowerCellAdjacency> pcs;
-  for (auto& c : cells) {
-    auto& pc = pcs[c.voro_id];
-    pc.voro_id = c.voro_id;
-    pc.cell_ids.insert(c.id);
-    pc.tet_ids.insert(c.tet_id);
-    if (!c.is_active_updated) c.reload_active();
-    if (!c.is_pc_explicit) c.reload_pc_explicit();
-  }
-  std::vector<int> keys;
-  for (auto& kv : pcs) keys.push_back(kv.first);
-  std::map<int, int> id2pos;
-  for (size_t q = 0; q < cells.size(); ++q) id2pos[cells[q].id] = (int)q;
-  auto neigh_of = [](const ConvexCellHost& c, cint2 hp) {
-    return hp.x == c.voro_id ? hp.y : hp.x;
-  };
-  auto work = [&](int sphere) {
-    PowerCellAdjacency& pc = pcs.at(sphere);
-    std::map<int, std::set<int>> tfid_to_cells;
-    for (int cid : pc.cell_ids) {
-      const ConvexCellHost& c = cells[id2pos.at(cid)];
-      // facets: halfplanes by neighbour, tet faces (and their centroid on
-      // the surface) by face id
-      int lfa = 0;
-      for (int p = 0; p < c.nb_p; ++p) {
-        if (c.active_clipping_planes[p] <= 0) continue;
-        cint2 hp = c.clip_id2_const(p);
-        if (hp.y != -1) {
-          pc.facet_neigh_to_cells[neigh_of(c, hp)].insert(cid);
-        } else {
-          if (hp.x <= max_surf_fid) {
-            cfloat3 cen{0.f, 0.f, 0.f};
-            const auto& face = c.pc_local_active_faces.at(lfa);
-            for (int lv : face) {
-              cen.x += c.pc_points.at(lv).x;
-              cen.y += c.pc_points.at(lv).y;
-              cen.z += c.pc_points.at(lv).z;
-            }
-            const float n = (float)face.size();
-            pc.cell_to_surfv2fid[cid].push_back(rpd_v2int(
-                rpd_vec3{{cen.x / n, cen.y / n, cen.z / n}}, hp.x));
-          }
-          tfid_to_cells[hp.x].insert(cid);
-          pc.cell_to_tfids[cid].insert(hp.x);
-        }
-        ++lfa;
-      }
-      // vertices of the power cell: 3 neighbour seeds, or 2 and -1 on the
-      // surface; convex-cell-only vertices are skipped
-      for (int v = 0; v < c.nb_v; ++v) {
-        std::set<int> seeds;
-        int surf_fid = -1;
-        for (int k = 0; k < 3; ++k) {
-          cint2 hp = c.clip_id2_const(c.ith_plane_const(v, k));
-          if (hp.y != -1)
-            seeds.insert(neigh_of(c, hp));
-          else if (hp.x <= max_surf_fid)
-            surf_fid = hp.x;
-        }
-        if (seeds.size() < 2) continue;
-        if (seeds.size() == 2) seeds.insert(-1);
-        aint3 key3;
-        int q = 0;
-        for (int sd : seeds) key3[q++] = sd;
-        std::sort(key3.begin(), key3.end());
-        const aint2 key{{cid, v}};
-        pc.vertex_2id[key] = key3;
-        pc.vertex_2pos[key] = rpd_v2int(vertex_pos(c, v), surf_fid);
-      }
-      // edges: feature edges of the tet; power-cell edges between two
-      // halfplanes with their end vertices
-      for (int e = 0; e < c.nb_e; ++e) {
-        if (c.active_edges[e] <= 0) continue;
-        cuchar2 ef = c.edge_id2_const(e);
-        cint2 h1 = c.clip_id2_const(ef.x), h2 = c.clip_id2_const(ef.y);
-        if (h1.y == -1 && h2.y == -1 && tet_es2fe_map) {
-          aint2 lfs{{ef.x, ef.y}};
-          std::sort(lfs.begin(), lfs.end());
-          auto it = tet_es2fe_map->find(aint3{{c.tet_id, lfs[0], lfs[1]}});
-          if (it != tet_es2fe_map->end()) {
-            const aint2 lv = shared_vertices(c, lfs[0], lfs[1]);
-            const aint3& fe = it->second;  // <fe_type, fe_id, fe_line_id>
-            const aint5 k5{{cid, lv[0], lv[1], fe[2], fe[1]}};
-            if (fe[0] == RPD_EDGE_SE)
-              pc.se_covered_lvids.insert(k5);
-            else if (fe[0] == RPD_EDGE_CE)
-              pc.ce_covered_lvids.insert(k5);
-            // an end of the feature line inside the power cell: the first
-            // halfplane at either end vertex names it
-            for (int lvid : lv)
-              for (int k = 0; k < 3; ++k) {
-                cint2 hp = c.clip_id2_const(c.ith_plane_const(lvid, k));
-                if (hp.y == -1) continue;
-                pc.se_line_endpos[aint4{{cid, lvid, neigh_of(c, hp), fe[2]}}] =
-                    vertex_pos(c, lvid);
-                break;
-              }
-          }
-          continue;
-        }
-        if (h1.y == -1 || h2.y == -1) continue;
-        aint2 eu{{neigh_of(c, h1), neigh_of(c, h2)}};
-        std::sort(eu.begin(), eu.end());
-        pc.e_to_cells[eu].insert(cid);
-        const aint2 ends = shared_vertices(c, ef.x, ef.y);
-        const aint2 k1{{cid, ends[0]}}, k2{{cid, ends[1]}};
-        if (!pc.vertex_2id.count(k1) || !pc.vertex_2id.count(k2))
-          throw std::runtime_error(
-              "adjacency: a power-cell edge ends at a non-power-cell vertex");
-        pc.edge_2endvertices[eu].push_back({{k1, k2}});
-      }
-    }
-    for (auto& kv : tfid_to_cells) {
-      if (kv.second.size() != 2) continue;
-      int a = *kv.second.begin(), b = *kv.second.rbegin();
-      pc.cell_neighbors[a].insert(b);
-      pc.cell_neighbors[b].insert(a);
-    }
-    // connected components (update_pc_cc_info, update_pc_facet_cc_info,
-    // update_pc_edge_cc_info)
-    rpd_connected_components(pc.cell_ids, pc.cell_neighbors, pc.cc_cells);
-    pc.cc_surf_v2fids = surf_groups(pc.cell_to_surfv2fid, pc.cc_cells);
-    for (auto& kv : pc.facet_neigh_to_cells) {
-      auto& groups = pc.facet_cc_cells[kv.first];
-      rpd_connected_components(kv.second, pc.cell_neighbors, groups);
-      std::stable_sort(groups.begin(), groups.end(),
-                       [](const std::set<int>& x, const std::set<int>& y) {
-                         return x.size() < y.size();
-                       });
-      pc.facet_cc_surf_v2fids[kv.first] =
-          surf_groups(pc.cell_to_surfv2fid, groups);
-    }
-    for (auto& kv : pc.e_to_cells)
-      rpd_connected_components(kv.second, pc.cell_neighbors,
-                               pc.edge_cc_cells[kv.first]);
-  };
-  int nt = n_threads > 0 ? n_threads : (int)std::thread::hardware_concurrency();
-  nt = std::max(1, std::min<int>(nt, (int)keys.size()));
-  std::vector<std::thread> pool;
-  for (int t = 0; t < nt; ++t)
-    pool.emplace_back([&, t]() {
-      for (size_t q = t; q < keys.size(); q += nt) work(keys[q]);
-    });
-  for (auto& th : pool) th.join();
-  return pcs;
-}
+}  // namespace libmat_rpd

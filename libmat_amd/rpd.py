@@ -156,6 +156,10 @@ def _declare_capi(L):
                                   ctypes.POINTER(NoAllocOut),
                                   ctypes.POINTER(Stats)]
     L.rpd_run_noalloc.restype = ctypes.c_int
+    L.rpd_site_centroids.argtypes = [vp, vp, ctypes.POINTER(vp)]
+    L.rpd_site_centroids.restype = ctypes.c_int
+    L.rpd_set_tet_subset.argtypes = [vp, i32, vp]
+    L.rpd_set_tet_subset.restype = ctypes.c_int
 
 
 CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
@@ -167,7 +171,8 @@ CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
                 "rpd_mlp_scores",
                 "rpd_sync", "rpd_get_stats", "rpd_download_cells",
                 "rpd_download_relation", "rpd_calib_copy",
-                "rpd_power_neighbors_gpu", "rpd_run_noalloc")
+                "rpd_power_neighbors_gpu", "rpd_run_noalloc",
+                "rpd_site_centroids", "rpd_set_tet_subset")
 
 
 def _p(a):
@@ -442,6 +447,31 @@ class Context:
     def calib_copy(self, nbytes):
         self._check(self._L.rpd_calib_copy(self._h, int(nbytes)),
                     "rpd_calib_copy")
+
+    def site_centroids(self):
+        """(n_site, 4): per site the volume-weighted centroid of its cells
+        in the last run and their total volume (a site without volume
+        keeps its centre)."""
+        n = int(self._site_count())
+        out = np.empty((max(n, 1), 4), np.float64)
+        self._check(self._L.rpd_site_centroids(self._h, _p(out), None),
+                    "rpd_site_centroids")
+        return out[:n]
+
+    def _site_count(self):
+        to, so = self.batch_offsets()
+        return so[-1]
+
+    def set_tet_subset(self, tet_ids=None):
+        """Clip only these tets of the resident mesh in the following
+        (plain) runs; None or empty clears the subset."""
+        if tet_ids is None or len(tet_ids) == 0:
+            self._check(self._L.rpd_set_tet_subset(self._h, 0, None),
+                        "rpd_set_tet_subset")
+            return
+        ids = np.ascontiguousarray(tet_ids, np.int32)
+        self._check(self._L.rpd_set_tet_subset(self._h, ids.size, _p(ids)),
+                    "rpd_set_tet_subset")
 
     def relation(self):
         st = self.stats()

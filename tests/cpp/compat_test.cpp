@@ -108,6 +108,60 @@ int main(int argc, char** argv) {
   }
   std::printf("total volume %.9f\n", vol);
   CHECK(std::fabs(vol - 1.0) < 1e-5);
+
+  // ---- a tet subset of the resident mesh against the reference's partial
+  // mesh (the tets' rows of indices / f_adjs / f_ids, in list order) run by
+  // compute_clipped_voro_diagram_GPU: the same cells, thread ids counting
+  // the partial tets; tet ids original vs partial positions
+  {
+    std::vector<int> sub;
+    for (int t = nt - 1; t >= 0; t -= 7) sub.push_back(t);
+    std::vector<int> pidx, pfadj, pfid;
+    for (int t : sub)
+      for (int j = 0; j < 4; ++j) {
+        pidx.push_back(tets[4 * t + j]);
+        pfadj.push_back(f_adj[4 * t + j]);
+        pfid.push_back(f_id[4 * t + j]);
+      }
+    std::vector<float> site = rpd.site, vol_a, vol_b;
+    std::vector<ConvexCellHost> a = compute_clipped_voro_diagram_GPU(
+        0, mesh.tet_vertices, pidx, mesh.v2tets, mesh.v_adjs, mesh.e_adjs,
+        pfadj, pfid, site, rpd.n_site, rpd.site_weights, rpd.site_flags,
+        rpd.site_knn, rpd.site_k, vol_a, true);
+    libmat_rpd::ResidentMesh rm(0);
+    rm.upload(mesh.tet_vertices, mesh.tet_indices, mesh.v_adjs, mesh.e_adjs,
+              mesh.f_adjs, mesh.f_ids);
+    std::vector<ConvexCellHost> b =
+        rm.run(rpd.site, rpd.n_site, rpd.site_weights, rpd.site_flags,
+               rpd.site_knn, rpd.site_k, vol_b, &sub);
+    CHECK(!a.empty() && a.size() == b.size());
+    std::map<std::array<int, 2>, const ConvexCellHost*> by;
+    for (const auto& c : b) by[{c.tet_id, c.voro_id}] = &c;
+    int same = 0;
+    for (const auto& c : a) {
+      auto it = by.find({sub.at(c.tet_id), c.voro_id});
+      CHECK(it != by.end());
+      const ConvexCellHost& d = *it->second;
+      CHECK(c.thread_id == d.thread_id && c.nb_v == d.nb_v &&
+            c.nb_p == d.nb_p && c.nb_e == d.nb_e && c.weight == d.weight &&
+            c.cell_vol == d.cell_vol);
+      CHECK(std::memcmp(c.ver_data_trans, d.ver_data_trans,
+                        c.nb_v * sizeof(c.ver_data_trans[0])) == 0);
+      CHECK(std::memcmp(c.clip_data_trans, d.clip_data_trans,
+                        c.nb_p * sizeof(c.clip_data_trans[0])) == 0);
+      CHECK(std::memcmp(c.clip_id2_data_trans, d.clip_id2_data_trans,
+                        c.nb_p * sizeof(c.clip_id2_data_trans[0])) == 0);
+      CHECK(std::memcmp(c.edge_data, d.edge_data,
+                        c.nb_e * sizeof(c.edge_data[0])) == 0);
+      ++same;
+    }
+    // per-site volume sums (in binary64 over cells in another order)
+    CHECK(vol_a.size() == vol_b.size());
+    for (size_t i = 0; i < vol_a.size(); ++i)
+      CHECK(std::fabs(vol_a[i] - vol_b[i]) <= 1e-6f * std::fabs(vol_a[i]));
+    std::printf("resident subset: %d cells of %zu tets identical to the "
+                "partial-mesh run\n", same, sub.size());
+  }
   // Euler characteristic of every (convex) cell is 1 with the reference's
   // adjacency weights summed over all cells of a tet ... per cell: V-E+F
   // with unit weights is 2 for a convex polytope

@@ -1,11 +1,11 @@
 // Host-code sanitizer driver (AddressSanitizer + UndefinedBehaviorSanitizer,
 // CPU only): exercises the library's host C++ -- input preparation
 // (rpd_prep.cpp: grids, spheres, tet adjacency, the power-diagram neighbour
-// builder) and the drop-in's cell methods and adjacency (rpd_compat.cpp) --
-// and the oracle (rpd_oracle.cpp, adj_oracle.cpp), compiled from source with
-// -fsanitize=address,undefined.  The device entry points that rpd_compat.cpp
-// calls are replaced below by definitions that report "no GPU"; nothing here
-// runs device code.
+// builder), the drop-in's run glue (rpd_compat.cpp) and its cell methods and
+// adjacency (rpd_cellhost.cpp) -- and the oracle (rpd_oracle.cpp,
+// adj_oracle.cpp), compiled from source with -fsanitize=address,undefined.
+// The device entry points that rpd_compat.cpp calls are replaced below by
+// definitions that report "no GPU"; nothing here runs device code.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -58,6 +58,11 @@ int rpd_download_cells(rpd_ctx*, rpd_cell*, uint32_t*, double*, double*) {
   return -3;
 }
 int rpd_download_relation(rpd_ctx*, int64_t*, int32_t*) { return -3; }
+int rpd_update_sites(rpd_ctx*, int32_t, const double*, const uint32_t*,
+                     const int64_t*, const int32_t*) {
+  return -3;
+}
+int rpd_set_tet_subset(rpd_ctx*, int32_t, const int32_t*) { return -3; }
 }
 
 // mirror of the oracle's CellOut

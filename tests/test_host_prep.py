@@ -44,6 +44,33 @@ def test_cpp_dropin_layout(tmp_path):
     assert "layout ok" in out
 
 
+def test_core_library_leaves_cell_methods_to_the_program(tmp_path):
+    """librpd_amd_core.so exports the RPD entry points but defines no
+    ConvexCellHost method, so a program with its own voronoi_defs.cxx
+    links it with one definition of each."""
+    lib = os.path.join(ROOT, "libmat_amd", "_lib")
+    core = os.path.join(lib, "librpd_amd_core.so")
+    syms = subprocess.run(["nm", "-DC", "--defined-only", core],
+                          capture_output=True, text=True, check=True).stdout
+    assert "ConvexCellHost::" not in syms
+    assert "is_convex_cell_valid" not in syms
+    assert "compute_clipped_voro_diagram_GPU(" in syms
+    assert "libmat_rpd::ResidentMesh::run(" in syms
+    full = subprocess.run(["nm", "-DC", "--defined-only",
+                           os.path.join(lib, "librpd_amd.so")],
+                          capture_output=True, text=True, check=True).stdout
+    assert "ConvexCellHost::reload_active()" in full
+    exe = tmp_path / "core_link"
+    subprocess.run(["g++", "-O1", "-std=c++17",
+                    "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests/cpp/core_link.cpp"), "-o",
+                    str(exe), "-L" + lib, "-lrpd_amd_core",
+                    "-Wl,-rpath," + lib], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True,
+                         check=True).stdout
+    assert "core link ok 1 1" in out
+
+
 def test_grid_mesh_is_conforming_and_positive():
     for n, jit in ((3, 0.0), (5, 0.1)):
         verts, tets = prep.grid_mesh(n, jit, 3)

@@ -1,6 +1,7 @@
 """AddressSanitizer + UndefinedBehaviorSanitizer build of the host C++ (CPU
-only): the library's input preparation and drop-in cell methods/adjacency
-(rpd_prep.cpp, rpd_compat.cpp) and the oracle (rpd_oracle.cpp,
+only): the library's input preparation, drop-in run glue and cell
+methods/adjacency (rpd_prep.cpp, rpd_compat.cpp, rpd_cellhost.cpp) and the
+oracle (rpd_oracle.cpp,
 adj_oracle.cpp), compiled from source with g++ and driven by
 tests/cpp/san_driver.cpp on a small clustered-sphere instance."""
 import os
@@ -8,7 +9,8 @@ import subprocess
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SOURCES = ["tests/cpp/san_driver.cpp", "libmat_amd/csrc/host/rpd_prep.cpp",
-           "libmat_amd/csrc/host/rpd_compat.cpp", "oracle/rpd_oracle.cpp",
+           "libmat_amd/csrc/host/rpd_compat.cpp",
+           "libmat_amd/csrc/host/rpd_cellhost.cpp", "oracle/rpd_oracle.cpp",
            "oracle/adj_oracle.cpp"]
 FLAGS = ["-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer",
          "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
