@@ -307,6 +307,65 @@ int rpd_download_relation(rpd_ctx* ctx, int64_t* offsets, int32_t* ids);
  * by the context and valid until the next call. */
 int rpd_site_centroids(rpd_ctx* ctx, double* out, const double** dev_out);
 
+/* Per-sphere power-cell adjacency of the last run's cells, computed on the
+ * device: what get_all_voro_info and update_pc_cc_info /
+ * update_pc_facet_cc_info / update_pc_edge_cc_info derive per sphere
+ * (rpd_update.cxx:80-340, 409-521) for the topology checks of
+ * fix_topo.cxx:87-270, plus the Euler sums of cal_cell_euler /
+ * cal_halfplane_facet_euler (voronoi_defs.cxx:190-267).  Two cells of one
+ * sphere are neighbours when both carry the tet face between their tets
+ * as an active plane (cell_neighbors); a facet is (site, neighbour site),
+ * the cells with an active bisector toward that neighbour
+ * (facet_neigh_to_cells); an edge is (site, neighbour, neighbour), the
+ * cells with an active edge between those two bisectors (e_to_cells).
+ * Components are taken over cell neighbours (within the facet's or edge's
+ * cells for those); a component is named by its smallest cell index.
+ * Cell indices are positions in the last run's cell stream; site ids are
+ * batch-wide.  Sizes go to *sizes (nullable); results stay on the device
+ * until rpd_adjacency_download. */
+typedef struct {
+  int64_t n_cells;       /* cells of the last run                      */
+  int64_t n_sites;       /* resident sites                             */
+  int64_t n_cell_pairs;  /* neighbouring cell pairs (shared tet faces) */
+  int64_t n_facets;      /* (site, neighbour) facets                   */
+  int64_t n_facet_cells; /* (facet, cell) records                      */
+  int64_t n_edges;       /* (site, neighbour pair) edges               */
+  int64_t n_edge_cells;  /* (edge, cell) records                       */
+} rpd_adj_sizes;
+int rpd_adjacency(rpd_ctx* ctx, rpd_adj_sizes* sizes);
+
+/* D2H of rpd_adjacency's results; NULL pointers are skipped.
+ *   cell_euler[n_cells]   cal_cell_euler of each cell (binary32)
+ *   cell_cc[n_cells]      component of each cell (its smallest cell)
+ *   cell_nbr[4*n_cells]   neighbour cell across local tet face k, or -1
+ *   site_cells, site_cc [n_sites]  number of cells, of components
+ *   site_euler[n_sites]   sum of cell_euler (binary64, ascending cells)
+ *                         minus the number of cells (is_to_fix_voro_euler)
+ *   facets[5*n_facets]    site, neighbour, first record, #cells,
+ *                         #components; sorted by (site, neighbour)
+ *   facet_euler[n_facets] binary64 sum of the records' facet Euler values
+ *                         in cell order (is_to_fix_facet_euler)
+ *   facet_cells[2*n_facet_cells]  per record: cell, its component
+ *   facet_cell_euler[n_facet_cells]  cal_halfplane_facet_euler (binary32)
+ *   edges[6*n_edges]      site, neighbour min, neighbour max, first
+ *                         record, #cells, #components; sorted
+ *   edge_cells[2*n_edge_cells]    per record: cell, its component */
+typedef struct {
+  float* cell_euler;
+  int32_t* cell_cc;
+  int32_t* cell_nbr;
+  int32_t* site_cells;
+  int32_t* site_cc;
+  double* site_euler;
+  int32_t* facets;
+  double* facet_euler;
+  int32_t* facet_cells;
+  float* facet_cell_euler;
+  int32_t* edges;
+  int32_t* edge_cells;
+} rpd_adj_out;
+int rpd_adjacency_download(rpd_ctx* ctx, const rpd_adj_out* out);
+
 /* Restrict the following rpd_run calls to n tets of the resident mesh (ids
  * in [0, n_tet)); other tets take no candidates.  The partial RPD of
  * calculate_partial (rpd_api.cxx:249-262) clips only the tets of the

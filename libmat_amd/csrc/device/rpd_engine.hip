@@ -228,12 +228,24 @@ struct rpd_ctx {
   TierBufs small, large, exact, mid;
   Buf out_cells, out_words, out_geom, out_areas;
   bool last_geometry = false;  // the last run computed cell geometry
+  bool has_results = false;    // the last run's cells belong to the
+                               // resident instance and sites
   int64_t n_knn = 0;
   // tet subset of the partial RPD (one byte per tet, 1 = clipped)
   Buf tet_sel;
   bool has_subset = false;
   // per-site centroids of the last run's cells
   Buf cs_keys_a, cs_keys_b, cs_vals_a, cs_vals_b, cs_start, cs_out;
+  // power-cell adjacency of the last run's cells
+  Buf adj_act, adj_euler, adj_cnt, adj_off, adj_nbr, adj_cpar, adj_err;
+  Buf tf_ka, tf_kb, tf_va, tf_vb;
+  Buf fac_ka, fac_kb, fac_va, fac_vb, fac_par, fac_eul, fac_flag, fac_pos,
+      fac_start, fac_runs, fac_reul, fac_rec;
+  Buf edg_ka, edg_kb, edg_va, edg_vb, edg_par, edg_flag, edg_pos, edg_start,
+      edg_runs, edg_rec;
+  Buf site_cells, site_cc, site_eul, adj_soff;
+  rpd_adj_sizes adj{};
+  bool adj_ready = false;
   // learned proposals: features, model (fp32 master, Adam state, bf16
   // copies), certificate and fallback
   Buf knn, feat, sel, omask, vowner, sg_keys_a, sg_keys_b, sg_start, sg_ids;
@@ -504,6 +516,8 @@ int rpd_update_sites_batch(rpd_ctx* ctx, int32_t n_inst,
   HIP_TRY(wait_stream(ctx, ctx->stream));
   ctx->has_sites = true;
   ctx->last_geometry = false;  // the last run's cells belong to other sites
+  ctx->has_results = false;
+  ctx->adj_ready = false;
   return 0;
 }
 
@@ -609,6 +623,8 @@ int rpd_upload_batch(rpd_ctx* ctx, int32_t n_inst,
   ctx->n_tet = (int32_t)nt_tot;
   ctx->has_sites = false;
   ctx->has_subset = false;
+  ctx->has_results = false;
+  ctx->adj_ready = false;
   HIP_TRY(ctx->vert.need((size_t)nv_tot * 32));
   HIP_TRY(ctx->tet_v.need((size_t)nt_tot * 16));
   HIP_TRY(ctx->tet_fid.need((size_t)nt_tot * 16));
@@ -1596,6 +1612,8 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     return fail(ctx, "rpd_run: a tet subset is for the plain path only "
                      "(learned = 0)");
   ctx->last_geometry = false;
+  ctx->has_results = false;
+  ctx->adj_ready = false;
   const int32_t site_k = ctx->st.site_k;
   std::memset(&ctx->st, 0, sizeof(ctx->st));
   ctx->st.site_k = site_k;
@@ -1719,6 +1737,321 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   for (int k = 0; k < ctx->n_small_ev; ++k)
     st.ms_clip_small_sum += el(16 + 2 * k, 17 + 2 * k);
   ctx->last_geometry = cfg.compute_geometry != 0;
+  ctx->has_results = true;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+// the last run's cells grouped by seed: cs_vals_b = cell indices sorted by
+// seed (stable: ascending within a seed), cs_start[s] = first position of
+// seed s (cs_start[n_site] = n_cells)
+int cells_by_seed(rpd_ctx* ctx) {
+  hipStream_t s = ctx->stream;
+  const int32_t ns = ctx->n_site;
+  const int64_t n = ctx->st.n_cells;
+  if (n >= (1LL << 31)) return fail(ctx, "too many cells");
+  HIP_TRY(ctx->cs_keys_a.need((size_t)std::max<int64_t>(n, 1) * 4));
+  HIP_TRY(ctx->cs_keys_b.need((size_t)std::max<int64_t>(n, 1) * 4));
+  HIP_TRY(ctx->cs_vals_a.need((size_t)std::max<int64_t>(n, 1) * 4));
+  HIP_TRY(ctx->cs_vals_b.need((size_t)std::max<int64_t>(n, 1) * 4));
+  HIP_TRY(ctx->cs_start.need(((size_t)ns + 1) * 8));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_cell_seed_keys, dim3(ceil_div(n, 256)), dim3(256), 0,
+                       s, n, ctx->out_cells.as<rpd_cell>(),
+                       ctx->cs_keys_a.as<uint32_t>(),
+                       ctx->cs_vals_a.as<int32_t>());
+    HIP_TRY(hipGetLastError());
+    int end_bit = 1;
+    while (end_bit < 32 && (1LL << end_bit) <= (int64_t)ns) ++end_bit;
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+        nullptr, tmp, ctx->cs_keys_a.as<uint32_t>(),
+        ctx->cs_keys_b.as<uint32_t>(), ctx->cs_vals_a.as<int32_t>(),
+        ctx->cs_vals_b.as<int32_t>(), (int)n, 0, end_bit, s));
+    HIP_TRY(ctx->scan_tmp.need(tmp));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+        ctx->scan_tmp.p, tmp, ctx->cs_keys_a.as<uint32_t>(),
+        ctx->cs_keys_b.as<uint32_t>(), ctx->cs_vals_a.as<int32_t>(),
+        ctx->cs_vals_b.as<int32_t>(), (int)n, 0, end_bit, s));
+  }
+  hipLaunchKernelGGL(k_seed_starts, dim3(ceil_div((int64_t)ns + 1, 256)),
+                     dim3(256), 0, s, ns, n, ctx->cs_keys_b.as<uint32_t>(),
+                     ctx->cs_start.as<int64_t>());
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// stable sort of (u64 key, i32 value) pairs over the low end_bit key bits;
+// results in ka/va (the b buffers are the sort's other half)
+int sort_pairs64(rpd_ctx* ctx, Buf& ka, Buf& kb, Buf& va, Buf& vb, int64_t n,
+                 int end_bit) {
+  if (n <= 1) return 0;
+  if (n >= (1LL << 31)) return fail(ctx, "too many records to sort");
+  hipStream_t s = ctx->stream;
+  size_t tmp = 0;
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+      nullptr, tmp, ka.as<uint64_t>(), kb.as<uint64_t>(), va.as<int32_t>(),
+      vb.as<int32_t>(), (int)n, 0, end_bit, s));
+  HIP_TRY(ctx->scan_tmp.need(tmp));
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+      ctx->scan_tmp.p, tmp, ka.as<uint64_t>(), kb.as<uint64_t>(),
+      va.as<int32_t>(), vb.as<int32_t>(), (int)n, 0, end_bit, s));
+  ka.swap(kb);
+  va.swap(vb);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rpd_adjacency(rpd_ctx* ctx, rpd_adj_sizes* sizes) {
+  if (!ctx) return -1;
+  if (!ctx->has_results)
+    return fail(ctx, "rpd_adjacency: no run on the resident instance");
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  ctx->adj_ready = false;
+  const int64_t n = ctx->st.n_cells;
+  const int32_t ns = ctx->n_site;
+  if (n >= (1LL << 29))
+    return fail(ctx, "rpd_adjacency: too many cells (4 per cell must fit "
+                     "32 bits)");
+  DevMesh M{ctx->n_vert, ctx->n_tet, ctx->vert.as<double4>(),
+            ctx->tet_v.as<int4>(), ctx->tet_fid.as<int4>(),
+            ctx->tet_meta.as<TetMeta>()};
+  DevSites S{ns,
+             ctx->site.as<double4>(),
+             ctx->flags.as<uint32_t>(),
+             ctx->knn_off.as<int64_t>(),
+             ctx->knn_ids.as<int32_t>(),
+             nullptr,
+             nullptr,
+             nullptr};
+  const size_t n1 = (size_t)std::max<int64_t>(n, 1);
+  HIP_TRY(ctx->adj_act.need(n1 * 32));
+  HIP_TRY(ctx->adj_euler.need(n1 * 4));
+  HIP_TRY(ctx->adj_cnt.need(3 * (n1 + 1) * 8));
+  HIP_TRY(ctx->adj_off.need(3 * (n1 + 1) * 8));
+  HIP_TRY(ctx->adj_nbr.need(n1 * 16));
+  HIP_TRY(ctx->adj_cpar.need(n1 * 4));
+  HIP_TRY(ctx->adj_err.need(16));
+  HIP_TRY(hipMemsetAsync(ctx->adj_err.p, 0, 16, s));
+  {  // first site of each instance
+    std::vector<int64_t> so(std::max<size_t>(ctx->inst.size(), 1), 0);
+    for (size_t b = 0; b < ctx->inst.size(); ++b) so[b] = ctx->inst[b].site_off;
+    HIP_TRY(ctx->adj_soff.need(so.size() * 8));
+    HIP_TRY(hipMemcpyAsync(ctx->adj_soff.p, so.data(), so.size() * 8,
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(wait_stream(ctx, s));  // so outlives the copy
+  }
+  int64_t* cnt = ctx->adj_cnt.as<int64_t>();
+  int64_t* off = ctx->adj_off.as<int64_t>();
+  const size_t stride = n1 + 1;  // three count / offset arrays of n+1
+  HIP_TRY(hipMemsetAsync(cnt, 0, 3 * stride * 8, s));
+  const rpd_cell* cells = ctx->out_cells.as<rpd_cell>();
+  const uint32_t* words = ctx->out_words.as<uint32_t>();
+  const int nblk = ceil_div(n, kAdjPerBlock);
+  if (n > 0)
+    hipLaunchKernelGGL(k_adj_cell, dim3(nblk), dim3(64 * kAdjPerBlock), 0, s,
+                       n, cells, words, M, ctx->adj_act.as<uint64_t>(),
+                       ctx->adj_euler.as<float>(), cnt, cnt + stride,
+                       cnt + 2 * stride);
+  HIP_TRY(hipGetLastError());
+  for (int q = 0; q < 3; ++q)
+    HIP_TRY(exclusive_sum<int64_t>(ctx, cnt + q * stride, off + q * stride,
+                                   n + 1));
+  int64_t tot[3];
+  HIP_TRY(read_back(ctx, s,
+                    {{&tot[0], off + n, 8},
+                     {&tot[1], off + stride + n, 8},
+                     {&tot[2], off + 2 * stride + n, 8}}));
+  const int64_t n_tf = tot[0], n_fac = tot[1], n_edg = tot[2];
+  if (n_fac >= (1LL << 31) || n_edg >= (1LL << 31) || n_tf >= (1LL << 31))
+    return fail(ctx, "rpd_adjacency: too many records");
+  auto need = [](Buf& b, int64_t m, size_t w) {
+    return b.need((size_t)std::max<int64_t>(m, 1) * w);
+  };
+  HIP_TRY(need(ctx->tf_ka, n_tf, 8));
+  HIP_TRY(need(ctx->tf_kb, n_tf, 8));
+  HIP_TRY(need(ctx->tf_va, n_tf, 4));
+  HIP_TRY(need(ctx->tf_vb, n_tf, 4));
+  HIP_TRY(need(ctx->fac_ka, n_fac, 8));
+  HIP_TRY(need(ctx->fac_kb, n_fac, 8));
+  HIP_TRY(need(ctx->fac_va, n_fac, 4));
+  HIP_TRY(need(ctx->fac_vb, n_fac, 4));
+  HIP_TRY(need(ctx->edg_ka, n_edg, 8));
+  HIP_TRY(need(ctx->edg_kb, n_edg, 8));
+  HIP_TRY(need(ctx->edg_va, n_edg, 4));
+  HIP_TRY(need(ctx->edg_vb, n_edg, 4));
+  if (n > 0)
+    hipLaunchKernelGGL(k_adj_fill, dim3(nblk), dim3(64 * kAdjPerBlock), 0, s,
+                       n, cells, words, M, S, ctx->adj_act.as<uint64_t>(), off,
+                       off + stride, off + 2 * stride,
+                       ctx->tf_ka.as<uint64_t>(), ctx->tf_va.as<int32_t>(),
+                       ctx->fac_ka.as<uint64_t>(), ctx->fac_va.as<int32_t>(),
+                       ctx->edg_ka.as<uint64_t>(), ctx->edg_va.as<int32_t>(),
+                       ctx->adj_err.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  // key widths: sites need sb bits
+  int sb = 1;
+  while (sb < 40 && (1LL << sb) <= (int64_t)std::max(ns, 1)) ++sb;
+  if (int rc = sort_pairs64(ctx, ctx->tf_ka, ctx->tf_kb, ctx->tf_va,
+                            ctx->tf_vb, n_tf, std::min(64, 32 + sb)))
+    return rc;
+  if (int rc = sort_pairs64(ctx, ctx->fac_ka, ctx->fac_kb, ctx->fac_va,
+                            ctx->fac_vb, n_fac, std::min(64, 32 + sb)))
+    return rc;
+  if (int rc = sort_pairs64(ctx, ctx->edg_ka, ctx->edg_kb, ctx->edg_va,
+                            ctx->edg_vb, n_edg,
+                            std::min(64, 2 * kEdgePosBits + sb)))
+    return rc;
+  // cell neighbours across tet faces, and cell components
+  int32_t* nbr = ctx->adj_nbr.as<int32_t>();
+  HIP_TRY(hipMemsetAsync(nbr, 0xff, n1 * 16, s));
+  int32_t* err = ctx->adj_err.as<int32_t>();
+  if (n_tf > 1)
+    hipLaunchKernelGGL(k_adj_pairs, dim3(ceil_div(n_tf, 256)), dim3(256), 0,
+                       s, n_tf, ctx->tf_ka.as<uint64_t>(),
+                       ctx->tf_va.as<int32_t>(), nbr, err + 2, err);
+  HIP_TRY(hipGetLastError());
+  int32_t* cpar = ctx->adj_cpar.as<int32_t>();
+  if (n > 0) {
+    hipLaunchKernelGGL(k_uf_init, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
+                       cpar);
+    hipLaunchKernelGGL(k_uf_cells, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
+                       nbr, cpar);
+    hipLaunchKernelGGL(k_uf_flatten, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                       n, cpar);
+  }
+  HIP_TRY(hipGetLastError());
+  // facet and edge records: components, runs
+  struct Kind {
+    int64_t m;
+    Buf *k, *v, *par, *flag, *pos, *start, *runs, *rec;
+    int width;
+  } kinds[2] = {{n_fac, &ctx->fac_ka, &ctx->fac_va, &ctx->fac_par,
+                 &ctx->fac_flag, &ctx->fac_pos, &ctx->fac_start,
+                 &ctx->fac_runs, &ctx->fac_rec, 5},
+                {n_edg, &ctx->edg_ka, &ctx->edg_va, &ctx->edg_par,
+                 &ctx->edg_flag, &ctx->edg_pos, &ctx->edg_start,
+                 &ctx->edg_runs, &ctx->edg_rec, 6}};
+  int64_t n_runs[2] = {0, 0};
+  for (int kd = 0; kd < 2; ++kd) {
+    Kind& K = kinds[kd];
+    const int64_t m = K.m;
+    HIP_TRY(need(*K.par, m, 4));
+    HIP_TRY(need(*K.flag, m + 1, 8));
+    HIP_TRY(need(*K.pos, m + 1, 8));
+    HIP_TRY(need(*K.rec, m, 8));
+    if (m > 0) {
+      hipLaunchKernelGGL(k_uf_init, dim3(ceil_div(m, 256)), dim3(256), 0, s,
+                         m, K.par->as<int32_t>());
+      hipLaunchKernelGGL(k_uf_records, dim3(ceil_div(m, 256)), dim3(256), 0,
+                         s, m, K.k->as<uint64_t>(), K.v->as<int32_t>(), nbr,
+                         K.par->as<int32_t>());
+      hipLaunchKernelGGL(k_uf_flatten, dim3(ceil_div(m, 256)), dim3(256), 0,
+                         s, m, K.par->as<int32_t>());
+      hipLaunchKernelGGL(k_run_flags, dim3(ceil_div(m, 256)), dim3(256), 0, s,
+                         m, K.k->as<uint64_t>(), K.flag->as<int64_t>());
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemsetAsync(K.flag->as<int64_t>() + m, 0, 8, s));
+    HIP_TRY(exclusive_sum<int64_t>(ctx, K.flag->as<int64_t>(),
+                                   K.pos->as<int64_t>(), m + 1));
+    HIP_TRY(read_back(ctx, s, {{&n_runs[kd], K.pos->as<int64_t>() + m, 8}}));
+    HIP_TRY(need(*K.start, n_runs[kd], 8));
+    HIP_TRY(need(*K.runs, n_runs[kd], 4 * K.width));
+    if (kd == 0) {
+      HIP_TRY(need(ctx->fac_reul, n_runs[kd], 8));
+      HIP_TRY(need(ctx->fac_eul, m, 4));
+    }
+    if (m > 0) {
+      hipLaunchKernelGGL(k_run_starts, dim3(ceil_div(m, 256)), dim3(256), 0,
+                         s, m, K.flag->as<int64_t>(), K.pos->as<int64_t>(),
+                         K.start->as<int64_t>());
+      if (kd == 0)
+        hipLaunchKernelGGL(k_fac_euler, dim3(ceil_div(m, 256)), dim3(256), 0,
+                           s, m, K.k->as<uint64_t>(), K.v->as<int32_t>(),
+                           cells, words, M, ctx->adj_act.as<uint64_t>(),
+                           ctx->site_inst.as<int32_t>(),
+                           ctx->adj_soff.as<int64_t>(),
+                           ctx->fac_eul.as<float>());
+      hipLaunchKernelGGL(
+          k_adj_runs, dim3(ceil_div(n_runs[kd], 256)), dim3(256), 0, s,
+          n_runs[kd], m, kd, K.start->as<int64_t>(), K.k->as<uint64_t>(),
+          K.v->as<int32_t>(), K.par->as<int32_t>(),
+          kd == 0 ? ctx->fac_eul.as<float>() : nullptr, S,
+          K.runs->as<int32_t>(),
+          kd == 0 ? ctx->fac_reul.as<double>() : nullptr,
+          K.rec->as<int32_t>());
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  // per site: cells, components, Euler number
+  if (int rc = cells_by_seed(ctx)) return rc;
+  const size_t ns1 = (size_t)std::max(ns, 1);
+  HIP_TRY(ctx->site_cells.need(ns1 * 4));
+  HIP_TRY(ctx->site_cc.need(ns1 * 4));
+  HIP_TRY(ctx->site_eul.need(ns1 * 8));
+  if (ns > 0)
+    hipLaunchKernelGGL(k_adj_sites, dim3(ceil_div(ns, 256)), dim3(256), 0, s,
+                       ns, ctx->cs_start.as<int64_t>(),
+                       ctx->cs_vals_b.as<int32_t>(), cpar,
+                       ctx->adj_euler.as<float>(),
+                       ctx->site_cells.as<int32_t>(),
+                       ctx->site_cc.as<int32_t>(),
+                       ctx->site_eul.as<double>());
+  HIP_TRY(hipGetLastError());
+  int32_t herr[4];
+  HIP_TRY(read_back(ctx, s, {{herr, err, 16}}));
+  if (herr[0])
+    return fail(ctx, "rpd_adjacency: a bisector's site is missing from the "
+                     "seed's neighbour list, or sits beyond position " +
+                         std::to_string((1 << kEdgePosBits) - 1));
+  if (herr[1])
+    return fail(ctx, "rpd_adjacency: a tet face is shared by more than two "
+                     "cells of one sphere");
+  rpd_adj_sizes& z = ctx->adj;
+  z.n_cells = n;
+  z.n_sites = ns;
+  z.n_cell_pairs = herr[2];
+  z.n_facets = n_runs[0];
+  z.n_facet_cells = n_fac;
+  z.n_edges = n_runs[1];
+  z.n_edge_cells = n_edg;
+  ctx->adj_ready = true;
+  if (sizes) *sizes = z;
+  return 0;
+}
+
+int rpd_adjacency_download(rpd_ctx* ctx, const rpd_adj_out* o) {
+  if (!ctx || !o) return -1;
+  if (!ctx->adj_ready)
+    return fail(ctx, "rpd_adjacency_download: call rpd_adjacency first");
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const rpd_adj_sizes& z = ctx->adj;
+  auto get = [&](void* dst, const Buf& src, size_t bytes) -> hipError_t {
+    if (!dst || bytes == 0) return hipSuccess;
+    return hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, s);
+  };
+  HIP_TRY(get(o->cell_euler, ctx->adj_euler, (size_t)z.n_cells * 4));
+  HIP_TRY(get(o->cell_cc, ctx->adj_cpar, (size_t)z.n_cells * 4));
+  HIP_TRY(get(o->cell_nbr, ctx->adj_nbr, (size_t)z.n_cells * 16));
+  HIP_TRY(get(o->site_cells, ctx->site_cells, (size_t)z.n_sites * 4));
+  HIP_TRY(get(o->site_cc, ctx->site_cc, (size_t)z.n_sites * 4));
+  HIP_TRY(get(o->site_euler, ctx->site_eul, (size_t)z.n_sites * 8));
+  HIP_TRY(get(o->facets, ctx->fac_runs, (size_t)z.n_facets * 20));
+  HIP_TRY(get(o->facet_euler, ctx->fac_reul, (size_t)z.n_facets * 8));
+  HIP_TRY(get(o->facet_cells, ctx->fac_rec, (size_t)z.n_facet_cells * 8));
+  HIP_TRY(get(o->facet_cell_euler, ctx->fac_eul,
+              (size_t)z.n_facet_cells * 4));
+  HIP_TRY(get(o->edges, ctx->edg_runs, (size_t)z.n_edges * 24));
+  HIP_TRY(get(o->edge_cells, ctx->edg_rec, (size_t)z.n_edge_cells * 8));
+  HIP_TRY(wait_stream(ctx, s));
   return 0;
 }
 
@@ -1756,38 +2089,8 @@ int rpd_site_centroids(rpd_ctx* ctx, double* out, const double** dev_out) {
   HIP_TRY(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   const int32_t ns = ctx->n_site;
-  const int64_t n = ctx->st.n_cells;
-  if (n >= (1LL << 31))
-    return fail(ctx, "rpd_site_centroids: too many cells");
-  HIP_TRY(ctx->cs_keys_a.need((size_t)std::max<int64_t>(n, 1) * 4));
-  HIP_TRY(ctx->cs_keys_b.need((size_t)std::max<int64_t>(n, 1) * 4));
-  HIP_TRY(ctx->cs_vals_a.need((size_t)std::max<int64_t>(n, 1) * 4));
-  HIP_TRY(ctx->cs_vals_b.need((size_t)std::max<int64_t>(n, 1) * 4));
-  HIP_TRY(ctx->cs_start.need(((size_t)ns + 1) * 8));
+  if (int rc = cells_by_seed(ctx)) return rc;
   HIP_TRY(ctx->cs_out.need((size_t)std::max(ns, 1) * 32));
-  if (n > 0) {
-    hipLaunchKernelGGL(k_cell_seed_keys, dim3(ceil_div(n, 256)), dim3(256), 0,
-                       s, n, ctx->out_cells.as<rpd_cell>(),
-                       ctx->cs_keys_a.as<uint32_t>(),
-                       ctx->cs_vals_a.as<int32_t>());
-    HIP_TRY(hipGetLastError());
-    int end_bit = 1;
-    while (end_bit < 32 && (1LL << end_bit) <= (int64_t)ns) ++end_bit;
-    size_t tmp = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
-        nullptr, tmp, ctx->cs_keys_a.as<uint32_t>(),
-        ctx->cs_keys_b.as<uint32_t>(), ctx->cs_vals_a.as<int32_t>(),
-        ctx->cs_vals_b.as<int32_t>(), (int)n, 0, end_bit, s));
-    HIP_TRY(ctx->scan_tmp.need(tmp));
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
-        ctx->scan_tmp.p, tmp, ctx->cs_keys_a.as<uint32_t>(),
-        ctx->cs_keys_b.as<uint32_t>(), ctx->cs_vals_a.as<int32_t>(),
-        ctx->cs_vals_b.as<int32_t>(), (int)n, 0, end_bit, s));
-  }
-  hipLaunchKernelGGL(k_seed_starts, dim3(ceil_div((int64_t)ns + 1, 256)),
-                     dim3(256), 0, s, ns, n, ctx->cs_keys_b.as<uint32_t>(),
-                     ctx->cs_start.as<int64_t>());
-  HIP_TRY(hipGetLastError());
   if (ns > 0)
     hipLaunchKernelGGL(k_site_centroid, dim3(ceil_div(ns, 4)), dim3(256), 0, s,
                        ns, ctx->cs_start.as<int64_t>(),
@@ -2016,11 +2319,13 @@ int rpd_run_noalloc(rpd_ctx* ctx, const rpd_instance_desc* desc,
     std::vector<rpd_ctx::Inst> inst;
     int32_t n_vert, n_tet, n_site;
     int64_t n_knn;
-    bool has_mesh, has_sites, has_subset, last_geometry;
+    bool has_mesh, has_sites, has_subset, last_geometry, has_results,
+        adj_ready;
     rpd_stats st;
-  } sv{ctx->inst,     ctx->n_vert,   ctx->n_tet,      ctx->n_site,
-       ctx->n_knn,    ctx->has_mesh, ctx->has_sites,  ctx->has_subset,
-       ctx->last_geometry, ctx->st};
+  } sv{ctx->inst,          ctx->n_vert,      ctx->n_tet,
+       ctx->n_site,        ctx->n_knn,       ctx->has_mesh,
+       ctx->has_sites,     ctx->has_subset,  ctx->last_geometry,
+       ctx->has_results,   ctx->adj_ready,   ctx->st};
   Arena A;
   A.base = static_cast<char*>(ctx->scratch_dev);
   A.cap = kScratchBytes;
@@ -2043,6 +2348,8 @@ int rpd_run_noalloc(rpd_ctx* ctx, const rpd_instance_desc* desc,
       c->has_sites = s.has_sites;
       c->has_subset = s.has_subset;
       c->last_geometry = s.last_geometry;
+      c->has_results = s.has_results;
+      c->adj_ready = s.adj_ready;
       c->st = s.st;
     }
   } guard{ctx, A, sv};

@@ -160,6 +160,23 @@ def _declare_capi(L):
     L.rpd_site_centroids.restype = ctypes.c_int
     L.rpd_set_tet_subset.argtypes = [vp, i32, vp]
     L.rpd_set_tet_subset.restype = ctypes.c_int
+    L.rpd_adjacency.argtypes = [vp, ctypes.POINTER(AdjSizes)]
+    L.rpd_adjacency.restype = ctypes.c_int
+    L.rpd_adjacency_download.argtypes = [vp, ctypes.POINTER(AdjOut)]
+    L.rpd_adjacency_download.restype = ctypes.c_int
+
+
+class AdjSizes(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in
+                ("n_cells", "n_sites", "n_cell_pairs", "n_facets",
+                 "n_facet_cells", "n_edges", "n_edge_cells")]
+
+
+class AdjOut(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("cell_euler", "cell_cc", "cell_nbr", "site_cells", "site_cc",
+                 "site_euler", "facets", "facet_euler", "facet_cells",
+                 "facet_cell_euler", "edges", "edge_cells")]
 
 
 CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
@@ -172,7 +189,8 @@ CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
                 "rpd_sync", "rpd_get_stats", "rpd_download_cells",
                 "rpd_download_relation", "rpd_calib_copy",
                 "rpd_power_neighbors_gpu", "rpd_run_noalloc",
-                "rpd_site_centroids", "rpd_set_tet_subset")
+                "rpd_site_centroids", "rpd_set_tet_subset", "rpd_adjacency",
+                "rpd_adjacency_download")
 
 
 def _p(a):
@@ -461,6 +479,36 @@ class Context:
     def _site_count(self):
         to, so = self.batch_offsets()
         return so[-1]
+
+    def adjacency(self):
+        """Power-cell adjacency of the last run's cells, computed on the
+        device (rpd_adjacency); returns a dict of numpy arrays named as the
+        fields of rpd_adj_out plus the sizes."""
+        z = AdjSizes()
+        self._check(self._L.rpd_adjacency(self._h, ctypes.byref(z)),
+                    "rpd_adjacency")
+        nc, ns, nf, nfc, ne, nec = (z.n_cells, z.n_sites, z.n_facets,
+                                    z.n_facet_cells, z.n_edges,
+                                    z.n_edge_cells)
+        a = dict(
+            cell_euler=np.empty(nc, np.float32),
+            cell_cc=np.empty(nc, np.int32),
+            cell_nbr=np.empty((nc, 4), np.int32),
+            site_cells=np.empty(ns, np.int32),
+            site_cc=np.empty(ns, np.int32),
+            site_euler=np.empty(ns, np.float64),
+            facets=np.empty((nf, 5), np.int32),
+            facet_euler=np.empty(nf, np.float64),
+            facet_cells=np.empty((nfc, 2), np.int32),
+            facet_cell_euler=np.empty(nfc, np.float32),
+            edges=np.empty((ne, 6), np.int32),
+            edge_cells=np.empty((nec, 2), np.int32))
+        o = AdjOut(*[a[f].ctypes.data if a[f].size else None
+                     for f, _ in AdjOut._fields_])
+        self._check(self._L.rpd_adjacency_download(self._h, ctypes.byref(o)),
+                    "rpd_adjacency_download")
+        a["sizes"] = {f: getattr(z, f) for f, _ in AdjSizes._fields_}
+        return a
 
     def set_tet_subset(self, tet_ids=None):
         """Clip only these tets of the resident mesh in the following
