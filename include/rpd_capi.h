@@ -307,6 +307,26 @@ int rpd_download_relation(rpd_ctx* ctx, int64_t* offsets, int32_t* ids);
  * by the context and valid until the next call. */
 int rpd_site_centroids(rpd_ctx* ctx, double* out, const double** dev_out);
 
+/* The plain pipeline of the resident batch captured once into a hipGraph
+ * and replayed: the batched, repeated RPD of sphere optimisation as one
+ * graph launch per iteration (no reference counterpart: the reference
+ * computes one instance per call and sizes every phase on the host).
+ * rpd_graph_build runs cfg eagerly (its results are the current ones),
+ * sizes every launch and buffer at the counts it saw times
+ * (1 + headroom), and captures the stages with every count kept on the
+ * device: the kernels read the counts the previous kernels left and stop
+ * at the capacity.  rpd_graph_run replays the graph on the current data
+ * (rpd_update_sites_batch may replace spheres and neighbour lists between
+ * runs) and reads the counts back once; if one went past its capacity, or
+ * the batch's per-instance tet or site counts, the tet subset switch or a
+ * device buffer changed, it captures again through rpd_graph_build (whose
+ * eager run gives the results) and sets *replayed = 0, else 1.  Results
+ * are read as after rpd_run.  Plain path only: cfg->learned = 0,
+ * count_work = 0, small_tier = 1. */
+int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom);
+int rpd_graph_run(rpd_ctx* ctx, int32_t* replayed);
+int rpd_graph_destroy(rpd_ctx* ctx);
+
 /* Per-sphere power-cell adjacency of the last run's cells, computed on the
  * device: what get_all_voro_info and update_pc_cc_info /
  * update_pc_facet_cc_info / update_pc_edge_cc_info derive per sphere

@@ -9,6 +9,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -70,6 +71,11 @@ struct Arena {
   void restore();
 };
 thread_local Arena* t_arena = nullptr;
+// set while a stream capture records the pipeline: nothing may allocate,
+// free or wait on the host then
+thread_local bool t_capture = false;
+// bumped whenever a buffer moves (a captured graph holds addresses)
+std::atomic<uint64_t> g_buf_generation{0};
 
 struct Buf {
   void* p = nullptr;
@@ -113,6 +119,8 @@ struct Buf {
   hipError_t need(size_t bytes) {
     if (t_arena) return arena_need(bytes, 0, nullptr);
     if (bytes <= cap && p) return hipSuccess;
+    if (t_capture) return hipErrorStreamCaptureUnsupported;
+    g_buf_generation.fetch_add(1);
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
@@ -126,6 +134,8 @@ struct Buf {
   hipError_t need_keep(size_t bytes, size_t keep, hipStream_t s) {
     if (t_arena) return arena_need(bytes, keep, s);
     if (bytes <= cap && p) return hipSuccess;
+    if (t_capture) return hipErrorStreamCaptureUnsupported;
+    g_buf_generation.fetch_add(1);
     size_t b = std::max<size_t>(bytes + bytes / 4, 256);
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, b);
@@ -196,6 +206,13 @@ struct TierBufs {
   }
 };
 
+// sizes of one plain run: sphere-bin registrations, candidates, middle /
+// full / exact re-run entries, output stream
+struct RunCaps {
+  int64_t n_reg = 0, n_cand = 0, n_mid = 0, n_full = 0, n_exact = 0;
+  int64_t cells = 0, words = 0, areas = 0;
+};
+
 }  // namespace
 
 struct rpd_ctx {
@@ -246,6 +263,15 @@ struct rpd_ctx {
   Buf site_cells, site_cc, site_eul, adj_soff;
   rpd_adj_sizes adj{};
   bool adj_ready = false;
+  RunCaps seen{};  // sizes of the last eager plain run
+  // the captured plain pipeline of rpd_graph_build, its capacities and the
+  // state it was captured against
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  RunCaps graph_cap{};
+  rpd_run_cfg graph_cfg{};
+  double graph_headroom = 0;
+  std::vector<int64_t> graph_sig;
   // learned proposals: features, model (fp32 master, Adam state, bf16
   // copies), certificate and fallback
   Buf knn, feat, sel, omask, vowner, sg_keys_a, sg_keys_b, sg_start, sg_ids;
@@ -331,6 +357,7 @@ struct D2H {
 };
 hipError_t read_back(rpd_ctx* ctx, hipStream_t s,
                      std::initializer_list<D2H> reads) {
+  if (t_capture) return hipErrorStreamCaptureUnsupported;
   size_t off = 0;
   for (const D2H& r : reads) {
     if (off + r.n > kPinBytes) return hipErrorInvalidValue;
@@ -358,6 +385,15 @@ hipError_t exclusive_sum(rpd_ctx* ctx, const T* in, T* out, int64_t n) {
   if ((e = ctx->scan_tmp.need(tmp)) != hipSuccess) return e;
   return hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.p, tmp, in, out,
                                           (int)n, ctx->stream);
+}
+
+// zero n int32 words with a kernel (a memset node in a captured graph is
+// avoided)
+hipError_t zero32(hipStream_t s, void* p, int64_t n) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill32, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                     static_cast<int32_t*>(p), n, 0);
+  return hipGetLastError();
 }
 
 }  // namespace
@@ -423,6 +459,8 @@ void rpd_ctx_destroy(rpd_ctx* ctx) {
   if (ctx->pin) (void)hipHostFree(ctx->pin);
   if (ctx->pin_inst) (void)hipHostFree(ctx->pin_inst);
   if (ctx->ev_inst) (void)hipEventDestroy(ctx->ev_inst);
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -776,6 +814,13 @@ struct RunState {
   int32_t r_full = 0;  // entries of the full-capacity list after a middle
                        // tier (tight small tiers only)
   CertArgs cert{};  // learned path: certificate evaluated in the clip
+  // Captured pipeline (rpd_graph_build): no host reads inside the run;
+  // launches and buffers are sized by cap, the counts stay on the device
+  // and the host checks them against cap after the graph has run.
+  bool graph = false;
+  RunCaps cap{};
+  // counts of an eager run (the capacities of a later capture)
+  RunCaps seen{};
 };
 
 MlpDev mlp_dev(rpd_ctx* ctx) {
@@ -855,7 +900,7 @@ int upload_inst_info(rpd_ctx* ctx, RunState& R) {
   HIP_TRY(ctx->inst_dev.need(bytes));
   // the previous copy out of the pinned block must have completed before
   // the block is rewritten (normally long done: runs end with a wait)
-  if (ctx->inst_copy_recorded) {
+  if (ctx->inst_copy_recorded && !t_capture) {
     hipError_t e;
     while ((e = hipEventQuery(ctx->ev_inst)) == hipErrorNotReady) {
     }
@@ -874,8 +919,10 @@ int upload_inst_info(rpd_ctx* ctx, RunState& R) {
   std::memcpy(ctx->pin_inst, ii.data(), bytes);
   HIP_TRY(hipMemcpyAsync(ctx->inst_dev.p, ctx->pin_inst, bytes,
                          hipMemcpyHostToDevice, R.s));
-  HIP_TRY(hipEventRecord(ctx->ev_inst, R.s));
-  ctx->inst_copy_recorded = true;
+  if (!R.graph) {
+    HIP_TRY(hipEventRecord(ctx->ev_inst, R.s));
+    ctx->inst_copy_recorded = true;
+  }
   R.n_bins = n_bins;
   R.n_sg = n_sg;
   R.inst_d = ctx->inst_dev.as<InstInfo>();
@@ -910,7 +957,7 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
                        ctx->knn_near.as<int32_t>(),
                        ctx->knn_site_near.as<double4>(), R.cnt + 0);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[10], s));
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[10], s));
   // sphere -> bin registrations: count, expand, radix sort by
   // (bin, sphere), bin starts by binary search
   HIP_TRY(ctx->reg_n.need(((size_t)ns + 1) * 8));
@@ -922,35 +969,44 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
   HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->reg_n.as<int64_t>(),
                                  ctx->reg_off.as<int64_t>(), ns + 1));
   int64_t n_reg = 0;
-  HIP_TRY(read_back(ctx, s, {{&n_reg, ctx->reg_off.as<int64_t>() + ns, 8}}));
+  if (R.graph)
+    n_reg = R.cap.n_reg;
+  else
+    HIP_TRY(read_back(ctx, s, {{&n_reg, ctx->reg_off.as<int64_t>() + ns, 8}}));
   if (n_reg >= (1LL << 31))
     return fail(ctx, "rpd_run: too many sphere-bin registrations");
   ctx->st.n_bin_registrations = n_reg;
+  R.seen.n_reg = n_reg;
   HIP_TRY(ctx->keys_a.need((size_t)std::max<int64_t>(n_reg, 1) * 8));
   HIP_TRY(ctx->keys_b.need((size_t)std::max<int64_t>(n_reg, 1) * 8));
   HIP_TRY(ctx->bin_start.need((n_bins + 1) * 4));
   HIP_TRY(ctx->bin_ids.need((size_t)std::max<int64_t>(n_reg, 1) * 4));
   if (n_reg > 0) {
+    int end_bit = 32;
+    while ((1LL << (end_bit - 32)) < n_bins + 1) ++end_bit;
+    // (a captured run pads its registrations past the device count with a
+    // key above every bin)
     hipLaunchKernelGGL(k_bin_expand, dim3(ceil_div(n_reg, 256)), dim3(256), 0,
                        s, ns, n_reg, ctx->bbox.as<double>(), R.inst_d,
                        R.site_inst_d, ctx->reg_off.as<int64_t>(),
-                       ctx->keys_a.as<uint64_t>());
+                       ctx->keys_a.as<uint64_t>(),
+                       R.graph ? ctx->reg_off.as<int64_t>() + ns : nullptr,
+                       end_bit >= 64 ? ~0ull : (1ull << end_bit) - 1);
     HIP_TRY(hipGetLastError());
-    int end_bit = 32;
-    while ((1LL << (end_bit - 32)) < n_bins + 1) ++end_bit;
     if (int rc = sort_keys(ctx, s, ctx->keys_a, ctx->keys_b, n_reg, end_bit))
       return rc;
   }
   hipLaunchKernelGGL(k_bin_starts,
                      dim3(ceil_div(std::max<int64_t>(n_bins + 1, n_reg), 256)),
                      dim3(256), 0, s, n_bins, n_reg, ctx->keys_b.as<uint64_t>(),
-                     ctx->bin_start.as<int32_t>(), ctx->bin_ids.as<int32_t>());
+                     ctx->bin_start.as<int32_t>(), ctx->bin_ids.as<int32_t>(),
+                     R.graph ? ctx->reg_off.as<int64_t>() + ns : nullptr);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[11], s));
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[11], s));
   HIP_TRY(ctx->tet_count.need(((size_t)nt + 1) * 8));
   HIP_TRY(ctx->tet_off.need(((size_t)nt + 1) * 8));
   HIP_TRY(ctx->tet_pass.need(((size_t)nt + 1) * 2 * 8));  // 2 words per tet
-  HIP_TRY(hipMemsetAsync(ctx->tet_count.as<int64_t>() + nt, 0, 8, s));
+  HIP_TRY(zero32(s, ctx->tet_count.as<int64_t>() + nt, 2));
   hipLaunchKernelGGL(k_tet_count, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
                      R.M, R.S, R.inst_d, R.tet_inst_d,
                      ctx->bin_start.as<int32_t>(), ctx->bin_ids.as<int32_t>(),
@@ -960,11 +1016,16 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
                          : nullptr,
                      ctx->has_subset ? ctx->tet_sel.as<uint8_t>() : nullptr);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[12], s));
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[12], s));
   HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
                                  ctx->tet_off.as<int64_t>(), nt + 1));
   int64_t n_cand = 0;
-  HIP_TRY(read_back(ctx, s, {{&n_cand, ctx->tet_off.as<int64_t>() + nt, 8}}));
+  if (R.graph)
+    n_cand = R.cap.n_cand;
+  else
+    HIP_TRY(
+        read_back(ctx, s, {{&n_cand, ctx->tet_off.as<int64_t>() + nt, 8}}));
+  R.seen.n_cand = n_cand;
   HIP_TRY(ctx->cand.need((size_t)std::max<int64_t>(n_cand, 1) * sizeof(Cand)));
   HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
   hipLaunchKernelGGL(k_tet_fill, dim3(ceil_div(nt, 256)), dim3(256), 0, s, R.M,
@@ -972,9 +1033,10 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
                      ctx->bin_ids.as<int32_t>(), ctx->tet_off.as<int64_t>(),
                      ctx->tet_pass.as<uint64_t>(), ctx->cand.as<Cand>(),
                      ctx->rel_ids.as<int32_t>(),
-                     ctx->has_subset ? ctx->tet_sel.as<uint8_t>() : nullptr);
+                     ctx->has_subset ? ctx->tet_sel.as<uint8_t>() : nullptr,
+                     n_cand, R.cnt + 7);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[13], s));
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[13], s));
   *n_cand_out = n_cand;
   return 0;
 }
@@ -985,7 +1047,7 @@ int site_inline(rpd_ctx* ctx, RunState& R) {
     hipLaunchKernelGGL(k_site_inline, dim3(ceil_div(ctx->n_knn, 256)),
                        dim3(256), 0, R.s, R.S, ctx->n_knn,
                        ctx->knn_near.as<int32_t>(),
-                       ctx->knn_site_near.as<double4>());
+                       ctx->knn_site_near.as<double4>(), nullptr);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1014,7 +1076,7 @@ int learned_features(rpd_ctx* ctx, RunState& R) {
                      dim3(ceil_div(std::max<int64_t>(R.n_sg + 1, ns), 256)),
                      dim3(256), 0, s, R.n_sg, (int64_t)ns,
                      ctx->sg_keys_b.as<uint64_t>(), ctx->sg_start.as<int32_t>(),
-                     ctx->sg_ids.as<int32_t>());
+                     ctx->sg_ids.as<int32_t>(), nullptr);
   HIP_TRY(hipGetLastError());
   HIP_TRY(ctx->knn.need((size_t)std::max(nt, 1) * kOut * 4));
   HIP_TRY(ctx->feat.need((size_t)std::max(nt, 1) * kIn * 2));
@@ -1106,6 +1168,12 @@ ClipArgs clip_args(rpd_ctx* ctx, RunState& R) {
   a.n_rerun_exact = R.cnt + 2;
   a.work = R.cfg.count_work ? ctx->work.as<unsigned long long>() : nullptr;
   a.cert = R.cert;
+  a.n_items_d64 = nullptr;
+  a.n_items_d32 = nullptr;
+  a.n_sub = 0;
+  a.cap_large = 0x7fffffff;
+  a.cap_exact = 0x7fffffff;
+  a.overflow = R.cnt + 7;
   return a;
 }
 
@@ -1134,11 +1202,14 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     HIP_TRY(ctx->trip_v.need((size_t)n * 4));
     HIP_TRY(ctx->trip_v2.need((size_t)n * 4));
   }
+  // (a captured run: the candidate count is tet_off[n_tet] on the device)
+  const int64_t* n_dev = R.graph ? ctx->tet_off.as<int64_t>() + R.nt : nullptr;
   if (R.cfg.use_active_mask && n > 0)
     hipLaunchKernelGGL(k_cand_mask, dim3(ceil_div(n, 256)), dim3(256), 0, s,
                        R.M, R.S, n, ctx->cand.as<Cand>() + c0,
                        by_steps ? ctx->trip_k.as<uint8_t>() : nullptr,
-                       by_steps ? ctx->trip_v.as<int32_t>() : nullptr, c0);
+                       by_steps ? ctx->trip_v.as<int32_t>() : nullptr, c0,
+                       n_dev);
   HIP_TRY(hipGetLastError());
   if (by_steps) {
     size_t tmp = 0;
@@ -1159,18 +1230,21 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
                                      (size_t)R.r_exact * 4, s));
   HIP_TRY(ctx->loc.need_keep((size_t)std::max<int64_t>(c1, 1) * 4,
                              (size_t)c0 * 4, s));
-  if (n > 0)
-    HIP_TRY(hipMemsetAsync(ctx->loc.as<int32_t>() + c0, 0, (size_t)n * 4, s));
+  HIP_TRY(zero32(s, ctx->loc.as<int32_t>() + c0, n));
   if (R.cfg.small_tier) {
     HIP_TRY(ctx->small.need_keep(c1, c0, sc.V, sc.P, sc.E, s));
   } else {
     HIP_TRY(ctx->small.need_keep(1, 0, sc.V, sc.P, sc.E, s));
   }
   ClipArgs a = clip_args(ctx, R);
+  // the re-run lists hold up to c1 entries (need_keep above)
+  a.cap_large = (int32_t)std::min<int64_t>(c1, 0x7fffffff);
+  a.cap_exact = a.cap_large;
   if (R.cfg.small_tier) {
     a.item_base = c0;
     a.slot_base = c0;
     a.n_items = n;
+    a.n_items_d64 = n_dev;
     if (by_steps) {  // slot = cell index, as with the identity order
       a.items = ctx->trip_v2.as<int32_t>();
       a.slot_by_cell = 1;
@@ -1180,16 +1254,19 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     // geometry sums over the lanes of a cell, so its rounding depends on
     // the cells per wavefront
     const int k_ev = ctx->n_small_ev;
-    if (n > 0 && k_ev < kSmallEv)
-      HIP_TRY(hipEventRecord(ctx->ev[16 + 2 * k_ev], s));
+    const bool ev = !R.graph && n > 0 && k_ev < kSmallEv;
+    if (ev) HIP_TRY(hipEventRecord(ctx->ev[16 + 2 * k_ev], s));
     if (int rc = launch_clip(ctx, 0, R.G, a, R.cfg.small_caps)) return rc;
-    if (n > 0 && k_ev < kSmallEv) {
+    if (ev) {
       HIP_TRY(hipEventRecord(ctx->ev[17 + 2 * k_ev], s));
       ctx->n_small_ev = k_ev + 1;
     }
     a.items = nullptr;  // the re-run tiers index their own lists
     a.slot_by_cell = 0;
+    a.n_items_d64 = nullptr;
   } else if (n > 0) {
+    if (R.graph)
+      return fail(ctx, "rpd_graph_build: the small tier is required");
     // every cell goes straight to the full-capacity tier
     std::vector<int32_t> idx(n);
     for (int64_t q = 0; q < n; ++q) idx[q] = (int32_t)(c0 + q);
@@ -1200,9 +1277,13 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     HIP_TRY(wait_stream(ctx, s));
   }
   if (events) HIP_TRY(hipEventRecord(ctx->ev[2], s));
-  int32_t host_cnt[8];
-  HIP_TRY(read_back(ctx, s, {{host_cnt, R.cnt, 32}}));
-  const int32_t n_over = host_cnt[1];
+  int32_t host_cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (!R.graph) HIP_TRY(read_back(ctx, s, {{host_cnt, R.cnt, 32}}));
+  // (a captured run launches its re-run tiers at capacity and reads their
+  // counts on the device)
+  const int32_t n_over =
+      R.graph ? (int32_t)(R.r_large + R.cap.n_mid) : host_cnt[1];
+  R.seen.n_mid = n_over - R.r_large;
   // cells over a tight small tier first get the middle tier (SmallCaps, 8
   // per wavefront; tier 3, slot = position in rerun_large); the ones over
   // that, or over any other small tier, get the full-capacity tier (tier 1,
@@ -1224,18 +1305,29 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
       m.out = ctx->mid.view(SmallCaps::V, SmallCaps::P, SmallCaps::E);
       m.rerun_large = ctx->rerun_full.as<int32_t>();
       m.n_rerun_large = R.cnt + 6;
+      m.cap_large = (int32_t)(R.r_full + (n_over - R.r_large));
+      if (R.graph) {
+        m.n_items_d32 = R.cnt + 1;
+        m.n_sub = R.r_large;
+      }
       if (m.work) m.work = ctx->work.as<unsigned long long>() + 256;
       if (int rc = launch_clip(ctx, 0, 8, m, 0)) return rc;
       hipLaunchKernelGGL(k_mark_rerun, dim3(ceil_div(n_over - R.r_large, 256)),
                          dim3(256), 0, s, ctx->rerun_large.as<int32_t>(),
-                         R.r_large, n_over, 3, ctx->loc.as<int32_t>());
+                         R.r_large, n_over, 3, ctx->loc.as<int32_t>(),
+                         R.graph ? R.cnt + 1 : nullptr);
       HIP_TRY(hipGetLastError());
-      HIP_TRY(read_back(ctx, s, {{host_cnt, R.cnt, 32}}));
+      if (!R.graph) HIP_TRY(read_back(ctx, s, {{host_cnt, R.cnt, 32}}));
     }
     full_list = ctx->rerun_full.as<int32_t>();
     full_r0 = R.r_full;
-    full_r1 = host_cnt[6];
+    full_r1 = R.graph ? (int32_t)(R.r_full + R.cap.n_full) : host_cnt[6];
+  } else if (R.graph) {
+    full_r1 = (int32_t)(R.r_large + R.cap.n_full);
   }
+  R.seen.n_full = full_r1 - full_r0;
+  // a captured run reads the full tier's count here
+  const int32_t* full_cnt = mid ? R.cnt + 6 : R.cnt + 1;
   HIP_TRY(ctx->large.need_keep(full_r1, full_r0, LargeCaps::V, LargeCaps::P,
                                LargeCaps::E, s));
   if (full_r1 > full_r0) {
@@ -1243,11 +1335,17 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     a.n_items = full_r1 - full_r0;
     a.slot_base = full_r0;
     a.out = ctx->large.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
+    if (R.graph) {
+      a.n_items_d32 = full_cnt;
+      a.n_sub = full_r0;
+    }
     if (a.work) a.work = ctx->work.as<unsigned long long>() + 256;
     if (int rc = launch_clip(ctx, 1, std::min(R.G, 4), a)) return rc;
-    HIP_TRY(read_back(ctx, s, {{host_cnt, R.cnt, 32}}));
+    if (!R.graph) HIP_TRY(read_back(ctx, s, {{host_cnt, R.cnt, 32}}));
   }
-  const int32_t n_exact = host_cnt[2];
+  const int32_t n_exact =
+      R.graph ? (int32_t)(R.r_exact + R.cap.n_exact) : host_cnt[2];
+  R.seen.n_exact = n_exact - R.r_exact;
   HIP_TRY(ctx->exact.need_keep(n_exact, R.r_exact, LargeCaps::V, LargeCaps::P,
                                LargeCaps::E, s));
   if (n_exact > R.r_exact) {
@@ -1255,17 +1353,22 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     a.n_items = n_exact - R.r_exact;
     a.slot_base = R.r_exact;
     a.out = ctx->exact.view(LargeCaps::V, LargeCaps::P, LargeCaps::E);
+    if (R.graph) {
+      a.n_items_d32 = R.cnt + 2;
+      a.n_sub = R.r_exact;
+    }
     if (a.work) a.work = ctx->work.as<unsigned long long>() + 512;
     if (int rc = launch_clip(ctx, 2, 1, a)) return rc;
   }
   if (full_r1 > full_r0)
     hipLaunchKernelGGL(k_mark_rerun, dim3(ceil_div(full_r1 - full_r0, 256)),
                        dim3(256), 0, s, full_list, full_r0, full_r1, 1,
-                       ctx->loc.as<int32_t>());
+                       ctx->loc.as<int32_t>(), R.graph ? full_cnt : nullptr);
   if (n_exact > R.r_exact)
     hipLaunchKernelGGL(k_mark_rerun, dim3(ceil_div(n_exact - R.r_exact, 256)),
                        dim3(256), 0, s, ctx->rerun_exact.as<int32_t>(),
-                       R.r_exact, n_exact, 2, ctx->loc.as<int32_t>());
+                       R.r_exact, n_exact, 2, ctx->loc.as<int32_t>(),
+                       R.graph ? R.cnt + 2 : nullptr);
   HIP_TRY(hipGetLastError());
   R.r_large = n_over;
   if (mid) R.r_full = full_r1;
@@ -1543,12 +1646,15 @@ int compact(rpd_ctx* ctx, RunState& R, int64_t n, const int32_t* perm,
   HIP_TRY(ctx->sizes.need(((size_t)nb + 1) * sizeof(Sizes3)));
   HIP_TRY(ctx->offs.need(((size_t)nb + 1) * sizeof(Sizes3)));
   HIP_TRY(ctx->drops.need(kDropSlots * 9 * 8));
-  HIP_TRY(hipMemsetAsync(ctx->drops.p, 0, kDropSlots * 9 * 8, s));
-  HIP_TRY(hipMemsetAsync(ctx->sizes.as<Sizes3>() + nb, 0, sizeof(Sizes3), s));
+  HIP_TRY(zero32(s, ctx->drops.p, kDropSlots * 9 * 2));
+  HIP_TRY(zero32(s, ctx->sizes.as<Sizes3>() + nb, sizeof(Sizes3) / 4));
+  // (a captured run: the cell count is tet_off[n_tet] on the device)
+  const int64_t* n_dev =
+      R.graph ? ctx->tet_off.as<int64_t>() + R.nt : nullptr;
   if (nb > 0)
     hipLaunchKernelGGL(k_block_sizes, dim3(nb), dim3(kCB), 0, s, n, perm,
                        ctx->loc.as<int32_t>(), tv, ctx->sizes.as<Sizes3>(),
-                       ctx->drops.as<int64_t>());
+                       ctx->drops.as<int64_t>(), n_dev);
   HIP_TRY(hipGetLastError());
   {
     size_t tmp = 0;
@@ -1561,8 +1667,14 @@ int compact(rpd_ctx* ctx, RunState& R, int64_t n, const int32_t* perm,
         ctx->scan_tmp.p, tmp, ctx->sizes.as<Sizes3>(), ctx->offs.as<Sizes3>(),
         Sizes3Sum(), zero, (int)(nb + 1), s));
   }
-  HIP_TRY(read_back(ctx, s,
-                    {{tot, ctx->offs.as<Sizes3>() + nb, sizeof(Sizes3)}}));
+  if (R.graph)
+    *tot = Sizes3{R.cap.cells, R.cap.words, R.cap.areas};
+  else
+    HIP_TRY(read_back(ctx, s,
+                      {{tot, ctx->offs.as<Sizes3>() + nb, sizeof(Sizes3)}}));
+  R.seen.cells = tot->cells;
+  R.seen.words = tot->words;
+  R.seen.areas = tot->areas;
   HIP_TRY(ctx->out_cells.need((size_t)std::max<int64_t>(tot->cells, 1) *
                               sizeof(rpd_cell)));
   HIP_TRY(ctx->out_words.need((size_t)std::max<int64_t>(tot->words, 1) * 4));
@@ -1574,21 +1686,13 @@ int compact(rpd_ctx* ctx, RunState& R, int64_t n, const int32_t* perm,
                        ctx->offs.as<Sizes3>(), ctx->out_cells.as<rpd_cell>(),
                        ctx->out_words.as<uint32_t>(),
                        ctx->out_geom.as<double>(), ctx->out_areas.as<double>(),
-                       R.cfg.compute_geometry);
+                       R.cfg.compute_geometry, n_dev, *tot, R.cnt + 7);
   HIP_TRY(hipGetLastError());
   return 0;
 }
 
-}  // namespace
-
-extern "C" {
-
-int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
-  if (!ctx) return -1;
-  if (!ctx->has_mesh || !ctx->has_sites)
-    return fail(ctx, "rpd_run: no resident instance");
-  HIP_TRY(hipSetDevice(ctx->device));
-  RunState R;
+// completes and checks a run configuration into R.cfg, R.sc, R.G
+int run_config(rpd_ctx* ctx, const rpd_run_cfg* cfg_in, RunState& R) {
   rpd_run_cfg& cfg = R.cfg;
   cfg = rpd_run_cfg{};
   cfg.cells_per_wave = 0;
@@ -1611,20 +1715,22 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   if (cfg.learned && ctx->has_subset)
     return fail(ctx, "rpd_run: a tet subset is for the plain path only "
                      "(learned = 0)");
-  ctx->last_geometry = false;
-  ctx->has_results = false;
-  ctx->adj_ready = false;
-  const int32_t site_k = ctx->st.site_k;
-  std::memset(&ctx->st, 0, sizeof(ctx->st));
-  ctx->st.site_k = site_k;
+  return 0;
+}
+
+// the device stages of one run: candidates, clip tiers, (learned)
+// certificate and fallback, compaction.  A captured run (R.graph) records
+// no events and reads nothing back.
+int run_stages(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out,
+               int64_t* n_out_out, Sizes3* tot) {
+  const rpd_run_cfg& cfg = R.cfg;
   HIP_TRY(ctx->counters.need(8 * 4));
   R.s = ctx->stream;
   hipStream_t s = R.s;
-  HIP_TRY(hipEventRecord(ctx->ev[0], s));
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[0], s));
   ctx->n_small_ev = 0;
-  HIP_TRY(hipMemsetAsync(ctx->counters.p, 0, 8 * 4, s));
+  HIP_TRY(zero32(s, ctx->counters.p, 8));
   if (int rc = init_state(ctx, R)) return rc;
-  const int32_t nt = R.nt;
   if (cfg.count_work) {
     HIP_TRY(ctx->rel_work.need(64 * 4 * 8));
     HIP_TRY(hipMemsetAsync(ctx->rel_work.p, 0, 64 * 4 * 8, s));
@@ -1643,7 +1749,7 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   // each cell certified as it is clipped)
   if (cfg.learned)
     if (int rc = certify_begin(ctx, R, n_cand)) return rc;
-  if (int rc = clip_range(ctx, R, 0, n_cand, true)) return rc;
+  if (int rc = clip_range(ctx, R, 0, n_cand, !R.graph)) return rc;
   // ---- 3. (learned) certificate and fallback; 4. compaction
   int64_t n_out = n_cand;
   const int32_t* perm = nullptr;
@@ -1651,28 +1757,35 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
     if (int rc = certify_and_fallback(ctx, R, n_cand, &n_out)) return rc;
     perm = ctx->perm.as<int32_t>();
   }
-  HIP_TRY(hipEventRecord(ctx->ev[5], s));
-  Sizes3 tot;
-  if (int rc = compact(ctx, R, n_out, perm, &tot)) return rc;
-  HIP_TRY(hipEventRecord(ctx->ev[4], s));
-
-  // ---- statistics (outside the timed phases)
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[5], s));
+  if (int rc = compact(ctx, R, n_out, perm, tot)) return rc;
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[4], s));
+  // tet_k: max candidates of one tet, into stats64[2]
   HIP_TRY(ctx->stats64.need(4 * 8));
-  HIP_TRY(hipMemsetAsync(ctx->stats64.p, 0, 4 * 8, s));
+  HIP_TRY(zero32(s, ctx->stats64.p, 8));
+  {
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceReduce::Max(nullptr, tmp, ctx->tet_count.as<int64_t>(),
+                                      ctx->stats64.as<int64_t>() + 2, R.nt, s));
+    HIP_TRY(ctx->scan_tmp.need(tmp));
+    HIP_TRY(hipcub::DeviceReduce::Max(ctx->scan_tmp.p, tmp,
+                                      ctx->tet_count.as<int64_t>(),
+                                      ctx->stats64.as<int64_t>() + 2, R.nt, s));
+  }
+  *n_cand_out = n_cand;
+  *n_out_out = n_out;
+  return 0;
+}
+
+// statistics of a finished eager run (outside the timed phases)
+int run_stats(rpd_ctx* ctx, RunState& R, int64_t n_cand, int64_t n_out,
+              const Sizes3& tot) {
+  const rpd_run_cfg& cfg = R.cfg;
+  hipStream_t s = R.s;
   if (n_cand > 0 && cfg.count_work)
     hipLaunchKernelGGL(k_mask_stats, dim3(ceil_div(n_cand, 256)), dim3(256), 0,
                        s, n_cand, ctx->cand.as<Cand>(), R.S,
                        ctx->stats64.as<unsigned long long>());
-  // tet_k: max candidates of one tet, into stats64[2]
-  {
-    size_t tmp = 0;
-    HIP_TRY(hipcub::DeviceReduce::Max(nullptr, tmp, ctx->tet_count.as<int64_t>(),
-                                      ctx->stats64.as<int64_t>() + 2, nt, s));
-    HIP_TRY(ctx->scan_tmp.need(tmp));
-    HIP_TRY(hipcub::DeviceReduce::Max(ctx->scan_tmp.p, tmp,
-                                      ctx->tet_count.as<int64_t>(),
-                                      ctx->stats64.as<int64_t>() + 2, nt, s));
-  }
   int64_t st64[4], drops[kDropSlots * 9];
   int32_t host_cnt[8];
   HIP_TRY(read_back(ctx, s,
@@ -1736,7 +1849,32 @@ int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
   st.ms_clip_small_sum = 0;
   for (int k = 0; k < ctx->n_small_ev; ++k)
     st.ms_clip_small_sum += el(16 + 2 * k, 17 + 2 * k);
-  ctx->last_geometry = cfg.compute_geometry != 0;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rpd_run(rpd_ctx* ctx, const rpd_run_cfg* cfg_in) {
+  if (!ctx) return -1;
+  if (!ctx->has_mesh || !ctx->has_sites)
+    return fail(ctx, "rpd_run: no resident instance");
+  HIP_TRY(hipSetDevice(ctx->device));
+  RunState R;
+  if (int rc = run_config(ctx, cfg_in, R)) return rc;
+  ctx->last_geometry = false;
+  ctx->has_results = false;
+  ctx->adj_ready = false;
+  const int32_t site_k = ctx->st.site_k;
+  std::memset(&ctx->st, 0, sizeof(ctx->st));
+  ctx->st.site_k = site_k;
+  int64_t n_cand = 0, n_out = 0;
+  Sizes3 tot{0, 0, 0};
+  if (int rc = run_stages(ctx, R, &n_cand, &n_out, &tot)) return rc;
+  if (int rc = run_stats(ctx, R, n_cand, n_out, tot)) return rc;
+  ctx->seen = R.seen;
+  ctx->last_geometry = R.cfg.compute_geometry != 0;
   ctx->has_results = true;
   return 0;
 }
@@ -2052,6 +2190,207 @@ int rpd_adjacency_download(rpd_ctx* ctx, const rpd_adj_out* o) {
   HIP_TRY(get(o->edges, ctx->edg_runs, (size_t)z.n_edges * 24));
   HIP_TRY(get(o->edge_cells, ctx->edg_rec, (size_t)z.n_edge_cells * 8));
   HIP_TRY(wait_stream(ctx, s));
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// what a captured graph depends on besides the data it reads: the batch's
+// per-instance tet and site counts, the tet subset switch, the pinned
+// instance table and every buffer address (g_buf_generation)
+std::vector<int64_t> graph_signature(const rpd_ctx* ctx) {
+  std::vector<int64_t> v;
+  v.push_back((int64_t)ctx->inst.size());
+  for (const auto& I : ctx->inst) {
+    v.push_back(I.n_tet);
+    v.push_back(I.n_site);
+  }
+  v.push_back(ctx->has_subset);
+  v.push_back((int64_t)(intptr_t)ctx->pin_inst);
+  v.push_back((int64_t)g_buf_generation.load());
+  return v;
+}
+
+void graph_drop(rpd_ctx* ctx) {
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->graph_exec = nullptr;
+  ctx->graph = nullptr;
+  ctx->graph_sig.clear();
+}
+
+int64_t grow(int64_t n, double f, int64_t pad) {
+  return (int64_t)std::ceil((double)n * f) + pad;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom) {
+  if (!ctx) return -1;
+  if (!ctx->has_mesh || !ctx->has_sites)
+    return fail(ctx, "rpd_graph_build: no resident instance");
+  if (headroom < 0 || headroom > 10)
+    return fail(ctx, "rpd_graph_build: headroom must be in [0, 10]");
+  HIP_TRY(hipSetDevice(ctx->device));
+  RunState R;
+  if (int rc = run_config(ctx, cfg, R)) return rc;
+  if (R.cfg.learned || R.cfg.count_work || !R.cfg.small_tier)
+    return fail(ctx, "rpd_graph_build: plain path with the small tier only "
+                     "(learned = 0, count_work = 0, small_tier = 1)");
+  graph_drop(ctx);
+  // 1. an eager run: the results, and the counts the capacities follow
+  if (int rc = rpd_run(ctx, &R.cfg)) return rc;
+  const rpd_stats st_eager = ctx->st;
+  const RunCaps& n = ctx->seen;
+  const double f = 1.0 + headroom;
+  RunCaps cap;
+  cap.n_reg = grow(n.n_reg, f, 1024);
+  cap.n_cand = grow(n.n_cand, f, 1024);
+  cap.n_mid = grow(n.n_mid, f, 256);
+  cap.n_full = grow(n.n_full, f, 256);
+  cap.n_exact = grow(n.n_exact, f, 256);
+  cap.cells = grow(n.cells, f, 1024);
+  cap.words = grow(n.words, f, 65536);
+  cap.areas = grow(n.areas, f, 16384);
+  if (cap.n_reg >= (1LL << 31) || cap.n_cand >= (1LL << 31))
+    return fail(ctx, "rpd_graph_build: capacities over 2^31 entries");
+  // the neighbour lists and the tables sized by them may lengthen in later
+  // sphere updates: room for them too, so that those keep their addresses
+  {
+    const size_t nk = (size_t)std::max<int64_t>(grow(ctx->n_knn, f, 1024), 1);
+    HIP_TRY(ctx->knn_bis.need(nk * 32));
+    HIP_TRY(ctx->knn_ids.need_keep(nk * 4, (size_t)ctx->n_knn * 4,
+                                   ctx->stream));
+    HIP_TRY(ctx->knn_near.need_keep(nk * 4, (size_t)ctx->n_knn * 4,
+                                    ctx->stream));
+    HIP_TRY(ctx->knn_site_near.need_keep(nk * 32, (size_t)ctx->n_knn * 32,
+                                         ctx->stream));
+  }
+  // 2. the captured stages once without capture: every buffer reaches its
+  // capacity here, so the capture allocates nothing
+  RunState P = R;
+  P.graph = true;
+  P.cap = cap;
+  int64_t nc = 0, no = 0;
+  Sizes3 tot{0, 0, 0};
+  if (int rc = run_stages(ctx, P, &nc, &no, &tot)) return rc;
+  HIP_TRY(wait_stream(ctx, ctx->stream));
+  // 3. capture
+  RunState Q = R;
+  Q.graph = true;
+  Q.cap = cap;
+  t_capture = true;
+  hipError_t e = hipStreamBeginCapture(ctx->stream,
+                                       hipStreamCaptureModeThreadLocal);
+  int rc = e == hipSuccess ? run_stages(ctx, Q, &nc, &no, &tot) : -2;
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+  t_capture = false;
+  if (e != hipSuccess)
+    return fail(ctx, std::string("rpd_graph_build: hipStreamBeginCapture: ") +
+                         hipGetErrorString(e), -2);
+  if (rc != 0) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e2 != hipSuccess || !g)
+    return fail(ctx, std::string("rpd_graph_build: hipStreamEndCapture: ") +
+                         hipGetErrorString(e2), -2);
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(ctx, std::string("rpd_graph_build: hipGraphInstantiate: ") +
+                         hipGetErrorString(e), -2);
+  }
+  ctx->graph = g;
+  ctx->graph_exec = x;
+  ctx->graph_cap = cap;
+  ctx->graph_cfg = R.cfg;
+  ctx->graph_headroom = headroom;
+  ctx->graph_sig = graph_signature(ctx);
+  ctx->st = st_eager;  // the eager run's results are the current ones
+  return 0;
+}
+
+int rpd_graph_run(rpd_ctx* ctx, int32_t* replayed) {
+  if (!ctx) return -1;
+  if (replayed) *replayed = 0;
+  if (!ctx->graph_exec)
+    return fail(ctx, "rpd_graph_run: no graph (rpd_graph_build first)");
+  if (!ctx->has_mesh || !ctx->has_sites)
+    return fail(ctx, "rpd_graph_run: no resident instance");
+  HIP_TRY(hipSetDevice(ctx->device));
+  // the batch or a buffer changed since the capture, or the neighbour
+  // lists outgrew the captured bisector table: capture again (its eager run
+  // gives this call's results)
+  if (graph_signature(ctx) != ctx->graph_sig ||
+      (size_t)std::max<int64_t>(ctx->n_knn, 1) * 32 > ctx->knn_bis.cap) {
+    const rpd_run_cfg c = ctx->graph_cfg;
+    return rpd_graph_build(ctx, &c, ctx->graph_headroom);
+  }
+  ctx->last_geometry = false;
+  ctx->has_results = false;
+  ctx->adj_ready = false;
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipEventRecord(ctx->ev[0], s));
+  HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
+  HIP_TRY(hipEventRecord(ctx->ev[4], s));
+  const RunCaps& cap = ctx->graph_cap;
+  const int64_t nb = ceil_div(cap.n_cand, kCB);
+  int64_t st64[4], drops[kDropSlots * 9], n_cand = 0, n_reg = 0;
+  int32_t cnt[8];
+  Sizes3 tot{0, 0, 0};
+  HIP_TRY(read_back(
+      ctx, s,
+      {{st64, ctx->stats64.p, 32},
+       {drops, ctx->drops.p, sizeof(drops)},
+       {cnt, ctx->counters.p, 32},
+       {&tot, ctx->offs.as<Sizes3>() + nb, sizeof(Sizes3)},
+       {&n_cand, ctx->tet_off.as<int64_t>() + ctx->n_tet, 8},
+       {&n_reg, ctx->reg_off.as<int64_t>() + ctx->n_site, 8}}));
+  const bool mid = small_caps(ctx->graph_cfg.small_caps).V < SmallCaps::V;
+  const bool over =
+      cnt[7] != 0 || n_reg > cap.n_reg || n_cand > cap.n_cand ||
+      (mid ? (cnt[1] > cap.n_mid || cnt[6] > cap.n_full)
+           : cnt[1] > cap.n_full) ||
+      cnt[2] > cap.n_exact || tot.cells > cap.cells ||
+      tot.words > cap.words || tot.areas > cap.areas;
+  if (over) {  // counts beyond the capture's capacities: capture again
+    const rpd_run_cfg c = ctx->graph_cfg;
+    return rpd_graph_build(ctx, &c, ctx->graph_headroom);
+  }
+  rpd_stats& st = ctx->st;
+  const int32_t site_k = st.site_k;
+  std::memset(&st, 0, sizeof(st));
+  st.site_k = site_k;
+  st.n_cells = tot.cells;
+  st.n_candidates = n_cand;
+  st.n_words = tot.words;
+  st.n_areas = tot.areas;
+  for (int q = 0; q < 9; ++q)
+    for (int b = 0; b < kDropSlots; ++b) st.n_dropped[q] += drops[9 * b + q];
+  st.n_rerun_large = cnt[1];
+  st.n_rerun_exact = cnt[2];
+  st.n_bbox_fallback = cnt[0];
+  st.n_bin_registrations = n_reg;
+  st.tet_k = (int32_t)st64[2];
+  float ms = 0;
+  st.ms_total =
+      hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[4]) == hipSuccess ? ms : -1;
+  ctx->last_geometry = ctx->graph_cfg.compute_geometry != 0;
+  ctx->has_results = true;
+  if (replayed) *replayed = 1;
+  return 0;
+}
+
+int rpd_graph_destroy(rpd_ctx* ctx) {
+  if (!ctx) return -1;
+  graph_drop(ctx);
   return 0;
 }
 

@@ -42,6 +42,24 @@ __device__ __forceinline__ rpd_d4 ld4(const double4* p) {
   return rpd_d4{v.x, v.y, v.z, v.w};
 }
 
+// Counts of a captured pipeline (hipGraph): a launch is sized by its
+// capacity and reads the count the previous kernels left on the device;
+// without a device count the host's count is exact.
+__device__ __forceinline__ int64_t dev_count(int64_t cap, const int64_t* d,
+                                             int64_t sub = 0) {
+  return d ? min(cap, *d - sub) : cap;
+}
+__device__ __forceinline__ int64_t dev_count32(int64_t cap, const int32_t* d,
+                                               int64_t sub = 0) {
+  return d ? min(cap, (int64_t)*d - sub) : cap;
+}
+
+__global__ __launch_bounds__(256) void k_fill32(int32_t* p, int64_t n,
+                                                int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // --------------------------------------------------------- predicates
 // det4x4(p1.x, p2.x, p3.x, e.x, p1.y, ..., p1.w, p2.w, p3.w, e.w) in the
 // reference's evaluation order (convex_cell.cu:458-459,
@@ -433,9 +451,15 @@ __global__ __launch_bounds__(256) void k_bin_expand(int32_t n_site,
                                                     const InstInfo* inst,
                                                     const int32_t* site_inst,
                                                     const int64_t* reg_off,
-                                                    uint64_t* keys) {
+                                                    uint64_t* keys,
+                                                    const int64_t* n_reg_dev,
+                                                    uint64_t pad) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_reg) return;
+  if (n_reg_dev && q >= dev_count(n_reg, n_reg_dev)) {
+    keys[q] = pad;  // sorts after every registration
+    return;
+  }
   // sphere i with reg_off[i] <= q < reg_off[i+1]
   int lo = 0, hi = n_site;
   while (hi - lo > 1) {
@@ -461,11 +485,13 @@ __global__ __launch_bounds__(256) void k_bin_expand(int32_t n_site,
 
 // bin_start[b] = first registration of bin >= b in the sorted keys
 __global__ __launch_bounds__(256) void k_bin_starts(int64_t n_bins,
-                                                    int64_t n_reg,
+                                                    int64_t n_reg_cap,
                                                     const uint64_t* keys,
                                                     int32_t* bin_start,
-                                                    int32_t* bin_ids) {
+                                                    int32_t* bin_ids,
+                                                    const int64_t* n_reg_dev) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n_reg = dev_count(n_reg_cap, n_reg_dev);
   if (b < n_reg) bin_ids[b] = (int32_t)(keys[b] & 0xffffffffu);
   if (b > n_bins) return;
   int64_t lo = 0, hi = n_reg;  // first index with key >> 32 >= b
@@ -586,10 +612,14 @@ __global__ __launch_bounds__(256) void k_tet_fill(
     DevMesh M, DevSites S, const InstInfo* inst, const int32_t* tet_inst,
     const int32_t* bin_start, const int32_t* bin_ids, const int64_t* tet_off,
     const uint64_t* tet_pass, Cand* cand, int32_t* rel_ids,
-    const uint8_t* tet_sel) {
+    const uint8_t* tet_sel, int64_t cap, int32_t* overflow) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= M.n_tet) return;
   if (tet_sel && !tet_sel[t]) return;
+  if (tet_off[t + 1] > cap) {  // over a captured pipeline's capacity
+    if (tet_off[t] < cap || t == 0) overflow[0] = 1;
+    return;
+  }
   TetLoad L;
   load_tet(M, t, L);
   double c[3];
@@ -691,9 +721,17 @@ __global__ __launch_bounds__(256) void k_cand_mask(DevMesh M, DevSites S,
                                                    Cand* cand,
                                                    uint8_t* trips,
                                                    int32_t* trip_ids,
-                                                   int64_t c0) {
+                                                   int64_t c0,
+                                                   const int64_t* n_dev) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_cand) return;
+  if (n_dev && c >= dev_count(n_cand, n_dev, c0)) {
+    if (trips) {  // past the count: sorts after every cell
+      trips[c] = 0;
+      trip_ids[c] = (int32_t)(c0 + c);
+    }
+    return;
+  }
   const int t = cand[c].tet, i = cand[c].seed;
   TetLoad L;
   load_tet(M, t, L);
@@ -773,6 +811,14 @@ struct ClipArgs {
   int32_t* n_rerun_large;
   int32_t* rerun_exact;
   int32_t* n_rerun_exact;
+  // a captured pipeline: n_items is the launch's capacity and the count is
+  // on the device (n_items_d64, or n_items_d32 minus n_sub); the re-run
+  // lists hold cap_large / cap_exact entries, more set overflow[0]
+  const int64_t* n_items_d64;
+  const int32_t* n_items_d32;
+  int64_t n_sub;
+  int32_t cap_large, cap_exact;
+  int32_t* overflow;
   unsigned long long* work;  // 64 x 4 spread counters, or nullptr
   CertArgs cert;
 };
@@ -971,7 +1017,10 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
     if (pos < per) blk = xcd * per + pos;
   }
   const int64_t item = blk * G + g;
-  if (item >= a.n_items) return;
+  const int64_t n_items =
+      a.n_items_d64 ? dev_count(a.n_items, a.n_items_d64)
+                    : dev_count32(a.n_items, a.n_items_d32, a.n_sub);
+  if (item >= n_items) return;
   const int64_t cidx = a.items ? a.items[item] : a.item_base + item;
   const int64_t slot = a.slot_by_cell ? cidx : a.slot_base + item;
   // the slot's edge records are written as the edges are appended (the
@@ -1573,10 +1622,16 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
                   ((uint32_t)nb_e << 16) | (status << 24);
     if (status == kStatusRerunLarge) {
       int r = atomicAdd(a.n_rerun_large, 1);
-      a.rerun_large[r] = (int32_t)cidx;
+      if (r < a.cap_large)
+        a.rerun_large[r] = (int32_t)cidx;
+      else
+        a.overflow[0] = 1;
     } else if (status == kStatusRerunExact) {
       int r = atomicAdd(a.n_rerun_exact, 1);
-      a.rerun_exact[r] = (int32_t)cidx;
+      if (r < a.cap_exact)
+        a.rerun_exact[r] = (int32_t)cidx;
+      else
+        a.overflow[0] = 1;
     }
   }
 }
@@ -1587,9 +1642,10 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
 __global__ __launch_bounds__(256) void k_mark_rerun(const int32_t* list,
                                                     int32_t r0, int32_t r1,
                                                     int32_t tier,
-                                                    int32_t* loc) {
+                                                    int32_t* loc,
+                                                    const int32_t* r1_dev) {
   const int r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= r1) return;
+  if (r >= (r1_dev ? min(r1, *r1_dev) : r1)) return;
   loc[list[r]] = (tier << 28) | r;
 }
 
@@ -1648,8 +1704,10 @@ __global__ __launch_bounds__(kCB) void k_block_sizes(int64_t n_cells,
                                                      const int32_t* loc,
                                                      TierViews tv,
                                                      Sizes3* bsum,
-                                                     int64_t* drops) {
+                                                     int64_t* drops,
+                                                     const int64_t* n_dev) {
   typedef hipcub::BlockReduce<uint64_t, kCB> Red;
+  n_cells = dev_count(n_cells, n_dev);
   __shared__ typename Red::TempStorage tmp;
   __shared__ int32_t h[9];
   if (threadIdx.x < 9) h[threadIdx.x] = 0;
@@ -1682,7 +1740,7 @@ __global__ __launch_bounds__(kCB) void k_compact(
     int64_t n_cells, const int32_t* perm, const Cand* cand,
     const int32_t* loc, TierViews tv, const Sizes3* boff, rpd_cell* out_cells,
     uint32_t* out_words, double* out_geom, double* out_areas,
-    int32_t geometry) {
+    int32_t geometry, const int64_t* n_dev, Sizes3 cap, int32_t* overflow) {
   typedef hipcub::BlockScan<uint64_t, kCB> Scan;
   __shared__ typename Scan::TempStorage tmp;
   __shared__ uint32_t s_w[kCB + 1];
@@ -1695,6 +1753,16 @@ __global__ __launch_bounds__(kCB) void k_compact(
   __shared__ uint32_t s_oe[kCB];
   const int i = threadIdx.x;
   const int64_t r = (int64_t)blockIdx.x * kCB + i;
+  n_cells = dev_count(n_cells, n_dev);
+  {  // a captured pipeline's output capacity: the block's whole range
+    const Sizes3 B0 = boff[blockIdx.x], B1 = boff[blockIdx.x + 1];
+    if (B1.cells > cap.cells || B1.words > cap.words ||
+        B1.areas > cap.areas) {
+      if (i == 0) overflow[0] = 1;
+      return;
+    }
+    (void)B0;
+  }
   CellRef x{0, 0, 0};
   int64_t c = 0;
   uint64_t z = 0;

@@ -85,9 +85,10 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
 // id word, near_word)
 __global__ __launch_bounds__(256) void k_site_inline(DevSites S, int64_t n_knn,
                                                      int32_t* knn_near,
-                                                     double4* knn_site_near) {
+                                                     double4* knn_site_near,
+                                                     const int64_t* n_dev) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n_knn) return;
+  if (q >= dev_count(n_knn, n_dev)) return;
   const int m = S.knn_ids[q];
   knn_near[q] = near_word(S, m);
   knn_site_near[q] = S.site[m];

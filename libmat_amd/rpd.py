@@ -160,6 +160,12 @@ def _declare_capi(L):
     L.rpd_site_centroids.restype = ctypes.c_int
     L.rpd_set_tet_subset.argtypes = [vp, i32, vp]
     L.rpd_set_tet_subset.restype = ctypes.c_int
+    L.rpd_graph_build.argtypes = [vp, ctypes.POINTER(RunCfg), ctypes.c_double]
+    L.rpd_graph_build.restype = ctypes.c_int
+    L.rpd_graph_run.argtypes = [vp, vp]
+    L.rpd_graph_run.restype = ctypes.c_int
+    L.rpd_graph_destroy.argtypes = [vp]
+    L.rpd_graph_destroy.restype = ctypes.c_int
     L.rpd_adjacency.argtypes = [vp, ctypes.POINTER(AdjSizes)]
     L.rpd_adjacency.restype = ctypes.c_int
     L.rpd_adjacency_download.argtypes = [vp, ctypes.POINTER(AdjOut)]
@@ -190,7 +196,8 @@ CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
                 "rpd_download_relation", "rpd_calib_copy",
                 "rpd_power_neighbors_gpu", "rpd_run_noalloc",
                 "rpd_site_centroids", "rpd_set_tet_subset", "rpd_adjacency",
-                "rpd_adjacency_download")
+                "rpd_adjacency_download", "rpd_graph_build", "rpd_graph_run",
+                "rpd_graph_destroy")
 
 
 def _p(a):
@@ -402,6 +409,28 @@ class Context:
                      int(proposal_k))
         self._check(self._L.rpd_run(self._h, ctypes.byref(cfg)), "rpd_run")
         return self.stats()
+
+    def graph_build(self, headroom=0.1, cells_per_wave=8, geometry=True,
+                    small_caps=0):
+        """Capture the plain pipeline of the resident batch into a hipGraph
+        (rpd_graph_build); the eager run it makes gives the results."""
+        cfg = RunCfg(cells_per_wave, 1, int(geometry), 0, 1, 0,
+                     int(small_caps), 0, 0.0, 8)
+        self._check(self._L.rpd_graph_build(self._h, ctypes.byref(cfg),
+                                            float(headroom)),
+                    "rpd_graph_build")
+        return self.stats()
+
+    def graph_run(self):
+        """Replay the captured pipeline on the current data; returns (stats,
+        replayed) -- replayed False when it had to capture again."""
+        r = ctypes.c_int32(0)
+        self._check(self._L.rpd_graph_run(self._h, ctypes.byref(r)),
+                    "rpd_graph_run")
+        return self.stats(), bool(r.value)
+
+    def graph_destroy(self):
+        self._check(self._L.rpd_graph_destroy(self._h), "rpd_graph_destroy")
 
     def run_noalloc(self, inst, out, cells_per_wave=8, small_caps=0):
         """rpd_run_noalloc of ``inst`` (host arrays) into the caller's device
