@@ -76,6 +76,8 @@ def parse():
                          "(1/16 of it at one thread)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the captured-graph leg of the batched modes")
     ap.add_argument("--no-work-count", action="store_true",
                     help="skip the extra untimed run that counts FP64 work "
                          "(keeps profiler averages to the timed launches)")
@@ -338,6 +340,10 @@ def bench_batched(a, emit=True):
         w = prep.batch_workload(a.batch_instances, 0, progress=progress)
         sets = [[(i.sites, i.flags, i.knn_off, i.knn_ids)
                  for i in w.instances]]
+        # the same iterations with every sphere a site (hidden ones
+        # unflagged), so that each instance keeps its site count: the
+        # captured-graph leg replays one graph over them
+        sets_all = [[prep.sites_both(x)[1] for x in w.spheres]]
         cur = list(w.spheres)
         # the neighbour lists of each sphere-update iteration come from the
         # device builder (identical to the host builder's)
@@ -347,14 +353,16 @@ def bench_batched(a, emit=True):
         def sites(x):
             nonlocal nb_host
             try:
-                return prep.sites_of(x, ctx=nb_ctx)[:4]
+                return prep.sites_both(x, ctx=nb_ctx)
             except RuntimeError:  # a cell over the device builder's caps
                 nb_host += 1
-                return prep.sites_of(x)[:4]
+                return prep.sites_both(x)
         for it in range(1, a.batch_iterations):
             cur = [prep.jitter_spheres(x, 0.002, 7919 * it + b)
                    for b, x in enumerate(cur)]
-            sets.append([sites(x) for x in cur])
+            both = [sites(x) for x in cur]
+            sets.append([x[0] for x in both])
+            sets_all.append([x[1] for x in both])
             print(f"[batched] neighbour lists of iteration {it} ready",
                   file=sys.stderr, flush=True)
         nb_ctx.close()
@@ -371,6 +379,7 @@ def bench_batched(a, emit=True):
         w = prep.gmm_workload(n_inst, 0, progress=progress)
         sets = [[(i.sites, i.flags, i.knn_off, i.knn_ids)
                  for i in w.instances]] + [None] * (a.steps - 1)
+        sets_all = sets
         desc = (f"{n_inst} instances, tets log-U[50k,500k] (jittered Kuhn "
                 f"grids), spheres log-U[5k,40k] from a 32-component "
                 f"Gaussian mixture (sigma~U[0.02,0.1]), r~lognormal(-3.5,"
@@ -389,6 +398,11 @@ def bench_batched(a, emit=True):
                    "ms_per_step": sum(ms) / len(ms), "step_ms": ms,
                    "step_cells": cells, "kernel_ms": time_batch.kernel}
         print(f"[{a.workload}] plain batched timed", file=sys.stderr,
+              flush=True)
+    graph = None
+    if not a.learned_only and not a.no_graph:
+        graph = bench_graph(ctx, sets_all, a.warmup, G, caps)
+        print(f"[{a.workload}] captured graph timed", file=sys.stderr,
               flush=True)
     learned = roofline = None
     if not a.no_learned:
@@ -469,7 +483,8 @@ def bench_batched(a, emit=True):
                    "path": ("learned+batched" if learned is not None
                             else "plain batched"),
                    "tuned_cells_per_wave": G, "tuned_small_caps": caps},
-        "plain": {"batched": batched, "one_at_a_time": one},
+        "plain": {"batched": batched, "one_at_a_time": one,
+                  "graph": graph},
         "learned": learned,
         "learned_vs_plain_batched": (
             learned["value"] / batched["value"]
@@ -490,6 +505,46 @@ def bench_batched(a, emit=True):
         print(json.dumps({k: v for k, v in out.items()
                           if not k.startswith("_")}))
     return out
+
+
+def bench_graph(ctx, sets, warmup, G, caps):
+    """The plain batched step as one replay of a captured hipGraph
+    (rpd_graph_build / rpd_graph_run) against eager rpd_run on the same
+    site sets (every sphere a site, so instances keep their site counts
+    across iterations); both timed with HIP events around the device work.
+    A replay whose counts outgrow the capture's capacities, or a changed
+    batch, is captured again (counted in "recaptured")."""
+    kw = dict(cells_per_wave=G, small_caps=caps)
+    ems, ecells, _, _ = time_batch(ctx, sets, warmup, **kw)
+    if sets[0] is not None:
+        ctx.update_sites_batch(sets[0])
+    t0 = time.perf_counter()
+    ctx.graph_build(headroom=0.1, **kw)
+    t_build = time.perf_counter() - t0
+    for _ in range(warmup):
+        ctx.graph_run()
+    gms, gcells, replayed, wall = [], [], 0, []
+    for it, st_set in enumerate(sets):
+        if it and st_set is not None:
+            ctx.update_sites_batch(st_set)
+        t1 = time.perf_counter()
+        st, rep = ctx.graph_run()
+        wall.append((time.perf_counter() - t1) * 1e3)
+        gms.append(st["ms_total"])
+        gcells.append(st["n_cells"])
+        replayed += rep
+    ref = ctx.download()
+    ctx.run(**kw)
+    identical = same_result(ctx.download(), ref) and gcells == ecells
+    ctx.graph_destroy()
+    return {"value": sum(gcells) / (sum(gms) * 1e-3), "unit": "cells/s",
+            "ms_per_step": sum(gms) / len(gms), "step_ms": gms,
+            "host_wall_ms_per_step": sum(wall) / len(wall),
+            "eager_ms_per_step": sum(ems) / len(ems),
+            "eager_value": sum(ecells) / (sum(ems) * 1e-3),
+            "replayed": replayed, "recaptured": len(sets) - replayed,
+            "build_s": t_build, "identical_to_eager": identical,
+            "sites": "every sphere a site (hidden ones unflagged)"}
 
 
 def bench_heldout10(a):

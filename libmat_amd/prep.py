@@ -168,6 +168,39 @@ def sites_of(xyzr, box=UNIT_BOX, n_threads=0, ctx=None):
             knn_off, knn_ids, keep)
 
 
+def sites_both(xyzr, box=UNIT_BOX, ctx=None):
+    """The sites of ``sites_of`` (non-hidden spheres, renumbered) and the
+    same neighbour lists over every sphere in its own numbering (flag 1 for
+    the non-hidden ones, 0 with an empty list for the hidden ones, which no
+    list holds): the second form keeps each instance's site count fixed
+    across sphere updates.  One neighbour computation for both."""
+    xyzw = np.ascontiguousarray(xyzr, np.float64).copy()
+    xyzw[:, 3] = xyzw[:, 3] * xyzw[:, 3]
+    if ctx is not None:
+        valid, off, ids, _ = ctx.power_neighbors(xyzw, box)
+    else:
+        valid, off, ids = power_neighbors(xyzw, box, 0)
+    valid = np.asarray(valid).astype(bool)
+    keep = np.nonzero(valid)[0].astype(np.int32)
+    remap = -np.ones(xyzw.shape[0], np.int64)
+    remap[keep] = np.arange(keep.size)
+    deg = np.diff(off)[keep]
+    knn_off = np.zeros(keep.size + 1, np.int64)
+    np.cumsum(deg, out=knn_off[1:])
+    sel = np.concatenate([np.arange(off[i], off[i + 1]) for i in keep]) \
+        if keep.size else np.zeros(0, np.int64)
+    knn_ids = remap[ids[sel]].astype(np.int32) if sel.size else \
+        np.zeros(0, np.int32)
+    compact = (np.ascontiguousarray(xyzw[keep]), np.ones(keep.size, np.uint32),
+               knn_off, knn_ids)
+    deg_all = np.where(valid, np.diff(off), 0)
+    off_all = np.zeros(xyzw.shape[0] + 1, np.int64)
+    np.cumsum(deg_all, out=off_all[1:])
+    ids_all = ids[sel].astype(np.int32) if sel.size else np.zeros(0, np.int32)
+    every = (xyzw, valid.astype(np.uint32), off_all, ids_all)
+    return compact, every
+
+
 @dataclass
 class BatchWorkload:
     """Independent RPD instances of uneven size and their sphere sets."""
