@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include <array>
+#include <functional>
 #include <map>
 #include <memory>
 #include <set>
@@ -233,6 +234,35 @@ struct PowerCellAdjacency {
 std::map<int, PowerCellAdjacency> update_power_cells_adjacency(
     std::vector<ConvexCellHost>& cells, int max_surf_fid,
     const std::map<aint3, aint3>* tet_es2fe_map = nullptr, int n_threads = 0);
+
+// ------------------------------------------------------------ export
+// The faces of one cell appended to a polygon soup, the reference's
+// get_one_convex_cell_faces_const (io_cuda.cxx:21-141): the cell's vertices
+// as binary32 points, each active plane's vertex cycle as a polygon (or
+// its triangle fan with is_triangle), the cell's sphere and the sphere
+// across each face (-1 for tet faces); with is_boundary_only only bisector
+// faces and tet faces with id < max_sf_fid.  Needs reload_active() first.
+void get_one_convex_cell_faces_const(
+    const ConvexCellHost& cc_trans, std::vector<cfloat3>& voro_points,
+    std::vector<std::vector<unsigned>>& one_voro_cell_faces,
+    std::vector<int>& voro_faces_sites, std::vector<int>& voro_faces_adj_sites,
+    bool is_triangle, int max_sf_fid, bool is_boundary_only);
+
+namespace libmat_rpd {
+// mean of a cell's binary32 vertices (compute_cell_barycenter,
+// io_cuda.cxx:6-17)
+std::array<float, 3> cell_barycenter(const ConvexCellHost& cell);
+// The cells' faces written as a version-5 binary Houdini geometry at
+// `path` (save_convex_cells_houdini, io_cuda.cxx:143-186, with the file
+// name given instead of derived from the run name and a time stamp):
+// points with P, polygons with the int primitive attribute "PrimAttr" =
+// the cell's sphere.  skip_cell(barycenter) == true leaves a cell out (the
+// reference's slice plane).  Reloads the cells' active elements.
+void save_convex_cells_bgeo(
+    const std::string& path, std::vector<ConvexCellHost>& cells,
+    int max_sf_fid, bool is_boundary_only,
+    const std::function<bool(const std::array<float, 3>&)>& skip_cell = {});
+}  // namespace libmat_rpd
 
 // Connected components of the elements of `to_visit` under `neighbors`
 // (edges to elements outside the set are ignored), in order of their least

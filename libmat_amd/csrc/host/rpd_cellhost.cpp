@@ -4,7 +4,12 @@
 // program that compiles LibMAT's own voronoi_defs.cxx links, defines none
 // of these methods a second time.
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <cstdint>
+#include <functional>
+#include <stdexcept>
+#include <string>
 #include <cstdio>
 #include <cstring>
 #include <iterator>
@@ -408,4 +413,192 @@ std::map<int, PowerCellAdjacency> update_power_cells_adjacency(
     });
   for (auto& th : pool) th.join();
   return pcs;
+}
+
+// ------------------------------------------------------------ export
+// The faces of one cell appended to a polygon soup (io_cuda.cxx:21-141):
+// every vertex of the cell becomes a point (binary32, from the cell's own
+// planes), every active plane a polygon walked around its vertices --
+// the next vertex holds the current one's plane (slot + 1) % 3 at its slot
+// (slot + 2) % 3 -- or the fan (0, p, p + 1) of triangles; each polygon
+// carries the cell's sphere and the sphere across it (-1 for a tet face).
+// With is_boundary_only, only bisector faces and tet faces with an id below
+// max_sf_fid are kept.  A cell with a non-finite vertex adds nothing.
+void get_one_convex_cell_faces_const(
+    const ConvexCellHost& c, std::vector<cfloat3>& points,
+    std::vector<std::vector<unsigned>>& faces, std::vector<int>& face_sites,
+    std::vector<int>& face_adj_sites, bool is_triangle, int max_sf_fid,
+    bool is_boundary_only) {
+  if (faces.size() != face_sites.size() ||
+      faces.size() != face_adj_sites.size())
+    throw std::runtime_error("faces and their site lists differ in length");
+  const size_t row = points.size();
+  std::vector<cfloat3> pts;
+  for (int v = 0; v < c.nb_v; ++v) {
+    const cuchar4 w = c.ver_trans_const(v);
+    const cfloat4 p = c.compute_vertex_coordinates(cuchar3{w.x, w.y, w.z});
+    if (std::isnan(p.x) || std::isnan(p.y) || std::isnan(p.z)) return;
+    pts.push_back(cfloat3{p.x, p.y, p.z});
+  }
+  points.insert(points.end(), pts.begin(), pts.end());
+  if (!c.is_active_updated)
+    throw std::runtime_error("get_one_convex_cell_faces_const: "
+                             "reload_active() first");
+  for (int plane = 0; plane < c.nb_p; ++plane) {
+    if (c.active_clipping_planes[plane] <= 0) continue;
+    const cint2 hp = c.clip_id2_const(plane);
+    if (is_boundary_only && !(hp.y != -1 || hp.x < max_sf_fid)) continue;
+    std::vector<int> tab_v, tab_lp;
+    for (int t = 0; t < c.nb_v; ++t) {
+      const cuchar4 w = c.ver_trans_const(t);
+      const int s = (int)w.x == plane ? 0
+                    : (int)w.y == plane ? 1
+                    : (int)w.z == plane ? 2 : -1;
+      if (s < 0) continue;
+      tab_v.push_back(t);
+      tab_lp.push_back(s);
+    }
+    std::vector<int> cyc;
+    std::vector<char> used(tab_v.size(), 0);
+    int i = 0;
+    while (cyc.size() < tab_v.size()) {
+      cyc.push_back(tab_v[i]);
+      used[i] = 1;
+      const int want = c.ith_plane_const(tab_v[i], (tab_lp[i] + 1) % 3);
+      int nxt = -1;
+      for (size_t j = 0; j < tab_v.size(); ++j)
+        if (!used[j] &&
+            c.ith_plane_const(tab_v[j], (tab_lp[j] + 2) % 3) == want) {
+          nxt = (int)j;
+          break;
+        }
+      if (nxt < 0) break;
+      i = nxt;
+    }
+    const int adj = hp.y != -1 ? (hp.x == c.voro_id ? hp.y : hp.x) : -1;
+    const int n = (int)cyc.size();
+    if (is_triangle) {
+      for (int p = 1; p + 1 < n; ++p) {
+        faces.push_back({(unsigned)(row + cyc[0]), (unsigned)(row + cyc[p]),
+                         (unsigned)(row + cyc[(p + 1) % n])});
+        face_sites.push_back(c.voro_id);
+        face_adj_sites.push_back(adj);
+      }
+    } else {
+      std::vector<unsigned> f(n);
+      for (int p = 0; p < n; ++p) f[p] = (unsigned)(row + cyc[p]);
+      faces.push_back(f);
+      face_sites.push_back(c.voro_id);
+      face_adj_sites.push_back(adj);
+    }
+  }
+}
+
+namespace {
+
+// big-endian writes of the bgeo stream
+struct BeOut {
+  std::FILE* f;
+  void u8(uint8_t x) { std::fputc(x, f); }
+  void u16(uint16_t x) {
+    u8((uint8_t)(x >> 8));
+    u8((uint8_t)x);
+  }
+  void u32(uint32_t x) {
+    u16((uint16_t)(x >> 16));
+    u16((uint16_t)x);
+  }
+  void f32(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    u32(b);
+  }
+  void str(const char* s) {
+    const size_t n = std::strlen(s);
+    u16((uint16_t)n);
+    std::fwrite(s, 1, n, f);
+  }
+};
+
+}  // namespace
+
+std::array<float, 3> libmat_rpd::cell_barycenter(const ConvexCellHost& c) {
+  // mean of the cell's vertices in binary32 (compute_cell_barycenter,
+  // io_cuda.cxx:6-17)
+  float x = 0, y = 0, z = 0;
+  for (int v = 0; v < c.nb_v; ++v) {
+    const cuchar4 w = c.ver_trans_const(v);
+    const cfloat4 p = c.compute_vertex_coordinates(cuchar3{w.x, w.y, w.z});
+    x = p.x + x;
+    y = p.y + y;
+    z = p.z + z;
+  }
+  return {x / c.nb_v, y / c.nb_v, z / c.nb_v};
+}
+
+void libmat_rpd::save_convex_cells_bgeo(
+    const std::string& path, std::vector<ConvexCellHost>& cells,
+    int max_sf_fid, bool is_boundary_only,
+    const std::function<bool(const std::array<float, 3>&)>& skip_cell) {
+  std::vector<cfloat3> points;
+  std::vector<std::vector<unsigned>> faces;
+  std::vector<int> sites, adj;
+  for (auto& c : cells) {
+    if (skip_cell && skip_cell(cell_barycenter(c))) continue;
+    if (!c.is_active_updated) c.reload_active();
+    get_one_convex_cell_faces_const(c, points, faces, sites, adj, false,
+                                    max_sf_fid, is_boundary_only);
+  }
+  if (points.empty())
+    throw std::runtime_error("save_convex_cells_bgeo: no points");
+  std::FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw std::runtime_error("save_convex_cells_bgeo: cannot open " +
+                                   path);
+  BeOut o{f};
+  // header of a version-5 binary geometry (io_utils.cpp:106-135): magic,
+  // 'V', version, points, primitives, point groups, primitive groups,
+  // point attributes beyond P, vertex attributes, primitive attributes,
+  // detail attributes
+  o.u32(0x4267656fu);  // "Bgeo"
+  o.u8('V');
+  o.u32(5);
+  o.u32((uint32_t)points.size());
+  o.u32((uint32_t)faces.size());
+  o.u32(0);
+  o.u32(0);
+  o.u32(0);
+  o.u32(0);
+  o.u32(1);
+  o.u32(0);
+  // points: x, y, z, w = 1
+  for (const cfloat3& p : points) {
+    o.f32(p.x);
+    o.f32(p.y);
+    o.f32(p.z);
+    o.f32(1.f);
+  }
+  // the primitive attribute "PrimAttr": one int (type 1), default 0
+  o.str("PrimAttr");
+  o.u16(1);
+  o.u32(1);
+  o.u32(0);
+  // polygons: type 1, vertex count, closed '<', indices (16 bit up to
+  // 65536 points, else 32 bit), then the attribute value
+  const bool wide = points.size() > (1u << 16);
+  for (size_t q = 0; q < faces.size(); ++q) {
+    o.u32(1);
+    o.u32((uint32_t)faces[q].size());
+    o.u8('<');
+    for (unsigned i : faces[q]) {
+      if (wide)
+        o.u32(i);
+      else
+        o.u16((uint16_t)i);
+    }
+    o.u32((uint32_t)sites[q]);
+  }
+  o.u8(0x00);
+  o.u8(0xff);
+  if (std::fclose(f) != 0)
+    throw std::runtime_error("save_convex_cells_bgeo: write failed");
 }
