@@ -668,6 +668,28 @@ def bench_cube(a, emit=True):
         if tj.get("kernel", "").replace("rpd::", "").replace(
                 "void ", "").startswith(clip_kernel):
             traffic = tj.get("bytes_per_launch")
+    # ---- the rows after the clip on the plain run's cells: per-sphere
+    # adjacency and components, per-site centroids (host wall time of the
+    # calls: device work plus their few count read-backs; nothing copied
+    # back to the host)
+    import ctypes
+    from libmat_amd.rpd import AdjSizes
+    ctx.run(**shape)
+    after = {}
+    for name, call in (
+            ("adjacency_ms", lambda: ctx._L.rpd_adjacency(
+                ctx._h, ctypes.byref(AdjSizes()))),
+            ("centroids_ms", lambda: ctx._L.rpd_site_centroids(
+                ctx._h, None, ctypes.byref(ctypes.c_void_p())))):
+        assert call() == 0
+        t1 = time.perf_counter()
+        for _ in range(3):
+            assert call() == 0
+        after[name] = (time.perf_counter() - t1) * 1e3 / 3
+    z = AdjSizes()
+    ctx._L.rpd_adjacency(ctx._h, ctypes.byref(z))
+    after.update({"facets": z.n_facets, "edges": z.n_edges,
+                  "cell_pairs": z.n_cell_pairs})
     # ---- learned path on the same resident instance (model trained on
     # other seeds of the same generator, untimed)
     learned = None
@@ -799,6 +821,7 @@ def bench_cube(a, emit=True):
                       "compaction": kn[6]},
         "host_prep_s": t_prep, "upload_s": t_upload,
         "neighbour_builder": neighbours,
+        "after_clip": after,
         "pcie_inclusive": {"value": n_cells / t_pcie, "unit": "cells/s",
                            "s": t_pcie, "note": "upload of host buffers + "
                            "one run + download of cells, words, geometry "
@@ -831,7 +854,7 @@ def headline(a):
             "value", "ms_per_step", "plain", "learned", "learned_vs_plain",
             "fallback_pct", "roofline_hbm", "roofline_fp64_valu", "n_cells",
             "n_candidates", "kernel_ms", "phase_ms", "neighbour_builder",
-            "pcie_inclusive")}
+            "after_clip", "pcie_inclusive")}
         out["cube64"]["config"] = c["config"]
         print("[headline] cube64 done", file=sys.stderr, flush=True)
     if not a.no_config3:

@@ -328,7 +328,10 @@ __global__ __launch_bounds__(256) void k_adj_pairs(int64_t n,
   const int32_t a = val[i], b = val[i + 1];
   nbr[a] = b >> 2;  // (cell a>>2, local face a&3) -> cell b>>2
   nbr[b] = a >> 2;
-  atomicAdd(n_pairs, 1);
+  // one atomic per wavefront: its first pairing lane adds the wave's count
+  const uint64_t m = __ballot(1);
+  if ((threadIdx.x & 63) == (uint32_t)(__ffsll((long long)m) - 1))
+    atomicAdd(n_pairs, (int)__popcll(m));
 }
 
 __device__ __forceinline__ int32_t uf_root(int32_t* par, int32_t x) {
@@ -466,22 +469,38 @@ __global__ __launch_bounds__(256) void k_adj_runs(
   }
 }
 
+// The cells' Euler values and component roots in seed order (cells sorted
+// by seed, ascending within a seed), so that the per-site loop below reads
+// contiguous memory
+__global__ __launch_bounds__(256) void k_adj_gather(int64_t n,
+                                                    const int32_t* idx,
+                                                    const int32_t* par,
+                                                    const float* euler,
+                                                    float* eul_sorted,
+                                                    uint8_t* root_sorted) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t c = idx[i];
+  eul_sorted[i] = euler[c];
+  root_sorted[i] = par[c] == c;
+}
+
 // One thread per site over its cells in ascending order: number of cells,
 // of components, and the Euler number euler_sum - num_cells of
-// is_to_fix_voro_euler (binary64 sum of the cells' binary32 values)
+// is_to_fix_voro_euler (binary64 sum of the cells' binary32 values, in
+// that order)
 __global__ __launch_bounds__(256) void k_adj_sites(
-    int32_t n_site, const int64_t* start, const int32_t* idx,
-    const int32_t* par, const float* euler, int32_t* site_cells,
-    int32_t* site_cc, double* site_euler) {
+    int32_t n_site, const int64_t* start, const float* eul_sorted,
+    const uint8_t* root_sorted, int32_t* site_cells, int32_t* site_cc,
+    double* site_euler) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_site) return;
   const int64_t b = start[s], e = start[s + 1];
   int32_t ncc = 0;
   double es = 0;
   for (int64_t i = b; i < e; ++i) {
-    const int32_t c = idx[i];
-    ncc += par[c] == c;
-    es += (double)euler[c];
+    ncc += root_sorted[i];
+    es += (double)eul_sorted[i];
   }
   site_cells[s] = (int32_t)(e - b);
   site_cc[s] = ncc;

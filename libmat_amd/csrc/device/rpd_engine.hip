@@ -260,7 +260,7 @@ struct rpd_ctx {
       fac_start, fac_runs, fac_reul, fac_rec;
   Buf edg_ka, edg_kb, edg_va, edg_vb, edg_par, edg_flag, edg_pos, edg_start,
       edg_runs, edg_rec;
-  Buf site_cells, site_cc, site_eul, adj_soff;
+  Buf site_cells, site_cc, site_eul, adj_soff, adj_eul_s, adj_root_s;
   rpd_adj_sizes adj{};
   bool adj_ready = false;
   RunCaps seen{};  // sizes of the last eager plain run
@@ -2134,11 +2134,19 @@ int rpd_adjacency(rpd_ctx* ctx, rpd_adj_sizes* sizes) {
   HIP_TRY(ctx->site_cells.need(ns1 * 4));
   HIP_TRY(ctx->site_cc.need(ns1 * 4));
   HIP_TRY(ctx->site_eul.need(ns1 * 8));
+  HIP_TRY(ctx->adj_eul_s.need(n1 * 4));
+  HIP_TRY(ctx->adj_root_s.need(n1));
+  if (n > 0)
+    hipLaunchKernelGGL(k_adj_gather, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                       n, ctx->cs_vals_b.as<int32_t>(), cpar,
+                       ctx->adj_euler.as<float>(),
+                       ctx->adj_eul_s.as<float>(),
+                       ctx->adj_root_s.as<uint8_t>());
   if (ns > 0)
     hipLaunchKernelGGL(k_adj_sites, dim3(ceil_div(ns, 256)), dim3(256), 0, s,
                        ns, ctx->cs_start.as<int64_t>(),
-                       ctx->cs_vals_b.as<int32_t>(), cpar,
-                       ctx->adj_euler.as<float>(),
+                       ctx->adj_eul_s.as<float>(),
+                       ctx->adj_root_s.as<uint8_t>(),
                        ctx->site_cells.as<int32_t>(),
                        ctx->site_cc.as<int32_t>(),
                        ctx->site_eul.as<double>());
