@@ -313,10 +313,11 @@ void power_neighbors(const std::vector<std::array<double, 4>>& xyzr,
 // each run sends only the spheres and, for a partial RPD, the ids of the
 // tets to clip (calculate_partial's partial tets, rpd_api.cxx:249-262,
 // clipped in place on the full mesh instead of sent as a partial mesh).
-// Owns its device context; one object is not for concurrent use.
+// Owns its device context (on `device`, or the calling thread's current
+// HIP device when negative); one object is not for concurrent use.
 class ResidentMesh {
  public:
-  explicit ResidentMesh(int device = 0);
+  explicit ResidentMesh(int device = -1);
   ~ResidentMesh();
   ResidentMesh(const ResidentMesh&) = delete;
   ResidentMesh& operator=(const ResidentMesh&) = delete;
@@ -432,7 +433,7 @@ class RPD3D_GPU_T {
   int num_itr_rpd = 0;
   int n_site = -1;
   int site_k = -1;
-  int device = 0;  // GPU of the resident mesh
+  int device = -1;  // GPU of the resident mesh (-1: the current device)
 
   int max_surf_fid = -1;  // largest surface facet id (-1: from f_adjs)
 

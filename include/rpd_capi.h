@@ -165,6 +165,9 @@ typedef struct {
 typedef struct rpd_ctx rpd_ctx;
 
 int rpd_ctx_create(int32_t device, rpd_ctx** out);
+/* The HIP device current on the calling thread (hipGetDevice), -1 if none:
+ * the device the drop-in's compute_clipped_voro_diagram_GPU runs on. */
+int32_t rpd_current_device(void);
 void rpd_ctx_destroy(rpd_ctx* ctx);
 const char* rpd_last_error(const rpd_ctx* ctx);
 /* hipStream_t of the context (as void*), for event timing by callers. */

@@ -465,6 +465,11 @@ void rpd_ctx_destroy(rpd_ctx* ctx) {
   delete ctx;
 }
 
+int32_t rpd_current_device(void) {
+  int d = -1;
+  return hipGetDevice(&d) == hipSuccess ? (int32_t)d : -1;
+}
+
 const char* rpd_last_error(const rpd_ctx* ctx) {
   return ctx ? ctx->err.c_str() : "null context";
 }

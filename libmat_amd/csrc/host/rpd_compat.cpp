@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstring>
 #include <iterator>
+#include <map>
 #include <mutex>
 #include <thread>
 
@@ -17,18 +18,22 @@ static_assert(alignof(ConvexCellHost) == 32, "ConvexCellHost alignment");
 namespace {
 
 std::mutex g_mu;
-rpd_ctx* g_ctx = nullptr;
+// one context per device, for the device current on the calling thread
+std::map<int, rpd_ctx*> g_ctx;
 
 rpd_ctx* shared_ctx() {
-  if (!g_ctx) {
-    int rc = rpd_ctx_create(0, &g_ctx);
+  const int dev = rpd_current_device();
+  if (dev < 0) throw std::runtime_error("RPD: no usable GPU");
+  rpd_ctx*& c = g_ctx[dev];
+  if (!c) {
+    int rc = rpd_ctx_create(dev, &c);
     if (rc != 0) {
-      g_ctx = nullptr;
+      c = nullptr;
       throw std::runtime_error("RPD: no usable GPU (rpd_ctx_create " +
                                std::to_string(rc) + ")");
     }
   }
-  return g_ctx;
+  return c;
 }
 
 void check(rpd_ctx* ctx, int rc, const char* what) {
@@ -353,7 +358,8 @@ void libmat_rpd::power_neighbors(
 // ------------------------------------------------------- resident mesh
 namespace libmat_rpd {
 
-ResidentMesh::ResidentMesh(int device) : device_(device) {}
+ResidentMesh::ResidentMesh(int device)
+    : device_(device >= 0 ? device : rpd_current_device()) {}
 
 ResidentMesh::~ResidentMesh() {
   if (ctx_) rpd_ctx_destroy(ctx_);

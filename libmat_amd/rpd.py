@@ -197,7 +197,7 @@ CAPI_SYMBOLS = ("rpd_ctx_create", "rpd_ctx_destroy", "rpd_last_error",
                 "rpd_power_neighbors_gpu", "rpd_run_noalloc",
                 "rpd_site_centroids", "rpd_set_tet_subset", "rpd_adjacency",
                 "rpd_adjacency_download", "rpd_graph_build", "rpd_graph_run",
-                "rpd_graph_destroy")
+                "rpd_graph_destroy", "rpd_current_device")
 
 
 def _p(a):

@@ -63,6 +63,7 @@ int rpd_update_sites(rpd_ctx*, int32_t, const double*, const uint32_t*,
   return -3;
 }
 int rpd_set_tet_subset(rpd_ctx*, int32_t, const int32_t*) { return -3; }
+int32_t rpd_current_device(void) { return -1; }
 }
 
 // mirror of the oracle's CellOut
