@@ -249,6 +249,7 @@ def time_batch(ctx, sets, warmup, **kw):
         "clip_first_round": avg(lambda k: k["ms_kernel"][5]),
         "clip_small_tier_all_rounds": avg(lambda k: k["ms_clip_small_sum"]),
         "clip_small_tier_launches": avg(lambda k: k["n_clip_small"]),
+        "host_reads": avg(lambda k: k["n_host_reads"]),
         "capacity_and_exact_reruns": avg(lambda k: k["ms_phase"][2]),
         "compaction": avg(lambda k: k["ms_phase"][3])})
     if kw.get("learned"):
@@ -735,6 +736,7 @@ def bench_cube(a, emit=True):
             "added_candidates": lst["learned"][2],
             "model": model, "logit_threshold": thr, "train_loss": loss,
             "train_s": t_train, "train_prep_s": t_learn_prep,
+            "host_reads": lst["n_host_reads"],
             "phase_ms": {"candidates": lp[0], "clip": lp[1], "rerun": lp[2],
                          "compaction": lp[3], "certificate": lp[4]},
             "kernel_ms": {"features": lk[0], "mlp": lk[1],
@@ -775,7 +777,8 @@ def bench_cube(a, emit=True):
     del res
     if a.calib_copy_gib > 0:
         ctx.calib_copy(int(a.calib_copy_gib * (1 << 30)))
-    plain = {"value": value, "ms_per_step": ms_step}
+    plain = {"value": value, "ms_per_step": ms_step,
+             "host_reads": st["n_host_reads"]}
     if learned is not None:
         value, ms_step = learned["value"], learned["ms_per_step"]
     out = {

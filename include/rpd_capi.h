@@ -155,7 +155,9 @@ typedef struct {
                                  plain path has one, the learned path one
                                  per round of its certificate)            */
   int32_t n_clip_small;       /* number of those launches                */
-  int32_t reserved0;
+  int32_t n_host_reads;       /* host waits on the device during the run
+                                 (count read-backs that size later work,
+                                 and the final statistics)               */
   double ms_features_kernel;  /* learned runs: device time of the
                                  proposal/feature kernel alone            */
   int64_t scratch_bytes;      /* rpd_run_noalloc: scratch bytes the call

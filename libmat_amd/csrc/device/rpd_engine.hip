@@ -304,6 +304,7 @@ struct rpd_ctx {
   size_t pin_inst_cap = 0;
   hipEvent_t ev_inst = nullptr;
   bool inst_copy_recorded = false;
+  int32_t n_reads = 0;  // read_back waits since the run began
   // this device's copy of the no-alloc scratch (nullptr if its address
   // could not be taken)
   void* scratch_dev = nullptr;
@@ -358,6 +359,7 @@ struct D2H {
 hipError_t read_back(rpd_ctx* ctx, hipStream_t s,
                      std::initializer_list<D2H> reads) {
   if (t_capture) return hipErrorStreamCaptureUnsupported;
+  ++ctx->n_reads;
   size_t off = 0;
   for (const D2H& r : reads) {
     if (off + r.n > kPinBytes) return hipErrorInvalidValue;
@@ -1734,6 +1736,7 @@ int run_stages(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out,
   hipStream_t s = R.s;
   if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[0], s));
   ctx->n_small_ev = 0;
+  ctx->n_reads = 0;
   HIP_TRY(zero32(s, ctx->counters.p, 8));
   if (int rc = init_state(ctx, R)) return rc;
   if (cfg.count_work) {
@@ -1851,6 +1854,7 @@ int run_stats(rpd_ctx* ctx, RunState& R, int64_t n_cand, int64_t n_out,
   st.tet_k = (int32_t)st64[2];
   st.ms_features_kernel = cfg.learned ? el(14, 15) : 0.0;
   st.n_clip_small = ctx->n_small_ev;
+  st.n_host_reads = ctx->n_reads;
   st.ms_clip_small_sum = 0;
   for (int k = 0; k < ctx->n_small_ev; ++k)
     st.ms_clip_small_sum += el(16 + 2 * k, 17 + 2 * k);
@@ -2392,6 +2396,7 @@ int rpd_graph_run(rpd_ctx* ctx, int32_t* replayed) {
   st.n_bbox_fallback = cnt[0];
   st.n_bin_registrations = n_reg;
   st.tet_k = (int32_t)st64[2];
+  st.n_host_reads = 1;  // the counts, once, after the graph
   float ms = 0;
   st.ms_total =
       hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[4]) == hipSuccess ? ms : -1;

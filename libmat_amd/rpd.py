@@ -88,7 +88,7 @@ class Stats(ctypes.Structure):
                 ("learned", ctypes.c_int64 * 4),
                 ("ms_clip_small_sum", ctypes.c_double),
                 ("n_clip_small", ctypes.c_int32),
-                ("reserved0", ctypes.c_int32),
+                ("n_host_reads", ctypes.c_int32),
                 ("ms_features_kernel", ctypes.c_double),
                 ("scratch_bytes", ctypes.c_int64)]
 
