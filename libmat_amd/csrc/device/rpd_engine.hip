@@ -16,6 +16,7 @@
 #include <initializer_list>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "rpd_capi.h"
@@ -1095,7 +1096,11 @@ int learned_features(rpd_ctx* ctx, RunState& R) {
                      ctx->sg_ids.as<int32_t>(), ctx->vowner.as<int32_t>());
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->ev[14], s));
-  hipLaunchKernelGGL(k_knn_features, dim3(ceil_div(nt, 64)), dim3(64), 0, s,
+  // 32-bit proposal entries hold instance-local ids of 16 bits
+  bool wide = false;
+  for (const rpd_ctx::Inst& I : ctx->inst) wide = wide || I.n_site > 65536;
+  auto* feat_kernel = wide ? k_knn_features_wide : k_knn_features;
+  hipLaunchKernelGGL(feat_kernel, dim3(ceil_div(nt, 64)), dim3(64), 0, s,
                      R.M, R.S, R.inst_d, R.tet_inst_d,
                      ctx->vowner.as<int32_t>(), ctx->knn.as<int32_t>(),
                      ctx->feat.as<uint16_t>(), ctx->omask.as<uint32_t>());

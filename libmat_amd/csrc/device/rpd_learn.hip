@@ -192,6 +192,34 @@ __global__ __launch_bounds__(256) void k_vertex_owner(
   owner[v] = st < 0 ? -1 : descend(S, c, st, &best);
 }
 
+// Proposal list entries: (key, id) packed into one unsigned word so that
+// integer order is (key, id) order.  The key keeps the top 16 bits of its
+// non-negative binary32 pattern (8 exponent and 8 mantissa bits, truncated;
+// owners 0, others at least 0x100, finite keys at most 0xff00); the id is
+// instance-local in 16 bits (instances of at most 65,536 sites), else
+// batch-wide in 32 bits of a 64-bit entry (same order: local = batch-wide
+// id - the instance's first site).  ~0 is the empty slot.
+template <bool kWide>
+struct PropEntry {
+  using T = typename std::conditional<kWide, uint64_t, uint32_t>::type;
+  static constexpr int kShift = kWide ? 32 : 16;
+  static constexpr T kEmpty = ~T(0);
+  __device__ __forceinline__ static uint32_t key16(float key) {
+    return __float_as_uint(key) >> 15;
+  }
+  __device__ __forceinline__ static T pack(uint32_t k16, int32_t id,
+                                           int32_t site_off) {
+    return kWide ? (T)(((uint64_t)k16 << 32) | (uint32_t)id)
+                 : (T)((k16 << 16) | (uint32_t)(id - site_off));
+  }
+  __device__ __forceinline__ static int32_t id(T e, int32_t site_off) {
+    return kWide ? (int32_t)(uint32_t)e : (int32_t)(e & 0xffffu) + site_off;
+  }
+  __device__ __forceinline__ static float key(T e) {
+    return __uint_as_float((uint32_t)(e >> kShift) << 15);
+  }
+};
+
 // One thread per tet.  The owners of its centroid and of its 4 corners
 // belong to its sphere set; the proposals are the union of their 1-rings
 // (with them), ranked by key(s) = min over the 5 points x of
@@ -200,12 +228,16 @@ __global__ __launch_bounds__(256) void k_vertex_owner(
 // proposals (the certificate makes the output exact whatever they are), so
 // they are evaluated in binary32 around the centroid c: with u = s - c and
 // the corner offsets d_x = x - c, pd(x, s) - pd_min(x) =
-// (|u|^2 - w) - 2 d_x.u + (|d_x|^2 - pd_min(x)).
-// (compiled for 4 resident wavefronts per SIMD: measured faster than the
-// 3 its register count otherwise allows)
-__global__ __launch_bounds__(64, 4) void k_knn_features(
-    DevMesh M, DevSites S, const InstInfo* inst, const int32_t* tet_inst,
-    const int32_t* vowner, int32_t* knn, uint16_t* X, uint32_t* omask) {
+// (|u|^2 - w) - 2 d_x.u + (|d_x|^2 - pd_min(x)), and kept to 16 bits
+// (PropEntry), so that a 32-entry sorted list of 32-bit words costs one
+// min and one max per slot and insertion.
+template <bool kWide>
+__device__ __forceinline__ void knn_features(
+    const DevMesh& M, const DevSites& S, const InstInfo* inst,
+    const int32_t* tet_inst, const int32_t* vowner, int32_t* knn, uint16_t* X,
+    uint32_t* omask) {
+  using PE = PropEntry<kWide>;
+  using E = typename PE::T;
   const int t = blockIdx.x * 64 + threadIdx.x;
   if (t >= M.n_tet) return;
   TetLoad L;
@@ -222,6 +254,7 @@ __global__ __launch_bounds__(64, 4) void k_knn_features(
                              D[k][2] * D[k][2]);
   }
   const InstInfo& I = inst[tet_inst[t]];
+  const int32_t soff = I.site_off;
   uint16_t* x = X + (int64_t)t * kIn;
   int32_t* kt = knn + (int64_t)t * kOut;
   int own[5];
@@ -229,35 +262,33 @@ __global__ __launch_bounds__(64, 4) void k_knn_features(
   own[2] = vowner[L.tv.y];
   own[3] = vowner[L.tv.z];
   own[4] = vowner[L.tv.w];
-  // the 32 least (key, id), sorted, in registers, as one u64 each: the key's
-  // bits (keys are >= 0, so their bit patterns order as the keys do) above
-  // the id.  An id already listed is dropped; otherwise the entry bubbles
-  // through the list by 64-bit min / max (fully unrolled, no divergence).
-  constexpr uint64_t kEmpty = (0x7f800000ull << 32) | 0x7fffffffull;
-  uint64_t lst[kOut];
+  // the 32 least entries, sorted, in registers.  A listed sphere comes back
+  // with the same key (same operands, same operations), so an entry equal
+  // to a listed one is a repeat and is dropped (made empty).  Inserting x
+  // into the sorted a[0..31]: slot q becomes the median of a[q-1], x and
+  // a[q] (a[q] if x >= a[q], x if a[q-1] <= x < a[q], else a[q-1]), slot 0
+  // min(x, a[0]) -- each slot from old values only, so the slots update in
+  // place from the last down, fully unrolled and without branches (an
+  // empty or too large x leaves the list as it is).
+  E lst[kOut];
 #pragma unroll
-  for (int q = 0; q < kOut; ++q) lst[q] = kEmpty;
-  // chk: the id may be listed already (entries of the second and later
+  for (int q = 0; q < kOut; ++q) lst[q] = PE::kEmpty;
+  // chk: the sphere may be listed already (entries of the second and later
   // rings; the owners and the first ring hold distinct ids)
-  auto ins = [&](float key, int32_t id, bool chk) {
-    uint64_t x = ((uint64_t)__float_as_uint(key) << 32) | (uint32_t)id;
-    if (!(x < lst[kOut - 1])) return;
+  auto ins = [&](E x, bool chk) {
     if (chk) {
       bool dup = false;
 #pragma unroll
-      for (int q = 0; q < kOut; ++q) dup |= (uint32_t)lst[q] == (uint32_t)id;
-      if (dup) return;
+      for (int q = 0; q < kOut; ++q) dup |= lst[q] == x;
+      x = dup ? PE::kEmpty : x;
     }
 #pragma unroll
-    for (int q = 0; q < kOut; ++q) {
-      const uint64_t y = lst[q];
-      const bool lt = x < y;
-      lst[q] = lt ? x : y;
-      x = lt ? y : x;
+    for (int q = kOut - 1; q > 0; --q) {
+      const E m = x < lst[q] ? x : lst[q];
+      lst[q] = lst[q - 1] < m ? m : lst[q - 1];
     }
+    lst[0] = x < lst[0] ? x : lst[0];
   };
-  auto key_of = [&](int q) { return __uint_as_float((uint32_t)(lst[q] >> 32)); };
-  auto id_of = [&](int q) { return (int32_t)(uint32_t)lst[q]; };
   uint32_t om = 0;
   if (own[1] >= 0) {
     // centroid owner: descend from the corner owner closest to it
@@ -293,7 +324,7 @@ __global__ __launch_bounds__(64, 4) void k_knn_features(
       for (int q = 0; q < j; ++q) seen = seen || own[q] == o;
       if (seen) continue;
       if (S.flags[o] == 1u) {
-        ins(0.f, o, false);
+        ins(PE::pack(0u, o, soff), false);
         ++n_own;
       }
       if (j > 0 && ((in_ring >> j) & 1u)) continue;
@@ -338,7 +369,8 @@ __global__ __launch_bounds__(64, 4) void k_knn_features(
             key = fminf(key, (base + qx[1 + k]) - 2.f * du);
           }
           // non-owners rank after the owners (key 0) whatever the rounding
-          ins(fmaxf(key, 1.17549435e-38f), id, j > 0);
+          ins(PE::pack(PE::key16(fmaxf(key, 1.17549435e-38f)), id, soff),
+              j > 0);
         }
       }
     }
@@ -347,15 +379,13 @@ __global__ __launch_bounds__(64, 4) void k_knn_features(
     om = n_own >= 32 ? ~0u : (1u << n_own) - 1u;
   }
   omask[t] = om;
+  auto id_at = [&](int q) {
+    return lst[q] != PE::kEmpty ? PE::id(lst[q], soff) : -1;
+  };
 #pragma unroll
-  for (int k = 0; k < kOut; k += 4) {  // 16-byte stores of the row
-    int4 w;
-    w.x = lst[k] != kEmpty ? id_of(k) : -1;
-    w.y = lst[k + 1] != kEmpty ? id_of(k + 1) : -1;
-    w.z = lst[k + 2] != kEmpty ? id_of(k + 2) : -1;
-    w.w = lst[k + 3] != kEmpty ? id_of(k + 3) : -1;
-    *reinterpret_cast<int4*>(kt + k) = w;
-  }
+  for (int k = 0; k < kOut; k += 4)  // 16-byte stores of the row
+    *reinterpret_cast<int4*>(kt + k) =
+        make_int4(id_at(k), id_at(k + 1), id_at(k + 2), id_at(k + 3));
   const double ext =
       fmax(fmax(I.dom[3] - I.dom[0], I.dom[4] - I.dom[1]), I.dom[5] - I.dom[2]);
   const double inv_ext = ext > 0 ? 1.0 / ext : 0.0;
@@ -368,13 +398,13 @@ __global__ __launch_bounds__(64, 4) void k_knn_features(
   f[2] = (float)((C[2] - I.dom[2]) * inv_ext);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    if (lst[k] != kEmpty) {
-      const double4 sp = S.site[id_of(k)];
+    if (lst[k] != PE::kEmpty) {
+      const double4 sp = S.site[PE::id(lst[k], soff)];
       f[3 + 5 * k + 0] = (float)(sp.x - C[0]) * inv_r;
       f[3 + 5 * k + 1] = (float)(sp.y - C[1]) * inv_r;
       f[3 + 5 * k + 2] = (float)(sp.z - C[2]) * inv_r;
       f[3 + 5 * k + 3] = (float)sqrt(sp.w) * inv_r;
-      f[3 + 5 * k + 4] = key_of(k) * (inv_r * inv_r);
+      f[3 + 5 * k + 4] = PE::key(lst[k]) * (inv_r * inv_r);
     }
   }
 #pragma unroll
@@ -386,6 +416,19 @@ __global__ __launch_bounds__(64, 4) void k_knn_features(
     v.w = (uint32_t)f2bf(f[q + 6]) | ((uint32_t)f2bf(f[q + 7]) << 16);
     *reinterpret_cast<uint4*>(x + q) = v;
   }
+}
+// 16-bit instance-local ids: compiled for 5 resident wavefronts per SIMD
+// (its few spills sit outside the ring loop); the 64-bit entries of larger
+// instances for 4
+__global__ __launch_bounds__(64, 5) void k_knn_features(
+    DevMesh M, DevSites S, const InstInfo* inst, const int32_t* tet_inst,
+    const int32_t* vowner, int32_t* knn, uint16_t* X, uint32_t* omask) {
+  knn_features<false>(M, S, inst, tet_inst, vowner, knn, X, omask);
+}
+__global__ __launch_bounds__(64, 4) void k_knn_features_wide(
+    DevMesh M, DevSites S, const InstInfo* inst, const int32_t* tet_inst,
+    const int32_t* vowner, int32_t* knn, uint16_t* X, uint32_t* omask) {
+  knn_features<true>(M, S, inst, tet_inst, vowner, knn, X, omask);
 }
 
 // Labels from a plain run's output cells (sorted by tet, site): proposal k

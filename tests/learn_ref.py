@@ -8,10 +8,12 @@ of each corner owner's list when that owner is not in the centroid owner's
 list; a non-owner s has key = min over the five points x of
 pd(x, s) - pd(x, owner(x)), evaluated in binary32 around the centroid
 exactly as the kernel does (same operations, same order, no contraction),
-clamped to the least normal float; the 32 least (key, id) are kept.
-Features (bf16): centroid in domain coordinates, then for the first 8
-proposals the centre offset and radius over the tet radius and the key over
-its square, zero padding to 48.
+clamped to the least normal float and truncated to the top 16 bits of its
+binary32 pattern (8 exponent and 8 mantissa bits); the 32 least
+(truncated key, id) are kept.  Features (bf16): centroid in domain
+coordinates, then for the first 8 proposals the centre offset and radius
+over the tet radius and the truncated key over its square, zero padding to
+48.
 """
 import numpy as np
 
@@ -27,6 +29,15 @@ def _pd(c, s):
 def bf16_bits(x):
     u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
     return (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16)
+
+
+def key16(key):
+    """the ranking key kept by the kernel: binary32 bits 30..15"""
+    return int(np.asarray(key, np.float32).view(np.uint32)) >> 15
+
+
+def key_of16(k16):
+    return np.array([k16 << 15], np.uint32).view(np.float32)[0]
 
 
 def proposals_and_features(inst, tets):
@@ -79,6 +90,7 @@ def proposals_and_features(inst, tets):
                     du = (D[k, 0] * u[0] + D[k, 1] * u[1]) + D[k, 2] * u[2]
                     key = min(key, (base + qx[1 + k]) - F(2) * du)
                 pool[s] = max(key, FLT_MIN)
+        pool = {s: key16(k) for s, k in pool.items()}
         order = sorted(pool.items(), key=lambda kv: (kv[1], kv[0]))[:32]
         for q, (s, _) in enumerate(order):
             knn[row, q] = s
@@ -91,6 +103,6 @@ def proposals_and_features(inst, tets):
             f[3 + 5 * k + 1] = F(sp[1] - C[1]) * inv_r
             f[3 + 5 * k + 2] = F(sp[2] - C[2]) * inv_r
             f[3 + 5 * k + 3] = F(np.sqrt(sp[3])) * inv_r
-            f[3 + 5 * k + 4] = key * (inv_r * inv_r)
+            f[3 + 5 * k + 4] = key_of16(key) * (inv_r * inv_r)
         X[row] = bf16_bits(f)
     return knn, X

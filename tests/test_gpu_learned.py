@@ -94,14 +94,24 @@ def test_mlp_inference_matches_oracle_forward(ctx):
     assert ((z_gpu > thr) == (ctx.scores() > thr)).all()
 
 
-def test_features_match_restatement(ctx):
+@pytest.mark.parametrize("case", ["cube10", "many_sites"])
+def test_features_match_restatement(ctx, case):
     # proposals (ids and order) and the bf16 feature rows of k_knn_features
-    # against the numpy restatement of the same binary32 arithmetic
+    # against the numpy restatement of the same binary32 arithmetic; an
+    # instance of more than 65,536 sites takes the kernel's 64-bit entries
     from learn_ref import proposals_and_features
-    inst = prep.cube10_instance(3)
+    if case == "cube10":
+        inst, n_sample = prep.cube10_instance(3), 800
+    else:
+        verts, tets = prep.grid_mesh(6, 0.1, 2)
+        inst = prep.build_instance(verts, tets,
+                                   prep.spheres(90000, 0, 4, 0.001, 0.002))
+        n_sample = 150
+        assert inst.n_site > 65536
+    print(f"[features] {case}: n_site={inst.n_site}", flush=True)
     ctx.upload(inst)
     knn, X = ctx.features()
-    sample = np.sort(np.random.default_rng(3).choice(inst.n_tet, 800,
+    sample = np.sort(np.random.default_rng(3).choice(inst.n_tet, n_sample,
                                                      replace=False))
     knn_r, X_r = proposals_and_features(inst, sample)
     bad_ids = np.nonzero((knn[sample] != knn_r).any(axis=1))[0]
