@@ -1823,23 +1823,69 @@ __global__ __launch_bounds__(kCB) void k_compact(
     }
     return lo;
   };
+  // Each wavefront copies a contiguous quarter of the block's elements, 64
+  // consecutive ones per step (coalesced stores); a lane finds the cell of
+  // its first element by binary search and then walks forward, past about
+  // 64 / (elements per cell) cells a step.  kU steps at a time: their
+  // sources first, then their loads together, then their stores.
+  constexpr int kU = 4;
+  const int wv = i >> 6, ln = i & 63;
+  auto range = [&](uint32_t n, uint32_t* q0, uint32_t* q1) {
+    *q0 = (uint32_t)((uint64_t)n * wv / (kCB / 64));
+    *q1 = (uint32_t)((uint64_t)n * (wv + 1) / (kCB / 64));
+  };
   const uint32_t W = s_w[kCB];
   uint32_t* wd = out_words + B.words;
-  for (uint32_t e = i; e < W; e += kCB) {
-    const int k = owner(s_w, e);
-    const uint32_t w = e - s_w[k], lay = s_lay[k];
-    const uint32_t nv = lay & 255, np = (lay >> 8) & 255;
-    const uint32_t src = w < nv ? w
-                         : (w < nv + np ? (lay >> 16) + (w - nv)
-                                        : s_oe[k] + (w - nv - np));
-    wd[e] = s_wsrc[k][src];
+  {
+    uint32_t q0, q1;
+    range(W, &q0, &q1);
+    int k = q0 + ln < q1 ? owner(s_w, q0 + ln) : 0;
+    for (uint32_t e0 = q0 + ln; e0 < q1; e0 += 64 * kU) {
+      const uint32_t* src[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const uint32_t e = e0 + 64 * u;
+        src[u] = nullptr;
+        if (e < q1) {
+          while (s_w[k + 1] <= e) ++k;
+          const uint32_t w = e - s_w[k], lay = s_lay[k];
+          const uint32_t nv = lay & 255, np = (lay >> 8) & 255;
+          src[u] = s_wsrc[k] + (w < nv ? w
+                                : (w < nv + np ? (lay >> 16) + (w - nv)
+                                               : s_oe[k] + (w - nv - np)));
+        }
+      }
+      uint32_t v[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) v[u] = src[u] ? *src[u] : 0u;
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (src[u]) wd[e0 + 64 * u] = v[u];
+    }
   }
   if (geometry) {
     const uint32_t A = s_a[kCB];
     double* ad = out_areas + B.areas;
-    for (uint32_t e = i; e < A; e += kCB) {
-      const int k = owner(s_a, e);
-      ad[e] = s_asrc[k][e - s_a[k]];
+    uint32_t q0, q1;
+    range(A, &q0, &q1);
+    int k = q0 + ln < q1 ? owner(s_a, q0 + ln) : 0;
+    for (uint32_t e0 = q0 + ln; e0 < q1; e0 += 64 * kU) {
+      const double* src[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const uint32_t e = e0 + 64 * u;
+        src[u] = nullptr;
+        if (e < q1) {
+          while (s_a[k + 1] <= e) ++k;
+          src[u] = s_asrc[k] + (e - s_a[k]);
+        }
+      }
+      double v[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) v[u] = src[u] ? *src[u] : 0.0;
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (src[u]) ad[e0 + 64 * u] = v[u];
     }
   }
 }

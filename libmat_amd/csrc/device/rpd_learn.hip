@@ -67,14 +67,18 @@ struct MlpDev {
   const float* b3;
 };
 
+// four consecutive bf16 (8-byte aligned) as an MFMA operand
 __device__ __forceinline__ bf16x4 ld_bf4(const uint16_t* p) {
-  const uint2 v = *reinterpret_cast<const uint2*>(p);
-  bf16x4 r;
-  r[0] = (short)(v.x & 0xffffu);
-  r[1] = (short)(v.x >> 16);
-  r[2] = (short)(v.y & 0xffffu);
-  r[3] = (short)(v.y >> 16);
-  return r;
+  return *reinterpret_cast<const bf16x4*>(p);
+}
+// two binary32 -> two bfloat16 in one word (v_cvt_pk_bf16_f32: round to
+// nearest even, as f2bf for finite inputs)
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t f2bf2(float a, float b) {
+  const f32x2 v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
 }
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
@@ -282,6 +286,7 @@ __device__ __forceinline__ void knn_features(
       for (int q = 0; q < kOut; ++q) dup |= lst[q] == x;
       x = dup ? PE::kEmpty : x;
     }
+    if (!(x < lst[kOut - 1])) return;  // past the last slot (or empty)
 #pragma unroll
     for (int q = kOut - 1; q > 0; --q) {
       const E m = x < lst[q] ? x : lst[q];
@@ -541,10 +546,12 @@ __device__ __forceinline__ void mlp_forward_tile(const MlpDev& P,
 
 // Inference: selection mask per row (bit k: logit_k > thr); optional
 // logits.  Eight waves share the weights, staged once per block in LDS
-// (rows padded against bank conflicts); each wave runs 16-row tiles with
-// one activation tile in LDS.
+// (rows padded against bank conflicts); each wave runs kInferTiles 16-row
+// tiles at a time, so that every weight fragment read from LDS feeds
+// kInferTiles independent MFMA chains.
 constexpr int kS1 = kIn + 8, kS2 = kHid + 8;
 constexpr int kInferWaves = 8;
+constexpr int kInferTiles = 2;
 
 // The layers run transposed (Z^T = W^T X^T: weights as the A operand,
 // activations as the B operand), so a layer's output fragment in a lane --
@@ -554,6 +561,7 @@ constexpr int kInferWaves = 8;
 __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(
     MlpDev P, const uint16_t* X, int64_t n, float thr, uint32_t* sel,
     float* logits) {
+  constexpr int T = kInferTiles;
   __shared__ uint16_t s_w1[kHid * kS1];
   __shared__ uint16_t s_w2[kHid * kS2];
   __shared__ uint16_t s_w3[kOut * kS2];
@@ -567,68 +575,95 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int col = l % 16, rq = 4 * (l / 16);
   const int64_t n_tiles = (n + 15) / 16;
-  for (int64_t tile = (int64_t)blockIdx.x * kInferWaves + wv; tile < n_tiles;
-       tile += (int64_t)gridDim.x * kInferWaves) {
-    const int64_t row0 = tile * 16;
-    const int64_t my_row = row0 + col;
-    bf16x4 x[3];
-    load_x(X, my_row < n ? my_row : -1, l, x);
-    // lane l: features 16 nb + rq + i of tet row0 + col
-    auto act = [](f32x4 c, const float* b) {
-      bf16x4 h;
+  // lane l: features 16 nb + rq + i of tet row0 + col
+  auto act = [](f32x4 c, const float* b) {
+    const u32x2 w = {f2bf2(fmaxf(c[0] + b[0], 0.f), fmaxf(c[1] + b[1], 0.f)),
+                     f2bf2(fmaxf(c[2] + b[2], 0.f), fmaxf(c[3] + b[3], 0.f))};
+    return __builtin_bit_cast(bf16x4, w);
+  };
+  for (int64_t tile0 = ((int64_t)blockIdx.x * kInferWaves + wv) * T;
+       tile0 < n_tiles; tile0 += (int64_t)gridDim.x * kInferWaves * T) {
+    // the biases are read where they are used (L1 hits): an empty asm
+    // statement hides their addresses' invariance, so the compiler does not
+    // keep 72 of them in registers across the loop (it spilled them)
+    const float *b1 = P.b1, *b2 = P.b2, *b3 = P.b3;
+    asm volatile("" : "+s"(b1), "+s"(b2), "+s"(b3));
+    int64_t my_row[T];
+    bf16x4 x[T][3];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) h[i] = (short)f2bf(fmaxf(c[i] + b[i], 0.f));
-      return h;
-    };
-    bf16x4 h1[8];
+    for (int t = 0; t < T; ++t) {
+      my_row[t] = (tile0 + t) * 16 + col;
+      load_x(X, my_row[t] < n ? my_row[t] : -1, l, x[t]);
+    }
+    bf16x4 h1[T][8];
 #pragma unroll
     for (int nb = 0; nb < 8; ++nb) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      f32x4 acc[T];
 #pragma unroll
-      for (int kb = 0; kb < 3; ++kb)
-        acc = mfma(ld_bf4(&s_w1[(16 * nb + col) * kS1 + 16 * kb + rq]), x[kb],
-                   acc);
-      h1[nb] = act(acc, P.b1 + 16 * nb + rq);
+      for (int t = 0; t < T; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 3; ++kb) {
+        const bf16x4 w = ld_bf4(&s_w1[(16 * nb + col) * kS1 + 16 * kb + rq]);
+#pragma unroll
+        for (int t = 0; t < T; ++t) acc[t] = mfma(w, x[t][kb], acc[t]);
+      }
+#pragma unroll
+      for (int t = 0; t < T; ++t) h1[t][nb] = act(acc[t], b1 + 16 * nb + rq);
     }
-    bf16x4 h2[8];
+    bf16x4 h2[T][8];
 #pragma unroll
     for (int nb = 0; nb < 8; ++nb) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      f32x4 acc[T];
 #pragma unroll
-      for (int kb = 0; kb < 8; ++kb)
-        acc = mfma(ld_bf4(&s_w2[(16 * nb + col) * kS2 + 16 * kb + rq]), h1[kb],
-                   acc);
-      h2[nb] = act(acc, P.b2 + 16 * nb + rq);
+      for (int t = 0; t < T; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 8; ++kb) {
+        const bf16x4 w = ld_bf4(&s_w2[(16 * nb + col) * kS2 + 16 * kb + rq]);
+#pragma unroll
+        for (int t = 0; t < T; ++t) acc[t] = mfma(w, h1[t][kb], acc[t]);
+      }
+#pragma unroll
+      for (int t = 0; t < T; ++t) h2[t][nb] = act(acc[t], b2 + 16 * nb + rq);
     }
-    f32x4 z[2];
+    f32x4 z[T][2];
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      f32x4 acc[T];
 #pragma unroll
-      for (int kb = 0; kb < 8; ++kb)
-        acc = mfma(ld_bf4(&s_w3[(16 * nb + col) * kS2 + 16 * kb + rq]), h2[kb],
-                   acc);
+      for (int t = 0; t < T; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int i = 0; i < 4; ++i) acc[i] += P.b3[16 * nb + rq + i];
-      z[nb] = acc;
+      for (int kb = 0; kb < 8; ++kb) {
+        const bf16x4 w = ld_bf4(&s_w3[(16 * nb + col) * kS2 + 16 * kb + rq]);
+#pragma unroll
+        for (int t = 0; t < T; ++t) acc[t] = mfma(w, h2[t][kb], acc[t]);
+      }
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[t][i] += b3[16 * nb + rq + i];
+        z[t][nb] = acc[t];
+      }
     }
-    // proposal mask of tet row0 + col: this lane's 8 bits, OR over the four
-    // lanes of the tet (l, l ^ 16, l ^ 32, l ^ 48)
-    uint32_t m = 0;
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        m |= (uint32_t)(z[nb][i] > thr) << (16 * nb + rq + i);
-    m |= (uint32_t)__shfl_xor((int)m, 16);
-    m |= (uint32_t)__shfl_xor((int)m, 32);
-    if (l < 16 && my_row < n) sel[my_row] = m;
-    if (logits && my_row < n) {
+    for (int t = 0; t < T; ++t) {
+      // proposal mask of tet my_row: this lane's 8 bits, OR over the four
+      // lanes of the tet (l, l ^ 16, l ^ 32, l ^ 48)
+      uint32_t m = 0;
 #pragma unroll
       for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          logits[my_row * kOut + 16 * nb + rq + i] = z[nb][i];
+          m |= (uint32_t)(z[t][nb][i] > thr) << (16 * nb + rq + i);
+      m |= (uint32_t)__shfl_xor((int)m, 16);
+      m |= (uint32_t)__shfl_xor((int)m, 32);
+      if (l < 16 && my_row[t] < n) sel[my_row[t]] = m;
+      if (logits && my_row[t] < n) {
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            logits[my_row[t] * kOut + 16 * nb + rq + i] = z[t][nb][i];
+      }
     }
   }
 }
