@@ -939,9 +939,43 @@ __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
                                                    Cand* cand,
                                                    int32_t* rel_ids) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  // (no early return: the whole wavefront takes part in the reduction)
+  const uint32_t m = t < n_tet ? sel[t] : 0u;
+  const int n = __popc(m);
+  int n_wave = n;  // the wavefront's largest set
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) n_wave = max(n_wave, __shfl_xor(n_wave, d));
   if (t >= n_tet) return;
+  const int64_t o = off[t];
+  if (n_wave <= 8) {
+    // small sets (the usual case): ids inserted one by one into 8 sorted
+    // registers, each slot updated in place to the median of its
+    // predecessor, the new id and itself
+    int32_t w[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = 0x7fffffff;
+    for (uint32_t rest = m; rest; rest &= rest - 1) {
+      const int32_t y = knn[(int64_t)t * kOut + (__ffs(rest) - 1)];
+      const int32_t x = y >= 0 ? y : 0x7fffffff;
+#pragma unroll
+      for (int k = 7; k > 0; --k) w[k] = max(w[k - 1], min(x, w[k]));
+      w[0] = min(x, w[0]);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k < n) {
+        Cand c;
+        c.tet = t;
+        c.seed = w[k];
+        c.m0 = 0;
+        c.m1 = 0;
+        cand[o + k] = c;
+        rel_ids[o + k] = w[k];
+      }
+    }
+    return;
+  }
   // selected ids ascending: bitonic sorting network on 32 registers
-  const uint32_t m = sel[t];
   int32_t v[kOut];
 #pragma unroll
   for (int k = 0; k < kOut; ++k) {
@@ -967,8 +1001,6 @@ __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
           v[j] = sw ? a : b;
         }
       }
-  const int64_t o = off[t];
-  const int n = __popc(m);
 #pragma unroll
   for (int k = 0; k < kOut; ++k) {
     if (k < n) {
