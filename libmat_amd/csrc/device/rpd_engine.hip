@@ -212,6 +212,13 @@ struct TierBufs {
 struct RunCaps {
   int64_t n_reg = 0, n_cand = 0, n_mid = 0, n_full = 0, n_exact = 0;
   int64_t cells = 0, words = 0, areas = 0;
+  // learned path: closure rounds run; the largest miss list, unique pairs
+  // and new candidates of a round; the merged pairs, tets re-run on the
+  // full relation, their candidates and the output candidates; the
+  // re-run tiers of the closure and fallback clips (largest launch)
+  int64_t rounds = 0, miss = 0, uniq = 0, added = 0, ext = 0;
+  int64_t flagged = 0, fb = 0, incl = 0;
+  int64_t c_mid = 0, c_full = 0, c_exact = 0;
 };
 
 }  // namespace
@@ -288,6 +295,10 @@ struct rpd_ctx {
   Buf flag, has_cell, flag32, flag_pos, flag_list, brute_n, brute_off,
       vals_a, vals_b, perm, miss_a, miss_b, umark, upos, ukeys, upass, ppos,
       mem_a, mem_b, reopened, out_off, out_cnt, out_ids, reop64;
+  // a captured learned run: device counts of its ranges, and where its
+  // final offsets land (eager learned runs swap the offset buffers)
+  Buf gcnt;
+  void* graph_tet_off = nullptr;
   rpd_stats st;
   int tuned_G = 0, tuned_caps = 0;  // rpd_tune's choice (0: not tuned)
   // ev[0..15]: pipeline phases; ev[16 + 2k], ev[17 + 2k]: around the k-th
@@ -358,7 +369,7 @@ struct D2H {
   size_t n;
 };
 hipError_t read_back(rpd_ctx* ctx, hipStream_t s,
-                     std::initializer_list<D2H> reads) {
+                     const std::vector<D2H>& reads) {
   if (t_capture) return hipErrorStreamCaptureUnsupported;
   ++ctx->n_reads;
   size_t off = 0;
@@ -377,6 +388,10 @@ hipError_t read_back(rpd_ctx* ctx, hipStream_t s,
     off += (r.n + 15) & ~(size_t)15;
   }
   return hipSuccess;
+}
+hipError_t read_back(rpd_ctx* ctx, hipStream_t s,
+                     std::initializer_list<D2H> reads) {
+  return read_back(ctx, s, std::vector<D2H>(reads));
 }
 
 template <class T>
@@ -829,6 +844,7 @@ struct RunState {
   RunCaps cap{};
   // counts of an eager run (the capacities of a later capture)
   RunCaps seen{};
+  int n_clip_calls = 0;  // clip_range calls of this run so far
 };
 
 MlpDev mlp_dev(rpd_ctx* ctx) {
@@ -1035,7 +1051,11 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
         read_back(ctx, s, {{&n_cand, ctx->tet_off.as<int64_t>() + nt, 8}}));
   R.seen.n_cand = n_cand;
   HIP_TRY(ctx->cand.need((size_t)std::max<int64_t>(n_cand, 1) * sizeof(Cand)));
-  HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
+  // (a captured run swaps rel_ids with the output ids every run: both hold
+  // the larger of the two capacities, so no capture reallocates them)
+  const int64_t n_ids =
+      R.graph ? std::max(n_cand, R.cap.incl) : std::max<int64_t>(n_cand, 1);
+  HIP_TRY(ctx->rel_ids.need((size_t)n_ids * 4));
   hipLaunchKernelGGL(k_tet_fill, dim3(ceil_div(nt, 256)), dim3(256), 0, s, R.M,
                      R.S, R.inst_d, R.tet_inst_d, ctx->bin_start.as<int32_t>(),
                      ctx->bin_ids.as<int32_t>(), ctx->tet_off.as<int64_t>(),
@@ -1095,7 +1115,7 @@ int learned_features(rpd_ctx* ctx, RunState& R) {
                      ctx->vert_inst.as<int32_t>(), ctx->sg_start.as<int32_t>(),
                      ctx->sg_ids.as<int32_t>(), ctx->vowner.as<int32_t>());
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[14], s));
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[14], s));
   // 32-bit proposal entries hold instance-local ids of 16 bits
   bool wide = false;
   for (const rpd_ctx::Inst& I : ctx->inst) wide = wide || I.n_site > 65536;
@@ -1105,7 +1125,7 @@ int learned_features(rpd_ctx* ctx, RunState& R) {
                      ctx->vowner.as<int32_t>(), ctx->knn.as<int32_t>(),
                      ctx->feat.as<uint16_t>(), ctx->omask.as<uint32_t>());
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[15], s));
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[15], s));
   ctx->feat_n = nt;
   return 0;
 }
@@ -1119,7 +1139,7 @@ int learned_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
   if (R.cfg.learned != 1 && R.cfg.learned != 2)
     return fail(ctx, "rpd_run: learned must be 0, 1 or 2");
   if (int rc = learned_features(ctx, R)) return rc;
-  HIP_TRY(hipEventRecord(ctx->ev[10], s));
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[10], s));
   HIP_TRY(ctx->sel.need((size_t)std::max(nt, 1) * 4));
   if (R.cfg.learned == 1) {
     const int64_t tiles = (nt + 15) / 16;
@@ -1136,27 +1156,39 @@ int learned_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
                        nt, R.cfg.proposal_k, ctx->sel.as<uint32_t>());
   }
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[11], s));
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[11], s));
   HIP_TRY(ctx->tet_count.need(((size_t)nt + 1) * 8));
   HIP_TRY(ctx->tet_off.need(((size_t)nt + 1) * 8));
-  HIP_TRY(hipMemsetAsync(ctx->tet_count.as<int64_t>() + nt, 0, 8, s));
+  HIP_TRY(zero32(s, ctx->tet_count.as<int64_t>() + nt, 2));
   hipLaunchKernelGGL(k_learn_count, dim3(ceil_div(nt, 64)), dim3(64), 0, s, nt,
                      R.M, R.S, ctx->knn.as<int32_t>(), ctx->sel.as<uint32_t>(),
                      ctx->omask.as<uint32_t>(), ctx->tet_count.as<int64_t>());
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[12], s));
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[12], s));
   HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->tet_count.as<int64_t>(),
                                  ctx->tet_off.as<int64_t>(), nt + 1));
-  int64_t n_cand = 0;
-  HIP_TRY(read_back(ctx, s, {{&n_cand, ctx->tet_off.as<int64_t>() + nt, 8}}));
+  // (a captured run: the first-round count stays at tet_off[n_tet], the
+  // lists hold the capacity)
+  int64_t n_cand = R.cap.n_cand;
+  if (!R.graph)
+    HIP_TRY(
+        read_back(ctx, s, {{&n_cand, ctx->tet_off.as<int64_t>() + nt, 8}}));
+  R.seen.n_cand = n_cand;
   HIP_TRY(ctx->cand.need((size_t)std::max<int64_t>(n_cand, 1) * sizeof(Cand)));
   HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
   hipLaunchKernelGGL(k_learn_fill, dim3(ceil_div(nt, 64)), dim3(64), 0, s, nt,
                      ctx->knn.as<int32_t>(), ctx->sel.as<uint32_t>(),
                      ctx->tet_off.as<int64_t>(), ctx->cand.as<Cand>(),
-                     ctx->rel_ids.as<int32_t>());
+                     ctx->rel_ids.as<int32_t>(),
+                     R.graph ? n_cand : INT64_MAX, R.cnt + 7);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ctx->ev[13], s));
+  if (R.graph) {
+    hipLaunchKernelGGL(k_clamp_offsets, dim3(ceil_div(nt + 1, 256)),
+                       dim3(256), 0, s, ctx->tet_off.as<int64_t>(),
+                       (int64_t)nt, n_cand, (const int32_t*)(R.cnt + 7));
+    HIP_TRY(hipGetLastError());
+  }
+  if (!R.graph) HIP_TRY(hipEventRecord(ctx->ev[13], s));
   *n_cand_out = n_cand;
   return 0;
 }
@@ -1198,11 +1230,28 @@ constexpr int kSmallEv = 16;
 // so the order cannot shorten it and the sort would only add latency)
 constexpr int64_t kStepOrderMin = 65536;
 
+// (a captured run: the range [c0, c1) is a capacity; its cells are the
+// first *rel_dev, ending at *abs_dev = c0 + *rel_dev, both on the device --
+// the first launch of a run, from 0, defaults to tet_off[n_tet])
 int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
-               bool events) {
+               bool events, const int64_t* rel_dev = nullptr,
+               const int64_t* abs_dev = nullptr) {
   hipStream_t s = R.s;
   const int64_t n = c1 - c0;
   const CapsRT sc = R.sc;
+  const bool first_call = R.n_clip_calls++ == 0;
+  // a captured run's re-run tiers are sized by the capacities of this
+  // launch (the first, or the largest of the later ones)
+  const int64_t cap_mid = first_call ? R.cap.n_mid : R.cap.c_mid;
+  const int64_t cap_full = first_call ? R.cap.n_full : R.cap.c_full;
+  const int64_t cap_exact = first_call ? R.cap.n_exact : R.cap.c_exact;
+  const int32_t r_large0 = R.r_large, r_exact0 = R.r_exact,
+                r_full0 = R.r_full;
+  if (R.graph) {
+    hipLaunchKernelGGL(k_set_rerun_counts, dim3(1), dim3(64), 0, s, R.cnt,
+                       R.r_large, R.r_exact, R.r_full);
+    HIP_TRY(hipGetLastError());
+  }
   // the small tier takes its cells in decreasing order of clip steps (a
   // stable sort of the step counts), so that the cells of one wavefront
   // need about as many steps as each other
@@ -1215,7 +1264,10 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     HIP_TRY(ctx->trip_v2.need((size_t)n * 4));
   }
   // (a captured run: the candidate count is tet_off[n_tet] on the device)
-  const int64_t* n_dev = R.graph ? ctx->tet_off.as<int64_t>() + R.nt : nullptr;
+  const int64_t* n_dev =
+      R.graph ? (abs_dev ? abs_dev : ctx->tet_off.as<int64_t>() + R.nt)
+              : nullptr;
+  const int64_t* n_rel = R.graph ? (rel_dev ? rel_dev : n_dev) : nullptr;
   if (R.cfg.use_active_mask && n > 0)
     hipLaunchKernelGGL(k_cand_mask, dim3(ceil_div(n, 256)), dim3(256), 0, s,
                        R.M, R.S, n, ctx->cand.as<Cand>() + c0,
@@ -1236,9 +1288,13 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
         ctx->trip_v2.as<int32_t>(), (int)n, 0, 8, s));
   }
   if (events) HIP_TRY(hipEventRecord(ctx->ev[1], s));
-  HIP_TRY(ctx->rerun_large.need_keep((size_t)std::max<int64_t>(c1, 1) * 4,
+  // the re-run lists: room for every cell of the range after the entries
+  // of earlier launches
+  const int64_t list_cap =
+      std::max<int64_t>({c1, (int64_t)R.r_large + n, (int64_t)R.r_exact + n, 1});
+  HIP_TRY(ctx->rerun_large.need_keep((size_t)list_cap * 4,
                                      (size_t)R.r_large * 4, s));
-  HIP_TRY(ctx->rerun_exact.need_keep((size_t)std::max<int64_t>(c1, 1) * 4,
+  HIP_TRY(ctx->rerun_exact.need_keep((size_t)list_cap * 4,
                                      (size_t)R.r_exact * 4, s));
   HIP_TRY(ctx->loc.need_keep((size_t)std::max<int64_t>(c1, 1) * 4,
                              (size_t)c0 * 4, s));
@@ -1249,14 +1305,14 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     HIP_TRY(ctx->small.need_keep(1, 0, sc.V, sc.P, sc.E, s));
   }
   ClipArgs a = clip_args(ctx, R);
-  // the re-run lists hold up to c1 entries (need_keep above)
-  a.cap_large = (int32_t)std::min<int64_t>(c1, 0x7fffffff);
+  // the re-run lists hold up to list_cap entries (need_keep above)
+  a.cap_large = (int32_t)std::min<int64_t>(list_cap, 0x7fffffff);
   a.cap_exact = a.cap_large;
   if (R.cfg.small_tier) {
     a.item_base = c0;
     a.slot_base = c0;
     a.n_items = n;
-    a.n_items_d64 = n_dev;
+    a.n_items_d64 = n_rel;
     if (by_steps) {  // slot = cell index, as with the identity order
       a.items = ctx->trip_v2.as<int32_t>();
       a.slot_by_cell = 1;
@@ -1293,14 +1349,22 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
   if (!R.graph) HIP_TRY(read_back(ctx, s, {{host_cnt, R.cnt, 32}}));
   // (a captured run launches its re-run tiers at capacity and reads their
   // counts on the device)
+  const bool mid = R.cfg.small_tier && R.sc.V < SmallCaps::V;
   const int32_t n_over =
-      R.graph ? (int32_t)(R.r_large + R.cap.n_mid) : host_cnt[1];
-  R.seen.n_mid = n_over - R.r_large;
+      R.graph ? (int32_t)(R.r_large + (mid ? cap_mid : cap_full))
+              : host_cnt[1];
+  // counts of the eager run (first launch; the largest later one)
+  auto note = [&](int64_t& first, int64_t& later, int64_t v) {
+    if (first_call)
+      first = v;
+    else
+      later = std::max(later, v);
+  };
+  note(R.seen.n_mid, R.seen.c_mid, n_over - R.r_large);
   // cells over a tight small tier first get the middle tier (SmallCaps, 8
   // per wavefront; tier 3, slot = position in rerun_large); the ones over
   // that, or over any other small tier, get the full-capacity tier (tier 1,
   // slot = position in its list)
-  const bool mid = R.cfg.small_tier && R.sc.V < SmallCaps::V;
   const int32_t* full_list = ctx->rerun_large.as<int32_t>();
   int32_t full_r0 = R.r_large, full_r1 = n_over;
   if (mid) {
@@ -1333,11 +1397,9 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     }
     full_list = ctx->rerun_full.as<int32_t>();
     full_r0 = R.r_full;
-    full_r1 = R.graph ? (int32_t)(R.r_full + R.cap.n_full) : host_cnt[6];
-  } else if (R.graph) {
-    full_r1 = (int32_t)(R.r_large + R.cap.n_full);
+    full_r1 = R.graph ? (int32_t)(R.r_full + cap_full) : host_cnt[6];
   }
-  R.seen.n_full = full_r1 - full_r0;
+  note(R.seen.n_full, R.seen.c_full, full_r1 - full_r0);
   // a captured run reads the full tier's count here
   const int32_t* full_cnt = mid ? R.cnt + 6 : R.cnt + 1;
   HIP_TRY(ctx->large.need_keep(full_r1, full_r0, LargeCaps::V, LargeCaps::P,
@@ -1356,8 +1418,8 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
     if (!R.graph) HIP_TRY(read_back(ctx, s, {{host_cnt, R.cnt, 32}}));
   }
   const int32_t n_exact =
-      R.graph ? (int32_t)(R.r_exact + R.cap.n_exact) : host_cnt[2];
-  R.seen.n_exact = n_exact - R.r_exact;
+      R.graph ? (int32_t)(R.r_exact + cap_exact) : host_cnt[2];
+  note(R.seen.n_exact, R.seen.c_exact, n_exact - R.r_exact);
   HIP_TRY(ctx->exact.need_keep(n_exact, R.r_exact, LargeCaps::V, LargeCaps::P,
                                LargeCaps::E, s));
   if (n_exact > R.r_exact) {
@@ -1382,6 +1444,15 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
                        R.r_exact, n_exact, 2, ctx->loc.as<int32_t>(),
                        R.graph ? R.cnt + 2 : nullptr);
   HIP_TRY(hipGetLastError());
+  if (R.graph) {  // this launch's re-run counts within what it reserved
+    hipLaunchKernelGGL(k_check_rerun_counts, dim3(1), dim3(64), 0, s, R.cnt,
+                       n_over, n_exact, mid ? full_r1 : 0x7fffffff,
+                       R.cnt + 7);
+    HIP_TRY(hipGetLastError());
+  }
+  (void)r_large0;
+  (void)r_exact0;
+  (void)r_full0;
   R.r_large = n_over;
   if (mid) R.r_full = full_r1;
   R.r_exact = n_exact;
@@ -1420,10 +1491,11 @@ int sort_u64(rpd_ctx* ctx, hipStream_t s, const uint64_t* in, uint64_t* out,
 // certificate state for a round of n new candidates: a cleared miss list
 // of capacity max(4096, 2n) and the current ext keys
 int certify_round(rpd_ctx* ctx, RunState& R, int64_t n, int64_t n_ext) {
-  const int64_t cap = std::max<int64_t>(4096, 2 * n);
+  // (a captured run: the capacity of its miss lists)
+  const int64_t cap = R.graph ? R.cap.miss : std::max<int64_t>(4096, 2 * n);
   HIP_TRY(ctx->miss_a.need((size_t)cap * 8));
   HIP_TRY(ctx->miss_b.need((size_t)cap * 8));
-  HIP_TRY(hipMemsetAsync(R.cnt + 4, 0, 4, R.s));
+  HIP_TRY(zero32(R.s, R.cnt + 4, 1));
   R.cert.ext = ctx->mem_a.as<uint64_t>();
   R.cert.n_ext = n_ext;
   R.cert.miss = ctx->miss_a.as<uint64_t>();
@@ -1437,12 +1509,14 @@ int certify_round(rpd_ctx* ctx, RunState& R, int64_t n, int64_t n_ext) {
 int certify_begin(rpd_ctx* ctx, RunState& R, int64_t n1) {
   hipStream_t s = R.s;
   const int32_t nt = R.nt;
-  HIP_TRY(ctx->flag.need((size_t)nt + 1));
-  HIP_TRY(ctx->has_cell.need((size_t)nt + 1));
-  HIP_TRY(ctx->reopened.need((size_t)nt + 1));
-  HIP_TRY(hipMemsetAsync(ctx->flag.p, 0, (size_t)nt + 1, s));
-  HIP_TRY(hipMemsetAsync(ctx->has_cell.p, 0, (size_t)nt + 1, s));
-  HIP_TRY(hipMemsetAsync(ctx->reopened.p, 0, (size_t)nt + 1, s));
+  // byte flags per tet, cleared by 32-bit words
+  const int64_t w = ((int64_t)nt + 4) / 4;
+  HIP_TRY(ctx->flag.need((size_t)w * 4));
+  HIP_TRY(ctx->has_cell.need((size_t)w * 4));
+  HIP_TRY(ctx->reopened.need((size_t)w * 4));
+  HIP_TRY(zero32(s, ctx->flag.p, w));
+  HIP_TRY(zero32(s, ctx->has_cell.p, w));
+  HIP_TRY(zero32(s, ctx->reopened.p, w));
   R.cert.tet_off = ctx->tet_off.as<int64_t>();
   R.cert.mem_ids = ctx->rel_ids.as<int32_t>();
   R.cert.flag = ctx->flag.as<uint8_t>();
@@ -1450,8 +1524,12 @@ int certify_begin(rpd_ctx* ctx, RunState& R, int64_t n1) {
   return certify_round(ctx, R, n1, 0);
 }
 
+int certify_captured(rpd_ctx* ctx, RunState& R, int64_t n1,
+                     int64_t* n_incl_out);
+
 int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
                          int64_t* n_incl_out) {
+  if (R.graph) return certify_captured(ctx, R, n1, n_incl_out);
   hipStream_t s = R.s;
   const int32_t nt = R.nt;
   HIP_TRY(ctx->flag32.need(((size_t)nt + 1) * 4));
@@ -1464,14 +1542,16 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
     int32_t nm = 0;
     HIP_TRY(read_back(ctx, s, {{&nm, R.cnt + 4, 4}}));
     const int64_t n_miss = std::min<int64_t>(nm, R.cert.cap);
+    R.seen.miss = std::max<int64_t>(R.seen.miss, n_miss);
     if (n_miss == 0) break;
     if (round == kClosureRounds) {  // still open: full relation
       hipLaunchKernelGGL(k_flag_miss, dim3(ceil_div(n_miss, 256)), dim3(256),
                          0, s, n_miss, ctx->miss_a.as<uint64_t>(),
-                         ctx->flag.as<uint8_t>());
+                         ctx->flag.as<uint8_t>(), (const int32_t*)nullptr);
       HIP_TRY(hipGetLastError());
       break;
     }
+    R.seen.rounds = round + 1;
     // unique missing pairs; related ones become candidates, all members
     if (int rc = sort_u64(ctx, s, ctx->miss_a.as<uint64_t>(),
                           ctx->miss_b.as<uint64_t>(), n_miss))
@@ -1480,7 +1560,7 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
     HIP_TRY(ctx->upos.need(((size_t)n_miss + 1) * 4));
     hipLaunchKernelGGL(k_unique_mark, dim3(ceil_div(n_miss + 1, 256)),
                        dim3(256), 0, s, n_miss, ctx->miss_b.as<uint64_t>(),
-                       ctx->umark.as<int32_t>());
+                       ctx->umark.as<int32_t>(), (const int32_t*)nullptr);
     HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->umark.as<int32_t>(),
                                    ctx->upos.as<int32_t>(), n_miss + 1));
     // the unique count n_u = upos[n_miss] stays on the device: buffers and
@@ -1504,6 +1584,8 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
     HIP_TRY(read_back(ctx, s,
                       {{&n_u, ctx->upos.as<int32_t>() + n_miss, 4},
                        {&n_new, ctx->ppos.as<int32_t>() + n_miss, 4}}));
+    R.seen.uniq = std::max<int64_t>(R.seen.uniq, n_u);
+    R.seen.added = std::max<int64_t>(R.seen.added, n_new);
     HIP_TRY(ctx->cand.need_keep((size_t)(c_hi + n_new) * sizeof(Cand),
                                 (size_t)c_hi * sizeof(Cand), s));
     if (n_u > 0)
@@ -1511,7 +1593,8 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
                          0, s, (int64_t)n_u, ctx->ukeys.as<uint64_t>(),
                          ctx->upass.as<uint8_t>(), ctx->ppos.as<int32_t>(),
                          ctx->cand.as<Cand>(), c_hi,
-                         ctx->reopened.as<uint8_t>());
+                         ctx->reopened.as<uint8_t>(), (const int32_t*)nullptr,
+                         INT64_MAX, R.cnt + 7);
     HIP_TRY(hipGetLastError());
     // ext += the unique pairs (disjoint from it), sorted again
     HIP_TRY(ctx->mem_a.need_keep((size_t)(n_ext + n_u) * 8, (size_t)n_ext * 8,
@@ -1532,6 +1615,7 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
     c_hi += n_new;
   }
   (void)c_lo;
+  R.seen.ext = n_ext;
   R.cert.tet_off = nullptr;  // the full-relation re-runs are final
   // tets without a cell, with a dropped cell or left open: full relation
   HIP_TRY(ctx->reop64.need(64 * 4));
@@ -1556,14 +1640,15 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
     hipLaunchKernelGGL(k_flag_list, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
                        nt, ctx->flag32.as<int32_t>(),
                        ctx->flag_pos.as<int32_t>(),
-                       ctx->flag_list.as<int32_t>());
+                       ctx->flag_list.as<int32_t>(), INT64_MAX, R.cnt + 7);
     HIP_TRY(ctx->brute_n.need(((size_t)nf + 1) * 8));
     HIP_TRY(ctx->brute_off.need(((size_t)nf + 1) * 8));
     HIP_TRY(hipMemsetAsync(ctx->brute_n.as<int64_t>() + nf, 0, 8, s));
     hipLaunchKernelGGL(k_brute, dim3(nf), dim3(64), 0, s, nf,
                        ctx->flag_list.as<int32_t>(), R.M, R.S, R.inst_d,
                        R.tet_inst_d, (const int64_t*)nullptr,
-                       ctx->brute_n.as<int64_t>(), (Cand*)nullptr, (int64_t)0);
+                       ctx->brute_n.as<int64_t>(), (Cand*)nullptr, (int64_t)0,
+                       (const int32_t*)nullptr, INT64_MAX);
     HIP_TRY(hipGetLastError());
     HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->brute_n.as<int64_t>(),
                                    ctx->brute_off.as<int64_t>(), nf + 1));
@@ -1573,10 +1658,13 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
     hipLaunchKernelGGL(k_brute, dim3(nf), dim3(64), 0, s, nf,
                        ctx->flag_list.as<int32_t>(), R.M, R.S, R.inst_d,
                        R.tet_inst_d, ctx->brute_off.as<int64_t>(),
-                       (int64_t*)nullptr, ctx->cand.as<Cand>(), c_hi);
+                       (int64_t*)nullptr, ctx->cand.as<Cand>(), c_hi,
+                       (const int32_t*)nullptr, INT64_MAX);
     HIP_TRY(hipGetLastError());
     if (int rc = clip_range(ctx, R, c_hi, c_hi + n_fb, false)) return rc;
   }
+  R.seen.flagged = nf;
+  R.seen.fb = n_fb;
   const int64_t n_all = c_hi + n_fb;
   ctx->st.learned[0] = n1;
   ctx->st.learned[1] = n_reopened;
@@ -1620,31 +1708,240 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
                                  ctx->out_off.as<int64_t>(), nt + 1));
   int64_t n_incl = 0;
   HIP_TRY(read_back(ctx, s, {{&n_incl, ctx->out_off.as<int64_t>() + nt, 8}}));
+  R.seen.incl = n_incl;
   HIP_TRY(ctx->perm.need((size_t)std::max<int64_t>(n_incl, 1) * 4));
   if (n1 > 0)
     hipLaunchKernelGGL(k_place_a, dim3(ceil_div(n1, 256)), dim3(256), 0, s, n1,
                        ctx->cand.as<Cand>(), ctx->flag.as<uint8_t>(),
                        ctx->reopened.as<uint8_t>(), ctx->tet_off.as<int64_t>(),
                        ctx->keys_b.as<uint64_t>(), (int64_t)n_b,
-                       ctx->out_off.as<int64_t>(), ctx->perm.as<int32_t>());
+                       ctx->out_off.as<int64_t>(), ctx->perm.as<int32_t>(),
+                       (const int64_t*)nullptr, INT64_MAX);
   if (n_b > 0)
     hipLaunchKernelGGL(k_place_b, dim3(ceil_div(n_b, 256)), dim3(256), 0, s,
                        (int64_t)n_b, ctx->keys_b.as<uint64_t>(),
                        ctx->vals_b.as<int32_t>(), ctx->flag.as<uint8_t>(),
                        ctx->tet_off.as<int64_t>(), ctx->rel_ids.as<int32_t>(),
-                       ctx->out_off.as<int64_t>(), ctx->perm.as<int32_t>());
+                       ctx->out_off.as<int64_t>(), ctx->perm.as<int32_t>(),
+                       INT64_MAX);
   HIP_TRY(hipGetLastError());
   // relation CSR of the output sets (tet_off, rel_ids of rpd_download_*)
   HIP_TRY(ctx->out_ids.need((size_t)std::max<int64_t>(n_incl, 1) * 4));
   if (n_incl > 0)
     hipLaunchKernelGGL(k_out_ids, dim3(ceil_div(n_incl, 256)), dim3(256), 0, s,
                        n_incl, ctx->cand.as<Cand>(), ctx->perm.as<int32_t>(),
-                       ctx->out_ids.as<int32_t>());
+                       ctx->out_ids.as<int32_t>(), (const int64_t*)nullptr);
   HIP_TRY(hipGetLastError());
   ctx->tet_off.swap(ctx->out_off);
   ctx->rel_ids.swap(ctx->out_ids);
   ctx->tet_count.swap(ctx->out_cnt);
   *n_incl_out = n_incl;
+  return 0;
+}
+
+// The certificate, closure rounds, fallback and output order of a captured
+// learned run (rpd_graph_build): the steps of certify_and_fallback with
+// every count left on the device.  Lists are sized by the capacities in
+// R.cap and padded (~0 keys sort last); closure round r puts its new
+// candidates at [n1_cap + r * added, + its count), the fallback after the
+// last round; every count past its capacity sets the overflow counter
+// (cnt[7]), and misses left after the last captured round do too unless
+// that round is the last the eager path runs.  The output order is the
+// eager one: the same (tet, site) pairs in the same merge.  Device counts:
+// gcnt[0] merged pairs, gcnt[1 + 2 r] / gcnt[2 + 2 r] the count and the end
+// of range r (r = rounds: the fallback).
+int certify_captured(rpd_ctx* ctx, RunState& R, int64_t n1_cap,
+                     int64_t* n_incl_out) {
+  hipStream_t s = R.s;
+  const int32_t nt = R.nt;
+  const RunCaps& C = R.cap;
+  const int rounds = (int)C.rounds;
+  const int64_t cap_miss = C.miss, cap_new = C.added, cap_ext = C.ext;
+  const int64_t cap_nf = C.flagged, cap_fb = C.fb, cap_incl = C.incl;
+  int32_t* ovf = R.cnt + 7;
+  auto grid = [](int64_t n) { return dim3(ceil_div(std::max<int64_t>(n, 1), 256)); };
+  const int64_t n_gc = 3 + 2 * (int64_t)rounds;
+  HIP_TRY(ctx->gcnt.need((size_t)n_gc * 8));
+  int64_t* gc = ctx->gcnt.as<int64_t>();
+  HIP_TRY(zero32(s, gc, 2 * n_gc));
+  HIP_TRY(ctx->flag32.need(((size_t)nt + 1) * 4));
+  HIP_TRY(ctx->flag_pos.need(((size_t)nt + 1) * 4));
+  HIP_TRY(ctx->umark.need(((size_t)cap_miss + 1) * 4));
+  HIP_TRY(ctx->upos.need(((size_t)cap_miss + 1) * 4));
+  HIP_TRY(ctx->ppos.need(((size_t)cap_miss + 1) * 4));
+  HIP_TRY(ctx->ukeys.need((size_t)std::max<int64_t>(cap_miss, 1) * 8));
+  HIP_TRY(ctx->upass.need((size_t)cap_miss + 1));
+  HIP_TRY(ctx->mem_a.need((size_t)std::max<int64_t>(cap_ext, 1) * 8));
+  HIP_TRY(ctx->mem_b.need((size_t)std::max<int64_t>(cap_ext, 1) * 8));
+  const int64_t fb_base = n1_cap + (int64_t)rounds * cap_new;
+  const int64_t c_end = fb_base + cap_fb;
+  HIP_TRY(ctx->cand.need_keep((size_t)c_end * sizeof(Cand),
+                              (size_t)n1_cap * sizeof(Cand), s));
+  const int32_t* nm = R.cnt + 4;
+  const int32_t* n_u = ctx->upos.as<int32_t>() + cap_miss;
+  int64_t c_hi = n1_cap;
+  for (int r = 0; r < rounds; ++r) {
+    // unique missing pairs of the round (a padded sort of the miss list)
+    hipLaunchKernelGGL(k_pad64<int32_t>, grid(cap_miss), dim3(256), 0, s,
+                       ctx->miss_a.as<uint64_t>(), cap_miss, nm);
+    if (int rc = sort_u64(ctx, s, ctx->miss_a.as<uint64_t>(),
+                          ctx->miss_b.as<uint64_t>(), cap_miss))
+      return rc;
+    hipLaunchKernelGGL(k_unique_mark, grid(cap_miss + 1), dim3(256), 0, s,
+                       cap_miss, ctx->miss_b.as<uint64_t>(),
+                       ctx->umark.as<int32_t>(), nm);
+    HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->umark.as<int32_t>(),
+                                   ctx->upos.as<int32_t>(), cap_miss + 1));
+    hipLaunchKernelGGL(k_closure_split, grid(cap_miss), dim3(256), 0, s,
+                       cap_miss, ctx->miss_b.as<uint64_t>(),
+                       ctx->umark.as<int32_t>(), ctx->upos.as<int32_t>(), R.M,
+                       R.S, ctx->ukeys.as<uint64_t>(), ctx->upass.as<uint8_t>());
+    hipLaunchKernelGGL(k_pass32, grid(cap_miss + 1), dim3(256), 0, s, cap_miss,
+                       n_u, ctx->upass.as<uint8_t>(), ctx->umark.as<int32_t>());
+    HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->umark.as<int32_t>(),
+                                   ctx->ppos.as<int32_t>(), cap_miss + 1));
+    // the related ones: new candidates at [c_hi, c_hi + count)
+    hipLaunchKernelGGL(k_closure_fill, grid(cap_miss), dim3(256), 0, s,
+                       cap_miss, ctx->ukeys.as<uint64_t>(),
+                       ctx->upass.as<uint8_t>(), ctx->ppos.as<int32_t>(),
+                       ctx->cand.as<Cand>(), c_hi, ctx->reopened.as<uint8_t>(),
+                       n_u, cap_new, ovf);
+    hipLaunchKernelGGL(k_range_count<int32_t>, dim3(1), dim3(64), 0, s,
+                       ctx->ppos.as<int32_t>() + cap_miss, c_hi, cap_new,
+                       gc + 1 + 2 * r, gc + 2 + 2 * r, ovf);
+    // merged pairs += the unique pairs (disjoint from them), sorted again
+    hipLaunchKernelGGL(k_append64, grid(cap_miss), dim3(256), 0, s,
+                       ctx->mem_a.as<uint64_t>(), (const int64_t*)gc,
+                       ctx->ukeys.as<uint64_t>(), n_u, cap_miss, cap_ext, ovf);
+    hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, gc, n_u, cap_ext);
+    hipLaunchKernelGGL(k_pad64<int64_t>, grid(cap_ext), dim3(256), 0, s,
+                       ctx->mem_a.as<uint64_t>(), cap_ext, (const int64_t*)gc);
+    HIP_TRY(hipGetLastError());
+    if (int rc = sort_u64(ctx, s, ctx->mem_a.as<uint64_t>(),
+                          ctx->mem_b.as<uint64_t>(), cap_ext))
+      return rc;
+    ctx->mem_a.swap(ctx->mem_b);
+    // the round's cells, certified as they are clipped (the padded pair
+    // list: its ~0 keys match no pair)
+    if (int rc = certify_round(ctx, R, 0, cap_ext)) return rc;
+    if (int rc = clip_range(ctx, R, c_hi, c_hi + cap_new, false,
+                            gc + 1 + 2 * r, gc + 2 + 2 * r))
+      return rc;
+    c_hi += cap_new;
+  }
+  if (rounds == kClosureRounds) {  // still open: full relation
+    hipLaunchKernelGGL(k_flag_miss, grid(cap_miss), dim3(256), 0, s, cap_miss,
+                       ctx->miss_a.as<uint64_t>(), ctx->flag.as<uint8_t>(), nm);
+  } else {  // misses left: the run needs more rounds than were captured
+    hipLaunchKernelGGL(k_flag_if_positive, dim3(1), dim3(64), 0, s, nm, ovf);
+  }
+  HIP_TRY(hipGetLastError());
+  R.cert.tet_off = nullptr;  // the full-relation re-runs are final
+  // tets without a cell, with a dropped cell or left open: full relation
+  HIP_TRY(ctx->reop64.need(64 * 4));
+  HIP_TRY(zero32(s, ctx->reop64.p, 64));
+  hipLaunchKernelGGL(k_flag_final, dim3(ceil_div(nt, 256)), dim3(256), 0, s,
+                     nt, ctx->flag.as<uint8_t>(), ctx->has_cell.as<uint8_t>(),
+                     ctx->reopened.as<uint8_t>(), ctx->flag32.as<int32_t>(),
+                     ctx->reop64.as<int32_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(zero32(s, ctx->flag32.as<int32_t>() + nt, 1));
+  HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->flag32.as<int32_t>(),
+                                 ctx->flag_pos.as<int32_t>(), nt + 1));
+  const int32_t* nf = ctx->flag_pos.as<int32_t>() + nt;
+  HIP_TRY(ctx->flag_list.need((size_t)std::max<int64_t>(cap_nf, 1) * 4));
+  hipLaunchKernelGGL(k_flag_list, dim3(ceil_div(nt, 256)), dim3(256), 0, s, nt,
+                     ctx->flag32.as<int32_t>(), ctx->flag_pos.as<int32_t>(),
+                     ctx->flag_list.as<int32_t>(), cap_nf, ovf);
+  HIP_TRY(ctx->brute_n.need(((size_t)cap_nf + 1) * 8));
+  HIP_TRY(ctx->brute_off.need(((size_t)cap_nf + 1) * 8));
+  HIP_TRY(zero32(s, ctx->brute_n.p, 2 * (cap_nf + 1)));
+  if (cap_nf > 0)
+    hipLaunchKernelGGL(k_brute, dim3(cap_nf), dim3(64), 0, s, (int32_t)cap_nf,
+                       ctx->flag_list.as<int32_t>(), R.M, R.S, R.inst_d,
+                       R.tet_inst_d, (const int64_t*)nullptr,
+                       ctx->brute_n.as<int64_t>(), (Cand*)nullptr, (int64_t)0,
+                       nf, INT64_MAX);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->brute_n.as<int64_t>(),
+                                 ctx->brute_off.as<int64_t>(), cap_nf + 1));
+  hipLaunchKernelGGL(k_range_count<int64_t>, dim3(1), dim3(64), 0, s,
+                     ctx->brute_off.as<int64_t>() + cap_nf, fb_base, cap_fb,
+                     gc + 1 + 2 * rounds, gc + 2 + 2 * rounds, ovf);
+  if (cap_nf > 0)
+    hipLaunchKernelGGL(k_brute, dim3(cap_nf), dim3(64), 0, s, (int32_t)cap_nf,
+                       ctx->flag_list.as<int32_t>(), R.M, R.S, R.inst_d,
+                       R.tet_inst_d, ctx->brute_off.as<int64_t>(),
+                       (int64_t*)nullptr, ctx->cand.as<Cand>(), fb_base, nf,
+                       c_end);
+  HIP_TRY(hipGetLastError());
+  if (int rc = clip_range(ctx, R, fb_base, c_end, false, gc + 1 + 2 * rounds,
+                          gc + 2 + 2 * rounds))
+    return rc;
+  // output order: the first round's candidates (kept order) merged per tet
+  // with the sorted added ones
+  const int64_t nB = c_end - n1_cap;
+  HIP_TRY(ctx->keys_a.need((size_t)std::max<int64_t>(nB, 1) * 8));
+  HIP_TRY(ctx->keys_b.need((size_t)std::max<int64_t>(nB, 1) * 8));
+  HIP_TRY(ctx->vals_a.need((size_t)std::max<int64_t>(nB, 1) * 4));
+  HIP_TRY(ctx->vals_b.need((size_t)std::max<int64_t>(nB, 1) * 4));
+  if (nB > 0) {
+    hipLaunchKernelGGL(k_b_keys_ranges, grid(nB), dim3(256), 0, s, n1_cap,
+                       std::max<int64_t>(cap_new, 1), (int32_t)rounds, fb_base,
+                       c_end, (const int64_t*)(gc + 1), ctx->cand.as<Cand>(),
+                       ctx->flag.as<uint8_t>(), ctx->keys_a.as<uint64_t>(),
+                       ctx->vals_a.as<int32_t>());
+    HIP_TRY(hipGetLastError());
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+        nullptr, tmp, ctx->keys_a.as<uint64_t>(), ctx->keys_b.as<uint64_t>(),
+        ctx->vals_a.as<int32_t>(), ctx->vals_b.as<int32_t>(), (int)nB, 0, 64,
+        s));
+    HIP_TRY(ctx->scan_tmp.need(tmp));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+        ctx->scan_tmp.p, tmp, ctx->keys_a.as<uint64_t>(),
+        ctx->keys_b.as<uint64_t>(), ctx->vals_a.as<int32_t>(),
+        ctx->vals_b.as<int32_t>(), (int)nB, 0, 64, s));
+  }
+  HIP_TRY(ctx->out_off.need(((size_t)nt + 1) * 8));
+  HIP_TRY(ctx->out_cnt.need(((size_t)nt + 1) * 8));
+  hipLaunchKernelGGL(k_out_counts, dim3(ceil_div(nt + 1, 256)), dim3(256), 0,
+                     s, nt, ctx->flag.as<uint8_t>(), ctx->reopened.as<uint8_t>(),
+                     ctx->tet_off.as<int64_t>(), ctx->keys_b.as<uint64_t>(), nB,
+                     ctx->out_cnt.as<int64_t>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(exclusive_sum<int64_t>(ctx, ctx->out_cnt.as<int64_t>(),
+                                 ctx->out_off.as<int64_t>(), nt + 1));
+  const int64_t* n_incl = ctx->out_off.as<int64_t>() + nt;
+  HIP_TRY(ctx->perm.need((size_t)std::max<int64_t>(cap_incl, 1) * 4));
+  // (cleared: after an overflow some positions stay unplaced, and later
+  // launches then read candidate 0 there)
+  HIP_TRY(zero32(s, ctx->perm.p, std::max<int64_t>(cap_incl, 1)));
+  if (n1_cap > 0)
+    hipLaunchKernelGGL(k_place_a, grid(n1_cap), dim3(256), 0, s, n1_cap,
+                       ctx->cand.as<Cand>(), ctx->flag.as<uint8_t>(),
+                       ctx->reopened.as<uint8_t>(), ctx->tet_off.as<int64_t>(),
+                       ctx->keys_b.as<uint64_t>(), nB,
+                       ctx->out_off.as<int64_t>(), ctx->perm.as<int32_t>(),
+                       (const int64_t*)(ctx->tet_off.as<int64_t>() + nt),
+                       cap_incl);
+  if (nB > 0)
+    hipLaunchKernelGGL(k_place_b, grid(nB), dim3(256), 0, s, nB,
+                       ctx->keys_b.as<uint64_t>(), ctx->vals_b.as<int32_t>(),
+                       ctx->flag.as<uint8_t>(), ctx->tet_off.as<int64_t>(),
+                       ctx->rel_ids.as<int32_t>(), ctx->out_off.as<int64_t>(),
+                       ctx->perm.as<int32_t>(), cap_incl);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(ctx->out_ids.need((size_t)std::max(cap_incl, n1_cap) * 4));
+  if (cap_incl > 0)
+    hipLaunchKernelGGL(k_out_ids, grid(cap_incl), dim3(256), 0, s, cap_incl,
+                       ctx->cand.as<Cand>(), ctx->perm.as<int32_t>(),
+                       ctx->out_ids.as<int32_t>(), n_incl);
+  HIP_TRY(hipGetLastError());
+  ctx->tet_off.swap(ctx->out_off);
+  ctx->rel_ids.swap(ctx->out_ids);
+  ctx->tet_count.swap(ctx->out_cnt);
+  *n_incl_out = cap_incl;
   return 0;
 }
 
@@ -2260,9 +2557,9 @@ int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom) {
   HIP_TRY(hipSetDevice(ctx->device));
   RunState R;
   if (int rc = run_config(ctx, cfg, R)) return rc;
-  if (R.cfg.learned || R.cfg.count_work || !R.cfg.small_tier)
-    return fail(ctx, "rpd_graph_build: plain path with the small tier only "
-                     "(learned = 0, count_work = 0, small_tier = 1)");
+  if (R.cfg.count_work || !R.cfg.small_tier)
+    return fail(ctx, "rpd_graph_build: the small tier without work counters "
+                     "only (count_work = 0, small_tier = 1)");
   graph_drop(ctx);
   // 1. an eager run: the results, and the counts the capacities follow
   if (int rc = rpd_run(ctx, &R.cfg)) return rc;
@@ -2278,7 +2575,24 @@ int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom) {
   cap.cells = grow(n.cells, f, 1024);
   cap.words = grow(n.words, f, 65536);
   cap.areas = grow(n.areas, f, 16384);
-  if (cap.n_reg >= (1LL << 31) || cap.n_cand >= (1LL << 31))
+  if (R.cfg.learned) {
+    // one closure round more than the eager run needed (up to the eager
+    // limit); the merged pairs get room for that round too
+    cap.rounds = std::min<int64_t>(n.rounds + 1, kClosureRounds);
+    cap.miss = grow(n.miss, f, 4096);
+    cap.uniq = cap.miss;
+    cap.added = grow(n.added, f, 1024);
+    cap.ext = grow(n.ext, f, 4096) + cap.miss;
+    cap.flagged = grow(n.flagged, f, 256);
+    cap.fb = grow(n.fb, f, 4096);
+    cap.incl = grow(n.incl, f, 1024);
+    cap.c_mid = grow(n.c_mid, f, 256);
+    cap.c_full = grow(n.c_full, f, 256);
+    cap.c_exact = grow(n.c_exact, f, 256);
+  }
+  if (cap.n_reg >= (1LL << 31) ||
+      cap.n_cand + cap.rounds * cap.added + cap.fb >= (1LL << 31) ||
+      cap.miss >= (1LL << 31) || cap.incl >= (1LL << 31))
     return fail(ctx, "rpd_graph_build: capacities over 2^31 entries");
   // the neighbour lists and the tables sized by them may lengthen in later
   // sphere updates: room for them too, so that those keep their addresses
@@ -2291,6 +2605,14 @@ int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom) {
                                     ctx->stream));
     HIP_TRY(ctx->knn_site_near.need_keep(nk * 32, (size_t)ctx->n_knn * 32,
                                          ctx->stream));
+  }
+  if (R.cfg.learned) {
+    // eager learned runs between replays size their first miss lists by
+    // their candidate count: room for up to the captured capacity, so that
+    // they reallocate nothing the graph uses
+    const size_t m = (size_t)std::max<int64_t>(4096, 2 * cap.n_cand) * 8;
+    HIP_TRY(ctx->miss_a.need(m));
+    HIP_TRY(ctx->miss_b.need(m));
   }
   // 2. the captured stages once without capture: every buffer reaches its
   // capacity here, so the capture allocates nothing
@@ -2331,6 +2653,15 @@ int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom) {
   }
   ctx->graph = g;
   ctx->graph_exec = x;
+  ctx->graph_tet_off = ctx->tet_off.p;  // where its final offsets land
+  if (R.cfg.learned) {
+    // the capture swapped the offset buffers without running: the current
+    // results are the pre-pass's, in the other ones (rpd_graph_run swaps
+    // back to graph_tet_off before a replay)
+    ctx->tet_off.swap(ctx->out_off);
+    ctx->rel_ids.swap(ctx->out_ids);
+    ctx->tet_count.swap(ctx->out_cnt);
+  }
   ctx->graph_cap = cap;
   ctx->graph_cfg = R.cfg;
   ctx->graph_headroom = headroom;
@@ -2355,6 +2686,18 @@ int rpd_graph_run(rpd_ctx* ctx, int32_t* replayed) {
     const rpd_run_cfg c = ctx->graph_cfg;
     return rpd_graph_build(ctx, &c, ctx->graph_headroom);
   }
+  const bool learned = ctx->graph_cfg.learned != 0;
+  if (learned && ctx->tet_off.p != ctx->graph_tet_off) {
+    // an eager learned run since the capture swapped the offset buffers
+    // the graph writes: swap them back
+    ctx->tet_off.swap(ctx->out_off);
+    ctx->rel_ids.swap(ctx->out_ids);
+    ctx->tet_count.swap(ctx->out_cnt);
+    if (ctx->tet_off.p != ctx->graph_tet_off) {
+      const rpd_run_cfg c = ctx->graph_cfg;
+      return rpd_graph_build(ctx, &c, ctx->graph_headroom);
+    }
+  }
   ctx->last_geometry = false;
   ctx->has_results = false;
   ctx->adj_ready = false;
@@ -2363,28 +2706,55 @@ int rpd_graph_run(rpd_ctx* ctx, int32_t* replayed) {
   HIP_TRY(hipGraphLaunch(ctx->graph_exec, s));
   HIP_TRY(hipEventRecord(ctx->ev[4], s));
   const RunCaps& cap = ctx->graph_cap;
-  const int64_t nb = ceil_div(cap.n_cand, kCB);
-  int64_t st64[4], drops[kDropSlots * 9], n_cand = 0, n_reg = 0;
-  int32_t cnt[8];
+  const int64_t nb = ceil_div(learned ? cap.incl : cap.n_cand, kCB);
+  int64_t st64[4], drops[kDropSlots * 9], n_cand = 0, n_reg = 0, n1 = 0;
+  int32_t cnt[8], reop[64], nf = 0;
+  int64_t gc[3 + 2 * kClosureRounds];
   Sizes3 tot{0, 0, 0};
-  HIP_TRY(read_back(
-      ctx, s,
-      {{st64, ctx->stats64.p, 32},
-       {drops, ctx->drops.p, sizeof(drops)},
-       {cnt, ctx->counters.p, 32},
-       {&tot, ctx->offs.as<Sizes3>() + nb, sizeof(Sizes3)},
-       {&n_cand, ctx->tet_off.as<int64_t>() + ctx->n_tet, 8},
-       {&n_reg, ctx->reg_off.as<int64_t>() + ctx->n_site, 8}}));
+  std::vector<D2H> items = {
+      {st64, ctx->stats64.p, 32},
+      {drops, ctx->drops.p, sizeof(drops)},
+      {cnt, ctx->counters.p, 32},
+      {&tot, ctx->offs.as<Sizes3>() + nb, sizeof(Sizes3)},
+      {&n_cand, ctx->tet_off.as<int64_t>() + ctx->n_tet, 8}};
+  if (learned) {  // first-round count, re-run tets, ranges, listed tets
+    items.push_back({&n1, ctx->out_off.as<int64_t>() + ctx->n_tet, 8});
+    items.push_back({reop, ctx->reop64.p, sizeof(reop)});
+    items.push_back({gc, ctx->gcnt.p, (size_t)(3 + 2 * cap.rounds) * 8});
+    items.push_back({&nf, ctx->flag_pos.as<int32_t>() + ctx->n_tet, 4});
+  } else {
+    items.push_back({&n_reg, ctx->reg_off.as<int64_t>() + ctx->n_site, 8});
+  }
+  HIP_TRY(read_back(ctx, s, items));
   const bool mid = small_caps(ctx->graph_cfg.small_caps).V < SmallCaps::V;
+  // (the re-run tiers of a learned run are checked on the device, launch by
+  // launch, into cnt[7])
+  const bool tiers_over =
+      !learned && ((mid ? (cnt[1] > cap.n_mid || cnt[6] > cap.n_full)
+                        : cnt[1] > cap.n_full) ||
+                   cnt[2] > cap.n_exact);
   const bool over =
-      cnt[7] != 0 || n_reg > cap.n_reg || n_cand > cap.n_cand ||
-      (mid ? (cnt[1] > cap.n_mid || cnt[6] > cap.n_full)
-           : cnt[1] > cap.n_full) ||
-      cnt[2] > cap.n_exact || tot.cells > cap.cells ||
-      tot.words > cap.words || tot.areas > cap.areas;
+      cnt[7] != 0 || tiers_over || n_reg > cap.n_reg ||
+      (learned ? (n1 > cap.n_cand || n_cand > cap.incl)
+               : n_cand > cap.n_cand) ||
+      tot.cells > cap.cells || tot.words > cap.words || tot.areas > cap.areas;
   if (over) {  // counts beyond the capture's capacities: capture again
+    std::string why;
+    if (learned && cap.rounds < kClosureRounds && cnt[4] > 0)
+      why += " closure rounds";
+    if (cnt[7] != 0) why += " device-side capacity";
+    if (tiers_over) why += " re-run tiers";
+    if (learned ? n1 > cap.n_cand : n_cand > cap.n_cand) why += " candidates";
+    if (learned && n_cand > cap.incl) why += " output candidates";
+    if (n_reg > cap.n_reg) why += " bin registrations";
+    if (tot.cells > cap.cells || tot.words > cap.words ||
+        tot.areas > cap.areas)
+      why += " output stream";
     const rpd_run_cfg c = ctx->graph_cfg;
-    return rpd_graph_build(ctx, &c, ctx->graph_headroom);
+    const int rc = rpd_graph_build(ctx, &c, ctx->graph_headroom);
+    if (rc == 0)  // (not an error: the reason of the new capture)
+      ctx->err = "rpd_graph_run: captured again, over capacity:" + why;
+    return rc;
   }
   rpd_stats& st = ctx->st;
   const int32_t site_k = st.site_k;
@@ -2402,6 +2772,16 @@ int rpd_graph_run(rpd_ctx* ctx, int32_t* replayed) {
   st.n_bin_registrations = n_reg;
   st.tet_k = (int32_t)st64[2];
   st.n_host_reads = 1;  // the counts, once, after the graph
+  if (learned) {
+    st.n_rerun_large = st.n_rerun_exact = 0;  // (list positions only)
+    int64_t re = 0, added = 0;
+    for (int q = 0; q < 64; ++q) re += reop[q];
+    for (int64_t r = 0; r <= cap.rounds; ++r) added += gc[1 + 2 * r];
+    st.learned[0] = n1;
+    st.learned[1] = re;
+    st.learned[2] = added;
+    st.learned[3] = nf;
+  }
   float ms = 0;
   st.ms_total =
       hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[4]) == hipSuccess ? ms : -1;

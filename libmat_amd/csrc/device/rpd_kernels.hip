@@ -60,6 +60,25 @@ __global__ __launch_bounds__(256) void k_fill32(int32_t* p, int64_t n,
   if (i < n) p[i] = v;
 }
 
+// A captured run's re-run lists: before a clip launch its counters start at
+// the list positions the host reserved (cnt[1] small-tier overflows, cnt[2]
+// exact re-runs, cnt[6] middle-tier overflows), and after its tiers the
+// counts are checked against what those reserved (*overflow otherwise).
+__global__ void k_set_rerun_counts(int32_t* cnt, int32_t r_large,
+                                   int32_t r_exact, int32_t r_full) {
+  if (threadIdx.x != 0) return;
+  cnt[1] = r_large;
+  cnt[2] = r_exact;
+  cnt[6] = r_full;
+}
+__global__ void k_check_rerun_counts(const int32_t* cnt, int32_t lim_large,
+                                     int32_t lim_exact, int32_t lim_full,
+                                     int32_t* overflow) {
+  if (threadIdx.x != 0) return;
+  if (cnt[1] > lim_large || cnt[2] > lim_exact || cnt[6] > lim_full)
+    *overflow = 1;
+}
+
 // --------------------------------------------------------- predicates
 // det4x4(p1.x, p2.x, p3.x, e.x, p1.y, ..., p1.w, p2.w, p3.w, e.w) in the
 // reference's evaluation order (convex_cell.cu:458-459,
@@ -616,10 +635,10 @@ __global__ __launch_bounds__(256) void k_tet_fill(
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= M.n_tet) return;
   if (tet_sel && !tet_sel[t]) return;
-  if (tet_off[t + 1] > cap) {  // over a captured pipeline's capacity
-    if (tet_off[t] < cap || t == 0) overflow[0] = 1;
-    return;
-  }
+  // over a captured pipeline's capacity: the entries below it are still
+  // written (every slot a later launch reads holds a pair of this run)
+  if (tet_off[t + 1] > cap && (tet_off[t] < cap || t == 0)) overflow[0] = 1;
+  if (tet_off[t] >= cap) return;
   TetLoad L;
   load_tet(M, t, L);
   double c[3];
@@ -639,6 +658,7 @@ __global__ __launch_bounds__(256) void k_tet_fill(
                    : d < 128 ? ((pass1 >> (d - 64)) & 1)
                              : relation(S, L, i, &iters);
     if (!r) continue;
+    if (o >= cap) break;
     Cand c;
     c.tet = t;
     c.seed = i;

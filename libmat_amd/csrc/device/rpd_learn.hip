@@ -932,12 +932,17 @@ __global__ __launch_bounds__(64) void k_learn_count(int32_t n_tet, DevMesh M,
   count[t] = n;
 }
 
+// (a captured run: the candidate lists hold cap entries; a tet whose
+// candidates pass them sets *overflow and writes those below cap, so that
+// every slot a later launch reads holds a pair of this run)
 __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
                                                    const int32_t* knn,
                                                    const uint32_t* sel,
                                                    const int64_t* off,
                                                    Cand* cand,
-                                                   int32_t* rel_ids) {
+                                                   int32_t* rel_ids,
+                                                   int64_t cap,
+                                                   int32_t* overflow) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   // (no early return: the whole wavefront takes part in the reduction)
   const uint32_t m = t < n_tet ? sel[t] : 0u;
@@ -947,6 +952,7 @@ __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
   for (int d = 32; d > 0; d >>= 1) n_wave = max(n_wave, __shfl_xor(n_wave, d));
   if (t >= n_tet) return;
   const int64_t o = off[t];
+  if (o + n > cap) *overflow = 1;
   if (n_wave <= 8) {
     // small sets (the usual case): ids inserted one by one into 8 sorted
     // registers, each slot updated in place to the median of its
@@ -963,7 +969,7 @@ __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      if (k < n) {
+      if (k < n && o + k < cap) {
         Cand c;
         c.tet = t;
         c.seed = w[k];
@@ -1003,7 +1009,7 @@ __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
       }
 #pragma unroll
   for (int k = 0; k < kOut; ++k) {
-    if (k < n) {
+    if (k < n && o + k < cap) {
       Cand c;
       c.tet = t;
       c.seed = v[k];
@@ -1018,18 +1024,23 @@ __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
 // tets of missing pairs left after the last closure round
 __global__ __launch_bounds__(256) void k_flag_miss(int64_t n,
                                                    const uint64_t* miss,
-                                                   uint8_t* flag) {
+                                                   uint8_t* flag,
+                                                   const int32_t* n_dev) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) flag[miss[q] >> 32] = 1;
+  const int64_t m = n_dev ? min(n, (int64_t)*n_dev) : n;
+  if (q < m) flag[miss[q] >> 32] = 1;
 }
 
-// sorted keys -> first occurrences (1) / repeats (0)
+// sorted keys -> first occurrences (1) / repeats (0); with n_dev only the
+// first min(*n_dev, n) keys count (a captured run sorts a padded list)
 __global__ __launch_bounds__(256) void k_unique_mark(int64_t n,
                                                      const uint64_t* keys,
-                                                     int32_t* mark) {
+                                                     int32_t* mark,
+                                                     const int32_t* n_dev) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q > n) return;
-  mark[q] = q < n && (q == 0 || keys[q] != keys[q - 1]) ? 1 : 0;
+  const int64_t m = n_dev ? min(n, (int64_t)*n_dev) : n;
+  mark[q] = q < m && (q == 0 || keys[q] != keys[q - 1]) ? 1 : 0;
 }
 
 // unique missing pairs: the relation decides which become candidates
@@ -1048,15 +1059,17 @@ __global__ __launch_bounds__(256) void k_closure_split(
   pass[u] = relation(S, L, (int)(k & 0xffffffffu), &it) ? 1 : 0;
 }
 
-__global__ __launch_bounds__(256) void k_closure_fill(int64_t n_u,
-                                                      const uint64_t* ukeys,
-                                                      const uint8_t* pass,
-                                                      const int32_t* ppos,
-                                                      Cand* cand,
-                                                      int64_t cand_base,
-                                                      uint8_t* reopened) {
+// (a captured run: n_u on the device, at most cap new candidates)
+__global__ __launch_bounds__(256) void k_closure_fill(
+    int64_t n_u, const uint64_t* ukeys, const uint8_t* pass,
+    const int32_t* ppos, Cand* cand, int64_t cand_base, uint8_t* reopened,
+    const int32_t* n_u_dev, int64_t cap, int32_t* overflow) {
   const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= n_u || !pass[u]) return;
+  if (u >= n_u || (n_u_dev && u >= *n_u_dev) || !pass[u]) return;
+  if (ppos[u] >= cap) {
+    *overflow = 1;
+    return;
+  }
   reopened[ukeys[u] >> 32] = 1;
   Cand c;
   c.tet = (int)(ukeys[u] >> 32);
@@ -1101,20 +1114,28 @@ __global__ __launch_bounds__(256) void k_flag_final(int32_t n_tet,
 __global__ __launch_bounds__(256) void k_flag_list(int32_t n_tet,
                                                    const int32_t* flag32,
                                                    const int32_t* pos,
-                                                   int32_t* list) {
+                                                   int32_t* list, int64_t cap,
+                                                   int32_t* overflow) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_tet) return;
-  if (flag32[t]) list[pos[t]] = t;
+  if (t >= n_tet || !flag32[t]) return;
+  if (pos[t] >= cap) {
+    *overflow = 1;
+    return;
+  }
+  list[pos[t]] = t;
 }
 
 // Fallback: the reference relation of each listed tet against every sphere
 // of its instance; one wave per tet, sites in ascending order.
+// (a captured run: the list length on the device, candidates written
+// below cand_end only)
 __global__ __launch_bounds__(64) void k_brute(
     int32_t n_list, const int32_t* list, DevMesh M, DevSites S,
     const InstInfo* inst, const int32_t* tet_inst, const int64_t* off,
-    int64_t* count, Cand* cand, int64_t cand_base) {
+    int64_t* count, Cand* cand, int64_t cand_base, const int32_t* n_list_dev,
+    int64_t cand_end) {
   const int f = blockIdx.x;
-  if (f >= n_list) return;
+  if (f >= n_list || (n_list_dev && f >= *n_list_dev)) return;
   const int lane = threadIdx.x;
   const int t = list[f];
   TetLoad L;
@@ -1130,13 +1151,14 @@ __global__ __launch_bounds__(64) void k_brute(
       r = relation(S, L, s, &it);
     }
     const uint64_t b = __ballot(r);
-    if (off && r) {
+    const int64_t at = o + n + __popcll(b & ((1ull << lane) - 1));
+    if (off && r && at < cand_end) {
       Cand c;
       c.tet = t;
       c.seed = s;
       c.m0 = 0;
       c.m1 = 0;
-      cand[o + n + __popcll(b & ((1ull << lane) - 1))] = c;
+      cand[at] = c;
     }
     n += __popcll(b);
   }
@@ -1194,12 +1216,15 @@ __global__ __launch_bounds__(256) void k_out_counts(
   count[t] = na + nb;
 }
 
+// (a captured run: n1 on the device; positions at or past perm_cap, which
+// the host then finds over capacity, are not written)
 __global__ __launch_bounds__(256) void k_place_a(
     int64_t n1, const Cand* cand, const uint8_t* flag,
     const uint8_t* reopened, const int64_t* a_off, const uint64_t* bkeys,
-    int64_t n_b, const int64_t* out_off, int32_t* perm) {
+    int64_t n_b, const int64_t* out_off, int32_t* perm, const int64_t* n1_dev,
+    int64_t perm_cap) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n1) return;
+  if (c >= n1 || (n1_dev && c >= *n1_dev)) return;
   const Cand cd = cand[c];
   if (flag[cd.tet]) return;
   int64_t before_b = 0;
@@ -1208,13 +1233,14 @@ __global__ __launch_bounds__(256) void k_place_a(
     before_b = lower_u64(bkeys, n_b, k0 | (uint32_t)cd.seed) -
                lower_u64(bkeys, n_b, k0);
   }
-  perm[out_off[cd.tet] + (c - a_off[cd.tet]) + before_b] = (int32_t)c;
+  const int64_t at = out_off[cd.tet] + (c - a_off[cd.tet]) + before_b;
+  if (at < perm_cap) perm[at] = (int32_t)c;
 }
 
 __global__ __launch_bounds__(256) void k_place_b(
     int64_t n_b, const uint64_t* bkeys, const int32_t* bvals,
     const uint8_t* flag, const int64_t* a_off, const int32_t* a_ids,
-    const int64_t* out_off, int32_t* perm) {
+    const int64_t* out_off, int32_t* perm, int64_t perm_cap) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_b) return;
   const uint64_t k = bkeys[r];
@@ -1235,15 +1261,113 @@ __global__ __launch_bounds__(256) void k_place_b(
     }
     before_a = lo - base;
   }
-  perm[out_off[t] + (r - start) + before_a] = bvals[r];
+  const int64_t at = out_off[t] + (r - start) + before_a;
+  if (at < perm_cap) perm[at] = bvals[r];
 }
 
-// relation CSR ids of the output order
+// relation CSR ids of the output order (a captured run: the count on the
+// device)
 __global__ __launch_bounds__(256) void k_out_ids(int64_t n, const Cand* cand,
                                                  const int32_t* perm,
-                                                 int32_t* ids) {
+                                                 int32_t* ids,
+                                                 const int64_t* n_dev) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < n) ids[r] = cand[perm[r]].seed;
+  if (r < n && (!n_dev || r < *n_dev)) ids[r] = cand[perm[r]].seed;
+}
+
+// ------------------------------------------ captured learned pipeline
+// Small helpers of the captured learned run, whose counts stay on the
+// device: lists are sized by capacities and padded, and every count that
+// passes its capacity sets *overflow (the host then runs eagerly and
+// captures again).
+
+// p[i] = ~0 for min(*n, cap) <= i < cap (a padded list sorts its real
+// entries first)
+template <typename T>
+__global__ __launch_bounds__(256) void k_pad64(uint64_t* p, int64_t cap,
+                                               const T* n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cap && i >= min(cap, (int64_t)*n)) p[i] = ~0ull;
+}
+
+// a range's count n_src (clamped to cap, *overflow past it) and its end
+// base + count
+template <typename T>
+__global__ void k_range_count(const T* n_src, int64_t base, int64_t cap,
+                              int64_t* rel, int64_t* abs_end,
+                              int32_t* overflow) {
+  if (threadIdx.x != 0) return;
+  int64_t n = (int64_t)*n_src;
+  if (n > cap) {
+    *overflow = 1;
+    n = cap;
+  }
+  *rel = n;
+  *abs_end = base + n;
+}
+
+// dst[*n_dst + u] = src[u] for u < *n_src (at most cap entries in dst)
+__global__ __launch_bounds__(256) void k_append64(
+    uint64_t* dst, const int64_t* n_dst, const uint64_t* src,
+    const int32_t* n_src, int64_t n, int64_t cap, int32_t* overflow) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n || u >= *n_src) return;
+  const int64_t i = *n_dst + u;
+  if (i >= cap) {
+    *overflow = 1;
+    return;
+  }
+  dst[i] = src[u];
+}
+
+// *n_dst += *n_src, clamped to cap
+__global__ void k_add_count(int64_t* n_dst, const int32_t* n_src,
+                            int64_t cap) {
+  if (threadIdx.x == 0) *n_dst = min(cap, *n_dst + (int64_t)*n_src);
+}
+
+// after an overflow (whose results are discarded) the first-round CSR is
+// cut to the capacity, so that no later launch reads past its lists
+__global__ __launch_bounds__(256) void k_clamp_offsets(int64_t* off,
+                                                       int64_t n, int64_t cap,
+                                                       const int32_t* overflow) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t <= n && *overflow) off[t] = min(off[t], cap);
+}
+
+// *overflow = 1 when *n > 0 (misses left after the last captured round)
+__global__ void k_flag_if_positive(const int32_t* n, int32_t* overflow) {
+  if (threadIdx.x == 0 && *n > 0) *overflow = 1;
+}
+
+// Output keys of the added candidates of a captured run: closure round r
+// holds its candidates at [n1_cap + r * round_cap, + rel[2 r]), the
+// fallback at [fb_base, + rel[2 rounds]) (rel: the device counts); slots
+// past a count are excluded (~0), and so are closure candidates of tets
+// re-run on the full relation, as in k_b_keys.
+__global__ __launch_bounds__(256) void k_b_keys_ranges(
+    int64_t n1_cap, int64_t round_cap, int32_t rounds, int64_t fb_base,
+    int64_t end, const int64_t* rel, const Cand* cand, const uint8_t* flag,
+    uint64_t* keys, int32_t* vals) {
+  const int64_t c = n1_cap + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= end) return;
+  bool valid, closure;
+  if (c < fb_base) {
+    const int64_t r = (c - n1_cap) / round_cap;
+    valid = r < rounds && c - (n1_cap + r * round_cap) < rel[2 * r];
+    closure = true;
+  } else {
+    valid = c - fb_base < rel[2 * rounds];
+    closure = false;
+  }
+  uint64_t k = ~0ull;
+  if (valid) {
+    const Cand cd = cand[c];
+    if (!(closure && flag[cd.tet]))
+      k = ((uint64_t)(uint32_t)cd.tet << 32) | (uint32_t)cd.seed;
+  }
+  keys[c - n1_cap] = k;
+  vals[c - n1_cap] = (int32_t)c;
 }
 
 }  // namespace rpd

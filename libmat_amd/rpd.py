@@ -411,11 +411,14 @@ class Context:
         return self.stats()
 
     def graph_build(self, headroom=0.1, cells_per_wave=8, geometry=True,
-                    small_caps=0):
-        """Capture the plain pipeline of the resident batch into a hipGraph
-        (rpd_graph_build); the eager run it makes gives the results."""
+                    small_caps=0, learned=0, logit_threshold=0.0,
+                    proposal_k=8):
+        """Capture the pipeline of the resident batch (plain, or learned
+        with learned = 1 / 2) into a hipGraph (rpd_graph_build); the eager
+        run it makes gives the results."""
         cfg = RunCfg(cells_per_wave, 1, int(geometry), 0, 1, 0,
-                     int(small_caps), 0, 0.0, 8)
+                     int(small_caps), int(learned), float(logit_threshold),
+                     int(proposal_k))
         self._check(self._L.rpd_graph_build(self._h, ctypes.byref(cfg),
                                             float(headroom)),
                     "rpd_graph_build")

@@ -1,9 +1,11 @@
-"""GPU: the plain pipeline of a resident batch captured into a hipGraph
-(rpd_graph_build / rpd_graph_run, through the C-ABI).  Every replay must
-give bit for bit the cells of an eager rpd_run on the same data: after the
-capture, after sphere updates that keep each instance's site count (a
-replay), and after updates that change it or outgrow the capacities (a new
-capture, whose eager run gives the results).
+"""GPU: the pipeline of a resident batch captured into a hipGraph
+(rpd_graph_build / rpd_graph_run, through the C-ABI), plain and learned
+(certificate, closure rounds, fallback and output order on the device).
+Every replay must give bit for bit the cells (and the relation CSR) of an
+eager rpd_run on the same data: after the capture, after sphere updates
+that keep each instance's site count (a replay), and after updates that
+change it or outgrow the capacities (a new capture, whose eager run gives
+the results).
 """
 import numpy as np
 import pytest
@@ -135,6 +137,84 @@ def test_graph_with_a_tet_subset(ctx):
     ctx.graph_destroy()
 
 
+def _same_relation(ctx, ref_rel):
+    off, ids = ctx.relation()
+    assert np.array_equal(off, ref_rel[0]) and np.array_equal(ids, ref_rel[1])
+
+
+@pytest.mark.parametrize("mode", ["topk", "model"])
+def test_learned_graph_replays_equal_eager_runs(ctx, workload, mode):
+    w = workload
+    ctx.upload_batch(w.instances)
+    if mode == "model":
+        ctx.mlp_init(3)
+        kw = dict(learned=1, logit_threshold=0.0)
+    else:
+        kw = dict(learned=2, proposal_k=4)
+    base = _sets(w)
+    st0, ref0 = _eager(ctx, base, **kw)
+    rel0 = ctx.relation()
+    # the plain path: the same cells
+    _, plain = _eager(ctx, base)
+    _same(plain, ref0)
+    print(f"[graph] learned {mode}: cells={st0['n_cells']} "
+          f"learned={st0['learned']}", flush=True)
+    ctx.graph_build(headroom=0.2, **kw)
+    _same(ctx.download(), ref0)
+    _same_relation(ctx, rel0)
+    st, rep = ctx.graph_run()
+    assert rep
+    assert st["n_cells"] == st0["n_cells"]
+    assert st["n_host_reads"] == 1
+    assert st["learned"][0] == st0["learned"][0]
+    _same(ctx.download(), ref0)
+    _same_relation(ctx, rel0)
+    # sphere updates with eager learned runs in between (they swap the
+    # offset buffers the graph writes): replays
+    replays = 0
+    for it in (1, 2):
+        sets = _jittered(w, it, keep_counts=True)
+        st_e, ref = _eager(ctx, sets, **kw)
+        rel = ctx.relation()
+        ctx.update_sites_batch(sets)
+        st, rep = ctx.graph_run()
+        replays += rep
+        why = "" if rep else ctx._L.rpd_last_error(ctx._h).decode()
+        print(f"[graph] learned {mode} iteration {it}: replayed {rep} cells "
+              f"{st['n_cells']} learned {st['learned']} ms "
+              f"{st['ms_total']:.3f} {why}", flush=True)
+        assert st["n_cells"] == st_e["n_cells"]
+        _same(ctx.download(), ref)
+        _same_relation(ctx, rel)
+    assert replays == 2
+    ctx.graph_destroy()
+
+
+def test_learned_graph_recaptures_when_counts_grow(ctx, workload):
+    w = workload
+    ctx.upload_batch(w.instances)
+    kw = dict(learned=2, proposal_k=2)  # few proposals: closure rounds
+    ctx.update_sites_batch(_sets(w))
+    ctx.graph_build(headroom=0.0, **kw)
+    big = []
+    for x in w.spheres:
+        y = x.copy()
+        y[:, 3] *= 1.5
+        big.append(prep.sites_of(y)[:4])
+    st_e, ref = _eager(ctx, big, **kw)
+    rel = ctx.relation()
+    ctx.update_sites_batch(big)
+    st, rep = ctx.graph_run()
+    print(f"[graph] learned, larger spheres: replayed {rep}", flush=True)
+    assert st["n_cells"] == st_e["n_cells"]
+    _same(ctx.download(), ref)
+    _same_relation(ctx, rel)
+    st, rep = ctx.graph_run()  # the new capture replays
+    assert rep
+    _same(ctx.download(), ref)
+    ctx.graph_destroy()
+
+
 def test_graph_rejects_other_paths(ctx):
     inst = prep.cube10_instance(0)
     ctx.upload(inst)
@@ -143,7 +223,7 @@ def test_graph_rejects_other_paths(ctx):
         ctx.graph_run()
     from libmat_amd.rpd import RunCfg
     import ctypes
-    cfg = RunCfg(8, 1, 1, 0, 1, 0, 0, 1, 0.0, 8)  # learned
+    cfg = RunCfg(8, 1, 1, 0, 1, 1, 0, 0, 0.0, 8)  # work counters
     rc = ctx._L.rpd_graph_build(ctx._h, ctypes.byref(cfg), 0.1)
     assert rc != 0
-    assert "plain path" in ctx._L.rpd_last_error(ctx._h).decode()
+    assert "work counters" in ctx._L.rpd_last_error(ctx._h).decode()
