@@ -453,6 +453,13 @@ def bench_batched(a, emit=True):
             "train_s": t_train, "kernel_ms": lkern}
         print(f"[{a.workload}] learned+batched timed", file=sys.stderr,
               flush=True)
+        if not a.learned_only and not a.no_graph:
+            # the learned step captured: certificate, closure rounds and
+            # output order with their counts on the device
+            learned["graph"] = bench_graph(ctx, sets_all, a.warmup, G, caps,
+                                           learned=1, logit_threshold=thr)
+            print(f"[{a.workload}] captured learned graph timed",
+                  file=sys.stderr, flush=True)
     ctx.close()
     if a.calib_copy_gib > 0:  # the copy kernel that calibrates PMC counters
         with Context(0) as cc:
@@ -508,14 +515,15 @@ def bench_batched(a, emit=True):
     return out
 
 
-def bench_graph(ctx, sets, warmup, G, caps):
-    """The plain batched step as one replay of a captured hipGraph
-    (rpd_graph_build / rpd_graph_run) against eager rpd_run on the same
-    site sets (every sphere a site, so instances keep their site counts
-    across iterations); both timed with HIP events around the device work.
-    A replay whose counts outgrow the capture's capacities, or a changed
-    batch, is captured again (counted in "recaptured")."""
-    kw = dict(cells_per_wave=G, small_caps=caps)
+def bench_graph(ctx, sets, warmup, G, caps, **path):
+    """The batched step (plain, or learned with path = learned=1, ...) as
+    one replay of a captured hipGraph (rpd_graph_build / rpd_graph_run)
+    against eager rpd_run on the same site sets (every sphere a site, so
+    instances keep their site counts across iterations); both timed with
+    HIP events around the device work.  A replay whose counts outgrow the
+    capture's capacities, or a changed batch, is captured again (counted in
+    "recaptured")."""
+    kw = dict(cells_per_wave=G, small_caps=caps, **path)
     ems, ecells, _, _ = time_batch(ctx, sets, warmup, **kw)
     if sets[0] is not None:
         ctx.update_sites_batch(sets[0])
@@ -745,6 +753,10 @@ def bench_cube(a, emit=True):
                           "compaction": lk[6], "certificate": lk[7]},
             "clip_alg_gbs": clip_bytes(lst, inst) / (lk[5] * 1e-3) / 1e9,
         }
+        if not a.no_graph:  # the learned step captured (one host read)
+            learned["graph"] = bench_graph(ctx, [None] * a.steps, a.warmup,
+                                           G, caps, learned=1,
+                                           logit_threshold=thr)
     # the regular-triangulation neighbours of the same sphere set: the host
     # builder against the device builder (same exact predicates)
     xyzw = (prep.spheres(50000, 1, 0) if a.workload == "cube64"

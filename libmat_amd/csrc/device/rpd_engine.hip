@@ -212,6 +212,7 @@ struct TierBufs {
 struct RunCaps {
   int64_t n_reg = 0, n_cand = 0, n_mid = 0, n_full = 0, n_exact = 0;
   int64_t cells = 0, words = 0, areas = 0;
+  int64_t n_knn = 0;  // neighbour-list entries the capture reserved
   // learned path: closure rounds run; the largest miss list, unique pairs
   // and new candidates of a round; the merged pairs, tets re-run on the
   // full relation, their candidates and the output candidates; the
@@ -1051,11 +1052,7 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
         read_back(ctx, s, {{&n_cand, ctx->tet_off.as<int64_t>() + nt, 8}}));
   R.seen.n_cand = n_cand;
   HIP_TRY(ctx->cand.need((size_t)std::max<int64_t>(n_cand, 1) * sizeof(Cand)));
-  // (a captured run swaps rel_ids with the output ids every run: both hold
-  // the larger of the two capacities, so no capture reallocates them)
-  const int64_t n_ids =
-      R.graph ? std::max(n_cand, R.cap.incl) : std::max<int64_t>(n_cand, 1);
-  HIP_TRY(ctx->rel_ids.need((size_t)n_ids * 4));
+  HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
   hipLaunchKernelGGL(k_tet_fill, dim3(ceil_div(nt, 256)), dim3(256), 0, s, R.M,
                      R.S, R.inst_d, R.tet_inst_d, ctx->bin_start.as<int32_t>(),
                      ctx->bin_ids.as<int32_t>(), ctx->tet_off.as<int64_t>(),
@@ -1071,11 +1068,14 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
 
 // neighbour spheres inline in list order (descent and relation read them)
 int site_inline(rpd_ctx* ctx, RunState& R) {
-  if (ctx->n_knn > 0)
-    hipLaunchKernelGGL(k_site_inline, dim3(ceil_div(ctx->n_knn, 256)),
-                       dim3(256), 0, R.s, R.S, ctx->n_knn,
-                       ctx->knn_near.as<int32_t>(),
-                       ctx->knn_site_near.as<double4>(), nullptr);
+  // (a captured run: the lists' length is knn_off[n_site] on the device,
+  // up to the capacity the capture reserved -- sphere updates change it)
+  const int64_t n = R.graph ? R.cap.n_knn : ctx->n_knn;
+  if (n > 0)
+    hipLaunchKernelGGL(k_site_inline, dim3(ceil_div(n, 256)), dim3(256), 0,
+                       R.s, R.S, n, ctx->knn_near.as<int32_t>(),
+                       ctx->knn_site_near.as<double4>(),
+                       R.graph ? ctx->knn_off.as<int64_t>() + R.ns : nullptr);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1175,7 +1175,11 @@ int learned_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
         read_back(ctx, s, {{&n_cand, ctx->tet_off.as<int64_t>() + nt, 8}}));
   R.seen.n_cand = n_cand;
   HIP_TRY(ctx->cand.need((size_t)std::max<int64_t>(n_cand, 1) * sizeof(Cand)));
-  HIP_TRY(ctx->rel_ids.need((size_t)std::max<int64_t>(n_cand, 1) * 4));
+  // (a captured run swaps rel_ids with the output ids every run: both hold
+  // the larger of the two capacities, so no capture reallocates them)
+  const int64_t n_ids =
+      R.graph ? std::max(n_cand, R.cap.incl) : std::max<int64_t>(n_cand, 1);
+  HIP_TRY(ctx->rel_ids.need((size_t)n_ids * 4));
   hipLaunchKernelGGL(k_learn_fill, dim3(ceil_div(nt, 64)), dim3(64), 0, s, nt,
                      ctx->knn.as<int32_t>(), ctx->sel.as<uint32_t>(),
                      ctx->tet_off.as<int64_t>(), ctx->cand.as<Cand>(),
@@ -2598,6 +2602,7 @@ int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom) {
   // sphere updates: room for them too, so that those keep their addresses
   {
     const size_t nk = (size_t)std::max<int64_t>(grow(ctx->n_knn, f, 1024), 1);
+    cap.n_knn = (int64_t)nk;
     HIP_TRY(ctx->knn_bis.need(nk * 32));
     HIP_TRY(ctx->knn_ids.need_keep(nk * 4, (size_t)ctx->n_knn * 4,
                                    ctx->stream));

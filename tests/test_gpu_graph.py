@@ -142,8 +142,21 @@ def _same_relation(ctx, ref_rel):
     assert np.array_equal(off, ref_rel[0]) and np.array_equal(ids, ref_rel[1])
 
 
-@pytest.mark.parametrize("mode", ["topk", "model"])
-def test_learned_graph_replays_equal_eager_runs(ctx, workload, mode):
+def _every(w, it):
+    """Every sphere a site (hidden ones unflagged, with empty lists), after
+    jitter iteration it (0: the original spheres): site counts stay fixed,
+    the neighbour lists' total length changes."""
+    out = []
+    for b, x in enumerate(w.spheres):
+        y = x if it == 0 else prep.jitter_spheres(x, 0.002, 7919 * it + b)
+        out.append(prep.sites_both(y)[1])
+    return out
+
+
+@pytest.mark.parametrize("mode,sites", [("topk", "visible"),
+                                        ("model", "visible"),
+                                        ("model", "every")])
+def test_learned_graph_replays_equal_eager_runs(ctx, workload, mode, sites):
     w = workload
     ctx.upload_batch(w.instances)
     if mode == "model":
@@ -151,7 +164,7 @@ def test_learned_graph_replays_equal_eager_runs(ctx, workload, mode):
         kw = dict(learned=1, logit_threshold=0.0)
     else:
         kw = dict(learned=2, proposal_k=4)
-    base = _sets(w)
+    base = _sets(w) if sites == "visible" else _every(w, 0)
     st0, ref0 = _eager(ctx, base, **kw)
     rel0 = ctx.relation()
     # the plain path: the same cells
@@ -173,7 +186,8 @@ def test_learned_graph_replays_equal_eager_runs(ctx, workload, mode):
     # offset buffers the graph writes): replays
     replays = 0
     for it in (1, 2):
-        sets = _jittered(w, it, keep_counts=True)
+        sets = (_jittered(w, it, keep_counts=True) if sites == "visible"
+                else _every(w, it))
         st_e, ref = _eager(ctx, sets, **kw)
         rel = ctx.relation()
         ctx.update_sites_batch(sets)
