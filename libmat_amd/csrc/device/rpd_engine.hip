@@ -1249,13 +1249,8 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
   const int64_t cap_mid = first_call ? R.cap.n_mid : R.cap.c_mid;
   const int64_t cap_full = first_call ? R.cap.n_full : R.cap.c_full;
   const int64_t cap_exact = first_call ? R.cap.n_exact : R.cap.c_exact;
-  const int32_t r_large0 = R.r_large, r_exact0 = R.r_exact,
-                r_full0 = R.r_full;
-  if (R.graph) {
-    hipLaunchKernelGGL(k_set_rerun_counts, dim3(1), dim3(64), 0, s, R.cnt,
-                       R.r_large, R.r_exact, R.r_full);
-    HIP_TRY(hipGetLastError());
-  }
+  // (a captured run's counters start each launch at R.r_large, R.r_exact,
+  // R.r_full: zero for the first, set by the previous launch's check)
   // the small tier takes its cells in decreasing order of clip steps (a
   // stable sort of the step counts), so that the cells of one wavefront
   // need about as many steps as each other
@@ -1448,18 +1443,20 @@ int clip_range(rpd_ctx* ctx, RunState& R, int64_t c0, int64_t c1,
                        R.r_exact, n_exact, 2, ctx->loc.as<int32_t>(),
                        R.graph ? R.cnt + 2 : nullptr);
   HIP_TRY(hipGetLastError());
-  if (R.graph) {  // this launch's re-run counts within what it reserved
-    hipLaunchKernelGGL(k_check_rerun_counts, dim3(1), dim3(64), 0, s, R.cnt,
-                       n_over, n_exact, mid ? full_r1 : 0x7fffffff,
-                       R.cnt + 7);
-    HIP_TRY(hipGetLastError());
-  }
-  (void)r_large0;
-  (void)r_exact0;
-  (void)r_full0;
   R.r_large = n_over;
   if (mid) R.r_full = full_r1;
   R.r_exact = n_exact;
+  if (R.graph) {
+    // this launch's re-run counts within what it reserved; a learned run's
+    // next launch starts at the next reserved positions (the plain run has
+    // one launch, whose counts the host reads)
+    const bool next = R.cfg.learned != 0;
+    hipLaunchKernelGGL(k_check_rerun_counts, dim3(1), dim3(64), 0, s, R.cnt,
+                       n_over, n_exact, mid ? full_r1 : 0x7fffffff,
+                       R.cnt + 7, next ? R.r_large : -1, R.r_exact,
+                       R.r_full);
+    HIP_TRY(hipGetLastError());
+  }
   if (events) HIP_TRY(hipEventRecord(ctx->ev[3], s));
   return 0;
 }
@@ -1494,7 +1491,8 @@ int sort_u64(rpd_ctx* ctx, hipStream_t s, const uint64_t* in, uint64_t* out,
 
 // certificate state for a round of n new candidates: a cleared miss list
 // of capacity max(4096, 2n) and the current ext keys
-int certify_round(rpd_ctx* ctx, RunState& R, int64_t n, int64_t n_ext) {
+int certify_round(rpd_ctx* ctx, RunState& R, int64_t n, int64_t n_ext,
+                  const int64_t* cap_n = nullptr) {
   // (a captured run: the capacity of its miss lists)
   const int64_t cap = R.graph ? R.cap.miss : std::max<int64_t>(4096, 2 * n);
   HIP_TRY(ctx->miss_a.need((size_t)cap * 8));
@@ -1505,6 +1503,7 @@ int certify_round(rpd_ctx* ctx, RunState& R, int64_t n, int64_t n_ext) {
   R.cert.miss = ctx->miss_a.as<uint64_t>();
   R.cert.cap = (int32_t)std::min<int64_t>(cap, INT32_MAX);
   R.cert.n_miss = R.cnt + 4;
+  R.cert.cap_n = R.graph ? cap_n : nullptr;
   return 0;
 }
 
@@ -1525,7 +1524,7 @@ int certify_begin(rpd_ctx* ctx, RunState& R, int64_t n1) {
   R.cert.mem_ids = ctx->rel_ids.as<int32_t>();
   R.cert.flag = ctx->flag.as<uint8_t>();
   R.cert.has_cell = ctx->has_cell.as<uint8_t>();
-  return certify_round(ctx, R, n1, 0);
+  return certify_round(ctx, R, n1, 0, ctx->tet_off.as<int64_t>() + R.nt);
 }
 
 int certify_captured(rpd_ctx* ctx, RunState& R, int64_t n1,
@@ -1751,9 +1750,9 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
 // last round; every count past its capacity sets the overflow counter
 // (cnt[7]), and misses left after the last captured round do too unless
 // that round is the last the eager path runs.  The output order is the
-// eager one: the same (tet, site) pairs in the same merge.  Device counts:
-// gcnt[0] merged pairs, gcnt[1 + 2 r] / gcnt[2 + 2 r] the count and the end
-// of range r (r = rounds: the fallback).
+// eager one: the same (tet, site) pairs in the same merge.  Device counts: gcnt[1 + 2 r] / gcnt[2 + 2 r] the count and the
+// end of range r (r = rounds: the fallback), gcnt[3 + 2 rounds + r] the
+// merged pairs before round r.
 int certify_captured(rpd_ctx* ctx, RunState& R, int64_t n1_cap,
                      int64_t* n_incl_out) {
   hipStream_t s = R.s;
@@ -1764,9 +1763,10 @@ int certify_captured(rpd_ctx* ctx, RunState& R, int64_t n1_cap,
   const int64_t cap_nf = C.flagged, cap_fb = C.fb, cap_incl = C.incl;
   int32_t* ovf = R.cnt + 7;
   auto grid = [](int64_t n) { return dim3(ceil_div(std::max<int64_t>(n, 1), 256)); };
-  const int64_t n_gc = 3 + 2 * (int64_t)rounds;
+  const int64_t n_gc = 4 + 3 * (int64_t)rounds;
   HIP_TRY(ctx->gcnt.need((size_t)n_gc * 8));
   int64_t* gc = ctx->gcnt.as<int64_t>();
+  int64_t* ext_n = gc + 3 + 2 * rounds;  // merged pairs before round r
   HIP_TRY(zero32(s, gc, 2 * n_gc));
   HIP_TRY(ctx->flag32.need(((size_t)nt + 1) * 4));
   HIP_TRY(ctx->flag_pos.need(((size_t)nt + 1) * 4));
@@ -1782,18 +1782,22 @@ int certify_captured(rpd_ctx* ctx, RunState& R, int64_t n1_cap,
   HIP_TRY(ctx->cand.need_keep((size_t)c_end * sizeof(Cand),
                               (size_t)n1_cap * sizeof(Cand), s));
   const int32_t* nm = R.cnt + 4;
+  int32_t* n_eff = R.cnt + 5;  // the misses listed
   const int32_t* n_u = ctx->upos.as<int32_t>() + cap_miss;
+  // candidates of the launch whose misses are listed
+  const int64_t* launch_n = ctx->tet_off.as<int64_t>() + nt;
   int64_t c_hi = n1_cap;
   for (int r = 0; r < rounds; ++r) {
     // unique missing pairs of the round (a padded sort of the miss list)
-    hipLaunchKernelGGL(k_pad64<int32_t>, grid(cap_miss), dim3(256), 0, s,
-                       ctx->miss_a.as<uint64_t>(), cap_miss, nm);
+    hipLaunchKernelGGL(k_pad_miss, grid(cap_miss), dim3(256), 0, s,
+                       ctx->miss_a.as<uint64_t>(), cap_miss, nm, launch_n,
+                       n_eff, ovf);
     if (int rc = sort_u64(ctx, s, ctx->miss_a.as<uint64_t>(),
                           ctx->miss_b.as<uint64_t>(), cap_miss))
       return rc;
     hipLaunchKernelGGL(k_unique_mark, grid(cap_miss + 1), dim3(256), 0, s,
                        cap_miss, ctx->miss_b.as<uint64_t>(),
-                       ctx->umark.as<int32_t>(), nm);
+                       ctx->umark.as<int32_t>(), (const int32_t*)n_eff);
     HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->umark.as<int32_t>(),
                                    ctx->upos.as<int32_t>(), cap_miss + 1));
     hipLaunchKernelGGL(k_closure_split, grid(cap_miss), dim3(256), 0, s,
@@ -1810,16 +1814,14 @@ int certify_captured(rpd_ctx* ctx, RunState& R, int64_t n1_cap,
                        ctx->upass.as<uint8_t>(), ctx->ppos.as<int32_t>(),
                        ctx->cand.as<Cand>(), c_hi, ctx->reopened.as<uint8_t>(),
                        n_u, cap_new, ovf);
-    hipLaunchKernelGGL(k_range_count<int32_t>, dim3(1), dim3(64), 0, s,
+    // the round's count and end; merged pairs += the unique pairs (disjoint
+    // from them), padded, sorted again
+    hipLaunchKernelGGL(k_round_close, grid(cap_ext), dim3(256), 0, s,
                        ctx->ppos.as<int32_t>() + cap_miss, c_hi, cap_new,
-                       gc + 1 + 2 * r, gc + 2 + 2 * r, ovf);
-    // merged pairs += the unique pairs (disjoint from them), sorted again
-    hipLaunchKernelGGL(k_append64, grid(cap_miss), dim3(256), 0, s,
-                       ctx->mem_a.as<uint64_t>(), (const int64_t*)gc,
-                       ctx->ukeys.as<uint64_t>(), n_u, cap_miss, cap_ext, ovf);
-    hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, gc, n_u, cap_ext);
-    hipLaunchKernelGGL(k_pad64<int64_t>, grid(cap_ext), dim3(256), 0, s,
-                       ctx->mem_a.as<uint64_t>(), cap_ext, (const int64_t*)gc);
+                       gc + 1 + 2 * r, gc + 2 + 2 * r,
+                       ctx->mem_a.as<uint64_t>(), cap_ext,
+                       ctx->ukeys.as<uint64_t>(), n_u,
+                       (const int64_t*)(ext_n + r), ext_n + r + 1, ovf);
     HIP_TRY(hipGetLastError());
     if (int rc = sort_u64(ctx, s, ctx->mem_a.as<uint64_t>(),
                           ctx->mem_b.as<uint64_t>(), cap_ext))
@@ -1827,15 +1829,22 @@ int certify_captured(rpd_ctx* ctx, RunState& R, int64_t n1_cap,
     ctx->mem_a.swap(ctx->mem_b);
     // the round's cells, certified as they are clipped (the padded pair
     // list: its ~0 keys match no pair)
-    if (int rc = certify_round(ctx, R, 0, cap_ext)) return rc;
+    if (int rc = certify_round(ctx, R, 0, cap_ext, gc + 1 + 2 * r))
+      return rc;
     if (int rc = clip_range(ctx, R, c_hi, c_hi + cap_new, false,
                             gc + 1 + 2 * r, gc + 2 + 2 * r))
       return rc;
+    launch_n = gc + 1 + 2 * r;
     c_hi += cap_new;
   }
-  if (rounds == kClosureRounds) {  // still open: full relation
+  if (rounds == kClosureRounds) {  // still open: full relation (every
+    // missed tet, listed or not, the same ones as the eager path)
+    hipLaunchKernelGGL(k_pad_miss, grid(cap_miss), dim3(256), 0, s,
+                       ctx->miss_a.as<uint64_t>(), cap_miss, nm, launch_n,
+                       n_eff, (int32_t*)nullptr);
     hipLaunchKernelGGL(k_flag_miss, grid(cap_miss), dim3(256), 0, s, cap_miss,
-                       ctx->miss_a.as<uint64_t>(), ctx->flag.as<uint8_t>(), nm);
+                       ctx->miss_a.as<uint64_t>(), ctx->flag.as<uint8_t>(),
+                       (const int32_t*)n_eff);
   } else {  // misses left: the run needs more rounds than were captured
     hipLaunchKernelGGL(k_flag_if_positive, dim3(1), dim3(64), 0, s, nm, ovf);
   }
@@ -2581,7 +2590,10 @@ int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom) {
   cap.areas = grow(n.areas, f, 16384);
   if (R.cfg.learned) {
     // one closure round more than the eager run needed (up to the eager
-    // limit); the merged pairs get room for that round too
+    // limit), the merged pairs with room for it: a replay that needs still
+    // more is captured again, so that its per-tet site sets are the eager
+    // ones too (a tet sent to the full relation keeps related sites whose
+    // cells are empty, one closed by the rounds does not)
     cap.rounds = std::min<int64_t>(n.rounds + 1, kClosureRounds);
     cap.miss = grow(n.miss, f, 4096);
     cap.uniq = cap.miss;
@@ -2745,8 +2757,6 @@ int rpd_graph_run(rpd_ctx* ctx, int32_t* replayed) {
       tot.cells > cap.cells || tot.words > cap.words || tot.areas > cap.areas;
   if (over) {  // counts beyond the capture's capacities: capture again
     std::string why;
-    if (learned && cap.rounds < kClosureRounds && cnt[4] > 0)
-      why += " closure rounds";
     if (cnt[7] != 0) why += " device-side capacity";
     if (tiers_over) why += " re-run tiers";
     if (learned ? n1 > cap.n_cand : n_cand > cap.n_cand) why += " candidates";

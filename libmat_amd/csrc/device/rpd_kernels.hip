@@ -60,23 +60,23 @@ __global__ __launch_bounds__(256) void k_fill32(int32_t* p, int64_t n,
   if (i < n) p[i] = v;
 }
 
-// A captured run's re-run lists: before a clip launch its counters start at
-// the list positions the host reserved (cnt[1] small-tier overflows, cnt[2]
-// exact re-runs, cnt[6] middle-tier overflows), and after its tiers the
-// counts are checked against what those reserved (*overflow otherwise).
-__global__ void k_set_rerun_counts(int32_t* cnt, int32_t r_large,
-                                   int32_t r_exact, int32_t r_full) {
-  if (threadIdx.x != 0) return;
-  cnt[1] = r_large;
-  cnt[2] = r_exact;
-  cnt[6] = r_full;
-}
-__global__ void k_check_rerun_counts(const int32_t* cnt, int32_t lim_large,
+// A captured run's re-run lists (cnt[1] small-tier overflows, cnt[2] exact
+// re-runs, cnt[6] middle-tier overflows): after a clip launch's tiers the
+// counts are checked against the list positions the host reserved
+// (*overflow otherwise); with next_large >= 0 the counters then start at
+// the positions reserved for the next launch.
+__global__ void k_check_rerun_counts(int32_t* cnt, int32_t lim_large,
                                      int32_t lim_exact, int32_t lim_full,
-                                     int32_t* overflow) {
+                                     int32_t* overflow, int32_t next_large,
+                                     int32_t next_exact, int32_t next_full) {
   if (threadIdx.x != 0) return;
   if (cnt[1] > lim_large || cnt[2] > lim_exact || cnt[6] > lim_full)
     *overflow = 1;
+  if (next_large >= 0) {
+    cnt[1] = next_large;
+    cnt[2] = next_exact;
+    cnt[6] = next_full;
+  }
 }
 
 // --------------------------------------------------------- predicates
@@ -809,6 +809,10 @@ struct CertArgs {
   int32_t* n_miss;
   uint8_t* flag;
   uint8_t* has_cell;
+  // (a captured run: the candidates of the launch, on the device; the miss
+  // list then holds min(cap, max(4096, 2 * *cap_n)) entries, as the eager
+  // path sizes it)
+  const int64_t* cap_n;
 };
 
 struct ClipArgs {
@@ -1617,7 +1621,10 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
         const uint64_t key = ((uint64_t)(uint32_t)t << 32) | (uint32_t)src;
         if (c.n_ext && has_key(c.ext, c.n_ext, key)) continue;
         const int r = atomicAdd(c.n_miss, 1);
-        if (r < c.cap)
+        const int64_t cap =
+            c.cap_n ? min((int64_t)c.cap, max((int64_t)4096, 2 * *c.cap_n))
+                    : (int64_t)c.cap;
+        if (r < cap)
           c.miss[r] = key;
         else
           c.flag[t] = 1;

@@ -1281,15 +1281,6 @@ __global__ __launch_bounds__(256) void k_out_ids(int64_t n, const Cand* cand,
 // passes its capacity sets *overflow (the host then runs eagerly and
 // captures again).
 
-// p[i] = ~0 for min(*n, cap) <= i < cap (a padded list sorts its real
-// entries first)
-template <typename T>
-__global__ __launch_bounds__(256) void k_pad64(uint64_t* p, int64_t cap,
-                                               const T* n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < cap && i >= min(cap, (int64_t)*n)) p[i] = ~0ull;
-}
-
 // a range's count n_src (clamped to cap, *overflow past it) and its end
 // base + count
 template <typename T>
@@ -1306,24 +1297,34 @@ __global__ void k_range_count(const T* n_src, int64_t base, int64_t cap,
   *abs_end = base + n;
 }
 
-// dst[*n_dst + u] = src[u] for u < *n_src (at most cap entries in dst)
-__global__ __launch_bounds__(256) void k_append64(
-    uint64_t* dst, const int64_t* n_dst, const uint64_t* src,
-    const int32_t* n_src, int64_t n, int64_t cap, int32_t* overflow) {
-  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= n || u >= *n_src) return;
-  const int64_t i = *n_dst + u;
-  if (i >= cap) {
-    *overflow = 1;
-    return;
+// The end of a captured closure round: its new candidates' count *n_new
+// (clamped to cap_new) and end base + count into rel / abs_end, and the
+// merged pair list ext += the round's n_u unique pairs (ukeys) with ~0 past
+// them, its length into *n_ext_out (*overflow past cap_ext or cap_new).
+__global__ __launch_bounds__(256) void k_round_close(
+    const int32_t* n_new, int64_t base, int64_t cap_new, int64_t* rel,
+    int64_t* abs_end, uint64_t* ext, int64_t cap_ext, const uint64_t* ukeys,
+    const int32_t* n_u, const int64_t* n_ext_in, int64_t* n_ext_out,
+    int32_t* overflow) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ne = *n_ext_in, nu = *n_u;
+  if (i == 0) {
+    int64_t n = *n_new;
+    if (n > cap_new) {
+      *overflow = 1;
+      n = cap_new;
+    }
+    *rel = n;
+    *abs_end = base + n;
+    int64_t tot = ne + nu;
+    if (tot > cap_ext) {
+      *overflow = 1;
+      tot = cap_ext;
+    }
+    *n_ext_out = tot;
   }
-  dst[i] = src[u];
-}
-
-// *n_dst += *n_src, clamped to cap
-__global__ void k_add_count(int64_t* n_dst, const int32_t* n_src,
-                            int64_t cap) {
-  if (threadIdx.x == 0) *n_dst = min(cap, *n_dst + (int64_t)*n_src);
+  if (i >= cap_ext || i < ne) return;
+  ext[i] = i < ne + nu ? ukeys[i - ne] : ~0ull;
 }
 
 // after an overflow (whose results are discarded) the first-round CSR is
@@ -1333,6 +1334,28 @@ __global__ __launch_bounds__(256) void k_clamp_offsets(int64_t* off,
                                                        const int32_t* overflow) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t <= n && *overflow) off[t] = min(off[t], cap);
+}
+
+// A captured miss list of nm misses from a launch of *cap_n candidates: the
+// eager path's list holds max(4096, 2 * *cap_n) of them, this one
+// lim = min(cap, that); *n_eff = min(nm, lim) are listed (the rest of
+// their tets were sent to the full relation by the certificate, as the
+// eager path does), entries past them are padded with ~0, and *overflow
+// is set (with overflow non-null) when this list cut misses the eager one
+// would hold.
+__global__ __launch_bounds__(256) void k_pad_miss(uint64_t* p, int64_t cap,
+                                                  const int32_t* nm,
+                                                  const int64_t* cap_n,
+                                                  int32_t* n_eff,
+                                                  int32_t* overflow) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t capc = max((int64_t)4096, 2 * *cap_n);
+  const int64_t m = min((int64_t)*nm, min(cap, capc));
+  if (i == 0) {
+    *n_eff = (int32_t)m;
+    if (overflow && (int64_t)*nm > cap && capc > cap) *overflow = 1;
+  }
+  if (i < cap && i >= m) p[i] = ~0ull;
 }
 
 // *overflow = 1 when *n > 0 (misses left after the last captured round)
