@@ -220,6 +220,9 @@ struct RunCaps {
   int64_t rounds = 0, miss = 0, uniq = 0, added = 0, ext = 0;
   int64_t flagged = 0, fb = 0, incl = 0;
   int64_t c_mid = 0, c_full = 0, c_exact = 0;
+  // per round: the miss list read at round k (k = the closure round, up to
+  // the last check), the round's new candidates, the merged pairs after it
+  int64_t miss_r[8] = {}, added_r[8] = {}, ext_r[8] = {};
 };
 
 }  // namespace
@@ -1476,6 +1479,7 @@ TierViews tier_views(rpd_ctx* ctx, const RunState& R) {
 // (tet, site) order.  Returns the number of candidates in the output order
 // (the first n_incl of perm).
 constexpr int kClosureRounds = 6;
+static_assert(kClosureRounds + 1 <= 8, "RunCaps holds 8 rounds");
 
 int sort_u64(rpd_ctx* ctx, hipStream_t s, const uint64_t* in, uint64_t* out,
              int64_t n) {
@@ -1492,9 +1496,9 @@ int sort_u64(rpd_ctx* ctx, hipStream_t s, const uint64_t* in, uint64_t* out,
 // certificate state for a round of n new candidates: a cleared miss list
 // of capacity max(4096, 2n) and the current ext keys
 int certify_round(rpd_ctx* ctx, RunState& R, int64_t n, int64_t n_ext,
-                  const int64_t* cap_n = nullptr) {
-  // (a captured run: the capacity of its miss lists)
-  const int64_t cap = R.graph ? R.cap.miss : std::max<int64_t>(4096, 2 * n);
+                  const int64_t* cap_n = nullptr, int64_t cap_graph = 0) {
+  // (a captured run: the capacity of this miss list)
+  const int64_t cap = R.graph ? cap_graph : std::max<int64_t>(4096, 2 * n);
   HIP_TRY(ctx->miss_a.need((size_t)cap * 8));
   HIP_TRY(ctx->miss_b.need((size_t)cap * 8));
   HIP_TRY(zero32(R.s, R.cnt + 4, 1));
@@ -1524,7 +1528,8 @@ int certify_begin(rpd_ctx* ctx, RunState& R, int64_t n1) {
   R.cert.mem_ids = ctx->rel_ids.as<int32_t>();
   R.cert.flag = ctx->flag.as<uint8_t>();
   R.cert.has_cell = ctx->has_cell.as<uint8_t>();
-  return certify_round(ctx, R, n1, 0, ctx->tet_off.as<int64_t>() + R.nt);
+  return certify_round(ctx, R, n1, 0, ctx->tet_off.as<int64_t>() + R.nt,
+                       R.cap.miss_r[0]);
 }
 
 int certify_captured(rpd_ctx* ctx, RunState& R, int64_t n1,
@@ -1546,6 +1551,7 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
     HIP_TRY(read_back(ctx, s, {{&nm, R.cnt + 4, 4}}));
     const int64_t n_miss = std::min<int64_t>(nm, R.cert.cap);
     R.seen.miss = std::max<int64_t>(R.seen.miss, n_miss);
+    R.seen.miss_r[round] = n_miss;
     if (n_miss == 0) break;
     if (round == kClosureRounds) {  // still open: full relation
       hipLaunchKernelGGL(k_flag_miss, dim3(ceil_div(n_miss, 256)), dim3(256),
@@ -1589,6 +1595,7 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
                        {&n_new, ctx->ppos.as<int32_t>() + n_miss, 4}}));
     R.seen.uniq = std::max<int64_t>(R.seen.uniq, n_u);
     R.seen.added = std::max<int64_t>(R.seen.added, n_new);
+    R.seen.added_r[round] = n_new;
     HIP_TRY(ctx->cand.need_keep((size_t)(c_hi + n_new) * sizeof(Cand),
                                 (size_t)c_hi * sizeof(Cand), s));
     if (n_u > 0)
@@ -1605,6 +1612,7 @@ int certify_and_fallback(rpd_ctx* ctx, RunState& R, int64_t n1,
     HIP_TRY(hipMemcpyAsync(ctx->mem_a.as<uint64_t>() + n_ext, ctx->ukeys.p,
                            (size_t)n_u * 8, hipMemcpyDeviceToDevice, s));
     n_ext += n_u;
+    R.seen.ext_r[round] = n_ext;
     HIP_TRY(ctx->mem_b.need((size_t)n_ext * 8));
     if (int rc = sort_u64(ctx, s, ctx->mem_a.as<uint64_t>(),
                           ctx->mem_b.as<uint64_t>(), n_ext))
@@ -1759,7 +1767,9 @@ int certify_captured(rpd_ctx* ctx, RunState& R, int64_t n1_cap,
   const int32_t nt = R.nt;
   const RunCaps& C = R.cap;
   const int rounds = (int)C.rounds;
-  const int64_t cap_miss = C.miss, cap_new = C.added, cap_ext = C.ext;
+  // buffers at the largest round's capacities; each round's lists, sorts
+  // and launches at its own
+  const int64_t cap_miss = C.miss, cap_ext = C.ext;
   const int64_t cap_nf = C.flagged, cap_fb = C.fb, cap_incl = C.incl;
   int32_t* ovf = R.cnt + 7;
   auto grid = [](int64_t n) { return dim3(ceil_div(std::max<int64_t>(n, 1), 256)); };
@@ -1777,72 +1787,77 @@ int certify_captured(rpd_ctx* ctx, RunState& R, int64_t n1_cap,
   HIP_TRY(ctx->upass.need((size_t)cap_miss + 1));
   HIP_TRY(ctx->mem_a.need((size_t)std::max<int64_t>(cap_ext, 1) * 8));
   HIP_TRY(ctx->mem_b.need((size_t)std::max<int64_t>(cap_ext, 1) * 8));
-  const int64_t fb_base = n1_cap + (int64_t)rounds * cap_new;
+  // candidate ranges: the first round, closure round r, the fallback
+  RegionTable reg{};
+  reg.n = rounds;
+  reg.base[0] = n1_cap;
+  for (int r = 0; r < rounds; ++r) reg.base[r + 1] = reg.base[r] + C.added_r[r];
+  const int64_t fb_base = reg.base[rounds];
   const int64_t c_end = fb_base + cap_fb;
   HIP_TRY(ctx->cand.need_keep((size_t)c_end * sizeof(Cand),
                               (size_t)n1_cap * sizeof(Cand), s));
   const int32_t* nm = R.cnt + 4;
   int32_t* n_eff = R.cnt + 5;  // the misses listed
-  const int32_t* n_u = ctx->upos.as<int32_t>() + cap_miss;
   // candidates of the launch whose misses are listed
   const int64_t* launch_n = ctx->tet_off.as<int64_t>() + nt;
-  int64_t c_hi = n1_cap;
   for (int r = 0; r < rounds; ++r) {
+    const int64_t cm = C.miss_r[r], cn = C.added_r[r], ce = C.ext_r[r];
+    const int32_t* n_u = ctx->upos.as<int32_t>() + cm;
+    const int64_t c_lo = reg.base[r];
     // unique missing pairs of the round (a padded sort of the miss list)
-    hipLaunchKernelGGL(k_pad_miss, grid(cap_miss), dim3(256), 0, s,
-                       ctx->miss_a.as<uint64_t>(), cap_miss, nm, launch_n,
-                       n_eff, ovf);
+    hipLaunchKernelGGL(k_pad_miss, grid(cm), dim3(256), 0, s,
+                       ctx->miss_a.as<uint64_t>(), cm, nm, launch_n, n_eff,
+                       ovf);
     if (int rc = sort_u64(ctx, s, ctx->miss_a.as<uint64_t>(),
-                          ctx->miss_b.as<uint64_t>(), cap_miss))
+                          ctx->miss_b.as<uint64_t>(), cm))
       return rc;
-    hipLaunchKernelGGL(k_unique_mark, grid(cap_miss + 1), dim3(256), 0, s,
-                       cap_miss, ctx->miss_b.as<uint64_t>(),
-                       ctx->umark.as<int32_t>(), (const int32_t*)n_eff);
+    hipLaunchKernelGGL(k_unique_mark, grid(cm + 1), dim3(256), 0, s, cm,
+                       ctx->miss_b.as<uint64_t>(), ctx->umark.as<int32_t>(),
+                       (const int32_t*)n_eff);
     HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->umark.as<int32_t>(),
-                                   ctx->upos.as<int32_t>(), cap_miss + 1));
-    hipLaunchKernelGGL(k_closure_split, grid(cap_miss), dim3(256), 0, s,
-                       cap_miss, ctx->miss_b.as<uint64_t>(),
-                       ctx->umark.as<int32_t>(), ctx->upos.as<int32_t>(), R.M,
-                       R.S, ctx->ukeys.as<uint64_t>(), ctx->upass.as<uint8_t>());
-    hipLaunchKernelGGL(k_pass32, grid(cap_miss + 1), dim3(256), 0, s, cap_miss,
-                       n_u, ctx->upass.as<uint8_t>(), ctx->umark.as<int32_t>());
+                                   ctx->upos.as<int32_t>(), cm + 1));
+    hipLaunchKernelGGL(k_closure_split, grid(cm), dim3(256), 0, s, cm,
+                       ctx->miss_b.as<uint64_t>(), ctx->umark.as<int32_t>(),
+                       ctx->upos.as<int32_t>(), R.M, R.S,
+                       ctx->ukeys.as<uint64_t>(), ctx->upass.as<uint8_t>());
+    hipLaunchKernelGGL(k_pass32, grid(cm + 1), dim3(256), 0, s, cm, n_u,
+                       ctx->upass.as<uint8_t>(), ctx->umark.as<int32_t>());
     HIP_TRY(exclusive_sum<int32_t>(ctx, ctx->umark.as<int32_t>(),
-                                   ctx->ppos.as<int32_t>(), cap_miss + 1));
-    // the related ones: new candidates at [c_hi, c_hi + count)
-    hipLaunchKernelGGL(k_closure_fill, grid(cap_miss), dim3(256), 0, s,
-                       cap_miss, ctx->ukeys.as<uint64_t>(),
-                       ctx->upass.as<uint8_t>(), ctx->ppos.as<int32_t>(),
-                       ctx->cand.as<Cand>(), c_hi, ctx->reopened.as<uint8_t>(),
-                       n_u, cap_new, ovf);
+                                   ctx->ppos.as<int32_t>(), cm + 1));
+    // the related ones: new candidates at [c_lo, c_lo + count)
+    hipLaunchKernelGGL(k_closure_fill, grid(cm), dim3(256), 0, s, cm,
+                       ctx->ukeys.as<uint64_t>(), ctx->upass.as<uint8_t>(),
+                       ctx->ppos.as<int32_t>(), ctx->cand.as<Cand>(), c_lo,
+                       ctx->reopened.as<uint8_t>(), n_u, cn, ovf);
     // the round's count and end; merged pairs += the unique pairs (disjoint
     // from them), padded, sorted again
-    hipLaunchKernelGGL(k_round_close, grid(cap_ext), dim3(256), 0, s,
-                       ctx->ppos.as<int32_t>() + cap_miss, c_hi, cap_new,
-                       gc + 1 + 2 * r, gc + 2 + 2 * r,
-                       ctx->mem_a.as<uint64_t>(), cap_ext,
+    hipLaunchKernelGGL(k_round_close, grid(ce), dim3(256), 0, s,
+                       ctx->ppos.as<int32_t>() + cm, c_lo, cn, gc + 1 + 2 * r,
+                       gc + 2 + 2 * r, ctx->mem_a.as<uint64_t>(), ce,
                        ctx->ukeys.as<uint64_t>(), n_u,
                        (const int64_t*)(ext_n + r), ext_n + r + 1, ovf);
     HIP_TRY(hipGetLastError());
     if (int rc = sort_u64(ctx, s, ctx->mem_a.as<uint64_t>(),
-                          ctx->mem_b.as<uint64_t>(), cap_ext))
+                          ctx->mem_b.as<uint64_t>(), ce))
       return rc;
     ctx->mem_a.swap(ctx->mem_b);
     // the round's cells, certified as they are clipped (the padded pair
     // list: its ~0 keys match no pair)
-    if (int rc = certify_round(ctx, R, 0, cap_ext, gc + 1 + 2 * r))
+    if (int rc = certify_round(ctx, R, 0, ce, gc + 1 + 2 * r,
+                               C.miss_r[r + 1]))
       return rc;
-    if (int rc = clip_range(ctx, R, c_hi, c_hi + cap_new, false,
-                            gc + 1 + 2 * r, gc + 2 + 2 * r))
+    if (int rc = clip_range(ctx, R, c_lo, c_lo + cn, false, gc + 1 + 2 * r,
+                            gc + 2 + 2 * r))
       return rc;
     launch_n = gc + 1 + 2 * r;
-    c_hi += cap_new;
   }
+  const int64_t cm_last = C.miss_r[rounds];
   if (rounds == kClosureRounds) {  // still open: full relation (every
     // missed tet, listed or not, the same ones as the eager path)
-    hipLaunchKernelGGL(k_pad_miss, grid(cap_miss), dim3(256), 0, s,
-                       ctx->miss_a.as<uint64_t>(), cap_miss, nm, launch_n,
+    hipLaunchKernelGGL(k_pad_miss, grid(cm_last), dim3(256), 0, s,
+                       ctx->miss_a.as<uint64_t>(), cm_last, nm, launch_n,
                        n_eff, (int32_t*)nullptr);
-    hipLaunchKernelGGL(k_flag_miss, grid(cap_miss), dim3(256), 0, s, cap_miss,
+    hipLaunchKernelGGL(k_flag_miss, grid(cm_last), dim3(256), 0, s, cm_last,
                        ctx->miss_a.as<uint64_t>(), ctx->flag.as<uint8_t>(),
                        (const int32_t*)n_eff);
   } else {  // misses left: the run needs more rounds than were captured
@@ -1899,8 +1914,7 @@ int certify_captured(rpd_ctx* ctx, RunState& R, int64_t n1_cap,
   HIP_TRY(ctx->vals_a.need((size_t)std::max<int64_t>(nB, 1) * 4));
   HIP_TRY(ctx->vals_b.need((size_t)std::max<int64_t>(nB, 1) * 4));
   if (nB > 0) {
-    hipLaunchKernelGGL(k_b_keys_ranges, grid(nB), dim3(256), 0, s, n1_cap,
-                       std::max<int64_t>(cap_new, 1), (int32_t)rounds, fb_base,
+    hipLaunchKernelGGL(k_b_keys_ranges, grid(nB), dim3(256), 0, s, reg,
                        c_end, (const int64_t*)(gc + 1), ctx->cand.as<Cand>(),
                        ctx->flag.as<uint8_t>(), ctx->keys_a.as<uint64_t>(),
                        ctx->vals_a.as<int32_t>());
@@ -2595,10 +2609,23 @@ int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom) {
     // ones too (a tet sent to the full relation keeps related sites whose
     // cells are empty, one closed by the rounds does not)
     cap.rounds = std::min<int64_t>(n.rounds + 1, kClosureRounds);
-    cap.miss = grow(n.miss, f, 4096);
+    // per round (the extra round with the last merged count plus its list)
+    cap.miss = cap.added = cap.ext = 0;
+    for (int64_t k = 0; k <= cap.rounds; ++k) {
+      cap.miss_r[k] = grow(k <= n.rounds ? n.miss_r[k] : 0, f, 4096);
+      cap.miss = std::max(cap.miss, cap.miss_r[k]);
+    }
+    for (int64_t r = 0; r < cap.rounds; ++r) {
+      const bool seen_r = r < n.rounds;
+      cap.added_r[r] = grow(seen_r ? n.added_r[r] : 0, f, 1024);
+      cap.ext_r[r] = seen_r ? grow(n.ext_r[r], f, 4096)
+                            : (r > 0 ? cap.ext_r[r - 1] : 4096) +
+                                  cap.miss_r[r];
+      cap.added += cap.added_r[r];
+      cap.ext = std::max(cap.ext, cap.ext_r[r]);
+    }
+    cap.ext = std::max<int64_t>(cap.ext, 1);
     cap.uniq = cap.miss;
-    cap.added = grow(n.added, f, 1024);
-    cap.ext = grow(n.ext, f, 4096) + cap.miss;
     cap.flagged = grow(n.flagged, f, 256);
     cap.fb = grow(n.fb, f, 4096);
     cap.incl = grow(n.incl, f, 1024);
@@ -2607,7 +2634,7 @@ int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom) {
     cap.c_exact = grow(n.c_exact, f, 256);
   }
   if (cap.n_reg >= (1LL << 31) ||
-      cap.n_cand + cap.rounds * cap.added + cap.fb >= (1LL << 31) ||
+      cap.n_cand + cap.added + cap.fb >= (1LL << 31) ||
       cap.miss >= (1LL << 31) || cap.incl >= (1LL << 31))
     return fail(ctx, "rpd_graph_build: capacities over 2^31 entries");
   // the neighbour lists and the tables sized by them may lengthen in later

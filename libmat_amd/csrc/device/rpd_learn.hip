@@ -1363,34 +1363,36 @@ __global__ void k_flag_if_positive(const int32_t* n, int32_t* overflow) {
   if (threadIdx.x == 0 && *n > 0) *overflow = 1;
 }
 
+// Candidate ranges of a captured learned run: closure round r at
+// [base[r], base[r + 1]), the fallback from base[n] on.
+struct RegionTable {
+  int64_t base[8];
+  int32_t n;
+};
+
 // Output keys of the added candidates of a captured run: closure round r
-// holds its candidates at [n1_cap + r * round_cap, + rel[2 r]), the
-// fallback at [fb_base, + rel[2 rounds]) (rel: the device counts); slots
-// past a count are excluded (~0), and so are closure candidates of tets
-// re-run on the full relation, as in k_b_keys.
+// holds its candidates at [base[r], + rel[2 r]), the fallback at
+// [base[n], + rel[2 n]) (rel: the device counts); slots past a count are
+// excluded (~0), and so are closure candidates of tets re-run on the full
+// relation, as in k_b_keys.
 __global__ __launch_bounds__(256) void k_b_keys_ranges(
-    int64_t n1_cap, int64_t round_cap, int32_t rounds, int64_t fb_base,
-    int64_t end, const int64_t* rel, const Cand* cand, const uint8_t* flag,
-    uint64_t* keys, int32_t* vals) {
-  const int64_t c = n1_cap + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    RegionTable reg, int64_t end, const int64_t* rel, const Cand* cand,
+    const uint8_t* flag, uint64_t* keys, int32_t* vals) {
+  const int64_t c = reg.base[0] + (int64_t)blockIdx.x * blockDim.x +
+                    threadIdx.x;
   if (c >= end) return;
-  bool valid, closure;
-  if (c < fb_base) {
-    const int64_t r = (c - n1_cap) / round_cap;
-    valid = r < rounds && c - (n1_cap + r * round_cap) < rel[2 * r];
-    closure = true;
-  } else {
-    valid = c - fb_base < rel[2 * rounds];
-    closure = false;
-  }
+  int r = 0;
+  while (r < reg.n && c >= reg.base[r + 1]) ++r;
+  const bool closure = r < reg.n;
+  const bool valid = c - reg.base[r] < rel[2 * r];
   uint64_t k = ~0ull;
   if (valid) {
     const Cand cd = cand[c];
     if (!(closure && flag[cd.tet]))
       k = ((uint64_t)(uint32_t)cd.tet << 32) | (uint32_t)cd.seed;
   }
-  keys[c - n1_cap] = k;
-  vals[c - n1_cap] = (int32_t)c;
+  keys[c - reg.base[0]] = k;
+  vals[c - reg.base[0]] = (int32_t)c;
 }
 
 }  // namespace rpd
