@@ -326,8 +326,12 @@ int rpd_site_centroids(rpd_ctx* ctx, double* out, const double** dev_out);
  * the batch's per-instance tet or site counts, the tet subset switch or a
  * device buffer changed, it captures again through rpd_graph_build (whose
  * eager run gives the results) and sets *replayed = 0, else 1.  Results
- * are read as after rpd_run.  Plain path only: cfg->learned = 0,
- * count_work = 0, small_tier = 1. */
+ * are read as after rpd_run.  Plain and learned paths (cfg->learned 0, 1
+ * or 2); count_work = 0, small_tier = 1.  A learned capture holds the
+ * certificate, one closure round more than its eager run needed (a replay
+ * needing more is captured again), the fallback and the output order, so
+ * a replay's cells and per-tet site sets are those of an eager learned
+ * run; its stats carry learned[0..3] and no re-run tier counts. */
 int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom);
 int rpd_graph_run(rpd_ctx* ctx, int32_t* replayed);
 int rpd_graph_destroy(rpd_ctx* ctx);
