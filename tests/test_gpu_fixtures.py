@@ -142,3 +142,13 @@ def test_degenerate_spheres_learned_matches_plain(ctx, which):
               f"{st['learned'][3]}", flush=True)
         _same_result(res, plain)
         compare_with_oracle(res, orc)
+        # the same step captured (exact tier, full-relation fallback and
+        # closure rounds on the device): the eager run's cells and sets
+        off_e, ids_e = ctx.relation()
+        ctx.graph_build(headroom=0.1, cells_per_wave=8, **kw)
+        st_g, rep = ctx.graph_run()
+        assert rep and st_g["learned"][3] == st["learned"][3]
+        _same_result(ctx.download(), plain)
+        off_g, ids_g = ctx.relation()
+        assert np.array_equal(off_g, off_e) and np.array_equal(ids_g, ids_e)
+        ctx.graph_destroy()
