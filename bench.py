@@ -894,7 +894,7 @@ def headline(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only "
+        print(json.dumps({"error": "bench.py runs on one GPU only "
                           "(multi-GPU sharding is a design note)",
                           "n_gpus": a.gpus}))
         sys.exit(2)

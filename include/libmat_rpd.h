@@ -282,7 +282,7 @@ struct RpdTetMesh {
   std::vector<int> v_adjs, e_adjs, f_adjs, f_ids;
   std::map<aint3, aint3> tet_es2fe_map;  // feature edges (input_types.h:147)
 };
-// the PowerCell fields (medial_sphere.h:145-260) the driver fills
+// the PowerCell fields (medial_sphere.h:145-260) its caller fills
 struct RpdPowerCell : PowerCellAdjacency {};
 struct RpdSphere {
   int id = -1;

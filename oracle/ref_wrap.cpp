@@ -76,7 +76,7 @@ int64_t ref_rt_neighbors(int32_t n, const double* xyzw, int64_t* off,
 }
 
 // VBW convex cell: bounding box, then the planes in order (keep side
-// a x + b y + c z + d >= 0 in this build's convention, checked by the
+// a x + b y + c z + d >= 0 in the library's convention, checked by the
 // tests).  out: volume, barycenter (3), then one facet area per plane.
 int ref_vbw_cell(const double* box6, int32_t n_planes, const double* planes,
                  double* out) {

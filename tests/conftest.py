@@ -11,7 +11,7 @@ for p in (ROOT, os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")):
 
 def pytest_configure(config):
     config.addinivalue_line(
-        "markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+        "markers", "gpu: needs an MI355X GPU (run with -m gpu)")
 
 
 def gpu_available():

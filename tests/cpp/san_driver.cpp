@@ -44,7 +44,7 @@ void orc_get_relation(const orc_result* r, int64_t* off, int32_t* ids);
 int orc_adjacency(const char* cell_path, const char* es2fe_path,
                   int max_surf_fid, const char* out_path);
 
-// ---- device entry points used by rpd_compat.cpp: no GPU in this build
+// ---- device entry points used by rpd_compat.cpp: no GPU in the sanitizer binary
 int rpd_ctx_create(int32_t, rpd_ctx** out) {
   *out = nullptr;
   return -3;
