@@ -212,7 +212,6 @@ struct TierBufs {
 struct RunCaps {
   int64_t n_reg = 0, n_cand = 0, n_mid = 0, n_full = 0, n_exact = 0;
   int64_t cells = 0, words = 0, areas = 0;
-  int64_t n_knn = 0;  // neighbour-list entries the capture reserved
   // learned path: closure rounds run; the largest miss list, unique pairs
   // and new candidates of a round; the merged pairs, tets re-run on the
   // full relation, their candidates and the output candidates; the
@@ -1071,14 +1070,12 @@ int plain_candidates(rpd_ctx* ctx, RunState& R, int64_t* n_cand_out) {
 
 // neighbour spheres inline in list order (descent and relation read them)
 int site_inline(rpd_ctx* ctx, RunState& R) {
-  // (a captured run: the lists' length is knn_off[n_site] on the device,
-  // up to the capacity the capture reserved -- sphere updates change it)
-  const int64_t n = R.graph ? R.cap.n_knn : ctx->n_knn;
-  if (n > 0)
-    hipLaunchKernelGGL(k_site_inline, dim3(ceil_div(n, 256)), dim3(256), 0,
-                       R.s, R.S, n, ctx->knn_near.as<int32_t>(),
-                       ctx->knn_site_near.as<double4>(),
-                       R.graph ? ctx->knn_off.as<int64_t>() + R.ns : nullptr);
+  // (one wavefront per site: the lists' lengths are read on the device, so
+  // a captured run follows sphere updates that change them)
+  if (R.ns > 0)
+    hipLaunchKernelGGL(k_site_inline, dim3(R.ns), dim3(64), 0, R.s, R.S,
+                       ctx->knn_near.as<int32_t>(),
+                       ctx->knn_site_near.as<double4>());
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -2641,7 +2638,6 @@ int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom) {
   // sphere updates: room for them too, so that those keep their addresses
   {
     const size_t nk = (size_t)std::max<int64_t>(grow(ctx->n_knn, f, 1024), 1);
-    cap.n_knn = (int64_t)nk;
     HIP_TRY(ctx->knn_bis.need(nk * 32));
     HIP_TRY(ctx->knn_ids.need_keep(nk * 4, (size_t)ctx->n_knn * 4,
                                    ctx->stream));

@@ -87,15 +87,51 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
 // ------------------------------------------------------------- features
 // neighbour lists in list order with their spheres inline (flag in the
 // id word, near_word)
-__global__ __launch_bounds__(256) void k_site_inline(DevSites S, int64_t n_knn,
-                                                     int32_t* knn_near,
-                                                     double4* knn_site_near,
-                                                     const int64_t* n_dev) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= dev_count(n_knn, n_dev)) return;
-  const int m = S.knn_ids[q];
-  knn_near[q] = near_word(S, m);
-  knn_site_near[q] = S.site[m];
+// One wavefront per site: its neighbour list nearest bisector first (the
+// signed distance of the site's centre to each bisector, then the list
+// position), written with the neighbour spheres inline.  The relation test
+// meets a rejecting neighbour early in that order (its outcome does not
+// depend on the order); lists longer than kNearMax keep their order.
+__global__ __launch_bounds__(64) void k_site_inline(DevSites S,
+                                                    int32_t* knn_near,
+                                                    double4* knn_site_near) {
+  __shared__ double s_key[kNearMax];
+  __shared__ int32_t s_near[kNearMax];
+  const int i = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t k0 = S.knn_off[i];
+  const int deg = (int)(S.knn_off[i + 1] - k0);
+  if (deg > kNearMax) {
+    for (int k = lane; k < deg; k += 64) {
+      const int m = S.knn_ids[k0 + k];
+      knn_near[k0 + k] = near_word(S, m);
+      knn_site_near[k0 + k] = S.site[m];
+    }
+    return;
+  }
+  const rpd_d4 si = ld4(S.site + i);
+  for (int k = lane; k < deg; k += 64) {
+    const rpd_d4 e = rpd_bisector(si, ld4(S.site + S.knn_ids[k0 + k]));
+    double key = (e.x * si.x + e.y * si.y + e.z * si.z + e.w) /
+                 sqrt(e.x * e.x + e.y * e.y + e.z * e.z);
+    s_key[k] = key == key ? key : -INFINITY;  // a total order
+  }
+  wsync();
+  for (int k = lane; k < deg; k += 64) {
+    const double kk = s_key[k];
+    int r = 0;
+    for (int u = 0; u < deg; ++u) {
+      const double ku = s_key[u];
+      r += ku < kk || (ku == kk && u < k);
+    }
+    s_near[r] = S.knn_ids[k0 + k];
+  }
+  wsync();
+  for (int k = lane; k < deg; k += 64) {
+    const int m = s_near[k];
+    knn_near[k0 + k] = near_word(S, m);
+    knn_site_near[k0 + k] = S.site[m];
+  }
 }
 
 __device__ __forceinline__ double pdist(const double c[3], const double4& s) {
