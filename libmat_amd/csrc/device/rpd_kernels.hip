@@ -686,40 +686,48 @@ __global__ __launch_bounds__(64) void k_knn_bis(DevSites S, double4* bis) {
   }
 }
 
-// dot(e, X) < 0 for certain: the filter of rpd_filtered_sign4 with the
-// products fused into the sums (no monomial passes through more roundings
-// than unfused, so its error constant holds)
-__device__ __forceinline__ bool certainly_neg4(const rpd_d4& e,
-                                               const rpd_d4& X,
-                                               const rpd_d4& Q) {
-#pragma clang fp contract(fast)
-  const double s = e.x * X.x + e.y * X.y + e.z * X.z + e.w * X.w;
-  const double b =
-      fabs(e.x) * Q.x + fabs(e.y) * Q.y + fabs(e.z) * Q.z + fabs(e.w) * Q.w;
-  return b < RPD_FILTER_MAX && s < -(RPD_FILTER_C * b + RPD_FILTER_ABS);
-}
-
 // The tet's four corners as the clip kernel's initial vertices: homogeneous
 // coordinates X and cofactor magnitudes Q of the dual triangles (1,3,2),
-// (0,2,3), (0,3,1), (0,1,2) of its binary64 face planes.
+// (0,2,3), (0,3,1), (0,1,2) of its binary64 face planes; sQ = the sum of
+// a corner's four magnitudes.
 struct CornerFrames {
-  rpd_d4 X[4], Q[4];
+  rpd_d4 X[4];
+  double sQ[4];
   // the plane e is certainly positive at all four corners, so it cannot
   // cut any state of a cell of this tet (every vertex of the cell lies in
-  // the convex hull of the corners)
+  // the convex hull of the corners).  The filter of rpd_filtered_sign4,
+  // products fused into the sums (no monomial passes through more
+  // roundings than unfused, so its error constant holds), with its bound
+  // sum|e_r| Q_r replaced by the larger max|e_r| * sum Q_r (one product
+  // instead of four; the computed product and sum fall short of the exact
+  // ones by at most 4u relative, which the constant's margin over gamma_10
+  // covers): a sign it decides is the exact one, and the few planes it
+  // leaves undecided that the tighter bound would skip are only clipped
+  // without effect.
   __device__ __forceinline__ bool inactive(const rpd_d4& e) const {
-    return certainly_neg4(e, X[0], Q[0]) && certainly_neg4(e, X[1], Q[1]) &&
-           certainly_neg4(e, X[2], Q[2]) && certainly_neg4(e, X[3], Q[3]);
+#pragma clang fp contract(fast)
+    const double m =
+        fmax(fmax(fabs(e.x), fabs(e.y)), fmax(fabs(e.z), fabs(e.w)));
+    bool neg = true;
+#pragma unroll
+    for (int k = 0; k < 4 && neg; ++k) {
+      const double s = e.x * X[k].x + e.y * X[k].y + e.z * X[k].z + e.w * X[k].w;
+      const double b = m * sQ[k];
+      neg = b < RPD_FILTER_MAX && s < -(RPD_FILTER_C * b + RPD_FILTER_ABS);
+    }
+    return neg;
   }
 };
 __device__ __forceinline__ void corner_frames(const TetLoad& L,
                                               CornerFrames& F) {
-  rpd_d4 pl[4];
+  rpd_d4 pl[4], Q[4];
   tet_planes(L.V, pl);
-  rpd_vertex_homog(pl[1], pl[3], pl[2], F.X[0], F.Q[0]);
-  rpd_vertex_homog(pl[0], pl[2], pl[3], F.X[1], F.Q[1]);
-  rpd_vertex_homog(pl[0], pl[3], pl[1], F.X[2], F.Q[2]);
-  rpd_vertex_homog(pl[0], pl[1], pl[2], F.X[3], F.Q[3]);
+  rpd_vertex_homog(pl[1], pl[3], pl[2], F.X[0], Q[0]);
+  rpd_vertex_homog(pl[0], pl[2], pl[3], F.X[1], Q[1]);
+  rpd_vertex_homog(pl[0], pl[3], pl[1], F.X[2], Q[2]);
+  rpd_vertex_homog(pl[0], pl[1], pl[2], F.X[3], Q[3]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) F.sQ[k] = ((Q[k].x + Q[k].y) + Q[k].z) + Q[k].w;
 }
 // a cell's number of clip steps: mask bits, plus the list positions past
 // 128 (always clipped), saturated to a byte
