@@ -239,6 +239,20 @@ __global__ __launch_bounds__(256) void k_vertex_owner(
 // instance-local in 16 bits (instances of at most 65,536 sites), else
 // batch-wide in 32 bits of a 64-bit entry (same order: local = batch-wide
 // id - the instance's first site).  ~0 is the empty slot.
+// the median of x, a and p where p <= a: max(p, min(x, a)).  32-bit words
+// take the full median form, max(min(x, a), min(max(x, a), p)), which
+// compiles to one v_med3_u32 (v_med3_i32 for signed words); 64-bit words
+// keep the two-step form (no 64-bit median instruction).
+template <typename T>
+__device__ __forceinline__ T slot_median(T x, T a, T p) {
+  if constexpr (sizeof(T) == 4) {
+    return max(min(x, a), min(max(x, a), p));
+  } else {
+    const T m = x < a ? x : a;
+    return p < m ? m : p;
+  }
+}
+
 template <bool kWide>
 struct PropEntry {
   using T = typename std::conditional<kWide, uint64_t, uint32_t>::type;
@@ -270,7 +284,7 @@ struct PropEntry {
 // the corner offsets d_x = x - c, pd(x, s) - pd_min(x) =
 // (|u|^2 - w) - 2 d_x.u + (|d_x|^2 - pd_min(x)), and kept to 16 bits
 // (PropEntry), so that a 32-entry sorted list of 32-bit words costs one
-// min and one max per slot and insertion.
+// median instruction per slot and insertion.
 template <bool kWide>
 __device__ __forceinline__ void knn_features(
     const DevMesh& M, const DevSites& S, const InstInfo* inst,
@@ -324,10 +338,7 @@ __device__ __forceinline__ void knn_features(
     }
     if (!(x < lst[kOut - 1])) return;  // past the last slot (or empty)
 #pragma unroll
-    for (int q = kOut - 1; q > 0; --q) {
-      const E m = x < lst[q] ? x : lst[q];
-      lst[q] = lst[q - 1] < m ? m : lst[q - 1];
-    }
+    for (int q = kOut - 1; q > 0; --q) lst[q] = slot_median(x, lst[q], lst[q - 1]);
     lst[0] = x < lst[0] ? x : lst[0];
   };
   uint32_t om = 0;
@@ -1000,7 +1011,7 @@ __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
       const int32_t y = knn[(int64_t)t * kOut + (__ffs(rest) - 1)];
       const int32_t x = y >= 0 ? y : 0x7fffffff;
 #pragma unroll
-      for (int k = 7; k > 0; --k) w[k] = max(w[k - 1], min(x, w[k]));
+      for (int k = 7; k > 0; --k) w[k] = slot_median(x, w[k], w[k - 1]);
       w[0] = min(x, w[0]);
     }
 #pragma unroll
