@@ -139,6 +139,45 @@ RPD_HD int rpd_filtered_sign_w(rpd_d4 X, rpd_d4 Q) {
   return 0;
 }
 
+// ------------------------------------------------ plane masks (binary32)
+// A bisector component rounded to binary32 for the plane masks: zero stays
+// zero; magnitudes in [1e-30, 1e30] round to nearest (relative error at
+// most 2^-24, no underflow or overflow); others become NaN, which
+// rpd_mask_negative never takes for certain.
+RPD_HD float rpd_mask_coef(double v) {
+  const double a = fabs(v);
+  return v == 0.0 ? 0.0f
+                  : (a >= 1e-30 && a <= 1e30 ? (float)v : __builtin_nanf(""));
+}
+
+// dot(e, X~_k) < 0 for certain at all four corners k of a tet cell
+// (X_k = rpd_vertex_homog of the corner's face planes, sQ_k the sum of its
+// four cofactor permanents), decided from f = rpd_mask_coef(e) alone
+// (|f_r - e_r| <= 5.97e-8 |f_r|): the filter of rpd_filtered_sign4 for
+// dot(f, X~), its bound sum|f_r| Q_r replaced by the larger
+// max|f_r| * sum Q_r (the computed product and sum short by at most 4u
+// relative; products may be fused into the sums, no monomial then passing
+// through more roundings), plus |dot(f - e, X~)| <= 5.97e-8 (1 + 1e-15)
+// max|f_r| sum Q_r (|X~_r| <= Q_r (1 + 6e-16)); 6e-8 covers both, 1e-250
+// the absolute terms.  A NaN component decides nothing.
+RPD_HD bool rpd_mask_negative(double fx, double fy, double fz, double fw,
+                              const rpd_d4 X[4], const double sQ[4]) {
+#if defined(__clang__)
+#pragma clang fp contract(fast)
+#endif
+  const double m = fmax(fmax(fabs(fx), fabs(fy)), fmax(fabs(fz), fabs(fw)));
+  bool neg = true;
+#if defined(__clang__)
+#pragma unroll
+#endif
+  for (int k = 0; k < 4 && neg; ++k) {
+    const double s = fx * X[k].x + fy * X[k].y + fz * X[k].z + fw * X[k].w;
+    const double b = m * sQ[k];
+    neg = b < RPD_FILTER_MAX && s < -(6e-8 * b + 1e-250);
+  }
+  return neg;
+}
+
 // ---------------------------------------------- expansion arithmetic
 // Components are stored in increasing magnitude, non-overlapping, zero-free
 // (a zero value is the one-component expansion {0}).

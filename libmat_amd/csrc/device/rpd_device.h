@@ -45,6 +45,9 @@ struct DevSites {
   const double4* knn_site_near;  // the spheres of knn_near, inline
   const double4* knn_bis;  // power bisector of each knn_ids entry (zero
                            // for a -1 entry), in knn_ids order
+  const float4* knn_bisf;  // the same rounded to binary32 for the plane
+                           // masks (NaN for components of magnitude
+                           // outside [1e-30, 1e30])
 };
 
 // capacities of a clip tier

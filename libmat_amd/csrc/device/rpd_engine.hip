@@ -244,6 +244,7 @@ struct rpd_ctx {
   Buf vert, tet_v, tet_fid, tet_meta, tet_inst, vert_inst, inst_dev;
   Buf site, flags, knn_off, knn_ids, knn_near, knn_site_near, site_inst;
   Buf knn_bis;
+  Buf knn_bisf;
   // pipeline
   Buf bbox, reg_n, reg_off, keys_a, keys_b, bin_start, bin_ids;
   Buf tet_count, tet_off, tet_pass, cand, rel_ids;
@@ -876,10 +877,12 @@ int init_state(rpd_ctx* ctx, RunState& R) {
   // the bisector of every neighbour-list entry, read by the plane masks
   // and the clip instead of two dependent loads and the arithmetic
   HIP_TRY(ctx->knn_bis.need((size_t)std::max<int64_t>(ctx->n_knn, 1) * 32));
+  HIP_TRY(ctx->knn_bisf.need((size_t)std::max<int64_t>(ctx->n_knn, 1) * 16));
   R.S.knn_bis = ctx->knn_bis.as<double4>();
+  R.S.knn_bisf = ctx->knn_bisf.as<float4>();
   if (R.ns > 0)
     hipLaunchKernelGGL(k_knn_bis, dim3(R.ns), dim3(64), 0, R.s, R.S,
-                       ctx->knn_bis.as<double4>());
+                       ctx->knn_bis.as<double4>(), ctx->knn_bisf.as<float4>());
   HIP_TRY(hipGetLastError());
   return upload_inst_info(ctx, R);
 }
@@ -2639,6 +2642,7 @@ int rpd_graph_build(rpd_ctx* ctx, const rpd_run_cfg* cfg, double headroom) {
   {
     const size_t nk = (size_t)std::max<int64_t>(grow(ctx->n_knn, f, 1024), 1);
     HIP_TRY(ctx->knn_bis.need(nk * 32));
+    HIP_TRY(ctx->knn_bisf.need(nk * 16));
     HIP_TRY(ctx->knn_ids.need_keep(nk * 4, (size_t)ctx->n_knn * 4,
                                    ctx->stream));
     HIP_TRY(ctx->knn_near.need_keep(nk * 4, (size_t)ctx->n_knn * 4,
@@ -2722,7 +2726,8 @@ int rpd_graph_run(rpd_ctx* ctx, int32_t* replayed) {
   // lists outgrew the captured bisector table: capture again (its eager run
   // gives this call's results)
   if (graph_signature(ctx) != ctx->graph_sig ||
-      (size_t)std::max<int64_t>(ctx->n_knn, 1) * 32 > ctx->knn_bis.cap) {
+      (size_t)std::max<int64_t>(ctx->n_knn, 1) * 32 > ctx->knn_bis.cap ||
+      (size_t)std::max<int64_t>(ctx->n_knn, 1) * 16 > ctx->knn_bisf.cap) {
     const rpd_run_cfg c = ctx->graph_cfg;
     return rpd_graph_build(ctx, &c, ctx->graph_headroom);
   }

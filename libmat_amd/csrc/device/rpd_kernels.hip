@@ -671,7 +671,8 @@ __global__ __launch_bounds__(256) void k_tet_fill(
 }
 
 // the bisector table of the neighbour lists: one wavefront per site
-__global__ __launch_bounds__(64) void k_knn_bis(DevSites S, double4* bis) {
+__global__ __launch_bounds__(64) void k_knn_bis(DevSites S, double4* bis,
+                                                float4* bisf) {
   const int i = blockIdx.x;
   const int64_t k0 = S.knn_off[i], k1 = S.knn_off[i + 1];
   const rpd_d4 si = ld4(S.site + i);
@@ -683,6 +684,8 @@ __global__ __launch_bounds__(64) void k_knn_bis(DevSites S, double4* bis) {
       e = make_double4(b.x, b.y, b.z, b.w);
     }
     bis[q] = e;
+    bisf[q] = make_float4(rpd_mask_coef(e.x), rpd_mask_coef(e.y),
+                          rpd_mask_coef(e.z), rpd_mask_coef(e.w));
   }
 }
 
@@ -693,29 +696,14 @@ __global__ __launch_bounds__(64) void k_knn_bis(DevSites S, double4* bis) {
 struct CornerFrames {
   rpd_d4 X[4];
   double sQ[4];
-  // the plane e is certainly positive at all four corners, so it cannot
-  // cut any state of a cell of this tet (every vertex of the cell lies in
-  // the convex hull of the corners).  The filter of rpd_filtered_sign4,
-  // products fused into the sums (no monomial passes through more
-  // roundings than unfused, so its error constant holds), with its bound
-  // sum|e_r| Q_r replaced by the larger max|e_r| * sum Q_r (one product
-  // instead of four; the computed product and sum fall short of the exact
-  // ones by at most 4u relative, which the constant's margin over gamma_10
-  // covers): a sign it decides is the exact one, and the few planes it
-  // leaves undecided that the tighter bound would skip are only clipped
-  // without effect.
-  __device__ __forceinline__ bool inactive(const rpd_d4& e) const {
-#pragma clang fp contract(fast)
-    const double m =
-        fmax(fmax(fabs(e.x), fabs(e.y)), fmax(fabs(e.z), fabs(e.w)));
-    bool neg = true;
-#pragma unroll
-    for (int k = 0; k < 4 && neg; ++k) {
-      const double s = e.x * X[k].x + e.y * X[k].y + e.z * X[k].z + e.w * X[k].w;
-      const double b = m * sQ[k];
-      neg = b < RPD_FILTER_MAX && s < -(RPD_FILTER_C * b + RPD_FILTER_ABS);
-    }
-    return neg;
+  // dot(e, X~) is certainly negative (no conflict) at all four corners, so
+  // the plane e cannot cut any state of a cell of this tet (every vertex of
+  // the cell lies in the convex hull of the corners); decided from the
+  // binary32 table entry f of e (rpd_mask_negative), one 16-byte load per
+  // plane.  The few planes it leaves undecided that a tighter test would
+  // skip are only clipped without effect.
+  __device__ __forceinline__ bool inactive(const float4& f) const {
+    return rpd_mask_negative(f.x, f.y, f.z, f.w, X, sQ);
   }
 };
 __device__ __forceinline__ void corner_frames(const TetLoad& L,
@@ -770,7 +758,7 @@ __global__ __launch_bounds__(256) void k_cand_mask(DevMesh M, DevSites S,
   uint64_t m0 = 0, m1 = 0;
   const int lim = deg < 128 ? deg : 128;
   for (int k = 0; k < lim; ++k)
-    if (!F.inactive(ld4(S.knn_bis + k0 + k))) {
+    if (!F.inactive(S.knn_bisf[k0 + k])) {
       if (k < 64)
         m0 |= 1ull << k;
       else

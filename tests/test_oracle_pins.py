@@ -94,6 +94,87 @@ def test_product_exact_predicates_match_rational_arithmetic(tmp_path):
     assert n_filtered > len(cases) // 8
 
 
+def plane_cases(n_tet, seed):
+    """tets of many sizes and offsets (every 7th a sliver, every 11th on a
+    1/64 grid) and 24 planes each, at signed distances 0 to 10 tet sizes
+    from the tet's first or last corner along random and axis normals of
+    scales 1e-6 to 1e6 (every 17th plane of scale 1e-36 or 1e36, outside
+    the binary32 table's range); returns the rows and the distances in tet
+    sizes"""
+    rng = np.random.default_rng(seed)
+    deltas = [0.0, 1e-16, 1e-15, 1e-14, 1e-12, 1e-9, 1e-6, 1e-3, 0.03, 0.1,
+              0.3, 1.0, 3.0, 10.0]
+    rows, dist = [], []
+    for it in range(n_tet):
+        size = 10.0 ** rng.uniform(-4, 0)
+        off = 0.0 if it % 3 == 0 else 10.0 ** rng.uniform(-2, 3) * rng.uniform(-1, 1)
+        V = off * np.array([1.0, 1.0, 0.5]) + size * rng.uniform(-1, 1, (4, 3))
+        if it % 7 == 1:
+            t = 10.0 ** rng.uniform(-15, -2)
+            a = rng.uniform()
+            b = rng.uniform() * (1 - a)
+            V[3] = (V[0] + a * (V[1] - V[0]) + b * (V[2] - V[0]) +
+                    t * size * rng.uniform(-1, 1, 3))
+        if it % 11 == 2:
+            V = np.round(V * 64) / 64
+        for j in range(24):
+            n = rng.uniform(-1, 1, 3)
+            if j % 5 == 0:
+                n = np.array([0.0, 0.0, 1.0])
+            elif j % 5 == 1:
+                n = np.array([1.0, -1.0, 0.0])
+            sc = 10.0 ** rng.uniform(-6, 6)
+            if (it * 24 + j) % 17 == 0:
+                sc = 1e-36 if j % 2 else 1e36
+            n = n * sc
+            # offset from the corner of largest (or least) n.V, so that
+            # grazing planes occur on both sides of both orientations
+            hi = rng.uniform() < 0.5
+            ext = (max if hi else min)(float(n @ v) for v in V)
+            nn = float(np.linalg.norm(n))
+            d = deltas[j % 14] * size * nn
+            d = d if rng.uniform() < 0.5 else -d
+            rows.append(list(V.ravel()) + list(n) + [-(ext + (d if hi else -d))])
+            dist.append(abs(d) / (size * nn))
+    return rows, dist
+
+
+def test_binary32_plane_masks_are_sound(tmp_path):
+    """The plane masks' test (rpd_mask_negative on the binary32 bisector
+    entry rpd_mask_coef): every plane it skips is negative at all four
+    corners by the exact predicate on the binary64 plane, and it skips
+    nearly all such planes that lie three tet sizes or more beyond a tet
+    that is not a sliver and lies within a thousand of its sizes of the
+    origin."""
+    exe = tmp_path / "mask_test"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall",
+                    "-Werror", os.path.join(ROOT, "tests/cpp/mask_test.cpp"),
+                    "-o", str(exe)], check=True)
+    rows, dist = plane_cases(3000, seed=5)
+    inp = "\n".join(" ".join(float(x).hex() for x in r) for r in rows)
+    out = subprocess.run([str(exe)], input=inp, capture_output=True,
+                         text=True, check=True).stdout.split("\n")
+    assert len(out) >= len(rows)
+    n_far = n_far_mask = n_mask = 0
+    for i, (r, d, line) in enumerate(zip(rows, dist, out)):
+        mask, *x = map(int, line.split())
+        if mask:
+            assert all(s < 0 for s in x), line
+            n_mask += 1
+        # efficacy over the tets that are not slivers and lie within 1e3 of
+        # their sizes of the origin (binary32 planes resolve a tet size
+        # there), planes within the table's range
+        V = np.asarray(r[:12]).reshape(4, 3)
+        ext = float((V.max(axis=0) - V.min(axis=0)).max())
+        scale = max(abs(v) for v in r[12:15])
+        if (d >= 3.0 and all(s < 0 for s in x) and 1e-20 < scale < 1e20 and
+                (i // 24) % 7 != 1 and np.abs(V).max() <= 1e3 * ext):
+            n_far += 1
+            n_far_mask += mask
+    assert n_far > 1000 and n_far_mask > 0.95 * n_far
+    assert n_mask > 1000
+
+
 @pytest.mark.skipif(not __import__("ref").available(),
                     reason="reference sources not mounted (oracle/_ref)")
 def test_exact_det4_sign_matches_reference_psm():
