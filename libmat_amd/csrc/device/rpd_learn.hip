@@ -232,13 +232,6 @@ __global__ __launch_bounds__(256) void k_vertex_owner(
   owner[v] = st < 0 ? -1 : descend(S, c, st, &best);
 }
 
-// Proposal list entries: (key, id) packed into one unsigned word so that
-// integer order is (key, id) order.  The key keeps the top 16 bits of its
-// non-negative binary32 pattern (8 exponent and 8 mantissa bits, truncated;
-// owners 0, others at least 0x100, finite keys at most 0xff00); the id is
-// instance-local in 16 bits (instances of at most 65,536 sites), else
-// batch-wide in 32 bits of a 64-bit entry (same order: local = batch-wide
-// id - the instance's first site).  ~0 is the empty slot.
 // the median of x, a and p where p <= a: max(p, min(x, a)).  32-bit words
 // take the full median form, max(min(x, a), min(max(x, a), p)), which
 // compiles to one v_med3_u32 (v_med3_i32 for signed words); 64-bit words
@@ -253,6 +246,13 @@ __device__ __forceinline__ T slot_median(T x, T a, T p) {
   }
 }
 
+// Proposal list entries: (key, id) packed into one unsigned word so that
+// integer order is (key, id) order.  The key keeps the top 16 bits of its
+// non-negative binary32 pattern (8 exponent and 8 mantissa bits, truncated;
+// owners 0, others at least 0x100, finite keys at most 0xff00); the id is
+// instance-local in 16 bits (instances of at most 65,536 sites), else
+// batch-wide in 32 bits of a 64-bit entry (same order: local = batch-wide
+// id - the instance's first site).  ~0 is the empty slot.
 template <bool kWide>
 struct PropEntry {
   using T = typename std::conditional<kWide, uint64_t, uint32_t>::type;
