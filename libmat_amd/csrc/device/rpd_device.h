@@ -61,11 +61,17 @@ using TightCaps = Caps<16, 12, 32>;  // small tier for small cells
 using MidCaps = Caps<24, 14, 40>;
 using LargeCaps = Caps<96, 64, 152>;
 
-// Output slot of one cell (indexed by cell or by re-run index).
+// Output slot of one cell (indexed by cell or by re-run index).  Its words
+// hold the edge records at 0 (appended as the clip creates edges; none is
+// ever removed), then the final vertices and plane sources right after the
+// last edge, so a cell's payload is one span [0, nb_e + nb_v + nb_p) at
+// the start of its slot; the stride rounds V + P + E up to 16 words (64
+// bytes), so the span starts on a 64-byte boundary.
+inline constexpr int32_t slot_stride_words(int V, int P, int E) {
+  return (V + P + E + 15) & ~15;
+}
 struct SlotView {
-  int32_t stride_words;   // V + P + E: vertices at 0, plane sources at
-  int32_t ofs_src;        // ofs_src = V, edges at ofs_edges = V + P
-  int32_t ofs_edges;
+  int32_t stride_words;   // slot_stride_words(V, P, E)
   int32_t stride_areas;   // P
   uint32_t* hdr;          // nb_v | nb_p<<8 | nb_e<<16 | status<<24
   uint32_t* words;

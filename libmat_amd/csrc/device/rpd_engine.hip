@@ -176,9 +176,7 @@ struct TierBufs {
   Buf hdr, words, geom, areas;
   SlotView view(int V, int P, int E) const {
     SlotView s;
-    s.stride_words = V + P + E;
-    s.ofs_src = V;
-    s.ofs_edges = V + P;
+    s.stride_words = slot_stride_words(V, P, E);
     s.stride_areas = P;
     s.hdr = hdr.as<uint32_t>();
     s.words = words.as<uint32_t>();
@@ -190,7 +188,8 @@ struct TierBufs {
     hipError_t e;
     n = std::max<int64_t>(n, 1);
     if ((e = hdr.need(n * 4)) != hipSuccess) return e;
-    if ((e = words.need(n * (V + P + E) * 4)) != hipSuccess) return e;
+    if ((e = words.need(n * slot_stride_words(V, P, E) * 4)) != hipSuccess)
+      return e;
     if ((e = geom.need(n * 4 * 8)) != hipSuccess) return e;
     return areas.need(n * P * 8);
   }
@@ -199,8 +198,8 @@ struct TierBufs {
     hipError_t e;
     n = std::max<int64_t>(n, 1);
     if ((e = hdr.need_keep(n * 4, keep * 4, s)) != hipSuccess) return e;
-    if ((e = words.need_keep(n * (V + P + E) * 4, keep * (V + P + E) * 4,
-                             s)) != hipSuccess)
+    const int64_t sw = slot_stride_words(V, P, E);
+    if ((e = words.need_keep(n * sw * 4, keep * sw * 4, s)) != hipSuccess)
       return e;
     if ((e = geom.need_keep(n * 32, keep * 32, s)) != hipSuccess) return e;
     return areas.need_keep(n * P * 8, keep * P * 8, s);

@@ -1043,10 +1043,11 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
   if (item >= n_items) return;
   const int64_t cidx = a.items ? a.items[item] : a.item_base + item;
   const int64_t slot = a.slot_by_cell ? cidx : a.slot_base + item;
-  // the slot's edge records are written as the edges are appended (the
-  // reference never removes an edge), so they need no LDS copy
+  // the slot's edge records are written at its start as the edges are
+  // appended (the reference never removes an edge), so they need no LDS
+  // copy; the vertices and plane sources follow the last one at the end
   uint32_t* const wslot = a.out.words + slot * a.out.stride_words;
-  uint32_t* const wedge = wslot + a.out.ofs_edges;
+  uint32_t* const wedge = wslot;
   const Cand cd = a.cand[cidx];
   const int t = cd.tet, s = cd.seed;
   const int4 tv = a.M.tet_v[t];
@@ -1560,10 +1561,10 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
     }
   }
   if (status == RPD_SUCCESS) {
-    // vertices at 0, plane sources at ofs_src (edges are in place)
-    for (int v = j; v < nb_v; v += L) wslot[v] = s_ver[g][v];
+    // vertices and plane sources right after the edges (in place)
+    for (int v = j; v < nb_v; v += L) wslot[nb_e + v] = s_ver[g][v];
     for (int p = j; p < nb_p; p += L)
-      wslot[o.ofs_src + p] = (uint32_t)s_src[g][p];
+      wslot[nb_e + nb_v + p] = (uint32_t)s_src[g][p];
   }
   if constexpr (kExact) {
     // a vertex exactly on a plane: the configuration is degenerate and a
@@ -1770,10 +1771,9 @@ __global__ __launch_bounds__(kCB) void k_compact(
   __shared__ uint32_t s_a[kCB + 1];
   __shared__ const uint32_t* s_wsrc[kCB];
   __shared__ const double* s_asrc[kCB];
-  // a slot holds vertices at 0, plane sources at ofs_src, edges at
-  // ofs_edges; the stream holds them back to back
-  __shared__ uint32_t s_lay[kCB];  // nb_v | nb_p << 8 | ofs_src << 16
-  __shared__ uint32_t s_oe[kCB];
+  // a slot holds the edges at 0 and the vertices and plane sources after
+  // them; the stream holds vertices, plane sources, edges
+  __shared__ uint32_t s_lay[kCB];  // nb_v | nb_p << 8 | nb_e << 16
   const int i = threadIdx.x;
   const int64_t r = (int64_t)blockIdx.x * kCB + i;
   n_cells = dev_count(n_cells, n_dev);
@@ -1809,8 +1809,7 @@ __global__ __launch_bounds__(kCB) void k_compact(
   const SlotView& sv = tv.t[x.tier];
   s_wsrc[i] = sv.words + x.idx * sv.stride_words;
   s_asrc[i] = sv.areas + x.idx * sv.stride_areas;
-  s_lay[i] = (x.hdr & 0xffffu) | ((uint32_t)sv.ofs_src << 16);
-  s_oe[i] = (uint32_t)sv.ofs_edges;
+  s_lay[i] = x.hdr & 0xffffffu;
   if (z) {
     rpd_cell rc;
     rc.tet_id = cand[c].tet;
@@ -1872,10 +1871,8 @@ __global__ __launch_bounds__(kCB) void k_compact(
         if (e < q1) {
           while (s_w[k + 1] <= e) ++k;
           const uint32_t w = e - s_w[k], lay = s_lay[k];
-          const uint32_t nv = lay & 255, np = (lay >> 8) & 255;
-          src[u] = s_wsrc[k] + (w < nv ? w
-                                : (w < nv + np ? (lay >> 16) + (w - nv)
-                                               : s_oe[k] + (w - nv - np)));
+          const uint32_t nvp = (lay & 255) + ((lay >> 8) & 255);
+          src[u] = s_wsrc[k] + (w < nvp ? (lay >> 16) + w : w - nvp);
         }
       }
       uint32_t v[kU];
