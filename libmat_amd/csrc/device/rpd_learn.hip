@@ -991,19 +991,22 @@ __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
                                                    int64_t cap,
                                                    int32_t* overflow) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  // (no early return: the whole wavefront takes part in the reduction)
-  const uint32_t m = t < n_tet ? sel[t] : 0u;
+  // (no early return: the whole wavefront takes part in the reduction and,
+  // on the small-set path, in the staging below)
+  const bool live = t < n_tet;
+  const uint32_t m = live ? sel[t] : 0u;
   const int n = __popc(m);
   int n_wave = n;  // the wavefront's largest set
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) n_wave = max(n_wave, __shfl_xor(n_wave, d));
-  if (t >= n_tet) return;
-  const int64_t o = off[t];
-  if (o + n > cap) *overflow = 1;
   if (n_wave <= 8) {
     // small sets (the usual case): ids inserted one by one into 8 sorted
     // registers, each slot updated in place to the median of its
-    // predecessor, the new id and itself
+    // predecessor, the new id and itself; the block's lists lie back to
+    // back from off[t0], so they are staged in LDS and written out by the
+    // whole block as one contiguous run of records
+    __shared__ int32_t s_seed[64 * 8];
+    __shared__ int32_t s_tet[64 * 8];
     int32_t w[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) w[k] = 0x7fffffff;
@@ -1014,20 +1017,38 @@ __global__ __launch_bounds__(64) void k_learn_fill(int32_t n_tet,
       for (int k = 7; k > 0; --k) w[k] = slot_median(x, w[k], w[k - 1]);
       w[0] = min(x, w[0]);
     }
+    const int t0 = blockIdx.x * blockDim.x;  // < n_tet: the block exists
+    const int64_t o0 = off[t0];
+    const int64_t total = off[min(t0 + (int)blockDim.x, n_tet)] - o0;
+    if (live) {
+      const int64_t o = off[t];
+      if (o + n > cap) *overflow = 1;
+      const int lo = (int)(o - o0);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (k < n && o + k < cap) {
+      for (int k = 0; k < 8; ++k)
+        if (k < n) {
+          s_seed[lo + k] = w[k];
+          s_tet[lo + k] = t;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+      const int64_t dst = o0 + e;
+      if (dst < cap) {
         Cand c;
-        c.tet = t;
-        c.seed = w[k];
+        c.tet = s_tet[e];
+        c.seed = s_seed[e];
         c.m0 = 0;
         c.m1 = 0;
-        cand[o + k] = c;
-        rel_ids[o + k] = w[k];
+        cand[dst] = c;
+        rel_ids[dst] = c.seed;
       }
     }
     return;
   }
+  if (!live) return;
+  const int64_t o = off[t];
+  if (o + n > cap) *overflow = 1;
   // selected ids ascending: bitonic sorting network on 32 registers
   int32_t v[kOut];
 #pragma unroll
