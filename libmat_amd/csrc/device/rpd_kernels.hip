@@ -757,13 +757,19 @@ __global__ __launch_bounds__(256) void k_cand_mask(DevMesh M, DevSites S,
   const int deg = (int)(S.knn_off[i + 1] - k0);
   uint64_t m0 = 0, m1 = 0;
   const int lim = deg < 128 ? deg : 128;
-  for (int k = 0; k < lim; ++k)
-    if (!F.inactive(S.knn_bisf[k0 + k])) {
+  // the next plane's load is issued before this plane's test (the test's
+  // corner branches keep the compiler from hoisting it)
+  float4 nxt = lim > 0 ? S.knn_bisf[k0] : make_float4(0, 0, 0, 0);
+  for (int k = 0; k < lim; ++k) {
+    const float4 f = nxt;
+    if (k + 1 < lim) nxt = S.knn_bisf[k0 + k + 1];
+    if (!F.inactive(f)) {
       if (k < 64)
         m0 |= 1ull << k;
       else
         m1 |= 1ull << (k - 64);
     }
+  }
   cand[c].m0 = m0;
   cand[c].m1 = m1;
   if (trips) {
