@@ -1098,7 +1098,35 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
   uint32_t status = RPD_SUCCESS;
   uint32_t n_det = 0, n_bis = 0, n_exd = 0;  // executed work (count_work)
   bool on_plane = false;  // exact tier: a vertex lay exactly on a plane
+  // next neighbour position after k that may cut the cell
+  auto next_pos = [&](int k) {
+    int nk = k + 1;
+    if (nk < 64) {
+      uint64_t r = m0 >> nk;
+      if (r)
+        nk += __ffsll((long long)r) - 1;
+      else
+        nk = 64;
+    }
+    if (nk >= 64 && nk < 128) {
+      uint64_t r = m1 >> (nk - 64);
+      if (r)
+        nk += __ffsll((long long)r) - 1;
+      else
+        nk = 128;
+    }
+    return nk;
+  };
+  // the plane of the next step is loaded one step ahead (the masks fix the
+  // positions in advance), so its latency overlaps the current step
   int k = -1;
+  int kp = next_pos(-1);
+  int mp = -1;
+  rpd_d4 ep{0, 0, 0, 0};
+  if (kp < deg) {
+    mp = a.S.knn_ids[k0 + kp];
+    ep = ld4(a.S.knn_bis + k0 + kp);
+  }
   for (;;) {
     // at the reference's plane capacity every further neighbour overflows,
     // whether or not it can cut (new_plane checks the count before the
@@ -1107,32 +1135,19 @@ __global__ __launch_bounds__(64, (clip_min_waves<G, C>())) void k_clip(
       status = RPD_VERTEX_OVERFLOW;
       break;
     }
-    // next neighbour position that may cut the cell
-    {
-      int nk = k + 1;
-      if (nk < 64) {
-        uint64_t r = m0 >> nk;
-        if (r)
-          nk += __ffsll((long long)r) - 1;
-        else
-          nk = 64;
-      }
-      if (nk >= 64 && nk < 128) {
-        uint64_t r = m1 >> (nk - 64);
-        if (r)
-          nk += __ffsll((long long)r) - 1;
-        else
-          nk = 128;
-      }
-      k = nk;
-    }
+    k = kp;
     if (k >= deg) break;
     if (nb_p >= P) {
       status = kFull ? RPD_VERTEX_OVERFLOW : kStatusRerunLarge;
       break;
     }
-    const int m = a.S.knn_ids[k0 + k];
-    const rpd_d4 e = ld4(a.S.knn_bis + k0 + k);
+    const int m = mp;
+    const rpd_d4 e = ep;
+    kp = next_pos(k);
+    if (kp < deg) {
+      mp = a.S.knn_ids[k0 + kp];
+      ep = ld4(a.S.knn_bis + k0 + kp);
+    }
     if (m == -1) break;
     const int cur = nb_p;
     n_bis += 1;
